@@ -1,0 +1,208 @@
+/*
+ * kwhip: MI355X (gfx950) implementation of the time-stepping loop of k-Wave's
+ * kspaceFirstOrder2D / kspaceFirstOrder3D, behind a C ABI.
+ *
+ * The problem description mirrors the variable set that the reference hands
+ * to its native engines after its MATLAB precompute
+ * (+kwave/+legacy/private/kspaceFirstOrder_saveToDisk.m:46-108,253-380 and
+ * +kwave/+legacy/kspaceFirstOrder3DC.m:468-546): grid sizes and spacings,
+ * time step, medium maps, PML sizes and alphas, scaled source signals with
+ * 1-based column-major index lists, the smoothed initial pressure, the
+ * sensor index list and the record flags.  As in the reference's native
+ * route the engine builds the PML profiles, the k-space correction kappa,
+ * the staggered shift-derivative operators and the absorption operators
+ * itself (in double precision, rounded once to single).
+ *
+ * All arrays are caller-owned, read-only, host memory in MATLAB column-major
+ * order (x fastest).  Return codes: 0 on success, a negative KW_E* code on
+ * failure with a message in `err` (truncated to errlen).  Every function is
+ * synchronous unless stated; a kw_sim handle is not thread-safe.
+ */
+#ifndef KWHIP_H
+#define KWHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define KW_ABI_VERSION 1
+
+/* error codes */
+#define KW_OK 0
+#define KW_EINVAL (-1)   /* invalid problem description */
+#define KW_EUNSUP (-2)   /* valid in the reference but not supported here */
+#define KW_EDEVICE (-3)  /* HIP runtime error */
+#define KW_ENOMEM (-4)   /* device allocation failed */
+#define KW_ESTATE (-5)   /* call not valid in the handle's current state */
+
+/* sensor.record flags (reference record names,
+ * +kwave/+legacy/private/kspaceFirstOrder_inputChecking.m:399-402) */
+#define KW_REC_P (1u << 0)
+#define KW_REC_P_MAX (1u << 1)
+#define KW_REC_P_MIN (1u << 2)
+#define KW_REC_P_RMS (1u << 3)
+#define KW_REC_P_MAX_ALL (1u << 4)
+#define KW_REC_P_MIN_ALL (1u << 5)
+#define KW_REC_P_FINAL (1u << 6)
+#define KW_REC_U (1u << 7)
+#define KW_REC_U_MAX (1u << 8)
+#define KW_REC_U_MIN (1u << 9)
+#define KW_REC_U_RMS (1u << 10)
+#define KW_REC_U_MAX_ALL (1u << 11)
+#define KW_REC_U_MIN_ALL (1u << 12)
+#define KW_REC_U_FINAL (1u << 13)
+
+/* source modes, numbered as the reference's p_source_mode / u_source_mode
+ * (kspaceFirstOrder_saveToDisk.m:225-231,262-268) */
+#define KW_SRC_DIRICHLET 0
+#define KW_SRC_ADDITIVE_NO_CORRECTION 1
+#define KW_SRC_ADDITIVE 2
+
+/* A time-varying source (the reference's source.p / source.ux ... with
+ * p_mask / u_mask), signals already scaled by the caller as in
+ * kspaceFirstOrder_scaleSourceTerms.m. */
+typedef struct kw_time_source {
+  int64_t flag;               /* number of time points; 0 = no source */
+  int32_t mode;               /* KW_SRC_* */
+  int64_t num_points;         /* source grid points */
+  const uint64_t* index;      /* [num_points], 1-based column-major (find(mask)) */
+  int64_t num_series;         /* rows of `input`: 1, num_points, or the label count */
+  const uint64_t* sig_index;  /* [num_points] 1-based signal row per point for
+                                 labelled masks, NULL otherwise */
+  const double* input;        /* [num_series x flag], column-major */
+} kw_time_source;
+
+typedef struct kw_problem {
+  int32_t dim;               /* 2 or 3 */
+  int64_t Nx, Ny, Nz;        /* Nz = 1 in 2D */
+  int64_t Nt;                /* number of time steps */
+  double dx, dy, dz, dt;
+  double c_ref;              /* reference sound speed (setSoundSpeedRef.m) */
+
+  /* medium: grids of Nx*Ny*Nz values, or NULL to use the scalar */
+  const double* c0;
+  double c0_scalar;
+  const double* rho0;
+  double rho0_scalar;
+  const double* rho0_sg[3];  /* density at x+dx/2, y+dy/2, z+dz/2 (already
+                                interpolated as kspaceFirstOrder3D.m:585-607);
+                                NULL uses rho0 */
+  int32_t nonlinear_flag;
+  const double* BonA;
+  double BonA_scalar;
+  int32_t absorbing_flag;    /* 0 lossless, 1 power law, 2 stokes */
+  const double* alpha_coeff; /* dB/(MHz^y cm), as medium.alpha_coeff */
+  double alpha_coeff_scalar;
+  double alpha_power;
+  int32_t alpha_mode;        /* 0 both terms, 1 'no_absorption', 2 'no_dispersion' */
+  double alpha_sign[2];      /* medium.alpha_sign; 0 means +1 */
+
+  int64_t pml_size[3];
+  double pml_alpha[3];
+  int32_t use_kspace;        /* 1 unless 'UsekSpace' false */
+  int32_t use_sg;            /* 1 unless 'UseSG' false */
+
+  /* sources */
+  const double* p0;          /* initial pressure (already smoothed) or NULL */
+  kw_time_source p_source;
+  kw_time_source u_source[3];
+
+  /* sensor: binary mask as an index list; num_points = Nx*Ny*Nz with
+   * index = NULL means every grid point in natural order */
+  int64_t sensor_num_points;
+  const uint64_t* sensor_mask_index; /* 1-based column-major */
+  uint32_t record_flags;     /* KW_REC_* */
+  int64_t record_start_index;/* 1-based */
+  /* time reversal: [sensor_num_points x Nt] in the original (not reversed)
+   * time order, or NULL */
+  const double* time_reversal_boundary_data;
+} kw_problem;
+
+typedef struct kw_sizes {
+  int64_t num_sensor_points;
+  int64_t num_recorded_time_points;
+  int64_t grid_cells;
+} kw_sizes;
+
+/* Caller-allocated outputs; a NULL pointer skips that output.  Shapes:
+ * time series [num_sensor_points x num_recorded_time_points] column-major,
+ * statistics [num_sensor_points], *_all and *_final [grid_cells]. */
+typedef struct kw_outputs {
+  float* p;
+  double* p_max;
+  double* p_min;
+  double* p_rms;
+  double* p_max_all;
+  double* p_min_all;
+  float* p_final;
+  float* u[3];
+  double* u_max[3];
+  double* u_min[3];
+  double* u_rms[3];
+  double* u_max_all[3];
+  double* u_min_all[3];
+  float* u_final[3];
+} kw_outputs;
+
+typedef struct kw_sim kw_sim;
+
+/* ABI version and the compute backend of this library build. */
+int kw_abi_version(void);
+const char* kw_backend(void);
+
+/* Output sizes of a problem.
+ * Replaces: kspaceFirstOrder_createStorageVariables.m:57-400 (allocation). */
+int kw_output_sizes(const kw_problem* prob, kw_sizes* sizes, char* err, size_t errlen);
+
+/* Run a whole simulation and copy its outputs.
+ * Replaces: the hot loop kspaceFirstOrder3D.m:769-1208 (2D: :712-1044)
+ * plus the native-engine call of kspaceFirstOrder3DC.m:546. */
+int kw_run(const kw_problem* prob, kw_outputs* out, char* err, size_t errlen);
+
+/* Device-resident simulation: inputs are copied to the device and all
+ * operators built by kw_sim_create; kw_sim_step advances nsteps time steps
+ * asynchronously on the handle's stream; kw_sim_read waits and copies the
+ * outputs recorded so far. */
+int kw_sim_create(const kw_problem* prob, kw_sim** sim, char* err, size_t errlen);
+int kw_sim_step(kw_sim* sim, int64_t nsteps, char* err, size_t errlen);
+int kw_sim_sync(kw_sim* sim, char* err, size_t errlen);
+int kw_sim_read(kw_sim* sim, kw_outputs* out, char* err, size_t errlen);
+int64_t kw_sim_steps_done(const kw_sim* sim);
+int64_t kw_sim_device_bytes(const kw_sim* sim);
+int kw_sim_destroy(kw_sim* sim);
+
+/* Per-kernel-kind launch statistics of a timed run: summed device time
+ * (each launch bracketed by events), launch count, and the compulsory HBM
+ * bytes of those launches (every array a launch reads or writes, counted
+ * once per launch). */
+#define KW_NKINDS 9
+typedef struct kw_kernel_stats {
+  double ms[KW_NKINDS];
+  int64_t launches[KW_NKINDS];
+  double bytes[KW_NKINDS];
+} kw_kernel_stats;
+
+/* Advance nsteps steps timed with device events on the handle's stream;
+ * *ms receives the elapsed device time.  With stats non-NULL every launch
+ * is additionally bracketed by events (slower; use a separate run for the
+ * headline time). */
+int kw_sim_step_timed(kw_sim* sim, int64_t nsteps, double* ms, kw_kernel_stats* stats,
+                      char* err, size_t errlen);
+/* Name of kernel kind k (0 <= k < KW_NKINDS). */
+const char* kw_kind_name(int k);
+
+/* Smoothing filter: out = real(ifftn(fftn(in) .* ifftshift(win))) on a 2D or
+ * 3D grid, computed on the device.  `win` is the centred window of
+ * getWin (not shifted).  Replaces the transform of +kwave/+legacy/smooth.m;
+ * the caller restores the maximum. */
+int kw_smooth(int32_t dim, const int64_t N[3], const double* in, const double* win,
+              double* out, char* err, size_t errlen);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* KWHIP_H */

@@ -1,0 +1,42 @@
+// Device backend used by the host orchestration (kw_sim.cpp).  Implemented
+// by kw_gpu.hip (HIP on gfx950: the product) and by kw_emu.cpp (the CPU
+// lane emulator used to validate kernel indexing without a GPU).
+#pragma once
+
+#include <stdexcept>
+#include <string>
+
+#include "kw_passes.h"
+
+namespace kw {
+
+struct DeviceError : std::runtime_error {
+  int code;
+  DeviceError(int c, const std::string& m) : std::runtime_error(m), code(c) {}
+};
+
+const char* backend_name();
+void* dev_alloc(size_t bytes);  // throws DeviceError(KW_ENOMEM) on failure
+void dev_free(void* p);
+void* dev_stream_create();
+void dev_stream_destroy(void* s);
+void dev_h2d(void* dst, const void* src, size_t bytes, void* stream);
+void dev_d2h(void* dst, const void* src, size_t bytes, void* stream);
+void dev_sync(void* stream);
+void dev_zero(void* dst, size_t bytes, void* stream);  // set-up only (not graph-safe)
+
+// launches (asynchronous on `stream`)
+void launch_pass(PassKind kind, int nblocks, int threads, size_t lds_bytes, const Batch& B,
+                 const PassArgs& A, void* stream);
+void launch_ew(int nblocks, const Batch& B, const EwArgs& A, void* stream);
+
+// event timing (GPU only; the emulator returns wall-clock times)
+void* dev_event_create();
+void dev_event_destroy(void* e);
+void dev_event_record(void* e, void* stream);
+double dev_event_elapsed_ms(void* a, void* b);  // synchronises on b
+
+// largest dynamic LDS per block the kernels may request
+size_t dev_max_lds();
+
+}  // namespace kw

@@ -1,0 +1,248 @@
+// CPU emulator backend: runs the kernel bodies of kw_passes.h with one
+// cooperative fiber per lane (ucontext), one block at a time per worker
+// thread, so barriers behave as on the GPU.  Every global load and store is
+// checked against the table of live allocations; LDS is filled with NaN at
+// the start of every block so reads of unwritten LDS show up in results; a
+// block whose lanes pass a different number of barriers is reported.
+//
+// This backend exists to validate indexing and parity on a machine without
+// a GPU.  It is built into libkwhip_emu.so, which only the tests load.
+#ifndef KW_EMU
+#define KW_EMU 1
+#endif
+
+#include <ucontext.h>
+
+#include <atomic>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <functional>
+#include <map>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "kw_backend.h"
+
+namespace kw {
+
+namespace {
+std::mutex g_mu;
+std::map<uintptr_t, size_t> g_allocs;
+std::atomic<int> g_fault{0};
+char g_fault_msg[512];
+alignas(64) char g_scratch[256];
+}  // namespace
+
+const void* emu_resolve(const void* p, size_t bytes, const char* what) {
+  const uintptr_t a = (uintptr_t)p;
+  auto it = g_allocs.upper_bound(a);
+  bool ok = it != g_allocs.begin();
+  if (ok) {
+    --it;
+    ok = a >= it->first && a + bytes <= it->first + it->second;
+  }
+  if (!ok) {
+    if (g_fault.exchange(1) == 0)
+      snprintf(g_fault_msg, sizeof g_fault_msg, "emulator: out-of-bounds %s of %zu bytes at %p",
+               what, bytes, p);
+    std::memset(g_scratch, 0, sizeof g_scratch);
+    return g_scratch;
+  }
+  return p;
+}
+
+static void emu_check(const void* p, size_t bytes, const char* what) {
+  (void)emu_resolve(p, bytes, what);
+}
+
+const char* backend_name() { return "emulator"; }
+
+void* dev_alloc(size_t bytes) {
+  const size_t n = bytes ? bytes : 1;
+  void* p = std::aligned_alloc(256, (n + 255) / 256 * 256);
+  if (!p) throw DeviceError(-4, "emulator: allocation failed");
+  std::lock_guard<std::mutex> lk(g_mu);
+  g_allocs[(uintptr_t)p] = n;
+  return p;
+}
+
+void dev_free(void* p) {
+  if (!p) return;
+  {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_allocs.erase((uintptr_t)p);
+  }
+  std::free(p);
+}
+
+void* dev_stream_create() { return nullptr; }
+void dev_stream_destroy(void*) {}
+void dev_h2d(void* dst, const void* src, size_t bytes, void*) {
+  emu_check(dst, bytes, "h2d");
+  std::memcpy(dst, src, bytes);
+}
+void dev_d2h(void* dst, const void* src, size_t bytes, void*) {
+  emu_check(src, bytes, "d2h");
+  std::memcpy(dst, src, bytes);
+}
+void dev_sync(void*) {}
+void dev_zero(void* dst, size_t bytes, void*) {
+  emu_check(dst, bytes, "zero");
+  std::memset(dst, 0, bytes);
+}
+size_t dev_max_lds() { return 65536; }
+
+namespace {
+
+constexpr size_t kStack = 96 * 1024;
+
+struct Runner;
+thread_local Runner* tl_runner = nullptr;
+
+struct EmuCtx {
+  Runner* R;
+  int t, b;
+  cf* L;
+  int tid() const { return t; }
+  int bid() const { return b; }
+  cf* lds() const { return L; }
+  void sync();
+};
+
+struct Lane {
+  ucontext_t ctx;
+  std::vector<char> stack;
+  bool done;
+  long syncs;
+};
+
+struct Runner {
+  ucontext_t sched;
+  std::vector<Lane> lanes;
+  std::vector<cf> lds;
+  int cur = 0;
+  int bid = 0;
+  const std::function<void(EmuCtx&)>* body = nullptr;
+
+  static void trampoline() {
+    Runner* R = tl_runner;
+    const int i = R->cur;
+    EmuCtx c{R, i, R->bid, R->lds.data()};
+    (*R->body)(c);
+    R->lanes[i].done = true;
+  }
+
+  void run_block(int b, int T) {
+    bid = b;
+    if ((int)lanes.size() < T) lanes.resize(T);
+    const float qnan = std::nanf("");
+    for (auto& v : lds) v = cf{qnan, qnan};
+    for (int i = 0; i < T; ++i) {
+      Lane& l = lanes[i];
+      if (l.stack.empty()) l.stack.resize(kStack);
+      l.done = false;
+      l.syncs = 0;
+      getcontext(&l.ctx);
+      l.ctx.uc_stack.ss_sp = l.stack.data();
+      l.ctx.uc_stack.ss_size = l.stack.size();
+      l.ctx.uc_link = &sched;
+      makecontext(&l.ctx, (void (*)())trampoline, 0);
+    }
+    for (;;) {
+      int alive = 0;
+      for (int i = 0; i < T; ++i) {
+        if (lanes[i].done) continue;
+        cur = i;
+        swapcontext(&sched, &lanes[i].ctx);
+        if (!lanes[i].done) ++alive;
+      }
+      if (alive == 0) break;
+    }
+    for (int i = 1; i < T; ++i) {
+      if (lanes[i].syncs != lanes[0].syncs) {
+        if (g_fault.exchange(1) == 0)
+          snprintf(g_fault_msg, sizeof g_fault_msg,
+                   "emulator: block %d lane %d passed %ld barriers, lane 0 passed %ld", b, i,
+                   lanes[i].syncs, lanes[0].syncs);
+        break;
+      }
+    }
+  }
+};
+
+void EmuCtx::sync() {
+  Lane& l = R->lanes[t];
+  ++l.syncs;
+  swapcontext(&l.ctx, &R->sched);
+}
+
+void emu_launch(int nblocks, int T, size_t lds_bytes,
+                const std::function<void(EmuCtx&)>& body) {
+  if (nblocks <= 0) return;
+  if (T <= 0 || T > 1024) throw DeviceError(-3, "emulator: bad block size");
+  int nw = (int)std::thread::hardware_concurrency();
+  if (nw < 1) nw = 1;
+  if (nw > 8) nw = 8;
+  if (nw > nblocks) nw = nblocks;
+  std::atomic<int> next{0};
+  auto work = [&]() {
+    Runner R;
+    R.body = &body;
+    R.lds.resize(lds_bytes / sizeof(cf) + 1);
+    tl_runner = &R;
+    for (;;) {
+      const int b = next.fetch_add(1);
+      if (b >= nblocks) break;
+      R.run_block(b, T);
+    }
+    tl_runner = nullptr;
+  };
+  std::vector<std::thread> th;
+  for (int w = 1; w < nw; ++w) th.emplace_back(work);
+  work();
+  for (auto& x : th) x.join();
+  if (g_fault.load()) {
+    g_fault = 0;
+    throw DeviceError(-3, g_fault_msg);
+  }
+}
+
+}  // namespace
+
+void launch_pass(PassKind kind, int nblocks, int threads, size_t lds_bytes, const Batch& B,
+                 const PassArgs& A, void*) {
+  std::function<void(EmuCtx&)> f;
+  switch (kind) {
+    case PK_YFWD: f = [&](EmuCtx& c) { body_ypass<false>(c, B, A); }; break;
+    case PK_YINV: f = [&](EmuCtx& c) { body_ypass<true>(c, B, A); }; break;
+    case PK_SPEC: f = [&](EmuCtx& c) { body_spec(c, B, A); }; break;
+    case PK_X_R2C: f = [&](EmuCtx& c) { body_x_r2c(c, B, A); }; break;
+    case PK_X_C2R: f = [&](EmuCtx& c) { body_x_c2r(c, B, A); }; break;
+    case PK_X_GRAD: f = [&](EmuCtx& c) { body_x_grad(c, B, A); }; break;
+    case PK_X_DIV: f = [&](EmuCtx& c) { body_x_div(c, B, A); }; break;
+    case PK_X_EOS: f = [&](EmuCtx& c) { body_x_eos(c, B, A); }; break;
+  }
+  emu_launch(nblocks, threads, lds_bytes, f);
+}
+
+void launch_ew(int nblocks, const Batch& B, const EwArgs& A, void*) {
+  std::function<void(EmuCtx&)> f = [&](EmuCtx& c) { body_ew(c, B, A); };
+  emu_launch(nblocks, EW_T, 0, f);
+}
+
+struct EmuEvent {
+  std::chrono::steady_clock::time_point t;
+};
+void* dev_event_create() { return new EmuEvent(); }
+void dev_event_destroy(void* e) { delete (EmuEvent*)e; }
+void dev_event_record(void* e, void*) { ((EmuEvent*)e)->t = std::chrono::steady_clock::now(); }
+double dev_event_elapsed_ms(void* a, void* b) {
+  return std::chrono::duration<double, std::milli>(((EmuEvent*)b)->t - ((EmuEvent*)a)->t)
+      .count();
+}
+
+}  // namespace kw

@@ -1,0 +1,1164 @@
+// Host side of kwhip: validates a kw_problem, builds the operators in double
+// precision and rounds them once to single (the reference's
+// 'DataCast','single' policy, kspaceFirstOrder_dataCast.m:31-268), lays the
+// problem out in device memory, sequences the passes of each time step and
+// implements the C ABI declared in include/kwhip.h.
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "kw_backend.h"
+#include "kwhip.h"
+
+namespace kw {
+
+[[noreturn]] static void fail(int code, const std::string& m) { throw DeviceError(code, m); }
+
+static const double kPi = 3.14159265358979323846264338327950288;
+
+// ---------------------------------------------------------------------------
+// operators (double precision)
+// ---------------------------------------------------------------------------
+
+// radices for a length: 8s first, then 4 or 2, then 3s and 5s
+static bool factorise(int N, std::vector<int>& r) {
+  r.clear();
+  int n = N;
+  while (n % 8 == 0) {
+    r.push_back(8);
+    n /= 8;
+  }
+  if (n % 4 == 0) {
+    r.push_back(4);
+    n /= 4;
+  }
+  if (n % 2 == 0) {
+    r.push_back(2);
+    n /= 2;
+  }
+  while (n % 3 == 0) {
+    r.push_back(3);
+    n /= 3;
+  }
+  while (n % 5 == 0) {
+    r.push_back(5);
+    n /= 5;
+  }
+  return n == 1 && !r.empty() && (int)r.size() <= KW_MAX_STAGES;
+}
+
+// wavenumber of ifftshifted index i (kWaveGrid.m:779-797 then ifftshift):
+// (2 pi/d) * (n/N) with n = i for i < ceil(N/2), else i - N
+static double kvec(int i, int N, double d) {
+  const int n = (i < (N + 1) / 2) ? i : i - N;
+  return (2.0 * kPi / d) * ((double)n / (double)N);
+}
+
+// getPML.m:52-86
+static std::vector<double> pml_profile(int N, double d, double dt, double c, int size,
+                                       double alpha, bool staggered) {
+  std::vector<double> v(N, 1.0);
+  for (int x = 1; x <= size && x <= N; ++x) {
+    const double xs = staggered ? x + 0.5 : (double)x;
+    const double left = alpha * (c / d) * std::pow((xs - size - 1) / (0.0 - size), 4);
+    const double right = alpha * (c / d) * std::pow(xs / size, 4);
+    v[x - 1] = std::exp(-left * dt / 2.0);
+    v[N - size + x - 1] = std::exp(-right * dt / 2.0);
+  }
+  return v;
+}
+
+static double sinc(double x) { return x == 0.0 ? 1.0 : std::sin(x) / x; }
+
+template <class F>
+static void parallel_for(int64_t n, F f) {
+  int nt = (int)std::thread::hardware_concurrency();
+  nt = std::max(1, std::min(nt, 16));
+  if (n < 100000) nt = 1;
+  std::vector<std::thread> th;
+  const int64_t chunk = (n + nt - 1) / nt;
+  for (int w = 0; w < nt; ++w) {
+    const int64_t a = w * chunk, b = std::min(n, a + chunk);
+    if (a >= b) break;
+    th.emplace_back([=]() {
+      for (int64_t i = a; i < b; ++i) f(i);
+    });
+  }
+  for (auto& t : th) t.join();
+}
+
+// ---------------------------------------------------------------------------
+// simulation object
+// ---------------------------------------------------------------------------
+
+struct Geometry {
+  int dim, Nx, Ny, Nz, Wm, nyq;
+  int64_t ncell, nrows, nspec_main, nspec;
+};
+
+static Geometry make_geometry(int dim, int64_t Nx, int64_t Ny, int64_t Nz) {
+  Geometry g;
+  g.dim = dim;
+  g.Nx = (int)Nx;
+  g.Ny = (int)Ny;
+  g.Nz = dim == 3 ? (int)Nz : 1;
+  g.nyq = (g.Nx % 2 == 0) ? 1 : 0;
+  g.Wm = g.nyq ? g.Nx / 2 : (g.Nx + 1) / 2;
+  g.ncell = (int64_t)g.Nx * g.Ny * g.Nz;
+  g.nrows = (int64_t)g.Ny * g.Nz;
+  g.nspec_main = (int64_t)g.Wm * g.Ny * g.Nz;
+  g.nspec = g.nspec_main + (g.nyq ? g.nrows : 0);
+  return g;
+}
+
+struct LaunchGeom {
+  int nb = 0, T = 0;
+  size_t lds = 0;
+};
+
+static int max_pencils_env() {
+  const char* e = std::getenv("KW_MAX_PENCILS");
+  if (!e) return 1 << 30;
+  const int v = std::atoi(e);
+  return v > 0 ? v : 1 << 30;
+}
+
+struct Sim {
+  Geometry G{};
+  ProbDev P{};
+  ProbDev* dP = nullptr;
+  int32_t* d_tile0 = nullptr;
+  void* stream = nullptr;
+  std::vector<void*> allocs;
+  int64_t bytes = 0;
+  int64_t steps_done = 0;
+  int64_t Nt = 0, Nt_eff = 0, n_rec = 0, nsens = 0, record_start = 1;
+  bool has_p0 = false, time_rev = false, full_mask = false;
+  uint32_t record = 0;
+  bool p_grid = false, u_grid[3] = {false, false, false};
+  float* skappa = nullptr;
+  float* p0 = nullptr;
+  LaunchGeom lx, ly, ll, lcell, lsens;
+  // per-kernel event timing
+  bool timing = false;
+  std::vector<std::pair<void*, void*>> ev_used;
+  std::vector<int> ev_kind;
+  std::vector<double> ev_bytes;
+  std::vector<void*> ev_free;
+
+  ~Sim() {
+    for (auto& e : ev_used) {
+      dev_event_destroy(e.first);
+      dev_event_destroy(e.second);
+    }
+    for (void* e : ev_free) dev_event_destroy(e);
+    for (void* p : allocs) dev_free(p);
+    dev_stream_destroy(stream);
+  }
+
+  template <class T>
+  T* alloc(size_t n) {
+    void* p = dev_alloc(n * sizeof(T));
+    allocs.push_back(p);
+    bytes += (int64_t)(n * sizeof(T));
+    return (T*)p;
+  }
+  template <class T>
+  T* alloc_zero(size_t n) {
+    T* p = alloc<T>(n);
+    dev_zero(p, n * sizeof(T), stream);
+    return p;
+  }
+  template <class T>
+  T* upload(const std::vector<T>& v) {
+    T* d = alloc<T>(v.size());
+    dev_h2d(d, v.data(), v.size() * sizeof(T), stream);
+    return d;
+  }
+
+  Batch batch() const { return Batch{dP, d_tile0, 1}; }
+
+  void* take_event() {
+    if (!ev_free.empty()) {
+      void* e = ev_free.back();
+      ev_free.pop_back();
+      return e;
+    }
+    return dev_event_create();
+  }
+
+  template <class L>
+  void timed(int kind, double nbytes, L launch) {
+    if (!timing) {
+      launch();
+      return;
+    }
+    void* a = take_event();
+    void* b = take_event();
+    dev_event_record(a, stream);
+    launch();
+    dev_event_record(b, stream);
+    ev_used.push_back({a, b});
+    ev_kind.push_back(kind);
+    ev_bytes.push_back(nbytes);
+  }
+
+  void pass(PassKind k, const PassArgs& a) {
+    const LaunchGeom* g = &lx;
+    if (k == PK_YFWD || k == PK_YINV) g = &ly;
+    if (k == PK_SPEC) g = &ll;
+    const Batch B = batch();
+    timed((int)k, timing ? pass_bytes(k, a) : 0.0,
+          [&]() { launch_pass(k, g->nb, g->T, g->lds, B, a, stream); });
+  }
+
+  void ew(const EwArgs& a, int nblocks) {
+    const Batch B = batch();
+    timed(8, timing ? ew_bytes(a) : 0.0, [&]() { launch_ew(nblocks, B, a, stream); });
+  }
+
+  // -- compulsory bytes of one launch: every array it reads or writes, once
+  double fb(const Field& f) const { return f.a ? 4.0 * G.ncell : 0.0; }
+  double src_bytes(const TimeSource& s, int t) const {
+    if (s.len < t) return 0.0;
+    if (s.mode == SRC_ADDITIVE) return 4.0 * G.ncell;  // K_M or the filtered grid
+    return 4.0 * G.ncell;                             // dense point map
+  }
+  double sensor_bytes(int t) const {
+    if (time_rev || t < record_start) return 0.0;
+    const double nc = (double)G.ncell;
+    const bool first = t == record_start;
+    const double stat = first ? 8.0 : 16.0;
+    double b = 0.0;
+    if (full_mask) {
+      if (record & KW_REC_P) b += 4.0 * nc;
+      if (record & KW_REC_P_MAX) b += stat * nc;
+      if (record & KW_REC_P_MIN) b += stat * nc;
+      if (record & KW_REC_P_RMS) b += stat * nc;
+    }
+    if (record & KW_REC_P_MAX_ALL) b += stat * nc;
+    if (record & KW_REC_P_MIN_ALL) b += stat * nc;
+    return b;
+  }
+  double pass_bytes(PassKind k, const PassArgs& a) const {
+    const int t = (int)(steps_done + 1);
+    const double nc = (double)G.ncell, sp = 8.0 * (double)G.nspec, tab = 4.0 * (double)G.nspec;
+    const int dim = G.dim;
+    switch (k) {
+      case PK_YFWD:
+      case PK_YINV:
+        return a.nbuf * 2.0 * sp;
+      case PK_SPEC:
+        if (a.mode == SM_GRAD) return sp + dim * sp + tab;
+        if (a.mode == SM_DIV) return 2.0 * dim * sp + tab;
+        if (a.mode == SM_ABS) return 4.0 * sp + 2.0 * tab;
+        return 2.0 * sp + tab;
+      case PK_X_R2C:
+        return a.nbuf * (4.0 * nc + sp);
+      case PK_X_C2R:
+        return a.nbuf * (sp + 4.0 * nc) * ((a.flags & XC_ACCUM) ? 1.5 : 1.0);
+      case PK_X_GRAD: {
+        double b = 0.0;
+        const bool init = a.flags & XG_INIT;
+        for (int d = 0; d < dim; ++d) {
+          b += sp + fb(P.rinv[d]) + 4.0 * nc;
+          if (!init) b += 4.0 * nc + sp + src_bytes(P.srcu[d], t);
+        }
+        return b;
+      }
+      case PK_X_DIV: {
+        double b = dim * (sp + 8.0 * nc) + fb(P.rho0) + dim * src_bytes(P.srcp, t);
+        if (P.eos == EOS_ABSORBING) return b + 2.0 * sp + 4.0 * nc;
+        b += fb(P.c0) + sp + sensor_bytes(t);
+        if (P.eos == EOS_STOKES) b += fb(P.tau);
+        if (P.nonlinear) b += fb(P.bona);
+        if (a.flags & X_STORE_P) b += 4.0 * nc;
+        if (time_rev) b += 4.0 * nc;
+        return b;
+      }
+      case PK_X_EOS: {
+        double b = 2.0 * sp + 4.0 * nc + fb(P.c0) + fb(P.tau) + fb(P.eta) + sp + sensor_bytes(t);
+        if (P.nonlinear) b += fb(P.bona) + fb(P.rho0);
+        if (a.flags & X_STORE_P) b += 4.0 * nc;
+        if (time_rev) b += 4.0 * nc;
+        return b;
+      }
+    }
+    return 0.0;
+  }
+  double ew_bytes(const EwArgs& a) const {
+    const double nc = (double)G.ncell;
+    switch (a.kind) {
+      case EW_INIT_P0: return (2.0 + G.dim) * 4.0 * nc;
+      case EW_TR_PROLOGUE: return (2.0 + G.dim) * 4.0 * nc;
+      case EW_SRC_GRID: return 8.0 * nc;
+      case EW_RECORD_POINTS: return 24.0 * (double)nsens;
+      case EW_RECORD_GRID: return 24.0 * nc;
+      default: return 0.0;
+    }
+  }
+
+  // -- tiling -------------------------------------------------------------
+  void plan_launches() {
+    const size_t maxl = dev_max_lds();
+    const int capP = max_pencils_env();
+    auto Gof = [](int N) { return (N + KW_E - 1) / KW_E; };
+    // x pass: <= 256 threads, LDS P*(Nx+1) complex
+    {
+      const int Gx = Gof(G.Nx);
+      int P = std::max(1, 256 / Gx);
+      P = std::min<int64_t>(P, (int64_t)(maxl / (sizeof(cf) * (G.Nx + 1))));
+      P = std::max(1, std::min(P, capP));
+      this->P.Gx = Gx;
+      this->P.Px = P;
+      const int64_t npairs = (G.nrows + 1) / 2;
+      lx.nb = (int)((npairs + P - 1) / P);
+      lx.T = P * Gx;
+      lx.lds = sizeof(cf) * (size_t)P * (G.Nx + 1);
+      if (Gx > 256 || lx.lds > maxl) fail(KW_EUNSUP, "grid size Nx too large for the x pass");
+    }
+    // strided y pass (3D): <= 1024 threads, LDS P*(Ny+1)
+    if (G.dim == 3) {
+      const int Gy = Gof(G.Ny);
+      int P = std::max(1, std::min(16, 1024 / Gy));
+      P = std::min<int64_t>(P, (int64_t)(maxl / (sizeof(cf) * (G.Ny + 1))));
+      P = std::max(1, std::min(P, capP));
+      this->P.Gy = Gy;
+      this->P.Py = P;
+      const int64_t nH = (int64_t)G.Wm * G.Nz;
+      ly.nb = (int)((nH + P - 1) / P + (G.nyq ? (G.Nz + P - 1) / P : 0));
+      ly.T = P * Gy;
+      ly.lds = sizeof(cf) * (size_t)P * (G.Ny + 1);
+      if (Gy > 1024 || ly.lds > maxl) fail(KW_EUNSUP, "grid size Ny too large for the y pass");
+    }
+    // last-axis pass: <= 1024 threads, LDS 2*P*N
+    {
+      const int Nl = G.dim == 3 ? G.Nz : G.Ny;
+      const int Gl = Gof(Nl);
+      int P = std::max(1, std::min(16, 1024 / Gl));
+      P = std::min<int64_t>(P, (int64_t)(maxl / (2 * sizeof(cf) * Nl)));
+      P = std::max(1, std::min(P, capP));
+      this->P.Gl = Gl;
+      this->P.Pl = P;
+      const int64_t nH = G.dim == 3 ? (int64_t)G.Wm * G.Ny : (int64_t)G.Wm;
+      const int64_t nQ = G.dim == 3 ? G.Ny : 1;
+      ll.nb = (int)((nH + P - 1) / P + (G.nyq ? (nQ + P - 1) / P : 0));
+      ll.T = P * Gl;
+      ll.lds = 2 * sizeof(cf) * (size_t)P * Nl;
+      if (Gl > 1024 || ll.lds > maxl) fail(KW_EUNSUP, "grid size too large for the last-axis pass");
+    }
+    lcell.nb = (int)((G.ncell + EW_T - 1) / EW_T);
+    lsens.nb = (int)((std::max<int64_t>(nsens, 1) + EW_T - 1) / EW_T);
+  }
+
+  FftPlan make_plan(int N) {
+    FftPlan pl{};
+    std::vector<int> r;
+    if (!factorise(N, r))
+      fail(KW_EUNSUP, "grid size " + std::to_string(N) +
+                          " is not a product of 2, 3 and 5 (supported FFT radices)");
+    pl.N = N;
+    pl.nst = (int)r.size();
+    for (size_t i = 0; i < r.size(); ++i) pl.radix[i] = r[i];
+    std::vector<cf> tw(N);
+    for (int j = 0; j < N; ++j) {
+      const double a = 2.0 * kPi * (double)j / (double)N;
+      tw[j] = cf{(float)std::cos(a), (float)-std::sin(a)};
+    }
+    pl.tw = upload(tw);
+    return pl;
+  }
+
+  // spectral-layout table of f(kx, ky, kz) (ifftshifted wavenumbers)
+  template <class F>
+  float* spectral_table(F f) {
+    std::vector<float> t(G.nspec);
+    const Geometry g = G;
+    parallel_for(g.nspec, [&](int64_t e) {
+      int ix, iy, iz;
+      if (e < g.nspec_main) {
+        ix = (int)(e % g.Wm);
+        const int64_t r = e / g.Wm;
+        iy = (int)(r % g.Ny);
+        iz = (int)(r / g.Ny);
+      } else {
+        ix = g.Nx / 2;
+        const int64_t r = e - g.nspec_main;
+        iy = (int)(r % g.Ny);
+        iz = (int)(r / g.Ny);
+      }
+      t[e] = (float)f(ix, iy, iz);
+    });
+    return upload(t);
+  }
+};
+
+// ---------------------------------------------------------------------------
+// problem validation and set-up
+// ---------------------------------------------------------------------------
+
+static std::vector<float> to_float(const double* a, int64_t n) {
+  std::vector<float> v(n);
+  parallel_for(n, [&](int64_t i) { v[i] = (float)a[i]; });
+  return v;
+}
+
+static Field make_field(Sim& S, const double* a, double s, int64_t n) {
+  Field f{};
+  if (a) {
+    f.a = S.upload(to_float(a, n));
+    f.s = 0.f;
+  } else {
+    f.a = nullptr;
+    f.s = (float)s;
+  }
+  return f;
+}
+
+static void check_index(const uint64_t* idx, int64_t n, int64_t ncell, const char* what) {
+  if (n > 0 && !idx) fail(KW_EINVAL, std::string(what) + ": index list is NULL");
+  for (int64_t i = 0; i < n; ++i)
+    if (idx[i] < 1 || (int64_t)idx[i] > ncell)
+      fail(KW_EINVAL, std::string(what) + ": index out of range (indices are 1-based)");
+}
+
+static void setup_source(Sim& S, const kw_time_source& src, TimeSource& ts, const char* what) {
+  ts = TimeSource{};
+  if (src.flag <= 0) return;
+  const int64_t ncell = S.G.ncell;
+  if (src.mode < 0 || src.mode > 2) fail(KW_EINVAL, std::string(what) + ": unknown mode");
+  if (src.num_points <= 0) fail(KW_EINVAL, std::string(what) + ": no source points");
+  check_index(src.index, src.num_points, ncell, what);
+  if (src.num_series < 1 || !src.input) fail(KW_EINVAL, std::string(what) + ": no signal");
+  const bool labelled = src.sig_index != nullptr;
+  if (!labelled && src.num_series != 1 && src.num_series != src.num_points)
+    fail(KW_EINVAL, std::string(what) +
+                        ": the number of time series must be 1 or the number of source points");
+  if (labelled)
+    for (int64_t i = 0; i < src.num_points; ++i)
+      if (src.sig_index[i] < 1 || (int64_t)src.sig_index[i] > src.num_series)
+        fail(KW_EINVAL, std::string(what) + ": signal label out of range");
+  const int64_t len = std::min<int64_t>(src.flag, S.Nt);
+  ts.len = (int32_t)src.flag;
+  ts.mode = src.mode;
+  ts.nseries = (int32_t)src.num_series;
+  ts.npts = (int32_t)src.num_points;
+  // signal matrix: only the first min(flag, Nt) columns are ever used
+  std::vector<float> sig((size_t)src.num_series * len);
+  for (int64_t k = 0; k < (int64_t)sig.size(); ++k) sig[k] = (float)src.input[k];
+  ts.sig = S.upload(sig);
+  if (src.num_series > 1) {
+    std::vector<int32_t> row(src.num_points);
+    for (int64_t i = 0; i < src.num_points; ++i)
+      row[i] = labelled ? (int32_t)(src.sig_index[i] - 1) : (int32_t)i;
+    ts.row = S.upload(row);
+  }
+  const bool need_map = src.mode != SRC_ADDITIVE || src.num_series > 1;
+  if (need_map) {
+    std::vector<int32_t> map(ncell, -1);
+    for (int64_t i = 0; i < src.num_points; ++i) map[src.index[i] - 1] = (int32_t)i;
+    ts.map = S.upload(map);
+  }
+}
+
+// K_M = real(ifftn(source_kappa .* fftn(M))) for the source mask M: the
+// additive correction of a single-series source is s(t) K_M
+// (kspaceFirstOrder3D.m:930-962 is linear in the signal)
+static float* source_km(Sim& S, const kw_time_source& src) {
+  std::vector<float> mask(S.G.ncell, 0.f);
+  for (int64_t i = 0; i < src.num_points; ++i) mask[src.index[i] - 1] = 1.f;
+  float* km = S.upload(mask);
+  PassArgs a{};
+  a.nbuf = 1;
+  a.buf[0] = 1;
+  a.real[0] = km;
+  S.pass(PK_X_R2C, a);
+  if (S.G.dim == 3) S.pass(PK_YFWD, a);
+  PassArgs f = a;
+  f.mode = SM_FILTER;
+  f.table = S.skappa;
+  S.pass(PK_SPEC, f);
+  if (S.G.dim == 3) S.pass(PK_YINV, a);
+  S.pass(PK_X_C2R, a);
+  return km;
+}
+
+static std::unique_ptr<Sim> create_sim(const kw_problem* pr) {
+  if (!pr) fail(KW_EINVAL, "problem is NULL");
+  if (pr->dim != 2 && pr->dim != 3) fail(KW_EINVAL, "dim must be 2 or 3");
+  if (pr->Nx < 2 || pr->Ny < 2 || (pr->dim == 3 && pr->Nz < 2))
+    fail(KW_EINVAL, "grid sizes must be at least 2");
+  if (pr->dim == 2 && pr->Nz != 1) fail(KW_EINVAL, "Nz must be 1 in 2D");
+  if (pr->Nt < 1) fail(KW_EINVAL, "Nt must be positive");
+  if (!(pr->dt > 0) || !(pr->dx > 0) || !(pr->dy > 0) || (pr->dim == 3 && !(pr->dz > 0)))
+    fail(KW_EINVAL, "dt and grid spacings must be positive");
+  if (!(pr->c_ref > 0)) fail(KW_EINVAL, "c_ref must be positive");
+
+  auto S = std::make_unique<Sim>();
+  S->stream = dev_stream_create();
+  S->G = make_geometry(pr->dim, pr->Nx, pr->Ny, pr->Nz);
+  const Geometry& G = S->G;
+  ProbDev& P = S->P;
+  P.dim = G.dim;
+  P.Nx = G.Nx;
+  P.Ny = G.Ny;
+  P.Nz = G.Nz;
+  P.Wm = G.Wm;
+  P.nyq = G.nyq;
+  P.ncell = G.ncell;
+  P.nrows = G.nrows;
+  P.nspec_main = G.nspec_main;
+  P.nspec = G.nspec;
+  S->Nt = pr->Nt;
+  P.Nt = (int32_t)pr->Nt;
+  P.dt = (float)pr->dt;
+  const int dim = G.dim;
+  const double d[3] = {pr->dx, pr->dy, pr->dz};
+  const int Nd[3] = {G.Nx, G.Ny, G.Nz};
+
+  // sensor / time reversal / record bookkeeping
+  S->time_rev = pr->time_reversal_boundary_data != nullptr;
+  S->has_p0 = pr->p0 != nullptr;
+  if (S->time_rev && S->has_p0) fail(KW_EINVAL, "time reversal with source.p0 is not valid");
+  if (S->has_p0 && pr->p_source.flag > 0)
+    fail(KW_EINVAL, "source.p0 and source.p can't be defined in the same simulation.");
+  S->nsens = pr->sensor_num_points;
+  if (S->nsens < 0) fail(KW_EINVAL, "negative number of sensor points");
+  S->record = pr->record_flags;
+  if (S->time_rev) {
+    if (S->nsens <= 0) fail(KW_EINVAL, "time reversal needs sensor points");
+    if (pr->Nt < 2) fail(KW_EINVAL, "time reversal needs Nt >= 2");
+    S->record = KW_REC_P_FINAL;
+  }
+  S->record_start = pr->record_start_index > 0 ? pr->record_start_index : 1;
+  if (!S->time_rev && S->nsens > 0 && S->record_start > pr->Nt)
+    fail(KW_EINVAL, "sensor.record_start_index must be less than or equal to kgrid.Nt");
+  S->Nt_eff = S->time_rev ? pr->Nt - 1 : pr->Nt;
+  S->n_rec = (S->nsens > 0 && !S->time_rev) ? pr->Nt - S->record_start + 1 : 0;
+  if (S->nsens == 0) S->record &= (KW_REC_P_FINAL | KW_REC_U_FINAL);
+  P.record = S->record;
+  P.record_start = (int32_t)S->record_start;
+  P.n_rec = (int32_t)S->n_rec;
+  P.nsens = (int32_t)S->nsens;
+  P.time_rev = S->time_rev ? 1 : 0;
+  if (S->nsens > 0) {
+    if (pr->sensor_mask_index) {
+      check_index(pr->sensor_mask_index, S->nsens, G.ncell, "sensor.mask");
+      bool identity = S->nsens == G.ncell;
+      for (int64_t i = 0; identity && i < S->nsens; ++i)
+        identity = pr->sensor_mask_index[i] == (uint64_t)(i + 1);
+      S->full_mask = identity;
+    } else {
+      if (S->nsens != G.ncell)
+        fail(KW_EINVAL, "sensor_mask_index is NULL but num_points is not the grid size");
+      S->full_mask = true;
+    }
+  }
+  P.full_mask = S->full_mask ? 1 : 0;
+
+  S->plan_launches();
+  P.fx = S->make_plan(G.Nx);
+  P.fy = S->make_plan(G.Ny);
+  P.fz = dim == 3 ? S->make_plan(G.Nz) : P.fy;
+
+  // medium (rounded once from double)
+  const int64_t n = G.ncell;
+  P.c0 = make_field(*S, pr->c0, pr->c0_scalar, n);
+  P.rho0 = make_field(*S, pr->rho0, pr->rho0_scalar, n);
+  if (!pr->c0 && !(pr->c0_scalar > 0)) fail(KW_EINVAL, "sound speed must be positive");
+  if (!pr->rho0 && !(pr->rho0_scalar > 0)) fail(KW_EINVAL, "density must be positive");
+  for (int a = 0; a < dim; ++a) {
+    const double* r = pr->rho0_sg[a] ? pr->rho0_sg[a] : pr->rho0;
+    if (r) {
+      std::vector<float> inv(n);
+      parallel_for(n, [&](int64_t i) { inv[i] = (float)(1.0 / r[i]); });
+      P.rinv[a] = Field{S->upload(inv), 0.f};
+    } else {
+      P.rinv[a] = Field{nullptr, (float)(1.0 / pr->rho0_scalar)};
+    }
+  }
+  P.nonlinear = pr->nonlinear_flag ? 1 : 0;
+  if (P.nonlinear) P.bona = make_field(*S, pr->BonA, pr->BonA_scalar, n);
+  P.eos = pr->absorbing_flag;
+  if (P.eos < 0 || P.eos > 2) fail(KW_EINVAL, "absorbing_flag must be 0, 1 or 2");
+  const double y = P.eos == EOS_STOKES ? 2.0 : pr->alpha_power;
+  if (P.eos == EOS_ABSORBING) {
+    if (!(y >= 0 && y < 3)) fail(KW_EINVAL, "medium.alpha_power must be a real number between 0 and 3.");
+    if (pr->alpha_mode != 2 && y == 1.0)
+      fail(KW_EINVAL, "The power law dispersion term in the equation of state is not valid for medium.alpha_power = 1.");
+  }
+  if (P.eos != EOS_LOSSLESS) {
+    // createAbsorptionVariables.m:29-100 with db2neper.m
+    const double s_tau = pr->alpha_sign[0] < 0 ? -1.0 : 1.0;
+    const double s_eta = pr->alpha_sign[1] < 0 ? -1.0 : 1.0;
+    const bool no_abs = P.eos == EOS_ABSORBING && pr->alpha_mode == 1;
+    const bool no_disp = P.eos == EOS_ABSORBING && pr->alpha_mode == 2;
+    const double conv = 100.0 * std::pow(1e-6 / (2.0 * kPi), y) / (20.0 * std::log10(std::exp(1.0)));
+    auto tau_of = [&](double a, double c) {
+      if (no_abs) return 0.0;
+      const double an = a * conv;
+      return P.eos == EOS_STOKES ? s_tau * (-2.0 * an * c) : s_tau * (-2.0 * an * std::pow(c, y - 1));
+    };
+    auto eta_of = [&](double a, double c) {
+      if (no_disp || P.eos == EOS_STOKES) return 0.0;
+      const double an = a * conv;
+      return s_eta * (2.0 * an * std::pow(c, y) * std::tan(kPi * y / 2));
+    };
+    if (pr->alpha_coeff || pr->c0) {
+      std::vector<float> tv(n), ev(n);
+      parallel_for(n, [&](int64_t i) {
+        const double a = pr->alpha_coeff ? pr->alpha_coeff[i] : pr->alpha_coeff_scalar;
+        const double c = pr->c0 ? pr->c0[i] : pr->c0_scalar;
+        tv[i] = (float)tau_of(a, c);
+        ev[i] = (float)eta_of(a, c);
+      });
+      P.tau = Field{S->upload(tv), 0.f};
+      if (P.eos == EOS_ABSORBING) P.eta = Field{S->upload(ev), 0.f};
+    } else {
+      P.tau = Field{nullptr, (float)tau_of(pr->alpha_coeff_scalar, pr->c0_scalar)};
+      P.eta = Field{nullptr, (float)eta_of(pr->alpha_coeff_scalar, pr->c0_scalar)};
+    }
+    if (pr->alpha_coeff)
+      for (int64_t i = 0; i < n; ++i)
+        if (!(pr->alpha_coeff[i] >= 0)) fail(KW_EINVAL, "medium.alpha_coeff must be non-negative and real.");
+  }
+
+  // PML and shift-derivative operators (kspaceFirstOrder3D.m:619-650)
+  const bool use_sg = pr->use_sg != 0;
+  for (int a = 0; a < dim; ++a) {
+    if (pr->pml_size[a] < 0 || 2 * pr->pml_size[a] > Nd[a]) fail(KW_EINVAL, "PML size larger than the grid");
+    const auto pr_ = pml_profile(Nd[a], d[a], pr->dt, pr->c_ref, (int)pr->pml_size[a], pr->pml_alpha[a], false);
+    const auto ps_ = pml_profile(Nd[a], d[a], pr->dt, pr->c_ref, (int)pr->pml_size[a], pr->pml_alpha[a], use_sg);
+    std::vector<float> f1(Nd[a]), f2(Nd[a]);
+    for (int i = 0; i < Nd[a]; ++i) {
+      f1[i] = (float)pr_[i];
+      f2[i] = (float)ps_[i];
+    }
+    P.pml[a] = S->upload(f1);
+    P.pml_sg[a] = S->upload(f2);
+    const int len = a == 0 ? G.Nx / 2 + 1 : Nd[a];
+    std::vector<cf> dp(len), dn(len);
+    for (int i = 0; i < len; ++i) {
+      const double k = kvec(i, Nd[a], d[a]);
+      std::complex<double> ik(0.0, k);
+      std::complex<double> p = ik, m = ik;
+      if (use_sg) {
+        p = ik * std::exp(std::complex<double>(0.0, k * d[a] / 2));
+        m = ik * std::exp(std::complex<double>(0.0, -k * d[a] / 2));
+      }
+      dp[i] = cf{(float)p.real(), (float)p.imag()};
+      dn[i] = cf{(float)m.real(), (float)m.imag()};
+    }
+    P.dpos[a] = S->upload(dp);
+    P.dneg[a] = S->upload(dn);
+  }
+
+  // spectral tables (kspaceFirstOrder3D.m:653-661; createAbsorptionVariables.m)
+  const double Ntot = (double)G.ncell;
+  const double cref = pr->c_ref, dt = pr->dt;
+  const bool use_k = pr->use_kspace != 0;
+  auto kmag = [&](int ix, int iy, int iz) {
+    const double kx = kvec(ix, G.Nx, d[0]), ky = kvec(iy, G.Ny, d[1]);
+    double acc = 0.0 + kx * kx;
+    acc = acc + ky * ky;
+    if (dim == 3) {
+      const double kz = kvec(iz, G.Nz, d[2]);
+      acc = acc + kz * kz;
+    }
+    return std::sqrt(acc);
+  };
+  P.kappa = S->spectral_table([&](int ix, int iy, int iz) {
+    return (use_k ? sinc(cref * kmag(ix, iy, iz) * dt / 2) : 1.0) / Ntot;
+  });
+  if (P.eos == EOS_ABSORBING) {
+    const bool no_abs = pr->alpha_mode == 1, no_disp = pr->alpha_mode == 2;
+    P.nabla1 = S->spectral_table([&](int ix, int iy, int iz) {
+      if (no_abs) return 0.0;
+      const double v = std::pow(kmag(ix, iy, iz), y - 2);
+      return std::isinf(v) ? 0.0 : v / Ntot;
+    });
+    P.nabla2 = S->spectral_table([&](int ix, int iy, int iz) {
+      if (no_disp) return 0.0;
+      const double v = std::pow(kmag(ix, iy, iz), y - 1);
+      return std::isinf(v) ? 0.0 : v / Ntot;
+    });
+  }
+
+  // fields
+  P.p = S->alloc_zero<float>(n);
+  for (int a = 0; a < dim; ++a) {
+    P.u[a] = S->alloc_zero<float>(n);
+    P.rho[a] = S->alloc_zero<float>(n);
+  }
+  if (P.eos == EOS_ABSORBING) P.rsum = S->alloc_zero<float>(n);
+  const int nS = std::max(dim, 2);
+  for (int b = 0; b < nS; ++b) P.S[b] = S->alloc_zero<cf>(G.nspec);
+  for (int b = nS; b < 3; ++b) P.S[b] = P.S[nS - 1];
+
+  // step counter
+  int32_t* dt_index = S->alloc<int32_t>(1);
+  const std::vector<int32_t> one{1};
+  dev_h2d(dt_index, one.data(), sizeof(int32_t), S->stream);
+  P.t_index = dt_index;
+  S->d_tile0 = S->alloc<int32_t>(1);
+  const std::vector<int32_t> zero{0};
+  dev_h2d(S->d_tile0, zero.data(), sizeof(int32_t), S->stream);
+
+  // sources
+  setup_source(*S, pr->p_source, P.srcp, "source.p");
+  for (int a = 0; a < dim; ++a) setup_source(*S, pr->u_source[a], P.srcu[a], "source.u");
+  const bool need_sk = (P.srcp.len > 0 && P.srcp.mode == SRC_ADDITIVE) ||
+                       [&]() {
+                         for (int a = 0; a < dim; ++a)
+                           if (P.srcu[a].len > 0 && P.srcu[a].mode == SRC_ADDITIVE) return true;
+                         return false;
+                       }();
+  if (need_sk)
+    S->skappa = S->spectral_table([&](int ix, int iy, int iz) {
+      return (use_k ? std::cos(cref * kmag(ix, iy, iz) * dt / 2) : 1.0) / Ntot;
+    });
+  if (S->has_p0) S->p0 = S->upload(to_float(pr->p0, n));
+
+  // time reversal data and map
+  if (S->time_rev) {
+    std::vector<int32_t> map(n, -1);
+    for (int64_t i = 0; i < S->nsens; ++i)
+      map[S->full_mask ? i : (int64_t)pr->sensor_mask_index[i] - 1] = (int32_t)i;
+    P.tr_map = S->upload(map);
+    P.tr_data = S->upload(to_float(pr->time_reversal_boundary_data, S->nsens * pr->Nt));
+  }
+  // sensor outputs
+  if (S->nsens > 0 && !S->time_rev) {
+    if (!S->full_mask) {
+      std::vector<int32_t> idx(S->nsens);
+      for (int64_t i = 0; i < S->nsens; ++i) idx[i] = (int32_t)(pr->sensor_mask_index[i] - 1);
+      P.sens_idx = S->upload(idx);
+    }
+    const int64_t ns = S->nsens;
+    const uint32_t r = S->record;
+    if (r & KW_REC_P) P.p_ts = S->alloc_zero<float>((size_t)ns * S->n_rec);
+    if (r & KW_REC_P_MAX) P.p_max = S->alloc_zero<double>(ns);
+    if (r & KW_REC_P_MIN) P.p_min = S->alloc_zero<double>(ns);
+    if (r & KW_REC_P_RMS) P.p_rms2 = S->alloc_zero<double>(ns);
+    if (r & KW_REC_P_MAX_ALL) P.p_max_all = S->alloc_zero<double>(n);
+    if (r & KW_REC_P_MIN_ALL) P.p_min_all = S->alloc_zero<double>(n);
+    for (int a = 0; a < dim; ++a) {
+      if (r & KW_REC_U) P.u_ts[a] = S->alloc_zero<float>((size_t)ns * S->n_rec);
+      if (r & KW_REC_U_MAX) P.u_max[a] = S->alloc_zero<double>(ns);
+      if (r & KW_REC_U_MIN) P.u_min[a] = S->alloc_zero<double>(ns);
+      if (r & KW_REC_U_RMS) P.u_rms2[a] = S->alloc_zero<double>(ns);
+      if (r & KW_REC_U_MAX_ALL) P.u_max_all[a] = S->alloc_zero<double>(n);
+      if (r & KW_REC_U_MIN_ALL) P.u_min_all[a] = S->alloc_zero<double>(n);
+    }
+  }
+
+  // device copy of the descriptor (needed before any pass runs)
+  S->dP = S->alloc<ProbDev>(1);
+  dev_h2d(S->dP, &S->P, sizeof(ProbDev), S->stream);
+
+  // additive source correction: K_M for single-series sources, per-step
+  // grids for several series
+  bool changed = false;
+  auto prep = [&](const kw_time_source& src, TimeSource& ts, bool& grid_flag) {
+    if (ts.len <= 0 || ts.mode != SRC_ADDITIVE) return;
+    if (ts.nseries == 1) {
+      ts.km = source_km(*S, src);
+    } else {
+      ts.grid = S->alloc_zero<float>(n);
+      grid_flag = true;
+    }
+    changed = true;
+  };
+  prep(pr->p_source, P.srcp, S->p_grid);
+  for (int a = 0; a < dim; ++a) prep(pr->u_source[a], P.srcu[a], S->u_grid[a]);
+  if (changed) dev_h2d(S->dP, &S->P, sizeof(ProbDev), S->stream);
+  dev_sync(S->stream);
+  return S;
+}
+
+// ---------------------------------------------------------------------------
+// time stepping
+// ---------------------------------------------------------------------------
+
+static PassArgs bufs(int nb, int b0 = 0, int b1 = 1, int b2 = 2) {
+  PassArgs a{};
+  a.nbuf = nb;
+  a.buf[0] = b0;
+  a.buf[1] = b1;
+  a.buf[2] = b2;
+  return a;
+}
+
+static void gradient(Sim& S, int xflags) {
+  const int dim = S.G.dim;
+  if (dim == 3) S.pass(PK_YFWD, bufs(1, 0));
+  PassArgs g{};
+  g.mode = SM_GRAD;
+  S.pass(PK_SPEC, g);
+  if (dim == 3) S.pass(PK_YINV, bufs(3));
+  PassArgs x{};
+  x.flags = xflags;
+  S.pass(PK_X_GRAD, x);
+}
+
+static void r2c_p(Sim& S) {
+  PassArgs a = bufs(1, 0);
+  a.real[0] = S.P.p;
+  S.pass(PK_X_R2C, a);
+}
+
+static void src_grid(Sim& S, int which, int t) {
+  const TimeSource& ts = which == 0 ? S.P.srcp : S.P.srcu[which - 1];
+  if (t > ts.len) return;
+  EwArgs e{};
+  e.kind = EW_SRC_GRID;
+  e.which = which;
+  S.ew(e, S.lcell.nb);
+  PassArgs a = bufs(1, 1);
+  a.real[0] = (float*)ts.grid;
+  S.pass(PK_X_R2C, a);
+  if (S.G.dim == 3) S.pass(PK_YFWD, a);
+  PassArgs f = a;
+  f.mode = SM_FILTER;
+  f.table = S.skappa;
+  S.pass(PK_SPEC, f);
+  if (S.G.dim == 3) S.pass(PK_YINV, a);
+  S.pass(PK_X_C2R, a);
+}
+
+static const uint32_t kPointP = KW_REC_P | KW_REC_P_MAX | KW_REC_P_MIN | KW_REC_P_RMS;
+static const uint32_t kPointU = KW_REC_U | KW_REC_U_MAX | KW_REC_U_MIN | KW_REC_U_RMS;
+
+static void record_step(Sim& S, int t, bool init) {
+  if (S.time_rev || S.nsens == 0 || t < S.record_start) return;
+  uint32_t pts = S.record & (kPointP | kPointU);
+  if (S.full_mask && !init) pts &= ~kPointP;  // fused into the p producer
+  if (pts) {
+    EwArgs e{};
+    e.kind = EW_RECORD_POINTS;
+    e.rec = pts;
+    S.ew(e, S.lsens.nb);
+  }
+  uint32_t grid = S.record & (KW_REC_U_MAX_ALL | KW_REC_U_MIN_ALL);
+  if (init) grid |= S.record & (KW_REC_P_MAX_ALL | KW_REC_P_MIN_ALL);
+  if (grid) {
+    EwArgs e{};
+    e.kind = EW_RECORD_GRID;
+    e.rec = grid;
+    S.ew(e, S.lcell.nb);
+  }
+}
+
+static void step_once(Sim& S) {
+  const int t = (int)(S.steps_done + 1);
+  const int dim = S.G.dim;
+  const bool last = S.steps_done + 1 == S.Nt_eff;
+  if (t == 1 && S.has_p0) {
+    // kspaceFirstOrder3D.m:1023-1037: the first step's update is replaced
+    // by the initial condition; the sensor records p0
+    EwArgs e{};
+    e.kind = EW_INIT_P0;
+    e.p0 = S.p0;
+    S.ew(e, S.lcell.nb);
+    r2c_p(S);
+    gradient(S, XG_INIT);
+    record_step(S, t, true);
+    r2c_p(S);
+  } else {
+    if (t == 1 && S.time_rev) {
+      EwArgs e{};
+      e.kind = EW_TR_PROLOGUE;
+      S.ew(e, S.lcell.nb);
+      r2c_p(S);
+    }
+    const bool need_p = (S.nsens > 0 && !S.full_mask && !S.time_rev && t >= S.record_start &&
+                         (S.record & kPointP)) ||
+                        (last && (S.record & KW_REC_P_FINAL));
+    const int pflag = need_p ? X_STORE_P : 0;
+    if (S.p_grid) src_grid(S, 0, t);
+    for (int a = 0; a < dim; ++a)
+      if (S.u_grid[a]) src_grid(S, a + 1, t);
+    gradient(S, 0);
+    if (dim == 3) S.pass(PK_YFWD, bufs(3));
+    PassArgs dv{};
+    dv.mode = SM_DIV;
+    S.pass(PK_SPEC, dv);
+    if (dim == 3) S.pass(PK_YINV, bufs(3));
+    PassArgs xd{};
+    xd.flags = pflag;
+    S.pass(PK_X_DIV, xd);
+    if (S.P.eos == EOS_ABSORBING) {
+      if (dim == 3) S.pass(PK_YFWD, bufs(2));
+      PassArgs ab{};
+      ab.mode = SM_ABS;
+      S.pass(PK_SPEC, ab);
+      if (dim == 3) S.pass(PK_YINV, bufs(2));
+      PassArgs xe{};
+      xe.flags = pflag;
+      S.pass(PK_X_EOS, xe);
+    }
+    record_step(S, t, false);
+  }
+  EwArgs adv{};
+  adv.kind = EW_ADVANCE;
+  S.ew(adv, 1);
+  S.steps_done++;
+}
+
+static void step_n(Sim& S, int64_t nsteps) {
+  if (nsteps < 0) fail(KW_EINVAL, "negative step count");
+  if (S.steps_done + nsteps > S.Nt_eff)
+    fail(KW_ESTATE, "requested steps exceed the number of time steps of the simulation");
+  for (int64_t i = 0; i < nsteps; ++i) step_once(S);
+}
+
+static int64_t recorded(const Sim& S) {
+  if (S.nsens == 0 || S.time_rev) return 0;
+  return std::max<int64_t>(0, std::min<int64_t>(S.steps_done, S.Nt) - S.record_start + 1);
+}
+
+static void read_outputs(Sim& S, kw_outputs* o) {
+  if (!o) fail(KW_EINVAL, "outputs is NULL");
+  dev_sync(S.stream);
+  const ProbDev& P = S.P;
+  const int64_t n = S.G.ncell, ns = S.nsens;
+  const int64_t nr = recorded(S);
+  auto get = [&](void* h, const void* dsrc, size_t bytes) {
+    if (h && dsrc) dev_d2h(h, dsrc, bytes, S.stream);
+  };
+  auto rms = [&](double* h, const double* d2) {
+    if (!h || !d2) return;
+    dev_d2h(h, d2, ns * sizeof(double), S.stream);
+    for (int64_t i = 0; i < ns; ++i) h[i] = nr > 0 ? std::sqrt(h[i] / (double)nr) : 0.0;
+  };
+  get(o->p, P.p_ts, (size_t)ns * S.n_rec * sizeof(float));
+  get(o->p_max, P.p_max, ns * sizeof(double));
+  get(o->p_min, P.p_min, ns * sizeof(double));
+  rms(o->p_rms, P.p_rms2);
+  get(o->p_max_all, P.p_max_all, n * sizeof(double));
+  get(o->p_min_all, P.p_min_all, n * sizeof(double));
+  if (o->p_final) get(o->p_final, P.p, n * sizeof(float));
+  for (int a = 0; a < S.G.dim; ++a) {
+    get(o->u[a], P.u_ts[a], (size_t)ns * S.n_rec * sizeof(float));
+    get(o->u_max[a], P.u_max[a], ns * sizeof(double));
+    get(o->u_min[a], P.u_min[a], ns * sizeof(double));
+    rms(o->u_rms[a], P.u_rms2[a]);
+    get(o->u_max_all[a], P.u_max_all[a], n * sizeof(double));
+    get(o->u_min_all[a], P.u_min_all[a], n * sizeof(double));
+    if (o->u_final[a]) get(o->u_final[a], P.u[a], n * sizeof(float));
+  }
+}
+
+// kw_smooth: one filter pass on a scratch problem
+static void smooth(int dim, const int64_t N[3], const double* in, const double* win, double* out) {
+  kw_problem pr{};
+  pr.dim = dim;
+  pr.Nx = N[0];
+  pr.Ny = N[1];
+  pr.Nz = dim == 3 ? N[2] : 1;
+  pr.Nt = 1;
+  pr.dx = pr.dy = pr.dz = 1.0;
+  pr.dt = 1.0;
+  pr.c_ref = 1.0;
+  pr.c0_scalar = 1.0;
+  pr.rho0_scalar = 1.0;
+  pr.use_kspace = 1;
+  pr.use_sg = 1;
+  auto S = create_sim(&pr);
+  const Geometry& G = S->G;
+  const double Ntot = (double)G.ncell;
+  const int Nd[3] = {G.Nx, G.Ny, G.Nz};
+  // ifftshift(win)[i] = win[(i + floor(N/2)) mod N] on each axis
+  float* table = S->spectral_table([&](int ix, int iy, int iz) {
+    const int64_t wx = (ix + Nd[0] / 2) % Nd[0];
+    const int64_t wy = (iy + Nd[1] / 2) % Nd[1];
+    const int64_t wz = dim == 3 ? (iz + Nd[2] / 2) % Nd[2] : 0;
+    return win[wx + (int64_t)Nd[0] * (wy + (int64_t)Nd[1] * wz)] / Ntot;
+  });
+  float* f = S->upload(to_float(in, G.ncell));
+  PassArgs a = bufs(1, 0);
+  a.real[0] = f;
+  S->pass(PK_X_R2C, a);
+  if (dim == 3) S->pass(PK_YFWD, a);
+  PassArgs fl = a;
+  fl.mode = SM_FILTER;
+  fl.table = table;
+  S->pass(PK_SPEC, fl);
+  if (dim == 3) S->pass(PK_YINV, a);
+  S->pass(PK_X_C2R, a);
+  std::vector<float> h(G.ncell);
+  dev_d2h(h.data(), f, G.ncell * sizeof(float), S->stream);
+  for (int64_t i = 0; i < G.ncell; ++i) out[i] = h[i];
+}
+
+}  // namespace kw
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+
+struct kw_sim {
+  std::unique_ptr<kw::Sim> s;
+};
+
+static void set_err(char* err, size_t errlen, const char* m) {
+  if (err && errlen) {
+    std::snprintf(err, errlen, "%s", m);
+  }
+}
+
+template <class F>
+static int guarded(char* err, size_t errlen, F f) {
+  try {
+    f();
+    set_err(err, errlen, "");
+    return KW_OK;
+  } catch (const kw::DeviceError& e) {
+    set_err(err, errlen, e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    set_err(err, errlen, "host allocation failed");
+    return KW_ENOMEM;
+  } catch (const std::exception& e) {
+    set_err(err, errlen, e.what());
+    return KW_EDEVICE;
+  }
+}
+
+static const char* kKindNames[KW_NKINDS] = {"y_fwd", "y_inv", "spectral", "x_r2c", "x_c2r",
+                                            "x_grad", "x_div", "x_eos", "elementwise"};
+
+extern "C" {
+
+int kw_abi_version(void) { return KW_ABI_VERSION; }
+
+const char* kw_backend(void) { return kw::backend_name(); }
+
+const char* kw_kind_name(int k) { return (k >= 0 && k < KW_NKINDS) ? kKindNames[k] : ""; }
+
+int kw_output_sizes(const kw_problem* prob, kw_sizes* sizes, char* err, size_t errlen) {
+  return guarded(err, errlen, [&]() {
+    if (!prob || !sizes) kw::fail(KW_EINVAL, "NULL argument");
+    const bool tr = prob->time_reversal_boundary_data != nullptr;
+    const int64_t rs = prob->record_start_index > 0 ? prob->record_start_index : 1;
+    sizes->num_sensor_points = prob->sensor_num_points;
+    sizes->num_recorded_time_points =
+        (prob->sensor_num_points > 0 && !tr) ? std::max<int64_t>(0, prob->Nt - rs + 1) : 0;
+    sizes->grid_cells = prob->Nx * prob->Ny * (prob->dim == 3 ? prob->Nz : 1);
+  });
+}
+
+int kw_sim_create(const kw_problem* prob, kw_sim** sim, char* err, size_t errlen) {
+  return guarded(err, errlen, [&]() {
+    if (!sim) kw::fail(KW_EINVAL, "NULL handle pointer");
+    *sim = nullptr;
+    auto s = kw::create_sim(prob);
+    *sim = new kw_sim{std::move(s)};
+  });
+}
+
+int kw_sim_step(kw_sim* sim, int64_t nsteps, char* err, size_t errlen) {
+  return guarded(err, errlen, [&]() {
+    if (!sim) kw::fail(KW_EINVAL, "NULL handle");
+    kw::step_n(*sim->s, nsteps);
+  });
+}
+
+int kw_sim_sync(kw_sim* sim, char* err, size_t errlen) {
+  return guarded(err, errlen, [&]() {
+    if (!sim) kw::fail(KW_EINVAL, "NULL handle");
+    kw::dev_sync(sim->s->stream);
+  });
+}
+
+int kw_sim_read(kw_sim* sim, kw_outputs* out, char* err, size_t errlen) {
+  return guarded(err, errlen, [&]() {
+    if (!sim) kw::fail(KW_EINVAL, "NULL handle");
+    kw::read_outputs(*sim->s, out);
+  });
+}
+
+int64_t kw_sim_steps_done(const kw_sim* sim) { return sim ? sim->s->steps_done : -1; }
+
+int64_t kw_sim_device_bytes(const kw_sim* sim) { return sim ? sim->s->bytes : -1; }
+
+int kw_sim_destroy(kw_sim* sim) {
+  if (!sim) return KW_OK;
+  try {
+    kw::dev_sync(sim->s->stream);
+  } catch (...) {
+  }
+  delete sim;
+  return KW_OK;
+}
+
+int kw_sim_step_timed(kw_sim* sim, int64_t nsteps, double* ms, kw_kernel_stats* stats,
+                      char* err, size_t errlen) {
+  return guarded(err, errlen, [&]() {
+    if (!sim) kw::fail(KW_EINVAL, "NULL handle");
+    kw::Sim& S = *sim->s;
+    void* a = kw::dev_event_create();
+    void* b = kw::dev_event_create();
+    S.timing = stats != nullptr;
+    try {
+      kw::dev_sync(S.stream);
+      kw::dev_event_record(a, S.stream);
+      kw::step_n(S, nsteps);
+      kw::dev_event_record(b, S.stream);
+      const double t = kw::dev_event_elapsed_ms(a, b);
+      if (ms) *ms = t;
+      if (stats) {
+        for (int k = 0; k < KW_NKINDS; ++k) {
+          stats->ms[k] = 0.0;
+          stats->launches[k] = 0;
+          stats->bytes[k] = 0.0;
+        }
+        for (size_t i = 0; i < S.ev_used.size(); ++i) {
+          const int k = S.ev_kind[i];
+          stats->ms[k] += kw::dev_event_elapsed_ms(S.ev_used[i].first, S.ev_used[i].second);
+          stats->launches[k] += 1;
+          stats->bytes[k] += S.ev_bytes[i];
+          S.ev_free.push_back(S.ev_used[i].first);
+          S.ev_free.push_back(S.ev_used[i].second);
+        }
+      }
+    } catch (...) {
+      S.timing = false;
+      kw::dev_event_destroy(a);
+      kw::dev_event_destroy(b);
+      throw;
+    }
+    S.ev_used.clear();
+    S.ev_kind.clear();
+    S.ev_bytes.clear();
+    S.timing = false;
+    kw::dev_event_destroy(a);
+    kw::dev_event_destroy(b);
+  });
+}
+
+int kw_run(const kw_problem* prob, kw_outputs* out, char* err, size_t errlen) {
+  return guarded(err, errlen, [&]() {
+    auto s = kw::create_sim(prob);
+    kw::step_n(*s, s->Nt_eff);
+    kw::read_outputs(*s, out);
+  });
+}
+
+int kw_smooth(int32_t dim, const int64_t N[3], const double* in, const double* win, double* out,
+              char* err, size_t errlen) {
+  return guarded(err, errlen, [&]() {
+    if (!N || !in || !win || !out) kw::fail(KW_EINVAL, "NULL argument");
+    if (dim != 2 && dim != 3) kw::fail(KW_EINVAL, "dim must be 2 or 3");
+    kw::smooth(dim, N, in, win, out);
+  });
+}
+
+}  // extern "C"

@@ -1,0 +1,10 @@
+"""kwhip: MI355X-native k-space pseudospectral time stepping with the call
+signature of k-Wave's kspaceFirstOrder2D / kspaceFirstOrder3D."""
+
+from ._abi import KWError, load
+from .kgrid import kWaveGrid
+from .sim import Simulation
+from .solver import cart2grid, kspaceFirstOrder2D, kspaceFirstOrder3D, prepare, smooth
+
+__all__ = ["kWaveGrid", "kspaceFirstOrder2D", "kspaceFirstOrder3D", "Simulation", "prepare",
+           "smooth", "cart2grid", "KWError", "load"]

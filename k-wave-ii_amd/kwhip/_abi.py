@@ -1,0 +1,191 @@
+"""ctypes view of the C ABI in include/kwhip.h.
+
+The product library is ``libkwhip.so`` (HIP, gfx950).  ``libkwhip_emu.so`` is
+the same host code over the CPU lane emulator; it is loaded only when a caller
+asks for it explicitly (``backend="emu"``), which only the test-suite does.
+There is no silent fallback: if the HIP library is missing, loading fails.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import os
+from pathlib import Path
+
+import numpy as np
+
+LIB_DIR = Path(__file__).resolve().parent / "_lib"
+LIB_NAMES = {"hip": "libkwhip.so", "emu": "libkwhip_emu.so"}
+
+KW_OK = 0
+KW_EINVAL = -1
+KW_EUNSUP = -2
+KW_EDEVICE = -3
+KW_ENOMEM = -4
+KW_ESTATE = -5
+
+REC = {
+    "p": 1 << 0, "p_max": 1 << 1, "p_min": 1 << 2, "p_rms": 1 << 3,
+    "p_max_all": 1 << 4, "p_min_all": 1 << 5, "p_final": 1 << 6,
+    "u": 1 << 7, "u_max": 1 << 8, "u_min": 1 << 9, "u_rms": 1 << 10,
+    "u_max_all": 1 << 11, "u_min_all": 1 << 12, "u_final": 1 << 13,
+}
+SRC_MODE = {"dirichlet": 0, "additive-no-correction": 1, "additive": 2}
+NKINDS = 9
+
+c_double_p = C.POINTER(C.c_double)
+c_float_p = C.POINTER(C.c_float)
+c_u64_p = C.POINTER(C.c_uint64)
+
+
+class TimeSource(C.Structure):
+    _fields_ = [
+        ("flag", C.c_int64),
+        ("mode", C.c_int32),
+        ("num_points", C.c_int64),
+        ("index", c_u64_p),
+        ("num_series", C.c_int64),
+        ("sig_index", c_u64_p),
+        ("input", c_double_p),
+    ]
+
+
+class Problem(C.Structure):
+    _fields_ = [
+        ("dim", C.c_int32),
+        ("Nx", C.c_int64), ("Ny", C.c_int64), ("Nz", C.c_int64),
+        ("Nt", C.c_int64),
+        ("dx", C.c_double), ("dy", C.c_double), ("dz", C.c_double), ("dt", C.c_double),
+        ("c_ref", C.c_double),
+        ("c0", c_double_p), ("c0_scalar", C.c_double),
+        ("rho0", c_double_p), ("rho0_scalar", C.c_double),
+        ("rho0_sg", c_double_p * 3),
+        ("nonlinear_flag", C.c_int32),
+        ("BonA", c_double_p), ("BonA_scalar", C.c_double),
+        ("absorbing_flag", C.c_int32),
+        ("alpha_coeff", c_double_p), ("alpha_coeff_scalar", C.c_double),
+        ("alpha_power", C.c_double),
+        ("alpha_mode", C.c_int32),
+        ("alpha_sign", C.c_double * 2),
+        ("pml_size", C.c_int64 * 3),
+        ("pml_alpha", C.c_double * 3),
+        ("use_kspace", C.c_int32),
+        ("use_sg", C.c_int32),
+        ("p0", c_double_p),
+        ("p_source", TimeSource),
+        ("u_source", TimeSource * 3),
+        ("sensor_num_points", C.c_int64),
+        ("sensor_mask_index", c_u64_p),
+        ("record_flags", C.c_uint32),
+        ("record_start_index", C.c_int64),
+        ("time_reversal_boundary_data", c_double_p),
+    ]
+
+
+class Sizes(C.Structure):
+    _fields_ = [("num_sensor_points", C.c_int64), ("num_recorded_time_points", C.c_int64),
+                ("grid_cells", C.c_int64)]
+
+
+class Outputs(C.Structure):
+    _fields_ = [
+        ("p", c_float_p), ("p_max", c_double_p), ("p_min", c_double_p), ("p_rms", c_double_p),
+        ("p_max_all", c_double_p), ("p_min_all", c_double_p), ("p_final", c_float_p),
+        ("u", c_float_p * 3), ("u_max", c_double_p * 3), ("u_min", c_double_p * 3),
+        ("u_rms", c_double_p * 3), ("u_max_all", c_double_p * 3), ("u_min_all", c_double_p * 3),
+        ("u_final", c_float_p * 3),
+    ]
+
+
+class KernelStats(C.Structure):
+    _fields_ = [("ms", C.c_double * NKINDS), ("launches", C.c_int64 * NKINDS),
+                ("bytes", C.c_double * NKINDS)]
+
+
+class KWError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"kwhip error {code}: {msg}")
+        self.code = code
+
+
+_libs: dict[str, C.CDLL] = {}
+
+
+def library_path(backend: str = "hip") -> Path:
+    return LIB_DIR / LIB_NAMES[backend]
+
+
+def load(backend: str = "hip") -> C.CDLL:
+    """Load the kwhip library.  ``backend="hip"`` is the product; a missing
+    library is an error (no fallback)."""
+    if backend in _libs:
+        return _libs[backend]
+    path = library_path(backend)
+    if not path.exists():
+        raise ImportError(f"{path} is not built; run __graft_entry__.build() first")
+    lib = C.CDLL(str(path))
+    P = C.POINTER
+    lib.kw_abi_version.restype = C.c_int
+    lib.kw_backend.restype = C.c_char_p
+    lib.kw_kind_name.restype = C.c_char_p
+    lib.kw_kind_name.argtypes = [C.c_int]
+    err = [C.c_char_p, C.c_size_t]
+    lib.kw_output_sizes.argtypes = [P(Problem), P(Sizes)] + err
+    lib.kw_run.argtypes = [P(Problem), P(Outputs)] + err
+    lib.kw_sim_create.argtypes = [P(Problem), P(C.c_void_p)] + err
+    lib.kw_sim_step.argtypes = [C.c_void_p, C.c_int64] + err
+    lib.kw_sim_sync.argtypes = [C.c_void_p] + err
+    lib.kw_sim_read.argtypes = [C.c_void_p, P(Outputs)] + err
+    lib.kw_sim_steps_done.argtypes = [C.c_void_p]
+    lib.kw_sim_steps_done.restype = C.c_int64
+    lib.kw_sim_device_bytes.argtypes = [C.c_void_p]
+    lib.kw_sim_device_bytes.restype = C.c_int64
+    lib.kw_sim_destroy.argtypes = [C.c_void_p]
+    lib.kw_sim_step_timed.argtypes = [C.c_void_p, C.c_int64, P(C.c_double), P(KernelStats)] + err
+    lib.kw_smooth.argtypes = [C.c_int32, P(C.c_int64), c_double_p, c_double_p, c_double_p] + err
+    if lib.kw_abi_version() != 1:
+        raise ImportError("kwhip ABI version mismatch")
+    _libs[backend] = lib
+    return lib
+
+
+def call(fn, *args):
+    buf = C.create_string_buffer(1024)
+    rc = fn(*args, buf, C.c_size_t(len(buf)))
+    if rc != KW_OK:
+        raise KWError(rc, buf.value.decode(errors="replace"))
+
+
+def dptr(a):
+    """double* of a C-contiguous float64 array (None -> NULL)."""
+    if a is None:
+        return None
+    return a.ctypes.data_as(c_double_p)
+
+
+def fptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(c_float_p)
+
+
+def uptr(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(c_u64_p)
+
+
+def as_f64(a):
+    return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
+
+
+def fortran_flat(a):
+    """Column-major (MATLAB) flattening of a grid array as float64."""
+    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel(order="F"))
+
+
+def backend_name(lib) -> str:
+    return lib.kw_backend().decode()
+
+
+os.environ.setdefault("HSA_ENABLE_IPC_MODE_LEGACY", "0")

@@ -1,0 +1,587 @@
+"""Drop-in kspaceFirstOrder2D / kspaceFirstOrder3D over the kwhip C ABI.
+
+Mirrors the reference call signature
+
+    sensor_data = kspaceFirstOrder2D(kgrid, medium, source, sensor, 'Name', value, ...)
+
+(+kwave/+legacy/kspaceFirstOrder2D.m:1, kspaceFirstOrder3D.m:1).  ``medium``,
+``source`` and ``sensor`` are dicts (or attribute objects) with the
+reference's field names; name-value options are given as keyword arguments
+or as trailing ('Name', value) pairs.
+
+This module does the reference's host-side precompute -- input checking,
+reference sound speed, time array, smoothing (the transform runs on the
+device through ``kw_smooth``), staggered density, source scaling, sensor and
+source index lists, Cartesian-to-grid mapping -- and then hands the variable
+set of the reference's native-offload contract to the device library, which
+builds the operators and runs the time loop.  Output selection follows
+kspaceFirstOrder3D.m:1360-1383.
+
+Precision: the device computes fields and FFTs in float32 with operators
+built in float64 and rounded once, and keeps the statistics (max, min, rms)
+in float64.  Time series and final fields are returned as float32,
+statistics as float64.
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import math
+from types import SimpleNamespace
+
+import numpy as np
+
+from . import _abi
+from .kgrid import kWaveGrid
+
+# options accepted for compatibility that only affect plotting / logging
+_VISUAL_OPTIONS = {"PlotSim", "PlotLayout", "PlotPML", "PlotScale", "PlotFreq", "DisplayMask",
+                   "LogScale", "RecordMovie", "MovieName", "MovieArgs", "MovieProfile",
+                   "CreateLog", "MeshPlot"}
+_OPTIONS = {"PMLSize", "PMLAlpha", "PMLInside", "Smooth", "DataCast", "DataRecast",
+            "CartInterp", "UsekSpace", "UseSG", "ScaleSourceTerms", "Backend"} | _VISUAL_OPTIONS
+_UNSUPPORTED = {"SaveToDisk", "StreamToDisk", "SaveToDiskExit", "HDFCompressionLevel",
+                "PMLRange", "MultiAxialPMLRatio", "UseFD"}
+
+RECORD_ALLOWED = ("p", "p_max", "p_min", "p_rms", "p_max_all", "p_min_all", "p_final",
+                  "u", "u_max", "u_min", "u_rms", "u_max_all", "u_min_all", "u_final")
+
+
+def _get(obj, name, default=None):
+    if obj is None:
+        return default
+    if isinstance(obj, dict):
+        v = obj.get(name, default)
+    else:
+        v = getattr(obj, name, default)
+    return default if v is None else v
+
+
+def _has(obj, name):
+    return _get(obj, name) is not None
+
+
+def _parse_options(args, kwargs):
+    opts = dict(kwargs)
+    if len(args) % 2:
+        raise ValueError("Optional inputs must be entered as Param, Value pairs.")
+    for i in range(0, len(args), 2):
+        opts[args[i]] = args[i + 1]
+    for k in opts:
+        if k in _UNSUPPORTED:
+            raise NotImplementedError(f"option '{k}' is not supported by kwhip")
+        if k not in _OPTIONS:
+            raise ValueError(f"Unknown optional input '{k}'.")
+    return opts
+
+
+# ---------------------------------------------------------------------------
+# smoothing window (+kwave/+legacy/getWin.m, Blackman, rotated)
+# ---------------------------------------------------------------------------
+
+def _blackman(N: int, a: float = 0.16) -> np.ndarray:
+    # getWin.m:230-231 and cosineSeries (:417-424)
+    n = np.arange(N, dtype=np.float64)
+    coeffs = [(1 - a) / 2, 0.5, a / 2]
+    w = coeffs[0] * np.ones(N)
+    for i in range(1, 3):
+        w = w + (-1) ** i * coeffs[i] * np.cos(i * 2 * math.pi * n / (N - 1))
+    return w
+
+
+def blackman_window(grid_size) -> np.ndarray:
+    """abs(getWin(size, 'Blackman', 'Rotation', true, 'Symmetric', rem(size,2)))
+    as used by smooth.m: odd sizes symmetric; even sizes built one longer and
+    trimmed (getWin.m:188, :300-347)."""
+    size = np.asarray(grid_size, dtype=np.int64)
+    symmetric = (size % 2).astype(bool)
+    N = size + (~symmetric).astype(np.int64)
+    L = int(N.max())
+    lin = _blackman(L)
+    radius = (L - 1) / 2.0
+    ref = np.linspace(-radius, radius, L)
+    axes = [np.linspace(-radius, radius, int(n)) for n in N]
+    r2 = np.zeros(tuple(int(n) for n in N))
+    for ax, v in enumerate(axes):
+        sh = [1] * len(N)
+        sh[ax] = -1
+        r2 = r2 + v.reshape(sh) ** 2
+    r = np.sqrt(r2)
+    r[r > radius] = radius
+    win = np.interp(r, ref, lin)
+    sl = tuple(slice(0, int(s)) for s in size)
+    return np.abs(win[sl])
+
+
+def smooth(mat, restore_max=False, backend="hip"):
+    """+kwave/+legacy/smooth.m: filter with the rotated Blackman window in
+    k-space (on the device), optionally restoring the maximum magnitude."""
+    lib = _abi.load(backend)
+    mat = np.asarray(mat, dtype=np.float64)
+    if mat.ndim not in (2, 3):
+        raise ValueError("smooth supports 2D and 3D grids")
+    win = blackman_window(mat.shape)
+    N = (C.c_int64 * 3)(*(list(mat.shape) + [1] * (3 - mat.ndim)))
+    src = _abi.fortran_flat(mat)
+    w = _abi.fortran_flat(win)
+    out = np.empty_like(src)
+    _abi.call(lib.kw_smooth, C.c_int32(mat.ndim), N, _abi.dptr(src), _abi.dptr(w), _abi.dptr(out))
+    res = out.reshape(mat.shape, order="F")
+    if restore_max:
+        res = (np.max(np.abs(mat)) / np.max(np.abs(res))) * res
+    return res
+
+
+# ---------------------------------------------------------------------------
+# Cartesian sensor masks (+kwave/+legacy/cart2grid.m, nearest)
+# ---------------------------------------------------------------------------
+
+def cart2grid(kgrid: kWaveGrid, cart_data):
+    """cart2grid.m:59-194: nearest grid point with MATLAB round (half away
+    from zero), later points overwrite earlier ones.  Returns the binary mask,
+    order_index and reorder_index (1-based)."""
+    cart = np.asarray(cart_data, dtype=np.float64)
+    if cart.ndim != 2 or cart.shape[0] != kgrid.dim:
+        raise ValueError(f"Cartesian sensor.mask for a {kgrid.dim}D simulation must be given as a {kgrid.dim} by N array.")
+    idx = []
+    for ax in range(kgrid.dim):
+        v = cart[ax] / kgrid.spacings[ax]
+        v = np.sign(v) * np.floor(np.abs(v) + 0.5)
+        idx.append(v.astype(np.int64) + kgrid.shape[ax] // 2 + 1)
+        if idx[ax].max() > kgrid.shape[ax] or idx[ax].min() < 1:
+            raise ValueError("Cartesian points must lie within the grid defined by kgrid.")
+    grid = np.zeros(kgrid.shape, dtype=np.int64)
+    for p in range(cart.shape[1]):
+        grid[tuple(int(idx[ax][p]) - 1 for ax in range(kgrid.dim))] = p + 1
+    flat = grid.ravel(order="F")
+    reorder_index = flat[flat != 0]
+    order_index = np.argsort(reorder_index, kind="stable") + 1
+    return (grid != 0), order_index, reorder_index
+
+
+def _find(mask) -> np.ndarray:
+    return np.flatnonzero(np.asarray(mask).ravel(order="F") != 0)
+
+
+# ---------------------------------------------------------------------------
+# problem preparation
+# ---------------------------------------------------------------------------
+
+class PreparedProblem:
+    """A kw_problem plus the arrays it points to and the post-processing
+    information; keeps everything alive while the device library uses it."""
+
+    def __init__(self):
+        self.keep = []
+        self.prob = _abi.Problem()
+        self.info = SimpleNamespace()
+
+    def hold(self, a):
+        if a is not None:
+            self.keep.append(a)
+        return a
+
+
+def _source_series(src, nm, n_points):
+    s = np.atleast_2d(np.asarray(_get(src, nm), dtype=np.float64)).copy()
+    return s
+
+
+def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> PreparedProblem:
+    """Host precompute of kspaceFirstOrder{2D,3D} up to the native-offload
+    variable set (kspaceFirstOrder_inputChecking.m, kspaceFirstOrder3D.m:585-
+    607, kspaceFirstOrder_scaleSourceTerms.m)."""
+    if kgrid.dim != dim:
+        raise ValueError(f"kgrid must be {dim}D for kspaceFirstOrder{dim}D")
+    backend = opts.get("Backend", "hip")
+    pp = PreparedProblem()
+    pr = pp.prob
+    shape = kgrid.shape
+    ncell = kgrid.total_grid_points
+    if not bool(opts.get("PMLInside", True)):
+        raise NotImplementedError("'PMLInside', false is not supported by kwhip")
+
+    # -- medium (inputChecking.m:148-281) -----------------------------------
+    if medium is None or not _has(medium, "sound_speed"):
+        raise ValueError("The field medium.sound_speed must be defined.")
+    c = np.asarray(_get(medium, "sound_speed"), dtype=np.float64)
+    if _has(medium, "density"):
+        rho = np.asarray(_get(medium, "density"), dtype=np.float64)
+    elif c.size == 1:
+        rho = np.asarray(1.0)
+    else:
+        raise ValueError("The field medium.density must be defined.")
+    for nm, v in (("sound_speed", c), ("density", rho)):
+        if v.size != 1 and v.shape != shape:
+            raise ValueError(f"medium.{nm} must be scalar or the same size as the grid")
+    absorbing = _has(medium, "alpha_coeff") or _has(medium, "alpha_power")
+    alpha_mode = _get(medium, "alpha_mode")
+    if alpha_mode is not None and alpha_mode not in ("no_absorption", "no_dispersion", "stokes"):
+        raise ValueError("medium.alpha_mode must be set to 'no_absorption', 'no_dispersion', or 'stokes'.")
+    if _has(medium, "alpha_filter"):
+        raise NotImplementedError("medium.alpha_filter is not supported by kwhip")
+    nonlinear = _has(medium, "BonA")
+    if nonlinear and not _has(medium, "density"):
+        raise ValueError("medium.density must be defined for nonlinear simulations")
+
+    cref_opt = _get(medium, "sound_speed_ref")
+    if cref_opt is None or (isinstance(cref_opt, str) and cref_opt == "max"):
+        c_ref = float(c.max())
+    elif isinstance(cref_opt, str) and cref_opt == "min":
+        c_ref = float(c.min())
+    elif isinstance(cref_opt, str) and cref_opt == "mean":
+        c_ref = float(c.mean())
+    elif isinstance(cref_opt, str):
+        raise ValueError("Unknown input for medium.sound_speed_ref.")
+    else:
+        c_ref = float(cref_opt)
+
+    # -- time (inputChecking.m:1227-1281) -----------------------------------
+    if isinstance(kgrid.Nt, str) or isinstance(kgrid.dt, str):
+        if sensor is not None and _has(sensor, "time_reversal_boundary_data"):
+            raise ValueError("kgrid.t_array must be explicitly defined for time reversal")
+        kgrid.makeTime(c, 0.3)
+    Nt, dt = int(kgrid.Nt), float(kgrid.dt)
+
+    # -- options --------------------------------------------------------------
+    pml_size = opts.get("PMLSize", 20 if dim == 2 else 10)
+    pml_alpha = opts.get("PMLAlpha", 2.0)
+    pml_size = np.broadcast_to(np.asarray(pml_size, dtype=np.int64), (dim,))
+    pml_alpha = np.broadcast_to(np.asarray(pml_alpha, dtype=np.float64), (dim,))
+    sm = np.atleast_1d(np.asarray(opts.get("Smooth", [True, False, False]), dtype=bool))
+    smooth_p0, smooth_c0, smooth_rho0 = bool(sm[0]), bool(sm[(len(sm) + 1) // 2 - 1]), bool(sm[-1])
+    cart_interp = opts.get("CartInterp", "linear")
+    data_cast = opts.get("DataCast", "off")
+    if data_cast not in ("off", "double", "single", "gpuArray-single", "gpuArray-double"):
+        raise ValueError(f"unknown DataCast '{data_cast}'")
+
+    # -- sensor ---------------------------------------------------------------
+    info = pp.info
+    info.dim, info.shape, info.Nt = dim, shape, Nt
+    info.record_given = False
+    info.reorder_index = None
+    info.time_rev = False
+    info.use_sensor = sensor is not None and _has(sensor, "mask")
+    record = ["p"]
+    tr_data = None
+    sens_idx = None
+    if info.use_sensor:
+        info.time_rev = _has(sensor, "time_reversal_boundary_data")
+        if _has(sensor, "record"):
+            record = list(_get(sensor, "record"))
+            info.record_given = True
+            bad = [r for r in record if r not in RECORD_ALLOWED]
+            if bad:
+                raise NotImplementedError(f"sensor.record entries not supported by kwhip: {bad}")
+        mask = np.asarray(_get(sensor, "mask"))
+        if mask.shape == shape:
+            sens_idx = _find(mask)
+            if info.time_rev:
+                tr_data = np.asarray(_get(sensor, "time_reversal_boundary_data"), dtype=np.float64)
+        elif mask.ndim == 2 and mask.shape[0] == dim:
+            if cart_interp != "nearest" and not info.time_rev:
+                raise NotImplementedError(
+                    "CartInterp 'linear' (Delaunay interpolation) is not supported; use 'nearest'")
+            gmask, order_index, reorder_index = cart2grid(kgrid, mask)
+            sens_idx = _find(gmask)
+            info.reorder_index = reorder_index
+            if info.time_rev:
+                d = np.asarray(_get(sensor, "time_reversal_boundary_data"), dtype=np.float64)
+                tr_data = d[np.argsort(order_index, kind="stable")]
+        else:
+            raise ValueError("sensor.mask must be a binary grid or a dim x N Cartesian array")
+        if info.time_rev:
+            record = []
+            if tr_data.shape != (len(sens_idx), Nt):
+                raise ValueError("sensor.time_reversal_boundary_data must be [num_sensor_points x Nt]")
+    info.record = record
+    rec_flags = 0
+    for r in record:
+        rec_flags |= _abi.REC[r]
+    record_start = int(_get(sensor, "record_start_index", 1)) if info.use_sensor else 1
+    if info.use_sensor and not info.time_rev and record_start > Nt:
+        raise ValueError("sensor.record_start_index must be less than or equal to kgrid.Nt")
+
+    # -- sources (inputChecking.m:748-1014) -----------------------------------
+    p0 = None
+    if source is not None and _has(source, "p0"):
+        p0 = np.asarray(_get(source, "p0"), dtype=np.float64)
+        if p0.shape != shape:
+            raise ValueError("source.p0 must be the same size as the computational grid.")
+        if _has(source, "p"):
+            raise ValueError("source.p0 and source.p can't be defined in the same simulation.")
+
+    def time_source(names, mask_name, mode_name):
+        present = [nm for nm in names if source is not None and _has(source, nm)]
+        if not present:
+            return None
+        m = np.asarray(_get(source, mask_name))
+        if m.shape != shape:
+            raise ValueError(f"source.{mask_name} must be the same size as the computational grid.")
+        pos = _find(m)
+        if len(pos) == 0:
+            raise ValueError(f"source.{mask_name} must be a binary grid with at least one element set to 1.")
+        mode = _get(source, mode_name, "additive")
+        if mode not in _abi.SRC_MODE:
+            raise ValueError(f"source.{mode_name} must be set to 'additive', 'additive-no-correction', or 'dirichlet'.")
+        uniq = np.unique(m)
+        labelled = not (uniq.size <= 2 and uniq.sum() == 1)
+        sig_index = None
+        if labelled:
+            sig_index = np.asarray(m).ravel(order="F")[pos].astype(np.uint64)
+        sigs = {nm: np.atleast_2d(np.asarray(_get(source, nm), dtype=np.float64)).copy()
+                for nm in present}
+        for nm, s in sigs.items():
+            if labelled and s.shape[0] != len(uniq) - 1:
+                raise ValueError("The number of time series must match the number of labelled source elements.")
+            if not labelled and s.shape[0] > 1 and s.shape[0] != len(pos):
+                raise ValueError("The number of time series must match the number of source elements.")
+        return SimpleNamespace(pos=pos, mode=mode, sigs=sigs, sig_index=sig_index)
+
+    src_p = time_source(["p"], "p_mask", "p_mode")
+    u_names = ["ux", "uy", "uz"][:dim]
+    src_u = time_source(u_names, "u_mask", "u_mode")
+
+    # -- smoothing (inputChecking.m:1501-1600) ---------------------------------
+    if p0 is not None and smooth_p0:
+        p0 = smooth(p0, True, backend=backend)
+    if smooth_c0 and c.size > 1:
+        c = smooth(c, False, backend=backend)
+    if smooth_rho0 and rho.size > 1:
+        rho = smooth(rho, False, backend=backend)
+
+    # -- source scaling (kspaceFirstOrder_scaleSourceTerms.m:43-391) ----------
+    scale = bool(opts.get("ScaleSourceTerms", True))
+    c_flat = c.ravel(order="F") if c.size > 1 else None
+    if src_p is not None and scale:
+        s = src_p.sigs["p"]
+        if src_p.mode == "dirichlet":
+            if c.size == 1:
+                s = s / (dim * float(c) ** 2)
+            else:
+                for i in range(s.shape[0]):  # row i scaled by source point i (reference behaviour)
+                    s[i] = s[i] / (dim * c_flat[src_p.pos[i]] ** 2)
+        else:
+            if c.size == 1:
+                s = s * (2 * dt / (dim * float(c) * kgrid.dx))
+            else:
+                for i in range(s.shape[0]):
+                    s[i] = s[i] * (2 * dt / (dim * c_flat[src_p.pos[i]] * kgrid.dx))
+        src_p.sigs["p"] = s
+    if src_u is not None and scale and src_u.mode != "dirichlet":
+        for ax, nm in enumerate(u_names):
+            if nm not in src_u.sigs:
+                continue
+            s = src_u.sigs[nm]
+            dd = kgrid.spacings[ax]
+            if c.size == 1:
+                s = s * (2 * float(c) * dt / dd)
+            else:
+                for i in range(s.shape[0]):
+                    s[i] = s[i] * (2 * c_flat[src_u.pos[i]] * dt / dd)
+            src_u.sigs[nm] = s
+
+    # -- staggered density (kspaceFirstOrder3D.m:585-607) ----------------------
+    use_sg = bool(opts.get("UseSG", True))
+    rho_sg = [None] * 3
+    if rho.size > 1 and use_sg:
+        for ax in range(dim):
+            r = np.moveaxis(rho, ax, 0)
+            sg = r.copy()
+            sg[:-1] = r[:-1] + 0.5 * (r[1:] - r[:-1])
+            rho_sg[ax] = _abi.fortran_flat(np.moveaxis(sg, 0, ax))
+
+    # -- fill the kw_problem ---------------------------------------------------
+    pr.dim = dim
+    pr.Nx, pr.Ny, pr.Nz = kgrid.Nx, kgrid.Ny, (kgrid.Nz if dim == 3 else 1)
+    pr.Nt = Nt
+    pr.dx, pr.dy, pr.dz = kgrid.dx, kgrid.dy, (kgrid.dz if dim == 3 else 0.0)
+    pr.dt = dt
+    pr.c_ref = c_ref
+
+    def grid_or_scalar(v):
+        if v.size == 1:
+            return None, float(v)
+        return pp.hold(_abi.fortran_flat(v)), 0.0
+
+    a, s_ = grid_or_scalar(c)
+    pr.c0, pr.c0_scalar = _abi.dptr(a), s_
+    a, s_ = grid_or_scalar(rho)
+    pr.rho0, pr.rho0_scalar = _abi.dptr(a), s_
+    for ax in range(3):
+        pr.rho0_sg[ax] = _abi.dptr(pp.hold(rho_sg[ax]))
+    if nonlinear:
+        pr.nonlinear_flag = 1
+        a, s_ = grid_or_scalar(np.asarray(_get(medium, "BonA"), dtype=np.float64))
+        pr.BonA, pr.BonA_scalar = _abi.dptr(a), s_
+    if absorbing:
+        if alpha_mode == "stokes":
+            pr.absorbing_flag = 2
+            pr.alpha_power = 2.0
+        else:
+            pr.absorbing_flag = 1
+            if not _has(medium, "alpha_coeff") or not _has(medium, "alpha_power"):
+                raise ValueError("medium.alpha_coeff and medium.alpha_power must be defined")
+            y = np.asarray(_get(medium, "alpha_power"), dtype=np.float64)
+            if y.size != 1:
+                raise ValueError("medium.alpha_power must be scalar.")
+            pr.alpha_power = float(y)
+            pr.alpha_mode = {"no_absorption": 1, "no_dispersion": 2}.get(alpha_mode, 0)
+        a, s_ = grid_or_scalar(np.asarray(_get(medium, "alpha_coeff"), dtype=np.float64))
+        pr.alpha_coeff, pr.alpha_coeff_scalar = _abi.dptr(a), s_
+        if _has(medium, "alpha_sign"):
+            sg = np.asarray(_get(medium, "alpha_sign"), dtype=np.float64).ravel()
+            if sg.size != 2:
+                raise ValueError("medium.alpha_sign must be given as a 2 element numerical array")
+            pr.alpha_sign[0], pr.alpha_sign[1] = float(np.sign(sg[0]) or 1), float(np.sign(sg[1]) or 1)
+        else:
+            pr.alpha_sign[0], pr.alpha_sign[1] = 1.0, 1.0
+    for ax in range(dim):
+        pr.pml_size[ax] = int(pml_size[ax])
+        pr.pml_alpha[ax] = float(pml_alpha[ax])
+    pr.use_kspace = 1 if bool(opts.get("UsekSpace", True)) else 0
+    pr.use_sg = 1 if use_sg else 0
+
+    if p0 is not None:
+        pr.p0 = _abi.dptr(pp.hold(_abi.fortran_flat(p0)))
+
+    def fill_source(ts: _abi.TimeSource, src, nm):
+        sig = src.sigs[nm]
+        ts.flag = sig.shape[1]
+        ts.mode = _abi.SRC_MODE[src.mode]
+        idx = pp.hold(np.ascontiguousarray(src.pos.astype(np.uint64) + 1))
+        ts.num_points = len(idx)
+        ts.index = _abi.uptr(idx)
+        ts.num_series = sig.shape[0]
+        if src.sig_index is not None:
+            ts.sig_index = _abi.uptr(pp.hold(np.ascontiguousarray(src.sig_index)))
+        ts.input = _abi.dptr(pp.hold(np.ascontiguousarray(sig.ravel(order="F"))))
+
+    if src_p is not None:
+        fill_source(pr.p_source, src_p, "p")
+    if src_u is not None:
+        for ax, nm in enumerate(u_names):
+            if nm in src_u.sigs:
+                fill_source(pr.u_source[ax], src_u, nm)
+
+    if info.use_sensor:
+        full = len(sens_idx) == ncell and np.array_equal(sens_idx, np.arange(ncell))
+        pr.sensor_num_points = len(sens_idx)
+        if not full:
+            pr.sensor_mask_index = _abi.uptr(pp.hold(np.ascontiguousarray(sens_idx.astype(np.uint64) + 1)))
+        pr.record_flags = rec_flags
+        pr.record_start_index = record_start
+        if info.time_rev:
+            pr.time_reversal_boundary_data = _abi.dptr(pp.hold(np.ascontiguousarray(tr_data.ravel(order="F"))))
+    info.n_sensor = len(sens_idx) if info.use_sensor else 0
+    info.n_rec = (Nt - record_start + 1) if (info.use_sensor and not info.time_rev) else 0
+    info.data_recast = bool(opts.get("DataRecast", False))
+    info.backend = backend
+    return pp
+
+
+# ---------------------------------------------------------------------------
+# outputs
+# ---------------------------------------------------------------------------
+
+def alloc_outputs(pp: PreparedProblem):
+    """Allocate the output arrays of a prepared problem and the kw_outputs
+    struct that points to them."""
+    info = pp.info
+    dim, ncell, ns, nr = info.dim, int(np.prod(info.shape)), info.n_sensor, info.n_rec
+    out = _abi.Outputs()
+    arrays = {}
+    rec = set(info.record)
+    if info.time_rev:
+        rec = {"p_final"}
+
+    def f32(n):
+        return np.zeros(n, dtype=np.float32)
+
+    def f64(n):
+        return np.zeros(n, dtype=np.float64)
+
+    if "p" in rec:
+        arrays["p"] = f32(ns * nr)
+        out.p = _abi.fptr(arrays["p"])
+    for nm in ("p_max", "p_min", "p_rms"):
+        if nm in rec:
+            arrays[nm] = f64(ns)
+            setattr(out, nm, _abi.dptr(arrays[nm]))
+    for nm in ("p_max_all", "p_min_all"):
+        if nm in rec:
+            arrays[nm] = f64(ncell)
+            setattr(out, nm, _abi.dptr(arrays[nm]))
+    if "p_final" in rec:
+        arrays["p_final"] = f32(ncell)
+        out.p_final = _abi.fptr(arrays["p_final"])
+    comps = ["ux", "uy", "uz"][:dim]
+    for a, u in enumerate(comps):
+        if "u" in rec:
+            arrays[u] = f32(ns * nr)
+            out.u[a] = _abi.fptr(arrays[u])
+        for stat in ("max", "min", "rms"):
+            if f"u_{stat}" in rec:
+                arrays[f"{u}_{stat}"] = f64(ns)
+                getattr(out, f"u_{stat}")[a] = _abi.dptr(arrays[f"{u}_{stat}"])
+        for stat in ("max_all", "min_all"):
+            if f"u_{stat}" in rec:
+                arrays[f"{u}_{stat}"] = f64(ncell)
+                getattr(out, f"u_{stat}")[a] = _abi.dptr(arrays[f"{u}_{stat}"])
+        if "u_final" in rec:
+            arrays[f"{u}_final"] = f32(ncell)
+            out.u_final[a] = _abi.fptr(arrays[f"{u}_final"])
+    return out, arrays
+
+
+def shape_outputs(pp: PreparedProblem, arrays: dict):
+    """Reshape to the reference's output layout, reorder Cartesian sensor
+    data (kspaceFirstOrder_reorderCartData.m) and select the return value
+    (kspaceFirstOrder3D.m:1360-1383)."""
+    info = pp.info
+    ns, nr, shape = info.n_sensor, info.n_rec, info.shape
+    sd = {}
+    for k, v in arrays.items():
+        if k in ("p", "ux", "uy", "uz"):
+            sd[k] = v.reshape((ns, nr), order="F")
+        elif k.endswith("_all") or k.endswith("_final"):
+            sd[k] = v.reshape(shape, order="F")
+        else:
+            sd[k] = v
+    if info.reorder_index is not None and not info.time_rev:
+        order = np.argsort(info.reorder_index, kind="stable")
+        for k in list(sd):
+            if not (k.endswith("_all") or k.endswith("_final")):
+                sd[k] = sd[k][order]
+    if info.data_recast:
+        sd = {k: np.asarray(v, dtype=np.float64) for k, v in sd.items()}
+    if not info.use_sensor:
+        return None
+    if info.time_rev:
+        return sd["p_final"]
+    if not info.record_given:
+        return sd["p"]
+    return sd
+
+
+def run_prepared(pp: PreparedProblem):
+    lib = _abi.load(pp.info.backend)
+    out, arrays = alloc_outputs(pp)
+    _abi.call(lib.kw_run, C.byref(pp.prob), C.byref(out))
+    return shape_outputs(pp, arrays)
+
+
+def _solve(dim, kgrid, medium, source, sensor, args, kwargs):
+    opts = _parse_options(args, kwargs)
+    pp = prepare(kgrid, medium, source, sensor, dim, opts)
+    return run_prepared(pp)
+
+
+def kspaceFirstOrder2D(kgrid, medium, source, sensor, *args, **kwargs):  # noqa: N802
+    """2D time-domain simulation (reference +kwave/+legacy/kspaceFirstOrder2D.m)."""
+    return _solve(2, kgrid, medium, source, sensor, args, kwargs)
+
+
+def kspaceFirstOrder3D(kgrid, medium, source, sensor, *args, **kwargs):  # noqa: N802
+    """3D time-domain simulation (reference +kwave/+legacy/kspaceFirstOrder3D.m)."""
+    return _solve(3, kgrid, medium, source, sensor, args, kwargs)

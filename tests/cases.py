@@ -1,0 +1,225 @@
+"""Problem builders shared by the emulator and GPU parity tests.
+
+Each builder returns (kgrid_args, Nt, dt, medium, source, sensor, options):
+plain dicts with the reference's field names, so that the same inputs go to
+the oracle (oracle.kwave_oracle.kspace_first_order) and to the product
+(kwhip.kspaceFirstOrder2D/3D).
+"""
+
+from __future__ import annotations
+
+import math
+import zlib
+
+import numpy as np
+
+
+def disc(shape, centre, radius):
+    """Binary disc/ball (makeDisc/makeBall convention: distance <= radius,
+    1-based centre)."""
+    grids = np.meshgrid(*[np.arange(1, n + 1) for n in shape], indexing="ij")
+    r2 = sum((g - c) ** 2 for g, c in zip(grids, centre))
+    return (r2 <= radius ** 2).astype(np.float64)
+
+
+def smooth_noise(shape, seed, sigma):
+    """Uniform noise smoothed with a periodic Gaussian (FFT), scaled to
+    max |g| = 1."""
+    rng = np.random.default_rng(seed)
+    g = rng.uniform(-1, 1, size=shape)
+    ks = np.meshgrid(*[np.fft.fftfreq(n) for n in shape], indexing="ij")
+    k2 = sum(k * k for k in ks)
+    g = np.real(np.fft.ifftn(np.fft.fftn(g) * np.exp(-2 * (math.pi ** 2) * (sigma ** 2) * k2)))
+    return g / np.max(np.abs(g))
+
+
+def hetero_medium(shape, seed=0, sigma=2.0):
+    c = np.clip(1500 + 100 * smooth_noise(shape, seed, sigma), 1400, 1600)
+    rho = np.clip(1000 + 100 * smooth_noise(shape, seed + 1, sigma), 900, 1100)
+    return c, rho
+
+
+def tone(Nt, dt, f0, cycles, amp=1.0):
+    t = np.arange(Nt) * dt
+    n = int(math.floor(cycles / f0 / dt)) + 1
+    s = np.zeros(Nt)
+    m = min(n, Nt)
+    w = np.hanning(n)[:m]
+    s[:m] = amp * np.sin(2 * math.pi * f0 * t[:m]) * w
+    return s[:m]
+
+
+def case(name, dim=2, N=None):
+    """Named parity cases.  Returns a dict with keys grid, Nt, dt, medium,
+    source, sensor, opts."""
+    rng = np.random.default_rng(zlib.crc32(name.encode()))
+    if N is None:
+        N = (32, 30) if dim == 2 else (16, 18, 20)
+    shape = tuple(N)
+    dx = 1e-4
+    c0 = 1500.0
+    dt = 0.3 * dx / 1600.0
+    Nt = 24
+    med = {"sound_speed": c0, "density": 1000.0}
+    src = {}
+    pml = 4 if dim == 2 else 3
+    opts = {"PMLSize": pml, "PMLAlpha": 2.0, "Smooth": False}
+    centre = [n // 2 + 1 for n in shape]
+    sens = {"mask": np.ones(shape), "record": ["p", "p_final", "p_max", "p_min", "p_rms"]}
+    if name == "p0_lossless":
+        src["p0"] = 2.0 * disc(shape, centre, 3)
+        opts["Smooth"] = True
+    elif name == "p0_hetero_absorbing":
+        c, rho = hetero_medium(shape, 3)
+        med = {"sound_speed": c, "density": rho, "alpha_coeff": 0.75, "alpha_power": 1.5}
+        src["p0"] = disc(shape, centre, 3)
+        opts["Smooth"] = True
+    elif name == "p0_nonlinear_absorbing":
+        c, rho = hetero_medium(shape, 5)
+        med = {"sound_speed": c, "density": rho, "alpha_coeff": 0.5, "alpha_power": 1.3,
+               "BonA": 6.0}
+        src["p0"] = 1e6 * disc(shape, centre, 3)
+    elif name == "p0_nonlinear_lossless":
+        c, rho = hetero_medium(shape, 7)
+        med = {"sound_speed": c, "density": rho, "BonA": 5.0 + rng.uniform(0, 2, size=shape)}
+        src["p0"] = 2e6 * disc(shape, centre, 3)
+    elif name == "stokes":
+        med = {"sound_speed": c0, "density": 1000.0, "alpha_coeff": 0.5, "alpha_mode": "stokes"}
+        src["p0"] = disc(shape, centre, 2)
+    elif name == "no_dispersion":
+        c, rho = hetero_medium(shape, 9)
+        med = {"sound_speed": c, "density": rho, "alpha_coeff": 2.0, "alpha_power": 1.0,
+               "alpha_mode": "no_dispersion"}
+        src["p0"] = disc(shape, centre, 2)
+    elif name == "no_absorption":
+        med = {"sound_speed": c0, "density": 1000.0, "alpha_coeff": 2.0, "alpha_power": 1.7,
+               "alpha_mode": "no_absorption"}
+        src["p0"] = disc(shape, centre, 2)
+    elif name.startswith("psrc_"):
+        mode = {"psrc_add": "additive", "psrc_dir": "dirichlet", "psrc_nocorr": "additive-no-correction",
+                "psrc_many": "additive", "psrc_label": "additive", "psrc_many_dir": "dirichlet"}[name]
+        c, rho = hetero_medium(shape, 11)
+        med = {"sound_speed": c, "density": rho}
+        m = np.zeros(shape)
+        sl = tuple([slice(pml + 2, pml + 3)] + [slice(pml + 2, n - pml - 2) for n in shape[1:]])
+        m[sl] = 1
+        npts = int(m.sum())
+        sig = tone(Nt, dt, 2e6, 3)
+        if name in ("psrc_many", "psrc_many_dir"):
+            sigs = np.stack([sig * (1 + 0.1 * i) for i in range(npts)])
+            src["p"] = sigs
+        elif name == "psrc_label":
+            lab = m.copy()
+            flat = np.flatnonzero(lab.ravel(order="F"))
+            labels = (np.arange(len(flat)) % 3) + 1
+            lf = lab.ravel(order="F")
+            lf[flat] = labels
+            m = lf.reshape(shape, order="F")
+            src["p"] = np.stack([sig, 0.5 * sig, -sig])
+        else:
+            src["p"] = sig
+        src["p_mask"] = m
+        src["p_mode"] = mode
+        sens = {"mask": _sparse_mask(shape, rng), "record": ["p", "p_max", "p_min", "p_rms", "u",
+                                                               "u_max", "u_min", "u_rms"]}
+    elif name.startswith("usrc_"):
+        mode = {"usrc_add": "additive", "usrc_dir": "dirichlet",
+                "usrc_nocorr": "additive-no-correction", "usrc_many": "additive"}[name]
+        m = np.zeros(shape)
+        sl = tuple([slice(pml + 3, pml + 4)] + [slice(pml + 2, n - pml - 2) for n in shape[1:]])
+        m[sl] = 1
+        npts = int(m.sum())
+        sig = tone(Nt, dt, 2e6, 2, amp=1e-3)
+        if name == "usrc_many":
+            src["ux"] = np.stack([sig * (1 + 0.05 * i) for i in range(npts)])
+            src["uy"] = np.stack([0.3 * sig * (1 - 0.05 * i) for i in range(npts)])
+        else:
+            src["ux"] = sig
+            src["uy"] = 0.5 * sig
+        if dim == 3:
+            src["uz"] = -0.25 * (src["ux"])
+        src["u_mask"] = m
+        src["u_mode"] = mode
+        sens = {"mask": _sparse_mask(shape, rng),
+                "record": ["p", "u", "u_final", "u_max_all", "u_min_all", "p_max_all", "p_min_all"],
+                "record_start_index": 5}
+    elif name == "time_reversal":
+        m = np.zeros(shape)
+        m[pml + 1, ...] = 1
+        m[..., pml + 1] = 1
+        npts = int(m.sum())
+        data = np.stack([tone(Nt, dt, 1.5e6, 2, amp=1 + 0.2 * np.sin(i)) for i in range(npts)])
+        if data.shape[1] < Nt:
+            data = np.pad(data, ((0, 0), (0, Nt - data.shape[1])))
+        sens = {"mask": m, "time_reversal_boundary_data": data + 0.01 * rng.standard_normal((npts, Nt))}
+    elif name == "cartesian_nearest":
+        src["p0"] = disc(shape, centre, 3)
+        npt = 12
+        ang = np.arange(npt) * 2 * math.pi / npt + math.pi / 2
+        r = (min(shape) // 2 - pml - 1) * dx
+        pts = [r * np.cos(ang), -r * np.sin(ang)]
+        if dim == 3:
+            pts.append(np.zeros(npt))
+        sens = {"mask": np.stack(pts), "record": ["p", "p_max", "p_rms"]}
+        opts["CartInterp"] = "nearest"
+    else:
+        raise KeyError(name)
+    return {"shape": shape, "dx": dx, "Nt": Nt, "dt": dt, "medium": med, "source": src,
+            "sensor": sens, "opts": opts, "dim": dim}
+
+
+def _sparse_mask(shape, rng, frac=0.1):
+    m = (rng.uniform(size=shape) < frac).astype(np.float64)
+    m.ravel()[0] = 1
+    return m
+
+
+CASES_2D = ["p0_lossless", "p0_hetero_absorbing", "p0_nonlinear_absorbing", "p0_nonlinear_lossless",
+            "stokes", "no_dispersion", "no_absorption", "psrc_add", "psrc_dir", "psrc_nocorr",
+            "psrc_many", "psrc_label", "psrc_many_dir", "usrc_add", "usrc_dir", "usrc_nocorr",
+            "usrc_many", "time_reversal", "cartesian_nearest"]
+CASES_3D = CASES_2D
+
+
+def oracle_opts(opts):
+    """Map reference option names to the oracle's keyword names."""
+    m = {"PMLSize": "pml_size", "PMLAlpha": "pml_alpha", "Smooth": "smooth",
+         "CartInterp": "cart_interp", "PMLInside": "pml_inside", "UsekSpace": "use_kspace",
+         "UseSG": "use_sg"}
+    return {m[k]: v for k, v in opts.items() if k in m}
+
+
+def run_oracle(cs, data_cast="single"):
+    from oracle import kwave_oracle as ko
+    shape, dx = cs["shape"], cs["dx"]
+    args = []
+    for n in shape:
+        args += [n, dx]
+    kg = ko.KGrid(*args)
+    kg.set_time(cs["Nt"], cs["dt"])
+    return ko.kspace_first_order(kg, cs["medium"], cs["source"], cs["sensor"],
+                                 data_cast=data_cast, **oracle_opts(cs["opts"]))
+
+
+def run_kwhip(cs, backend):
+    import kwhip
+    shape, dx = cs["shape"], cs["dx"]
+    args = []
+    for n in shape:
+        args += [n, dx]
+    kg = kwhip.kWaveGrid(*args)
+    kg.setTime(cs["Nt"], cs["dt"])
+    fn = kwhip.kspaceFirstOrder2D if cs["dim"] == 2 else kwhip.kspaceFirstOrder3D
+    return fn(kg, cs["medium"], cs["source"], cs["sensor"], Backend=backend, **cs["opts"])
+
+
+def compare(got, ref):
+    """Relative L-inf per output field; returns {field: err}."""
+    from oracle import kwave_oracle as ko
+    if not isinstance(ref, dict):
+        got, ref = {"p": got}, {"p": ref}
+    errs = {}
+    for k, r in ref.items():
+        g = got[k]
+        errs[k] = ko.rel_linf(np.asarray(g, dtype=np.float64), np.asarray(r, dtype=np.float64))
+    return errs
