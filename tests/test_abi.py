@@ -1,0 +1,97 @@
+"""The C-ABI libraries load and export every function include/kwhip.h
+declares; struct layouts of the ctypes view match the header (CPU only, no
+device calls)."""
+
+import ctypes as C
+import re
+from pathlib import Path
+
+import pytest
+
+ROOT = Path(__file__).resolve().parent.parent
+HEADER = ROOT / "include" / "kwhip.h"
+
+
+def declared_functions():
+    text = HEADER.read_text()
+    text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:int|int64_t|const char\*)\s+(kw_\w+)\s*\(", text, flags=re.M)))
+
+
+def test_header_declares_api():
+    names = declared_functions()
+    for required in ("kw_run", "kw_output_sizes", "kw_sim_create", "kw_sim_step", "kw_sim_read",
+                     "kw_sim_destroy", "kw_smooth", "kw_backend", "kw_abi_version"):
+        assert required in names
+
+
+@pytest.mark.parametrize("backend", ["hip", "emu"])
+def test_library_exports_every_symbol(backend):
+    from kwhip import _abi
+
+    path = _abi.library_path(backend)
+    assert path.exists(), f"{path} not built"
+    lib = C.CDLL(str(path))
+    missing = [n for n in declared_functions() if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_emulator_reports_its_backend():
+    from kwhip import _abi
+
+    lib = _abi.load("emu")
+    assert lib.kw_abi_version() == 1
+    assert lib.kw_backend().decode() == "emulator"
+
+
+def test_struct_sizes_match_header():
+    """Compile a probe of the header's struct sizes with the host compiler
+    and compare with ctypes."""
+    import subprocess
+    import tempfile
+
+    from kwhip import _abi
+
+    src = r"""
+#include <stdio.h>
+#include <stddef.h>
+#include "kwhip.h"
+int main(void) {
+  printf("%zu %zu %zu %zu %zu\n", sizeof(kw_problem), sizeof(kw_time_source), sizeof(kw_outputs),
+         sizeof(kw_sizes), sizeof(kw_kernel_stats));
+  printf("%zu %zu %zu\n", offsetof(kw_problem, p_source), offsetof(kw_problem, sensor_num_points),
+         offsetof(kw_problem, time_reversal_boundary_data));
+  return 0;
+}
+"""
+    with tempfile.TemporaryDirectory() as d:
+        c = Path(d) / "probe.c"
+        c.write_text(src)
+        exe = Path(d) / "probe"
+        subprocess.run(["gcc", f"-I{ROOT / 'include'}", str(c), "-o", str(exe)], check=True)
+        out = subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()
+    got = [int(x) for x in out]
+    want = [C.sizeof(_abi.Problem), C.sizeof(_abi.TimeSource), C.sizeof(_abi.Outputs),
+            C.sizeof(_abi.Sizes), C.sizeof(_abi.KernelStats),
+            _abi.Problem.p_source.offset, _abi.Problem.sensor_num_points.offset,
+            _abi.Problem.time_reversal_boundary_data.offset]
+    assert got == want
+
+
+def test_errors_are_reported_not_raised():
+    """Invalid problems come back as negative codes with a message."""
+    from kwhip import _abi
+
+    lib = _abi.load("emu")
+    prob = _abi.Problem()
+    prob.dim = 4
+    h = C.c_void_p()
+    buf = C.create_string_buffer(256)
+    rc = lib.kw_sim_create(C.byref(prob), C.byref(h), buf, 256)
+    assert rc == _abi.KW_EINVAL
+    assert b"dim" in buf.value
+    prob.dim, prob.Nx, prob.Ny, prob.Nz, prob.Nt = 2, 14, 16, 1, 3   # 14 = 2 x 7: radix 7 unsupported
+    prob.dx = prob.dy = 1e-4
+    prob.dt, prob.c_ref, prob.c0_scalar, prob.rho0_scalar = 1e-8, 1500.0, 1500.0, 1000.0
+    rc = lib.kw_sim_create(C.byref(prob), C.byref(h), buf, 256)
+    assert rc == _abi.KW_EUNSUP
