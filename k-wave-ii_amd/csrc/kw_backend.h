@@ -26,8 +26,10 @@ void dev_sync(void* stream);
 void dev_zero(void* dst, size_t bytes, void* stream);  // set-up only (not graph-safe)
 
 // launches (asynchronous on `stream`)
-void launch_pass(PassKind kind, int nblocks, int threads, size_t lds_bytes, const Batch& B,
-                 const PassArgs& A, void* stream);
+// `fixed` is the compile-time FFT length of the variant to run (0: the
+// runtime-length variant); `dim` selects the 2D/3D body of the x passes
+void launch_pass(PassKind kind, int fixed, int dim, int nblocks, int threads, size_t lds_bytes,
+                 const Batch& B, const PassArgs& A, void* stream);
 void launch_ew(int nblocks, const Batch& B, const EwArgs& A, void* stream);
 
 // event timing (GPU only; the emulator returns wall-clock times)

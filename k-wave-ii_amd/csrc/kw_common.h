@@ -52,9 +52,76 @@ inline T* emu_ptr(T* p, const char* what) {
 }
 #define KW_LD(ptr) (*::kw::emu_ptr((ptr), "load"))
 #define KW_ST(ptr, val) (*::kw::emu_ptr((ptr), "store") = (val))
+#elif defined(KW_GPU)
+// HBM accesses go through the global address space explicitly: the
+// pointers come from descriptors in memory, so without the cast the
+// compiler emits generic (flat) accesses, which wait on both the vector and
+// the LDS counters and serialise loads
+#define KW_GLOBAL __attribute__((address_space(1)))
+template <class T>
+__device__ __forceinline__ T kw_ld(const T* p) {
+  return *((const KW_GLOBAL T*)p);
+}
+template <class T, class V>
+__device__ __forceinline__ void kw_st(T* p, V v) {
+  *((KW_GLOBAL T*)p) = (T)v;
+}
+// complex values move as a native 2-float vector: copying the struct through
+// an address-space pointer would bind its copy constructor to a generic
+// reference and bring the flat access back
+typedef float kw_v2f __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ cf kw_ld(const cf* p) {
+  const kw_v2f v = *((const KW_GLOBAL kw_v2f*)p);
+  return cf{v.x, v.y};
+}
+__device__ __forceinline__ void kw_st(cf* p, cf v) {
+  const kw_v2f w = {v.x, v.y};
+  *((KW_GLOBAL kw_v2f*)p) = w;
+}
+#define KW_LD(ptr) (::kw::kw_ld(ptr))
+#define KW_ST(ptr, val) (::kw::kw_st((ptr), (val)))
+#define KW_LDS_PTR __attribute__((address_space(3)))
+__device__ __forceinline__ cf lds_ld(const cf* p) {
+  const kw_v2f v = *((const KW_LDS_PTR kw_v2f*)p);
+  return cf{v.x, v.y};
+}
+__device__ __forceinline__ void lds_st(cf* p, cf v) {
+  const kw_v2f w = {v.x, v.y};
+  *((KW_LDS_PTR kw_v2f*)p) = w;
+}
 #else
 #define KW_LD(ptr) (*(ptr))
 #define KW_ST(ptr, val) (*(ptr) = (val))
+#endif
+
+// Loads and stores at a 32-bit element index from a base pointer: the byte
+// offset is formed in 32 bits so that the compiler can address with a
+// uniform 64-bit base plus a 32-bit per-lane offset (one VGPR per address
+// instead of a 64-bit pointer).  Every array is below 4 GiB (checked on the
+// host).
+template <class T>
+KW_HOSTDEV const T* at32(const T* base, uint32_t i) {
+  return (const T*)((const char*)base + (size_t)(uint32_t)(i * (uint32_t)sizeof(T)));
+}
+template <class T>
+KW_HOSTDEV T* at32(T* base, uint32_t i) {
+  return (T*)((char*)base + (size_t)(uint32_t)(i * (uint32_t)sizeof(T)));
+}
+#define KW_LD32(base, i) KW_LD(::kw::at32((base), (uint32_t)(i)))
+#define KW_ST32(base, i, v) KW_ST(::kw::at32((base), (uint32_t)(i)), (v))
+
+// Scheduling fence between epilogue phases: each phase batches its own
+// loads, but loads are not hoisted across phases (which would hold every
+// phase's operands in registers at once).
+#ifdef KW_GPU
+#define KW_PHASE_FENCE() __builtin_amdgcn_sched_barrier(0)
+#else
+#define KW_PHASE_FENCE() ((void)0)
+#endif
+
+#ifndef KW_GPU
+inline cf lds_ld(const cf* p) { return *p; }
+inline void lds_st(cf* p, cf v) { *p = v; }
 #endif
 
 // Elements held per thread per FFT stage (upper bound).  A pencil of length

@@ -26,6 +26,7 @@
 #include <vector>
 
 #include "kw_backend.h"
+#include "kw_variants.h"
 
 namespace kw {
 
@@ -213,20 +214,43 @@ void emu_launch(int nblocks, int T, size_t lds_bytes,
 
 }  // namespace
 
-void launch_pass(PassKind kind, int nblocks, int threads, size_t lds_bytes, const Batch& B,
-                 const PassArgs& A, void*) {
+template <class Sel>
+static void emu_pass(PassKind kind, int dim, int nblocks, int threads, size_t lds_bytes,
+                     const Batch& B, const PassArgs& A) {
   std::function<void(EmuCtx&)> f;
   switch (kind) {
-    case PK_YFWD: f = [&](EmuCtx& c) { body_ypass<false>(c, B, A); }; break;
-    case PK_YINV: f = [&](EmuCtx& c) { body_ypass<true>(c, B, A); }; break;
-    case PK_SPEC: f = [&](EmuCtx& c) { body_spec(c, B, A); }; break;
-    case PK_X_R2C: f = [&](EmuCtx& c) { body_x_r2c(c, B, A); }; break;
-    case PK_X_C2R: f = [&](EmuCtx& c) { body_x_c2r(c, B, A); }; break;
-    case PK_X_GRAD: f = [&](EmuCtx& c) { body_x_grad(c, B, A); }; break;
-    case PK_X_DIV: f = [&](EmuCtx& c) { body_x_div(c, B, A); }; break;
-    case PK_X_EOS: f = [&](EmuCtx& c) { body_x_eos(c, B, A); }; break;
+    case PK_YFWD: f = [&](EmuCtx& c) { body_ypass<false, Sel>(c, B, A); }; break;
+    case PK_YINV: f = [&](EmuCtx& c) { body_ypass<true, Sel>(c, B, A); }; break;
+    case PK_SPEC: f = [&](EmuCtx& c) { body_spec<Sel>(c, B, A); }; break;
+    case PK_X_R2C: f = [&](EmuCtx& c) { body_x_r2c<Sel>(c, B, A); }; break;
+    case PK_X_C2R: f = [&](EmuCtx& c) { body_x_c2r<Sel>(c, B, A); }; break;
+    case PK_X_GRAD:
+      if (dim == 2)
+        f = [&](EmuCtx& c) { body_x_grad<Sel, 2>(c, B, A); };
+      else
+        f = [&](EmuCtx& c) { body_x_grad<Sel, 3>(c, B, A); };
+      break;
+    case PK_X_DIV:
+      if (dim == 2)
+        f = [&](EmuCtx& c) { body_x_div<Sel, 2>(c, B, A); };
+      else
+        f = [&](EmuCtx& c) { body_x_div<Sel, 3>(c, B, A); };
+      break;
+    case PK_X_EOS: f = [&](EmuCtx& c) { body_x_eos<Sel>(c, B, A); }; break;
   }
   emu_launch(nblocks, threads, lds_bytes, f);
+}
+
+void launch_pass(PassKind kind, int fixed, int dim, int nblocks, int threads, size_t lds_bytes,
+                 const Batch& B, const PassArgs& A, void*) {
+  switch (fixed) {
+    case 0: emu_pass<PlanRt>(kind, dim, nblocks, threads, lds_bytes, B, A); break;
+#define KW_EMU_SIZE(SIZE) \
+    case SIZE: emu_pass<PlanCt<SIZE>>(kind, dim, nblocks, threads, lds_bytes, B, A); break;
+    KW_FIXED_SIZES(KW_EMU_SIZE)
+#undef KW_EMU_SIZE
+    default: throw DeviceError(-3, "no kernel variant for this length");
+  }
 }
 
 void launch_ew(int nblocks, const Batch& B, const EwArgs& A, void*) {

@@ -7,15 +7,25 @@
 // DFT and writes element (b - k)*R + k + r*Ns.  After the last stage the
 // pencil is in natural order (self-sorting, no bit reversal).  Each stage
 // reads all of its inputs, synchronises the block, then writes its outputs,
-// so a single LDS buffer is transformed in place.
+// so a single LDS buffer per pencil is transformed in place.
 //
-// The first stage takes its inputs from a caller functor `src(n)` and the
-// last stage hands its outputs to `dst(n, value, slot)`, which is how loads
-// from HBM (with any fused pre-processing) and stores to HBM (with any fused
-// epilogue) are folded into the FFT.  `slot` is a compile-time-constant index
-// of the output within the calling thread, identical for every transform
-// with the same plan, so epilogues can keep per-output values in registers
-// across several transforms.
+// A thread works on NP pencils at once (for example the x, y and z
+// components of a field): their loads are issued together, the twiddles and
+// barriers are shared, so NP independent transforms cost one transform's
+// latency.
+//
+// The first stage takes its inputs from a caller functor `src(j, n)` (pencil
+// j, element n) and the last stage hands its outputs to
+// `dst(j, n, value, slot)`, which is how loads from HBM (with any fused
+// pre-processing) and stores to HBM (with any fused epilogue) are folded into
+// the FFT.  `slot` is a compile-time-constant index of the output within the
+// calling thread, identical for every transform with the same plan, so
+// epilogues can keep per-output values in registers across transforms.
+//
+// Two engines share the stage code: `fft_rt` takes a runtime plan (any
+// length that factors into 2, 3 and 5; one code path per radix) and
+// `fft_ct<N>` a length fixed at compile time, for which every index is a
+// constant and the stages unroll into straight-line code.
 //
 // Twiddles come from a per-length table of the N roots of unity
 // exp(-2 pi i j / N) computed in double precision on the host and rounded
@@ -33,11 +43,39 @@ struct FftPlan {
   const cf* tw;  // N entries
 };
 
+// factorisation rule shared by the host planner and the compile-time plans:
+// radix-8 stages first, then one radix-4 or radix-2 stage, then 3s and 5s
+constexpr int ct_nst(int N) {
+  int n = N, c = 0;
+  while (n % 8 == 0) { n /= 8; ++c; }
+  if (n % 4 == 0) { n /= 4; ++c; }
+  if (n % 2 == 0) { n /= 2; ++c; }
+  while (n % 3 == 0) { n /= 3; ++c; }
+  while (n % 5 == 0) { n /= 5; ++c; }
+  return n == 1 ? c : -1;
+}
+constexpr int ct_radix(int N, int s) {
+  int n = N, c = 0;
+  while (n % 8 == 0) { n /= 8; if (c++ == s) return 8; }
+  if (n % 4 == 0) { n /= 4; if (c++ == s) return 4; }
+  if (n % 2 == 0) { n /= 2; if (c++ == s) return 2; }
+  while (n % 3 == 0) { n /= 3; if (c++ == s) return 3; }
+  while (n % 5 == 0) { n /= 5; if (c++ == s) return 5; }
+  return 0;
+}
+// product of the radices before stage s
+constexpr int ct_ns(int N, int s) {
+  int p = 1;
+  for (int i = 0; i < s; ++i) p *= ct_radix(N, i);
+  return p;
+}
+
+// upper bound of butterflies per thread for the runtime engine
 template <int R>
 struct MaxB {
   static constexpr int v = (KW_E + R - 1) / R;
 };
-// largest number of last-stage outputs one thread can own (over all radices)
+// largest number of last-stage outputs one thread can own with G = ceil(N/KW_E)
 constexpr int kMaxSlots = 10;
 static_assert(MaxB<2>::v * 2 <= kMaxSlots && MaxB<3>::v * 3 <= kMaxSlots &&
                   MaxB<4>::v * 4 <= kMaxSlots && MaxB<5>::v * 5 <= kMaxSlots &&
@@ -153,98 +191,127 @@ KW_BODY void dft<8, false>(cf* v) { dft8<false>(v); }
 template <>
 KW_BODY void dft<8, true>(cf* v) { dft8<true>(v); }
 
-// One Stockham stage for one pencil (see the file comment).
-template <int R, bool INV, class Ctx, class Src, class Dst>
-KW_BODY void stage(Ctx& c, int N, int Ns, int g, int G, bool active, const cf* tw, Src& src,
+KW_BODY int mod_ns(int b, int Ns) {
+  // b mod Ns; Ns is a compile-time constant in the fixed-length engine and
+  // a power of two for every stage before the first radix-3/5 stage
+  return (Ns & (Ns - 1)) == 0 ? (b & (Ns - 1)) : b % Ns;
+}
+
+// One Stockham stage over NP pencils (see the file comment).  MB is the
+// number of butterflies per thread (exact for the fixed-length engine, an
+// upper bound for the runtime engine).  `src` must be safe to call for
+// every thread, including threads with active == false.
+template <int R, int NP, bool INV, int MB, class Ctx, class Src, class Dst>
+KW_BODY void stage(Ctx& c, int N, int Ns, int G, int g, bool active, const cf* tw, Src& src,
                    Dst& dst) {
-  constexpr int MB = MaxB<R>::v;
   const int nb = N / R;
   const int twstep = N / (Ns * R);
-  cf v[MB][R];
+  cf v[NP][MB][R];
+  // loads and arithmetic run for every thread (callers make src safe for
+  // threads without a pencil) so that they are not split into conditional
+  // regions; only the stores are guarded by `active`
 #pragma unroll
   for (int m = 0; m < MB; ++m) {
     const int b = g + m * G;
-    if (active && b < nb) {
-#pragma unroll
-      for (int r = 0; r < R; ++r) v[m][r] = src(b + r * nb);
+    if (MB * G == nb || b < nb) {
+      cf w[R];
       if (Ns > 1) {
-        const int k = b % Ns;
+        const int k = mod_ns(b, Ns);
 #pragma unroll
-        for (int r = 1; r < R; ++r) v[m][r] = cmul(v[m][r], twiddle<INV>(tw, k * r * twstep));
+        for (int r = 1; r < R; ++r) w[r] = twiddle<INV>(tw, k * r * twstep);
       }
-      dft<R, INV>(v[m]);
+#pragma unroll
+      for (int j = 0; j < NP; ++j) {
+#pragma unroll
+        for (int r = 0; r < R; ++r) v[j][m][r] = src(j, b + r * nb);
+      }
+#pragma unroll
+      for (int j = 0; j < NP; ++j) {
+        if (Ns > 1) {
+#pragma unroll
+          for (int r = 1; r < R; ++r) v[j][m][r] = cmul(v[j][m][r], w[r]);
+        }
+        dft<R, INV>(v[j][m]);
+      }
     }
   }
   c.sync();
 #pragma unroll
-  for (int m = 0; m < MB; ++m) {
-    const int b = g + m * G;
-    if (active && b < nb) {
-      const int k = b % Ns;
-      const int j0 = (b - k) * R + k;
+  for (int j = 0; j < NP; ++j) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) dst(j0 + r * Ns, v[m][r], m * R + r);
+    for (int m = 0; m < MB; ++m) {
+      const int b = g + m * G;
+      if (active && (MB * G == nb || b < nb)) {
+        const int k = mod_ns(b, Ns);
+        const int j0 = (b - k) * R + k;
+#pragma unroll
+        for (int r = 0; r < R; ++r) dst(j, j0 + r * Ns, v[j][m][r], m * R + r);
+      }
     }
   }
   c.sync();
 }
 
-template <bool INV, class Ctx, class Src, class Dst>
-KW_BODY void stage_dispatch(int R, Ctx& c, int N, int Ns, int g, int G, bool active,
-                            const cf* tw, Src& src, Dst& dst) {
-  switch (R) {
-    case 2: stage<2, INV>(c, N, Ns, g, G, active, tw, src, dst); break;
-    case 3: stage<3, INV>(c, N, Ns, g, G, active, tw, src, dst); break;
-    case 4: stage<4, INV>(c, N, Ns, g, G, active, tw, src, dst); break;
-    case 5: stage<5, INV>(c, N, Ns, g, G, active, tw, src, dst); break;
-    default: stage<8, INV>(c, N, Ns, g, G, active, tw, src, dst); break;
-  }
-}
-
-// LDS view of one pencil: element n at base[n * sn]
-struct LdsPencil {
+// LDS view of NP pencils: element n of pencil j at base[j*sj + n*sn + pad]
+// with pad = (n >> 3) when `padded` (breaks the power-of-two strides of the
+// Stockham writes across LDS banks)
+struct LdsPencils {
   cf* base;
-  int sn;
-  KW_BODY cf get(int n) const { return base[n * sn]; }
-  KW_BODY void set(int n, cf v) const { base[n * sn] = v; }
+  int sj, sn;
+  bool padded;
+  KW_BODY int at(int j, int n) const { return j * sj + n * sn + (padded ? (n >> 3) : 0); }
+  KW_BODY cf get(int j, int n) const { return lds_ld(base + at(j, n)); }
+  KW_BODY void set(int j, int n, cf v) const { lds_st(base + at(j, n), v); }
 };
 
-// Transform one pencil (called by all threads of the block; `active` is
-// false for threads without a pencil, which still take part in barriers).
-template <bool INV, class Ctx, class Src, class Dst>
-KW_BODY void fft(Ctx& c, const FftPlan& pl, int g, int G, bool active, Src& src0, Dst& dstN,
-                 const LdsPencil& lds) {
-  auto lsrc = [&](int n) -> cf { return lds.get(n); };
-  auto ldst = [&](int n, cf v, int) { lds.set(n, v); };
+// ---- runtime-length engine --------------------------------------------------
+
+template <int NP, bool INV, class Ctx, class Src, class Dst>
+KW_BODY void stage_rt(int R, Ctx& c, int N, int Ns, int G, int g, bool active, const cf* tw,
+                      Src& src, Dst& dst) {
+  switch (R) {
+    case 2: stage<2, NP, INV, MaxB<2>::v>(c, N, Ns, G, g, active, tw, src, dst); break;
+    case 3: stage<3, NP, INV, MaxB<3>::v>(c, N, Ns, G, g, active, tw, src, dst); break;
+    case 4: stage<4, NP, INV, MaxB<4>::v>(c, N, Ns, G, g, active, tw, src, dst); break;
+    case 5: stage<5, NP, INV, MaxB<5>::v>(c, N, Ns, G, g, active, tw, src, dst); break;
+    default: stage<8, NP, INV, MaxB<8>::v>(c, N, Ns, G, g, active, tw, src, dst); break;
+  }
+}
+
+template <int NP, bool INV, class Ctx, class Src, class Dst>
+KW_BODY void fft_rt(Ctx& c, const FftPlan& pl, int g, int G, bool active, Src& src0, Dst& dstN,
+                    const LdsPencils& lds) {
+  auto lsrc = [&](int j, int n) -> cf { return lds.get(j, n); };
+  auto ldst = [&](int j, int n, cf v, int) { lds.set(j, n, v); };
   int Ns = 1;
   const int nst = pl.nst;
   for (int s = 0; s < nst; ++s) {
     const int R = pl.radix[s];
     const bool first = (s == 0), last = (s == nst - 1);
     if (first && last)
-      stage_dispatch<INV>(R, c, pl.N, Ns, g, G, active, pl.tw, src0, dstN);
+      stage_rt<NP, INV>(R, c, pl.N, Ns, G, g, active, pl.tw, src0, dstN);
     else if (first)
-      stage_dispatch<INV>(R, c, pl.N, Ns, g, G, active, pl.tw, src0, ldst);
+      stage_rt<NP, INV>(R, c, pl.N, Ns, G, g, active, pl.tw, src0, ldst);
     else if (last)
-      stage_dispatch<INV>(R, c, pl.N, Ns, g, G, active, pl.tw, lsrc, dstN);
+      stage_rt<NP, INV>(R, c, pl.N, Ns, G, g, active, pl.tw, lsrc, dstN);
     else
-      stage_dispatch<INV>(R, c, pl.N, Ns, g, G, active, pl.tw, lsrc, ldst);
+      stage_rt<NP, INV>(R, c, pl.N, Ns, G, g, active, pl.tw, lsrc, ldst);
     Ns *= R;
   }
 }
 
-// Visit the (slot, n) pairs that this thread owns in the last stage of `pl`
-// (the same pairs `dst` receives), in slot order.
+// Visit the (slot, n) pairs this thread owns in the last stage of a runtime
+// plan (the same pairs `dst` receives), in slot order.  Every thread visits
+// its pairs whether or not it has a pencil: the visitor guards its stores.
 template <int R, class F>
-KW_BODY void last_slots_r(const FftPlan& pl, int g, int G, bool active, F& f) {
+KW_BODY void last_slots_r(int N, int G, int g, bool, F& f) {
   constexpr int MB = MaxB<R>::v;
-  const int N = pl.N;
   const int nb = N / R;
-  const int Ns = N / R;  // product of the earlier radices
+  const int Ns = N / R;
 #pragma unroll
   for (int m = 0; m < MB; ++m) {
     const int b = g + m * G;
-    if (active && b < nb) {
+    if (b < nb) {
       const int k = b % Ns;
       const int j0 = (b - k) * R + k;
 #pragma unroll
@@ -254,14 +321,99 @@ KW_BODY void last_slots_r(const FftPlan& pl, int g, int G, bool active, F& f) {
 }
 
 template <class F>
-KW_BODY void last_slots(const FftPlan& pl, int g, int G, bool active, F& f) {
+KW_BODY void last_slots_rt(const FftPlan& pl, int g, int G, bool active, F& f) {
   switch (pl.radix[pl.nst - 1]) {
-    case 2: last_slots_r<2>(pl, g, G, active, f); break;
-    case 3: last_slots_r<3>(pl, g, G, active, f); break;
-    case 4: last_slots_r<4>(pl, g, G, active, f); break;
-    case 5: last_slots_r<5>(pl, g, G, active, f); break;
-    default: last_slots_r<8>(pl, g, G, active, f); break;
+    case 2: last_slots_r<2>(pl.N, G, g, active, f); break;
+    case 3: last_slots_r<3>(pl.N, G, g, active, f); break;
+    case 4: last_slots_r<4>(pl.N, G, g, active, f); break;
+    case 5: last_slots_r<5>(pl.N, G, g, active, f); break;
+    default: last_slots_r<8>(pl.N, G, g, active, f); break;
   }
 }
+
+// ---- fixed-length engine ------------------------------------------------------
+
+template <int N, int G, int S, int NST, int NP, bool INV>
+struct CtStages {
+  static constexpr int R = ct_radix(N, S);
+  static constexpr int Ns = ct_ns(N, S);
+  static constexpr int MB = (N / R + G - 1) / G;
+  template <class Ctx, class Src, class Dst, class LSrc, class LDst>
+  KW_BODY static void run(Ctx& c, int g, bool active, const cf* tw, Src& src0, Dst& dstN,
+                          LSrc& lsrc, LDst& ldst) {
+    if constexpr (S == 0 && NST == 1) {
+      stage<R, NP, INV, MB>(c, N, Ns, G, g, active, tw, src0, dstN);
+    } else if constexpr (S == 0) {
+      stage<R, NP, INV, MB>(c, N, Ns, G, g, active, tw, src0, ldst);
+    } else if constexpr (S == NST - 1) {
+      stage<R, NP, INV, MB>(c, N, Ns, G, g, active, tw, lsrc, dstN);
+    } else {
+      stage<R, NP, INV, MB>(c, N, Ns, G, g, active, tw, lsrc, ldst);
+    }
+    if constexpr (S + 1 < NST)
+      CtStages<N, G, S + 1, NST, NP, INV>::run(c, g, active, tw, src0, dstN, lsrc, ldst);
+  }
+};
+
+template <int N, int G, int NP, bool INV, class Ctx, class Src, class Dst>
+KW_BODY void fft_ct(Ctx& c, const cf* tw, int g, bool active, Src& src0, Dst& dstN,
+                    const LdsPencils& lds) {
+  static_assert(ct_nst(N) > 0, "length must factor into 2, 3 and 5");
+  auto lsrc = [&](int j, int n) -> cf { return lds.get(j, n); };
+  auto ldst = [&](int j, int n, cf v, int) { lds.set(j, n, v); };
+  CtStages<N, G, 0, ct_nst(N), NP, INV>::run(c, g, active, tw, src0, dstN, lsrc, ldst);
+}
+
+template <int N, int G, class F>
+KW_BODY void last_slots_ct(int g, bool, F& f) {
+  constexpr int S = ct_nst(N) - 1;
+  constexpr int R = ct_radix(N, S);
+  constexpr int Ns = N / R;
+  constexpr int nb = N / R;
+  constexpr int MB = (nb + G - 1) / G;
+#pragma unroll
+  for (int m = 0; m < MB; ++m) {
+    const int b = g + m * G;
+    if (MB * G == nb || b < nb) {
+      const int k = mod_ns(b, Ns);
+      const int j0 = (b - k) * R + k;
+#pragma unroll
+      for (int r = 0; r < R; ++r) f(m * R + r, j0 + r * Ns);
+    }
+  }
+}
+
+// ---- plan selectors -----------------------------------------------------------
+// A kernel body templated on a selector runs the runtime engine (PlanRt) or
+// the fixed-length engine (PlanCt<N>) for one axis.
+
+struct PlanRt {
+  static constexpr int fixed = 0;
+  template <int NP, bool INV, class Ctx, class Src, class Dst>
+  KW_BODY static void fft(Ctx& c, const FftPlan& pl, int g, int G, bool active, Src& src,
+                          Dst& dst, const LdsPencils& lds) {
+    fft_rt<NP, INV>(c, pl, g, G, active, src, dst, lds);
+  }
+  template <class F>
+  KW_BODY static void last_slots(const FftPlan& pl, int g, int G, bool active, F& f) {
+    last_slots_rt(pl, g, G, active, f);
+  }
+};
+
+template <int N_>
+struct PlanCt {
+  static constexpr int fixed = N_;
+  static constexpr int G = N_ / KW_E;
+  static_assert(N_ % KW_E == 0, "fixed lengths must be multiples of KW_E");
+  template <int NP, bool INV, class Ctx, class Src, class Dst>
+  KW_BODY static void fft(Ctx& c, const FftPlan& pl, int g, int, bool active, Src& src, Dst& dst,
+                          const LdsPencils& lds) {
+    fft_ct<N_, G, NP, INV>(c, pl.tw, g, active, src, dst, lds);
+  }
+  template <class F>
+  KW_BODY static void last_slots(const FftPlan&, int g, int, bool active, F& f) {
+    last_slots_ct<N_, G>(g, active, f);
+  }
+};
 
 }  // namespace kw

@@ -6,17 +6,21 @@
 #include <string>
 
 #include "kw_backend.h"
+#include "kw_variants.h"
 
 namespace kw {
 
-void launch_k_yfwd(int, int, size_t, void*, const Batch&, const PassArgs&);
-void launch_k_yinv(int, int, size_t, void*, const Batch&, const PassArgs&);
-void launch_k_spec(int, int, size_t, void*, const Batch&, const PassArgs&);
-void launch_k_x_r2c(int, int, size_t, void*, const Batch&, const PassArgs&);
-void launch_k_x_c2r(int, int, size_t, void*, const Batch&, const PassArgs&);
-void launch_k_x_grad(int, int, size_t, void*, const Batch&, const PassArgs&);
-void launch_k_x_div(int, int, size_t, void*, const Batch&, const PassArgs&);
-void launch_k_x_eos(int, int, size_t, void*, const Batch&, const PassArgs&);
+#define KW_DECLARE_KINDS(SIZE)                                                    \
+  KW_DECLARE_VARIANT(0, SIZE)                                                     \
+  KW_DECLARE_VARIANT(1, SIZE)                                                     \
+  KW_DECLARE_VARIANT(2, SIZE)                                                     \
+  KW_DECLARE_VARIANT(3, SIZE)                                                     \
+  KW_DECLARE_VARIANT(4, SIZE)                                                     \
+  KW_DECLARE_VARIANT(5, SIZE)                                                     \
+  KW_DECLARE_VARIANT(6, SIZE)                                                     \
+  KW_DECLARE_VARIANT(7, SIZE)
+KW_DECLARE_KINDS(0)
+KW_FIXED_SIZES(KW_DECLARE_KINDS)
 void launch_k_ew(int, int, size_t, void*, const Batch&, const EwArgs&);
 
 static void check(hipError_t e, const char* what) {
@@ -60,19 +64,31 @@ void dev_zero(void* dst, size_t bytes, void* s) {
 
 size_t dev_max_lds() { return 65536; }
 
-void launch_pass(PassKind kind, int nblocks, int threads, size_t lds_bytes, const Batch& B,
-                 const PassArgs& A, void* stream) {
+void launch_pass(PassKind kind, int fixed, int dim, int nblocks, int threads, size_t lds_bytes,
+                 const Batch& B, const PassArgs& A, void* stream) {
   if (nblocks <= 0) return;
-  switch (kind) {
-    case PK_YFWD: launch_k_yfwd(nblocks, threads, lds_bytes, stream, B, A); break;
-    case PK_YINV: launch_k_yinv(nblocks, threads, lds_bytes, stream, B, A); break;
-    case PK_SPEC: launch_k_spec(nblocks, threads, lds_bytes, stream, B, A); break;
-    case PK_X_R2C: launch_k_x_r2c(nblocks, threads, lds_bytes, stream, B, A); break;
-    case PK_X_C2R: launch_k_x_c2r(nblocks, threads, lds_bytes, stream, B, A); break;
-    case PK_X_GRAD: launch_k_x_grad(nblocks, threads, lds_bytes, stream, B, A); break;
-    case PK_X_DIV: launch_k_x_div(nblocks, threads, lds_bytes, stream, B, A); break;
-    case PK_X_EOS: launch_k_x_eos(nblocks, threads, lds_bytes, stream, B, A); break;
+#define KW_CASE_KIND(KIND, SIZE) \
+  case KIND: launch_v_##KIND##_##SIZE(dim, nblocks, threads, lds_bytes, stream, B, A); break;
+#define KW_CASE_SIZE(SIZE)        \
+  case SIZE:                      \
+    switch ((int)kind) {          \
+      KW_CASE_KIND(0, SIZE)       \
+      KW_CASE_KIND(1, SIZE)       \
+      KW_CASE_KIND(2, SIZE)       \
+      KW_CASE_KIND(3, SIZE)       \
+      KW_CASE_KIND(4, SIZE)       \
+      KW_CASE_KIND(5, SIZE)       \
+      KW_CASE_KIND(6, SIZE)       \
+      KW_CASE_KIND(7, SIZE)       \
+    }                             \
+    break;
+  switch (fixed) {
+    KW_CASE_SIZE(0)
+    KW_FIXED_SIZES(KW_CASE_SIZE)
+    default: throw DeviceError(-3, "no kernel variant for this length");
   }
+#undef KW_CASE_SIZE
+#undef KW_CASE_KIND
   check(hipGetLastError(), "kernel launch");
 }
 

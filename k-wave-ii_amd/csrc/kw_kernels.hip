@@ -1,9 +1,12 @@
-// gfx950 kernels.  This file is compiled once per kernel (KW_KERNEL = 0..8)
-// so that the kernels build in parallel; each compile defines one kernel and
-// its host launcher.
+// gfx950 kernels.  This file is compiled once per (pass kind, fixed length)
+// variant (KW_KIND = 0..7 and KW_SIZE = 0 for runtime lengths or one of
+// KW_FIXED_SIZES), or with KW_KIND = 8 for the elementwise kernel, so that
+// the variants build in parallel; each compile defines its kernels and one
+// host launcher.
 #include <hip/hip_runtime.h>
 
 #include "kw_backend.h"
+#include "kw_variants.h"
 
 extern __shared__ kw::cf kw_lds_dyn[];
 
@@ -16,37 +19,100 @@ struct GpuCtx {
   __device__ __forceinline__ cf* lds() const { return kw_lds_dyn; }
 };
 
-#define KW_DEFINE(ID, NAME, BOUNDS, ARGT, CALL)                                        \
-  __global__ __launch_bounds__(BOUNDS) void NAME(Batch B, ARGT A) {                    \
-    GpuCtx c;                                                                          \
-    CALL;                                                                              \
-  }                                                                                    \
-  void launch_##NAME(int nb, int T, size_t lds, void* s, const Batch& B, const ARGT& A) { \
-    hipLaunchKernelGGL(NAME, dim3(nb), dim3(T), lds, (hipStream_t)s, B, A);              \
-  }
+#define KW_CAT2(a, b) a##b
+#define KW_CAT(a, b) KW_CAT2(a, b)
+
+#if KW_KIND == 8
+
+__global__ __launch_bounds__(256) void k_ew(Batch B, EwArgs A) {
+  GpuCtx c;
+  body_ew(c, B, A);
+}
+void launch_k_ew(int nb, int T, size_t lds, void* s, const Batch& B, const EwArgs& A) {
+  hipLaunchKernelGGL(k_ew, dim3(nb), dim3(T), lds, (hipStream_t)s, B, A);
+}
+
+#else
+
+// kernel names unique per variant: kv_<kind>_<size>[_d2]
+#define kern KW_CAT(KW_CAT(KW_CAT(kv_, KW_KIND), _), KW_SIZE)
+#define kern2 KW_CAT(kern, _d2)
+
+#if KW_SIZE == 0
+using Sel = PlanRt;
+#else
+using Sel = PlanCt<KW_SIZE>;
+#endif
 
 // x passes run at most 256 threads per block; strided and last-axis passes
-// at most 1024
-#if KW_KERNEL == 0
-KW_DEFINE(0, k_yfwd, 1024, PassArgs, body_ypass<false>(c, B, A))
-#elif KW_KERNEL == 1
-KW_DEFINE(1, k_yinv, 1024, PassArgs, body_ypass<true>(c, B, A))
-#elif KW_KERNEL == 2
-KW_DEFINE(2, k_spec, 1024, PassArgs, body_spec(c, B, A))
-#elif KW_KERNEL == 3
-KW_DEFINE(3, k_x_r2c, 256, PassArgs, body_x_r2c(c, B, A))
-#elif KW_KERNEL == 4
-KW_DEFINE(4, k_x_c2r, 256, PassArgs, body_x_c2r(c, B, A))
-#elif KW_KERNEL == 5
-KW_DEFINE(5, k_x_grad, 256, PassArgs, body_x_grad(c, B, A))
-#elif KW_KERNEL == 6
-KW_DEFINE(6, k_x_div, 256, PassArgs, body_x_div(c, B, A))
-#elif KW_KERNEL == 7
-KW_DEFINE(7, k_x_eos, 256, PassArgs, body_x_eos(c, B, A))
-#elif KW_KERNEL == 8
-KW_DEFINE(8, k_ew, 256, EwArgs, body_ew(c, B, A))
+// at most 512
+#if KW_KIND == 0
+__global__ __launch_bounds__(512) void kern(Batch B, PassArgs A) {
+  GpuCtx c;
+  body_ypass<false, Sel>(c, B, A);
+}
+#elif KW_KIND == 1
+__global__ __launch_bounds__(512) void kern(Batch B, PassArgs A) {
+  GpuCtx c;
+  body_ypass<true, Sel>(c, B, A);
+}
+#elif KW_KIND == 2
+__global__ __launch_bounds__(512) void kern(Batch B, PassArgs A) {
+  GpuCtx c;
+  body_spec<Sel>(c, B, A);
+}
+#elif KW_KIND == 3
+__global__ __launch_bounds__(256) void kern(Batch B, PassArgs A) {
+  GpuCtx c;
+  body_x_r2c<Sel>(c, B, A);
+}
+#elif KW_KIND == 4
+__global__ __launch_bounds__(256) void kern(Batch B, PassArgs A) {
+  GpuCtx c;
+  body_x_c2r<Sel>(c, B, A);
+}
+#elif KW_KIND == 5
+__global__ __launch_bounds__(256) void kern2(Batch B, PassArgs A) {
+  GpuCtx c;
+  body_x_grad<Sel, 2>(c, B, A);
+}
+__global__ __launch_bounds__(256) void kern(Batch B, PassArgs A) {
+  GpuCtx c;
+  body_x_grad<Sel, 3>(c, B, A);
+}
+#define KW_HAS_DIM2 1
+#elif KW_KIND == 6
+__global__ __launch_bounds__(256) void kern2(Batch B, PassArgs A) {
+  GpuCtx c;
+  body_x_div<Sel, 2>(c, B, A);
+}
+__global__ __launch_bounds__(256) void kern(Batch B, PassArgs A) {
+  GpuCtx c;
+  body_x_div<Sel, 3>(c, B, A);
+}
+#define KW_HAS_DIM2 1
+#elif KW_KIND == 7
+__global__ __launch_bounds__(256) void kern(Batch B, PassArgs A) {
+  GpuCtx c;
+  body_x_eos<Sel>(c, B, A);
+}
 #else
-#error "KW_KERNEL must be 0..8"
+#error "KW_KIND must be 0..8"
+#endif
+
+void KW_CAT(KW_CAT(KW_CAT(launch_v_, KW_KIND), _), KW_SIZE)(int dim, int nb, int T, size_t lds,
+                                                            void* s, const Batch& B,
+                                                            const PassArgs& A) {
+#ifdef KW_HAS_DIM2
+  if (dim == 2) {
+    hipLaunchKernelGGL(kern2, dim3(nb), dim3(T), lds, (hipStream_t)s, B, A);
+    return;
+  }
+#endif
+  (void)dim;
+  hipLaunchKernelGGL(kern, dim3(nb), dim3(T), lds, (hipStream_t)s, B, A);
+}
+
 #endif
 
 }  // namespace kw

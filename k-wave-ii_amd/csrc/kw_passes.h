@@ -11,6 +11,11 @@
 // without leaving LDS; the x passes fuse the C2R and R2C transforms with
 // every elementwise update of the reference (u, rho, sources, equation of
 // state, sensors, time-reversal enforcement).
+//
+// Bodies are templated on a plan selector per axis (PlanRt for any length,
+// PlanCt<N> for a length fixed at compile time) and the x passes on the
+// number of dimensions, which is the number of field components they
+// transform together.
 #pragma once
 
 #include "kw_prob.h"
@@ -24,34 +29,38 @@ namespace kw {
 
 // element n (0 <= n < Nx) of the Hermitian extension of row `row`, with the
 // imaginary parts of the DC and Nyquist bins dropped, which is what
-// real(ifft(.)) does with them
-KW_BODY cf half_get(const ProbDev& P, const cf* S, int64_t row, int n) {
+// real(ifft(.)) does with them; branch-free so the loads batch
+KW_BODY cf half_get(const ProbDev& P, const cf* S, uint32_t hrow, uint32_t qrow, int n) {
+  // hrow = row * Wm (main block), qrow = offset of the row in the Nyquist plane
   const int N = P.Nx;
-  cf v;
-  if (n < P.Wm) {
-    v = KW_LD(S + row * P.Wm + n);
-    if (n == 0) v.y = 0.f;
-  } else if (P.nyq && n == (N >> 1)) {
-    v = KW_LD(S + P.nspec_main + row);
-    v.y = 0.f;
-  } else {
-    v = cconj(KW_LD(S + row * P.Wm + (N - n)));
-  }
+  const int h = N >> 1;
+  const bool mirror = n > h;
+  const bool isq = P.nyq && n == h;
+  const uint32_t k = (uint32_t)(mirror ? N - n : n);
+  const uint32_t idx = isq ? qrow : hrow + k;
+  cf v = KW_LD32(S, idx);
+  v.y = (n == 0 || isq) ? 0.f : (mirror ? -v.y : v.y);
   return v;
 }
 
-KW_BODY void half_put(const ProbDev& P, cf* S, int64_t row, int k, cf v) {
-  if (k < P.Wm)
-    KW_ST(S + row * P.Wm + k, v);
-  else
-    KW_ST(S + P.nspec_main + row, v);
+KW_BODY void half_put(const ProbDev& P, cf* S, uint32_t hrow, uint32_t qrow, int k, cf v) {
+  const uint32_t idx = k < P.Wm ? hrow + (uint32_t)k : qrow;
+  KW_ST32(S, idx, v);
 }
+
+// pencils per row pair held in LDS by the x passes
+constexpr int kNPX = 3;
+
+KW_HOSTDEV int x_lds_stride(int Nx) { return Nx + (Nx >> 3) + 1; }
 
 struct XTile {
   int g, G;
   bool active, has1;
   int64_t r0, r1;
-  LdsPencil lds;
+  // 32-bit offsets of the two rows: first cell (row * Nx), half-spectrum row
+  // (row * Wm), Nyquist entry (nspec_main + row); y and z of each row
+  uint32_t c0, c1, h0, h1, q0, q1, y0, y1, z0, z1;
+  LdsPencils lds;
 };
 
 template <class Ctx>
@@ -62,126 +71,128 @@ KW_BODY XTile x_tile(Ctx& c, const ProbDev& P, int tile) {
   int p = tid / T.G;
   T.g = tid % T.G;
   const int64_t npairs = (P.nrows + 1) / 2;
-  const int64_t pair = (int64_t)tile * P.Px + p;
+  int64_t pair = (int64_t)tile * P.Px + p;
   T.active = p < P.Px && pair < npairs;
-  if (p >= P.Px) p = 0;
+  // threads without a row pair read row pair 0 (their loads stay in bounds
+  // and unconditional) and never store
+  if (!T.active) {
+    p = 0;
+    pair = 0;
+  }
   T.r0 = 2 * pair;
-  T.r1 = 2 * pair + 1;
-  T.has1 = T.r1 < P.nrows;
-  T.lds = LdsPencil{c.lds() + (int64_t)p * (P.Nx + 1), 1};
+  T.has1 = 2 * pair + 1 < P.nrows;
+  T.r1 = T.has1 ? 2 * pair + 1 : T.r0;
+  const uint32_t r0 = (uint32_t)T.r0, r1 = (uint32_t)T.r1;
+  T.c0 = r0 * (uint32_t)P.Nx;
+  T.c1 = r1 * (uint32_t)P.Nx;
+  T.h0 = r0 * (uint32_t)P.Wm;
+  T.h1 = r1 * (uint32_t)P.Wm;
+  T.q0 = (uint32_t)P.nspec_main + r0;
+  T.q1 = (uint32_t)P.nspec_main + r1;
+  T.y0 = r0 % (uint32_t)P.Ny;
+  T.z0 = r0 / (uint32_t)P.Ny;
+  T.y1 = r1 % (uint32_t)P.Ny;
+  T.z1 = r1 / (uint32_t)P.Ny;
+  const int stride = x_lds_stride(P.Nx);
+  T.lds = LdsPencils{c.lds() + (int64_t)p * kNPX * stride, stride, 1, true};
   return T;
 }
 
-// LDS holds Z = fft(a + i b) in natural order; split into the half spectra
-// of a (row r0) and b (row r1) and store them.
-template <class Ctx>
-KW_BODY void x_split_store(Ctx& c, const ProbDev& P, const XTile& T, cf* S) {
+// LDS pencil j holds Z = fft(a + i b) in natural order; split into the half
+// spectra of a (row r0) and b (row r1) and store them to S[j]
+template <int NP, class Ctx>
+KW_BODY void x_split_store(Ctx& c, const ProbDev& P, const XTile& T, cf* const* S) {
   const int N = P.Nx;
   const int K = N / 2 + 1;
   if (T.active) {
     for (int k = T.g; k < K; k += T.G) {
-      const cf zk = T.lds.get(k);
-      const cf zm = T.lds.get(k == 0 ? 0 : N - k);
-      const cf A = cf{0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y)};
-      const cf B = cf{0.5f * (zk.y + zm.y), 0.5f * (zm.x - zk.x)};
-      half_put(P, S, T.r0, k, A);
-      if (T.has1) half_put(P, S, T.r1, k, B);
+#pragma unroll
+      for (int j = 0; j < NP; ++j) {
+        const cf zk = T.lds.get(j, k);
+        const cf zm = T.lds.get(j, k == 0 ? 0 : N - k);
+        const cf A = cf{0.5f * (zk.x + zm.x), 0.5f * (zk.y - zm.y)};
+        const cf B = cf{0.5f * (zk.y + zm.y), 0.5f * (zm.x - zk.x)};
+        // r1 == r0 for a single-row pair: the r0 store comes last and wins
+        half_put(P, S[j], T.h1, T.q1, k, B);
+        half_put(P, S[j], T.h0, T.q0, k, A);
+      }
     }
   }
   c.sync();
 }
 
-// forward transform of the complex pencil already in LDS, then split/store
-template <class Ctx>
-KW_BODY void x_fwd_from_lds(Ctx& c, const ProbDev& P, const XTile& T, cf* S) {
-  auto src = [&](int n) -> cf { return T.lds.get(n); };
-  auto dst = [&](int n, cf v, int) { T.lds.set(n, v); };
-  fft<false>(c, P.fx, T.g, T.G, T.active, src, dst, T.lds);
-  x_split_store(c, P, T, S);
+// forward transform of NP complex pencils already in LDS, then split/store
+template <class XS, int NP, class Ctx>
+KW_BODY void x_fwd_from_lds(Ctx& c, const ProbDev& P, const XTile& T, cf* const* S) {
+  auto src = [&](int j, int n) -> cf { return T.lds.get(j, n); };
+  auto dst = [&](int j, int n, cf v, int) { T.lds.set(j, n, v); };
+  XS::template fft<NP, false>(c, P.fx, T.g, T.G, T.active, src, dst, T.lds);
+  x_split_store<NP>(c, P, T, S);
 }
 
-// inverse two-for-one load functor: Z[n] = A[n] + i B[n]
+// inverse two-for-one load functor: Z_j[n] = A_j[n] + i B_j[n]; row r1 is
+// r0 when the pair has a single row (the B part is then zeroed by select)
+template <int NP>
 struct HalfPairLoad {
   const ProbDev* P;
-  const cf* S;
-  int64_t r0, r1;
+  const cf* S[NP];
+  uint32_t h0, h1, q0, q1;
   bool has1;
-  KW_BODY cf operator()(int n) const {
-    const cf a = half_get(*P, S, r0, n);
-    const cf b = has1 ? half_get(*P, S, r1, n) : cf{0.f, 0.f};
+  KW_BODY cf operator()(int j, int n) const {
+    const cf a = half_get(*P, S[j], h0, q0, n);
+    cf b = half_get(*P, S[j], h1, q1, n);
+    b.x = has1 ? b.x : 0.f;
+    b.y = has1 ? b.y : 0.f;
     return cf{a.x - b.y, a.y + b.x};
   }
 };
 
-KW_BODY float pml_row(const float* v, const ProbDev& P, int d, int64_t row) {
-  // value of the 1-D profile of axis d (d = 1: y, d = 2: z) for a row
-  const int64_t y = row % P.Ny;
-  const int64_t z = row / P.Ny;
-  return KW_LD(v + (d == 1 ? y : z));
+// 1-D PML factors of axis d for the two rows at x = n: x varies along the
+// row, y and z are constant along it
+KW_BODY float pml_at(const float* v, const XTile& T, int d, int row, int n) {
+  if (d == 0) return KW_LD32(v, (uint32_t)n);
+  const uint32_t i = d == 1 ? (row == 0 ? T.y0 : T.y1) : (row == 0 ? T.z0 : T.z1);
+  return KW_LD32(v, i);
 }
 
-// time-reversal enforcement, sensors and optional store for a new p value
-KW_BODY float finish_p(const ProbDev& P, int64_t cell, float p, int t, bool store) {
-  if (P.time_rev) {
-    const int32_t i = KW_LD(P.tr_map + cell);
-    if (i >= 0) {
-      // data of step t+1 of the reversed series = original column Nt - t
-      p = KW_LD(P.tr_data + (int64_t)i + (int64_t)(P.Nt - t - 1) * P.nsens);
-      const float c0 = P.c0.at(cell);
-      const float c2 = c0 * c0;
-      float r;
-      if (P.dim == 3)
-        r = p / (3.f * c2);
-      else
-        r = (0.5f * p) / c2;
-      for (int d = 0; d < P.dim; ++d) KW_ST(P.rho[d] + cell, r);
-    }
-  }
-  if (!P.time_rev && t >= P.record_start) {
-    const int fi = t - P.record_start + 1;
-    const double pd = (double)p;
-    const uint32_t rec = P.record;
-    if (P.full_mask) {
-      // sensor mask == whole grid: point statistics fused here
-      if (rec & REC_P) KW_ST(P.p_ts + cell + (int64_t)(fi - 1) * P.ncell, p);
-      if (rec & REC_P_MAX) {
-        double m = fi == 1 ? pd : KW_LD(P.p_max + cell);
-        KW_ST(P.p_max + cell, pd > m ? pd : m);
-      }
-      if (rec & REC_P_MIN) {
-        double m = fi == 1 ? pd : KW_LD(P.p_min + cell);
-        KW_ST(P.p_min + cell, pd < m ? pd : m);
-      }
-      if (rec & REC_P_RMS) {
-        double s = fi == 1 ? 0.0 : KW_LD(P.p_rms2 + cell);
-        KW_ST(P.p_rms2 + cell, s + pd * pd);
-      }
-    }
-    if (rec & REC_P_MAX_ALL) {
-      double m = fi == 1 ? pd : KW_LD(P.p_max_all + cell);
-      KW_ST(P.p_max_all + cell, pd > m ? pd : m);
-    }
-    if (rec & REC_P_MIN_ALL) {
-      double m = fi == 1 ? pd : KW_LD(P.p_min_all + cell);
-      KW_ST(P.p_min_all + cell, pd < m ? pd : m);
-    }
-  }
-  if (store) KW_ST(P.p + cell, p);
-  return p;
-}
-
-// value added to / imposed on a field by a time source at one cell
-// returns the new value
-KW_BODY float apply_src(const TimeSource& s, int64_t cell, float v, int t) {
+// Time-source update of one cell, for a source form fixed at compile time:
+// 0 additive single series (s(t) K_M), 1 additive several series (filtered
+// grid), 2 dirichlet (map), 3 additive-no-correction (map)
+template <int FORM>
+KW_BODY float apply_src(const TimeSource& s, uint32_t cell, float v, int t) {
   const int t0 = t - 1;
-  if (s.mode == SRC_ADDITIVE) {
-    if (s.km) return v + s.value(0, t0) * KW_LD(s.km + cell);
-    return v + KW_LD(s.grid + cell);
+  if constexpr (FORM == 0) {
+    return v + s.value(0, t0) * KW_LD32(s.km, cell);
+  } else if constexpr (FORM == 1) {
+    return v + KW_LD32(s.grid, cell);
+  } else {
+    const int32_t i = KW_LD32(s.map, cell);
+    const float sv = s.value(i < 0 ? 0 : i, t0);
+    if constexpr (FORM == 2)
+      return i >= 0 ? sv : v;
+    else
+      return i >= 0 ? v + sv : v;
   }
-  const int32_t i = KW_LD(s.map + cell);
-  if (i < 0) return v;
-  const float sv = s.value(i, t0);
-  return s.mode == SRC_DIRICHLET ? sv : v + sv;
+}
+
+KW_BODY int src_form(const TimeSource& s) {
+  if (s.mode == SRC_ADDITIVE) return s.km ? 0 : 1;
+  return s.mode == SRC_DIRICHLET ? 2 : 3;
+}
+
+// run f(IntC<form>) for the source's form, or f(IntC<-1>) when it is off
+template <class F>
+KW_BODY void with_src(const TimeSource& s, bool on, F&& f) {
+  if (!on) {
+    f(IntC<-1>{});
+    return;
+  }
+  switch (src_form(s)) {
+    case 0: f(IntC<0>{}); break;
+    case 1: f(IntC<1>{}); break;
+    case 2: f(IntC<2>{}); break;
+    default: f(IntC<3>{}); break;
+  }
 }
 
 template <class Ctx>
@@ -194,7 +205,7 @@ KW_BODY const ProbDev& batch_prob(Ctx& c, const Batch& B, int& tile) {
 // ---------------------------------------------------------------------------
 // strided pass along y (3D only), forward or inverse, in place
 // ---------------------------------------------------------------------------
-template <bool INV, class Ctx>
+template <bool INV, class YS, class Ctx>
 KW_BODY void body_ypass(Ctx& c, const Batch& B, const PassArgs& A) {
   int tile;
   const ProbDev& P = batch_prob(c, B, tile);
@@ -203,32 +214,36 @@ KW_BODY void body_ypass(Ctx& c, const Batch& B, const PassArgs& A) {
   const int64_t tH = (nH + Pn - 1) / Pn;
   const int tid = c.tid();
   int p, g;
-  int64_t base, stride;
+  uint32_t base, stride;
   bool active;
-  LdsPencil lds;
+  LdsPencils lds;
   if (tile < tH) {
     p = tid % Pn;
     g = tid / Pn;
-    const int64_t q = (int64_t)tile * Pn + p;
+    int64_t q = (int64_t)tile * Pn + p;
     active = q < nH;
-    base = (q / P.Wm) * ((int64_t)P.Wm * P.Ny) + (q % P.Wm);
-    stride = P.Wm;
-    lds = LdsPencil{c.lds() + p, Pn};
+    if (!active) q = 0;  // read a valid pencil, never store
+    base = (uint32_t)((q / P.Wm) * ((int64_t)P.Wm * P.Ny) + (q % P.Wm));
+    stride = (uint32_t)P.Wm;
+    lds = LdsPencils{c.lds() + p, 0, Pn, false};
   } else {
     p = tid / G;
     g = tid % G;
-    const int64_t q = (tile - tH) * Pn + p;
+    int64_t q = (tile - tH) * Pn + p;
     active = p < Pn && q < P.Nz;
-    if (p >= Pn) p = 0;
-    base = P.nspec_main + q * P.Ny;
+    if (!active) {
+      p = 0;
+      q = 0;
+    }
+    base = (uint32_t)(P.nspec_main + q * P.Ny);
     stride = 1;
-    lds = LdsPencil{c.lds() + (int64_t)p * (N + 1), 1};
+    lds = LdsPencils{c.lds() + (int64_t)p * (N + 1), 0, 1, false};
   }
   for (int b = 0; b < A.nbuf; ++b) {
     cf* S = P.S[A.buf[b]];
-    auto ld = [&](int n) -> cf { return KW_LD(S + base + n * stride); };
-    auto st = [&](int n, cf v, int) { KW_ST(S + base + n * stride, v); };
-    fft<INV>(c, P.fy, g, G, active, ld, st, lds);
+    auto ld = [&](int, int n) -> cf { return KW_LD32(S, base + (uint32_t)n * stride); };
+    auto st = [&](int, int n, cf v, int) { KW_ST32(S, base + (uint32_t)n * stride, v); };
+    YS::template fft<1, INV>(c, P.fy, g, G, active, ld, st, lds);
   }
 }
 
@@ -237,7 +252,7 @@ KW_BODY void body_ypass(Ctx& c, const Batch& B, const PassArgs& A) {
 // FFT, all in LDS.  Operators (kspaceFirstOrder3D.m:629-661, 792-803,
 // 907-909, 977-978; createAbsorptionVariables.m:29-84).
 // ---------------------------------------------------------------------------
-template <class Ctx>
+template <class LS, class Ctx>
 KW_BODY void body_spec(Ctx& c, const Batch& B, const PassArgs& A) {
   int tile;
   const ProbDev& P = batch_prob(c, B, tile);
@@ -250,70 +265,75 @@ KW_BODY void body_spec(Ctx& c, const Batch& B, const PassArgs& A) {
   const int tid = c.tid();
   const int p = tid % Pn;
   const int g = tid / Pn;
-  int64_t q, base, stride;
+  int64_t q;
+  uint32_t base, stride;
   int kx, ky;  // ky = -1: ky is the pencil coordinate (2D)
   bool active;
   if (tile < tH) {
     q = (int64_t)tile * Pn + p;
     active = q < nH;
-    base = q;
-    stride = d3 ? (int64_t)P.Wm * P.Ny : (int64_t)P.Wm;
+    base = (uint32_t)q;
+    stride = (uint32_t)(d3 ? (int64_t)P.Wm * P.Ny : (int64_t)P.Wm);
     kx = (int)(q % P.Wm);
     ky = d3 ? (int)(q / P.Wm) : -1;
   } else {
     q = (tile - tH) * Pn + p;
     active = q < (d3 ? P.Ny : 1);
-    base = P.nspec_main + q;
-    stride = d3 ? P.Ny : 1;
+    base = (uint32_t)(P.nspec_main + q);
+    stride = (uint32_t)(d3 ? P.Ny : 1);
     kx = P.Nx / 2;
     ky = d3 ? (int)q : -1;
   }
   if (!active) {
+    // read pencil 0 of the main block, never store
+    q = 0;
+    base = 0;
+    stride = (uint32_t)(d3 ? (int64_t)P.Wm * P.Ny : (int64_t)P.Wm);
     kx = 0;
     ky = d3 ? 0 : -1;
   }
-  LdsPencil la{c.lds() + p, Pn};
-  LdsPencil lb{c.lds() + (int64_t)Pn * N + p, Pn};
-  auto addr = [&](int n) -> int64_t { return base + n * stride; };
-  auto to_a = [&](int n, cf v, int) { la.set(n, v); };
+  LdsPencils la{c.lds() + p, 0, Pn, false};
+  LdsPencils lb{c.lds() + (int64_t)Pn * N + p, 0, Pn, false};
+  auto addr = [&](int n) -> uint32_t { return base + (uint32_t)n * stride; };
+  auto to_a = [&](int, int n, cf v, int) { la.set(0, n, v); };
 
   if (A.mode == SM_GRAD) {
     cf* S0 = P.S[0];
-    auto ld = [&](int n) -> cf { return KW_LD(S0 + addr(n)); };
-    fft<false>(c, pl, g, G, active, ld, to_a, la);
+    auto ld = [&](int, int n) -> cf { return KW_LD32(S0, addr(n)); };
+    LS::template fft<1, false>(c, pl, g, G, active, ld, to_a, la);
     // C = ifft(kappa .* S); x (and in 3D y) derivatives are per-pencil
     // constants times C
     const cf dx = active ? KW_LD(P.dpos[0] + kx) : cf{0.f, 0.f};
     const cf dy = (active && d3) ? KW_LD(P.dpos[1] + ky) : cf{0.f, 0.f};
-    auto src1 = [&](int n) -> cf { return cscale(la.get(n), KW_LD(P.kappa + addr(n))); };
-    auto st1 = [&](int n, cf v, int) {
-      KW_ST(P.S[0] + addr(n), cmul(dx, v));
-      if (d3) KW_ST(P.S[1] + addr(n), cmul(dy, v));
+    auto src1 = [&](int, int n) -> cf { return cscale(la.get(0, n), KW_LD32(P.kappa, addr(n))); };
+    auto st1 = [&](int, int n, cf v, int) {
+      KW_ST32(P.S[0], addr(n), cmul(dx, v));
+      if (d3) KW_ST32(P.S[1], addr(n), cmul(dy, v));
     };
-    fft<true>(c, pl, g, G, active, src1, st1, lb);
+    LS::template fft<1, true>(c, pl, g, G, active, src1, st1, lb);
     // derivative along the pencil axis
     const cf* dl = P.dpos[last];
-    auto src2 = [&](int n) -> cf {
-      return cmul(KW_LD(dl + n), cscale(la.get(n), KW_LD(P.kappa + addr(n))));
+    auto src2 = [&](int, int n) -> cf {
+      return cmul(KW_LD32(dl, (uint32_t)n), cscale(la.get(0, n), KW_LD32(P.kappa, addr(n))));
     };
     cf* So = P.S[last];
-    auto st2 = [&](int n, cf v, int) { KW_ST(So + addr(n), v); };
-    fft<true>(c, pl, g, G, active, src2, st2, lb);
+    auto st2 = [&](int, int n, cf v, int) { KW_ST32(So, addr(n), v); };
+    LS::template fft<1, true>(c, pl, g, G, active, src2, st2, lb);
   } else if (A.mode == SM_DIV) {
     for (int d = 0; d < P.dim; ++d) {
       cf* S = P.S[d];
-      auto ld = [&](int n) -> cf { return KW_LD(S + addr(n)); };
-      fft<false>(c, pl, g, G, active, ld, to_a, la);
+      auto ld = [&](int, int n) -> cf { return KW_LD32(S, addr(n)); };
+      LS::template fft<1, false>(c, pl, g, G, active, ld, to_a, la);
       const cf dc = (d == last || !active) ? cf{0.f, 0.f}
                                            : KW_LD(P.dneg[d] + (d == 0 ? kx : ky));
       const cf* dl = P.dneg[d];
       const bool along = d == last;
-      auto src = [&](int n) -> cf {
-        const cf s = cscale(la.get(n), KW_LD(P.kappa + addr(n)));
-        return cmul(along ? KW_LD(dl + n) : dc, s);
+      auto src = [&](int, int n) -> cf {
+        const cf s = cscale(la.get(0, n), KW_LD32(P.kappa, addr(n)));
+        return cmul(along ? KW_LD32(dl, (uint32_t)n) : dc, s);
       };
-      auto st = [&](int n, cf v, int) { KW_ST(S + addr(n), v); };
-      fft<true>(c, pl, g, G, active, src, st, la);
+      auto st = [&](int, int n, cf v, int) { KW_ST32(S, addr(n), v); };
+      LS::template fft<1, true>(c, pl, g, G, active, src, st, la);
     }
   } else {
     // SM_ABS: S0 <- nabla1 .* S0, S1 <- nabla2 .* S1;  SM_FILTER: one table
@@ -321,11 +341,11 @@ KW_BODY void body_spec(Ctx& c, const Batch& B, const PassArgs& A) {
     for (int j = 0; j < nb; ++j) {
       cf* S = P.S[A.mode == SM_ABS ? j : A.buf[0]];
       const float* tab = A.mode == SM_ABS ? (j == 0 ? P.nabla1 : P.nabla2) : A.table;
-      auto ld = [&](int n) -> cf { return KW_LD(S + addr(n)); };
-      fft<false>(c, pl, g, G, active, ld, to_a, la);
-      auto src = [&](int n) -> cf { return cscale(la.get(n), KW_LD(tab + addr(n))); };
-      auto st = [&](int n, cf v, int) { KW_ST(S + addr(n), v); };
-      fft<true>(c, pl, g, G, active, src, st, la);
+      auto ld = [&](int, int n) -> cf { return KW_LD32(S, addr(n)); };
+      LS::template fft<1, false>(c, pl, g, G, active, ld, to_a, la);
+      auto src = [&](int, int n) -> cf { return cscale(la.get(0, n), KW_LD32(tab, addr(n))); };
+      auto st = [&](int, int n, cf v, int) { KW_ST32(S, addr(n), v); };
+      LS::template fft<1, true>(c, pl, g, G, active, src, st, la);
     }
   }
 }
@@ -335,7 +355,7 @@ KW_BODY void body_spec(Ctx& c, const Batch& B, const PassArgs& A) {
 // ---------------------------------------------------------------------------
 
 // real grid(s) -> half spectrum (PassArgs.real[b] -> S[buf[b]])
-template <class Ctx>
+template <class XS, class Ctx>
 KW_BODY void body_x_r2c(Ctx& c, const Batch& B, const PassArgs& A) {
   int tile;
   const ProbDev& P = batch_prob(c, B, tile);
@@ -343,19 +363,20 @@ KW_BODY void body_x_r2c(Ctx& c, const Batch& B, const PassArgs& A) {
   const int N = P.Nx;
   for (int b = 0; b < A.nbuf; ++b) {
     const float* f = A.real[b] ? A.real[b] : P.p;
-    auto src = [&](int n) -> cf {
-      const float a = KW_LD(f + T.r0 * N + n);
-      const float bb = T.has1 ? KW_LD(f + T.r1 * N + n) : 0.f;
-      return cf{a, bb};
+    auto src = [&](int, int n) -> cf {
+      const float a = KW_LD32(f, T.c0 + n);
+      const float bb = KW_LD32(f, T.c1 + n);
+      return cf{a, T.has1 ? bb : 0.f};
     };
-    auto dst = [&](int n, cf v, int) { T.lds.set(n, v); };
-    fft<false>(c, P.fx, T.g, T.G, T.active, src, dst, T.lds);
-    x_split_store(c, P, T, P.S[A.buf[b]]);
+    auto dst = [&](int j, int n, cf v, int) { T.lds.set(j, n, v); };
+    XS::template fft<1, false>(c, P.fx, T.g, T.G, T.active, src, dst, T.lds);
+    cf* S[1] = {P.S[A.buf[b]]};
+    x_split_store<1>(c, P, T, S);
   }
 }
 
 // half spectrum -> real grid (S[buf[b]] -> PassArgs.real[b]); XC_ACCUM adds
-template <class Ctx>
+template <class XS, class Ctx>
 KW_BODY void body_x_c2r(Ctx& c, const Batch& B, const PassArgs& A) {
   int tile;
   const ProbDev& P = batch_prob(c, B, tile);
@@ -364,24 +385,24 @@ KW_BODY void body_x_c2r(Ctx& c, const Batch& B, const PassArgs& A) {
   const bool acc = A.flags & XC_ACCUM;
   for (int b = 0; b < A.nbuf; ++b) {
     float* f = A.real[b];
-    HalfPairLoad src{&P, P.S[A.buf[b]], T.r0, T.r1, T.has1};
-    auto dst = [&](int n, cf v, int) {
-      float* f0 = f + T.r0 * N + n;
-      KW_ST(f0, acc ? KW_LD(f0) + v.x : v.x);
-      if (T.has1) {
-        float* f1 = f + T.r1 * N + n;
-        KW_ST(f1, acc ? KW_LD(f1) + v.y : v.y);
-      }
+    HalfPairLoad<1> src{&P, {P.S[A.buf[b]]}, T.h0, T.h1, T.q0, T.q1, T.has1};
+    auto dst = [&](int, int n, cf v, int) {
+      const uint32_t i0 = T.c0 + n, i1 = T.c1 + n;
+      const float o0 = acc ? KW_LD32(f, i0) : 0.f;
+      const float o1 = acc ? KW_LD32(f, i1) : 0.f;
+      KW_ST32(f, i1, o1 + v.y);
+      KW_ST32(f, i0, o0 + v.x);
     };
-    fft<true>(c, P.fx, T.g, T.G, T.active, src, dst, T.lds);
+    XS::template fft<1, true>(c, P.fx, T.g, T.G, T.active, src, dst, T.lds);
   }
 }
+
 
 // end of the gradient: u_d = pml_sg_d (pml_sg_d u_d - dt rho0_sg_d^-1 g_d)
 // (kspaceFirstOrder3D.m:792-803), velocity sources (:806-886), then
 // S[d] = fft_x(u_d) for the divergence.  XG_INIT: p0 initial condition
 // u_d = dt rho0_sg_d^-1 g_d / 2 (:1023-1037), no fft.
-template <class Ctx>
+template <class XS, int DIM, class Ctx>
 KW_BODY void body_x_grad(Ctx& c, const Batch& B, const PassArgs& A) {
   int tile;
   const ProbDev& P = batch_prob(c, B, tile);
@@ -390,65 +411,197 @@ KW_BODY void body_x_grad(Ctx& c, const Batch& B, const PassArgs& A) {
   const int t = KW_LD(P.t_index);
   const bool init = A.flags & XG_INIT;
   const float dt = P.dt;
-  for (int d = 0; d < P.dim; ++d) {
-    cf* S = P.S[d];
+  cf gv[DIM][kMaxSlots];
+  HalfPairLoad<DIM> src{&P, {}, T.h0, T.h1, T.q0, T.q1, T.has1};
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) src.S[d] = P.S[d];
+  auto dst = [&](int d, int, cf v, int slot) { gv[d][slot] = v; };
+  XS::template fft<DIM, true>(c, P.fx, T.g, T.G, T.active, src, dst, T.lds);
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
     float* u = P.u[d];
-    const float* pv = P.pml_sg[d];
-    float pr0 = 1.f, pr1 = 1.f;
-    if (d > 0 && T.active) {
-      pr0 = pml_row(pv, P, d, T.r0);
-      if (T.has1) pr1 = pml_row(pv, P, d, T.r1);
-    }
-    const TimeSource& s = P.srcu[d];
-    const bool src_on = !init && s.len >= t;
     const Field ri = P.rinv[d];
-    auto upd = [&](int64_t row, int n, float g, float pr) -> float {
-      const int64_t cell = row * N + n;
-      if (init) {
-        const float un = dt * ri.at(cell) * g / 2.f;
-        KW_ST(u + cell, un);
-        return un;
+    const float* pv = P.pml_sg[d];
+    auto run = [&](auto INIT, auto FORM) {
+      KW_PHASE_FENCE();
+      float av[kMaxSlots], bv[kMaxSlots];
+      auto upd = [&](int slot, int n) {
+        const uint32_t c0 = T.c0 + n, c1 = T.c1 + n;
+        float a, b;
+        if constexpr (INIT.value) {
+          a = dt * ri.at(c0) * gv[d][slot].x / 2.f;
+          b = dt * ri.at(c1) * gv[d][slot].y / 2.f;
+        } else {
+          const float p0 = pml_at(pv, T, d, 0, n), p1 = pml_at(pv, T, d, 1, n);
+          a = p0 * (p0 * KW_LD32(u, c0) - dt * ri.at(c0) * gv[d][slot].x);
+          b = p1 * (p1 * KW_LD32(u, c1) - dt * ri.at(c1) * gv[d][slot].y);
+          if constexpr (decltype(FORM)::value >= 0) {
+            a = apply_src<decltype(FORM)::value>(P.srcu[d], c0, a, t);
+            b = apply_src<decltype(FORM)::value>(P.srcu[d], c1, b, t);
+          }
+        }
+        av[slot] = a;
+        bv[slot] = b;
+      };
+      XS::last_slots(P.fx, T.g, T.G, T.active, upd);
+      if (T.active) {
+        auto put = [&](int slot, int n) {
+          KW_ST32(u, T.c1 + n, bv[slot]);
+          KW_ST32(u, T.c0 + n, av[slot]);
+          T.lds.set(d, n, cf{av[slot], T.has1 ? bv[slot] : 0.f});
+        };
+        XS::last_slots(P.fx, T.g, T.G, T.active, put);
       }
-      const float pm = d == 0 ? KW_LD(pv + n) : pr;
-      float un = pm * (pm * KW_LD(u + cell) - dt * ri.at(cell) * g);
-      if (src_on) un = apply_src(s, cell, un, t);
-      KW_ST(u + cell, un);
-      return un;
     };
-    HalfPairLoad src{&P, S, T.r0, T.r1, T.has1};
-    auto dst = [&](int n, cf v, int) {
-      const float a = upd(T.r0, n, v.x, pr0);
-      const float b = T.has1 ? upd(T.r1, n, v.y, pr1) : 0.f;
-      T.lds.set(n, cf{a, b});
-    };
-    fft<true>(c, P.fx, T.g, T.G, T.active, src, dst, T.lds);
-    if (!init) x_fwd_from_lds(c, P, T, S);
+    if (init)
+      run(BoolC<true>{}, IntC<-1>{});
+    else
+      with_src(P.srcu[d], P.srcu[d].len >= t, [&](auto FORM) { run(BoolC<false>{}, FORM); });
+  }
+  if (!init) {
+    c.sync();
+    x_fwd_from_lds<XS, DIM>(c, P, T, P.S);
   }
 }
 
-// equation of state for one cell (kspaceFirstOrder3D.m:965-1019); `ab` are
-// the two absorbing terms (tau-term input A and eta-term input B) already
-// transformed back; `sdu` the velocity divergence sum (stokes)
-KW_BODY float eos_p(const ProbDev& P, int64_t cell, float rs, float sdu, float Aterm,
+// equation of state for one cell (kspaceFirstOrder3D.m:965-1019) for an
+// equation of state and nonlinearity fixed at compile time; Aterm and Bterm
+// are the transformed-back absorbing terms, sdu the velocity divergence sum
+// (stokes)
+template <int EOS, bool NL>
+KW_BODY float eos_p(const ProbDev& P, uint32_t cell, float rs, float sdu, float Aterm,
                     float Bterm) {
   const float c0 = P.c0.at(cell);
   float inner;
-  if (P.eos == EOS_ABSORBING) {
+  if constexpr (EOS == EOS_ABSORBING) {
     inner = (rs + P.tau.at(cell) * Aterm) - P.eta.at(cell) * Bterm;
-  } else if (P.eos == EOS_STOKES) {
+  } else if constexpr (EOS == EOS_STOKES) {
     inner = rs + P.tau.at(cell) * P.rho0.at(cell) * sdu;
   } else {
     inner = rs;
   }
-  if (P.nonlinear) inner = inner + P.bona.at(cell) * (rs * rs) / (2.f * P.rho0.at(cell));
+  if constexpr (NL) inner = inner + P.bona.at(cell) * (rs * rs) / (2.f * P.rho0.at(cell));
   return (c0 * c0) * inner;
+}
+
+// run f(IntC<eos>, BoolC<nonlinear>) with both fixed at compile time
+template <class F>
+KW_BODY void with_eos(const ProbDev& P, F&& f) {
+  with_flag(P.nonlinear != 0, [&](auto NL) {
+    if (P.eos == EOS_ABSORBING)
+      f(IntC<EOS_ABSORBING>{}, NL);
+    else if (P.eos == EOS_STOKES)
+      f(IntC<EOS_STOKES>{}, NL);
+    else
+      f(IntC<EOS_LOSSLESS>{}, NL);
+  });
+}
+
+// New p of the cells of this thread's slots (pv[slot] = rows r0, r1): time
+// reversal, sensors and the optional store, each as one batched phase over
+// the slots; finally the values go to LDS pencil 0 for fft_x(p).
+template <class XS, class Ctx>
+KW_BODY void x_finish_p(Ctx& c, const ProbDev& P, const XTile& T, cf* pv, int t, bool store) {
+  const int N = P.Nx;
+  if (P.time_rev) {
+    // cells on the sensor take the boundary data of step t+1 of the reversed
+    // series (original column Nt - t) and rho_d = p / (dim c0^2)
+    int32_t iv[kMaxSlots][2];
+    float rv[kMaxSlots][2];
+    auto f = [&](int slot, int n) {
+      const uint32_t cells[2] = {T.c0 + n, T.c1 + n};
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int32_t i = KW_LD32(P.tr_map, cells[h]);
+        const float p =
+            KW_LD(P.tr_data + (int64_t)(i < 0 ? 0 : i) + (int64_t)(P.Nt - t - 1) * P.nsens);
+        const float c0 = P.c0.at(cells[h]);
+        const float c2 = c0 * c0;
+        rv[slot][h] = P.dim == 3 ? p / (3.f * c2) : (0.5f * p) / c2;
+        iv[slot][h] = i;
+        float& pp = h == 0 ? pv[slot].x : pv[slot].y;
+        pp = i >= 0 ? p : pp;
+      }
+    };
+    XS::last_slots(P.fx, T.g, T.G, T.active, f);
+    if (T.active) {
+      auto put = [&](int slot, int n) {
+        for (int h = 1; h >= 0; --h) {
+          if (iv[slot][h] < 0) continue;
+          const uint32_t cell = (h == 0 ? T.c0 : T.c1) + n;
+          for (int d = 0; d < P.dim; ++d) KW_ST32(P.rho[d], cell, rv[slot][h]);
+        }
+      };
+      XS::last_slots(P.fx, T.g, T.G, T.active, put);
+    }
+  }
+  const bool rec = !P.time_rev && t >= P.record_start;
+  const int fi = t - P.record_start + 1;
+  const uint32_t flags = P.record;
+  auto stat = [&](double* a, auto KIND) {
+    // KIND 0: max, 1: min, 2: sum of squares
+    constexpr int K = decltype(KIND)::value;
+    KW_PHASE_FENCE();
+    double n0v[kMaxSlots], n1v[kMaxSlots];
+    auto f = [&](int slot, int n) {
+      const double v0 = (double)pv[slot].x, v1 = (double)pv[slot].y;
+      const double o0 = KW_LD32(a, T.c0 + n), o1 = KW_LD32(a, T.c1 + n);
+      if constexpr (K == 2) {
+        n0v[slot] = (fi == 1 ? 0.0 : o0) + v0 * v0;
+        n1v[slot] = (fi == 1 ? 0.0 : o1) + v1 * v1;
+      } else if constexpr (K == 0) {
+        n0v[slot] = fi == 1 ? v0 : (v0 > o0 ? v0 : o0);
+        n1v[slot] = fi == 1 ? v1 : (v1 > o1 ? v1 : o1);
+      } else {
+        n0v[slot] = fi == 1 ? v0 : (v0 < o0 ? v0 : o0);
+        n1v[slot] = fi == 1 ? v1 : (v1 < o1 ? v1 : o1);
+      }
+    };
+    XS::last_slots(P.fx, T.g, T.G, T.active, f);
+    if (T.active) {
+      auto put = [&](int slot, int n) {
+        KW_ST32(a, T.c1 + n, n1v[slot]);
+        KW_ST32(a, T.c0 + n, n0v[slot]);
+      };
+      XS::last_slots(P.fx, T.g, T.G, T.active, put);
+    }
+  };
+  if (rec) {
+    if (P.full_mask) {
+      // sensor mask == whole grid: point statistics fused here
+      if ((flags & REC_P) && T.active) {
+        auto f = [&](int slot, int n) {
+          const int64_t off = (int64_t)(fi - 1) * P.ncell;
+          KW_ST(P.p_ts + off + (int64_t)T.c1 + n, pv[slot].y);
+          KW_ST(P.p_ts + off + (int64_t)T.c0 + n, pv[slot].x);
+        };
+        XS::last_slots(P.fx, T.g, T.G, T.active, f);
+      }
+      if (flags & REC_P_MAX) stat(P.p_max, IntC<0>{});
+      if (flags & REC_P_MIN) stat(P.p_min, IntC<1>{});
+      if (flags & REC_P_RMS) stat(P.p_rms2, IntC<2>{});
+    }
+    if (flags & REC_P_MAX_ALL) stat(P.p_max_all, IntC<0>{});
+    if (flags & REC_P_MIN_ALL) stat(P.p_min_all, IntC<1>{});
+  }
+  if (T.active) {
+    auto fin = [&](int slot, int n) {
+      if (store) {
+        KW_ST32(P.p, T.c1 + n, pv[slot].y);
+        KW_ST32(P.p, T.c0 + n, pv[slot].x);
+      }
+      T.lds.set(0, n, cf{pv[slot].x, T.has1 ? pv[slot].y : 0.f});
+    };
+    XS::last_slots(P.fx, T.g, T.G, T.active, fin);
+  }
+  c.sync();
 }
 
 // end of the divergence: rho_d update (:912-927), pressure source
 // (:930-962); then either the lossless/stokes EOS with sensors and fft_x(p)
 // into S[0], or (absorbing) S[0] = fft_x(rho0 * sum du), S[1] = fft_x(sum rho)
 // and rsum = sum rho
-template <class Ctx>
+template <class XS, int DIM, class Ctx>
 KW_BODY void body_x_div(Ctx& c, const Batch& B, const PassArgs& A) {
   int tile;
   const ProbDev& P = batch_prob(c, B, tile);
@@ -458,85 +611,107 @@ KW_BODY void body_x_div(Ctx& c, const Batch& B, const PassArgs& A) {
   const float dt = P.dt;
   const bool psrc = P.srcp.len >= t;
   const bool store_p = A.flags & X_STORE_P;
-  cf sdu[kMaxSlots], srho[kMaxSlots], rpr[kMaxSlots];
+  cf gv[DIM][kMaxSlots];
+  HalfPairLoad<DIM> src{&P, {}, T.h0, T.h1, T.q0, T.q1, T.has1};
 #pragma unroll
-  for (int i = 0; i < kMaxSlots; ++i) {
-    sdu[i] = cf{0.f, 0.f};
-    srho[i] = cf{0.f, 0.f};
-    rpr[i] = cf{0.f, 0.f};
-  }
-  for (int d = 0; d < P.dim; ++d) {
-    cf* S = P.S[d];
+  for (int d = 0; d < DIM; ++d) src.S[d] = P.S[d];
+  auto dst = [&](int d, int, cf v, int slot) { gv[d][slot] = v; };
+  XS::template fft<DIM, true>(c, P.fx, T.g, T.G, T.active, src, dst, T.lds);
+
+  cf sdu[kMaxSlots], srho[kMaxSlots], coef[kMaxSlots];
+  // mass conservation coefficient: rho0 (linear) or 2 sum(rho) + rho0
+  // (nonlinear, from the densities before this update)
+  with_flag(P.nonlinear != 0, [&](auto NL) {
+    auto f = [&](int slot, int n) {
+      const uint32_t c0 = T.c0 + n, c1 = T.c1 + n;
+      sdu[slot] = cf{0.f, 0.f};
+      srho[slot] = cf{0.f, 0.f};
+      float a = P.rho0.at(c0), b = P.rho0.at(c1);
+      if constexpr (NL.value) {
+        float s0 = KW_LD32(P.rho[0], c0) + KW_LD32(P.rho[1], c0);
+        float s1 = KW_LD32(P.rho[0], c1) + KW_LD32(P.rho[1], c1);
+        if (DIM == 3) {
+          s0 = s0 + KW_LD32(P.rho[DIM - 1], c0);
+          s1 = s1 + KW_LD32(P.rho[DIM - 1], c1);
+        }
+        a = 2.f * s0 + a;
+        b = 2.f * s1 + b;
+      }
+      coef[slot] = cf{a, b};
+    };
+    XS::last_slots(P.fx, T.g, T.G, T.active, f);
+  });
+#pragma unroll
+  for (int d = 0; d < DIM; ++d) {
     float* rho = P.rho[d];
     const float* pv = P.pml[d];
-    float pr0 = 1.f, pr1 = 1.f;
-    if (d > 0 && T.active) {
-      pr0 = pml_row(pv, P, d, T.r0);
-      if (T.has1) pr1 = pml_row(pv, P, d, T.r1);
-    }
-    auto upd = [&](int64_t row, int n, float g, float pr, float& acc_du, float& acc_r,
-                   float& rp) -> void {
-      const int64_t cell = row * N + n;
-      if (d == 0 && P.nonlinear) {
-        float s = KW_LD(P.rho[0] + cell) + KW_LD(P.rho[1] + cell);
-        if (P.dim == 3) s = s + KW_LD(P.rho[2] + cell);
-        rp = 2.f * s + P.rho0.at(cell);
+    with_src(P.srcp, psrc, [&](auto FORM) {
+      KW_PHASE_FENCE();
+      float av[kMaxSlots], bv[kMaxSlots];
+      auto f = [&](int slot, int n) {
+        const uint32_t c0 = T.c0 + n, c1 = T.c1 + n;
+        const float p0 = pml_at(pv, T, d, 0, n), p1 = pml_at(pv, T, d, 1, n);
+        const cf g = gv[d][slot];
+        float a = p0 * (p0 * KW_LD32(rho, c0) - dt * coef[slot].x * g.x);
+        float b = p1 * (p1 * KW_LD32(rho, c1) - dt * coef[slot].y * g.y);
+        if constexpr (decltype(FORM)::value >= 0) {
+          a = apply_src<decltype(FORM)::value>(P.srcp, c0, a, t);
+          b = apply_src<decltype(FORM)::value>(P.srcp, c1, b, t);
+        }
+        av[slot] = a;
+        bv[slot] = b;
+        sdu[slot] = cf{sdu[slot].x + g.x, sdu[slot].y + g.y};
+        srho[slot] = cf{srho[slot].x + a, srho[slot].y + b};
+      };
+      XS::last_slots(P.fx, T.g, T.G, T.active, f);
+      if (T.active) {
+        auto put = [&](int slot, int n) {
+          KW_ST32(rho, T.c1 + n, bv[slot]);
+          KW_ST32(rho, T.c0 + n, av[slot]);
+        };
+        XS::last_slots(P.fx, T.g, T.G, T.active, put);
       }
-      const float pm = d == 0 ? KW_LD(pv + n) : pr;
-      const float coef = P.nonlinear ? rp : P.rho0.at(cell);
-      float rn = pm * (pm * KW_LD(rho + cell) - dt * coef * g);
-      if (psrc) rn = apply_src(P.srcp, cell, rn, t);
-      KW_ST(rho + cell, rn);
-      acc_du = acc_du + g;
-      acc_r = acc_r + rn;
-    };
-    HalfPairLoad src{&P, S, T.r0, T.r1, T.has1};
-    auto dst = [&](int n, cf v, int slot) {
-      upd(T.r0, n, v.x, pr0, sdu[slot].x, srho[slot].x, rpr[slot].x);
-      if (T.has1) upd(T.r1, n, v.y, pr1, sdu[slot].y, srho[slot].y, rpr[slot].y);
-    };
-    fft<true>(c, P.fx, T.g, T.G, T.active, src, dst, T.lds);
+    });
   }
   if (P.eos == EOS_ABSORBING) {
-    auto put_a = [&](int slot, int n) {
-      const float a = P.rho0.at(T.r0 * N + n) * sdu[slot].x;
-      const float b = T.has1 ? P.rho0.at(T.r1 * N + n) * sdu[slot].y : 0.f;
-      T.lds.set(n, cf{a, b});
+    KW_PHASE_FENCE();
+    float av[kMaxSlots], bv[kMaxSlots];
+    auto ab = [&](int slot, int n) {
+      av[slot] = P.rho0.at(T.c0 + n) * sdu[slot].x;
+      bv[slot] = P.rho0.at(T.c1 + n) * sdu[slot].y;
     };
-    last_slots(P.fx, T.g, T.G, T.active, put_a);
+    XS::last_slots(P.fx, T.g, T.G, T.active, ab);
+    if (T.active) {
+      auto put_ab = [&](int slot, int n) {
+        KW_ST32(P.rsum, T.c1 + n, srho[slot].y);
+        KW_ST32(P.rsum, T.c0 + n, srho[slot].x);
+        T.lds.set(0, n, cf{av[slot], T.has1 ? bv[slot] : 0.f});
+        T.lds.set(1, n, cf{srho[slot].x, T.has1 ? srho[slot].y : 0.f});
+      };
+      XS::last_slots(P.fx, T.g, T.G, T.active, put_ab);
+    }
     c.sync();
-    x_fwd_from_lds(c, P, T, P.S[0]);
-    auto put_b = [&](int slot, int n) {
-      KW_ST(P.rsum + T.r0 * N + n, srho[slot].x);
-      if (T.has1) KW_ST(P.rsum + T.r1 * N + n, srho[slot].y);
-      T.lds.set(n, srho[slot]);
-    };
-    last_slots(P.fx, T.g, T.G, T.active, put_b);
-    c.sync();
-    x_fwd_from_lds(c, P, T, P.S[1]);
+    x_fwd_from_lds<XS, 2>(c, P, T, P.S);
   } else {
-    auto put_p = [&](int slot, int n) {
-      const int64_t c0 = T.r0 * N + n;
-      float a = eos_p(P, c0, srho[slot].x, sdu[slot].x, 0.f, 0.f);
-      a = finish_p(P, c0, a, t, store_p);
-      float b = 0.f;
-      if (T.has1) {
-        const int64_t c1 = T.r1 * N + n;
-        b = eos_p(P, c1, srho[slot].y, sdu[slot].y, 0.f, 0.f);
-        b = finish_p(P, c1, b, t, store_p);
-      }
-      T.lds.set(n, cf{a, b});
-    };
-    last_slots(P.fx, T.g, T.G, T.active, put_p);
-    c.sync();
-    x_fwd_from_lds(c, P, T, P.S[0]);
+    cf pv[kMaxSlots];
+    with_eos(P, [&](auto EOS, auto NL) {
+      auto f = [&](int slot, int n) {
+        constexpr int E = decltype(EOS)::value;
+        constexpr bool L = decltype(NL)::value;
+        pv[slot].x = eos_p<E, L>(P, T.c0 + n, srho[slot].x, sdu[slot].x, 0.f, 0.f);
+        pv[slot].y = eos_p<E, L>(P, T.c1 + n, srho[slot].y, sdu[slot].y, 0.f, 0.f);
+      };
+      XS::last_slots(P.fx, T.g, T.G, T.active, f);
+    });
+    x_finish_p<XS>(c, P, T, pv, t, store_p);
+    x_fwd_from_lds<XS, 1>(c, P, T, P.S);
   }
 }
 
 // end of the absorbing EOS: p = c0^2 (rsum + tau A - eta B [+ BonA term])
 // with A = ifft(nabla1 fft(rho0 sum du)), B = ifft(nabla2 fft(rsum)); sensors;
 // S[0] = fft_x(p)
-template <class Ctx>
+template <class XS, class Ctx>
 KW_BODY void body_x_eos(Ctx& c, const Batch& B, const PassArgs& A) {
   int tile;
   const ProbDev& P = batch_prob(c, B, tile);
@@ -544,35 +719,28 @@ KW_BODY void body_x_eos(Ctx& c, const Batch& B, const PassArgs& A) {
   const int N = P.Nx;
   const int t = KW_LD(P.t_index);
   const bool store_p = A.flags & X_STORE_P;
-  cf ta[kMaxSlots];
-#pragma unroll
-  for (int i = 0; i < kMaxSlots; ++i) ta[i] = cf{0.f, 0.f};
-  {
-    HalfPairLoad src{&P, P.S[0], T.r0, T.r1, T.has1};
-    auto dst = [&](int n, cf v, int slot) { ta[slot] = v; };
-    fft<true>(c, P.fx, T.g, T.G, T.active, src, dst, T.lds);
-  }
-  {
-    HalfPairLoad src{&P, P.S[1], T.r0, T.r1, T.has1};
-    auto dst = [&](int n, cf v, int slot) {
-      const int64_t c0 = T.r0 * N + n;
-      float a = eos_p(P, c0, KW_LD(P.rsum + c0), 0.f, ta[slot].x, v.x);
-      a = finish_p(P, c0, a, t, store_p);
-      float b = 0.f;
-      if (T.has1) {
-        const int64_t c1 = T.r1 * N + n;
-        b = eos_p(P, c1, KW_LD(P.rsum + c1), 0.f, ta[slot].y, v.y);
-        b = finish_p(P, c1, b, t, store_p);
-      }
-      T.lds.set(n, cf{a, b});
+  cf gv[2][kMaxSlots];
+  HalfPairLoad<2> src{&P, {P.S[0], P.S[1]}, T.h0, T.h1, T.q0, T.q1, T.has1};
+  auto dst = [&](int j, int, cf v, int slot) { gv[j][slot] = v; };
+  XS::template fft<2, true>(c, P.fx, T.g, T.G, T.active, src, dst, T.lds);
+  cf pv[kMaxSlots];
+  with_flag(P.nonlinear != 0, [&](auto NL) {
+    auto f = [&](int slot, int n) {
+      constexpr bool L = decltype(NL)::value;
+      const uint32_t c0 = T.c0 + n, c1 = T.c1 + n;
+      pv[slot].x = eos_p<EOS_ABSORBING, L>(P, c0, KW_LD32(P.rsum, c0), 0.f, gv[0][slot].x,
+                                           gv[1][slot].x);
+      pv[slot].y = eos_p<EOS_ABSORBING, L>(P, c1, KW_LD32(P.rsum, c1), 0.f, gv[0][slot].y,
+                                           gv[1][slot].y);
     };
-    fft<true>(c, P.fx, T.g, T.G, T.active, src, dst, T.lds);
-  }
-  x_fwd_from_lds(c, P, T, P.S[0]);
+    XS::last_slots(P.fx, T.g, T.G, T.active, f);
+  });
+  x_finish_p<XS>(c, P, T, pv, t, store_p);
+  x_fwd_from_lds<XS, 1>(c, P, T, P.S);
 }
 
 // ---------------------------------------------------------------------------
-// elementwise kernels (block of EW_T threads, EW_PER cells per thread)
+// elementwise kernels (block of EW_T threads, one cell per thread)
 // ---------------------------------------------------------------------------
 constexpr int EW_T = 256;
 

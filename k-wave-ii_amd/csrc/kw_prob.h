@@ -40,11 +40,30 @@ enum : uint32_t {
 enum : int32_t { EOS_LOSSLESS = 0, EOS_ABSORBING = 1, EOS_STOKES = 2 };
 enum : int32_t { SRC_DIRICHLET = 0, SRC_ADD_NO_CORR = 1, SRC_ADDITIVE = 2 };
 
-// a grid quantity that is either a full array or a scalar
+// a grid quantity that is either a full array (mask = all ones) or a
+// scalar held in a one-element array (mask = 0); the load is unconditional
 struct Field {
   const float* a;
-  float s;
-  KW_BODY float at(int64_t i) const { return a ? KW_LD(a + i) : s; }
+  uint32_t mask;
+  KW_BODY float at(uint32_t i) const { return KW_LD32(a, i & mask); }
+  KW_BODY bool grid() const { return mask != 0; }
+};
+
+template <bool B>
+struct BoolC {
+  static constexpr bool value = B;
+};
+// run f(BoolC<b>) with the flag as a compile-time constant
+template <class F>
+KW_BODY void with_flag(bool b, F&& f) {
+  if (b)
+    f(BoolC<true>{});
+  else
+    f(BoolC<false>{});
+}
+template <int V>
+struct IntC {
+  static constexpr int value = V;
 };
 
 // a time-varying source on a set of grid points
@@ -54,13 +73,15 @@ struct TimeSource {
   int32_t nseries;    // rows of the signal matrix
   int32_t npts;       // source points
   const float* sig;   // [nseries x len], (i, t) at i + t*nseries, already scaled
-  const int32_t* row; // per point: signal row (labelled / many), null = row 0
+  const int32_t* row; // per point: signal row; one entry 0 with rowmask 0 for
+                      // a single series
+  int32_t rowmask;
   const int32_t* map; // per cell: source point index or -1 (dirichlet / no-correction)
   const float* km;    // additive, single series: real(ifftn(source_kappa.*fftn(mask)))
   const float* grid;  // additive, several series: per-step filtered source grid
   KW_BODY float value(int32_t pt, int32_t t0) const {
-    const int32_t r = row ? KW_LD(row + pt) : 0;
-    return KW_LD(sig + (int64_t)r + (int64_t)t0 * nseries);
+    const int32_t r = KW_LD32(row, (uint32_t)(pt & rowmask));
+    return KW_LD32(sig, (uint32_t)r + (uint32_t)t0 * (uint32_t)nseries);
   }
 };
 

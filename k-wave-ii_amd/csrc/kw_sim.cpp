@@ -15,6 +15,7 @@
 #include <vector>
 
 #include "kw_backend.h"
+#include "kw_variants.h"
 #include "kwhip.h"
 
 namespace kw {
@@ -210,13 +211,23 @@ struct Sim {
     ev_bytes.push_back(nbytes);
   }
 
+  // compile-time FFT length of the kernel variant for a pass (0: runtime)
+  int fixed_len(PassKind k) const {
+    if (std::getenv("KW_NO_FIXED")) return 0;
+    const int N = (k == PK_YFWD || k == PK_YINV) ? G.Ny
+                  : (k == PK_SPEC)               ? (G.dim == 3 ? G.Nz : G.Ny)
+                                                 : G.Nx;
+    return KW_IS_FIXED_SIZE(N) ? N : 0;
+  }
+
   void pass(PassKind k, const PassArgs& a) {
     const LaunchGeom* g = &lx;
     if (k == PK_YFWD || k == PK_YINV) g = &ly;
     if (k == PK_SPEC) g = &ll;
     const Batch B = batch();
+    const int fx = fixed_len(k);
     timed((int)k, timing ? pass_bytes(k, a) : 0.0,
-          [&]() { launch_pass(k, g->nb, g->T, g->lds, B, a, stream); });
+          [&]() { launch_pass(k, fx, G.dim, g->nb, g->T, g->lds, B, a, stream); });
   }
 
   void ew(const EwArgs& a, int nblocks) {
@@ -225,7 +236,7 @@ struct Sim {
   }
 
   // -- compulsory bytes of one launch: every array it reads or writes, once
-  double fb(const Field& f) const { return f.a ? 4.0 * G.ncell : 0.0; }
+  double fb(const Field& f) const { return f.grid() ? 4.0 * G.ncell : 0.0; }
   double src_bytes(const TimeSource& s, int t) const {
     if (s.len < t) return 0.0;
     if (s.mode == SRC_ADDITIVE) return 4.0 * G.ncell;  // K_M or the filtered grid
@@ -310,24 +321,25 @@ struct Sim {
     const size_t maxl = dev_max_lds();
     const int capP = max_pencils_env();
     auto Gof = [](int N) { return (N + KW_E - 1) / KW_E; };
-    // x pass: <= 256 threads, LDS P*(Nx+1) complex
+    // x pass: <= 256 threads, LDS P * kNPX padded pencils
     {
       const int Gx = Gof(G.Nx);
+      const size_t pencil = sizeof(cf) * (size_t)kNPX * x_lds_stride(G.Nx);
       int P = std::max(1, 256 / Gx);
-      P = std::min<int64_t>(P, (int64_t)(maxl / (sizeof(cf) * (G.Nx + 1))));
+      P = std::min<int64_t>(P, (int64_t)(maxl / pencil));
       P = std::max(1, std::min(P, capP));
       this->P.Gx = Gx;
       this->P.Px = P;
       const int64_t npairs = (G.nrows + 1) / 2;
       lx.nb = (int)((npairs + P - 1) / P);
       lx.T = P * Gx;
-      lx.lds = sizeof(cf) * (size_t)P * (G.Nx + 1);
+      lx.lds = pencil * (size_t)P;
       if (Gx > 256 || lx.lds > maxl) fail(KW_EUNSUP, "grid size Nx too large for the x pass");
     }
-    // strided y pass (3D): <= 1024 threads, LDS P*(Ny+1)
+    // strided y pass (3D): <= 512 threads, LDS P*(Ny+1)
     if (G.dim == 3) {
       const int Gy = Gof(G.Ny);
-      int P = std::max(1, std::min(16, 1024 / Gy));
+      int P = std::max(1, std::min(16, 512 / Gy));
       P = std::min<int64_t>(P, (int64_t)(maxl / (sizeof(cf) * (G.Ny + 1))));
       P = std::max(1, std::min(P, capP));
       this->P.Gy = Gy;
@@ -336,13 +348,13 @@ struct Sim {
       ly.nb = (int)((nH + P - 1) / P + (G.nyq ? (G.Nz + P - 1) / P : 0));
       ly.T = P * Gy;
       ly.lds = sizeof(cf) * (size_t)P * (G.Ny + 1);
-      if (Gy > 1024 || ly.lds > maxl) fail(KW_EUNSUP, "grid size Ny too large for the y pass");
+      if (Gy > 512 || ly.lds > maxl) fail(KW_EUNSUP, "grid size Ny too large for the y pass");
     }
-    // last-axis pass: <= 1024 threads, LDS 2*P*N
+    // last-axis pass: <= 512 threads, LDS 2*P*N
     {
       const int Nl = G.dim == 3 ? G.Nz : G.Ny;
       const int Gl = Gof(Nl);
-      int P = std::max(1, std::min(16, 1024 / Gl));
+      int P = std::max(1, std::min(16, 512 / Gl));
       P = std::min<int64_t>(P, (int64_t)(maxl / (2 * sizeof(cf) * Nl)));
       P = std::max(1, std::min(P, capP));
       this->P.Gl = Gl;
@@ -352,7 +364,7 @@ struct Sim {
       ll.nb = (int)((nH + P - 1) / P + (G.nyq ? (nQ + P - 1) / P : 0));
       ll.T = P * Gl;
       ll.lds = 2 * sizeof(cf) * (size_t)P * Nl;
-      if (Gl > 1024 || ll.lds > maxl) fail(KW_EUNSUP, "grid size too large for the last-axis pass");
+      if (Gl > 512 || ll.lds > maxl) fail(KW_EUNSUP, "grid size too large for the last-axis pass");
     }
     lcell.nb = (int)((G.ncell + EW_T - 1) / EW_T);
     lsens.nb = (int)((std::max<int64_t>(nsens, 1) + EW_T - 1) / EW_T);
@@ -410,16 +422,13 @@ static std::vector<float> to_float(const double* a, int64_t n) {
   return v;
 }
 
+static Field scalar_field(Sim& S, float v) {
+  return Field{S.upload(std::vector<float>{v}), 0u};
+}
+
 static Field make_field(Sim& S, const double* a, double s, int64_t n) {
-  Field f{};
-  if (a) {
-    f.a = S.upload(to_float(a, n));
-    f.s = 0.f;
-  } else {
-    f.a = nullptr;
-    f.s = (float)s;
-  }
-  return f;
+  if (a) return Field{S.upload(to_float(a, n)), ~0u};
+  return scalar_field(S, (float)s);
 }
 
 static void check_index(const uint64_t* idx, int64_t n, int64_t ncell, const char* what) {
@@ -459,6 +468,10 @@ static void setup_source(Sim& S, const kw_time_source& src, TimeSource& ts, cons
     for (int64_t i = 0; i < src.num_points; ++i)
       row[i] = labelled ? (int32_t)(src.sig_index[i] - 1) : (int32_t)i;
     ts.row = S.upload(row);
+    ts.rowmask = -1;
+  } else {
+    ts.row = S.upload(std::vector<int32_t>{0});
+    ts.rowmask = 0;
   }
   const bool need_map = src.mode != SRC_ADDITIVE || src.num_series > 1;
   if (need_map) {
@@ -501,6 +514,12 @@ static std::unique_ptr<Sim> create_sim(const kw_problem* pr) {
     fail(KW_EINVAL, "dt and grid spacings must be positive");
   if (!(pr->c_ref > 0)) fail(KW_EINVAL, "c_ref must be positive");
 
+  {
+    // kernels address every array with 32-bit byte offsets
+    const Geometry g = make_geometry(pr->dim, pr->Nx, pr->Ny, pr->Nz);
+    if ((double)g.nspec * 8.0 >= 4294967296.0 || (double)g.ncell * 8.0 >= 4294967296.0)
+      fail(KW_EUNSUP, "grid too large: every array must stay below 4 GiB (at most 2^29 cells)");
+  }
   auto S = std::make_unique<Sim>();
   S->stream = dev_stream_create();
   S->G = make_geometry(pr->dim, pr->Nx, pr->Ny, pr->Nz);
@@ -579,13 +598,16 @@ static std::unique_ptr<Sim> create_sim(const kw_problem* pr) {
     if (r) {
       std::vector<float> inv(n);
       parallel_for(n, [&](int64_t i) { inv[i] = (float)(1.0 / r[i]); });
-      P.rinv[a] = Field{S->upload(inv), 0.f};
+      P.rinv[a] = Field{S->upload(inv), ~0u};
     } else {
-      P.rinv[a] = Field{nullptr, (float)(1.0 / pr->rho0_scalar)};
+      P.rinv[a] = scalar_field(*S, (float)(1.0 / pr->rho0_scalar));
     }
   }
+  for (int a = dim; a < 3; ++a) P.rinv[a] = P.rinv[0];
   P.nonlinear = pr->nonlinear_flag ? 1 : 0;
-  if (P.nonlinear) P.bona = make_field(*S, pr->BonA, pr->BonA_scalar, n);
+  P.bona = P.nonlinear ? make_field(*S, pr->BonA, pr->BonA_scalar, n) : scalar_field(*S, 0.f);
+  P.tau = scalar_field(*S, 0.f);
+  P.eta = P.tau;
   P.eos = pr->absorbing_flag;
   if (P.eos < 0 || P.eos > 2) fail(KW_EINVAL, "absorbing_flag must be 0, 1 or 2");
   const double y = P.eos == EOS_STOKES ? 2.0 : pr->alpha_power;
@@ -619,11 +641,11 @@ static std::unique_ptr<Sim> create_sim(const kw_problem* pr) {
         tv[i] = (float)tau_of(a, c);
         ev[i] = (float)eta_of(a, c);
       });
-      P.tau = Field{S->upload(tv), 0.f};
-      if (P.eos == EOS_ABSORBING) P.eta = Field{S->upload(ev), 0.f};
+      P.tau = Field{S->upload(tv), ~0u};
+      if (P.eos == EOS_ABSORBING) P.eta = Field{S->upload(ev), ~0u};
     } else {
-      P.tau = Field{nullptr, (float)tau_of(pr->alpha_coeff_scalar, pr->c0_scalar)};
-      P.eta = Field{nullptr, (float)eta_of(pr->alpha_coeff_scalar, pr->c0_scalar)};
+      P.tau = scalar_field(*S, (float)tau_of(pr->alpha_coeff_scalar, pr->c0_scalar));
+      P.eta = scalar_field(*S, (float)eta_of(pr->alpha_coeff_scalar, pr->c0_scalar));
     }
     if (pr->alpha_coeff)
       for (int64_t i = 0; i < n; ++i)
