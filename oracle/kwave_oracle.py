@@ -185,6 +185,15 @@ def get_pml(N, d, dt, c, pml_size, pml_alpha, staggered):
     return pml
 
 
+def shift_derivative_ops(k_vec, d, use_sg=True):
+    """kspaceFirstOrder3D.m:629-642: ifftshift(i k exp(+-i k d/2)) (the
+    staggered-grid shift folded into the derivative), or ifftshift(i k)."""
+    if use_sg:
+        return (np.fft.ifftshift(1j * k_vec * np.exp(1j * k_vec * d / 2)),
+                np.fft.ifftshift(1j * k_vec * np.exp(-1j * k_vec * d / 2)))
+    return np.fft.ifftshift(1j * k_vec), np.fft.ifftshift(1j * k_vec)
+
+
 def cosine_series(n, N, coeffs):
     """getWin.m:417-424 (cosineSeries)."""
     s = coeffs[0] * np.ones_like(n, dtype=np.float64)
@@ -595,13 +604,9 @@ def kspace_first_order(kgrid: KGrid, medium, source, sensor, *, data_cast="off",
     kv = kgrid.k_vecs()
     ddk_pos, ddk_neg = [], []
     for a in range(dim):
-        d = kgrid.spacings[a]
-        if use_sg:
-            ddk_pos.append(np.fft.ifftshift(1j * kv[a] * np.exp(1j * kv[a] * d / 2)))
-            ddk_neg.append(np.fft.ifftshift(1j * kv[a] * np.exp(-1j * kv[a] * d / 2)))
-        else:
-            ddk_pos.append(np.fft.ifftshift(1j * kv[a]))
-            ddk_neg.append(np.fft.ifftshift(1j * kv[a]))
+        dp, dn = shift_derivative_ops(kv[a], kgrid.spacings[a], use_sg)
+        ddk_pos.append(dp)
+        ddk_neg.append(dn)
     additive = ((src_p is not None and src_p.mode == "additive") or
                 (src_u is not None and src_u.mode == "additive"))
     if use_kspace:

@@ -1,0 +1,149 @@
+"""Pins the oracle's operators against closed forms and library routines
+(CPU only).
+
+Each check is chosen so that a plausible mistake fails it: a wrong sign of
+the staggered shift, a transposed window coefficient, an off-by-one in the
+PML profile, MATLAB's round-half-away-from-zero, the Nyquist bin.
+Mirrors +kwave/+tests/+unit/TestFourierCollocation.m:47-302 (spectral
+derivatives of sinusoids with +-dx/2 staggering, sinc) and
+TestSplitFieldPML.m:74-107 (PML is identity inside the domain).
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import kwave_oracle as ko
+
+
+@pytest.mark.parametrize("N", [8, 9, 16, 27, 64, 75])
+def test_make_dim_matches_fftfreq(N):
+    dx = 1e-4
+    k = ko.make_dim(N, dx)
+    ref = np.fft.fftshift(np.fft.fftfreq(N, d=dx)) * 2 * math.pi
+    assert np.allclose(k, ref, rtol=0, atol=1e-9 * np.max(np.abs(ref)))
+    assert k[N // 2] == 0.0
+
+
+def test_sinc_closed_form():
+    x = np.array([0.0, 0.1, 1.0, math.pi, 2.5, -3.0])
+    got = ko.sinc(x)
+    assert got[0] == pytest.approx(1.0, abs=1e-15)
+    assert got[3] == pytest.approx(0.0, abs=1e-15)
+    assert np.allclose(got[1:], np.sinc(x[1:] / math.pi), rtol=1e-14, atol=1e-15)
+
+
+def test_db2neper_definition():
+    # alpha0 dB/(MHz^y cm) at f MHz is alpha0 f^y dB/cm = alpha0 f^y 100/(20 log10 e) Np/m,
+    # and the Np coefficient multiplies omega^y
+    for a0, y, f in ((0.75, 1.5, 1.0), (0.5, 1.3, 2.5), (2.0, 2.0, 0.3)):
+        a_np = ko.db2neper(a0, y)
+        np_per_m = a0 * f ** y * 100 / (20 * math.log10(math.e))
+        assert a_np * (2 * math.pi * f * 1e6) ** y == pytest.approx(np_per_m, rel=1e-12)
+
+
+@pytest.mark.parametrize("staggered", [False, True])
+def test_pml_profile(staggered):
+    N, n, dx, dt, c, a = 64, 10, 1e-4, 2e-8, 1500.0, 2.0
+    pml = ko.get_pml(N, dx, dt, c, n, a, staggered)
+    assert np.all(pml[n:N - n] == 1.0)            # identity inside
+    assert np.all(pml[:n] < 1.0) and np.all(pml[N - n:] < 1.0)
+    # quartic profile: left x=1 -> full strength; right x=n -> full strength
+    off = 0.5 if staggered else 0.0
+    x = np.arange(1, n + 1) + off
+    right = np.exp(-a * (c / dx) * (x / n) ** 4 * dt / 2)
+    left = np.exp(-a * (c / dx) * ((x - n - 1) / -n) ** 4 * dt / 2)
+    assert np.allclose(pml[N - n:], right, rtol=1e-14)
+    assert np.allclose(pml[:n], left, rtol=1e-14)
+    if not staggered:
+        assert np.allclose(pml[:n], pml[N - n:][::-1], rtol=1e-14)   # symmetric
+    assert np.all(np.diff(pml[:n]) > 0)             # absorption grows towards the edge
+
+
+@pytest.mark.parametrize("N", [8, 17, 64])
+def test_blackman_coefficients(N):
+    assert np.allclose(ko.blackman_1d(N), np.blackman(N), rtol=0, atol=1e-14)
+
+
+@pytest.mark.parametrize("shape", [(32, 32), (31, 33), (16, 18, 15)])
+def test_smooth_preserves_constants_and_restores_max(shape):
+    c = np.full(shape, 3.5)
+    assert np.allclose(ko.smooth_mat(c), 3.5, rtol=1e-12)
+    rng = np.random.default_rng(0)
+    f = rng.standard_normal(shape)
+    sm = ko.smooth_mat(f, True)
+    assert np.max(np.abs(sm)) == pytest.approx(np.max(np.abs(f)), rel=1e-12)
+    # a low-pass filter: the high-frequency energy drops
+    F, S = np.abs(np.fft.fftn(f)), np.abs(np.fft.fftn(ko.smooth_mat(f)))
+    assert S.ravel()[0] == pytest.approx(F.ravel()[0], rel=1e-12)
+    assert np.sum(S ** 2) < 0.6 * np.sum(F ** 2)
+
+
+@pytest.mark.parametrize("N", [32, 33])
+@pytest.mark.parametrize("use_sg", [True, False])
+def test_spectral_derivative_of_sinusoid(N, use_sg):
+    """D+ gives the derivative at x + dx/2, D- at x - dx/2 (TestFourierCollocation)."""
+    dx = 1e-3
+    L = N * dx
+    k = ko.make_dim(N, dx)
+    x = np.arange(N) * dx
+    for m in (1, 3, 7):
+        f = np.sin(2 * math.pi * m * x / L)
+        dp, dn = ko.shift_derivative_ops(k, dx, use_sg)
+        gp = np.real(np.fft.ifft(dp * np.fft.fft(f)))
+        gn = np.real(np.fft.ifft(dn * np.fft.fft(f)))
+        w = 2 * math.pi * m / L
+        sh = dx / 2 if use_sg else 0.0
+        assert np.allclose(gp, w * np.cos(w * (x + sh)), atol=1e-9 * w)
+        assert np.allclose(gn, w * np.cos(w * (x - sh)), atol=1e-9 * w)
+
+
+def test_derivative_of_nyquist_mode_even_grid():
+    """The Nyquist mode of an even grid: D+ applied to (-1)^n is real-valued
+    (Hermitian symmetry kept), and D+ then D- equals -k_nyq^2."""
+    N, dx = 16, 1e-3
+    k = ko.make_dim(N, dx)
+    dp, dn = ko.shift_derivative_ops(k, dx, True)
+    f = (-1.0) ** np.arange(N)
+    g = np.fft.ifft(dp * np.fft.fft(f))
+    assert np.max(np.abs(np.imag(g))) < 1e-10 * np.max(np.abs(g))
+    h = np.real(np.fft.ifft(dn * np.fft.fft(np.real(g))))
+    assert np.allclose(h, -(math.pi / dx) ** 2 * f, rtol=1e-12)
+
+
+def test_brute_force_dft_derivative_tiny_grid():
+    """The FFT-based operator equals an explicit DFT-matrix evaluation."""
+    N, dx = 8, 0.5e-3
+    k = ko.make_dim(N, dx)
+    dp, _ = ko.shift_derivative_ops(k, dx, True)
+    F = np.exp(-2j * math.pi * np.outer(np.arange(N), np.arange(N)) / N)
+    Fi = np.conj(F) / N
+    D = Fi @ np.diag(dp) @ F
+    rng = np.random.default_rng(1)
+    f = rng.standard_normal(N)
+    assert np.allclose(np.real(D @ f), np.real(np.fft.ifft(dp * np.fft.fft(f))), atol=1e-9)
+
+
+def test_cart2grid_rounding_and_duplicates():
+    kg = ko.KGrid(10, 1.0, 10, 1.0)
+    # grid index = round(x/dx) + floor(N/2) + 1 with MATLAB round (half away from 0)
+    pts = np.array([[0.0, 0.5, -0.5, 2.4, -2.5, 0.49],
+                    [0.0, 0.0, 0.0, 1.0, 1.0, 0.0]])
+    mask, order, reorder = ko.cart2grid(kg, pts)
+    want = {(5, 5): 6, (6, 5): 2, (4, 5): 3, (7, 6): 4, (2, 6): 5}   # 0-based; point 1 overwritten by 6
+    got = {tuple(int(v) for v in ix): None for ix in np.argwhere(mask)}
+    assert set(got) == set(want)
+    flat = np.zeros((10, 10), dtype=int)
+    for (i, j), p in want.items():
+        flat[i, j] = p
+    assert list(reorder) == list(flat.ravel(order="F")[flat.ravel(order="F") != 0])
+    assert list(order) == list(np.argsort(reorder, kind="stable") + 1)
+    with pytest.raises(ValueError):
+        ko.cart2grid(kg, np.array([[20.0], [0.0]]))
+
+
+def test_find_column_major():
+    m = np.zeros((3, 4))
+    m[2, 0] = m[0, 1] = m[1, 3] = 1
+    assert list(ko.find_F(m)) == [2, 3, 10]

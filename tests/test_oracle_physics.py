@@ -1,0 +1,524 @@
+"""Pins the oracle's time loop against physics it must reproduce (CPU only).
+
+Every check compares with something the oracle does not compute the same
+way: a closed-form solution, a textbook law, or a second route through the
+solver whose equality follows from the reference's definitions.  Each is
+chosen so that a plausible slip fails it (a wrong sign of tau or eta, a
+dropped nonlinear term, a factor of 2 in a source scaling, a swapped axis
+spacing, an off-by-one in the time index of a source or of the time-reversal
+data).
+
+Mirrors the analytic tests of +kwave/+tests/+unit/TestAcousticSolver.m
+(plane waves equal across dimensions :49-317, axis permutation :455-614,
+d'Alembert initial value problem :769-806).
+"""
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import kwave_oracle as ko
+
+W = 2  # FFT workers per oracle run
+
+
+def run(kg, medium, source, sensor, **kw):
+    kw.setdefault("smooth", (False,))
+    return ko.kspace_first_order(kg, medium, source, sensor, fft_workers=W, **kw)
+
+
+def kmag(shape, spacings):
+    """|k| in FFT order, built from numpy's fftfreq."""
+    acc = 0.0
+    for ax, (n, d) in enumerate(zip(shape, spacings)):
+        sh = [1] * len(shape)
+        sh[ax] = n
+        acc = acc + (2 * math.pi * np.fft.fftfreq(n, d)).reshape(sh) ** 2
+    return np.sqrt(acc)
+
+
+# ---------------------------------------------------------------------------
+# homogeneous lossless: exact solutions
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("shape,sp", [((24, 20), (1e-4, 1.2e-4)),
+                                      ((15, 16), (1e-4, 1e-4)),
+                                      ((12, 10, 9), (1e-4, 0.9e-4, 1.1e-4))])
+def test_periodic_initial_value_is_exact(shape, sp):
+    """With the PML switched off the grid is periodic, and k-space time
+    stepping with kappa = sinc(c k dt/2) is exact for c = c_ref: step n
+    holds P0(k) cos(c |k| (n-1) dt), for every mode including Nyquist."""
+    c, Nt, dt = 1500.0, 30, 2.5e-8
+    kg = ko.KGrid(*[v for pair in zip(shape, sp) for v in pair])
+    kg.set_time(Nt, dt)
+    rng = np.random.default_rng(len(shape))
+    p0 = rng.standard_normal(shape)
+    got = run(kg, {"sound_speed": c, "density": 1000.0}, {"p0": p0},
+              {"mask": np.ones(shape)}, pml_size=2, pml_alpha=0.0)
+    P0, K = np.fft.fftn(p0), kmag(shape, sp)
+    for n in (0, 1, 2, 7, Nt - 1):
+        want = np.real(np.fft.ifftn(P0 * np.cos(c * K * n * dt))).ravel(order="F")
+        assert np.max(np.abs(got[:, n] - want)) < 1e-10 * np.max(np.abs(p0))
+
+
+@pytest.mark.parametrize("dim,axis", [(2, 0), (2, 1), (3, 2)])
+def test_dalembert_plane_wave(dim, axis):
+    """A Gaussian p0 that varies along one axis splits into two half-amplitude
+    pulses travelling at c (TestAcousticSolver.m:769-806 without the initial
+    velocity, which the legacy solver does not take)."""
+    N, dx, c, cfl, Nt, w = 256, 4e-3, 1500.0, 0.25, 150, 5.0
+    dt = cfl * dx / c
+    shape = [6] * dim
+    shape[axis] = N
+    args = []
+    for n in shape:
+        args += [n, dx]
+    kg = ko.KGrid(*args)
+    kg.set_time(Nt, dt)
+    x = (np.arange(N) - N // 2) * dx
+    prof = lambda s: np.exp(-(s / (w * dx)) ** 2)   # noqa: E731
+    sh = [1] * dim
+    sh[axis] = N
+    p0 = np.broadcast_to(prof(x).reshape(sh), shape).copy()
+    pml_size = [0] * dim
+    pml_size[axis] = 20
+    pml_alpha = [0.0] * dim
+    pml_alpha[axis] = 2.0
+    sd = run(kg, {"sound_speed": c, "density": 1000.0}, {"p0": p0},
+             {"mask": np.ones(shape), "record": ["p_final"]},
+             pml_size=pml_size, pml_alpha=pml_alpha)
+    T = (Nt - 1) * dt
+    want = 0.5 * (prof(x - c * T) + prof(x + c * T))
+    got = np.moveaxis(sd["p_final"], axis, 0).reshape(N, -1)
+    assert np.max(np.abs(got - want[:, None])) < 1e-9
+
+
+def test_dirichlet_pressure_source_sets_pressure_each_step():
+    """A dirichlet p source holds p at its points to the signal value of the
+    same step (signal column t at step t), for a linear lossless medium."""
+    shape = (20, 18)
+    kg = ko.KGrid(20, 1e-4, 18, 1e-4)
+    kg.set_time(25, 2e-8)
+    rng = np.random.default_rng(3)
+    pm = np.zeros(shape)
+    pm[5, 7] = pm[11, 3] = pm[14, 12] = 1
+    sig = rng.standard_normal((3, 25))
+    c = 1500 + 50 * rng.random(shape)
+    sd = run(kg, {"sound_speed": c, "density": 1000.0},
+             {"p_mask": pm, "p": sig, "p_mode": "dirichlet"}, {"mask": pm}, pml_size=3)
+    assert np.allclose(sd, sig, rtol=1e-12, atol=1e-12)
+
+
+def test_dirichlet_velocity_source_sets_velocity_each_step():
+    shape = (16, 12, 10)
+    kg = ko.KGrid(16, 1e-4, 12, 1e-4, 10, 1e-4)
+    kg.set_time(12, 2e-8)
+    um = np.zeros(shape)
+    um[4, 5, 6] = um[9, 2, 3] = 1
+    rng = np.random.default_rng(4)
+    ux, uz = rng.standard_normal((2, 12)), rng.standard_normal((2, 12))
+    sd = run(kg, {"sound_speed": 1500.0, "density": 1000.0},
+             {"u_mask": um, "ux": ux, "uz": uz, "u_mode": "dirichlet"},
+             {"mask": um, "record": ["u"]})
+    assert np.allclose(sd["ux"], ux, rtol=1e-12, atol=1e-14)
+    assert np.allclose(sd["uz"], uz, rtol=1e-12, atol=1e-14)
+    assert np.max(np.abs(sd["uy"])) > 0       # uy is driven by the pressure field
+
+
+# ---------------------------------------------------------------------------
+# invariances of the full heterogeneous, absorbing, nonlinear loop
+# ---------------------------------------------------------------------------
+
+def _medium(shape, rng, absorbing=True, nonlinear=True):
+    c = 1500 + 100 * ko.smooth_mat(rng.standard_normal(shape), True)
+    rho = 1000 + 80 * ko.smooth_mat(rng.standard_normal(shape), True)
+    med = {"sound_speed": c, "density": rho}
+    if absorbing:
+        med.update(alpha_coeff=0.75 + 0.2 * rng.random(shape), alpha_power=1.5)
+    if nonlinear:
+        med["BonA"] = 6 + rng.random(shape)
+    return med
+
+
+def _compact_p0(shape, rng, margin):
+    p0 = np.zeros(shape)
+    sl = tuple(slice(margin, n - margin) for n in shape)
+    p0[sl] = rng.standard_normal(p0[sl].shape)
+    return 1e5 * p0
+
+
+def test_axis_permutation_3d():
+    """Relabelling the axes (sizes, spacings, PML settings, medium and
+    source together) permutes the result; a swapped spacing, PML axis or
+    staggered-density axis breaks it."""
+    shape, sp = (12, 10, 14), (1e-4, 1.3e-4, 0.8e-4)
+    rng = np.random.default_rng(5)
+    med = _medium(shape, rng)
+    p0 = _compact_p0(shape, rng, 3)
+    pml_size, pml_alpha = (3, 2, 4), (2.0, 1.5, 2.5)
+    kg = ko.KGrid(shape[0], sp[0], shape[1], sp[1], shape[2], sp[2])
+    kg.set_time(20, 1.5e-8)
+    sd = run(kg, med, {"p0": p0}, {"mask": np.ones(shape), "record": ["p_final", "u_final"]},
+             pml_size=pml_size, pml_alpha=pml_alpha)
+    perm = (2, 0, 1)
+    kp = ko.KGrid(*[v for a in perm for v in (shape[a], sp[a])])
+    kp.set_time(20, 1.5e-8)
+    medp = {k: np.transpose(v, perm) for k, v in med.items() if k != "alpha_power"}
+    medp["alpha_power"] = 1.5
+    sdp = run(kp, medp, {"p0": np.transpose(p0, perm)},
+              {"mask": np.ones(kp.shape), "record": ["p_final", "u_final"]},
+              pml_size=[pml_size[a] for a in perm], pml_alpha=[pml_alpha[a] for a in perm])
+    ref = np.max(np.abs(sd["p_final"]))
+    assert np.max(np.abs(sdp["p_final"] - np.transpose(sd["p_final"], perm))) < 1e-10 * ref
+    names = ["ux_final", "uy_final", "uz_final"]
+    for new_ax, old_ax in enumerate(perm):
+        a, b = sdp[names[new_ax]], np.transpose(sd[names[old_ax]], perm)
+        assert np.max(np.abs(a - b)) < 1e-10 * np.max(np.abs(b))
+
+
+def test_2d_equals_3d_extruded():
+    """A 3D problem that does not vary along z (periodic in z, no z PML)
+    equals the 2D problem in every z plane, with heterogeneity, power-law
+    absorption and nonlinearity.  Catches any dimension-specific term."""
+    rng = np.random.default_rng(6)
+    s2 = (20, 18)
+    med2 = _medium(s2, rng)
+    p0 = _compact_p0(s2, rng, 4)
+    sens = np.zeros(s2)
+    sens[2:18:3, 1:17:5] = 1
+    kg2 = ko.KGrid(20, 1e-4, 18, 1.2e-4)
+    kg2.set_time(30, 1.5e-8)
+    rec = ["p", "p_max", "p_rms", "p_final"]
+    a = run(kg2, med2, {"p0": p0}, {"mask": sens, "record": rec}, pml_size=3, pml_alpha=2.0)
+    Nz = 5
+    ext = lambda v: np.repeat(np.asarray(v)[:, :, None], Nz, axis=2)   # noqa: E731
+    med3 = {k: (ext(v) if np.ndim(v) == 2 else v) for k, v in med2.items()}
+    sens3 = np.zeros(s2 + (Nz,))
+    sens3[:, :, 2] = sens
+    kg3 = ko.KGrid(20, 1e-4, 18, 1.2e-4, Nz, 0.7e-4)
+    kg3.set_time(30, 1.5e-8)
+    b = run(kg3, med3, {"p0": ext(p0)}, {"mask": sens3, "record": rec},
+            pml_size=(3, 3, 1), pml_alpha=(2.0, 2.0, 0.0))
+    for key in ("p", "p_max", "p_rms"):
+        assert np.max(np.abs(a[key] - b[key])) < 1e-10 * np.max(np.abs(a[key]))
+    for z in range(Nz):
+        assert np.max(np.abs(a["p_final"] - b["p_final"][:, :, z])) < 1e-10 * np.max(np.abs(a["p_final"]))
+
+
+# ---------------------------------------------------------------------------
+# power-law absorption and dispersion (Treeby & Cox 2010)
+# ---------------------------------------------------------------------------
+
+def _mode_history(alpha0, y, m, mode=None, N=64, Nt=160):
+    """p(t) of the single periodic mode cos(2 pi m x / L) (amplitude at x=0)."""
+    dx, c = 1e-4, 1500.0
+    dt = 0.3 * dx / c
+    kg = ko.KGrid(N, dx, 4, dx)
+    kg.set_time(Nt, dt)
+    p0 = np.cos(2 * math.pi * m * np.arange(N) / N)[:, None] * np.ones((1, 4))
+    med = {"sound_speed": c, "density": 1000.0, "alpha_coeff": alpha0, "alpha_power": y}
+    if mode:
+        med["alpha_mode"] = mode
+    mask = np.zeros((N, 4))
+    mask[0, 0] = 1
+    a = run(kg, med, {"p0": p0}, {"mask": mask}, pml_size=2, pml_alpha=0.0)[0]
+    # one mode of a linear two-field scheme: a[n+1] = c1 a[n] + c2 a[n-1]
+    # exactly (after the forced first step); c2 = -r^2, c1 = 2 r cos(theta)
+    h = a[1:]
+    A = np.stack([h[1:-1], h[:-2]], 1)
+    (c1, c2), *_ = np.linalg.lstsq(A, h[2:], rcond=None)
+    r = math.sqrt(-c2)
+    theta = math.acos(max(-1.0, min(1.0, c1 / (2 * r))))
+    k = 2 * math.pi * m / (N * dx)
+    return -math.log(r) / dt, theta / dt, c, k
+
+
+def _theory(alpha0, y, c, k, dispersion=True, absorption=True):
+    """Roots of s^2 + 2 a c^(y+1) k^y s + c^2 k^2 (1 - eta k^(y-1)) = 0 with
+    eta = 2 a c^y tan(pi y / 2), a in Np/((rad/s)^y m)."""
+    a = ko.db2neper(alpha0, y)
+    g = a * c ** (y + 1) * k ** y if absorption else 0.0
+    eta = 2 * a * c ** y * math.tan(math.pi * y / 2) if dispersion else 0.0
+    return g, math.sqrt(c * c * k * k * (1 - eta * k ** (y - 1)) - g * g)
+
+
+@pytest.mark.parametrize("y", [1.5, 1.3])
+def test_absorption_decay_rate_and_dispersion(y):
+    for m in (2, 4):
+        g, w, c, k = _mode_history(1.0, y, m)
+        g_th, w_th = _theory(1.0, y, c, k)
+        assert g == pytest.approx(g_th, rel=2e-3)
+        # phase-speed change from dispersion, relative to the lossless c k
+        assert (w - c * k) == pytest.approx(w_th - c * k, rel=0.08)
+    g2, *_ = _mode_history(1.0, y, 2)
+    g4, *_ = _mode_history(1.0, y, 4)
+    assert g4 / g2 == pytest.approx(2 ** y, rel=2e-3)
+
+
+def test_absorption_modes():
+    g, w, c, k = _mode_history(1.0, 1.5, 4, "no_absorption")
+    _, w_th = _theory(1.0, 1.5, c, k, absorption=False)
+    assert abs(g) < 1e-6
+    assert (w - c * k) == pytest.approx(w_th - c * k, rel=5e-3)
+    g, w, c, k = _mode_history(1.0, 1.5, 4, "no_dispersion")
+    g_th, _ = _theory(1.0, 1.5, c, k, dispersion=False)
+    assert g == pytest.approx(g_th, rel=2e-3)
+    _, w_disp = _theory(1.0, 1.5, c, k)
+    assert abs(w - c * k) < 0.1 * abs(w_disp - c * k)
+    # Stokes: alpha_power is taken as 2 whatever is given
+    g, w, c, k = _mode_history(1.0, 1.1, 4, "stokes")
+    g_th, _ = _theory(1.0, 2.0, c, k, dispersion=False)
+    assert g == pytest.approx(g_th, rel=2e-3)
+    # no absorption at all: the exact lossless phase c k dt per step
+    g, w, c, k = _mode_history(0.0, 1.5, 4)
+    assert abs(g) < 1e-6 and w == pytest.approx(c * k, rel=1e-12)
+
+
+def test_alpha_sign_flips_absorption():
+    g, _, c, k = _mode_history(1.0, 1.5, 4)
+    dx, N = 1e-4, 64
+    kg = ko.KGrid(N, dx, 4, dx)
+    kg.set_time(160, 0.3 * dx / 1500.0)
+    p0 = np.cos(2 * math.pi * 4 * np.arange(N) / N)[:, None] * np.ones((1, 4))
+    mask = np.zeros((N, 4))
+    mask[0, 0] = 1
+    med = {"sound_speed": 1500.0, "density": 1000.0, "alpha_coeff": 1.0, "alpha_power": 1.5,
+           "alpha_sign": [-1, 1]}
+    a = run(kg, med, {"p0": p0}, {"mask": mask}, pml_size=2, pml_alpha=0.0)[0]
+    # with tau negated the mode grows at the same rate
+    env = np.max(np.abs(a[-60:])) / np.max(np.abs(a[1:61]))
+    dt = 0.3 * dx / 1500.0
+    assert env == pytest.approx(math.exp(g * 99 * dt), rel=0.05)
+
+
+# ---------------------------------------------------------------------------
+# heterogeneous medium and nonlinearity
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("c2,r2", [(1875.0, 1600.0), (1500.0, 3000.0), (1200.0, 1000.0)])
+def test_reflection_and_transmission_coefficients(c2, r2):
+    """A pulse at normal incidence on a planar interface: reflected and
+    transmitted pressure amplitudes (Z2-Z1)/(Z2+Z1) and 2 Z2/(Z1+Z2)."""
+    N, Ny, dx, c1, r1, w = 512, 2, 1e-4, 1500.0, 1000.0, 6.0
+    dt = 0.3 * dx / max(c1, c2)
+    i = np.arange(N)
+    xi, src = N // 2 + 40, N // 2
+    c = np.where(i < xi, c1, c2)[:, None] * np.ones((1, Ny))
+    rho = np.where(i < xi, r1, r2)[:, None] * np.ones((1, Ny))
+    p0 = np.exp(-((i - src) / w) ** 2)[:, None] * np.ones((1, Ny))
+    mask = np.zeros((N, Ny))
+    mask[src - 60, 0] = mask[xi + 60, 0] = 1
+    Nt = int((200 * dx / min(c1, c2)) / dt)
+    kg = ko.KGrid(N, dx, Ny, dx)
+    kg.set_time(Nt, dt)
+    left, right = run(kg, {"sound_speed": c, "density": rho}, {"p0": p0}, {"mask": mask},
+                      pml_size=(20, 0), pml_alpha=(2.0, 0.0))
+    t = np.arange(Nt) * dt
+    split = t < 100 * dx / c1
+    direct, refl = left[split], left[~split]
+    pk = lambda v: v[np.argmax(np.abs(v))]   # noqa: E731
+    Z1, Z2 = r1 * c1, r2 * c2
+    assert pk(direct) == pytest.approx(0.5, rel=1e-4)
+    assert t[np.argmax(direct)] == pytest.approx(60 * dx / c1, abs=dt)
+    assert pk(refl) / pk(direct) == pytest.approx((Z2 - Z1) / (Z2 + Z1), rel=0.025)
+    assert pk(right) / pk(direct) == pytest.approx(2 * Z2 / (Z1 + Z2), rel=0.005)
+
+
+def _second_harmonic(BonA, dist=100):
+    N, Ny, dx, c, rho, P, ppw = 384, 2, 1e-4, 1500.0, 1000.0, 1e6, 24
+    dt = 0.3 * dx / c
+    f = c / (ppw * dx)
+    spp = 1 / (f * dt)
+    Nt = int((dist + 5) * dx / c / dt + 10 * spp)
+    kg = ko.KGrid(N, dx, Ny, dx)
+    kg.set_time(Nt, dt)
+    t = np.arange(Nt) * dt
+    ramp = 0.5 - 0.5 * np.cos(math.pi * np.clip(t * f / 2, 0, 1))
+    pm = np.zeros((N, Ny))
+    pm[40, :] = 1
+    mask = np.zeros((N, Ny))
+    mask[40 + dist, 0] = 1
+    med = {"sound_speed": c, "density": rho}
+    if BonA is not None:
+        med["BonA"] = BonA
+    d = run(kg, med, {"p_mask": pm, "p": (P * ramp * np.sin(2 * math.pi * f * t))[None, :],
+                      "p_mode": "dirichlet"}, {"mask": mask},
+            pml_size=(20, 0), pml_alpha=(2.0, 0.0))[0]
+    n = int(round(6 * spp))
+    ts, seg = t[-n:], d[-n:]
+    w1 = 2 * math.pi * f
+    A = np.stack([fn(h * w1 * ts) for h in (1, 2, 3) for fn in (np.cos, np.sin)]
+                 + [np.ones_like(ts)], 1)
+    co = np.linalg.lstsq(A, seg, rcond=None)[0]
+    p1, p2 = math.hypot(co[0], co[1]), math.hypot(co[2], co[3])
+    # Fubini, small distance: p2 = beta w p1^2 x / (2 rho c^3), beta = 1 + B/2A
+    beta = 1 + (BonA or 0.0) / 2
+    return p1, p2, beta * w1 * p1 ** 2 * (dist * dx) / (2 * rho * c ** 3)
+
+
+def test_second_harmonic_growth():
+    p1, p2, th = _second_harmonic(6.0)
+    assert p2 == pytest.approx(th, rel=0.02)
+    q1, q2, qth = _second_harmonic(0.0)      # convective nonlinearity alone (beta = 1)
+    assert q2 == pytest.approx(qth, rel=0.02)
+    l1, l2, _ = _second_harmonic(None)       # linear: no harmonic
+    assert l2 < 1e-6 * l1
+
+
+# ---------------------------------------------------------------------------
+# sources, time reversal and sensor bookkeeping
+# ---------------------------------------------------------------------------
+
+def test_time_reversal_equals_shifted_dirichlet_source():
+    """Time reversal enforces the flipped boundary data on p (and on the split
+    densities) at the start of each step and once more after the last one;
+    for a linear lossless medium that is a dirichlet p source one step earlier
+    with the data column order reversed and the first enforced column zero."""
+    shape = (24, 20)
+    kg = ko.KGrid(24, 1e-4, 20, 1e-4)
+    Nt = 30
+    kg.set_time(Nt, 2e-8)
+    rng = np.random.default_rng(8)
+    sens = np.zeros(shape)
+    sens[3, 2:18] = 1
+    sens[4:20, 17] = 1
+    ns = int(sens.sum())
+    data = rng.standard_normal((ns, Nt))
+    data[:, -1] = 0      # first column used after the flip
+    med = {"sound_speed": 1500 + 60 * rng.random(shape), "density": 1000.0}
+    tr = run(kg, med, {}, {"mask": sens, "time_reversal_boundary_data": data}, pml_size=3)
+    kd = ko.KGrid(24, 1e-4, 20, 1e-4)
+    kd.set_time(Nt - 1, 2e-8)
+    sd = run(kd, med, {"p_mask": sens, "p": data[:, ::-1][:, 1:], "p_mode": "dirichlet"},
+             {"mask": sens, "record": ["p_final"]}, pml_size=3)
+    assert np.max(np.abs(tr - sd["p_final"])) < 1e-12 * np.max(np.abs(tr))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_additive_source_is_filtered_no_correction_source(dim):
+    """An 'additive' p source equals an 'additive-no-correction' source on
+    every cell whose signal is the source mask filtered by
+    cos(c_ref k dt / 2) in k-space times the time signal; both are scaled by
+    2 dt / (dim c dx)."""
+    shape = (16, 14) if dim == 2 else (10, 12, 9)
+    sp = (1e-4, 1.1e-4, 0.9e-4)[:dim]
+    args = [v for pair in zip(shape, sp) for v in pair]
+    rng = np.random.default_rng(9)
+    c = 1500.0
+    Nt, dt = 20, 2e-8
+    mask = np.zeros(shape)
+    mask[tuple(rng.integers(2, n - 2, 5) for n in shape)] = 1
+    s = rng.standard_normal(Nt)
+    med = {"sound_speed": c, "density": 1000.0, "alpha_coeff": 0.5, "alpha_power": 1.5}
+    kg = ko.KGrid(*args)
+    kg.set_time(Nt, dt)
+    a = run(kg, med, {"p_mask": mask, "p": s[None, :], "p_mode": "additive"},
+            {"mask": np.ones(shape), "record": ["p_final", "p_max"]}, pml_size=2)
+    K = np.real(np.fft.ifftn(np.cos(c * kmag(shape, sp) * dt / 2) * np.fft.fftn(mask)))
+    sig = K.ravel(order="F")[:, None] * s[None, :]
+    kg.set_time(Nt, dt)
+    b = run(kg, med, {"p_mask": np.ones(shape), "p": sig, "p_mode": "additive-no-correction"},
+            {"mask": np.ones(shape), "record": ["p_final", "p_max"]}, pml_size=2)
+    ref = np.max(np.abs(a["p_final"]))
+    assert ref > 0
+    assert np.max(np.abs(a["p_final"] - b["p_final"])) < 1e-11 * ref
+    assert np.max(np.abs(a["p_max"] - b["p_max"])) < 1e-11 * np.max(np.abs(a["p_max"]))
+
+
+def test_source_linearity_and_scaling():
+    """Linear medium: doubling the signal doubles the field; the additive
+    scaling 2 dt/(dim c dx) makes a no-correction point source independent
+    of how the same total signal is split over two coincident series."""
+    shape = (18, 16)
+    kg = ko.KGrid(18, 1e-4, 16, 1e-4)
+    kg.set_time(15, 2e-8)
+    rng = np.random.default_rng(10)
+    m = np.zeros(shape)
+    m[5, 6] = m[12, 9] = 1
+    s = rng.standard_normal((2, 15))
+    med = {"sound_speed": 1500.0, "density": 1000.0}
+    sens = {"mask": np.ones(shape), "record": ["p_final"]}
+    one = run(kg, med, {"p_mask": m, "p": s, "p_mode": "additive-no-correction"}, sens, pml_size=3)
+    two = run(kg, med, {"p_mask": m, "p": 2 * s, "p_mode": "additive-no-correction"}, sens, pml_size=3)
+    assert np.allclose(two["p_final"], 2 * one["p_final"], rtol=1e-12, atol=1e-20)
+    # labelled mask: both points read series 1 and 2 by label
+    lab = m.copy()
+    lab[5, 6], lab[12, 9] = 2, 1
+    sw = run(kg, med, {"p_mask": lab, "p": s[::-1], "p_mode": "additive-no-correction"}, sens, pml_size=3)
+    assert np.allclose(sw["p_final"], one["p_final"], rtol=1e-12, atol=1e-20)
+
+
+def test_sensor_statistics_match_time_series():
+    rng = np.random.default_rng(11)
+    shape = (20, 16, 12)
+    med = _medium(shape, rng, absorbing=True, nonlinear=False)
+    p0 = _compact_p0(shape, rng, 3)
+    sens = np.zeros(shape)
+    sens[::3, ::4, ::5] = 1
+    kg = ko.KGrid(20, 1e-4, 16, 1e-4, 12, 1e-4)
+    kg.set_time(25, 1.5e-8)
+    rec = ["p", "p_max", "p_min", "p_rms", "p_max_all", "p_min_all", "p_final",
+           "u", "u_max", "u_min", "u_rms", "u_max_all", "u_min_all", "u_final"]
+    sd, st = run(kg, med, {"p0": p0}, {"mask": sens, "record": rec, "record_start_index": 6},
+                 pml_size=3, return_state=True)
+    p = sd["p"]
+    assert p.shape == (int(sens.sum()), 20)
+    assert np.array_equal(sd["p_max"], p.max(1)) and np.array_equal(sd["p_min"], p.min(1))
+    assert np.allclose(sd["p_rms"], np.sqrt(np.mean(p ** 2, 1)), rtol=1e-12)
+    assert np.array_equal(sd["p_final"], st.p)
+    assert np.array_equal(sd["p_final"].ravel(order="F")[ko.find_F(sens)], p[:, -1])
+    for u, v in zip(("ux", "uy", "uz"), st.u):
+        ts = sd[u]
+        assert np.array_equal(sd[u + "_max"], ts.max(1)) and np.array_equal(sd[u + "_min"], ts.min(1))
+        assert np.allclose(sd[u + "_rms"], np.sqrt(np.mean(ts ** 2, 1)), rtol=1e-12)
+        assert np.array_equal(sd[u + "_final"], v)
+        assert np.array_equal(sd[u + "_max_all"].ravel(order="F")[ko.find_F(sens)], ts.max(1))
+    assert np.array_equal(sd["p_max_all"].ravel(order="F")[ko.find_F(sens)], p.max(1))
+    assert np.array_equal(sd["p_min_all"].ravel(order="F")[ko.find_F(sens)], p.min(1))
+
+
+def test_record_start_index_drops_leading_steps():
+    shape = (16, 16)
+    kg = ko.KGrid(16, 1e-4, 16, 1e-4)
+    kg.set_time(20, 2e-8)
+    rng = np.random.default_rng(12)
+    p0 = _compact_p0(shape, rng, 3)
+    sens = np.zeros(shape)
+    sens[4, :] = 1
+    full = run(kg, {"sound_speed": 1500.0, "density": 1000.0}, {"p0": p0}, {"mask": sens}, pml_size=3)
+    late = run(kg, {"sound_speed": 1500.0, "density": 1000.0}, {"p0": p0},
+               {"mask": sens, "record_start_index": 8}, pml_size=3)
+    assert np.array_equal(late, full[:, 7:])
+
+
+def test_single_precision_tracks_double():
+    """'DataCast','single' stays within FP32 rounding growth of the FP64 run."""
+    rng = np.random.default_rng(13)
+    shape = (32, 28)
+    med = _medium(shape, rng)
+    p0 = _compact_p0(shape, rng, 5)
+    sens = np.zeros(shape)
+    sens[6, 4:24] = 1
+    kg = ko.KGrid(32, 1e-4, 28, 1e-4)
+    kg.set_time(60, 1.5e-8)
+    d = run(kg, med, {"p0": p0}, {"mask": sens}, pml_size=4)
+    s = run(kg, med, {"p0": p0}, {"mask": sens}, pml_size=4, data_cast="single")
+    assert s.dtype == np.float32
+    assert 1e-9 < ko.rel_linf(s, d) < 1e-5
+
+
+def test_pml_absorbs_outgoing_energy():
+    """Waves leaving the domain are absorbed: after the pulse has crossed the
+    PML the remaining field is a small fraction of the initial one."""
+    shape = (64, 64)
+    kg = ko.KGrid(64, 1e-4, 64, 1e-4)
+    kg.set_time(400, 2e-8)
+    x = np.arange(64) - 32
+    p0 = np.exp(-(x[:, None] ** 2 + x[None, :] ** 2) / 9.0)
+    sd = run(kg, {"sound_speed": 1500.0, "density": 1000.0}, {"p0": p0},
+             {"mask": np.ones(shape), "record": ["p_final"]}, pml_size=12, pml_alpha=2.0)
+    assert np.max(np.abs(sd["p_final"])) < 0.01 * np.max(p0)
+    sd0 = run(kg, {"sound_speed": 1500.0, "density": 1000.0}, {"p0": p0},
+              {"mask": np.ones(shape), "record": ["p_final"]}, pml_size=12, pml_alpha=0.0)
+    assert np.max(np.abs(sd0["p_final"])) > 0.03 * np.max(p0)
