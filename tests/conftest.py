@@ -16,6 +16,16 @@ def pytest_configure(config):
     config.addinivalue_line("markers", "slow: long CPU test")
 
 
+def pytest_sessionstart(session):
+    """Tests load the libraries that are built; only when none is built is
+    the build run first."""
+    lib = ROOT / "k-wave-ii_amd" / "kwhip" / "_lib"
+    if not (lib / "libkwhip.so").exists() or not (lib / "libkwhip_emu.so").exists():
+        import __graft_entry__
+
+        __graft_entry__.build()
+
+
 @pytest.fixture(scope="session")
 def hip_lib():
     """The product library on the GPU; fails loudly if it is missing."""
