@@ -94,11 +94,9 @@ __device__ __forceinline__ void lds_st(cf* p, cf v) {
 #define KW_ST(ptr, val) (*(ptr) = (val))
 #endif
 
-// Loads and stores at a 32-bit element index from a base pointer: the byte
-// offset is formed in 32 bits so that the compiler can address with a
-// uniform 64-bit base plus a 32-bit per-lane offset (one VGPR per address
-// instead of a 64-bit pointer).  Every array is below 4 GiB (checked on the
-// host).
+// Loads and stores at a 32-bit element index from a base pointer.  Every
+// array they address is below 4 GiB (checked on the host), so the byte
+// offset fits 32 bits.
 template <class T>
 KW_HOSTDEV const T* at32(const T* base, uint32_t i) {
   return (const T*)((const char*)base + (size_t)(uint32_t)(i * (uint32_t)sizeof(T)));
@@ -107,8 +105,47 @@ template <class T>
 KW_HOSTDEV T* at32(T* base, uint32_t i) {
   return (T*)((char*)base + (size_t)(uint32_t)(i * (uint32_t)sizeof(T)));
 }
+#ifdef KW_GPU
+// On the GPU they are raw buffer instructions: the base (uniform in every
+// kernel here) becomes a buffer resource in scalar registers and each lane
+// supplies one 32-bit byte offset.  With plain global instructions the
+// compiler hoists zero-extended 64-bit offsets out of loops and spends two
+// VGPRs per address.  The resource spans the whole 32-bit offset range
+// (gfx9 resource word 3: 0x00020000, untyped 32-bit data).
+typedef __amdgpu_buffer_rsrc_t kw_rsrc;
+__device__ __forceinline__ kw_rsrc kw_res(const void* base) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)0xffffffff,
+                                           0x00020000);
+}
+template <class T>
+__device__ __forceinline__ T kw_ld32(const T* base, uint32_t i) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "4- or 8-byte elements");
+  const int off = (int)(i * (uint32_t)sizeof(T));
+  if constexpr (sizeof(T) == 4) {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b32(kw_res(base), off, 0, 0));
+  } else {
+    return __builtin_bit_cast(T, __builtin_amdgcn_raw_buffer_load_b64(kw_res(base), off, 0, 0));
+  }
+}
+template <class T, class V>
+__device__ __forceinline__ void kw_st32(T* base, uint32_t i, V v) {
+  static_assert(sizeof(T) == 4 || sizeof(T) == 8, "4- or 8-byte elements");
+  const int off = (int)(i * (uint32_t)sizeof(T));
+  const T w = (T)v;
+  if constexpr (sizeof(T) == 4) {
+    __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, w), kw_res(base), off, 0,
+                                          0);
+  } else {
+    typedef uint32_t u2 __attribute__((ext_vector_type(2)));
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, w), kw_res(base), off, 0, 0);
+  }
+}
+#define KW_LD32(base, i) (::kw::kw_ld32((base), (uint32_t)(i)))
+#define KW_ST32(base, i, v) (::kw::kw_st32((base), (uint32_t)(i), (v)))
+#else
 #define KW_LD32(base, i) KW_LD(::kw::at32((base), (uint32_t)(i)))
 #define KW_ST32(base, i, v) KW_ST(::kw::at32((base), (uint32_t)(i)), (v))
+#endif
 
 // Scheduling fence between epilogue phases: each phase batches its own
 // loads, but loads are not hoisted across phases (which would hold every
@@ -119,10 +156,45 @@ KW_HOSTDEV T* at32(T* base, uint32_t i) {
 #define KW_PHASE_FENCE() ((void)0)
 #endif
 
-#ifndef KW_GPU
+#ifdef KW_EMU
+// every LDS access is checked by the emulator for races between lanes (a
+// read of an element written by another lane with no barrier of the
+// required scope in between)
+void emu_lds_access(const cf* p, bool write);
+inline cf lds_ld(const cf* p) {
+  emu_lds_access(p, false);
+  return *p;
+}
+inline void lds_st(cf* p, cf v) {
+  emu_lds_access(p, true);
+  *p = v;
+}
+#elif !defined(KW_GPU)
 inline cf lds_ld(const cf* p) { return *p; }
 inline void lds_st(cf* p, cf v) { *p = v; }
 #endif
+
+// Barrier of a pencil's threads.  When every pencil of a block lies inside
+// one 64-lane wavefront (the G threads of pencil p are tid p*G .. p*G+G-1
+// and G divides 64), LDS exchanges within a pencil only need the wavefront
+// to be ordered, not the block: `wave` selects a wavefront barrier instead
+// of a block barrier.  `wave` must be uniform across the block.
+template <class Ctx>
+struct PencilCtx {
+  Ctx& c;
+  bool wave;
+  KW_BODY int tid() const { return c.tid(); }
+  KW_BODY int bid() const { return c.bid(); }
+  KW_BODY cf* lds() const { return c.lds(); }
+  KW_BODY void sync() const {
+    if (wave)
+      c.wsync();
+    else
+      c.sync();
+  }
+};
+
+KW_HOSTDEV bool pencils_in_wave(int G) { return G > 0 && G <= 64 && 64 % G == 0; }
 
 // Elements held per thread per FFT stage (upper bound).  A pencil of length
 // N is processed by G = ceil(N / KW_E) threads.

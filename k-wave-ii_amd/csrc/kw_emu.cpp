@@ -3,7 +3,17 @@
 // thread, so barriers behave as on the GPU.  Every global load and store is
 // checked against the table of live allocations; LDS is filled with NaN at
 // the start of every block so reads of unwritten LDS show up in results; a
-// block whose lanes pass a different number of barriers is reported.
+// block whose lanes pass different sequences of barriers is reported.
+//
+// Lanes are grouped into 64-lane wavefronts as on gfx950.  Block barriers
+// (sync) and wavefront barriers (wsync) both yield to the scheduler, which
+// resumes the lanes round-robin; with every lane passing the same sequence
+// of barriers (checked) each lane's n-th segment runs after all lanes'
+// (n-1)-th segments, which is what both barrier kinds guarantee.  What the
+// round-robin order does not show, a missing barrier scope, is checked on
+// every LDS read: an element written by a lane of another wavefront must
+// have a block barrier between the write and the read, an element written
+// by another lane of the same wavefront at least a wavefront barrier.
 //
 // This backend exists to validate indexing and parity on a machine without
 // a GPU.  It is built into libkwhip_emu.so, which only the tests load.
@@ -112,22 +122,39 @@ struct EmuCtx {
   int bid() const { return b; }
   cf* lds() const { return L; }
   void sync();
+  void wsync();
 };
 
 struct Lane {
   ucontext_t ctx;
   std::vector<char> stack;
   bool done;
-  long syncs;
+  long syncs;   // block barriers passed
+  long wsyncs;  // wavefront barriers passed
+  uint64_t seq; // hash of the sequence of barrier kinds
 };
 
 struct Runner {
   ucontext_t sched;
   std::vector<Lane> lanes;
   std::vector<cf> lds;
+  // last writer of each LDS element: lane, its block-barrier count and its
+  // total barrier count at the write
+  std::vector<int32_t> lw_lane;
+  std::vector<long> lw_sync, lw_tick;
   int cur = 0;
   int bid = 0;
   const std::function<void(EmuCtx&)>* body = nullptr;
+
+  void yield(int kind) {
+    Lane& l = lanes[cur];
+    if (kind == 0)
+      ++l.syncs;
+    else
+      ++l.wsyncs;
+    l.seq = l.seq * 1000003u + (uint64_t)(kind + 1);
+    swapcontext(&l.ctx, &sched);
+  }
 
   static void trampoline() {
     Runner* R = tl_runner;
@@ -142,11 +169,16 @@ struct Runner {
     if ((int)lanes.size() < T) lanes.resize(T);
     const float qnan = std::nanf("");
     for (auto& v : lds) v = cf{qnan, qnan};
+    lw_lane.assign(lds.size(), -1);
+    lw_sync.assign(lds.size(), 0);
+    lw_tick.assign(lds.size(), 0);
     for (int i = 0; i < T; ++i) {
       Lane& l = lanes[i];
       if (l.stack.empty()) l.stack.resize(kStack);
       l.done = false;
       l.syncs = 0;
+      l.wsyncs = 0;
+      l.seq = 0;
       getcontext(&l.ctx);
       l.ctx.uc_stack.ss_sp = l.stack.data();
       l.ctx.uc_stack.ss_size = l.stack.size();
@@ -164,21 +196,26 @@ struct Runner {
       if (alive == 0) break;
     }
     for (int i = 1; i < T; ++i) {
-      if (lanes[i].syncs != lanes[0].syncs) {
+      if (lanes[i].syncs != lanes[0].syncs || lanes[i].wsyncs != lanes[0].wsyncs ||
+          lanes[i].seq != lanes[0].seq) {
         if (g_fault.exchange(1) == 0)
           snprintf(g_fault_msg, sizeof g_fault_msg,
-                   "emulator: block %d lane %d passed %ld barriers, lane 0 passed %ld", b, i,
-                   lanes[i].syncs, lanes[0].syncs);
+                   "emulator: block %d lane %d passed %ld block and %ld wavefront barriers, "
+                   "lane 0 passed %ld and %ld (or in another order)",
+                   b, i, lanes[i].syncs, lanes[i].wsyncs, lanes[0].syncs, lanes[0].wsyncs);
         break;
       }
     }
   }
 };
 
-void EmuCtx::sync() {
-  Lane& l = R->lanes[t];
-  ++l.syncs;
-  swapcontext(&l.ctx, &R->sched);
+void EmuCtx::sync() { R->yield(0); }
+void EmuCtx::wsync() { R->yield(1); }
+
+void lds_fault(const char* msg, int b, int lane, long el, int other) {
+  if (g_fault.exchange(1) == 0)
+    snprintf(g_fault_msg, sizeof g_fault_msg, "emulator: block %d lane %d LDS element %ld: %s %d",
+             b, lane, el, msg, other);
 }
 
 void emu_launch(int nblocks, int T, size_t lds_bytes,
@@ -213,6 +250,29 @@ void emu_launch(int nblocks, int T, size_t lds_bytes,
 }
 
 }  // namespace
+
+void emu_lds_access(const cf* p, bool write) {
+  Runner* R = tl_runner;
+  if (!R) return;
+  const long i = (long)(p - R->lds.data());
+  const int me = R->cur;
+  if (i < 0 || i >= (long)R->lds.size()) {
+    lds_fault("out of bounds of the block's LDS of size", R->bid, me, i, (int)R->lds.size());
+    return;
+  }
+  const Lane& l = R->lanes[me];
+  if (write) {
+    R->lw_lane[i] = me;
+    R->lw_sync[i] = l.syncs;
+    R->lw_tick[i] = l.syncs + l.wsyncs;
+    return;
+  }
+  const int w = R->lw_lane[i];
+  if (w < 0 || w == me) return;
+  const bool ok = (w / 64 != me / 64) ? l.syncs > R->lw_sync[i]
+                                      : l.syncs + l.wsyncs > R->lw_tick[i];
+  if (!ok) lds_fault("read without a barrier after the write of lane", R->bid, me, i, w);
+}
 
 template <class Sel>
 static void emu_pass(PassKind kind, int dim, int nblocks, int threads, size_t lds_bytes,

@@ -84,7 +84,7 @@ static_assert(MaxB<2>::v * 2 <= kMaxSlots && MaxB<3>::v * 3 <= kMaxSlots &&
 
 template <bool INV>
 KW_BODY cf twiddle(const cf* tw, int idx) {
-  cf w = KW_LD(tw + idx);
+  cf w = KW_LD32(tw, (uint32_t)idx);
   if (INV) w.y = -w.y;
   return w;
 }

@@ -16,20 +16,38 @@ struct GpuCtx {
   __device__ __forceinline__ int tid() const { return (int)threadIdx.x; }
   __device__ __forceinline__ int bid() const { return (int)blockIdx.x; }
   __device__ __forceinline__ void sync() const { __syncthreads(); }
+  // wavefront barrier: LDS writes of the wavefront before it are visible to
+  // its LDS reads after it (a wavefront's LDS operations execute in order;
+  // the fences keep the compiler from moving accesses across)
+  __device__ __forceinline__ void wsync() const {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
   __device__ __forceinline__ cf* lds() const { return kw_lds_dyn; }
 };
 
 #define KW_CAT2(a, b) a##b
 #define KW_CAT(a, b) KW_CAT2(a, b)
 
+// Kernels take the problem descriptors and the block-to-problem table as
+// read-only, non-aliased parameters: nothing a kernel stores can change
+// them, so the compiler may fetch descriptor fields with scalar loads and
+// keep every field base address in scalar registers.
+#define KW_KPARAMS \
+  const ProbDev* __restrict__ probs, const int32_t* __restrict__ tiles, int32_t nprob
+#define KW_KBATCH const Batch B{probs, tiles, nprob}
+
 #if KW_KIND == 8
 
-__global__ __launch_bounds__(256) void k_ew(Batch B, EwArgs A) {
+__global__ __launch_bounds__(256) void k_ew(KW_KPARAMS, EwArgs A) {
   GpuCtx c;
+  KW_KBATCH;
   body_ew(c, B, A);
 }
 void launch_k_ew(int nb, int T, size_t lds, void* s, const Batch& B, const EwArgs& A) {
-  hipLaunchKernelGGL(k_ew, dim3(nb), dim3(T), lds, (hipStream_t)s, B, A);
+  hipLaunchKernelGGL(k_ew, dim3(nb), dim3(T), lds, (hipStream_t)s, B.probs, B.tile_start,
+                     B.nprob, A);
 }
 
 #else
@@ -47,53 +65,63 @@ using Sel = PlanCt<KW_SIZE>;
 // x passes run at most 256 threads per block; strided and last-axis passes
 // at most 512
 #if KW_KIND == 0
-__global__ __launch_bounds__(512) void kern(Batch B, PassArgs A) {
+__global__ __launch_bounds__(512) void kern(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
+  KW_KBATCH;
   body_ypass<false, Sel>(c, B, A);
 }
 #elif KW_KIND == 1
-__global__ __launch_bounds__(512) void kern(Batch B, PassArgs A) {
+__global__ __launch_bounds__(512) void kern(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
+  KW_KBATCH;
   body_ypass<true, Sel>(c, B, A);
 }
 #elif KW_KIND == 2
-__global__ __launch_bounds__(512) void kern(Batch B, PassArgs A) {
+__global__ __launch_bounds__(512) void kern(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
+  KW_KBATCH;
   body_spec<Sel>(c, B, A);
 }
 #elif KW_KIND == 3
-__global__ __launch_bounds__(256) void kern(Batch B, PassArgs A) {
+__global__ __launch_bounds__(256) void kern(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
+  KW_KBATCH;
   body_x_r2c<Sel>(c, B, A);
 }
 #elif KW_KIND == 4
-__global__ __launch_bounds__(256) void kern(Batch B, PassArgs A) {
+__global__ __launch_bounds__(256) void kern(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
+  KW_KBATCH;
   body_x_c2r<Sel>(c, B, A);
 }
 #elif KW_KIND == 5
-__global__ __launch_bounds__(256) void kern2(Batch B, PassArgs A) {
+__global__ __launch_bounds__(256) void kern2(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
+  KW_KBATCH;
   body_x_grad<Sel, 2>(c, B, A);
 }
-__global__ __launch_bounds__(256) void kern(Batch B, PassArgs A) {
+__global__ __launch_bounds__(256) void kern(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
+  KW_KBATCH;
   body_x_grad<Sel, 3>(c, B, A);
 }
 #define KW_HAS_DIM2 1
 #elif KW_KIND == 6
-__global__ __launch_bounds__(256) void kern2(Batch B, PassArgs A) {
+__global__ __launch_bounds__(256) void kern2(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
+  KW_KBATCH;
   body_x_div<Sel, 2>(c, B, A);
 }
-__global__ __launch_bounds__(256) void kern(Batch B, PassArgs A) {
+__global__ __launch_bounds__(256) void kern(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
+  KW_KBATCH;
   body_x_div<Sel, 3>(c, B, A);
 }
 #define KW_HAS_DIM2 1
 #elif KW_KIND == 7
-__global__ __launch_bounds__(256) void kern(Batch B, PassArgs A) {
+__global__ __launch_bounds__(256) void kern(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
+  KW_KBATCH;
   body_x_eos<Sel>(c, B, A);
 }
 #else
@@ -105,12 +133,14 @@ void KW_CAT(KW_CAT(KW_CAT(launch_v_, KW_KIND), _), KW_SIZE)(int dim, int nb, int
                                                             const PassArgs& A) {
 #ifdef KW_HAS_DIM2
   if (dim == 2) {
-    hipLaunchKernelGGL(kern2, dim3(nb), dim3(T), lds, (hipStream_t)s, B, A);
+    hipLaunchKernelGGL(kern2, dim3(nb), dim3(T), lds, (hipStream_t)s, B.probs, B.tile_start,
+                       B.nprob, A);
     return;
   }
 #endif
   (void)dim;
-  hipLaunchKernelGGL(kern, dim3(nb), dim3(T), lds, (hipStream_t)s, B, A);
+  hipLaunchKernelGGL(kern, dim3(nb), dim3(T), lds, (hipStream_t)s, B.probs, B.tile_start,
+                     B.nprob, A);
 }
 
 #endif

@@ -48,14 +48,16 @@ KW_BODY void half_put(const ProbDev& P, cf* S, uint32_t hrow, uint32_t qrow, int
   KW_ST32(S, idx, v);
 }
 
-// pencils per row pair held in LDS by the x passes
-constexpr int kNPX = 3;
+// pencils per row pair held in LDS by the x passes: the field components
+// are transformed one after another through the same pencil
+constexpr int kNPX = 1;
 
 KW_HOSTDEV int x_lds_stride(int Nx) { return Nx + (Nx >> 3) + 1; }
 
 struct XTile {
   int g, G;
   bool active, has1;
+  bool wave;  // the G threads of every pencil lie in one wavefront
   int64_t r0, r1;
   // 32-bit offsets of the two rows: first cell (row * Nx), half-spectrum row
   // (row * Wm), Nyquist entry (nspec_main + row); y and z of each row
@@ -63,10 +65,11 @@ struct XTile {
   LdsPencils lds;
 };
 
-template <class Ctx>
+template <class XS, class Ctx>
 KW_BODY XTile x_tile(Ctx& c, const ProbDev& P, int tile) {
   XTile T;
-  T.G = P.Gx;
+  T.G = XS::fixed ? XS::fixed / KW_E : P.Gx;
+  T.wave = XS::fixed ? pencils_in_wave(XS::fixed / KW_E) : pencils_in_wave(P.Gx);
   const int tid = c.tid();
   int p = tid / T.G;
   T.g = tid % T.G;
@@ -74,11 +77,9 @@ KW_BODY XTile x_tile(Ctx& c, const ProbDev& P, int tile) {
   int64_t pair = (int64_t)tile * P.Px + p;
   T.active = p < P.Px && pair < npairs;
   // threads without a row pair read row pair 0 (their loads stay in bounds
-  // and unconditional) and never store
-  if (!T.active) {
-    p = 0;
-    pair = 0;
-  }
+  // and unconditional) and never store; they keep their own LDS pencil
+  if (!T.active) pair = 0;
+  if (p >= P.Px) p = 0;
   T.r0 = 2 * pair;
   T.has1 = 2 * pair + 1 < P.nrows;
   T.r1 = T.has1 ? 2 * pair + 1 : T.r0;
@@ -157,12 +158,13 @@ KW_BODY float pml_at(const float* v, const XTile& T, int d, int row, int n) {
 
 // Time-source update of one cell, for a source form fixed at compile time:
 // 0 additive single series (s(t) K_M), 1 additive several series (filtered
-// grid), 2 dirichlet (map), 3 additive-no-correction (map)
+// grid), 2 dirichlet (map), 3 additive-no-correction (map).  `s0` is the
+// signal value s(t) of form 0, read once per launch by src_value0.
 template <int FORM>
-KW_BODY float apply_src(const TimeSource& s, uint32_t cell, float v, int t) {
+KW_BODY float apply_src(const TimeSource& s, uint32_t cell, float v, int t, float s0) {
   const int t0 = t - 1;
   if constexpr (FORM == 0) {
-    return v + s.value(0, t0) * KW_LD32(s.km, cell);
+    return v + s0 * KW_LD32(s.km, cell);
   } else if constexpr (FORM == 1) {
     return v + KW_LD32(s.grid, cell);
   } else {
@@ -173,6 +175,14 @@ KW_BODY float apply_src(const TimeSource& s, uint32_t cell, float v, int t) {
     else
       return i >= 0 ? v + sv : v;
   }
+}
+
+template <int FORM>
+KW_BODY float src_value0(const TimeSource& s, int t) {
+  if constexpr (FORM == 0)
+    return s.value(0, t - 1);
+  else
+    return 0.f;
 }
 
 KW_BODY int src_form(const TimeSource& s) {
@@ -356,11 +366,11 @@ KW_BODY void body_spec(Ctx& c, const Batch& B, const PassArgs& A) {
 
 // real grid(s) -> half spectrum (PassArgs.real[b] -> S[buf[b]])
 template <class XS, class Ctx>
-KW_BODY void body_x_r2c(Ctx& c, const Batch& B, const PassArgs& A) {
+KW_BODY void body_x_r2c(Ctx& cb, const Batch& B, const PassArgs& A) {
   int tile;
-  const ProbDev& P = batch_prob(c, B, tile);
-  const XTile T = x_tile(c, P, tile);
-  const int N = P.Nx;
+  const ProbDev& P = batch_prob(cb, B, tile);
+  const XTile T = x_tile<XS>(cb, P, tile);
+  PencilCtx<Ctx> c{cb, T.wave};
   for (int b = 0; b < A.nbuf; ++b) {
     const float* f = A.real[b] ? A.real[b] : P.p;
     auto src = [&](int, int n) -> cf {
@@ -377,11 +387,11 @@ KW_BODY void body_x_r2c(Ctx& c, const Batch& B, const PassArgs& A) {
 
 // half spectrum -> real grid (S[buf[b]] -> PassArgs.real[b]); XC_ACCUM adds
 template <class XS, class Ctx>
-KW_BODY void body_x_c2r(Ctx& c, const Batch& B, const PassArgs& A) {
+KW_BODY void body_x_c2r(Ctx& cb, const Batch& B, const PassArgs& A) {
   int tile;
-  const ProbDev& P = batch_prob(c, B, tile);
-  const XTile T = x_tile(c, P, tile);
-  const int N = P.Nx;
+  const ProbDev& P = batch_prob(cb, B, tile);
+  const XTile T = x_tile<XS>(cb, P, tile);
+  PencilCtx<Ctx> c{cb, T.wave};
   const bool acc = A.flags & XC_ACCUM;
   for (int b = 0; b < A.nbuf; ++b) {
     float* f = A.real[b];
@@ -397,70 +407,90 @@ KW_BODY void body_x_c2r(Ctx& c, const Batch& B, const PassArgs& A) {
   }
 }
 
+// inverse x transform of the half spectrum S into this thread's last-stage
+// outputs gv[slot] (real parts: row r0, imaginary parts: row r1)
+template <class XS, class PC>
+KW_BODY void x_inverse(PC& c, const ProbDev& P, const XTile& T, const cf* S, cf* gv) {
+  HalfPairLoad<1> src{&P, {S}, T.h0, T.h1, T.q0, T.q1, T.has1};
+  auto dst = [&](int, int, cf v, int slot) { gv[slot] = v; };
+  KW_PHASE_FENCE();
+  XS::template fft<1, true>(c, P.fx, T.g, T.G, T.active, src, dst, T.lds);
+  KW_PHASE_FENCE();
+}
+
+// this thread's values (a: row r0, b: row r1) to LDS pencil 0, then the
+// forward x transform of the pencil into the half spectrum S
+template <class XS, class PC>
+KW_BODY void x_forward(PC& c, const ProbDev& P, const XTile& T, const float* av, const float* bv,
+                       cf* S) {
+  KW_PHASE_FENCE();
+  if (T.active) {
+    auto put = [&](int slot, int n) { T.lds.set(0, n, cf{av[slot], T.has1 ? bv[slot] : 0.f}); };
+    XS::last_slots(P.fx, T.g, T.G, T.active, put);
+  }
+  c.sync();
+  KW_PHASE_FENCE();
+  cf* Sa[1] = {S};
+  x_fwd_from_lds<XS, 1>(c, P, T, Sa);
+}
 
 // end of the gradient: u_d = pml_sg_d (pml_sg_d u_d - dt rho0_sg_d^-1 g_d)
 // (kspaceFirstOrder3D.m:792-803), velocity sources (:806-886), then
 // S[d] = fft_x(u_d) for the divergence.  XG_INIT: p0 initial condition
 // u_d = dt rho0_sg_d^-1 g_d / 2 (:1023-1037), no fft.
 template <class XS, int DIM, class Ctx>
-KW_BODY void body_x_grad(Ctx& c, const Batch& B, const PassArgs& A) {
+KW_BODY void body_x_grad(Ctx& cb, const Batch& B, const PassArgs& A) {
   int tile;
-  const ProbDev& P = batch_prob(c, B, tile);
-  const XTile T = x_tile(c, P, tile);
-  const int N = P.Nx;
+  const ProbDev& P = batch_prob(cb, B, tile);
+  const XTile T = x_tile<XS>(cb, P, tile);
+  PencilCtx<Ctx> c{cb, T.wave};
   const int t = KW_LD(P.t_index);
   const bool init = A.flags & XG_INIT;
   const float dt = P.dt;
-  cf gv[DIM][kMaxSlots];
-  HalfPairLoad<DIM> src{&P, {}, T.h0, T.h1, T.q0, T.q1, T.has1};
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) src.S[d] = P.S[d];
-  auto dst = [&](int d, int, cf v, int slot) { gv[d][slot] = v; };
-  XS::template fft<DIM, true>(c, P.fx, T.g, T.G, T.active, src, dst, T.lds);
-#pragma unroll
+  // one component at a time: inverse transform, update, forward transform
+#pragma unroll 1
   for (int d = 0; d < DIM; ++d) {
+    cf gv[kMaxSlots];
+    x_inverse<XS>(c, P, T, P.S[d], gv);
     float* u = P.u[d];
     const Field ri = P.rinv[d];
     const float* pv = P.pml_sg[d];
+    float av[kMaxSlots], bv[kMaxSlots];
     auto run = [&](auto INIT, auto FORM) {
-      KW_PHASE_FENCE();
-      float av[kMaxSlots], bv[kMaxSlots];
+      const float s0 = src_value0<decltype(FORM)::value>(P.srcu[d], t);
       auto upd = [&](int slot, int n) {
         const uint32_t c0 = T.c0 + n, c1 = T.c1 + n;
         float a, b;
         if constexpr (INIT.value) {
-          a = dt * ri.at(c0) * gv[d][slot].x / 2.f;
-          b = dt * ri.at(c1) * gv[d][slot].y / 2.f;
+          a = dt * ri.at(c0) * gv[slot].x / 2.f;
+          b = dt * ri.at(c1) * gv[slot].y / 2.f;
         } else {
           const float p0 = pml_at(pv, T, d, 0, n), p1 = pml_at(pv, T, d, 1, n);
-          a = p0 * (p0 * KW_LD32(u, c0) - dt * ri.at(c0) * gv[d][slot].x);
-          b = p1 * (p1 * KW_LD32(u, c1) - dt * ri.at(c1) * gv[d][slot].y);
+          a = p0 * (p0 * KW_LD32(u, c0) - dt * ri.at(c0) * gv[slot].x);
+          b = p1 * (p1 * KW_LD32(u, c1) - dt * ri.at(c1) * gv[slot].y);
           if constexpr (decltype(FORM)::value >= 0) {
-            a = apply_src<decltype(FORM)::value>(P.srcu[d], c0, a, t);
-            b = apply_src<decltype(FORM)::value>(P.srcu[d], c1, b, t);
+            a = apply_src<decltype(FORM)::value>(P.srcu[d], c0, a, t, s0);
+            b = apply_src<decltype(FORM)::value>(P.srcu[d], c1, b, t, s0);
           }
         }
         av[slot] = a;
         bv[slot] = b;
       };
       XS::last_slots(P.fx, T.g, T.G, T.active, upd);
-      if (T.active) {
-        auto put = [&](int slot, int n) {
-          KW_ST32(u, T.c1 + n, bv[slot]);
-          KW_ST32(u, T.c0 + n, av[slot]);
-          T.lds.set(d, n, cf{av[slot], T.has1 ? bv[slot] : 0.f});
-        };
-        XS::last_slots(P.fx, T.g, T.G, T.active, put);
-      }
     };
     if (init)
       run(BoolC<true>{}, IntC<-1>{});
     else
       with_src(P.srcu[d], P.srcu[d].len >= t, [&](auto FORM) { run(BoolC<false>{}, FORM); });
-  }
-  if (!init) {
-    c.sync();
-    x_fwd_from_lds<XS, DIM>(c, P, T, P.S);
+    KW_PHASE_FENCE();
+    if (T.active) {
+      auto put = [&](int slot, int n) {
+        KW_ST32(u, T.c1 + n, bv[slot]);
+        KW_ST32(u, T.c0 + n, av[slot]);
+      };
+      XS::last_slots(P.fx, T.g, T.G, T.active, put);
+    }
+    if (!init) x_forward<XS>(c, P, T, av, bv, P.S[d]);
   }
 }
 
@@ -602,30 +632,25 @@ KW_BODY void x_finish_p(Ctx& c, const ProbDev& P, const XTile& T, cf* pv, int t,
 // into S[0], or (absorbing) S[0] = fft_x(rho0 * sum du), S[1] = fft_x(sum rho)
 // and rsum = sum rho
 template <class XS, int DIM, class Ctx>
-KW_BODY void body_x_div(Ctx& c, const Batch& B, const PassArgs& A) {
+KW_BODY void body_x_div(Ctx& cb, const Batch& B, const PassArgs& A) {
   int tile;
-  const ProbDev& P = batch_prob(c, B, tile);
-  const XTile T = x_tile(c, P, tile);
-  const int N = P.Nx;
+  const ProbDev& P = batch_prob(cb, B, tile);
+  const XTile T = x_tile<XS>(cb, P, tile);
+  PencilCtx<Ctx> c{cb, T.wave};
   const int t = KW_LD(P.t_index);
   const float dt = P.dt;
   const bool psrc = P.srcp.len >= t;
   const bool store_p = A.flags & X_STORE_P;
-  cf gv[DIM][kMaxSlots];
-  HalfPairLoad<DIM> src{&P, {}, T.h0, T.h1, T.q0, T.q1, T.has1};
-#pragma unroll
-  for (int d = 0; d < DIM; ++d) src.S[d] = P.S[d];
-  auto dst = [&](int d, int, cf v, int slot) { gv[d][slot] = v; };
-  XS::template fft<DIM, true>(c, P.fx, T.g, T.G, T.active, src, dst, T.lds);
 
-  cf sdu[kMaxSlots], srho[kMaxSlots], coef[kMaxSlots];
-  // mass conservation coefficient: rho0 (linear) or 2 sum(rho) + rho0
-  // (nonlinear, from the densities before this update)
+  // sums over the components: velocity divergence (absorbing / stokes EOS)
+  // and density; mass conservation coefficient rho0 (linear) or
+  // 2 sum(rho) + rho0 (nonlinear, from the densities before this update)
+  float du0[kMaxSlots], du1[kMaxSlots], rs0[kMaxSlots], rs1[kMaxSlots];
+  float k0[kMaxSlots], k1[kMaxSlots];
   with_flag(P.nonlinear != 0, [&](auto NL) {
     auto f = [&](int slot, int n) {
       const uint32_t c0 = T.c0 + n, c1 = T.c1 + n;
-      sdu[slot] = cf{0.f, 0.f};
-      srho[slot] = cf{0.f, 0.f};
+      du0[slot] = du1[slot] = rs0[slot] = rs1[slot] = 0.f;
       float a = P.rho0.at(c0), b = P.rho0.at(c1);
       if constexpr (NL.value) {
         float s0 = KW_LD32(P.rho[0], c0) + KW_LD32(P.rho[1], c0);
@@ -637,74 +662,78 @@ KW_BODY void body_x_div(Ctx& c, const Batch& B, const PassArgs& A) {
         a = 2.f * s0 + a;
         b = 2.f * s1 + b;
       }
-      coef[slot] = cf{a, b};
+      k0[slot] = a;
+      k1[slot] = b;
     };
     XS::last_slots(P.fx, T.g, T.G, T.active, f);
   });
-#pragma unroll
+#pragma unroll 1
   for (int d = 0; d < DIM; ++d) {
+    cf gv[kMaxSlots];
+    x_inverse<XS>(c, P, T, P.S[d], gv);
     float* rho = P.rho[d];
     const float* pv = P.pml[d];
+    float av[kMaxSlots], bv[kMaxSlots];
     with_src(P.srcp, psrc, [&](auto FORM) {
-      KW_PHASE_FENCE();
-      float av[kMaxSlots], bv[kMaxSlots];
+      const float s0 = src_value0<decltype(FORM)::value>(P.srcp, t);
       auto f = [&](int slot, int n) {
         const uint32_t c0 = T.c0 + n, c1 = T.c1 + n;
         const float p0 = pml_at(pv, T, d, 0, n), p1 = pml_at(pv, T, d, 1, n);
-        const cf g = gv[d][slot];
-        float a = p0 * (p0 * KW_LD32(rho, c0) - dt * coef[slot].x * g.x);
-        float b = p1 * (p1 * KW_LD32(rho, c1) - dt * coef[slot].y * g.y);
+        const cf g = gv[slot];
+        float a = p0 * (p0 * KW_LD32(rho, c0) - dt * k0[slot] * g.x);
+        float b = p1 * (p1 * KW_LD32(rho, c1) - dt * k1[slot] * g.y);
         if constexpr (decltype(FORM)::value >= 0) {
-          a = apply_src<decltype(FORM)::value>(P.srcp, c0, a, t);
-          b = apply_src<decltype(FORM)::value>(P.srcp, c1, b, t);
+          a = apply_src<decltype(FORM)::value>(P.srcp, c0, a, t, s0);
+          b = apply_src<decltype(FORM)::value>(P.srcp, c1, b, t, s0);
         }
         av[slot] = a;
         bv[slot] = b;
-        sdu[slot] = cf{sdu[slot].x + g.x, sdu[slot].y + g.y};
-        srho[slot] = cf{srho[slot].x + a, srho[slot].y + b};
+        du0[slot] = du0[slot] + g.x;
+        du1[slot] = du1[slot] + g.y;
+        rs0[slot] = rs0[slot] + a;
+        rs1[slot] = rs1[slot] + b;
       };
       XS::last_slots(P.fx, T.g, T.G, T.active, f);
-      if (T.active) {
-        auto put = [&](int slot, int n) {
-          KW_ST32(rho, T.c1 + n, bv[slot]);
-          KW_ST32(rho, T.c0 + n, av[slot]);
-        };
-        XS::last_slots(P.fx, T.g, T.G, T.active, put);
-      }
     });
+    KW_PHASE_FENCE();
+    if (T.active) {
+      auto put = [&](int slot, int n) {
+        KW_ST32(rho, T.c1 + n, bv[slot]);
+        KW_ST32(rho, T.c0 + n, av[slot]);
+      };
+      XS::last_slots(P.fx, T.g, T.G, T.active, put);
+    }
   }
   if (P.eos == EOS_ABSORBING) {
-    KW_PHASE_FENCE();
     float av[kMaxSlots], bv[kMaxSlots];
     auto ab = [&](int slot, int n) {
-      av[slot] = P.rho0.at(T.c0 + n) * sdu[slot].x;
-      bv[slot] = P.rho0.at(T.c1 + n) * sdu[slot].y;
+      av[slot] = P.rho0.at(T.c0 + n) * du0[slot];
+      bv[slot] = P.rho0.at(T.c1 + n) * du1[slot];
     };
     XS::last_slots(P.fx, T.g, T.G, T.active, ab);
     if (T.active) {
-      auto put_ab = [&](int slot, int n) {
-        KW_ST32(P.rsum, T.c1 + n, srho[slot].y);
-        KW_ST32(P.rsum, T.c0 + n, srho[slot].x);
-        T.lds.set(0, n, cf{av[slot], T.has1 ? bv[slot] : 0.f});
-        T.lds.set(1, n, cf{srho[slot].x, T.has1 ? srho[slot].y : 0.f});
+      auto put = [&](int slot, int n) {
+        KW_ST32(P.rsum, T.c1 + n, rs1[slot]);
+        KW_ST32(P.rsum, T.c0 + n, rs0[slot]);
       };
-      XS::last_slots(P.fx, T.g, T.G, T.active, put_ab);
+      XS::last_slots(P.fx, T.g, T.G, T.active, put);
     }
-    c.sync();
-    x_fwd_from_lds<XS, 2>(c, P, T, P.S);
+    x_forward<XS>(c, P, T, av, bv, P.S[0]);
+    x_forward<XS>(c, P, T, rs0, rs1, P.S[1]);
   } else {
     cf pv[kMaxSlots];
     with_eos(P, [&](auto EOS, auto NL) {
       auto f = [&](int slot, int n) {
         constexpr int E = decltype(EOS)::value;
         constexpr bool L = decltype(NL)::value;
-        pv[slot].x = eos_p<E, L>(P, T.c0 + n, srho[slot].x, sdu[slot].x, 0.f, 0.f);
-        pv[slot].y = eos_p<E, L>(P, T.c1 + n, srho[slot].y, sdu[slot].y, 0.f, 0.f);
+        pv[slot].x = eos_p<E, L>(P, T.c0 + n, rs0[slot], du0[slot], 0.f, 0.f);
+        pv[slot].y = eos_p<E, L>(P, T.c1 + n, rs1[slot], du1[slot], 0.f, 0.f);
       };
       XS::last_slots(P.fx, T.g, T.G, T.active, f);
     });
     x_finish_p<XS>(c, P, T, pv, t, store_p);
-    x_fwd_from_lds<XS, 1>(c, P, T, P.S);
+    cf* S0[1] = {P.S[0]};
+    x_fwd_from_lds<XS, 1>(c, P, T, S0);
   }
 }
 
@@ -712,31 +741,53 @@ KW_BODY void body_x_div(Ctx& c, const Batch& B, const PassArgs& A) {
 // with A = ifft(nabla1 fft(rho0 sum du)), B = ifft(nabla2 fft(rsum)); sensors;
 // S[0] = fft_x(p)
 template <class XS, class Ctx>
-KW_BODY void body_x_eos(Ctx& c, const Batch& B, const PassArgs& A) {
+KW_BODY void body_x_eos(Ctx& cb, const Batch& B, const PassArgs& A) {
   int tile;
-  const ProbDev& P = batch_prob(c, B, tile);
-  const XTile T = x_tile(c, P, tile);
-  const int N = P.Nx;
+  const ProbDev& P = batch_prob(cb, B, tile);
+  const XTile T = x_tile<XS>(cb, P, tile);
+  PencilCtx<Ctx> c{cb, T.wave};
   const int t = KW_LD(P.t_index);
   const bool store_p = A.flags & X_STORE_P;
-  cf gv[2][kMaxSlots];
-  HalfPairLoad<2> src{&P, {P.S[0], P.S[1]}, T.h0, T.h1, T.q0, T.q1, T.has1};
-  auto dst = [&](int j, int, cf v, int slot) { gv[j][slot] = v; };
-  XS::template fft<2, true>(c, P.fx, T.g, T.G, T.active, src, dst, T.lds);
-  cf pv[kMaxSlots];
-  with_flag(P.nonlinear != 0, [&](auto NL) {
+  // rsum + tau A first, then - eta B and the nonlinear term
+  cf part[kMaxSlots], rs[kMaxSlots];
+  {
+    cf gv[kMaxSlots];
+    x_inverse<XS>(c, P, T, P.S[0], gv);
     auto f = [&](int slot, int n) {
-      constexpr bool L = decltype(NL)::value;
       const uint32_t c0 = T.c0 + n, c1 = T.c1 + n;
-      pv[slot].x = eos_p<EOS_ABSORBING, L>(P, c0, KW_LD32(P.rsum, c0), 0.f, gv[0][slot].x,
-                                           gv[1][slot].x);
-      pv[slot].y = eos_p<EOS_ABSORBING, L>(P, c1, KW_LD32(P.rsum, c1), 0.f, gv[0][slot].y,
-                                           gv[1][slot].y);
+      rs[slot] = cf{KW_LD32(P.rsum, c0), KW_LD32(P.rsum, c1)};
+      part[slot] = cf{rs[slot].x + P.tau.at(c0) * gv[slot].x, rs[slot].y + P.tau.at(c1) * gv[slot].y};
     };
     XS::last_slots(P.fx, T.g, T.G, T.active, f);
-  });
+  }
+  cf pv[kMaxSlots];
+  {
+    cf gv[kMaxSlots];
+    x_inverse<XS>(c, P, T, P.S[1], gv);
+    with_flag(P.nonlinear != 0, [&](auto NL) {
+      auto f = [&](int slot, int n) {
+        const uint32_t cells[2] = {T.c0 + n, T.c1 + n};
+        const float pa[2] = {part[slot].x, part[slot].y};
+        const float ra[2] = {rs[slot].x, rs[slot].y};
+        const float ga[2] = {gv[slot].x, gv[slot].y};
+        float out[2];
+#pragma unroll
+        for (int h = 0; h < 2; ++h) {
+          const uint32_t cell = cells[h];
+          float inner = pa[h] - P.eta.at(cell) * ga[h];
+          if constexpr (decltype(NL)::value)
+            inner = inner + P.bona.at(cell) * (ra[h] * ra[h]) / (2.f * P.rho0.at(cell));
+          const float c0 = P.c0.at(cell);
+          out[h] = (c0 * c0) * inner;
+        }
+        pv[slot] = cf{out[0], out[1]};
+      };
+      XS::last_slots(P.fx, T.g, T.G, T.active, f);
+    });
+  }
   x_finish_p<XS>(c, P, T, pv, t, store_p);
-  x_fwd_from_lds<XS, 1>(c, P, T, P.S);
+  cf* S0[1] = {P.S[0]};
+  x_fwd_from_lds<XS, 1>(c, P, T, S0);
 }
 
 // ---------------------------------------------------------------------------
