@@ -194,6 +194,70 @@ int kw_sim_step_timed(kw_sim* sim, int64_t nsteps, double* ms, kw_kernel_stats* 
 /* Name of kernel kind k (0 <= k < KW_NKINDS). */
 const char* kw_kind_name(int k);
 
+/* ---------------------------------------------------------------------------
+ * Engine: asynchronous submission of many simulations, stepped together.
+ *
+ * Replaces the one-simulation-per-call contract of kspaceFirstOrder3D.m /
+ * kspaceFirstOrder3DC.m:441-546 for callers that run many simulations
+ * (batches of forward runs and time-reversal reconstructions, request
+ * streams).  kw_submit copies the problem (every array) into the engine
+ * before it returns, so the caller's buffers may be freed at once.  An
+ * admission thread moves queued simulations to the device while their
+ * estimated device footprint fits the engine's budget; a stepping thread
+ * advances all admitted simulations together, one launch per pass for all
+ * simulations with the same dimension and equation of state, whatever their
+ * grid sizes, media, sources, sensors and current step.  Results are the
+ * same as kw_run on the same problem (the per-simulation arithmetic does
+ * not change).  Every engine function is thread-safe.
+ * ------------------------------------------------------------------------- */
+typedef struct kw_engine kw_engine;
+
+typedef struct kw_engine_config {
+  int64_t device_bytes;    /* device memory the engine may hold for simulations;
+                              0: 90% of the free memory at creation */
+  int32_t steps_per_round; /* steps between admission/retirement decisions; 0: 32 */
+} kw_engine_config;
+
+#define KW_TICKET_QUEUED 0
+#define KW_TICKET_RUNNING 1
+#define KW_TICKET_DONE 2
+#define KW_TICKET_FAILED 3
+#define KW_TICKET_CANCELLED 4
+
+typedef struct kw_ticket_info {
+  int32_t status;          /* KW_TICKET_* */
+  int32_t error;           /* KW_E* code when FAILED */
+  double submit_s;         /* engine clock (seconds since creation) */
+  double start_s;          /* admitted to the device (-1 before) */
+  double done_s;           /* finished, failed or cancelled (-1 before) */
+  int64_t steps_done;
+  int64_t steps_total;
+  int64_t grid_cells;
+  int64_t device_bytes;    /* held on the device while running */
+} kw_ticket_info;
+
+typedef struct kw_engine_stats {
+  int64_t submitted, completed, failed, cancelled, queued, running;
+  int64_t device_bytes;    /* held now */
+  int64_t device_budget;
+  double cell_steps;       /* grid cells x steps executed */
+  double busy_s;           /* time with at least one simulation running */
+  int64_t launches;        /* kernel launches issued by the stepping thread */
+} kw_engine_stats;
+
+int kw_engine_create(const kw_engine_config* cfg, kw_engine** engine, char* err, size_t errlen);
+int kw_submit(kw_engine* engine, const kw_problem* prob, int64_t* ticket, char* err,
+              size_t errlen);
+int kw_poll(kw_engine* engine, int64_t ticket, kw_ticket_info* info, char* err, size_t errlen);
+/* Blocks until the ticket has finished, copies its outputs (as kw_run) into
+ * `out` (NULL: discard them) and releases the ticket.  Returns the
+ * simulation's error code if it failed, KW_ESTATE if it was cancelled. */
+int kw_wait(kw_engine* engine, int64_t ticket, kw_outputs* out, char* err, size_t errlen);
+int kw_cancel(kw_engine* engine, int64_t ticket, char* err, size_t errlen);
+int kw_engine_get_stats(kw_engine* engine, kw_engine_stats* stats, char* err, size_t errlen);
+/* Cancels queued and running simulations, stops the threads, frees all. */
+int kw_engine_destroy(kw_engine* engine);
+
 /* Smoothing filter: out = real(ifftn(fftn(in) .* ifftshift(win))) on a 2D or
  * 3D grid, computed on the device.  `win` is the centred window of
  * getWin (not shifted).  Replaces the transform of +kwave/+legacy/smooth.m;

@@ -41,4 +41,7 @@ double dev_event_elapsed_ms(void* a, void* b);  // synchronises on b
 // largest dynamic LDS per block the kernels may request
 size_t dev_max_lds();
 
+// free and total device memory in bytes
+void dev_mem_info(size_t* free_bytes, size_t* total_bytes);
+
 }  // namespace kw

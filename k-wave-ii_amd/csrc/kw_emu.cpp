@@ -32,6 +32,7 @@
 #include <functional>
 #include <map>
 #include <mutex>
+#include <shared_mutex>
 #include <thread>
 #include <vector>
 
@@ -41,7 +42,10 @@
 namespace kw {
 
 namespace {
-std::mutex g_mu;
+// the allocation table is read by every emulated access and written by
+// allocations, which may come from another host thread (the engine admits
+// simulations while it steps others)
+std::shared_mutex g_mu;
 std::map<uintptr_t, size_t> g_allocs;
 std::atomic<int> g_fault{0};
 char g_fault_msg[512];
@@ -50,11 +54,15 @@ alignas(64) char g_scratch[256];
 
 const void* emu_resolve(const void* p, size_t bytes, const char* what) {
   const uintptr_t a = (uintptr_t)p;
-  auto it = g_allocs.upper_bound(a);
-  bool ok = it != g_allocs.begin();
-  if (ok) {
-    --it;
-    ok = a >= it->first && a + bytes <= it->first + it->second;
+  bool ok;
+  {
+    std::shared_lock<std::shared_mutex> lk(g_mu);
+    auto it = g_allocs.upper_bound(a);
+    ok = it != g_allocs.begin();
+    if (ok) {
+      --it;
+      ok = a >= it->first && a + bytes <= it->first + it->second;
+    }
   }
   if (!ok) {
     if (g_fault.exchange(1) == 0)
@@ -76,7 +84,7 @@ void* dev_alloc(size_t bytes) {
   const size_t n = bytes ? bytes : 1;
   void* p = std::aligned_alloc(256, (n + 255) / 256 * 256);
   if (!p) throw DeviceError(-4, "emulator: allocation failed");
-  std::lock_guard<std::mutex> lk(g_mu);
+  std::unique_lock<std::shared_mutex> lk(g_mu);
   g_allocs[(uintptr_t)p] = n;
   return p;
 }
@@ -84,7 +92,7 @@ void* dev_alloc(size_t bytes) {
 void dev_free(void* p) {
   if (!p) return;
   {
-    std::lock_guard<std::mutex> lk(g_mu);
+    std::unique_lock<std::shared_mutex> lk(g_mu);
     g_allocs.erase((uintptr_t)p);
   }
   std::free(p);
@@ -106,6 +114,18 @@ void dev_zero(void* dst, size_t bytes, void*) {
   std::memset(dst, 0, bytes);
 }
 size_t dev_max_lds() { return 65536; }
+
+// the emulator's "device" is host memory; report a fixed 16 GiB
+void dev_mem_info(size_t* free_bytes, size_t* total_bytes) {
+  size_t used = 0;
+  {
+    std::shared_lock<std::shared_mutex> lk(g_mu);
+    for (auto& a : g_allocs) used += a.second;
+  }
+  const size_t total = (size_t)16 << 30;
+  if (free_bytes) *free_bytes = used < total ? total - used : 0;
+  if (total_bytes) *total_bytes = total;
+}
 
 namespace {
 

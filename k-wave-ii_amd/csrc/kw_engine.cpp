@@ -1,0 +1,817 @@
+// Engine: many simulations on one GPU (kw_engine_* / kw_submit / kw_wait in
+// include/kwhip.h).
+//
+// An admission thread takes submitted problems in order and moves each to
+// the device (operators, uploads, first step) on its own stream while the
+// estimated device footprint of everything admitted fits the engine's
+// budget.  A stepping thread keeps the admitted simulations in groups (same
+// dimension and equation-of-state pass sequence) and advances each group
+// with one launch per pass for all of its members: every block finds its
+// problem in a per-pass tile table, each problem keeps its own FFT plans,
+// tiling, medium, sources and sensors, and every per-step decision is taken
+// on the device from the problem's own step counter, so members may differ
+// in size and be at different steps.  Finished simulations are read back to
+// host staging buffers and their device memory is released, which admits
+// the next ones.
+#include <chrono>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <mutex>
+
+#include "kw_sim.h"
+
+namespace kw {
+namespace {
+
+using Clock = std::chrono::steady_clock;
+
+// ---------------------------------------------------------------------------
+// deep copy of a submitted problem
+// ---------------------------------------------------------------------------
+struct ProblemCopy {
+  kw_problem pb{};
+  std::vector<double> c0, rho0, rho0_sg[3], bona, alpha, p0, tr;
+  std::vector<uint64_t> sens;
+  struct Src {
+    std::vector<uint64_t> index, sig_index;
+    std::vector<double> input;
+  } src[4];
+
+  explicit ProblemCopy(const kw_problem& in) : pb(in) {
+    const int64_t n = in.Nx * in.Ny * (in.dim == 3 ? in.Nz : 1);
+    auto grid = [&](const double* a, std::vector<double>& v) -> const double* {
+      if (!a) return nullptr;
+      v.assign(a, a + n);
+      return v.data();
+    };
+    pb.c0 = grid(in.c0, c0);
+    pb.rho0 = grid(in.rho0, rho0);
+    for (int d = 0; d < 3; ++d) pb.rho0_sg[d] = grid(in.rho0_sg[d], rho0_sg[d]);
+    pb.BonA = grid(in.BonA, bona);
+    pb.alpha_coeff = grid(in.alpha_coeff, alpha);
+    pb.p0 = grid(in.p0, p0);
+    pb.sensor_mask_index = nullptr;
+    if (in.sensor_mask_index && in.sensor_num_points > 0) {
+      sens.assign(in.sensor_mask_index, in.sensor_mask_index + in.sensor_num_points);
+      pb.sensor_mask_index = sens.data();
+    }
+    pb.time_reversal_boundary_data = nullptr;
+    if (in.time_reversal_boundary_data && in.sensor_num_points > 0) {
+      tr.assign(in.time_reversal_boundary_data,
+                in.time_reversal_boundary_data + in.sensor_num_points * in.Nt);
+      pb.time_reversal_boundary_data = tr.data();
+    }
+    copy_src(in.p_source, pb.p_source, src[0]);
+    for (int d = 0; d < 3; ++d) copy_src(in.u_source[d], pb.u_source[d], src[1 + d]);
+  }
+
+  static void copy_src(const kw_time_source& a, kw_time_source& b, Src& s) {
+    b = a;
+    b.index = nullptr;
+    b.sig_index = nullptr;
+    b.input = nullptr;
+    if (a.flag <= 0) return;
+    if (a.index && a.num_points > 0) {
+      s.index.assign(a.index, a.index + a.num_points);
+      b.index = s.index.data();
+    }
+    if (a.sig_index && a.num_points > 0) {
+      s.sig_index.assign(a.sig_index, a.sig_index + a.num_points);
+      b.sig_index = s.sig_index.data();
+    }
+    if (a.input && a.num_series > 0) {
+      s.input.assign(a.input, a.input + a.num_series * a.flag);
+      b.input = s.input.data();
+    }
+  }
+};
+
+// the sizes kw_submit must trust to copy the arrays
+void check_shape(const kw_problem* p) {
+  if (!p) fail(KW_EINVAL, "problem is NULL");
+  if (p->dim != 2 && p->dim != 3) fail(KW_EINVAL, "dim must be 2 or 3");
+  if (p->Nx < 2 || p->Ny < 2 || (p->dim == 3 && p->Nz < 2) || (p->dim == 2 && p->Nz != 1))
+    fail(KW_EINVAL, "invalid grid size");
+  if ((double)p->Nx * (double)p->Ny * (double)(p->dim == 3 ? p->Nz : 1) > 4.0e9)
+    fail(KW_EUNSUP, "grid too large");
+  if (p->Nt < 1) fail(KW_EINVAL, "Nt must be positive");
+  if (p->sensor_num_points < 0) fail(KW_EINVAL, "negative number of sensor points");
+  const kw_time_source* s[4] = {&p->p_source, &p->u_source[0], &p->u_source[1], &p->u_source[2]};
+  for (auto* t : s)
+    if (t->flag > 0 && (t->num_points < 0 || t->num_series < 0))
+      fail(KW_EINVAL, "invalid source sizes");
+}
+
+// device bytes a problem will hold (the allocations of create_sim, with
+// margin); used to admit simulations before they are created
+int64_t estimate_bytes(const kw_problem& p) {
+  const Geometry G = make_geometry(p.dim, p.Nx, p.Ny, p.Nz);
+  const double nc = (double)G.ncell, ns = (double)G.nspec;
+  const int dim = p.dim;
+  double b = 4.0 * nc * (2 + 2 * dim) + 8.0 * ns * 3 + 4.0 * ns * 3;
+  auto grid = [&](const void* a) { return a ? 4.0 * nc : 0.0; };
+  b += grid(p.c0) + grid(p.rho0) + grid(p.BonA) + grid(p.p0);
+  if (p.rho0 || p.rho0_sg[0]) b += 4.0 * nc * dim;
+  if (p.absorbing_flag && (p.c0 || p.alpha_coeff)) b += 8.0 * nc;
+  const kw_time_source* srcs[4] = {&p.p_source, &p.u_source[0], &p.u_source[1], &p.u_source[2]};
+  for (auto* s : srcs) {
+    if (s->flag <= 0) continue;
+    b += 8.0 * nc + 4.0 * (double)s->num_series * (double)std::min<int64_t>(s->flag, p.Nt) +
+         8.0 * (double)s->num_points;
+    if (s->mode == KW_SRC_ADDITIVE) b += 4.0 * ns;
+  }
+  const double nsens = (double)p.sensor_num_points;
+  if (p.time_reversal_boundary_data) {
+    b += 4.0 * nc + 4.0 * nsens * (double)p.Nt;
+  } else if (nsens > 0) {
+    const int64_t rs = p.record_start_index > 0 ? p.record_start_index : 1;
+    const double nrec = (double)std::max<int64_t>(0, p.Nt - rs + 1);
+    const uint32_t r = p.record_flags;
+    if (r & KW_REC_P) b += 4.0 * nsens * nrec;
+    if (r & KW_REC_U) b += 4.0 * nsens * nrec * dim;
+    for (uint32_t f : {KW_REC_P_MAX, KW_REC_P_MIN, KW_REC_P_RMS})
+      if (r & f) b += 8.0 * nsens;
+    for (uint32_t f : {KW_REC_U_MAX, KW_REC_U_MIN, KW_REC_U_RMS})
+      if (r & f) b += 8.0 * nsens * dim;
+    for (uint32_t f : {KW_REC_P_MAX_ALL, KW_REC_P_MIN_ALL})
+      if (r & f) b += 8.0 * nc;
+    for (uint32_t f : {KW_REC_U_MAX_ALL, KW_REC_U_MIN_ALL})
+      if (r & f) b += 8.0 * nc * dim;
+    b += 4.0 * nsens;
+  }
+  return (int64_t)(b * 1.1) + ((int64_t)4 << 20);
+}
+
+// ---------------------------------------------------------------------------
+// host staging of a finished simulation's outputs
+// ---------------------------------------------------------------------------
+struct OutputStage {
+  std::vector<float> p, p_final, u[3], u_final[3];
+  std::vector<double> p_max, p_min, p_rms, p_max_all, p_min_all;
+  std::vector<double> u_max[3], u_min[3], u_rms[3], u_max_all[3], u_min_all[3];
+
+  void read(Sim& S) {
+    const int64_t ns = S.nsens, nc = S.G.ncell, nr = S.n_rec;
+    const uint32_t r = S.record;
+    kw_outputs o{};
+    auto f32 = [](std::vector<float>& v, int64_t n, bool on) -> float* {
+      if (!on) return nullptr;
+      v.assign(n, 0.f);
+      return v.data();
+    };
+    auto f64 = [](std::vector<double>& v, int64_t n, bool on) -> double* {
+      if (!on) return nullptr;
+      v.assign(n, 0.0);
+      return v.data();
+    };
+    o.p = f32(p, ns * nr, (r & KW_REC_P) && ns > 0);
+    o.p_max = f64(p_max, ns, (r & KW_REC_P_MAX) && ns > 0);
+    o.p_min = f64(p_min, ns, (r & KW_REC_P_MIN) && ns > 0);
+    o.p_rms = f64(p_rms, ns, (r & KW_REC_P_RMS) && ns > 0);
+    o.p_max_all = f64(p_max_all, nc, r & KW_REC_P_MAX_ALL);
+    o.p_min_all = f64(p_min_all, nc, r & KW_REC_P_MIN_ALL);
+    o.p_final = f32(p_final, nc, r & KW_REC_P_FINAL);
+    for (int a = 0; a < S.G.dim; ++a) {
+      o.u[a] = f32(u[a], ns * nr, (r & KW_REC_U) && ns > 0);
+      o.u_max[a] = f64(u_max[a], ns, (r & KW_REC_U_MAX) && ns > 0);
+      o.u_min[a] = f64(u_min[a], ns, (r & KW_REC_U_MIN) && ns > 0);
+      o.u_rms[a] = f64(u_rms[a], ns, (r & KW_REC_U_RMS) && ns > 0);
+      o.u_max_all[a] = f64(u_max_all[a], nc, r & KW_REC_U_MAX_ALL);
+      o.u_min_all[a] = f64(u_min_all[a], nc, r & KW_REC_U_MIN_ALL);
+      o.u_final[a] = f32(u_final[a], nc, r & KW_REC_U_FINAL);
+    }
+    read_outputs(S, &o);
+  }
+
+  // copy into the caller's buffers (NULL pointers and outputs the problem
+  // did not record are skipped, as kw_run does)
+  void copy_to(kw_outputs* o) const {
+    if (!o) return;
+    auto c = [](auto* dst, const auto& v) {
+      if (dst && !v.empty()) std::memcpy(dst, v.data(), v.size() * sizeof(v[0]));
+    };
+    c(o->p, p);
+    c(o->p_max, p_max);
+    c(o->p_min, p_min);
+    c(o->p_rms, p_rms);
+    c(o->p_max_all, p_max_all);
+    c(o->p_min_all, p_min_all);
+    c(o->p_final, p_final);
+    for (int a = 0; a < 3; ++a) {
+      c(o->u[a], u[a]);
+      c(o->u_max[a], u_max[a]);
+      c(o->u_min[a], u_min[a]);
+      c(o->u_rms[a], u_rms[a]);
+      c(o->u_max_all[a], u_max_all[a]);
+      c(o->u_min_all[a], u_min_all[a]);
+      c(o->u_final[a], u_final[a]);
+    }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// groups: one launch per pass for all members
+// ---------------------------------------------------------------------------
+enum GeoKind { GX = 0, GY, GL, GCELL, GSENS, GONE, NGEO };
+
+struct Group {
+  int dim = 0;
+  bool absorbing = false;
+  void* stream = nullptr;
+  std::vector<Sim*> sims;
+  bool dirty = true;
+  ProbDev* probs = nullptr;
+  int32_t* tiles[NGEO] = {};
+  LaunchGeom geo[NGEO];
+  int fixed[3] = {0, 0, 0};  // x, y, last-axis pass lengths (0: runtime plans)
+  bool any_points = false, any_grid = false;
+  std::vector<void*> live, retired;  // device tables of this / earlier memberships
+  int64_t launches = 0;
+
+  Group() = default;
+  Group(const Group&) = delete;
+  Group& operator=(const Group&) = delete;
+  ~Group() {
+    for (void* p : live) dev_free(p);
+    for (void* p : retired) dev_free(p);
+  }
+
+  // tables of earlier memberships may be read by launches still queued;
+  // free them only after the stream has been synchronised
+  void free_retired() {
+    for (void* p : retired) dev_free(p);
+    retired.clear();
+  }
+
+  static const LaunchGeom& geom_of(const Sim& S, int k) {
+    static const LaunchGeom one{1, EW_T, 0};
+    switch (k) {
+      case GX: return S.lx;
+      case GY: return S.ly;
+      case GL: return S.ll;
+      case GCELL: return S.lcell;
+      case GSENS: return S.lsens;
+      default: return one;
+    }
+  }
+
+  void rebuild() {
+    for (void* p : live) retired.push_back(p);
+    live.clear();
+    const int n = (int)sims.size();
+    if (n == 0) {
+      dirty = false;
+      return;
+    }
+    std::vector<ProbDev> hp(n);
+    for (int i = 0; i < n; ++i) hp[i] = sims[i]->P;
+    probs = (ProbDev*)dev_alloc(sizeof(ProbDev) * n);
+    live.push_back(probs);
+    dev_h2d(probs, hp.data(), sizeof(ProbDev) * n, stream);
+    for (int k = 0; k < NGEO; ++k) {
+      if (k == GY && dim != 3) {
+        tiles[k] = nullptr;
+        geo[k] = LaunchGeom{};
+        continue;
+      }
+      std::vector<int32_t> start(n);
+      int64_t nb = 0;
+      int T = 0;
+      for (int i = 0; i < n; ++i) {
+        const LaunchGeom& g = geom_of(*sims[i], k);
+        start[i] = (int32_t)nb;
+        nb += g.nb;
+        T = std::max(T, g.T);
+      }
+      if (nb >= INT32_MAX) fail(KW_EUNSUP, "too many blocks in one group launch");
+      // a block has T threads for every member: an x pass member gives each
+      // of its T / Gx pencil slots its own LDS pencil; the strided and
+      // last-axis passes keep their own pencil count
+      size_t lds = 0;
+      for (int i = 0; i < n; ++i) {
+        const Sim& S = *sims[i];
+        size_t l = geom_of(S, k).lds;
+        if (k == GX)
+          l = sizeof(cf) * (size_t)kNPX * (size_t)x_lds_stride(S.G.Nx) *
+              (size_t)((T + S.P.Gx - 1) / S.P.Gx);
+        lds = std::max(lds, l);
+      }
+      if (lds > dev_max_lds()) fail(KW_EUNSUP, "group launch needs too much LDS");
+      geo[k] = LaunchGeom{(int)nb, T, lds};
+      tiles[k] = (int32_t*)dev_alloc(sizeof(int32_t) * n);
+      live.push_back(tiles[k]);
+      dev_h2d(tiles[k], start.data(), sizeof(int32_t) * n, stream);
+    }
+    auto common = [&](auto len) -> int {
+      const int N = len(*sims[0]);
+      for (Sim* s : sims)
+        if (len(*s) != N) return 0;
+      return (KW_IS_FIXED_SIZE(N) && !std::getenv("KW_NO_FIXED")) ? N : 0;
+    };
+    fixed[0] = common([](const Sim& s) { return s.G.Nx; });
+    fixed[1] = dim == 3 ? common([](const Sim& s) { return s.G.Ny; }) : 0;
+    fixed[2] = common([&](const Sim& s) { return dim == 3 ? s.G.Nz : s.G.Ny; });
+    any_points = any_grid = false;
+    for (Sim* s : sims) {
+      any_points = any_points || s->P.rec_points != 0;
+      any_grid = any_grid || s->P.rec_grid != 0;
+    }
+    dirty = false;
+  }
+
+  Batch batch(int k) const { return Batch{probs, tiles[k], (int32_t)sims.size()}; }
+
+  void pass(PassKind kind, const PassArgs& a) {
+    const int k = (kind == PK_YFWD || kind == PK_YINV) ? GY : (kind == PK_SPEC ? GL : GX);
+    const int fx = fixed[k == GX ? 0 : (k == GY ? 1 : 2)];
+    launch_pass(kind, fx, dim, geo[k].nb, geo[k].T, geo[k].lds, batch(k), a, stream);
+    ++launches;
+  }
+
+  void ew(const EwArgs& a, int k) {
+    launch_ew(geo[k].nb, batch(k), a, stream);
+    ++launches;
+  }
+};
+
+// one time step (t >= 2) of every member: the sequence of step_once for a
+// single simulation, with each pass launched once for the group
+void group_step(Group& g) {
+  for (Sim* s : g.sims) {
+    const int t = (int)(s->steps_done + 1);
+    if (s->p_grid) src_grid(*s, 0, t);
+    for (int a = 0; a < g.dim; ++a)
+      if (s->u_grid[a]) src_grid(*s, a + 1, t);
+  }
+  const int dim = g.dim;
+  if (dim == 3) g.pass(PK_YFWD, bufs(1, 0));
+  PassArgs gr{};
+  gr.mode = SM_GRAD;
+  g.pass(PK_SPEC, gr);
+  if (dim == 3) g.pass(PK_YINV, bufs(3));
+  g.pass(PK_X_GRAD, PassArgs{});
+  if (dim == 3) g.pass(PK_YFWD, bufs(3));
+  PassArgs dv{};
+  dv.mode = SM_DIV;
+  g.pass(PK_SPEC, dv);
+  if (dim == 3) g.pass(PK_YINV, bufs(3));
+  g.pass(PK_X_DIV, PassArgs{});
+  if (g.absorbing) {
+    if (dim == 3) g.pass(PK_YFWD, bufs(2));
+    PassArgs ab{};
+    ab.mode = SM_ABS;
+    g.pass(PK_SPEC, ab);
+    if (dim == 3) g.pass(PK_YINV, bufs(2));
+    g.pass(PK_X_EOS, PassArgs{});
+  }
+  if (g.any_points) {
+    EwArgs e{};
+    e.kind = EW_RECORD_POINTS;
+    e.rec = kRecAuto;
+    g.ew(e, GSENS);
+  }
+  if (g.any_grid) {
+    EwArgs e{};
+    e.kind = EW_RECORD_GRID;
+    e.rec = kRecAuto;
+    g.ew(e, GCELL);
+  }
+  EwArgs adv{};
+  adv.kind = EW_ADVANCE;
+  g.ew(adv, GONE);
+  for (Sim* s : g.sims) s->steps_done++;
+}
+
+// ---------------------------------------------------------------------------
+// tickets and the engine
+// ---------------------------------------------------------------------------
+struct Ticket {
+  int64_t id = 0;
+  std::unique_ptr<ProblemCopy> prob;
+  int32_t status = KW_TICKET_QUEUED;
+  int32_t error = 0;
+  std::string msg;
+  double submit_s = 0, start_s = -1, done_s = -1;
+  int64_t est_bytes = 0, bytes = 0, steps_total = 0, steps_done = 0, cells = 0;
+  bool cancel = false;
+  std::unique_ptr<Sim> sim;  // owned by the stepping thread while running
+  OutputStage out;
+};
+
+bool terminal(int32_t s) {
+  return s == KW_TICKET_DONE || s == KW_TICKET_FAILED || s == KW_TICKET_CANCELLED;
+}
+
+}  // namespace
+
+struct Engine {
+  std::mutex mu;
+  std::condition_variable cv_admit, cv_step, cv_done;
+  std::deque<Ticket*> pending;  // submitted, not yet admitted
+  std::vector<Ticket*> ready;   // on the device after their first step
+  std::map<int64_t, std::unique_ptr<Ticket>> tickets;
+  int64_t next_id = 1;
+  int64_t budget = 0, held = 0;  // device bytes: budget, held (estimates until created)
+  int32_t steps_per_round = 32;
+  bool stop = false;
+  void* step_stream = nullptr;
+  void* admit_stream = nullptr;
+  Clock::time_point t0 = Clock::now();
+  // statistics (under mu)
+  int64_t n_submitted = 0, n_completed = 0, n_failed = 0, n_cancelled = 0, n_running = 0;
+  double cell_steps = 0, busy_s = 0;
+  int64_t launches = 0;
+  std::thread admit_thr, step_thr;
+
+  double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+
+  void finish(Ticket* t, int32_t status, int32_t err, const std::string& m) {
+    // under mu
+    t->status = status;
+    t->error = err;
+    t->msg = m;
+    t->done_s = now();
+    if (status == KW_TICKET_DONE) ++n_completed;
+    if (status == KW_TICKET_FAILED) ++n_failed;
+    if (status == KW_TICKET_CANCELLED) ++n_cancelled;
+    cv_done.notify_all();
+  }
+
+  // -- admission ------------------------------------------------------------
+  void admit_loop() {
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+      cv_admit.wait(lk, [&]() {
+        if (stop) return true;
+        if (pending.empty()) return false;
+        const Ticket* t = pending.front();
+        // admit while the budget holds; always admit when nothing is held
+        return t->cancel || held == 0 || held + t->est_bytes <= budget;
+      });
+      if (stop) return;
+      Ticket* t = pending.front();
+      pending.pop_front();
+      if (t->cancel) {
+        finish(t, KW_TICKET_CANCELLED, KW_ESTATE, "cancelled");
+        continue;
+      }
+      held += t->est_bytes;
+      t->status = KW_TICKET_RUNNING;
+      t->start_s = now();
+      ++n_running;
+      lk.unlock();
+      std::unique_ptr<Sim> sim;
+      int code = KW_OK;
+      std::string msg;
+      try {
+        sim = create_sim(&t->prob->pb, admit_stream);
+        if (sim->Nt_eff > 0) step_once(*sim);
+        dev_sync(admit_stream);
+        sim->stream = step_stream;
+      } catch (const DeviceError& e) {
+        code = e.code;
+        msg = e.what();
+      } catch (const std::bad_alloc&) {
+        code = KW_ENOMEM;
+        msg = "host allocation failed";
+      } catch (const std::exception& e) {
+        code = KW_EDEVICE;
+        msg = e.what();
+      }
+      lk.lock();
+      if (code != KW_OK) {
+        sim.reset();
+        held -= t->est_bytes;
+        --n_running;
+        finish(t, KW_TICKET_FAILED, code, msg);
+        continue;
+      }
+      held += sim->bytes - t->est_bytes;
+      t->bytes = sim->bytes;
+      t->steps_total = sim->Nt_eff;
+      t->steps_done = sim->steps_done;
+      t->cells = sim->G.ncell;
+      t->prob.reset();  // the device holds everything now
+      t->sim = std::move(sim);
+      ready.push_back(t);
+      cv_step.notify_one();
+    }
+  }
+
+  // -- stepping -------------------------------------------------------------
+  void retire(Ticket* t, std::unique_lock<std::mutex>& lk, bool cancelled) {
+    // called without mu held for the device work
+    std::string msg;
+    int code = KW_OK;
+    if (!cancelled) {
+      try {
+        t->out.read(*t->sim);
+      } catch (const DeviceError& e) {
+        code = e.code;
+        msg = e.what();
+      }
+    } else {
+      try {
+        dev_sync(step_stream);
+      } catch (...) {
+      }
+    }
+    const int64_t b = t->bytes;
+    t->sim.reset();
+    lk.lock();
+    held -= b;
+    --n_running;
+    if (cancelled)
+      finish(t, KW_TICKET_CANCELLED, KW_ESTATE, "cancelled");
+    else if (code != KW_OK)
+      finish(t, KW_TICKET_FAILED, code, msg);
+    else
+      finish(t, KW_TICKET_DONE, KW_OK, "");
+    cv_admit.notify_one();
+    lk.unlock();
+  }
+
+  void step_loop() {
+    std::vector<Ticket*> active;
+    std::map<std::pair<int, int>, std::unique_ptr<Group>> groups;
+    std::unique_lock<std::mutex> lk(mu);
+    for (;;) {
+      cv_step.wait(lk, [&]() { return stop || !ready.empty() || !active.empty(); });
+      if (stop) break;
+      bool changed = !ready.empty();
+      for (Ticket* t : ready) active.push_back(t);
+      ready.clear();
+      // cancellations and simulations that finished in their first step
+      std::vector<Ticket*> drop, done;
+      for (Ticket* t : active) {
+        if (t->cancel)
+          drop.push_back(t);
+        else if (t->sim->steps_done >= t->sim->Nt_eff)
+          done.push_back(t);
+      }
+      lk.unlock();
+      auto remove = [&](Ticket* t) {
+        active.erase(std::find(active.begin(), active.end(), t));
+        changed = true;
+      };
+      for (Ticket* t : drop) {
+        remove(t);
+        retire(t, lk, true);
+      }
+      for (Ticket* t : done) {
+        remove(t);
+        retire(t, lk, false);
+      }
+      if (changed) {
+        for (auto& kv : groups) {
+          kv.second->sims.clear();
+          kv.second->dirty = true;
+        }
+        for (Ticket* t : active) {
+          Sim* s = t->sim.get();
+          const std::pair<int, int> key{s->G.dim, s->P.eos == EOS_ABSORBING ? 1 : 0};
+          auto& g = groups[key];
+          if (!g) {
+            g = std::make_unique<Group>();
+            g->dim = key.first;
+            g->absorbing = key.second != 0;
+            g->stream = step_stream;
+          }
+          g->sims.push_back(s);
+        }
+      }
+      const auto r0 = Clock::now();
+      double cells = 0;
+      int64_t nl = 0;
+      std::string err;
+      int code = KW_OK;
+      try {
+        for (auto& kv : groups) {
+          Group& g = *kv.second;
+          if (g.sims.empty()) continue;
+          if (g.dirty) g.rebuild();
+          int64_t k = steps_per_round;
+          for (Sim* s : g.sims) k = std::min(k, s->Nt_eff - s->steps_done);
+          const int64_t l0 = g.launches;
+          for (int64_t i = 0; i < k; ++i) group_step(g);
+          nl += g.launches - l0;
+          for (Sim* s : g.sims) cells += (double)s->G.ncell * (double)k;
+        }
+      } catch (const DeviceError& e) {
+        code = e.code;
+        err = e.what();
+      }
+      // retire the simulations that are done (reading back synchronises)
+      std::vector<Ticket*> fin;
+      for (Ticket* t : active)
+        if (code != KW_OK || t->sim->steps_done >= t->sim->Nt_eff) fin.push_back(t);
+      for (Ticket* t : fin) {
+        active.erase(std::find(active.begin(), active.end(), t));
+        if (code != KW_OK) {
+          t->sim.reset();
+          lk.lock();
+          held -= t->bytes;
+          --n_running;
+          finish(t, KW_TICKET_FAILED, code, err);
+          lk.unlock();
+        } else {
+          retire(t, lk, false);
+        }
+      }
+      if (!fin.empty()) {
+        for (auto& kv : groups) {
+          kv.second->sims.clear();
+          kv.second->dirty = true;
+        }
+        for (Ticket* t : active) {
+          Sim* s = t->sim.get();
+          groups[{s->G.dim, s->P.eos == EOS_ABSORBING ? 1 : 0}]->sims.push_back(s);
+        }
+        for (auto& kv : groups) kv.second->free_retired();  // the reads synchronised
+      }
+      const double dt = std::chrono::duration<double>(Clock::now() - r0).count();
+      lk.lock();
+      for (Ticket* t : active) t->steps_done = t->sim->steps_done;
+      cell_steps += cells;
+      launches += nl;
+      if (nl > 0) busy_s += dt;
+    }
+    // shutting down: release everything still on the device
+    lk.unlock();
+    try {
+      dev_sync(step_stream);
+    } catch (...) {
+    }
+    groups.clear();
+    lk.lock();
+    for (Ticket* t : active) {
+      t->sim.reset();
+      held -= t->bytes;
+      --n_running;
+      finish(t, KW_TICKET_CANCELLED, KW_ESTATE, "engine destroyed");
+    }
+    for (Ticket* t : ready) {
+      t->sim.reset();
+      held -= t->bytes;
+      --n_running;
+      finish(t, KW_TICKET_CANCELLED, KW_ESTATE, "engine destroyed");
+    }
+    ready.clear();
+  }
+};
+
+}  // namespace kw
+
+struct kw_engine {
+  kw::Engine e;
+};
+
+extern "C" {
+
+int kw_engine_create(const kw_engine_config* cfg, kw_engine** engine, char* err, size_t errlen) {
+  return kw_guarded(err, errlen, [&]() {
+    if (!engine) kw::fail(KW_EINVAL, "NULL engine pointer");
+    *engine = nullptr;
+    auto h = std::make_unique<kw_engine>();
+    kw::Engine& E = h->e;
+    size_t fr = 0, tot = 0;
+    kw::dev_mem_info(&fr, &tot);
+    E.budget = (cfg && cfg->device_bytes > 0) ? cfg->device_bytes : (int64_t)((double)fr * 0.9);
+    E.steps_per_round = (cfg && cfg->steps_per_round > 0) ? cfg->steps_per_round : 32;
+    E.step_stream = kw::dev_stream_create();
+    E.admit_stream = kw::dev_stream_create();
+    E.admit_thr = std::thread([&E]() { E.admit_loop(); });
+    E.step_thr = std::thread([&E]() { E.step_loop(); });
+    *engine = h.release();
+  });
+}
+
+int kw_submit(kw_engine* engine, const kw_problem* prob, int64_t* ticket, char* err,
+              size_t errlen) {
+  return kw_guarded(err, errlen, [&]() {
+    if (!engine || !ticket) kw::fail(KW_EINVAL, "NULL argument");
+    kw::check_shape(prob);
+    auto t = std::make_unique<kw::Ticket>();
+    t->prob = std::make_unique<kw::ProblemCopy>(*prob);
+    t->est_bytes = kw::estimate_bytes(*prob);
+    kw::Engine& E = engine->e;
+    std::lock_guard<std::mutex> lk(E.mu);
+    if (E.stop) kw::fail(KW_ESTATE, "engine is shutting down");
+    t->id = E.next_id++;
+    t->submit_s = E.now();
+    t->cells = prob->Nx * prob->Ny * (prob->dim == 3 ? prob->Nz : 1);
+    *ticket = t->id;
+    E.pending.push_back(t.get());
+    E.tickets[t->id] = std::move(t);
+    ++E.n_submitted;
+    E.cv_admit.notify_one();
+  });
+}
+
+int kw_poll(kw_engine* engine, int64_t ticket, kw_ticket_info* info, char* err, size_t errlen) {
+  return kw_guarded(err, errlen, [&]() {
+    if (!engine || !info) kw::fail(KW_EINVAL, "NULL argument");
+    kw::Engine& E = engine->e;
+    std::lock_guard<std::mutex> lk(E.mu);
+    auto it = E.tickets.find(ticket);
+    if (it == E.tickets.end()) kw::fail(KW_EINVAL, "unknown ticket");
+    const kw::Ticket& t = *it->second;
+    info->status = t.status;
+    info->error = t.error;
+    info->submit_s = t.submit_s;
+    info->start_s = t.start_s;
+    info->done_s = t.done_s;
+    info->steps_done = t.steps_done;
+    info->steps_total = t.steps_total;
+    info->grid_cells = t.cells;
+    info->device_bytes = t.bytes;
+  });
+}
+
+int kw_wait(kw_engine* engine, int64_t ticket, kw_outputs* out, char* err, size_t errlen) {
+  if (!engine) {
+    kw_set_err(err, errlen, "NULL engine");
+    return KW_EINVAL;
+  }
+  kw::Engine& E = engine->e;
+  std::unique_ptr<kw::Ticket> t;
+  {
+    std::unique_lock<std::mutex> lk(E.mu);
+    auto it = E.tickets.find(ticket);
+    if (it == E.tickets.end()) {
+      kw_set_err(err, errlen, "unknown ticket");
+      return KW_EINVAL;
+    }
+    kw::Ticket* tp = it->second.get();
+    E.cv_done.wait(lk, [&]() { return kw::terminal(tp->status); });
+    t = std::move(it->second);
+    E.tickets.erase(it);
+  }
+  if (t->status == KW_TICKET_DONE) {
+    t->out.copy_to(out);
+    kw_set_err(err, errlen, "");
+    return KW_OK;
+  }
+  kw_set_err(err, errlen, t->msg.c_str());
+  return t->status == KW_TICKET_CANCELLED ? KW_ESTATE : t->error;
+}
+
+int kw_cancel(kw_engine* engine, int64_t ticket, char* err, size_t errlen) {
+  return kw_guarded(err, errlen, [&]() {
+    if (!engine) kw::fail(KW_EINVAL, "NULL engine");
+    kw::Engine& E = engine->e;
+    std::lock_guard<std::mutex> lk(E.mu);
+    auto it = E.tickets.find(ticket);
+    if (it == E.tickets.end()) kw::fail(KW_EINVAL, "unknown ticket");
+    kw::Ticket* t = it->second.get();
+    if (kw::terminal(t->status)) return;
+    t->cancel = true;
+    if (t->status == KW_TICKET_QUEUED) {
+      auto p = std::find(E.pending.begin(), E.pending.end(), t);
+      if (p != E.pending.end()) E.pending.erase(p);
+      E.finish(t, KW_TICKET_CANCELLED, KW_ESTATE, "cancelled");
+    }
+    E.cv_step.notify_one();
+  });
+}
+
+int kw_engine_get_stats(kw_engine* engine, kw_engine_stats* st, char* err, size_t errlen) {
+  return kw_guarded(err, errlen, [&]() {
+    if (!engine || !st) kw::fail(KW_EINVAL, "NULL argument");
+    kw::Engine& E = engine->e;
+    std::lock_guard<std::mutex> lk(E.mu);
+    st->submitted = E.n_submitted;
+    st->completed = E.n_completed;
+    st->failed = E.n_failed;
+    st->cancelled = E.n_cancelled;
+    st->queued = (int64_t)E.pending.size();
+    st->running = E.n_running;
+    st->device_bytes = E.held;
+    st->device_budget = E.budget;
+    st->cell_steps = E.cell_steps;
+    st->busy_s = E.busy_s;
+    st->launches = E.launches;
+  });
+}
+
+int kw_engine_destroy(kw_engine* engine) {
+  if (!engine) return KW_OK;
+  kw::Engine& E = engine->e;
+  {
+    std::lock_guard<std::mutex> lk(E.mu);
+    E.stop = true;
+    for (kw::Ticket* t : E.pending) E.finish(t, KW_TICKET_CANCELLED, KW_ESTATE, "engine destroyed");
+    E.pending.clear();
+  }
+  E.cv_admit.notify_all();
+  E.cv_step.notify_all();
+  if (E.admit_thr.joinable()) E.admit_thr.join();
+  if (E.step_thr.joinable()) E.step_thr.join();
+  kw::dev_stream_destroy(E.step_stream);
+  kw::dev_stream_destroy(E.admit_stream);
+  delete engine;
+  return KW_OK;
+}
+
+}  // extern "C"

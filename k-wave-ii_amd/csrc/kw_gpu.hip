@@ -64,6 +64,13 @@ void dev_zero(void* dst, size_t bytes, void* s) {
 
 size_t dev_max_lds() { return 65536; }
 
+void dev_mem_info(size_t* free_bytes, size_t* total_bytes) {
+  size_t f = 0, t = 0;
+  check(hipMemGetInfo(&f, &t), "hipMemGetInfo");
+  if (free_bytes) *free_bytes = f;
+  if (total_bytes) *total_bytes = t;
+}
+
 void launch_pass(PassKind kind, int fixed, int dim, int nblocks, int threads, size_t lds_bytes,
                  const Batch& B, const PassArgs& A, void* stream) {
   if (nblocks <= 0) return;

@@ -78,8 +78,8 @@ KW_BODY XTile x_tile(Ctx& c, const ProbDev& P, int tile) {
   T.active = p < P.Px && pair < npairs;
   // threads without a row pair read row pair 0 (their loads stay in bounds
   // and unconditional) and never store; they keep their own LDS pencil
+  // (a launch shared by several problems sizes its LDS for T / G pencils)
   if (!T.active) pair = 0;
-  if (p >= P.Px) p = 0;
   T.r0 = 2 * pair;
   T.has1 = 2 * pair + 1 < P.nrows;
   T.r1 = T.has1 ? 2 * pair + 1 : T.r0;
@@ -231,8 +231,9 @@ KW_BODY void body_ypass(Ctx& c, const Batch& B, const PassArgs& A) {
     p = tid % Pn;
     g = tid / Pn;
     int64_t q = (int64_t)tile * Pn + p;
-    active = q < nH;
-    if (!active) q = 0;  // read a valid pencil, never store
+    active = q < nH && g < G;
+    if (g >= G) g = G - 1;  // a block larger than this problem's tiling
+    if (!active) q = 0;     // read a valid pencil, never store
     base = (uint32_t)((q / P.Wm) * ((int64_t)P.Wm * P.Ny) + (q % P.Wm));
     stride = (uint32_t)P.Wm;
     lds = LdsPencils{c.lds() + p, 0, Pn, false};
@@ -274,21 +275,22 @@ KW_BODY void body_spec(Ctx& c, const Batch& B, const PassArgs& A) {
   const int64_t tH = (nH + Pn - 1) / Pn;
   const int tid = c.tid();
   const int p = tid % Pn;
-  const int g = tid / Pn;
+  const bool gin = tid / Pn < G;           // false: block larger than this problem's tiling
+  const int g = gin ? tid / Pn : G - 1;
   int64_t q;
   uint32_t base, stride;
   int kx, ky;  // ky = -1: ky is the pencil coordinate (2D)
   bool active;
   if (tile < tH) {
     q = (int64_t)tile * Pn + p;
-    active = q < nH;
+    active = q < nH && gin;
     base = (uint32_t)q;
     stride = (uint32_t)(d3 ? (int64_t)P.Wm * P.Ny : (int64_t)P.Wm);
     kx = (int)(q % P.Wm);
     ky = d3 ? (int)(q / P.Wm) : -1;
   } else {
     q = (tile - tH) * Pn + p;
-    active = q < (d3 ? P.Ny : 1);
+    active = q < (d3 ? P.Ny : 1) && gin;
     base = (uint32_t)(P.nspec_main + q);
     stride = (uint32_t)(d3 ? P.Ny : 1);
     kx = P.Nx / 2;
@@ -640,7 +642,7 @@ KW_BODY void body_x_div(Ctx& cb, const Batch& B, const PassArgs& A) {
   const int t = KW_LD(P.t_index);
   const float dt = P.dt;
   const bool psrc = P.srcp.len >= t;
-  const bool store_p = A.flags & X_STORE_P;
+  const bool store_p = (A.flags & X_STORE_P) || t >= P.p_store_from || t == P.p_store_at;
 
   // sums over the components: velocity divergence (absorbing / stokes EOS)
   // and density; mass conservation coefficient rho0 (linear) or
@@ -747,7 +749,7 @@ KW_BODY void body_x_eos(Ctx& cb, const Batch& B, const PassArgs& A) {
   const XTile T = x_tile<XS>(cb, P, tile);
   PencilCtx<Ctx> c{cb, T.wave};
   const int t = KW_LD(P.t_index);
-  const bool store_p = A.flags & X_STORE_P;
+  const bool store_p = (A.flags & X_STORE_P) || t >= P.p_store_from || t == P.p_store_at;
   // rsum + tau A first, then - eta B and the nonlinear term
   cf part[kMaxSlots], rs[kMaxSlots];
   {
@@ -805,6 +807,10 @@ enum EwKind : int32_t {
   EW_FILL,           // zero a float buffer (graph-safe memset)
 };
 
+// EwArgs.rec value for "the problem's own statistics" (launches shared by
+// several problems)
+constexpr uint32_t kRecAuto = 0xffffffffu;
+
 struct EwArgs {
   int32_t kind;
   int32_t which;         // EW_SRC_GRID: 0 = p source, 1..3 = u source d-1
@@ -858,7 +864,7 @@ KW_BODY void body_ew(Ctx& c, const Batch& B, const EwArgs& A) {
       if (i >= P.nsens || t < P.record_start) return;
       const int fi = t - P.record_start + 1;
       const int64_t cell = P.sens_idx ? (int64_t)KW_LD(P.sens_idx + i) : i;
-      const uint32_t rec = A.rec;
+      const uint32_t rec = A.rec == kRecAuto ? P.rec_points : A.rec;
       const float pv = KW_LD(P.p + cell);
       const double pd = (double)pv;
       if (rec & REC_P) KW_ST(P.p_ts + i + (int64_t)(fi - 1) * P.nsens, pv);
@@ -896,7 +902,7 @@ KW_BODY void body_ew(Ctx& c, const Batch& B, const EwArgs& A) {
     case EW_RECORD_GRID: {
       if (i >= P.ncell || t < P.record_start) return;
       const int fi = t - P.record_start + 1;
-      const uint32_t rec = A.rec;
+      const uint32_t rec = A.rec == kRecAuto ? P.rec_grid : A.rec;
       if (rec & (REC_P_MAX_ALL | REC_P_MIN_ALL)) {
         const double pd = (double)KW_LD(P.p + i);
         if (rec & REC_P_MAX_ALL) {

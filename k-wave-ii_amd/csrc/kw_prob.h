@@ -138,6 +138,15 @@ struct ProbDev {
   const float* tr_data;   // [nsens x Nt], original (not reversed) column order
   // current step (1-based), advanced on the device at the end of each step
   int32_t* t_index;
+  // per-step decisions taken on the device from t_index, so that one launch
+  // can serve problems that are at different steps: p is stored to HBM at
+  // steps t >= p_store_from (a sparse sensor reads it) and at t == p_store_at
+  // (p_final); rec_points / rec_grid are the statistics the sensor kernels
+  // update after the first step
+  int32_t p_store_from;
+  int32_t p_store_at;
+  uint32_t rec_points;
+  uint32_t rec_grid;
 };
 
 // per-launch mapping of blocks to problems: problem i owns blocks

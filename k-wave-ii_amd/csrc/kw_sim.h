@@ -1,0 +1,447 @@
+// Internal host-side interface of kwhip: the simulation object (device
+// allocations, operators, launch geometry, per-launch byte accounting) and
+// the step functions, shared by the single-simulation C ABI (kw_sim.cpp) and
+// the engine that steps many simulations together (kw_engine.cpp).
+#pragma once
+
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "kw_backend.h"
+#include "kw_variants.h"
+#include "kwhip.h"
+
+namespace kw {
+[[noreturn]] inline void fail(int code, const std::string& m) { throw DeviceError(code, m); }
+
+constexpr double kPi = 3.14159265358979323846264338327950288;
+
+// ---------------------------------------------------------------------------
+// operators (double precision)
+// ---------------------------------------------------------------------------
+
+// radices for a length: 8s first, then 4 or 2, then 3s and 5s
+inline bool factorise(int N, std::vector<int>& r) {
+  r.clear();
+  int n = N;
+  while (n % 8 == 0) {
+    r.push_back(8);
+    n /= 8;
+  }
+  if (n % 4 == 0) {
+    r.push_back(4);
+    n /= 4;
+  }
+  if (n % 2 == 0) {
+    r.push_back(2);
+    n /= 2;
+  }
+  while (n % 3 == 0) {
+    r.push_back(3);
+    n /= 3;
+  }
+  while (n % 5 == 0) {
+    r.push_back(5);
+    n /= 5;
+  }
+  return n == 1 && !r.empty() && (int)r.size() <= KW_MAX_STAGES;
+}
+
+// wavenumber of ifftshifted index i (kWaveGrid.m:779-797 then ifftshift):
+// (2 pi/d) * (n/N) with n = i for i < ceil(N/2), else i - N
+inline double kvec(int i, int N, double d) {
+  const int n = (i < (N + 1) / 2) ? i : i - N;
+  return (2.0 * kPi / d) * ((double)n / (double)N);
+}
+
+// getPML.m:52-86
+inline std::vector<double> pml_profile(int N, double d, double dt, double c, int size,
+                                       double alpha, bool staggered) {
+  std::vector<double> v(N, 1.0);
+  for (int x = 1; x <= size && x <= N; ++x) {
+    const double xs = staggered ? x + 0.5 : (double)x;
+    const double left = alpha * (c / d) * std::pow((xs - size - 1) / (0.0 - size), 4);
+    const double right = alpha * (c / d) * std::pow(xs / size, 4);
+    v[x - 1] = std::exp(-left * dt / 2.0);
+    v[N - size + x - 1] = std::exp(-right * dt / 2.0);
+  }
+  return v;
+}
+
+inline double sinc(double x) { return x == 0.0 ? 1.0 : std::sin(x) / x; }
+
+template <class F>
+void parallel_for(int64_t n, F f) {
+  int nt = (int)std::thread::hardware_concurrency();
+  nt = std::max(1, std::min(nt, 16));
+  if (n < 100000) nt = 1;
+  std::vector<std::thread> th;
+  const int64_t chunk = (n + nt - 1) / nt;
+  for (int w = 0; w < nt; ++w) {
+    const int64_t a = w * chunk, b = std::min(n, a + chunk);
+    if (a >= b) break;
+    th.emplace_back([=]() {
+      for (int64_t i = a; i < b; ++i) f(i);
+    });
+  }
+  for (auto& t : th) t.join();
+}
+
+// ---------------------------------------------------------------------------
+// simulation object
+// ---------------------------------------------------------------------------
+
+struct Geometry {
+  int dim, Nx, Ny, Nz, Wm, nyq;
+  int64_t ncell, nrows, nspec_main, nspec;
+};
+
+inline Geometry make_geometry(int dim, int64_t Nx, int64_t Ny, int64_t Nz) {
+  Geometry g;
+  g.dim = dim;
+  g.Nx = (int)Nx;
+  g.Ny = (int)Ny;
+  g.Nz = dim == 3 ? (int)Nz : 1;
+  g.nyq = (g.Nx % 2 == 0) ? 1 : 0;
+  g.Wm = g.nyq ? g.Nx / 2 : (g.Nx + 1) / 2;
+  g.ncell = (int64_t)g.Nx * g.Ny * g.Nz;
+  g.nrows = (int64_t)g.Ny * g.Nz;
+  g.nspec_main = (int64_t)g.Wm * g.Ny * g.Nz;
+  g.nspec = g.nspec_main + (g.nyq ? g.nrows : 0);
+  return g;
+}
+
+struct LaunchGeom {
+  int nb = 0, T = 0;
+  size_t lds = 0;
+};
+
+inline int max_pencils_env() {
+  const char* e = std::getenv("KW_MAX_PENCILS");
+  if (!e) return 1 << 30;
+  const int v = std::atoi(e);
+  return v > 0 ? v : 1 << 30;
+}
+
+struct Sim {
+  Geometry G{};
+  ProbDev P{};
+  ProbDev* dP = nullptr;
+  int32_t* d_tile0 = nullptr;
+  void* stream = nullptr;
+  bool own_stream = true;
+  std::vector<void*> allocs;
+  int64_t bytes = 0;
+  int64_t steps_done = 0;
+  int64_t Nt = 0, Nt_eff = 0, n_rec = 0, nsens = 0, record_start = 1;
+  bool has_p0 = false, time_rev = false, full_mask = false;
+  uint32_t record = 0;
+  bool p_grid = false, u_grid[3] = {false, false, false};
+  float* skappa = nullptr;
+  float* p0 = nullptr;
+  LaunchGeom lx, ly, ll, lcell, lsens;
+  // per-kernel event timing
+  bool timing = false;
+  std::vector<std::pair<void*, void*>> ev_used;
+  std::vector<int> ev_kind;
+  std::vector<double> ev_bytes;
+  std::vector<void*> ev_free;
+
+  ~Sim() {
+    for (auto& e : ev_used) {
+      dev_event_destroy(e.first);
+      dev_event_destroy(e.second);
+    }
+    for (void* e : ev_free) dev_event_destroy(e);
+    for (void* p : allocs) dev_free(p);
+    if (own_stream) dev_stream_destroy(stream);
+  }
+
+  template <class T>
+  T* alloc(size_t n) {
+    void* p = dev_alloc(n * sizeof(T));
+    allocs.push_back(p);
+    bytes += (int64_t)(n * sizeof(T));
+    return (T*)p;
+  }
+  template <class T>
+  T* alloc_zero(size_t n) {
+    T* p = alloc<T>(n);
+    dev_zero(p, n * sizeof(T), stream);
+    return p;
+  }
+  template <class T>
+  T* upload(const std::vector<T>& v) {
+    T* d = alloc<T>(v.size());
+    dev_h2d(d, v.data(), v.size() * sizeof(T), stream);
+    return d;
+  }
+
+  Batch batch() const { return Batch{dP, d_tile0, 1}; }
+
+  void* take_event() {
+    if (!ev_free.empty()) {
+      void* e = ev_free.back();
+      ev_free.pop_back();
+      return e;
+    }
+    return dev_event_create();
+  }
+
+  template <class L>
+  void timed(int kind, double nbytes, L launch) {
+    if (!timing) {
+      launch();
+      return;
+    }
+    void* a = take_event();
+    void* b = take_event();
+    dev_event_record(a, stream);
+    launch();
+    dev_event_record(b, stream);
+    ev_used.push_back({a, b});
+    ev_kind.push_back(kind);
+    ev_bytes.push_back(nbytes);
+  }
+
+  // compile-time FFT length of the kernel variant for a pass (0: runtime)
+  int fixed_len(PassKind k) const {
+    if (std::getenv("KW_NO_FIXED")) return 0;
+    const int N = (k == PK_YFWD || k == PK_YINV) ? G.Ny
+                  : (k == PK_SPEC)               ? (G.dim == 3 ? G.Nz : G.Ny)
+                                                 : G.Nx;
+    return KW_IS_FIXED_SIZE(N) ? N : 0;
+  }
+
+  void pass(PassKind k, const PassArgs& a) {
+    const LaunchGeom* g = &lx;
+    if (k == PK_YFWD || k == PK_YINV) g = &ly;
+    if (k == PK_SPEC) g = &ll;
+    const Batch B = batch();
+    const int fx = fixed_len(k);
+    timed((int)k, timing ? pass_bytes(k, a) : 0.0,
+          [&]() { launch_pass(k, fx, G.dim, g->nb, g->T, g->lds, B, a, stream); });
+  }
+
+  void ew(const EwArgs& a, int nblocks) {
+    const Batch B = batch();
+    timed(8, timing ? ew_bytes(a) : 0.0, [&]() { launch_ew(nblocks, B, a, stream); });
+  }
+
+  // -- compulsory bytes of one launch: every array it reads or writes, once
+  double fb(const Field& f) const { return f.grid() ? 4.0 * G.ncell : 0.0; }
+  double src_bytes(const TimeSource& s, int t) const {
+    if (s.len < t) return 0.0;
+    if (s.mode == SRC_ADDITIVE) return 4.0 * G.ncell;  // K_M or the filtered grid
+    return 4.0 * G.ncell;                             // dense point map
+  }
+  double sensor_bytes(int t) const {
+    if (time_rev || t < record_start) return 0.0;
+    const double nc = (double)G.ncell;
+    const bool first = t == record_start;
+    const double stat = first ? 8.0 : 16.0;
+    double b = 0.0;
+    if (full_mask) {
+      if (record & KW_REC_P) b += 4.0 * nc;
+      if (record & KW_REC_P_MAX) b += stat * nc;
+      if (record & KW_REC_P_MIN) b += stat * nc;
+      if (record & KW_REC_P_RMS) b += stat * nc;
+    }
+    if (record & KW_REC_P_MAX_ALL) b += stat * nc;
+    if (record & KW_REC_P_MIN_ALL) b += stat * nc;
+    return b;
+  }
+  double pass_bytes(PassKind k, const PassArgs& a) const {
+    const int t = (int)(steps_done + 1);
+    const double nc = (double)G.ncell, sp = 8.0 * (double)G.nspec, tab = 4.0 * (double)G.nspec;
+    const int dim = G.dim;
+    switch (k) {
+      case PK_YFWD:
+      case PK_YINV:
+        return a.nbuf * 2.0 * sp;
+      case PK_SPEC:
+        if (a.mode == SM_GRAD) return sp + dim * sp + tab;
+        if (a.mode == SM_DIV) return 2.0 * dim * sp + tab;
+        if (a.mode == SM_ABS) return 4.0 * sp + 2.0 * tab;
+        return 2.0 * sp + tab;
+      case PK_X_R2C:
+        return a.nbuf * (4.0 * nc + sp);
+      case PK_X_C2R:
+        return a.nbuf * (sp + 4.0 * nc) * ((a.flags & XC_ACCUM) ? 1.5 : 1.0);
+      case PK_X_GRAD: {
+        double b = 0.0;
+        const bool init = a.flags & XG_INIT;
+        for (int d = 0; d < dim; ++d) {
+          b += sp + fb(P.rinv[d]) + 4.0 * nc;
+          if (!init) b += 4.0 * nc + sp + src_bytes(P.srcu[d], t);
+        }
+        return b;
+      }
+      case PK_X_DIV: {
+        double b = dim * (sp + 8.0 * nc) + fb(P.rho0) + dim * src_bytes(P.srcp, t);
+        if (P.eos == EOS_ABSORBING) return b + 2.0 * sp + 4.0 * nc;
+        b += fb(P.c0) + sp + sensor_bytes(t);
+        if (P.eos == EOS_STOKES) b += fb(P.tau);
+        if (P.nonlinear) b += fb(P.bona);
+        if (a.flags & X_STORE_P) b += 4.0 * nc;
+        if (time_rev) b += 4.0 * nc;
+        return b;
+      }
+      case PK_X_EOS: {
+        double b = 2.0 * sp + 4.0 * nc + fb(P.c0) + fb(P.tau) + fb(P.eta) + sp + sensor_bytes(t);
+        if (P.nonlinear) b += fb(P.bona) + fb(P.rho0);
+        if (a.flags & X_STORE_P) b += 4.0 * nc;
+        if (time_rev) b += 4.0 * nc;
+        return b;
+      }
+    }
+    return 0.0;
+  }
+  double ew_bytes(const EwArgs& a) const {
+    const double nc = (double)G.ncell;
+    switch (a.kind) {
+      case EW_INIT_P0: return (2.0 + G.dim) * 4.0 * nc;
+      case EW_TR_PROLOGUE: return (2.0 + G.dim) * 4.0 * nc;
+      case EW_SRC_GRID: return 8.0 * nc;
+      case EW_RECORD_POINTS: return 24.0 * (double)nsens;
+      case EW_RECORD_GRID: return 24.0 * nc;
+      default: return 0.0;
+    }
+  }
+
+  // -- tiling -------------------------------------------------------------
+  void plan_launches() {
+    const size_t maxl = dev_max_lds();
+    const int capP = max_pencils_env();
+    auto Gof = [](int N) { return (N + KW_E - 1) / KW_E; };
+    // x pass: <= 256 threads, LDS P * kNPX padded pencils
+    {
+      const int Gx = Gof(G.Nx);
+      const size_t pencil = sizeof(cf) * (size_t)kNPX * x_lds_stride(G.Nx);
+      int P = std::max(1, 256 / Gx);
+      P = std::min<int64_t>(P, (int64_t)(maxl / pencil));
+      P = std::max(1, std::min(P, capP));
+      this->P.Gx = Gx;
+      this->P.Px = P;
+      const int64_t npairs = (G.nrows + 1) / 2;
+      lx.nb = (int)((npairs + P - 1) / P);
+      lx.T = P * Gx;
+      lx.lds = pencil * (size_t)P;
+      if (Gx > 256 || lx.lds > maxl) fail(KW_EUNSUP, "grid size Nx too large for the x pass");
+    }
+    // strided y pass (3D): <= 512 threads, LDS P*(Ny+1)
+    if (G.dim == 3) {
+      const int Gy = Gof(G.Ny);
+      int P = std::max(1, std::min(16, 512 / Gy));
+      P = std::min<int64_t>(P, (int64_t)(maxl / (sizeof(cf) * (G.Ny + 1))));
+      P = std::max(1, std::min(P, capP));
+      this->P.Gy = Gy;
+      this->P.Py = P;
+      const int64_t nH = (int64_t)G.Wm * G.Nz;
+      ly.nb = (int)((nH + P - 1) / P + (G.nyq ? (G.Nz + P - 1) / P : 0));
+      ly.T = P * Gy;
+      ly.lds = sizeof(cf) * (size_t)P * (G.Ny + 1);
+      if (Gy > 512 || ly.lds > maxl) fail(KW_EUNSUP, "grid size Ny too large for the y pass");
+    }
+    // last-axis pass: <= 512 threads, LDS 2*P*N
+    {
+      const int Nl = G.dim == 3 ? G.Nz : G.Ny;
+      const int Gl = Gof(Nl);
+      int P = std::max(1, std::min(16, 512 / Gl));
+      P = std::min<int64_t>(P, (int64_t)(maxl / (2 * sizeof(cf) * Nl)));
+      P = std::max(1, std::min(P, capP));
+      this->P.Gl = Gl;
+      this->P.Pl = P;
+      const int64_t nH = G.dim == 3 ? (int64_t)G.Wm * G.Ny : (int64_t)G.Wm;
+      const int64_t nQ = G.dim == 3 ? G.Ny : 1;
+      ll.nb = (int)((nH + P - 1) / P + (G.nyq ? (nQ + P - 1) / P : 0));
+      ll.T = P * Gl;
+      ll.lds = 2 * sizeof(cf) * (size_t)P * Nl;
+      if (Gl > 512 || ll.lds > maxl) fail(KW_EUNSUP, "grid size too large for the last-axis pass");
+    }
+    lcell.nb = (int)((G.ncell + EW_T - 1) / EW_T);
+    lsens.nb = (int)((std::max<int64_t>(nsens, 1) + EW_T - 1) / EW_T);
+  }
+
+  FftPlan make_plan(int N) {
+    FftPlan pl{};
+    std::vector<int> r;
+    if (!factorise(N, r))
+      fail(KW_EUNSUP, "grid size " + std::to_string(N) +
+                          " is not a product of 2, 3 and 5 (supported FFT radices)");
+    pl.N = N;
+    pl.nst = (int)r.size();
+    for (size_t i = 0; i < r.size(); ++i) pl.radix[i] = r[i];
+    std::vector<cf> tw(N);
+    for (int j = 0; j < N; ++j) {
+      const double a = 2.0 * kPi * (double)j / (double)N;
+      tw[j] = cf{(float)std::cos(a), (float)-std::sin(a)};
+    }
+    pl.tw = upload(tw);
+    return pl;
+  }
+
+  // spectral-layout table of f(kx, ky, kz) (ifftshifted wavenumbers)
+  template <class F>
+  float* spectral_table(F f) {
+    std::vector<float> t(G.nspec);
+    const Geometry g = G;
+    parallel_for(g.nspec, [&](int64_t e) {
+      int ix, iy, iz;
+      if (e < g.nspec_main) {
+        ix = (int)(e % g.Wm);
+        const int64_t r = e / g.Wm;
+        iy = (int)(r % g.Ny);
+        iz = (int)(r / g.Ny);
+      } else {
+        ix = g.Nx / 2;
+        const int64_t r = e - g.nspec_main;
+        iy = (int)(r % g.Ny);
+        iz = (int)(r / g.Ny);
+      }
+      t[e] = (float)f(ix, iy, iz);
+    });
+    return upload(t);
+  }
+};
+
+std::unique_ptr<Sim> create_sim(const kw_problem* pr, void* stream = nullptr);
+PassArgs bufs(int nb, int b0 = 0, int b1 = 1, int b2 = 2);
+void src_grid(Sim& S, int which, int t);
+void step_once(Sim& S);
+void step_n(Sim& S, int64_t nsteps);
+int64_t recorded(const Sim& S);
+void read_outputs(Sim& S, kw_outputs* o);
+
+}  // namespace kw
+
+// C ABI error handling shared by the wrappers
+inline void kw_set_err(char* err, size_t errlen, const char* m) {
+  if (err && errlen) std::snprintf(err, errlen, "%s", m);
+}
+
+template <class F>
+int kw_guarded(char* err, size_t errlen, F f) {
+  try {
+    f();
+    kw_set_err(err, errlen, "");
+    return KW_OK;
+  } catch (const kw::DeviceError& e) {
+    kw_set_err(err, errlen, e.what());
+    return e.code;
+  } catch (const std::bad_alloc&) {
+    kw_set_err(err, errlen, "host allocation failed");
+    return KW_ENOMEM;
+  } catch (const std::exception& e) {
+    kw_set_err(err, errlen, e.what());
+    return KW_EDEVICE;
+  }
+}

@@ -102,6 +102,27 @@ class KernelStats(C.Structure):
                 ("bytes", C.c_double * NKINDS)]
 
 
+class EngineConfig(C.Structure):
+    _fields_ = [("device_bytes", C.c_int64), ("steps_per_round", C.c_int32)]
+
+
+class TicketInfo(C.Structure):
+    _fields_ = [("status", C.c_int32), ("error", C.c_int32), ("submit_s", C.c_double),
+                ("start_s", C.c_double), ("done_s", C.c_double), ("steps_done", C.c_int64),
+                ("steps_total", C.c_int64), ("grid_cells", C.c_int64),
+                ("device_bytes", C.c_int64)]
+
+
+class EngineStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("completed", C.c_int64), ("failed", C.c_int64),
+                ("cancelled", C.c_int64), ("queued", C.c_int64), ("running", C.c_int64),
+                ("device_bytes", C.c_int64), ("device_budget", C.c_int64),
+                ("cell_steps", C.c_double), ("busy_s", C.c_double), ("launches", C.c_int64)]
+
+
+TICKET_STATUS = {0: "queued", 1: "running", 2: "done", 3: "failed", 4: "cancelled"}
+
+
 class KWError(RuntimeError):
     def __init__(self, code, msg):
         super().__init__(f"kwhip error {code}: {msg}")
@@ -143,6 +164,13 @@ def load(backend: str = "hip") -> C.CDLL:
     lib.kw_sim_destroy.argtypes = [C.c_void_p]
     lib.kw_sim_step_timed.argtypes = [C.c_void_p, C.c_int64, P(C.c_double), P(KernelStats)] + err
     lib.kw_smooth.argtypes = [C.c_int32, P(C.c_int64), c_double_p, c_double_p, c_double_p] + err
+    lib.kw_engine_create.argtypes = [P(EngineConfig), P(C.c_void_p)] + err
+    lib.kw_submit.argtypes = [C.c_void_p, P(Problem), P(C.c_int64)] + err
+    lib.kw_poll.argtypes = [C.c_void_p, C.c_int64, P(TicketInfo)] + err
+    lib.kw_wait.argtypes = [C.c_void_p, C.c_int64, P(Outputs)] + err
+    lib.kw_cancel.argtypes = [C.c_void_p, C.c_int64] + err
+    lib.kw_engine_get_stats.argtypes = [C.c_void_p, P(EngineStats)] + err
+    lib.kw_engine_destroy.argtypes = [C.c_void_p]
     if lib.kw_abi_version() != 1:
         raise ImportError("kwhip ABI version mismatch")
     _libs[backend] = lib

@@ -1,0 +1,123 @@
+"""Engine: simulations submitted together are stepped together (one launch
+per pass for every simulation with the same dimension and equation of
+state, whatever their sizes, media, sources, sensors and current step);
+each result must equal the same problem run alone.  CPU tests use the
+emulator build; the GPU test runs the product library."""
+
+import numpy as np
+import pytest
+
+import cases
+
+TOL = 1e-6   # same arithmetic per problem; only FFT plan variants may differ
+
+
+def _alone(cs, backend):
+    return cases.run_kwhip(cs, backend)
+
+
+def _submit(eng, cs):
+    import kwhip
+    args = []
+    for n in cs["shape"]:
+        args += [n, cs["dx"]]
+    kg = kwhip.kWaveGrid(*args)
+    kg.setTime(cs["Nt"], cs["dt"])
+    return eng.submit(kg, cs["medium"], cs["source"], cs["sensor"], cs["dim"], **cs["opts"])
+
+
+def _same(a, b):
+    if not isinstance(a, dict):
+        a, b = {"p": a}, {"p": b}
+    assert sorted(a) == sorted(b)
+    for k in a:
+        x, y = np.asarray(a[k], np.float64), np.asarray(b[k], np.float64)
+        assert x.shape == y.shape, k
+        scale = max(np.max(np.abs(y)), 1e-30)
+        assert np.max(np.abs(x - y)) <= TOL * scale, (k, np.max(np.abs(x - y)) / scale)
+
+
+MIX = [
+    # (dim, case, shape): different sizes, media, sources and sensors in one batch
+    (2, "p0_hetero_absorbing", (32, 30)),
+    (2, "psrc_add", (24, 40)),
+    (2, "p0_nonlinear_absorbing", (45, 27)),
+    (2, "usrc_many", (30, 30)),
+    (2, "time_reversal", (36, 32)),
+    (2, "p0_lossless", (40, 24)),
+    (2, "psrc_dir", (27, 36)),
+    (2, "stokes", (30, 25)),
+    (3, "p0_hetero_absorbing", (16, 18, 20)),
+    (3, "psrc_label", (20, 16, 15)),
+    (3, "time_reversal", (16, 16, 16)),
+    (3, "cartesian_nearest", (18, 20, 16)),
+]
+
+
+def _run_mix(backend, mix, steps_per_round=0, device_bytes=0):
+    import kwhip
+    css = [cases.case(name, dim, shape) for dim, name, shape in mix]
+    with kwhip.Engine(device_bytes=device_bytes, steps_per_round=steps_per_round,
+                      backend=backend) as eng:
+        tickets = [_submit(eng, cs) for cs in css]
+        got = [eng.wait(t) for t in tickets]
+        st = eng.stats()
+    return css, got, st
+
+
+@pytest.fixture
+def few_pencils(monkeypatch):
+    # fewer pencils per emulated block: same code paths, faster emulation
+    monkeypatch.setenv("KW_MAX_PENCILS", "4")
+
+
+def test_engine_mix_equals_alone_emu(few_pencils):
+    css, got, st = _run_mix("emu", MIX, steps_per_round=5)
+    assert st["completed"] == len(MIX) and st["failed"] == 0
+    assert st["device_bytes"] == 0
+    for cs, g in zip(css, got):
+        _same(g, _alone(cs, "emu"))
+
+
+def test_engine_small_budget_admits_in_turn_emu(few_pencils):
+    """A budget that holds about one simulation at a time still completes
+    every request, in order of submission."""
+    mix = MIX[:4]
+    css, got, st = _run_mix("emu", mix, device_bytes=1, steps_per_round=7)
+    assert st["completed"] == len(mix)
+    for cs, g in zip(css, got):
+        _same(g, _alone(cs, "emu"))
+
+
+def test_engine_poll_cancel_and_errors_emu(few_pencils):
+    import kwhip
+    cs = cases.case("p0_hetero_absorbing", 2, (32, 30))
+    with kwhip.Engine(backend="emu", device_bytes=1) as eng:
+        t1 = _submit(eng, cs)
+        t2 = _submit(eng, cs)
+        t2.cancel()
+        assert eng.poll(t1)["status"] in ("queued", "running", "done")
+        assert eng.poll(t2)["status"] == "cancelled"
+        with pytest.raises(kwhip.KWError):
+            eng.wait(t2)
+        _same(eng.wait(t1), _alone(cs, "emu"))
+        info = eng.stats()
+        assert info["cancelled"] == 1 and info["completed"] == 1
+
+
+def test_engine_full_tiling_emu():
+    """Blocks of the GPU's size (up to 256 / 512 threads), so members with
+    smaller tilings share blocks sized for the largest one."""
+    mix = [MIX[0], MIX[2], MIX[8], MIX[9]]
+    css, got, st = _run_mix("emu", mix, steps_per_round=4)
+    for cs, g in zip(css, got):
+        _same(g, _alone(cs, "emu"))
+
+
+@pytest.mark.gpu
+def test_engine_mix_equals_alone_hip(hip_lib):
+    print("[engine] mixed batch of", len(MIX), "problems", flush=True)
+    css, got, st = _run_mix("hip", MIX, steps_per_round=5)
+    assert st["completed"] == len(MIX) and st["failed"] == 0
+    for cs, g in zip(css, got):
+        _same(g, _alone(cs, "hip"))
