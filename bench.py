@@ -81,6 +81,124 @@ def accuracy_check(N=64, Nt=300):
     return res
 
 
+def _tr_sensor(sens, data):
+    return {"mask": sens["mask"], "time_reversal_boundary_data": data}
+
+
+def batched_bench(n_problems, seed0=0):
+    """Photoacoustic time reversal of n problems of uneven size (2D 256^2 -
+    1024^2 and 3D 64^3 - 192^3): each problem is a forward run recording p
+    on one face and a time-reversal run re-injecting it.  The engine runs
+    all of them together (a problem's reconstruction is submitted when its
+    forward run returns); the plain path runs the same simulations one after
+    another with kspaceFirstOrder2D/3D.  Inputs are generated before the
+    timed regions; both timed regions include host pre-processing,
+    uploads, the simulations and the read-back of every result."""
+    import numpy as np
+
+    import kwhip
+    from kwhip import workloads
+
+    probs = [workloads.batch_tr_problem(seed0 + i) for i in range(n_problems)]
+    cell_steps = float(sum(kg.total_grid_points * (2 * kg.Nt - 1) for kg, *_ in probs))
+    fn = {2: kwhip.kspaceFirstOrder2D, 3: kwhip.kspaceFirstOrder3D}
+
+    t0 = time.perf_counter()
+    with kwhip.Engine() as eng:
+        fwd = [eng.submit(kg, med, src, sens, kg.dim, **opts) for kg, med, src, sens, opts in probs]
+        trs = []
+        for (kg, med, src, sens, opts), t in zip(probs, fwd):
+            data = eng.wait(t)
+            trs.append(eng.submit(kg, med, {}, _tr_sensor(sens, data), kg.dim, **opts))
+        rec_eng = [eng.wait(t) for t in trs]
+        st = eng.stats()
+    t_eng = time.perf_counter() - t0
+    log(f"[bench] batched: {n_problems} problems (forward + time reversal) in {t_eng:.2f} s, "
+        f"{cell_steps / t_eng / 1e6:.0f} Mcell-steps/s, {st['launches']} launches")
+
+    t0 = time.perf_counter()
+    rec_plain = []
+    for kg, med, src, sens, opts in probs:
+        data = fn[kg.dim](kg, med, src, sens, **opts)
+        rec_plain.append(fn[kg.dim](kg, med, {}, _tr_sensor(sens, data), **opts))
+    t_plain = time.perf_counter() - t0
+    log(f"[bench] plain: the same simulations one by one in {t_plain:.2f} s, "
+        f"{cell_steps / t_plain / 1e6:.0f} Mcell-steps/s")
+    diff = max(float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
+               for a, b in zip(rec_eng, rec_plain))
+    n2 = sum(1 for kg, *_ in probs if kg.dim == 2)
+    return {"value": round(cell_steps / t_eng / 1e6, 1), "unit": "Mcell-steps/s",
+            "plain_value": round(cell_steps / t_plain / 1e6, 1),
+            "speedup_vs_plain": round(t_plain / t_eng, 2),
+            "seconds": round(t_eng, 3), "plain_seconds": round(t_plain, 3),
+            "rel_linf_vs_plain": diff,
+            "sample": f"{n_problems} problems (seeds {seed0}..{seed0 + n_problems - 1}: {n2} 2D, "
+                      f"{n_problems - n2} 3D), forward + time reversal each, "
+                      f"{cell_steps / 1e9:.1f} G cell-steps"}
+
+
+def served_bench(n_req, rate, steps_lo, steps_hi, seed=7):
+    """A Poisson request stream of forward simulations (2D 256^2 - 1024^2,
+    3D 96^3 - 256^3, heterogeneous absorbing nonlinear medium, tone-burst
+    disc source, whole-grid p_max/p_rms): latency from arrival to result for
+    the engine, against a plain server that runs the same requests one at a
+    time in arrival order (its latencies follow from the measured service
+    time of each request and the same arrival times)."""
+    import numpy as np
+
+    import kwhip
+    from kwhip import workloads
+
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_req))
+    steps = rng.integers(steps_lo, steps_hi + 1, size=n_req)
+    reqs = [workloads.served_problem(seed * 1000 + i, int(steps[i])) for i in range(n_req)]
+    cell_steps = float(sum(kg.total_grid_points * kg.Nt for kg, *_ in reqs))
+    fn = {2: kwhip.kspaceFirstOrder2D, 3: kwhip.kspaceFirstOrder3D}
+
+    with kwhip.Engine() as eng:
+        t0 = time.perf_counter()
+        tickets = []
+        for i, (kg, med, src, sens, opts) in enumerate(reqs):
+            delay = t0 + arrivals[i] - time.perf_counter()
+            if delay > 0:
+                time.sleep(delay)
+            t = eng.submit(kg, med, src, sens, kg.dim, **opts)
+            # wall clock minus engine clock at submission
+            tickets.append((time.perf_counter() - t0 - eng.poll(t)["submit_s"], t))
+        done_at = np.zeros(n_req)
+        for i, (offset, t) in enumerate(tickets):
+            _, info = eng.wait(t, with_info=True)
+            done_at[i] = info["done_s"] + offset
+    # each request's own completion time (engine clock moved to wall clock)
+    lat = done_at - arrivals
+    eng_tp = cell_steps / float(np.max(done_at)) / 1e6
+
+    service = np.zeros(n_req)
+    for i, (kg, med, src, sens, opts) in enumerate(reqs):
+        t1 = time.perf_counter()
+        fn[kg.dim](kg, med, src, sens, **opts)
+        service[i] = time.perf_counter() - t1
+    finish = np.zeros(n_req)
+    tfree = 0.0
+    for i in range(n_req):
+        tfree = max(tfree, arrivals[i]) + service[i]
+        finish[i] = tfree
+    lat_plain = finish - arrivals
+    plain_tp = cell_steps / float(np.sum(service)) / 1e6
+    pct = lambda a, q: round(float(np.percentile(a, q)), 3)  # noqa: E731
+    log(f"[bench] served: {n_req} requests at {rate}/s: p50 {pct(lat, 50)} s p99 {pct(lat, 99)} s, "
+        f"{eng_tp:.0f} Mcell-steps/s; one at a time: p50 {pct(lat_plain, 50)} s "
+        f"p99 {pct(lat_plain, 99)} s, {plain_tp:.0f} Mcell-steps/s")
+    return {"value": round(eng_tp, 1), "unit": "Mcell-steps/s",
+            "p50_s": pct(lat, 50), "p99_s": pct(lat, 99),
+            "plain_value": round(plain_tp, 1), "plain_p50_s": pct(lat_plain, 50),
+            "plain_p99_s": pct(lat_plain, 99),
+            "speedup_vs_plain": round(eng_tp / plain_tp, 2),
+            "sample": f"{n_req} requests, Poisson at {rate}/s, {steps_lo}-{steps_hi} steps each "
+                      f"(seed {seed}), {cell_steps / 1e9:.1f} G cell-steps"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -91,6 +209,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--batched", type=int, default=24,
+                    help="problems in the batched time-reversal leg (0: skip)")
+    ap.add_argument("--served", type=int, default=24,
+                    help="requests in the served leg (0: skip)")
+    ap.add_argument("--served-rate", type=float, default=10.0)
+    ap.add_argument("--served-steps", type=int, nargs=2, default=[200, 600])
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
@@ -170,6 +294,10 @@ def main():
         log("[bench] accuracy: 64^3 x 300 steps of the same recipe vs the FP64/FP32 oracle")
         acc = accuracy_check()
 
+    batched = batched_bench(args.batched) if args.batched > 0 else None
+    served = (served_bench(args.served, args.served_rate, *args.served_steps)
+              if args.served > 0 else None)
+
     line = {
         "metric": METRIC, "value": round(mcps, 2), "unit": "Mcell-steps/s", "n_gpus": 1,
         "steps": K, "warmup": W, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True,
@@ -185,6 +313,8 @@ def main():
         "rel_err_64cubed_300steps": acc,
         "device_gb": round(dev_gb, 3),
         "kernels": kernels,
+        "batched": batched,
+        "served": served,
     }
     print(json.dumps(line), flush=True)
 

@@ -25,6 +25,10 @@ void dev_d2h(void* dst, const void* src, size_t bytes, void* stream);
 void dev_sync(void* stream);
 void dev_zero(void* dst, size_t bytes, void* stream);  // set-up only (not graph-safe)
 
+// compile-time FFT length of the variant built for pass `kind` on an axis of
+// length N, or 0 when only the runtime-length variant exists
+int fixed_variant(int N, PassKind kind);
+
 // launches (asynchronous on `stream`)
 // `fixed` is the compile-time FFT length of the variant to run (0: the
 // runtime-length variant); `dim` selects the 2D/3D body of the x passes
@@ -37,6 +41,7 @@ void* dev_event_create();
 void dev_event_destroy(void* e);
 void dev_event_record(void* e, void* stream);
 double dev_event_elapsed_ms(void* a, void* b);  // synchronises on b
+bool dev_event_done(void* e);                   // the work before e has finished
 
 // largest dynamic LDS per block the kernels may request
 size_t dev_max_lds();

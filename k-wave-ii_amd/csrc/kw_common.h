@@ -157,16 +157,17 @@ __device__ __forceinline__ void kw_st32(T* base, uint32_t i, V v) {
 #endif
 
 #ifdef KW_EMU
-// every LDS access is checked by the emulator for races between lanes (a
-// read of an element written by another lane with no barrier of the
-// required scope in between)
-void emu_lds_access(const cf* p, bool write);
+// every LDS access is checked by the emulator for races between lanes: a
+// read of an element written by another lane, or a write of a different
+// value over another lane's write, with no barrier of the required scope in
+// between
+void emu_lds_access(const cf* p, bool write, const cf* value);
 inline cf lds_ld(const cf* p) {
-  emu_lds_access(p, false);
+  emu_lds_access(p, false, nullptr);
   return *p;
 }
 inline void lds_st(cf* p, cf v) {
-  emu_lds_access(p, true);
+  emu_lds_access(p, true, &v);
   *p = v;
 }
 #elif !defined(KW_GPU)
@@ -181,7 +182,7 @@ inline void lds_st(cf* p, cf v) { *p = v; }
 // of a block barrier.  `wave` must be uniform across the block.
 template <class Ctx>
 struct PencilCtx {
-  Ctx& c;
+  Ctx c;  // contexts are small and copied by value
   bool wave;
   KW_BODY int tid() const { return c.tid(); }
   KW_BODY int bid() const { return c.bid(); }

@@ -141,8 +141,8 @@ struct EmuCtx {
   int tid() const { return t; }
   int bid() const { return b; }
   cf* lds() const { return L; }
-  void sync();
-  void wsync();
+  void sync() const;
+  void wsync() const;
 };
 
 struct Lane {
@@ -229,8 +229,8 @@ struct Runner {
   }
 };
 
-void EmuCtx::sync() { R->yield(0); }
-void EmuCtx::wsync() { R->yield(1); }
+void EmuCtx::sync() const { R->yield(0); }
+void EmuCtx::wsync() const { R->yield(1); }
 
 void lds_fault(const char* msg, int b, int lane, long el, int other) {
   if (g_fault.exchange(1) == 0)
@@ -271,7 +271,7 @@ void emu_launch(int nblocks, int T, size_t lds_bytes,
 
 }  // namespace
 
-void emu_lds_access(const cf* p, bool write) {
+void emu_lds_access(const cf* p, bool write, const cf* value) {
   Runner* R = tl_runner;
   if (!R) return;
   const long i = (long)(p - R->lds.data());
@@ -281,17 +281,21 @@ void emu_lds_access(const cf* p, bool write) {
     return;
   }
   const Lane& l = R->lanes[me];
+  const int w = R->lw_lane[i];
+  const bool ordered = w < 0 || w == me ||
+                       ((w / 64 != me / 64) ? l.syncs > R->lw_sync[i]
+                                            : l.syncs + l.wsyncs > R->lw_tick[i]);
   if (write) {
+    // two lanes writing different values with nothing ordering them
+    if (!ordered && std::memcmp(value, p, sizeof(cf)) != 0)
+      lds_fault("write of a different value without a barrier after the write of lane",
+                R->bid, me, i, w);
     R->lw_lane[i] = me;
     R->lw_sync[i] = l.syncs;
     R->lw_tick[i] = l.syncs + l.wsyncs;
     return;
   }
-  const int w = R->lw_lane[i];
-  if (w < 0 || w == me) return;
-  const bool ok = (w / 64 != me / 64) ? l.syncs > R->lw_sync[i]
-                                      : l.syncs + l.wsyncs > R->lw_tick[i];
-  if (!ok) lds_fault("read without a barrier after the write of lane", R->bid, me, i, w);
+  if (!ordered) lds_fault("read without a barrier after the write of lane", R->bid, me, i, w);
 }
 
 template <class Sel>
@@ -327,9 +331,20 @@ void launch_pass(PassKind kind, int fixed, int dim, int nblocks, int threads, si
     case 0: emu_pass<PlanRt>(kind, dim, nblocks, threads, lds_bytes, B, A); break;
 #define KW_EMU_SIZE(SIZE) \
     case SIZE: emu_pass<PlanCt<SIZE>>(kind, dim, nblocks, threads, lds_bytes, B, A); break;
-    KW_FIXED_SIZES(KW_EMU_SIZE)
+    KW_EMU_FIXED_SIZES(KW_EMU_SIZE)
 #undef KW_EMU_SIZE
     default: throw DeviceError(-3, "no kernel variant for this length");
+  }
+}
+
+int fixed_variant(int N, PassKind) {
+  switch (N) {
+#define KW_EMU_CASE(SIZE) case SIZE:
+    KW_EMU_FIXED_SIZES(KW_EMU_CASE)
+    return N;
+#undef KW_EMU_CASE
+    default:
+      return 0;
   }
 }
 
@@ -344,6 +359,7 @@ struct EmuEvent {
 void* dev_event_create() { return new EmuEvent(); }
 void dev_event_destroy(void* e) { delete (EmuEvent*)e; }
 void dev_event_record(void* e, void*) { ((EmuEvent*)e)->t = std::chrono::steady_clock::now(); }
+bool dev_event_done(void*) { return true; }  // emulated launches are synchronous
 double dev_event_elapsed_ms(void* a, void* b) {
   return std::chrono::duration<double, std::milli>(((EmuEvent*)b)->t - ((EmuEvent*)a)->t)
       .count();

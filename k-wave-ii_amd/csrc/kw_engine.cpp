@@ -4,15 +4,20 @@
 // An admission thread takes submitted problems in order and moves each to
 // the device (operators, uploads, first step) on its own stream while the
 // estimated device footprint of everything admitted fits the engine's
-// budget.  A stepping thread keeps the admitted simulations in groups (same
-// dimension and equation-of-state pass sequence) and advances each group
-// with one launch per pass for all of its members: every block finds its
-// problem in a per-pass tile table, each problem keeps its own FFT plans,
-// tiling, medium, sources and sensors, and every per-step decision is taken
-// on the device from the problem's own step counter, so members may differ
-// in size and be at different steps.  Finished simulations are read back to
-// host staging buffers and their device memory is released, which admits
-// the next ones.
+// budget.  A stepping thread keeps the admitted simulations in groups and
+// advances each group with one launch per pass for all of its members:
+// every block finds its problem in a per-pass tile table, each problem keeps
+// its own FFT plans, tiling, medium, sources and sensors, and every per-step
+// decision is taken on the device from the problem's own step counter, so
+// members may be at different steps.  A group holds simulations with the
+// same pass sequence (dimension, equation of state) and, where fixed-length
+// kernels exist for the grid, the same size; simulations of other sizes
+// share a group per pass sequence that runs the runtime-length kernels.
+// Groups run on a pool of streams, each with one round of steps in flight,
+// so small groups overlap on the device.  Finished simulations are read
+// back to host staging buffers and their device memory is released, which
+// admits the next ones.
+#include <array>
 #include <chrono>
 #include <condition_variable>
 #include <deque>
@@ -215,10 +220,15 @@ struct OutputStage {
 // ---------------------------------------------------------------------------
 enum GeoKind { GX = 0, GY, GL, GCELL, GSENS, GONE, NGEO };
 
+struct Ticket;
+
 struct Group {
   int dim = 0;
   bool absorbing = false;
   void* stream = nullptr;
+  void* ev = nullptr;       // recorded after the last round issued
+  bool in_flight = false;   // that round has not finished
+  std::vector<Ticket*> members, joining;
   std::vector<Sim*> sims;
   bool dirty = true;
   ProbDev* probs = nullptr;
@@ -235,6 +245,7 @@ struct Group {
   ~Group() {
     for (void* p : live) dev_free(p);
     for (void* p : retired) dev_free(p);
+    if (ev) dev_event_destroy(ev);
   }
 
   // tables of earlier memberships may be read by launches still queued;
@@ -303,15 +314,17 @@ struct Group {
       live.push_back(tiles[k]);
       dev_h2d(tiles[k], start.data(), sizeof(int32_t) * n, stream);
     }
-    auto common = [&](auto len) -> int {
+    // a fixed-length variant serves the launch only if every member has
+    // that length on the pass's axis
+    auto common = [&](auto len, PassKind kind) -> int {
       const int N = len(*sims[0]);
       for (Sim* s : sims)
         if (len(*s) != N) return 0;
-      return (KW_IS_FIXED_SIZE(N) && !std::getenv("KW_NO_FIXED")) ? N : 0;
+      return std::getenv("KW_NO_FIXED") ? 0 : fixed_variant(N, kind);
     };
-    fixed[0] = common([](const Sim& s) { return s.G.Nx; });
-    fixed[1] = dim == 3 ? common([](const Sim& s) { return s.G.Ny; }) : 0;
-    fixed[2] = common([&](const Sim& s) { return dim == 3 ? s.G.Nz : s.G.Ny; });
+    fixed[0] = common([](const Sim& s) { return s.G.Nx; }, PK_X_GRAD);
+    fixed[1] = dim == 3 ? common([](const Sim& s) { return s.G.Ny; }, PK_YFWD) : 0;
+    fixed[2] = common([&](const Sim& s) { return dim == 3 ? s.G.Nz : s.G.Ny; }, PK_SPEC);
     any_points = any_grid = false;
     for (Sim* s : sims) {
       any_points = any_points || s->P.rec_points != 0;
@@ -415,7 +428,7 @@ struct Engine {
   int64_t budget = 0, held = 0;  // device bytes: budget, held (estimates until created)
   int32_t steps_per_round = 32;
   bool stop = false;
-  void* step_stream = nullptr;
+  std::vector<void*> streams;  // compute streams, one per group (round robin)
   void* admit_stream = nullptr;
   Clock::time_point t0 = Clock::now();
   // statistics (under mu)
@@ -468,7 +481,6 @@ struct Engine {
         sim = create_sim(&t->prob->pb, admit_stream);
         if (sim->Nt_eff > 0) step_once(*sim);
         dev_sync(admit_stream);
-        sim->stream = step_stream;
       } catch (const DeviceError& e) {
         code = e.code;
         msg = e.what();
@@ -513,7 +525,7 @@ struct Engine {
       }
     } else {
       try {
-        dev_sync(step_stream);
+        dev_sync(t->sim->stream);
       } catch (...) {
       }
     }
@@ -532,132 +544,169 @@ struct Engine {
     lk.unlock();
   }
 
+  // Group of a simulation: same pass sequence and, when fixed-length
+  // variants exist for every axis of its grid, the same grid size (the group
+  // then runs those variants); other sizes share one group per pass sequence
+  // and run the runtime-length variants.
+  static std::array<int, 5> group_key(const Sim& s) {
+    const int dim = s.G.dim;
+    const int absorbing = s.P.eos == EOS_ABSORBING ? 1 : 0;
+    const int Nl = dim == 3 ? s.G.Nz : s.G.Ny;
+    const bool fixed = !std::getenv("KW_NO_FIXED") && fixed_variant(s.G.Nx, PK_X_GRAD) != 0 &&
+                       (dim == 2 || fixed_variant(s.G.Ny, PK_YFWD) != 0) &&
+                       fixed_variant(Nl, PK_SPEC) != 0;
+    if (fixed) return {dim, absorbing, s.G.Nx, s.G.Ny, s.G.Nz};
+    return {dim, absorbing, 0, 0, 0};
+  }
+
+  // Each group runs on one stream of the pool and has at most one round
+  // (steps_per_round steps of all members) in flight; groups on different
+  // streams overlap on the device.  Members join and leave a group only
+  // between its rounds.
   void step_loop() {
-    std::vector<Ticket*> active;
-    std::map<std::pair<int, int>, std::unique_ptr<Group>> groups;
+    std::map<std::array<int, 5>, std::unique_ptr<Group>> groups;
+    size_t next_stream = 0;
+    auto last = Clock::now();
     std::unique_lock<std::mutex> lk(mu);
     for (;;) {
-      cv_step.wait(lk, [&]() { return stop || !ready.empty() || !active.empty(); });
+      bool any = false;
+      for (auto& kv : groups) any = any || !kv.second->members.empty() || !kv.second->joining.empty();
+      if (!any) cv_step.wait(lk, [&]() { return stop || !ready.empty(); });
       if (stop) break;
-      bool changed = !ready.empty();
-      for (Ticket* t : ready) active.push_back(t);
+      std::vector<Ticket*> joining(ready.begin(), ready.end());
       ready.clear();
-      // cancellations and simulations that finished in their first step
-      std::vector<Ticket*> drop, done;
-      for (Ticket* t : active) {
-        if (t->cancel)
-          drop.push_back(t);
-        else if (t->sim->steps_done >= t->sim->Nt_eff)
-          done.push_back(t);
-      }
       lk.unlock();
-      auto remove = [&](Ticket* t) {
-        active.erase(std::find(active.begin(), active.end(), t));
-        changed = true;
-      };
-      for (Ticket* t : drop) {
-        remove(t);
-        retire(t, lk, true);
-      }
-      for (Ticket* t : done) {
-        remove(t);
-        retire(t, lk, false);
-      }
-      if (changed) {
-        for (auto& kv : groups) {
-          kv.second->sims.clear();
-          kv.second->dirty = true;
+      for (Ticket* t : joining) {
+        if (t->sim->steps_done >= t->sim->Nt_eff) {
+          retire(t, lk, false);  // finished in its first step
+          continue;
         }
-        for (Ticket* t : active) {
-          Sim* s = t->sim.get();
-          const std::pair<int, int> key{s->G.dim, s->P.eos == EOS_ABSORBING ? 1 : 0};
-          auto& g = groups[key];
-          if (!g) {
-            g = std::make_unique<Group>();
-            g->dim = key.first;
-            g->absorbing = key.second != 0;
-            g->stream = step_stream;
-          }
-          g->sims.push_back(s);
+        const auto key = group_key(*t->sim);
+        auto& g = groups[key];
+        if (!g) {
+          g = std::make_unique<Group>();
+          g->dim = key[0];
+          g->absorbing = key[1] != 0;
+          g->stream = streams[next_stream++ % streams.size()];
+          g->ev = dev_event_create();
         }
+        g->joining.push_back(t);
       }
-      const auto r0 = Clock::now();
+      bool issued = false, busy = false;
       double cells = 0;
       int64_t nl = 0;
-      std::string err;
-      int code = KW_OK;
-      try {
-        for (auto& kv : groups) {
-          Group& g = *kv.second;
-          if (g.sims.empty()) continue;
-          if (g.dirty) g.rebuild();
+      for (auto& kv : groups) {
+        Group& g = *kv.second;
+        int code = KW_OK;
+        std::string err;
+        try {
+          if (g.in_flight) {
+            if (!dev_event_done(g.ev)) {
+              busy = true;
+              continue;
+            }
+            g.in_flight = false;
+          }
+          // between rounds: members that finished or were cancelled leave,
+          // new ones join
+          std::vector<Ticket*> keep;
+          for (Ticket* t : g.members) {
+            lk.lock();
+            const bool cancelled = t->cancel;
+            lk.unlock();
+            if (cancelled || t->sim->steps_done >= t->sim->Nt_eff)
+              retire(t, lk, cancelled);
+            else
+              keep.push_back(t);
+          }
+          const bool changed = keep.size() != g.members.size() || !g.joining.empty();
+          for (Ticket* t : g.joining) {
+            t->sim->stream = g.stream;
+            keep.push_back(t);
+          }
+          g.joining.clear();
+          g.members.swap(keep);
+          if (changed) {
+            g.sims.clear();
+            for (Ticket* t : g.members) g.sims.push_back(t->sim.get());
+            g.free_retired();  // the group's stream is idle
+            g.rebuild();
+          }
+          if (g.members.empty()) continue;
           int64_t k = steps_per_round;
           for (Sim* s : g.sims) k = std::min(k, s->Nt_eff - s->steps_done);
           const int64_t l0 = g.launches;
           for (int64_t i = 0; i < k; ++i) group_step(g);
+          dev_event_record(g.ev, g.stream);
+          g.in_flight = true;
+          issued = busy = true;
           nl += g.launches - l0;
           for (Sim* s : g.sims) cells += (double)s->G.ncell * (double)k;
+        } catch (const DeviceError& e) {
+          code = e.code;
+          err = e.what();
         }
-      } catch (const DeviceError& e) {
-        code = e.code;
-        err = e.what();
-      }
-      // retire the simulations that are done (reading back synchronises)
-      std::vector<Ticket*> fin;
-      for (Ticket* t : active)
-        if (code != KW_OK || t->sim->steps_done >= t->sim->Nt_eff) fin.push_back(t);
-      for (Ticket* t : fin) {
-        active.erase(std::find(active.begin(), active.end(), t));
         if (code != KW_OK) {
-          t->sim.reset();
-          lk.lock();
-          held -= t->bytes;
-          --n_running;
-          finish(t, KW_TICKET_FAILED, code, err);
-          lk.unlock();
-        } else {
-          retire(t, lk, false);
+          // a device error ends every member of the group
+          try {
+            dev_sync(g.stream);
+          } catch (...) {
+          }
+          g.in_flight = false;
+          std::vector<Ticket*> all = g.members;
+          all.insert(all.end(), g.joining.begin(), g.joining.end());
+          g.members.clear();
+          g.joining.clear();
+          g.sims.clear();
+          for (Ticket* t : all) {
+            const int64_t b = t->bytes;
+            t->sim.reset();
+            lk.lock();
+            held -= b;
+            --n_running;
+            finish(t, KW_TICKET_FAILED, code, err);
+            lk.unlock();
+          }
         }
       }
-      if (!fin.empty()) {
-        for (auto& kv : groups) {
-          kv.second->sims.clear();
-          kv.second->dirty = true;
-        }
-        for (Ticket* t : active) {
-          Sim* s = t->sim.get();
-          groups[{s->G.dim, s->P.eos == EOS_ABSORBING ? 1 : 0}]->sims.push_back(s);
-        }
-        for (auto& kv : groups) kv.second->free_retired();  // the reads synchronised
-      }
-      const double dt = std::chrono::duration<double>(Clock::now() - r0).count();
+      const auto nowt = Clock::now();
       lk.lock();
-      for (Ticket* t : active) t->steps_done = t->sim->steps_done;
+      for (auto& kv : groups)
+        for (Ticket* t : kv.second->members) t->steps_done = t->sim->steps_done;
       cell_steps += cells;
       launches += nl;
-      if (nl > 0) busy_s += dt;
+      if (busy) busy_s += std::chrono::duration<double>(nowt - last).count();
+      last = nowt;
+      if (!issued && busy) {
+        // every group has a round in flight: wait a little for one to end
+        // (or for new simulations)
+        cv_step.wait_for(lk, std::chrono::microseconds(100),
+                         [&]() { return stop || !ready.empty(); });
+      }
     }
     // shutting down: release everything still on the device
     lk.unlock();
-    try {
-      dev_sync(step_stream);
-    } catch (...) {
+    for (void* st : streams) {
+      try {
+        dev_sync(st);
+      } catch (...) {
+      }
+    }
+    std::vector<Ticket*> all;
+    for (auto& kv : groups) {
+      all.insert(all.end(), kv.second->members.begin(), kv.second->members.end());
+      all.insert(all.end(), kv.second->joining.begin(), kv.second->joining.end());
     }
     groups.clear();
     lk.lock();
-    for (Ticket* t : active) {
-      t->sim.reset();
-      held -= t->bytes;
-      --n_running;
-      finish(t, KW_TICKET_CANCELLED, KW_ESTATE, "engine destroyed");
-    }
-    for (Ticket* t : ready) {
-      t->sim.reset();
-      held -= t->bytes;
-      --n_running;
-      finish(t, KW_TICKET_CANCELLED, KW_ESTATE, "engine destroyed");
-    }
+    all.insert(all.end(), ready.begin(), ready.end());
     ready.clear();
+    for (Ticket* t : all) {
+      t->sim.reset();
+      held -= t->bytes;
+      --n_running;
+      finish(t, KW_TICKET_CANCELLED, KW_ESTATE, "engine destroyed");
+    }
   }
 };
 
@@ -679,7 +728,9 @@ int kw_engine_create(const kw_engine_config* cfg, kw_engine** engine, char* err,
     kw::dev_mem_info(&fr, &tot);
     E.budget = (cfg && cfg->device_bytes > 0) ? cfg->device_bytes : (int64_t)((double)fr * 0.9);
     E.steps_per_round = (cfg && cfg->steps_per_round > 0) ? cfg->steps_per_round : 32;
-    E.step_stream = kw::dev_stream_create();
+    // as many compute streams as the device has hardware queues for this
+    // process by default (4), so that groups overlap without oversubscribing
+    for (int i = 0; i < 4; ++i) E.streams.push_back(kw::dev_stream_create());
     E.admit_stream = kw::dev_stream_create();
     E.admit_thr = std::thread([&E]() { E.admit_loop(); });
     E.step_thr = std::thread([&E]() { E.step_loop(); });
@@ -808,7 +859,7 @@ int kw_engine_destroy(kw_engine* engine) {
   E.cv_step.notify_all();
   if (E.admit_thr.joinable()) E.admit_thr.join();
   if (E.step_thr.joinable()) E.step_thr.join();
-  kw::dev_stream_destroy(E.step_stream);
+  for (void* st : E.streams) kw::dev_stream_destroy(st);
   kw::dev_stream_destroy(E.admit_stream);
   delete engine;
   return KW_OK;

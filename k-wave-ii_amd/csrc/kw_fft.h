@@ -265,69 +265,73 @@ struct LdsPencils {
 };
 
 // ---- runtime-length engine --------------------------------------------------
+// Any N = 2^a 3^b 5^c.  The transform is staged entirely in LDS: a strided,
+// coalesced pass loads the pencil into LDS through the caller's functor, the
+// Stockham stages run from LDS to LDS, and a strided pass hands element
+// n = g + i*G to the caller as slot i.  The caller's functors are
+// instantiated once per call site instead of once per radix and stage
+// position, which would multiply the code size past the instruction cache.
 
-template <int NP, bool INV, class Ctx, class Src, class Dst>
-KW_BODY void stage_rt(int R, Ctx& c, int N, int Ns, int G, int g, bool active, const cf* tw,
-                      Src& src, Dst& dst) {
-  switch (R) {
-    case 2: stage<2, NP, INV, MaxB<2>::v>(c, N, Ns, G, g, active, tw, src, dst); break;
-    case 3: stage<3, NP, INV, MaxB<3>::v>(c, N, Ns, G, g, active, tw, src, dst); break;
-    case 4: stage<4, NP, INV, MaxB<4>::v>(c, N, Ns, G, g, active, tw, src, dst); break;
-    case 5: stage<5, NP, INV, MaxB<5>::v>(c, N, Ns, G, g, active, tw, src, dst); break;
-    default: stage<8, NP, INV, MaxB<8>::v>(c, N, Ns, G, g, active, tw, src, dst); break;
-  }
-}
+constexpr int kRtSlots = 10;  // ceil(N / G) for G = ceil(N / KW_E)
+static_assert(kRtSlots <= kMaxSlots, "slot bound");
 
-template <int NP, bool INV, class Ctx, class Src, class Dst>
-KW_BODY void fft_rt(Ctx& c, const FftPlan& pl, int g, int G, bool active, Src& src0, Dst& dstN,
-                    const LdsPencils& lds) {
+// Threads without a pencil (active == false) may share another pencil's LDS
+// to keep their loads in bounds, so they never store, here as in the
+// fixed-length engine.
+template <int NP, bool INV, class Ctx>
+KW_BODY void fft_stages_rt(Ctx& c, const FftPlan& pl, int g, int G, bool active,
+                           const LdsPencils& lds) {
   auto lsrc = [&](int j, int n) -> cf { return lds.get(j, n); };
   auto ldst = [&](int j, int n, cf v, int) { lds.set(j, n, v); };
   int Ns = 1;
-  const int nst = pl.nst;
-  for (int s = 0; s < nst; ++s) {
+  for (int s = 0; s < pl.nst; ++s) {
     const int R = pl.radix[s];
-    const bool first = (s == 0), last = (s == nst - 1);
-    if (first && last)
-      stage_rt<NP, INV>(R, c, pl.N, Ns, G, g, active, pl.tw, src0, dstN);
-    else if (first)
-      stage_rt<NP, INV>(R, c, pl.N, Ns, G, g, active, pl.tw, src0, ldst);
-    else if (last)
-      stage_rt<NP, INV>(R, c, pl.N, Ns, G, g, active, pl.tw, lsrc, dstN);
-    else
-      stage_rt<NP, INV>(R, c, pl.N, Ns, G, g, active, pl.tw, lsrc, ldst);
+    switch (R) {
+      case 2: stage<2, NP, INV, MaxB<2>::v>(c, pl.N, Ns, G, g, active, pl.tw, lsrc, ldst); break;
+      case 3: stage<3, NP, INV, MaxB<3>::v>(c, pl.N, Ns, G, g, active, pl.tw, lsrc, ldst); break;
+      case 4: stage<4, NP, INV, MaxB<4>::v>(c, pl.N, Ns, G, g, active, pl.tw, lsrc, ldst); break;
+      case 5: stage<5, NP, INV, MaxB<5>::v>(c, pl.N, Ns, G, g, active, pl.tw, lsrc, ldst); break;
+      default: stage<8, NP, INV, MaxB<8>::v>(c, pl.N, Ns, G, g, active, pl.tw, lsrc, ldst); break;
+    }
     Ns *= R;
   }
 }
 
-// Visit the (slot, n) pairs this thread owns in the last stage of a runtime
-// plan (the same pairs `dst` receives), in slot order.  Every thread visits
-// its pairs whether or not it has a pencil: the visitor guards its stores.
-template <int R, class F>
-KW_BODY void last_slots_r(int N, int G, int g, bool, F& f) {
-  constexpr int MB = MaxB<R>::v;
-  const int nb = N / R;
-  const int Ns = N / R;
+template <int NP, bool INV, class Ctx, class Src, class Dst>
+KW_BODY void fft_rt(Ctx& c, const FftPlan& pl, int g, int G, bool active, Src& src, Dst& dst,
+                    const LdsPencils& lds) {
+  const int N = pl.N;
 #pragma unroll
-  for (int m = 0; m < MB; ++m) {
-    const int b = g + m * G;
-    if (b < nb) {
-      const int k = b % Ns;
-      const int j0 = (b - k) * R + k;
+  for (int i = 0; i < kRtSlots; ++i) {
+    const int n = g + i * G;
+    if (n < N) {
 #pragma unroll
-      for (int r = 0; r < R; ++r) f(m * R + r, j0 + r * Ns);
+      for (int j = 0; j < NP; ++j) {
+        const cf v = src(j, n);
+        if (active) lds.set(j, n, v);
+      }
     }
   }
+  c.sync();
+  fft_stages_rt<NP, INV>(c, pl, g, G, active, lds);
+#pragma unroll
+  for (int i = 0; i < kRtSlots; ++i) {
+    const int n = g + i * G;
+    if (active && n < N) {
+#pragma unroll
+      for (int j = 0; j < NP; ++j) dst(j, n, lds.get(j, n), i);
+    }
+  }
+  c.sync();
 }
 
+// the (slot, n) pairs this thread owns after a runtime-length transform
 template <class F>
-KW_BODY void last_slots_rt(const FftPlan& pl, int g, int G, bool active, F& f) {
-  switch (pl.radix[pl.nst - 1]) {
-    case 2: last_slots_r<2>(pl.N, G, g, active, f); break;
-    case 3: last_slots_r<3>(pl.N, G, g, active, f); break;
-    case 4: last_slots_r<4>(pl.N, G, g, active, f); break;
-    case 5: last_slots_r<5>(pl.N, G, g, active, f); break;
-    default: last_slots_r<8>(pl.N, G, g, active, f); break;
+KW_BODY void last_slots_rt(const FftPlan& pl, int g, int G, bool, F& f) {
+#pragma unroll
+  for (int i = 0; i < kRtSlots; ++i) {
+    const int n = g + i * G;
+    if (n < pl.N) f(i, n);
   }
 }
 
@@ -403,8 +407,11 @@ struct PlanRt {
 template <int N_>
 struct PlanCt {
   static constexpr int fixed = N_;
-  static constexpr int G = N_ / KW_E;
-  static_assert(N_ % KW_E == 0, "fixed lengths must be multiples of KW_E");
+  // threads per pencil; when N is not a multiple of KW_E some threads get
+  // fewer butterflies (the stages guard them)
+  static constexpr int G = (N_ + KW_E - 1) / KW_E;
+  static constexpr int last_R = ct_radix(N_, ct_nst(N_) - 1);
+  static_assert(((N_ / last_R + G - 1) / G) * last_R <= kMaxSlots, "slot bound");
   template <int NP, bool INV, class Ctx, class Src, class Dst>
   KW_BODY static void fft(Ctx& c, const FftPlan& pl, int g, int, bool active, Src& src, Dst& dst,
                           const LdsPencils& lds) {

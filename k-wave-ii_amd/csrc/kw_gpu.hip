@@ -10,17 +10,20 @@
 
 namespace kw {
 
-#define KW_DECLARE_KINDS(SIZE)                                                    \
-  KW_DECLARE_VARIANT(0, SIZE)                                                     \
-  KW_DECLARE_VARIANT(1, SIZE)                                                     \
+#define KW_DECLARE_XL(SIZE)                                                       \
   KW_DECLARE_VARIANT(2, SIZE)                                                     \
   KW_DECLARE_VARIANT(3, SIZE)                                                     \
   KW_DECLARE_VARIANT(4, SIZE)                                                     \
   KW_DECLARE_VARIANT(5, SIZE)                                                     \
   KW_DECLARE_VARIANT(6, SIZE)                                                     \
   KW_DECLARE_VARIANT(7, SIZE)
-KW_DECLARE_KINDS(0)
-KW_FIXED_SIZES(KW_DECLARE_KINDS)
+#define KW_DECLARE_ALL(SIZE)                                                      \
+  KW_DECLARE_VARIANT(0, SIZE)                                                     \
+  KW_DECLARE_VARIANT(1, SIZE)                                                     \
+  KW_DECLARE_XL(SIZE)
+KW_DECLARE_ALL(0)
+KW_FIXED_SIZES_ALL(KW_DECLARE_ALL)
+KW_FIXED_SIZES_XL(KW_DECLARE_XL)
 void launch_k_ew(int, int, size_t, void*, const Batch&, const EwArgs&);
 
 static void check(hipError_t e, const char* what) {
@@ -71,12 +74,27 @@ void dev_mem_info(size_t* free_bytes, size_t* total_bytes) {
   if (total_bytes) *total_bytes = t;
 }
 
+int fixed_variant(int N, PassKind kind) {
+  switch (N) {
+#define KW_ALL_CASE(SIZE) case SIZE:
+    KW_FIXED_SIZES_ALL(KW_ALL_CASE)
+    return N;
+#undef KW_ALL_CASE
+#define KW_XL_CASE(SIZE) case SIZE:
+    KW_FIXED_SIZES_XL(KW_XL_CASE)
+    return (kind == PK_YFWD || kind == PK_YINV) ? 0 : N;
+#undef KW_XL_CASE
+    default:
+      return 0;
+  }
+}
+
 void launch_pass(PassKind kind, int fixed, int dim, int nblocks, int threads, size_t lds_bytes,
                  const Batch& B, const PassArgs& A, void* stream) {
   if (nblocks <= 0) return;
 #define KW_CASE_KIND(KIND, SIZE) \
   case KIND: launch_v_##KIND##_##SIZE(dim, nblocks, threads, lds_bytes, stream, B, A); break;
-#define KW_CASE_SIZE(SIZE)        \
+#define KW_CASE_ALL(SIZE)         \
   case SIZE:                      \
     switch ((int)kind) {          \
       KW_CASE_KIND(0, SIZE)       \
@@ -89,12 +107,26 @@ void launch_pass(PassKind kind, int fixed, int dim, int nblocks, int threads, si
       KW_CASE_KIND(7, SIZE)       \
     }                             \
     break;
+#define KW_CASE_XL(SIZE)                                                          \
+  case SIZE:                                                                      \
+    switch ((int)kind) {                                                          \
+      KW_CASE_KIND(2, SIZE)                                                       \
+      KW_CASE_KIND(3, SIZE)                                                       \
+      KW_CASE_KIND(4, SIZE)                                                       \
+      KW_CASE_KIND(5, SIZE)                                                       \
+      KW_CASE_KIND(6, SIZE)                                                       \
+      KW_CASE_KIND(7, SIZE)                                                       \
+      default: throw DeviceError(-3, "no strided-pass variant for this length"); \
+    }                                                                             \
+    break;
   switch (fixed) {
-    KW_CASE_SIZE(0)
-    KW_FIXED_SIZES(KW_CASE_SIZE)
+    KW_CASE_ALL(0)
+    KW_FIXED_SIZES_ALL(KW_CASE_ALL)
+    KW_FIXED_SIZES_XL(KW_CASE_XL)
     default: throw DeviceError(-3, "no kernel variant for this length");
   }
-#undef KW_CASE_SIZE
+#undef KW_CASE_XL
+#undef KW_CASE_ALL
 #undef KW_CASE_KIND
   check(hipGetLastError(), "kernel launch");
 }
@@ -115,6 +147,12 @@ void dev_event_destroy(void* e) {
 }
 void dev_event_record(void* e, void* s) {
   check(hipEventRecord((hipEvent_t)e, (hipStream_t)s), "hipEventRecord");
+}
+bool dev_event_done(void* e) {
+  const hipError_t r = hipEventQuery((hipEvent_t)e);
+  if (r == hipErrorNotReady) return false;
+  check(r, "hipEventQuery");
+  return true;
 }
 double dev_event_elapsed_ms(void* a, void* b) {
   check(hipEventSynchronize((hipEvent_t)b), "hipEventSynchronize");

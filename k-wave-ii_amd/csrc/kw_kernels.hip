@@ -1,8 +1,8 @@
 // gfx950 kernels.  This file is compiled once per (pass kind, fixed length)
-// variant (KW_KIND = 0..7 and KW_SIZE = 0 for runtime lengths or one of
-// KW_FIXED_SIZES), or with KW_KIND = 8 for the elementwise kernel, so that
-// the variants build in parallel; each compile defines its kernels and one
-// host launcher.
+// variant (KW_KIND = 0..7 and KW_SIZE = 0 for runtime lengths or one of the
+// fixed lengths of kw_variants.h), or with KW_KIND = 8 for the elementwise
+// kernel, so that the variants build in parallel; each compile defines its
+// kernels and one host launcher.
 #include <hip/hip_runtime.h>
 
 #include "kw_backend.h"

@@ -68,8 +68,9 @@ struct XTile {
 template <class XS, class Ctx>
 KW_BODY XTile x_tile(Ctx& c, const ProbDev& P, int tile) {
   XTile T;
-  T.G = XS::fixed ? XS::fixed / KW_E : P.Gx;
-  T.wave = XS::fixed ? pencils_in_wave(XS::fixed / KW_E) : pencils_in_wave(P.Gx);
+  constexpr int Gc = XS::fixed ? (XS::fixed + KW_E - 1) / KW_E : 0;
+  T.G = XS::fixed ? Gc : P.Gx;
+  T.wave = XS::fixed ? pencils_in_wave(Gc) : pencils_in_wave(P.Gx);
   const int tid = c.tid();
   int p = tid / T.G;
   T.g = tid % T.G;
@@ -260,7 +261,8 @@ KW_BODY void body_ypass(Ctx& c, const Batch& B, const PassArgs& A) {
 
 // ---------------------------------------------------------------------------
 // last-axis pass (z in 3D, y in 2D): forward FFT, spectral operator, inverse
-// FFT, all in LDS.  Operators (kspaceFirstOrder3D.m:629-661, 792-803,
+// FFT, all in LDS.  The forward transform ends in pencil la; the inverse
+// transforms read la.  Operators (kspaceFirstOrder3D.m:629-661, 792-803,
 // 907-909, 977-978; createAbsorptionVariables.m:29-84).
 // ---------------------------------------------------------------------------
 template <class LS, class Ctx>
@@ -306,6 +308,12 @@ KW_BODY void body_spec(Ctx& c, const Batch& B, const PassArgs& A) {
   }
   LdsPencils la{c.lds() + p, 0, Pn, false};
   LdsPencils lb{c.lds() + (int64_t)Pn * N + p, 0, Pn, false};
+  // workspace of the inverse transforms that read la: la itself for a
+  // fixed-length plan (every thread of a block then has a pencil or a
+  // pencil of its own), lb for runtime plans, whose launches may be shared
+  // by problems of different tilings (a thread repeating another's work
+  // would read la while its twin overwrites it)
+  const LdsPencils& lw = LS::fixed ? la : lb;
   auto addr = [&](int n) -> uint32_t { return base + (uint32_t)n * stride; };
   auto to_a = [&](int, int n, cf v, int) { la.set(0, n, v); };
 
@@ -345,7 +353,7 @@ KW_BODY void body_spec(Ctx& c, const Batch& B, const PassArgs& A) {
         return cmul(along ? KW_LD32(dl, (uint32_t)n) : dc, s);
       };
       auto st = [&](int, int n, cf v, int) { KW_ST32(S, addr(n), v); };
-      LS::template fft<1, true>(c, pl, g, G, active, src, st, la);
+      LS::template fft<1, true>(c, pl, g, G, active, src, st, lw);
     }
   } else {
     // SM_ABS: S0 <- nabla1 .* S0, S1 <- nabla2 .* S1;  SM_FILTER: one table
@@ -357,7 +365,7 @@ KW_BODY void body_spec(Ctx& c, const Batch& B, const PassArgs& A) {
       LS::template fft<1, false>(c, pl, g, G, active, ld, to_a, la);
       auto src = [&](int, int n) -> cf { return cscale(la.get(0, n), KW_LD32(tab, addr(n))); };
       auto st = [&](int, int n, cf v, int) { KW_ST32(S, addr(n), v); };
-      LS::template fft<1, true>(c, pl, g, G, active, src, st, la);
+      LS::template fft<1, true>(c, pl, g, G, active, src, st, lw);
     }
   }
 }

@@ -218,7 +218,7 @@ struct Sim {
     const int N = (k == PK_YFWD || k == PK_YINV) ? G.Ny
                   : (k == PK_SPEC)               ? (G.dim == 3 ? G.Nz : G.Ny)
                                                  : G.Nx;
-    return KW_IS_FIXED_SIZE(N) ? N : 0;
+    return fixed_variant(N, k);
   }
 
   void pass(PassKind k, const PassArgs& a) {

@@ -16,6 +16,7 @@ Each result equals what kspaceFirstOrder2D/3D returns for the same inputs.
 from __future__ import annotations
 
 import ctypes as C
+import time
 
 from . import _abi
 from .solver import _parse_options, alloc_outputs, prepare, shape_outputs
@@ -67,12 +68,21 @@ class Engine:
                 "steps_done": info.steps_done, "steps_total": info.steps_total,
                 "grid_cells": info.grid_cells, "device_bytes": info.device_bytes}
 
-    def wait(self, ticket: Ticket):
+    def wait(self, ticket: Ticket, with_info: bool = False):
         """Block until the simulation has finished and return its sensor data
-        (raises KWError if it failed or was cancelled)."""
+        (raises KWError if it failed or was cancelled).  With `with_info` the
+        ticket's final poll() record (engine-clock times) is returned too."""
+        info = None
+        if with_info:
+            while True:
+                info = self.poll(ticket)
+                if info["status"] in ("done", "failed", "cancelled"):
+                    break
+                time.sleep(0.0005)
         out, arrays = alloc_outputs(ticket.pp)
         _abi.call(self.lib.kw_wait, self.h, C.c_int64(ticket.id), C.byref(out))
-        return shape_outputs(ticket.pp, arrays)
+        res = shape_outputs(ticket.pp, arrays)
+        return (res, info) if with_info else res
 
     def cancel(self, ticket: Ticket):
         _abi.call(self.lib.kw_cancel, self.h, C.c_int64(ticket.id))

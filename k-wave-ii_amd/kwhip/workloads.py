@@ -11,6 +11,18 @@
   (seeds 0 and 1), power-law absorption 0.75 dB/(MHz^1.5 cm), PML 10, a 1 MHz
   10-cycle Hann-windowed 1 Pa tone burst on a disc of radius 30 points in the
   plane z = PML + 2, CFL 0.3, 1000 steps, full-grid p_max and p_rms.
+* ``batch_tr_problem``: one problem of the batched photoacoustic workload:
+  a 2D grid from 256^2 to 1024^2 or a 3D grid from 64^3 to 192^3 (sizes
+  2^a 3^b 5^c), homogeneous c0 ~ U[1450, 1600] m/s, p0 = 5-20 random discs
+  (2D) or tubes (3D) of radius 1-4 points and amplitude U[0.5, 1], a line
+  (2D) or plane (3D) sensor on one face just inside the PML, time from
+  makeTime (CFL 0.3); the forward run records p, the time-reversal run
+  re-injects it.
+* ``served_problem``: one request of the served workload: a 2D grid from
+  256^2 to 1024^2 or a 3D grid from 96^3 to 256^3 with the heterogeneous
+  medium recipe (smoothed-noise c and rho, power-law absorption y = 1.5,
+  B/A = 6) and a 1 MHz tone-burst disc source, a given number of steps,
+  p_max and p_rms on the whole grid.
 """
 
 from __future__ import annotations
@@ -83,4 +95,101 @@ def cfg2(N=256, Nt=1000, pml=10, src_radius=30, seeds=(0, 1)):
     source = {"p": sig[None, :], "p_mask": mask, "p_mode": "additive"}
     sensor = {"mask": np.ones(shape, dtype=bool), "record": ["p_max", "p_rms"]}
     opts = {"PMLSize": pml, "PMLAlpha": 2.0}
+    return kg, medium, source, sensor, opts
+
+
+def fft_friendly(lo, hi):
+    """Sizes 2^a 3^b 5^c in [lo, hi], sorted."""
+    out = []
+    for n in range(lo, hi + 1):
+        m = n
+        for p in (2, 3, 5):
+            while m % p == 0:
+                m //= p
+        if m == 1:
+            out.append(n)
+    return out
+
+
+SIZES_2D = fft_friendly(256, 1024)
+SIZES_3D = fft_friendly(64, 192)
+SIZES_3D_SERVED = fft_friendly(96, 256)
+
+
+def _random_p0(rng, shape):
+    """5-20 discs (2D) or tubes along a random axis (3D), radius 1-4 points,
+    amplitude U[0.5, 1]."""
+    dim = len(shape)
+    p0 = np.zeros(shape)
+    grids = np.meshgrid(*[np.arange(n) for n in shape], indexing="ij")
+    for _ in range(int(rng.integers(5, 21))):
+        r = rng.uniform(1.0, 4.0)
+        amp = rng.uniform(0.5, 1.0)
+        c = [rng.uniform(0.25 * n, 0.75 * n) for n in shape]
+        if dim == 2:
+            d2 = (grids[0] - c[0]) ** 2 + (grids[1] - c[1]) ** 2
+        else:
+            ax = int(rng.integers(0, 3))
+            others = [a for a in range(3) if a != ax]
+            d2 = sum((grids[a] - c[a]) ** 2 for a in others)
+            lo, hi = sorted(rng.uniform(0.2 * shape[ax], 0.8 * shape[ax], size=2))
+            d2 = np.where((grids[ax] >= lo) & (grids[ax] <= hi), d2, np.inf)
+        p0 = np.where(d2 <= r * r, np.maximum(p0, amp), p0)
+    return p0
+
+
+def batch_tr_problem(seed, dim=None):
+    """(kgrid, medium, source, sensor, opts) of one batched forward problem;
+    the time-reversal run uses the same grid and medium with
+    sensor.time_reversal_boundary_data set to the forward run's p."""
+    rng = np.random.default_rng(seed)
+    if dim is None:
+        dim = int(rng.integers(2, 4))
+    N = int(rng.choice(SIZES_2D if dim == 2 else SIZES_3D))
+    shape = (N,) * dim
+    dx = 1e-4
+    c0 = float(rng.uniform(1450.0, 1600.0))
+    pml = 20 if dim == 2 else 10
+    kg = kWaveGrid(*sum(([n, dx] for n in shape), []))
+    kg.makeTime(c0, 0.3)
+    medium = {"sound_speed": c0, "density": 1000.0}
+    source = {"p0": _random_p0(rng, shape)}
+    mask = np.zeros(shape, dtype=bool)
+    mask[pml] = True  # the face x = PML + 1 (1-based)
+    sensor = {"mask": mask}
+    opts = {"PMLSize": pml, "PMLAlpha": 2.0, "Smooth": False}
+    return kg, medium, source, sensor, opts
+
+
+def served_problem(seed, steps, dim=None):
+    """(kgrid, medium, source, sensor, opts) of one served request."""
+    rng = np.random.default_rng(seed)
+    if dim is None:
+        dim = int(rng.integers(2, 4))
+    N = int(rng.choice(SIZES_2D if dim == 2 else SIZES_3D_SERVED))
+    shape = (N,) * dim
+    dx = 1e-4
+    c = np.clip(1500.0 + 100.0 * smoothed_noise(shape, int(rng.integers(1 << 30)), 4.0),
+                1400.0, 1600.0)
+    rho = np.clip(1000.0 + 100.0 * smoothed_noise(shape, int(rng.integers(1 << 30)), 4.0),
+                  900.0, 1100.0)
+    kg = kWaveGrid(*sum(([n, dx] for n in shape), []))
+    dt = 0.3 * dx / float(c.max())
+    kg.setTime(int(steps), dt)
+    medium = {"sound_speed": c, "density": rho, "alpha_coeff": 0.75, "alpha_power": 1.5,
+              "BonA": 6.0}
+    f0 = 1e6
+    n_sig = min(int(steps), int(math.floor(10 / f0 / dt)) + 1)
+    t = np.arange(n_sig) * dt
+    sig = 1e6 * np.sin(2 * math.pi * f0 * t) * np.hanning(n_sig)
+    pml = 20 if dim == 2 else 10
+    mask = np.zeros(shape)
+    r = max(4, N // 8)
+    if dim == 2:
+        mask[pml + 1, N // 2 - r:N // 2 + r + 1] = 1
+    else:
+        mask[:, :, pml + 1] = disc_mask(N, N, N // 2 + 1, N // 2 + 1, r)
+    source = {"p": sig[None, :], "p_mask": mask, "p_mode": "additive"}
+    sensor = {"mask": np.ones(shape, dtype=bool), "record": ["p_max", "p_rms"]}
+    opts = {"PMLSize": pml, "PMLAlpha": 2.0, "Smooth": False}
     return kg, medium, source, sensor, opts

@@ -52,6 +52,14 @@ def test_emu_parity_3d(name):
     (3, "usrc_dir", (128, 16, 16)),
     (3, "no_dispersion", (8, 8, 256)),
     (3, "psrc_dir", (16, 384, 16)),
+    # lengths that are not multiples of 8 elements per thread, odd lengths,
+    # and the long lengths built without strided-pass variants
+    (2, "p0_hetero_absorbing", (75, 30)),
+    (3, "psrc_add", (16, 81, 16)),
+    (2, "time_reversal", (135, 24)),
+    (3, "p0_nonlinear_absorbing", (12, 10, 135)),
+    (2, "psrc_label", (270, 16)),
+    (2, "stokes", (16, 270)),
 ])
 def test_emu_parity_fixed_lengths(dim, name, shape):
     _check(cases.case(name, dim, shape))

@@ -51,6 +51,11 @@ MIX = [
     (3, "psrc_label", (20, 16, 15)),
     (3, "time_reversal", (16, 16, 16)),
     (3, "cartesian_nearest", (18, 20, 16)),
+    # sizes with fixed-length kernels: same-size groups on their own streams
+    (2, "p0_hetero_absorbing", (64, 64)),
+    (2, "psrc_add", (64, 64)),
+    (2, "p0_lossless", (75, 75)),
+    (2, "time_reversal", (75, 75)),
 ]
 
 
