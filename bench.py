@@ -106,10 +106,19 @@ def batched_bench(n_problems, seed0=0):
     t0 = time.perf_counter()
     with kwhip.Engine() as eng:
         fwd = [eng.submit(kg, med, src, sens, kg.dim, **opts) for kg, med, src, sens, opts in probs]
-        trs = []
-        for (kg, med, src, sens, opts), t in zip(probs, fwd):
-            data = eng.wait(t)
-            trs.append(eng.submit(kg, med, {}, _tr_sensor(sens, data), kg.dim, **opts))
+        # each reconstruction is submitted as soon as its forward run is done
+        trs = [None] * len(probs)
+        left = list(range(len(probs)))
+        while left:
+            done = [i for i in left if eng.poll(fwd[i])["status"] != "running"
+                    and eng.poll(fwd[i])["status"] != "queued"]
+            for i in done:
+                kg, med, src, sens, opts = probs[i]
+                data = eng.wait(fwd[i])
+                trs[i] = eng.submit(kg, med, {}, _tr_sensor(sens, data), kg.dim, **opts)
+                left.remove(i)
+            if not done:
+                time.sleep(0.002)
         rec_eng = [eng.wait(t) for t in trs]
         st = eng.stats()
     t_eng = time.perf_counter() - t0

@@ -205,10 +205,12 @@ const char* kw_kind_name(int k);
  * admission thread moves queued simulations to the device while their
  * estimated device footprint fits the engine's budget; a stepping thread
  * advances all admitted simulations together, one launch per pass for all
- * simulations with the same dimension and equation of state, whatever their
- * grid sizes, media, sources, sensors and current step.  Results are the
- * same as kw_run on the same problem (the per-simulation arithmetic does
- * not change).  Every engine function is thread-safe.
+ * simulations with the same dimension and equation of state (and the same
+ * grid size where fixed-length kernels exist for it), whatever their media,
+ * sources, sensors and current step.  Groups run on a pool of streams and
+ * each group's rounds of steps are captured once as a graph and replayed.
+ * Results are the same as kw_run on the same problem (the per-simulation
+ * arithmetic does not change).  Every engine function is thread-safe.
  * ------------------------------------------------------------------------- */
 typedef struct kw_engine kw_engine;
 

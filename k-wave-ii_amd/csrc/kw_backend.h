@@ -43,6 +43,16 @@ void dev_event_record(void* e, void* stream);
 double dev_event_elapsed_ms(void* a, void* b);  // synchronises on b
 bool dev_event_done(void* e);                   // the work before e has finished
 
+// Graphs of launches: capture what one host thread launches on `stream`
+// between begin and end into an executable graph, replay it, destroy it.
+// dev_capture_begin returns false when the backend cannot capture (the
+// emulator); the launches then have to be issued directly.
+bool dev_capture_begin(void* stream);
+void* dev_capture_end(void* stream);
+void dev_capture_abort(void* stream);  // end a capture and drop what it recorded
+void dev_graph_launch(void* exec, void* stream);
+void dev_graph_destroy(void* exec);
+
 // largest dynamic LDS per block the kernels may request
 size_t dev_max_lds();
 

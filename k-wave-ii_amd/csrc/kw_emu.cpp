@@ -360,6 +360,11 @@ void* dev_event_create() { return new EmuEvent(); }
 void dev_event_destroy(void* e) { delete (EmuEvent*)e; }
 void dev_event_record(void* e, void*) { ((EmuEvent*)e)->t = std::chrono::steady_clock::now(); }
 bool dev_event_done(void*) { return true; }  // emulated launches are synchronous
+bool dev_capture_begin(void*) { return false; }
+void* dev_capture_end(void*) { throw DeviceError(-3, "emulator: no graph capture"); }
+void dev_capture_abort(void*) {}
+void dev_graph_launch(void*, void*) { throw DeviceError(-3, "emulator: no graph capture"); }
+void dev_graph_destroy(void*) {}
 double dev_event_elapsed_ms(void* a, void* b) {
   return std::chrono::duration<double, std::milli>(((EmuEvent*)b)->t - ((EmuEvent*)a)->t)
       .count();

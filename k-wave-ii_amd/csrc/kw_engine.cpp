@@ -228,6 +228,8 @@ struct Group {
   void* stream = nullptr;
   void* ev = nullptr;       // recorded after the last round issued
   bool in_flight = false;   // that round has not finished
+  void* graph = nullptr;    // captured round of graph_k steps of this membership
+  int64_t graph_k = 0, graph_launches = 0;
   std::vector<Ticket*> members, joining;
   std::vector<Sim*> sims;
   bool dirty = true;
@@ -243,9 +245,19 @@ struct Group {
   Group(const Group&) = delete;
   Group& operator=(const Group&) = delete;
   ~Group() {
+    dev_graph_destroy(graph);
     for (void* p : live) dev_free(p);
     for (void* p : retired) dev_free(p);
     if (ev) dev_event_destroy(ev);
+  }
+
+  // a round can be captured once and replayed when nothing in it depends on
+  // host state that changes from step to step (the per-step source grids of
+  // several-series additive sources are chosen on the host)
+  bool graphable() const {
+    for (Sim* s : sims)
+      if (s->p_grid || s->u_grid[0] || s->u_grid[1] || s->u_grid[2]) return false;
+    return true;
   }
 
   // tables of earlier memberships may be read by launches still queued;
@@ -268,6 +280,9 @@ struct Group {
   }
 
   void rebuild() {
+    dev_graph_destroy(graph);  // called between rounds: the stream is idle
+    graph = nullptr;
+    graph_k = 0;
     for (void* p : live) retired.push_back(p);
     live.clear();
     const int n = (int)sims.size();
@@ -427,6 +442,7 @@ struct Engine {
   int64_t next_id = 1;
   int64_t budget = 0, held = 0;  // device bytes: budget, held (estimates until created)
   int32_t steps_per_round = 32;
+  bool use_graphs = true;
   bool stop = false;
   std::vector<void*> streams;  // compute streams, one per group (round robin)
   void* admit_stream = nullptr;
@@ -636,7 +652,31 @@ struct Engine {
           int64_t k = steps_per_round;
           for (Sim* s : g.sims) k = std::min(k, s->Nt_eff - s->steps_done);
           const int64_t l0 = g.launches;
-          for (int64_t i = 0; i < k; ++i) group_step(g);
+          if (use_graphs && k == steps_per_round && g.graphable()) {
+            // full rounds of a membership are captured once and replayed:
+            // every kernel takes its step from the device, so the launches
+            // of one round are the same for every round
+            if (!g.graph && dev_capture_begin(g.stream)) {
+              try {
+                for (int64_t i = 0; i < k; ++i) group_step(g);
+              } catch (...) {
+                dev_capture_abort(g.stream);
+                throw;
+              }
+              g.graph = dev_capture_end(g.stream);
+              g.graph_k = k;
+              g.graph_launches = g.launches - l0;
+              dev_graph_launch(g.graph, g.stream);
+            } else if (g.graph) {
+              dev_graph_launch(g.graph, g.stream);
+              for (Sim* s : g.sims) s->steps_done += g.graph_k;
+              g.launches += g.graph_launches;
+            } else {
+              for (int64_t i = 0; i < k; ++i) group_step(g);
+            }
+          } else {
+            for (int64_t i = 0; i < k; ++i) group_step(g);
+          }
           dev_event_record(g.ev, g.stream);
           g.in_flight = true;
           issued = busy = true;
@@ -728,6 +768,7 @@ int kw_engine_create(const kw_engine_config* cfg, kw_engine** engine, char* err,
     kw::dev_mem_info(&fr, &tot);
     E.budget = (cfg && cfg->device_bytes > 0) ? cfg->device_bytes : (int64_t)((double)fr * 0.9);
     E.steps_per_round = (cfg && cfg->steps_per_round > 0) ? cfg->steps_per_round : 32;
+    E.use_graphs = std::getenv("KW_NO_GRAPHS") == nullptr;
     // as many compute streams as the device has hardware queues for this
     // process by default (4), so that groups overlap without oversubscribing
     for (int i = 0; i < 4; ++i) E.streams.push_back(kw::dev_stream_create());

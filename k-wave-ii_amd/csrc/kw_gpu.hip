@@ -148,6 +148,34 @@ void dev_event_destroy(void* e) {
 void dev_event_record(void* e, void* s) {
   check(hipEventRecord((hipEvent_t)e, (hipStream_t)s), "hipEventRecord");
 }
+// thread-local capture: the admission thread may allocate and copy while the
+// stepping thread captures
+bool dev_capture_begin(void* s) {
+  check(hipStreamBeginCapture((hipStream_t)s, hipStreamCaptureModeThreadLocal),
+        "hipStreamBeginCapture");
+  return true;
+}
+void* dev_capture_end(void* s) {
+  hipGraph_t g = nullptr;
+  check(hipStreamEndCapture((hipStream_t)s, &g), "hipStreamEndCapture");
+  hipGraphExec_t x = nullptr;
+  const hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(g);
+  check(e, "hipGraphInstantiate");
+  return (void*)x;
+}
+void dev_capture_abort(void* s) {
+  hipGraph_t g = nullptr;
+  (void)hipStreamEndCapture((hipStream_t)s, &g);
+  if (g) (void)hipGraphDestroy(g);
+}
+void dev_graph_launch(void* x, void* s) {
+  check(hipGraphLaunch((hipGraphExec_t)x, (hipStream_t)s), "hipGraphLaunch");
+}
+void dev_graph_destroy(void* x) {
+  if (x) (void)hipGraphExecDestroy((hipGraphExec_t)x);
+}
+
 bool dev_event_done(void* e) {
   const hipError_t r = hipEventQuery((hipEvent_t)e);
   if (r == hipErrorNotReady) return false;
