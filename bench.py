@@ -208,6 +208,73 @@ def served_bench(n_req, rate, steps_lo, steps_hi, seed=7):
                       f"(seed {seed}), {cell_steps / 1e9:.1f} G cell-steps"}
 
 
+COPY_BYTES = 1 << 30  # calibration copy: 1 GiB read + 1 GiB written per dispatch
+
+
+def copy_calibration():
+    """A plain device-to-device copy of COPY_BYTES (torch's copy kernel), five
+    times: the reference dispatch for scaling the FETCH_SIZE / WRITE_SIZE
+    counters to bytes."""
+    import torch
+
+    x = torch.ones(COPY_BYTES // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    log(f"[bench] copy calibration: 5 copies of {COPY_BYTES} bytes")
+
+
+def _counter_means(run_dir, counter, name_part):
+    """Mean per-dispatch value of a counter over the dispatches whose kernel
+    name contains name_part, from a rocprofv3 run directory (CSV output or
+    the rocpd SQLite database)."""
+    import csv
+    import sqlite3
+
+    per = {}
+    for f in Path(run_dir).rglob("*counter_collection.csv"):
+        for r in csv.DictReader(open(f)):
+            if r.get("Counter_Name") == counter and name_part in r.get("Kernel_Name", ""):
+                key = (str(f), r.get("Dispatch_Id"))
+                per[key] = per.get(key, 0.0) + float(r["Counter_Value"])
+    for f in Path(run_dir).rglob("*.db"):
+        con = sqlite3.connect(str(f))
+        rows = con.execute("select dispatch_id, name, counter_value from pmc_events "
+                           "where counter_name = ?", (counter,)).fetchall()
+        for dispatch, name, value in rows:
+            if name_part in name:
+                key = (str(f), dispatch)
+                per[key] = per.get(key, 0.0) + float(value)
+    if not per:
+        raise SystemExit(f"no {counter} values for kernels matching {name_part!r} in {run_dir}")
+    return sum(per.values()) / len(per), len(per)
+
+
+def reduce_traffic(fetch_dir, write_dir, cal_fetch_dir, cal_write_dir, kernel, variant, N):
+    """HBM bytes per launch of one kernel from FETCH_SIZE and WRITE_SIZE runs
+    (KiB per dispatch), each scaled by factor = known bytes / counter bytes of
+    the calibration copy; written to profiles/traffic_<kernel>_cfg2_<N>.json,
+    which the bench reports as roofline.traffic."""
+    f_kib, nf = _counter_means(fetch_dir, "FETCH_SIZE", variant)
+    w_kib, nw = _counter_means(write_dir, "WRITE_SIZE", variant)
+    # the calibration copy runs as the runtime's copy kernel (copyBuffer)
+    cf_kib, _ = _counter_means(cal_fetch_dir, "FETCH_SIZE", "copyBuffer")
+    cw_kib, _ = _counter_means(cal_write_dir, "WRITE_SIZE", "copyBuffer")
+    f_scale = COPY_BYTES / (cf_kib * 1024.0)
+    w_scale = COPY_BYTES / (cw_kib * 1024.0)
+    fetch_b = f_kib * 1024.0 * f_scale
+    write_b = w_kib * 1024.0 * w_scale
+    out = {"kernel": kernel, "variant": variant, "N": N,
+           "bytes_per_launch": fetch_b + write_b, "fetch_bytes": fetch_b, "write_bytes": write_b,
+           "fetch_kib_raw": f_kib, "write_kib_raw": w_kib, "dispatches": [nf, nw],
+           "copy_fetch_kib": cf_kib, "copy_write_kib": cw_kib,
+           "fetch_factor": f_scale, "write_factor": w_scale}
+    path = ROOT / "profiles" / f"traffic_{kernel}_cfg2_{N}.json"
+    path.write_text(json.dumps(out, indent=1) + "\n")
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -224,7 +291,19 @@ def main():
                     help="requests in the served leg (0: skip)")
     ap.add_argument("--served-rate", type=float, default=10.0)
     ap.add_argument("--served-steps", type=int, nargs=2, default=[200, 600])
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only the calibration copy (under rocprofv3 --pmc)")
+    ap.add_argument("--reduce-traffic", nargs=4, metavar="DIR",
+                    help="FETCH_SIZE run, WRITE_SIZE run, and the two calibration runs")
+    ap.add_argument("--traffic-kernel", default="spectral")
+    ap.add_argument("--traffic-variant", default="kv_2_256")
     args = ap.parse_args()
+    if args.copy_calibration:
+        copy_calibration()
+        return
+    if args.reduce_traffic:
+        reduce_traffic(*args.reduce_traffic, args.traffic_kernel, args.traffic_variant, args.N)
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
               "(independent simulations shard across GPUs without communication)", file=sys.stderr)
