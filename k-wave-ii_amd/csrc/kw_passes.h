@@ -206,6 +206,27 @@ KW_BODY void with_src(const TimeSource& s, bool on, F&& f) {
   }
 }
 
+// Time source applied to this thread's values of the two rows (a: row r0,
+// b: row r1) of an x pass, in a phase of its own after the update, so that
+// the source's loads (up to three dependent loads per cell) are not batched
+// together with the update's; nothing when the source is off at step t.
+template <class XS>
+KW_BODY void x_source(const ProbDev& P, const XTile& T, const TimeSource& src, int t, float* av,
+                      float* bv) {
+  with_src(src, src.len >= t, [&](auto FORM) {
+    constexpr int F = decltype(FORM)::value;
+    if constexpr (F >= 0) {
+      KW_PHASE_FENCE();
+      const float s0 = src_value0<F>(src, t);
+      auto f = [&](int slot, int n) {
+        av[slot] = apply_src<F>(src, T.c0 + n, av[slot], t, s0);
+        bv[slot] = apply_src<F>(src, T.c1 + n, bv[slot], t, s0);
+      };
+      XS::last_slots(P.fx, T.g, T.G, T.active, f);
+    }
+  });
+}
+
 template <class Ctx>
 KW_BODY const ProbDev& batch_prob(Ctx& c, const Batch& B, int& tile) {
   const int pi = find_prob<Ctx>(B, c.bid());
@@ -315,17 +336,30 @@ KW_BODY void body_spec(Ctx& c, const Batch& B, const PassArgs& A) {
   // would read la while its twin overwrites it)
   const LdsPencils& lw = LS::fixed ? la : lb;
   auto addr = [&](int n) -> uint32_t { return base + (uint32_t)n * stride; };
+  // operator tables are stored in the order of this pass: element n of
+  // pencil p of tile `tile` at (tile N + n) Pn + p (padded to whole tiles,
+  // so threads without a pencil read in bounds too)
+  const uint32_t trow = (uint32_t)tile * (uint32_t)N;
+  auto taddr = [&](int n) -> uint32_t {
+    return (trow + (uint32_t)n) * (uint32_t)Pn + (uint32_t)p;
+  };
   auto to_a = [&](int, int n, cf v, int) { la.set(0, n, v); };
+  // kappa is applied as the forward transform ends, so the gradient's two
+  // inverse transforms share one read of it (the kernel is at the 128-VGPR
+  // limit of four waves per SIMD, so kappa is not kept in registers)
+  auto to_ak = [&](int, int n, cf v, int) {
+    la.set(0, n, cscale(v, KW_LD32(P.kappa, taddr(n))));
+  };
 
   if (A.mode == SM_GRAD) {
     cf* S0 = P.S[0];
     auto ld = [&](int, int n) -> cf { return KW_LD32(S0, addr(n)); };
-    LS::template fft<1, false>(c, pl, g, G, active, ld, to_a, la);
+    LS::template fft<1, false>(c, pl, g, G, active, ld, to_ak, la);
     // C = ifft(kappa .* S); x (and in 3D y) derivatives are per-pencil
     // constants times C
     const cf dx = active ? KW_LD(P.dpos[0] + kx) : cf{0.f, 0.f};
     const cf dy = (active && d3) ? KW_LD(P.dpos[1] + ky) : cf{0.f, 0.f};
-    auto src1 = [&](int, int n) -> cf { return cscale(la.get(0, n), KW_LD32(P.kappa, addr(n))); };
+    auto src1 = [&](int, int n) -> cf { return la.get(0, n); };
     auto st1 = [&](int, int n, cf v, int) {
       KW_ST32(P.S[0], addr(n), cmul(dx, v));
       if (d3) KW_ST32(P.S[1], addr(n), cmul(dy, v));
@@ -333,9 +367,7 @@ KW_BODY void body_spec(Ctx& c, const Batch& B, const PassArgs& A) {
     LS::template fft<1, true>(c, pl, g, G, active, src1, st1, lb);
     // derivative along the pencil axis
     const cf* dl = P.dpos[last];
-    auto src2 = [&](int, int n) -> cf {
-      return cmul(KW_LD32(dl, (uint32_t)n), cscale(la.get(0, n), KW_LD32(P.kappa, addr(n))));
-    };
+    auto src2 = [&](int, int n) -> cf { return cmul(KW_LD32(dl, (uint32_t)n), la.get(0, n)); };
     cf* So = P.S[last];
     auto st2 = [&](int, int n, cf v, int) { KW_ST32(So, addr(n), v); };
     LS::template fft<1, true>(c, pl, g, G, active, src2, st2, lb);
@@ -343,14 +375,13 @@ KW_BODY void body_spec(Ctx& c, const Batch& B, const PassArgs& A) {
     for (int d = 0; d < P.dim; ++d) {
       cf* S = P.S[d];
       auto ld = [&](int, int n) -> cf { return KW_LD32(S, addr(n)); };
-      LS::template fft<1, false>(c, pl, g, G, active, ld, to_a, la);
+      LS::template fft<1, false>(c, pl, g, G, active, ld, to_ak, la);
       const cf dc = (d == last || !active) ? cf{0.f, 0.f}
                                            : KW_LD(P.dneg[d] + (d == 0 ? kx : ky));
       const cf* dl = P.dneg[d];
       const bool along = d == last;
       auto src = [&](int, int n) -> cf {
-        const cf s = cscale(la.get(0, n), KW_LD32(P.kappa, addr(n)));
-        return cmul(along ? KW_LD32(dl, (uint32_t)n) : dc, s);
+        return cmul(along ? KW_LD32(dl, (uint32_t)n) : dc, la.get(0, n));
       };
       auto st = [&](int, int n, cf v, int) { KW_ST32(S, addr(n), v); };
       LS::template fft<1, true>(c, pl, g, G, active, src, st, lw);
@@ -363,7 +394,7 @@ KW_BODY void body_spec(Ctx& c, const Batch& B, const PassArgs& A) {
       const float* tab = A.mode == SM_ABS ? (j == 0 ? P.nabla1 : P.nabla2) : A.table;
       auto ld = [&](int, int n) -> cf { return KW_LD32(S, addr(n)); };
       LS::template fft<1, false>(c, pl, g, G, active, ld, to_a, la);
-      auto src = [&](int, int n) -> cf { return cscale(la.get(0, n), KW_LD32(tab, addr(n))); };
+      auto src = [&](int, int n) -> cf { return cscale(la.get(0, n), KW_LD32(tab, taddr(n))); };
       auto st = [&](int, int n, cf v, int) { KW_ST32(S, addr(n), v); };
       LS::template fft<1, true>(c, pl, g, G, active, src, st, lw);
     }
@@ -466,8 +497,7 @@ KW_BODY void body_x_grad(Ctx& cb, const Batch& B, const PassArgs& A) {
     const Field ri = P.rinv[d];
     const float* pv = P.pml_sg[d];
     float av[kMaxSlots], bv[kMaxSlots];
-    auto run = [&](auto INIT, auto FORM) {
-      const float s0 = src_value0<decltype(FORM)::value>(P.srcu[d], t);
+    auto run = [&](auto INIT) {
       auto upd = [&](int slot, int n) {
         const uint32_t c0 = T.c0 + n, c1 = T.c1 + n;
         float a, b;
@@ -478,20 +508,19 @@ KW_BODY void body_x_grad(Ctx& cb, const Batch& B, const PassArgs& A) {
           const float p0 = pml_at(pv, T, d, 0, n), p1 = pml_at(pv, T, d, 1, n);
           a = p0 * (p0 * KW_LD32(u, c0) - dt * ri.at(c0) * gv[slot].x);
           b = p1 * (p1 * KW_LD32(u, c1) - dt * ri.at(c1) * gv[slot].y);
-          if constexpr (decltype(FORM)::value >= 0) {
-            a = apply_src<decltype(FORM)::value>(P.srcu[d], c0, a, t, s0);
-            b = apply_src<decltype(FORM)::value>(P.srcu[d], c1, b, t, s0);
-          }
         }
         av[slot] = a;
         bv[slot] = b;
       };
       XS::last_slots(P.fx, T.g, T.G, T.active, upd);
     };
-    if (init)
-      run(BoolC<true>{}, IntC<-1>{});
-    else
-      with_src(P.srcu[d], P.srcu[d].len >= t, [&](auto FORM) { run(BoolC<false>{}, FORM); });
+    if (init) {
+      run(BoolC<true>{});
+    } else {
+      run(BoolC<false>{});
+      // velocity source (kspaceFirstOrder3D.m:806-886) as a phase of its own
+      x_source<XS>(P, T, P.srcu[d], t, av, bv);
+    }
     KW_PHASE_FENCE();
     if (T.active) {
       auto put = [&](int slot, int n) {
@@ -653,67 +682,70 @@ KW_BODY void body_x_div(Ctx& cb, const Batch& B, const PassArgs& A) {
   const bool store_p = (A.flags & X_STORE_P) || t >= P.p_store_from || t == P.p_store_at;
 
   // sums over the components: velocity divergence (absorbing / stokes EOS)
-  // and density; mass conservation coefficient rho0 (linear) or
-  // 2 sum(rho) + rho0 (nonlinear, from the densities before this update)
+  // and density; mass conservation coefficient rho0 (linear, read where it
+  // is used) or 2 sum(rho) + rho0 (nonlinear, from the densities before this
+  // update, kept in registers)
   float du0[kMaxSlots], du1[kMaxSlots], rs0[kMaxSlots], rs1[kMaxSlots];
-  float k0[kMaxSlots], k1[kMaxSlots];
   with_flag(P.nonlinear != 0, [&](auto NL) {
+    constexpr bool nl = decltype(NL)::value;
+    float k0[nl ? kMaxSlots : 1], k1[nl ? kMaxSlots : 1];
     auto f = [&](int slot, int n) {
       const uint32_t c0 = T.c0 + n, c1 = T.c1 + n;
       du0[slot] = du1[slot] = rs0[slot] = rs1[slot] = 0.f;
-      float a = P.rho0.at(c0), b = P.rho0.at(c1);
-      if constexpr (NL.value) {
+      if constexpr (nl) {
         float s0 = KW_LD32(P.rho[0], c0) + KW_LD32(P.rho[1], c0);
         float s1 = KW_LD32(P.rho[0], c1) + KW_LD32(P.rho[1], c1);
         if (DIM == 3) {
           s0 = s0 + KW_LD32(P.rho[DIM - 1], c0);
           s1 = s1 + KW_LD32(P.rho[DIM - 1], c1);
         }
-        a = 2.f * s0 + a;
-        b = 2.f * s1 + b;
+        k0[slot] = 2.f * s0 + P.rho0.at(c0);
+        k1[slot] = 2.f * s1 + P.rho0.at(c1);
       }
-      k0[slot] = a;
-      k1[slot] = b;
     };
     XS::last_slots(P.fx, T.g, T.G, T.active, f);
-  });
 #pragma unroll 1
-  for (int d = 0; d < DIM; ++d) {
-    cf gv[kMaxSlots];
-    x_inverse<XS>(c, P, T, P.S[d], gv);
-    float* rho = P.rho[d];
-    const float* pv = P.pml[d];
-    float av[kMaxSlots], bv[kMaxSlots];
-    with_src(P.srcp, psrc, [&](auto FORM) {
-      const float s0 = src_value0<decltype(FORM)::value>(P.srcp, t);
+    for (int d = 0; d < DIM; ++d) {
+      cf gv[kMaxSlots];
+      x_inverse<XS>(c, P, T, P.S[d], gv);
+      float* rho = P.rho[d];
+      const float* pv = P.pml[d];
+      float av[kMaxSlots], bv[kMaxSlots];
       auto f = [&](int slot, int n) {
         const uint32_t c0 = T.c0 + n, c1 = T.c1 + n;
         const float p0 = pml_at(pv, T, d, 0, n), p1 = pml_at(pv, T, d, 1, n);
         const cf g = gv[slot];
-        float a = p0 * (p0 * KW_LD32(rho, c0) - dt * k0[slot] * g.x);
-        float b = p1 * (p1 * KW_LD32(rho, c1) - dt * k1[slot] * g.y);
-        if constexpr (decltype(FORM)::value >= 0) {
-          a = apply_src<decltype(FORM)::value>(P.srcp, c0, a, t, s0);
-          b = apply_src<decltype(FORM)::value>(P.srcp, c1, b, t, s0);
+        float ka, kb;
+        if constexpr (nl) {
+          ka = k0[slot];
+          kb = k1[slot];
+        } else {
+          ka = P.rho0.at(c0);
+          kb = P.rho0.at(c1);
         }
-        av[slot] = a;
-        bv[slot] = b;
+        av[slot] = p0 * (p0 * KW_LD32(rho, c0) - dt * ka * g.x);
+        bv[slot] = p1 * (p1 * KW_LD32(rho, c1) - dt * kb * g.y);
         du0[slot] = du0[slot] + g.x;
         du1[slot] = du1[slot] + g.y;
-        rs0[slot] = rs0[slot] + a;
-        rs1[slot] = rs1[slot] + b;
       };
       XS::last_slots(P.fx, T.g, T.G, T.active, f);
-    });
-    KW_PHASE_FENCE();
-    if (T.active) {
-      auto put = [&](int slot, int n) {
-        KW_ST32(rho, T.c1 + n, bv[slot]);
-        KW_ST32(rho, T.c0 + n, av[slot]);
+      // pressure source (kspaceFirstOrder3D.m:930-962) as a phase of its own
+      if (psrc) x_source<XS>(P, T, P.srcp, t, av, bv);
+      auto acc = [&](int slot, int) {
+        rs0[slot] = rs0[slot] + av[slot];
+        rs1[slot] = rs1[slot] + bv[slot];
       };
-      XS::last_slots(P.fx, T.g, T.G, T.active, put);
+      XS::last_slots(P.fx, T.g, T.G, T.active, acc);
+      KW_PHASE_FENCE();
+      if (T.active) {
+        auto put = [&](int slot, int n) {
+          KW_ST32(rho, T.c1 + n, bv[slot]);
+          KW_ST32(rho, T.c0 + n, av[slot]);
+        };
+        XS::last_slots(P.fx, T.g, T.G, T.active, put);
+      }
     }
-  }
+  });
   if (P.eos == EOS_ABSORBING) {
     float av[kMaxSlots], bv[kMaxSlots];
     auto ab = [&](int slot, int n) {
@@ -758,43 +790,45 @@ KW_BODY void body_x_eos(Ctx& cb, const Batch& B, const PassArgs& A) {
   PencilCtx<Ctx> c{cb, T.wave};
   const int t = KW_LD(P.t_index);
   const bool store_p = (A.flags & X_STORE_P) || t >= P.p_store_from || t == P.p_store_at;
-  // rsum + tau A first, then - eta B and the nonlinear term
-  cf part[kMaxSlots], rs[kMaxSlots];
-  {
-    cf gv[kMaxSlots];
-    x_inverse<XS>(c, P, T, P.S[0], gv);
-    auto f = [&](int slot, int n) {
-      const uint32_t c0 = T.c0 + n, c1 = T.c1 + n;
-      rs[slot] = cf{KW_LD32(P.rsum, c0), KW_LD32(P.rsum, c1)};
-      part[slot] = cf{rs[slot].x + P.tau.at(c0) * gv[slot].x, rs[slot].y + P.tau.at(c1) * gv[slot].y};
-    };
-    XS::last_slots(P.fx, T.g, T.G, T.active, f);
-  }
+  // rsum + tau A first, then - eta B and the nonlinear term; rsum is kept
+  // across the second inverse only for the nonlinear term
   cf pv[kMaxSlots];
-  {
-    cf gv[kMaxSlots];
-    x_inverse<XS>(c, P, T, P.S[1], gv);
-    with_flag(P.nonlinear != 0, [&](auto NL) {
+  with_flag(P.nonlinear != 0, [&](auto NL) {
+    constexpr bool nl = decltype(NL)::value;
+    cf part[kMaxSlots], rs[nl ? kMaxSlots : 1];
+    {
+      cf gv[kMaxSlots];
+      x_inverse<XS>(c, P, T, P.S[0], gv);
       auto f = [&](int slot, int n) {
-        const uint32_t cells[2] = {T.c0 + n, T.c1 + n};
-        const float pa[2] = {part[slot].x, part[slot].y};
-        const float ra[2] = {rs[slot].x, rs[slot].y};
-        const float ga[2] = {gv[slot].x, gv[slot].y};
-        float out[2];
-#pragma unroll
-        for (int h = 0; h < 2; ++h) {
-          const uint32_t cell = cells[h];
-          float inner = pa[h] - P.eta.at(cell) * ga[h];
-          if constexpr (decltype(NL)::value)
-            inner = inner + P.bona.at(cell) * (ra[h] * ra[h]) / (2.f * P.rho0.at(cell));
-          const float c0 = P.c0.at(cell);
-          out[h] = (c0 * c0) * inner;
-        }
-        pv[slot] = cf{out[0], out[1]};
+        const uint32_t c0 = T.c0 + n, c1 = T.c1 + n;
+        const cf r = cf{KW_LD32(P.rsum, c0), KW_LD32(P.rsum, c1)};
+        if constexpr (nl) rs[slot] = r;
+        part[slot] = cf{r.x + P.tau.at(c0) * gv[slot].x, r.y + P.tau.at(c1) * gv[slot].y};
       };
       XS::last_slots(P.fx, T.g, T.G, T.active, f);
-    });
-  }
+    }
+    cf gv[kMaxSlots];
+    x_inverse<XS>(c, P, T, P.S[1], gv);
+    auto f = [&](int slot, int n) {
+      const uint32_t cells[2] = {T.c0 + n, T.c1 + n};
+      const float pa[2] = {part[slot].x, part[slot].y};
+      const float ga[2] = {gv[slot].x, gv[slot].y};
+      float out[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const uint32_t cell = cells[h];
+        float inner = pa[h] - P.eta.at(cell) * ga[h];
+        if constexpr (nl) {
+          const float r = h == 0 ? rs[slot].x : rs[slot].y;
+          inner = inner + P.bona.at(cell) * (r * r) / (2.f * P.rho0.at(cell));
+        }
+        const float c0 = P.c0.at(cell);
+        out[h] = (c0 * c0) * inner;
+      }
+      pv[slot] = cf{out[0], out[1]};
+    };
+    XS::last_slots(P.fx, T.g, T.G, T.active, f);
+  });
   x_finish_p<XS>(c, P, T, pv, t, store_p);
   cf* S0[1] = {P.S[0]};
   x_fwd_from_lds<XS, 1>(c, P, T, S0);

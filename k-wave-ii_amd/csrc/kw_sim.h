@@ -389,25 +389,42 @@ struct Sim {
     return pl;
   }
 
-  // spectral-layout table of f(kx, ky, kz) (ifftshifted wavenumbers)
+  // Table of f(kx, ky, kz) (ifftshifted wavenumbers) over the half
+  // spectrum, stored in the order the last-axis pass reads it: element n of
+  // pencil p of tile t at (t * N + n) * Pl + p, the tiles of the main block
+  // first, then those of the Nyquist plane, padding pencils zero.  A block
+  // then reads whole contiguous lines of a table; in the natural layout its
+  // Pl floats per element would be half of a 128-byte line whose other half
+  // another block (on another L2) fetches again.
   template <class F>
   float* spectral_table(F f) {
-    std::vector<float> t(G.nspec);
     const Geometry g = G;
-    parallel_for(g.nspec, [&](int64_t e) {
-      int ix, iy, iz;
-      if (e < g.nspec_main) {
-        ix = (int)(e % g.Wm);
-        const int64_t r = e / g.Wm;
-        iy = (int)(r % g.Ny);
-        iz = (int)(r / g.Ny);
+    const int Pn = P.Pl;
+    const int N = g.dim == 3 ? g.Nz : g.Ny;
+    const int64_t nH = g.dim == 3 ? (int64_t)g.Wm * g.Ny : (int64_t)g.Wm;
+    const int64_t nQ = g.nyq ? (g.dim == 3 ? g.Ny : 1) : 0;
+    const int64_t tH = (nH + Pn - 1) / Pn;
+    const int64_t nt = ll.nb;
+    std::vector<float> t((size_t)(nt * N * Pn), 0.f);
+    parallel_for(nt * Pn, [&](int64_t e) {
+      const int64_t tile = e / Pn;
+      const int p = (int)(e % Pn);
+      int ix, iy;
+      if (tile < tH) {
+        const int64_t q = tile * Pn + p;
+        if (q >= nH) return;
+        ix = (int)(q % g.Wm);
+        iy = (int)(q / g.Wm);
       } else {
+        const int64_t q = (tile - tH) * Pn + p;
+        if (q >= nQ) return;
         ix = g.Nx / 2;
-        const int64_t r = e - g.nspec_main;
-        iy = (int)(r % g.Ny);
-        iz = (int)(r / g.Ny);
+        iy = (int)q;
       }
-      t[e] = (float)f(ix, iy, iz);
+      for (int n = 0; n < N; ++n) {
+        const double v = g.dim == 3 ? f(ix, iy, n) : f(ix, n, 0);
+        t[(size_t)((tile * N + n) * Pn + p)] = (float)v;
+      }
     });
     return upload(t);
   }
