@@ -147,6 +147,22 @@ __device__ __forceinline__ void kw_st32(T* base, uint32_t i, V v) {
 #define KW_ST32(base, i, v) KW_ST(::kw::at32((base), (uint32_t)(i)), (v))
 #endif
 
+// The value of v, opaque to the optimiser.  The FFT engines derive every
+// LDS, twiddle and element index from the thread's index in its pencil; fed
+// a fresh copy of it on entry, each transform computes these indices where
+// it uses them, instead of sharing them with earlier transforms of the
+// kernel or hoisting them out of the loops over field components, which
+// keeps dozens of them live in registers across the whole kernel and cuts
+// the waves per SIMD.
+#ifdef KW_GPU
+__device__ __forceinline__ int kw_fresh(int v) {
+  __asm__ volatile("" : "+v"(v));
+  return v;
+}
+#else
+inline int kw_fresh(int v) { return v; }
+#endif
+
 // Scheduling fence between epilogue phases: each phase batches its own
 // loads, but loads are not hoisted across phases (which would hold every
 // phase's operands in registers at once).

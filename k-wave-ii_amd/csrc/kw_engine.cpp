@@ -353,7 +353,8 @@ struct Group {
   void pass(PassKind kind, const PassArgs& a) {
     const int k = (kind == PK_YFWD || kind == PK_YINV) ? GY : (kind == PK_SPEC ? GL : GX);
     const int fx = fixed[k == GX ? 0 : (k == GY ? 1 : 2)];
-    launch_pass(kind, fx, dim, geo[k].nb, geo[k].T, geo[k].lds, batch(k), a, stream);
+    launch_pass(kind, fx, dim, geo[k].nb, geo[k].T, launch_lds(kind, fx, a, geo[k].lds), batch(k),
+                a, stream);
     ++launches;
   }
 

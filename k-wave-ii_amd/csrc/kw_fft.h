@@ -301,6 +301,7 @@ template <int NP, bool INV, class Ctx, class Src, class Dst>
 KW_BODY void fft_rt(Ctx& c, const FftPlan& pl, int g, int G, bool active, Src& src, Dst& dst,
                     const LdsPencils& lds) {
   const int N = pl.N;
+  g = kw_fresh(g);
 #pragma unroll
   for (int i = 0; i < kRtSlots; ++i) {
     const int n = g + i * G;
@@ -328,6 +329,7 @@ KW_BODY void fft_rt(Ctx& c, const FftPlan& pl, int g, int G, bool active, Src& s
 // the (slot, n) pairs this thread owns after a runtime-length transform
 template <class F>
 KW_BODY void last_slots_rt(const FftPlan& pl, int g, int G, bool, F& f) {
+  g = kw_fresh(g);
 #pragma unroll
   for (int i = 0; i < kRtSlots; ++i) {
     const int n = g + i * G;
@@ -363,6 +365,7 @@ template <int N, int G, int NP, bool INV, class Ctx, class Src, class Dst>
 KW_BODY void fft_ct(Ctx& c, const cf* tw, int g, bool active, Src& src0, Dst& dstN,
                     const LdsPencils& lds) {
   static_assert(ct_nst(N) > 0, "length must factor into 2, 3 and 5");
+  g = kw_fresh(g);
   auto lsrc = [&](int j, int n) -> cf { return lds.get(j, n); };
   auto ldst = [&](int j, int n, cf v, int) { lds.set(j, n, v); };
   CtStages<N, G, 0, ct_nst(N), NP, INV>::run(c, g, active, tw, src0, dstN, lsrc, ldst);
@@ -375,6 +378,7 @@ KW_BODY void last_slots_ct(int g, bool, F& f) {
   constexpr int Ns = N / R;
   constexpr int nb = N / R;
   constexpr int MB = (nb + G - 1) / G;
+  g = kw_fresh(g);
 #pragma unroll
   for (int m = 0; m < MB; ++m) {
     const int b = g + m * G;

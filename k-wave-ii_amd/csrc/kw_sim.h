@@ -124,6 +124,15 @@ struct LaunchGeom {
   size_t lds = 0;
 };
 
+// LDS of a launch of pass k, whose geometry reserves `full` bytes.  The
+// last-axis pass holds two pencil buffers per pencil (la, lb); with a
+// fixed-length plan only the gradient uses lb (its two inverse transforms
+// both read la), so the other modes launch with la alone and fit twice as
+// many blocks per CU.
+inline size_t launch_lds(PassKind k, int fixed, const PassArgs& a, size_t full) {
+  return (k == PK_SPEC && fixed && a.mode != SM_GRAD) ? full / 2 : full;
+}
+
 inline int max_pencils_env() {
   const char* e = std::getenv("KW_MAX_PENCILS");
   if (!e) return 1 << 30;
@@ -228,7 +237,9 @@ struct Sim {
     const Batch B = batch();
     const int fx = fixed_len(k);
     timed((int)k, timing ? pass_bytes(k, a) : 0.0,
-          [&]() { launch_pass(k, fx, G.dim, g->nb, g->T, g->lds, B, a, stream); });
+          [&]() {
+            launch_pass(k, fx, G.dim, g->nb, g->T, launch_lds(k, fx, a, g->lds), B, a, stream);
+          });
   }
 
   void ew(const EwArgs& a, int nblocks) {
