@@ -682,16 +682,19 @@ KW_BODY void body_x_div(Ctx& cb, const Batch& B, const PassArgs& A) {
   const bool store_p = (A.flags & X_STORE_P) || t >= P.p_store_from || t == P.p_store_at;
 
   // sums over the components: velocity divergence (absorbing / stokes EOS)
-  // and density; mass conservation coefficient rho0 (linear, read where it
-  // is used) or 2 sum(rho) + rho0 (nonlinear, from the densities before this
-  // update, kept in registers)
+  // and density; mass conservation coefficient k = rho0 (linear) or
+  // 2 sum(rho) + rho0 (nonlinear, from the densities before this update),
+  // kept in registers across the components
   float du0[kMaxSlots], du1[kMaxSlots], rs0[kMaxSlots], rs1[kMaxSlots];
-  with_flag(P.nonlinear != 0, [&](auto NL) {
+  float k0[kMaxSlots], k1[kMaxSlots];
+  const bool nonlin = P.nonlinear != 0;
+  with_flag(nonlin, [&](auto NL) {
     constexpr bool nl = decltype(NL)::value;
-    float k0[nl ? kMaxSlots : 1], k1[nl ? kMaxSlots : 1];
     auto f = [&](int slot, int n) {
       const uint32_t c0 = T.c0 + n, c1 = T.c1 + n;
       du0[slot] = du1[slot] = rs0[slot] = rs1[slot] = 0.f;
+      k0[slot] = P.rho0.at(c0);
+      k1[slot] = P.rho0.at(c1);
       if constexpr (nl) {
         float s0 = KW_LD32(P.rho[0], c0) + KW_LD32(P.rho[1], c0);
         float s1 = KW_LD32(P.rho[0], c1) + KW_LD32(P.rho[1], c1);
@@ -699,8 +702,8 @@ KW_BODY void body_x_div(Ctx& cb, const Batch& B, const PassArgs& A) {
           s0 = s0 + KW_LD32(P.rho[DIM - 1], c0);
           s1 = s1 + KW_LD32(P.rho[DIM - 1], c1);
         }
-        k0[slot] = 2.f * s0 + P.rho0.at(c0);
-        k1[slot] = 2.f * s1 + P.rho0.at(c1);
+        k0[slot] = 2.f * s0 + k0[slot];
+        k1[slot] = 2.f * s1 + k1[slot];
       }
     };
     XS::last_slots(P.fx, T.g, T.G, T.active, f);
@@ -715,16 +718,8 @@ KW_BODY void body_x_div(Ctx& cb, const Batch& B, const PassArgs& A) {
         const uint32_t c0 = T.c0 + n, c1 = T.c1 + n;
         const float p0 = pml_at(pv, T, d, 0, n), p1 = pml_at(pv, T, d, 1, n);
         const cf g = gv[slot];
-        float ka, kb;
-        if constexpr (nl) {
-          ka = k0[slot];
-          kb = k1[slot];
-        } else {
-          ka = P.rho0.at(c0);
-          kb = P.rho0.at(c1);
-        }
-        av[slot] = p0 * (p0 * KW_LD32(rho, c0) - dt * ka * g.x);
-        bv[slot] = p1 * (p1 * KW_LD32(rho, c1) - dt * kb * g.y);
+        av[slot] = p0 * (p0 * KW_LD32(rho, c0) - dt * k0[slot] * g.x);
+        bv[slot] = p1 * (p1 * KW_LD32(rho, c1) - dt * k1[slot] * g.y);
         du0[slot] = du0[slot] + g.x;
         du1[slot] = du1[slot] + g.y;
       };
@@ -748,9 +743,10 @@ KW_BODY void body_x_div(Ctx& cb, const Batch& B, const PassArgs& A) {
   });
   if (P.eos == EOS_ABSORBING) {
     float av[kMaxSlots], bv[kMaxSlots];
+    // rho0 sum du (k is rho0 in the linear case)
     auto ab = [&](int slot, int n) {
-      av[slot] = P.rho0.at(T.c0 + n) * du0[slot];
-      bv[slot] = P.rho0.at(T.c1 + n) * du1[slot];
+      av[slot] = (nonlin ? P.rho0.at(T.c0 + n) : k0[slot]) * du0[slot];
+      bv[slot] = (nonlin ? P.rho0.at(T.c1 + n) : k1[slot]) * du1[slot];
     };
     XS::last_slots(P.fx, T.g, T.G, T.active, ab);
     if (T.active) {
