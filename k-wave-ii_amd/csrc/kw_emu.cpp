@@ -298,7 +298,7 @@ void emu_lds_access(const cf* p, bool write, const cf* value) {
   if (!ordered) lds_fault("read without a barrier after the write of lane", R->bid, me, i, w);
 }
 
-template <class Sel>
+template <class Sel, class SelX>
 static void emu_pass(PassKind kind, int dim, int nblocks, int threads, size_t lds_bytes,
                      const Batch& B, const PassArgs& A) {
   std::function<void(EmuCtx&)> f;
@@ -306,21 +306,21 @@ static void emu_pass(PassKind kind, int dim, int nblocks, int threads, size_t ld
     case PK_YFWD: f = [&](EmuCtx& c) { body_ypass<false, Sel>(c, B, A); }; break;
     case PK_YINV: f = [&](EmuCtx& c) { body_ypass<true, Sel>(c, B, A); }; break;
     case PK_SPEC: f = [&](EmuCtx& c) { body_spec<Sel>(c, B, A); }; break;
-    case PK_X_R2C: f = [&](EmuCtx& c) { body_x_r2c<Sel>(c, B, A); }; break;
-    case PK_X_C2R: f = [&](EmuCtx& c) { body_x_c2r<Sel>(c, B, A); }; break;
+    case PK_X_R2C: f = [&](EmuCtx& c) { body_x_r2c<SelX>(c, B, A); }; break;
+    case PK_X_C2R: f = [&](EmuCtx& c) { body_x_c2r<SelX>(c, B, A); }; break;
     case PK_X_GRAD:
       if (dim == 2)
-        f = [&](EmuCtx& c) { body_x_grad<Sel, 2>(c, B, A); };
+        f = [&](EmuCtx& c) { body_x_grad<SelX, 2>(c, B, A); };
       else
-        f = [&](EmuCtx& c) { body_x_grad<Sel, 3>(c, B, A); };
+        f = [&](EmuCtx& c) { body_x_grad<SelX, 3>(c, B, A); };
       break;
     case PK_X_DIV:
       if (dim == 2)
-        f = [&](EmuCtx& c) { body_x_div<Sel, 2>(c, B, A); };
+        f = [&](EmuCtx& c) { body_x_div<SelX, 2>(c, B, A); };
       else
-        f = [&](EmuCtx& c) { body_x_div<Sel, 3>(c, B, A); };
+        f = [&](EmuCtx& c) { body_x_div<SelX, 3>(c, B, A); };
       break;
-    case PK_X_EOS: f = [&](EmuCtx& c) { body_x_eos<Sel>(c, B, A); }; break;
+    case PK_X_EOS: f = [&](EmuCtx& c) { body_x_eos<SelX>(c, B, A); }; break;
   }
   emu_launch(nblocks, threads, lds_bytes, f);
 }
@@ -328,9 +328,9 @@ static void emu_pass(PassKind kind, int dim, int nblocks, int threads, size_t ld
 void launch_pass(PassKind kind, int fixed, int dim, int nblocks, int threads, size_t lds_bytes,
                  const Batch& B, const PassArgs& A, void*) {
   switch (fixed) {
-    case 0: emu_pass<PlanRt>(kind, dim, nblocks, threads, lds_bytes, B, A); break;
+    case 0: emu_pass<PlanRt, PlanRt>(kind, dim, nblocks, threads, lds_bytes, B, A); break;
 #define KW_EMU_SIZE(SIZE) \
-    case SIZE: emu_pass<PlanCt<SIZE>>(kind, dim, nblocks, threads, lds_bytes, B, A); break;
+    case SIZE: emu_pass<PlanCt<SIZE>, PlanCtX<SIZE>>(kind, dim, nblocks, threads, lds_bytes, B, A); break;
     KW_EMU_FIXED_SIZES(KW_EMU_SIZE)
 #undef KW_EMU_SIZE
     default: throw DeviceError(-3, "no kernel variant for this length");

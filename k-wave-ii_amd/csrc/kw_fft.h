@@ -397,6 +397,7 @@ KW_BODY void last_slots_ct(int g, bool, F& f) {
 
 struct PlanRt {
   static constexpr int fixed = 0;
+  static constexpr int G = 0;  // threads per pencil: chosen at run time
   template <int NP, bool INV, class Ctx, class Src, class Dst>
   KW_BODY static void fft(Ctx& c, const FftPlan& pl, int g, int G, bool active, Src& src,
                           Dst& dst, const LdsPencils& lds) {
@@ -408,12 +409,27 @@ struct PlanRt {
   }
 };
 
-template <int N_>
+// threads per pencil of a fixed-length plan: ceil(N / KW_E)
+KW_HOSTDEV constexpr int ct_threads(int N) { return (N + KW_E - 1) / KW_E; }
+
+// threads per pencil of the fixed-length x passes: ceil(N / KW_E) rounded up
+// to a power of two when that is at most 64, so that every pencil lies in
+// one wavefront and the stages synchronise with wavefront barriers instead
+// of block barriers (the extra lanes idle in stages with fewer butterflies
+// than lanes)
+KW_HOSTDEV constexpr int x_threads(int N) {
+  int g = ct_threads(N), p = 1;
+  if (g > 64) return g;
+  while (p < g) p <<= 1;
+  return p;
+}
+
+template <int N_, int G_ = ct_threads(N_)>
 struct PlanCt {
   static constexpr int fixed = N_;
-  // threads per pencil; when N is not a multiple of KW_E some threads get
+  // threads per pencil; when N is not a multiple of G some threads get
   // fewer butterflies (the stages guard them)
-  static constexpr int G = (N_ + KW_E - 1) / KW_E;
+  static constexpr int G = G_;
   static constexpr int last_R = ct_radix(N_, ct_nst(N_) - 1);
   static_assert(((N_ / last_R + G - 1) / G) * last_R <= kMaxSlots, "slot bound");
   template <int NP, bool INV, class Ctx, class Src, class Dst>
@@ -426,5 +442,9 @@ struct PlanCt {
     last_slots_ct<N_, G>(g, active, f);
   }
 };
+
+// the plan of the x passes for a fixed length
+template <int N>
+using PlanCtX = PlanCt<N, x_threads(N)>;
 
 }  // namespace kw

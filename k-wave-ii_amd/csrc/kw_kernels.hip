@@ -58,8 +58,10 @@ void launch_k_ew(int nb, int T, size_t lds, void* s, const Batch& B, const EwArg
 
 #if KW_SIZE == 0
 using Sel = PlanRt;
+using SelX = PlanRt;
 #else
 using Sel = PlanCt<KW_SIZE>;
+using SelX = PlanCtX<KW_SIZE>;
 #endif
 
 // x passes run at most 256 threads per block; strided and last-axis passes
@@ -86,43 +88,43 @@ __global__ __launch_bounds__(512) void kern(KW_KPARAMS, PassArgs A) {
 __global__ __launch_bounds__(256) void kern(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
   KW_KBATCH;
-  body_x_r2c<Sel>(c, B, A);
+  body_x_r2c<SelX>(c, B, A);
 }
 #elif KW_KIND == 4
 __global__ __launch_bounds__(256) void kern(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
   KW_KBATCH;
-  body_x_c2r<Sel>(c, B, A);
+  body_x_c2r<SelX>(c, B, A);
 }
 #elif KW_KIND == 5
 __global__ __launch_bounds__(256) void kern2(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
   KW_KBATCH;
-  body_x_grad<Sel, 2>(c, B, A);
+  body_x_grad<SelX, 2>(c, B, A);
 }
 __global__ __launch_bounds__(256) void kern(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
   KW_KBATCH;
-  body_x_grad<Sel, 3>(c, B, A);
+  body_x_grad<SelX, 3>(c, B, A);
 }
 #define KW_HAS_DIM2 1
 #elif KW_KIND == 6
 __global__ __launch_bounds__(256) void kern2(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
   KW_KBATCH;
-  body_x_div<Sel, 2>(c, B, A);
+  body_x_div<SelX, 2>(c, B, A);
 }
 __global__ __launch_bounds__(256) void kern(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
   KW_KBATCH;
-  body_x_div<Sel, 3>(c, B, A);
+  body_x_div<SelX, 3>(c, B, A);
 }
 #define KW_HAS_DIM2 1
 #elif KW_KIND == 7
 __global__ __launch_bounds__(256) void kern(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
   KW_KBATCH;
-  body_x_eos<Sel>(c, B, A);
+  body_x_eos<SelX>(c, B, A);
 }
 #else
 #error "KW_KIND must be 0..8"

@@ -68,7 +68,7 @@ struct XTile {
 template <class XS, class Ctx>
 KW_BODY XTile x_tile(Ctx& c, const ProbDev& P, int tile) {
   XTile T;
-  constexpr int Gc = XS::fixed ? (XS::fixed + KW_E - 1) / KW_E : 0;
+  constexpr int Gc = XS::fixed ? XS::G : 0;
   T.G = XS::fixed ? Gc : P.Gx;
   T.wave = XS::fixed ? pencils_in_wave(Gc) : pencils_in_wave(P.Gx);
   const int tid = c.tid();

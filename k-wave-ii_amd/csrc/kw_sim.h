@@ -333,9 +333,10 @@ struct Sim {
     const size_t maxl = dev_max_lds();
     const int capP = max_pencils_env();
     auto Gof = [](int N) { return (N + KW_E - 1) / KW_E; };
-    // x pass: <= 256 threads, LDS P * kNPX padded pencils
+    // x pass: <= 256 threads, LDS P * kNPX padded pencils; the fixed-length
+    // x kernels run x_threads(Nx) threads per pencil
     {
-      const int Gx = Gof(G.Nx);
+      const int Gx = fixed_len(PK_X_GRAD) ? x_threads(G.Nx) : Gof(G.Nx);
       const size_t pencil = sizeof(cf) * (size_t)kNPX * x_lds_stride(G.Nx);
       int P = std::max(1, 256 / Gx);
       P = std::min<int64_t>(P, (int64_t)(maxl / pencil));
