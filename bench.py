@@ -152,7 +152,12 @@ def served_bench(n_req, rate, steps_lo, steps_hi, seed=7):
     disc source, whole-grid p_max/p_rms): latency from arrival to result for
     the engine, against a plain server that runs the same requests one at a
     time in arrival order (its latencies follow from the measured service
-    time of each request and the same arrival times)."""
+    time of each request and the same arrival times).  Each request is
+    submitted by a handler thread of its own at its arrival time, as a
+    server's connection handlers would (the engine is thread-safe), so the
+    host-side pre-processing of one request does not delay the next."""
+    from concurrent.futures import ThreadPoolExecutor
+
     import numpy as np
 
     import kwhip
@@ -165,16 +170,22 @@ def served_bench(n_req, rate, steps_lo, steps_hi, seed=7):
     cell_steps = float(sum(kg.total_grid_points * kg.Nt for kg, *_ in reqs))
     fn = {2: kwhip.kspaceFirstOrder2D, 3: kwhip.kspaceFirstOrder3D}
 
-    with kwhip.Engine() as eng:
+    with kwhip.Engine() as eng, ThreadPoolExecutor(max_workers=8) as handlers:
         t0 = time.perf_counter()
-        tickets = []
-        for i, (kg, med, src, sens, opts) in enumerate(reqs):
+
+        def handle(i):
+            kg, med, src, sens, opts = reqs[i]
+            t = eng.submit(kg, med, src, sens, kg.dim, **opts)
+            # wall clock minus engine clock at submission
+            return time.perf_counter() - t0 - eng.poll(t)["submit_s"], t
+
+        futures = []
+        for i in range(n_req):
             delay = t0 + arrivals[i] - time.perf_counter()
             if delay > 0:
                 time.sleep(delay)
-            t = eng.submit(kg, med, src, sens, kg.dim, **opts)
-            # wall clock minus engine clock at submission
-            tickets.append((time.perf_counter() - t0 - eng.poll(t)["submit_s"], t))
+            futures.append(handlers.submit(handle, i))
+        tickets = [f.result() for f in futures]
         done_at = np.zeros(n_req)
         for i, (offset, t) in enumerate(tickets):
             _, info = eng.wait(t, with_info=True)

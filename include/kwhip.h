@@ -201,9 +201,11 @@ const char* kw_kind_name(int k);
  * kspaceFirstOrder3DC.m:441-546 for callers that run many simulations
  * (batches of forward runs and time-reversal reconstructions, request
  * streams).  kw_submit copies the problem (every array) into the engine
- * before it returns, so the caller's buffers may be freed at once.  An
- * admission thread moves queued simulations to the device while their
- * estimated device footprint fits the engine's budget; a stepping thread
+ * before it returns, so the caller's buffers may be freed at once.
+ * Admission threads move queued simulations to the device, in submission
+ * order, while their estimated device footprint fits the engine's budget
+ * (several are built at once: the host-side operator construction of a
+ * large grid takes a large part of a second); a stepping thread
  * advances all admitted simulations together, one launch per pass for all
  * simulations with the same dimension and equation of state (and the same
  * grid size where fixed-length kernels exist for it), whatever their media,
@@ -218,6 +220,8 @@ typedef struct kw_engine_config {
   int64_t device_bytes;    /* device memory the engine may hold for simulations;
                               0: 90% of the free memory at creation */
   int32_t steps_per_round; /* steps between admission/retirement decisions; 0: 32 */
+  int32_t admit_threads;   /* simulations built on the host and device at the
+                              same time (operators, uploads, first step); 0: 3 */
 } kw_engine_config;
 
 #define KW_TICKET_QUEUED 0

@@ -452,7 +452,8 @@ struct Engine {
   int64_t n_submitted = 0, n_completed = 0, n_failed = 0, n_cancelled = 0, n_running = 0;
   double cell_steps = 0, busy_s = 0;
   int64_t launches = 0;
-  std::thread admit_thr, step_thr;
+  std::vector<std::thread> admit_thr;
+  std::thread step_thr;
 
   double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
 
@@ -509,6 +510,9 @@ struct Engine {
         msg = e.what();
       }
       lk.lock();
+      // the estimate is replaced by what the simulation holds: another
+      // admission worker may now fit
+      cv_admit.notify_all();
       if (code != KW_OK) {
         sim.reset();
         held -= t->est_bytes;
@@ -773,8 +777,12 @@ int kw_engine_create(const kw_engine_config* cfg, kw_engine** engine, char* err,
     // as many compute streams as the device has hardware queues for this
     // process by default (4), so that groups overlap without oversubscribing
     for (int i = 0; i < 4; ++i) E.streams.push_back(kw::dev_stream_create());
+    // admission workers share one stream: their device work (uploads, the
+    // first step) is short next to the host-side operator construction they
+    // overlap
     E.admit_stream = kw::dev_stream_create();
-    E.admit_thr = std::thread([&E]() { E.admit_loop(); });
+    const int na = (cfg && cfg->admit_threads > 0) ? std::min(cfg->admit_threads, 16) : 3;
+    for (int i = 0; i < na; ++i) E.admit_thr.emplace_back([&E]() { E.admit_loop(); });
     E.step_thr = std::thread([&E]() { E.step_loop(); });
     *engine = h.release();
   });
@@ -899,7 +907,8 @@ int kw_engine_destroy(kw_engine* engine) {
   }
   E.cv_admit.notify_all();
   E.cv_step.notify_all();
-  if (E.admit_thr.joinable()) E.admit_thr.join();
+  for (std::thread& th : E.admit_thr)
+    if (th.joinable()) th.join();
   if (E.step_thr.joinable()) E.step_thr.join();
   for (void* st : E.streams) kw::dev_stream_destroy(st);
   kw::dev_stream_destroy(E.admit_stream);

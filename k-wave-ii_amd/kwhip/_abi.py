@@ -103,7 +103,8 @@ class KernelStats(C.Structure):
 
 
 class EngineConfig(C.Structure):
-    _fields_ = [("device_bytes", C.c_int64), ("steps_per_round", C.c_int32)]
+    _fields_ = [("device_bytes", C.c_int64), ("steps_per_round", C.c_int32),
+                ("admit_threads", C.c_int32)]
 
 
 class TicketInfo(C.Structure):

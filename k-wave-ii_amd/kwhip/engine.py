@@ -41,10 +41,11 @@ class Ticket:
 
 
 class Engine:
-    def __init__(self, device_bytes: int = 0, steps_per_round: int = 0, backend: str = "hip"):
+    def __init__(self, device_bytes: int = 0, steps_per_round: int = 0, backend: str = "hip",
+                 admit_threads: int = 0):
         self.backend = backend
         self.lib = _abi.load(backend)
-        cfg = _abi.EngineConfig(int(device_bytes), int(steps_per_round))
+        cfg = _abi.EngineConfig(int(device_bytes), int(steps_per_round), int(admit_threads))
         self.h = C.c_void_p()
         _abi.call(self.lib.kw_engine_create, C.byref(cfg), C.byref(self.h))
 

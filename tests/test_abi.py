@@ -61,6 +61,7 @@ int main(void) {
          sizeof(kw_sizes), sizeof(kw_kernel_stats));
   printf("%zu %zu %zu\n", offsetof(kw_problem, p_source), offsetof(kw_problem, sensor_num_points),
          offsetof(kw_problem, time_reversal_boundary_data));
+  printf("%zu %zu %zu\n", sizeof(kw_engine_config), sizeof(kw_ticket_info), sizeof(kw_engine_stats));
   return 0;
 }
 """
@@ -74,7 +75,8 @@ int main(void) {
     want = [C.sizeof(_abi.Problem), C.sizeof(_abi.TimeSource), C.sizeof(_abi.Outputs),
             C.sizeof(_abi.Sizes), C.sizeof(_abi.KernelStats),
             _abi.Problem.p_source.offset, _abi.Problem.sensor_num_points.offset,
-            _abi.Problem.time_reversal_boundary_data.offset]
+            _abi.Problem.time_reversal_boundary_data.offset,
+            C.sizeof(_abi.EngineConfig), C.sizeof(_abi.TicketInfo), C.sizeof(_abi.EngineStats)]
     assert got == want
 
 
