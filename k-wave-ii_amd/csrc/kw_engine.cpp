@@ -1,8 +1,8 @@
 // Engine: many simulations on one GPU (kw_engine_* / kw_submit / kw_wait in
 // include/kwhip.h).
 //
-// An admission thread takes submitted problems in order and moves each to
-// the device (operators, uploads, first step) on its own stream while the
+// Admission threads take submitted problems in order and move them to the
+// device (operators, uploads, first step), several at a time, while the
 // estimated device footprint of everything admitted fits the engine's
 // budget.  A stepping thread keeps the admitted simulations in groups and
 // advances each group with one launch per pass for all of its members:
@@ -13,10 +13,13 @@
 // same pass sequence (dimension, equation of state) and, where fixed-length
 // kernels exist for the grid, the same size; simulations of other sizes
 // share a group per pass sequence that runs the runtime-length kernels.
-// Groups run on a pool of streams, each with one round of steps in flight,
-// so small groups overlap on the device.  Finished simulations are read
+// Each group has at most one round of steps in flight, on whichever stream
+// of a pool is free (least remaining work first), so groups overlap on the
+// device and short simulations do not wait behind the rounds of long ones.
+// Finished simulations are read
 // back to host staging buffers and their device memory is released, which
 // admits the next ones.
+#include <algorithm>
 #include <array>
 #include <chrono>
 #include <condition_variable>
@@ -45,10 +48,14 @@ struct ProblemCopy {
 
   explicit ProblemCopy(const kw_problem& in) : pb(in) {
     const int64_t n = in.Nx * in.Ny * (in.dim == 3 ? in.Nz : 1);
+    // grids are copied on several threads (a 256^3 problem carries about
+    // 1 GB of them)
     auto grid = [&](const double* a, std::vector<double>& v) -> const double* {
       if (!a) return nullptr;
-      v.assign(a, a + n);
-      return v.data();
+      v.resize((size_t)n);
+      double* d = v.data();
+      parallel_for(n, [&](int64_t i) { d[i] = a[i]; });
+      return d;
     };
     pb.c0 = grid(in.c0, c0);
     pb.rho0 = grid(in.rho0, rho0);
@@ -233,6 +240,8 @@ struct Group {
   std::vector<Ticket*> members, joining;
   std::vector<Sim*> sims;
   bool dirty = true;
+  bool changed = true;  // membership differs from the one the tables describe
+  int slot = -1;        // stream of the pool its round in flight uses
   ProbDev* probs = nullptr;
   int32_t* tiles[NGEO] = {};
   LaunchGeom geo[NGEO];
@@ -580,13 +589,15 @@ struct Engine {
     return {dim, absorbing, 0, 0, 0};
   }
 
-  // Each group runs on one stream of the pool and has at most one round
-  // (steps_per_round steps of all members) in flight; groups on different
-  // streams overlap on the device.  Members join and leave a group only
-  // between its rounds.
+  // A group has at most one round (steps_per_round steps of all members) in
+  // flight, on whichever stream of the pool is free when the round is
+  // issued; rounds of different groups overlap on the device.  Members join
+  // and leave a group only between its rounds.  Free streams go to the
+  // groups with the least remaining work first, so short simulations are not
+  // queued behind the rounds of long ones.
   void step_loop() {
     std::map<std::array<int, 5>, std::unique_ptr<Group>> groups;
-    size_t next_stream = 0;
+    std::vector<char> stream_busy(streams.size(), 0);
     auto last = Clock::now();
     std::unique_lock<std::mutex> lk(mu);
     for (;;) {
@@ -608,7 +619,6 @@ struct Engine {
           g = std::make_unique<Group>();
           g->dim = key[0];
           g->absorbing = key[1] != 0;
-          g->stream = streams[next_stream++ % streams.size()];
           g->ev = dev_event_create();
         }
         g->joining.push_back(t);
@@ -616,51 +626,72 @@ struct Engine {
       bool issued = false, busy = false;
       double cells = 0;
       int64_t nl = 0;
+      // rounds that have ended free their streams; between rounds, members
+      // that finished or were cancelled leave and new ones join
+      std::vector<std::pair<double, Group*>> idle;
       for (auto& kv : groups) {
         Group& g = *kv.second;
+        if (g.in_flight) {
+          if (!dev_event_done(g.ev)) {
+            busy = true;
+            continue;
+          }
+          g.in_flight = false;
+          stream_busy[g.slot] = 0;
+        }
+        std::vector<Ticket*> keep;
+        for (Ticket* t : g.members) {
+          lk.lock();
+          const bool cancelled = t->cancel;
+          lk.unlock();
+          if (cancelled || t->sim->steps_done >= t->sim->Nt_eff)
+            retire(t, lk, cancelled);
+          else
+            keep.push_back(t);
+        }
+        g.changed = g.changed || keep.size() != g.members.size() || !g.joining.empty();
+        for (Ticket* t : g.joining) keep.push_back(t);
+        g.joining.clear();
+        g.members.swap(keep);
+        if (g.members.empty()) continue;
+        double work = 0;
+        for (Ticket* t : g.members)
+          work += (double)t->sim->G.ncell * (double)(t->sim->Nt_eff - t->sim->steps_done);
+        idle.push_back({work, &g});
+      }
+      std::sort(idle.begin(), idle.end(),
+                [](const std::pair<double, Group*>& a, const std::pair<double, Group*>& b) {
+                  return a.first < b.first;
+                });
+      for (auto& wg : idle) {
+        Group& g = *wg.second;
+        int slot = -1;
+        for (size_t k = 0; k < streams.size() && slot < 0; ++k)
+          if (!stream_busy[k]) slot = (int)k;
+        if (slot < 0) {
+          busy = true;  // every stream has a round in flight
+          break;
+        }
         int code = KW_OK;
         std::string err;
         try {
-          if (g.in_flight) {
-            if (!dev_event_done(g.ev)) {
-              busy = true;
-              continue;
-            }
-            g.in_flight = false;
-          }
-          // between rounds: members that finished or were cancelled leave,
-          // new ones join
-          std::vector<Ticket*> keep;
-          for (Ticket* t : g.members) {
-            lk.lock();
-            const bool cancelled = t->cancel;
-            lk.unlock();
-            if (cancelled || t->sim->steps_done >= t->sim->Nt_eff)
-              retire(t, lk, cancelled);
-            else
-              keep.push_back(t);
-          }
-          const bool changed = keep.size() != g.members.size() || !g.joining.empty();
-          for (Ticket* t : g.joining) {
-            t->sim->stream = g.stream;
-            keep.push_back(t);
-          }
-          g.joining.clear();
-          g.members.swap(keep);
-          if (changed) {
+          g.slot = slot;
+          g.stream = streams[slot];
+          for (Ticket* t : g.members) t->sim->stream = g.stream;
+          if (g.changed) {
             g.sims.clear();
             for (Ticket* t : g.members) g.sims.push_back(t->sim.get());
-            g.free_retired();  // the group's stream is idle
+            g.free_retired();  // the group's last round has ended
             g.rebuild();
+            g.changed = false;
           }
-          if (g.members.empty()) continue;
           int64_t k = steps_per_round;
           for (Sim* s : g.sims) k = std::min(k, s->Nt_eff - s->steps_done);
           const int64_t l0 = g.launches;
           if (use_graphs && k == steps_per_round && g.graphable()) {
-            // full rounds of a membership are captured once and replayed:
-            // every kernel takes its step from the device, so the launches
-            // of one round are the same for every round
+            // full rounds of a membership are captured once and replayed
+            // (on any stream): every kernel takes its step from the device,
+            // so the launches of one round are the same for every round
             if (!g.graph && dev_capture_begin(g.stream)) {
               try {
                 for (int64_t i = 0; i < k; ++i) group_step(g);
@@ -684,6 +715,7 @@ struct Engine {
           }
           dev_event_record(g.ev, g.stream);
           g.in_flight = true;
+          stream_busy[slot] = 1;
           issued = busy = true;
           nl += g.launches - l0;
           for (Sim* s : g.sims) cells += (double)s->G.ncell * (double)k;
@@ -698,11 +730,11 @@ struct Engine {
           } catch (...) {
           }
           g.in_flight = false;
+          stream_busy[slot] = 0;
           std::vector<Ticket*> all = g.members;
-          all.insert(all.end(), g.joining.begin(), g.joining.end());
           g.members.clear();
-          g.joining.clear();
           g.sims.clear();
+          g.changed = true;
           for (Ticket* t : all) {
             const int64_t b = t->bytes;
             t->sim.reset();
@@ -713,6 +745,14 @@ struct Engine {
             lk.unlock();
           }
         }
+      }
+      // groups left without members and without a round in flight go
+      for (auto it = groups.begin(); it != groups.end();) {
+        Group& g = *it->second;
+        if (g.members.empty() && g.joining.empty() && !g.in_flight)
+          it = groups.erase(it);
+        else
+          ++it;
       }
       const auto nowt = Clock::now();
       lk.lock();

@@ -78,11 +78,13 @@ inline std::vector<double> pml_profile(int N, double d, double dt, double c, int
 
 inline double sinc(double x) { return x == 0.0 ? 1.0 : std::sin(x) / x; }
 
+// f(i) for i < n on up to 16 host threads; `work` is the cost of one item in
+// element operations (small loops stay on the calling thread)
 template <class F>
-void parallel_for(int64_t n, F f) {
+void parallel_for(int64_t n, F f, int64_t work = 1) {
   int nt = (int)std::thread::hardware_concurrency();
   nt = std::max(1, std::min(nt, 16));
-  if (n < 100000) nt = 1;
+  if (n * work < 100000) nt = 1;
   std::vector<std::thread> th;
   const int64_t chunk = (n + nt - 1) / nt;
   for (int w = 0; w < nt; ++w) {
@@ -418,7 +420,7 @@ struct Sim {
     const int64_t tH = (nH + Pn - 1) / Pn;
     const int64_t nt = ll.nb;
     std::vector<float> t((size_t)(nt * N * Pn), 0.f);
-    parallel_for(nt * Pn, [&](int64_t e) {
+    auto pencil = [&](int64_t e) {
       const int64_t tile = e / Pn;
       const int p = (int)(e % Pn);
       int ix, iy;
@@ -437,7 +439,8 @@ struct Sim {
         const double v = g.dim == 3 ? f(ix, iy, n) : f(ix, n, 0);
         t[(size_t)((tile * N + n) * Pn + p)] = (float)v;
       }
-    });
+    };
+    parallel_for(nt * Pn, pencil, N);
     return upload(t);
   }
 };

@@ -208,9 +208,41 @@ def as_f64(a):
     return None if a is None else np.ascontiguousarray(a, dtype=np.float64)
 
 
+_FLAT_POOL = None
+
+
+def host_pool():
+    """A few host threads for slab-parallel NumPy work on large grids (NumPy
+    releases the GIL inside its copy and arithmetic loops)."""
+    global _FLAT_POOL
+    if _FLAT_POOL is None:
+        from concurrent.futures import ThreadPoolExecutor
+        _FLAT_POOL = ThreadPoolExecutor(max_workers=8)
+    return _FLAT_POOL
+
+
+def by_slabs(fn, n, k=8):
+    """fn(lo, hi) over k slabs of range(n), on the host pool."""
+    k = max(1, min(k, n))
+    bounds = [n * i // k for i in range(k + 1)]
+    list(host_pool().map(lambda i: fn(bounds[i], bounds[i + 1]), range(k)))
+
+
 def fortran_flat(a):
-    """Column-major (MATLAB) flattening of a grid array as float64."""
-    return np.ascontiguousarray(np.asarray(a, dtype=np.float64).ravel(order="F"))
+    """Column-major (MATLAB) flattening of a grid array as float64.  Large
+    row-major grids are transposed by slabs of the last axis on a few host
+    threads (NumPy releases the GIL while it copies), several times faster
+    than one strided copy."""
+    a = np.asarray(a, dtype=np.float64)
+    if a.ndim < 2 or a.size < (1 << 20) or a.flags.f_contiguous:
+        return np.ascontiguousarray(a.ravel(order="F"))
+    out = np.empty(a.shape, dtype=np.float64, order="F")
+
+    def part(lo, hi):
+        out[..., lo:hi] = a[..., lo:hi]
+
+    by_slabs(part, a.shape[-1])
+    return out.ravel(order="F")  # a view: out is column-major
 
 
 def backend_name(lib) -> str:

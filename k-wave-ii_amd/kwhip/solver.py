@@ -160,7 +160,15 @@ def cart2grid(kgrid: kWaveGrid, cart_data):
 
 
 def _find(mask) -> np.ndarray:
-    return np.flatnonzero(np.asarray(mask).ravel(order="F") != 0)
+    """Column-major linear indices of the nonzero cells of a grid (MATLAB's
+    find).  Sparse masks are searched in memory order and the indices
+    sorted, which avoids a transposed copy of the whole grid."""
+    m = np.asarray(mask)
+    if m.ndim > 1 and np.count_nonzero(m) * 8 < m.size:
+        lin = np.ravel_multi_index(np.nonzero(m), m.shape, order="F")
+        lin.sort()
+        return lin
+    return np.flatnonzero(np.asarray(m != 0).ravel(order="F"))
 
 
 # ---------------------------------------------------------------------------
@@ -265,6 +273,8 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
     record = ["p"]
     tr_data = None
     sens_idx = None
+    n_sens = 0
+    sens_full = False  # the mask is the whole grid (no index list is built)
     if info.use_sensor:
         info.time_rev = _has(sensor, "time_reversal_boundary_data")
         if _has(sensor, "record"):
@@ -275,7 +285,11 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
                 raise NotImplementedError(f"sensor.record entries not supported by kwhip: {bad}")
         mask = np.asarray(_get(sensor, "mask"))
         if mask.shape == shape:
-            sens_idx = _find(mask)
+            if np.all(mask):
+                sens_full, n_sens = True, ncell
+            else:
+                sens_idx = _find(mask)
+                n_sens = len(sens_idx)
             if info.time_rev:
                 tr_data = np.asarray(_get(sensor, "time_reversal_boundary_data"), dtype=np.float64)
         elif mask.ndim == 2 and mask.shape[0] == dim:
@@ -284,6 +298,7 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
                     "CartInterp 'linear' (Delaunay interpolation) is not supported; use 'nearest'")
             gmask, order_index, reorder_index = cart2grid(kgrid, mask)
             sens_idx = _find(gmask)
+            n_sens = len(sens_idx)
             info.reorder_index = reorder_index
             if info.time_rev:
                 d = np.asarray(_get(sensor, "time_reversal_boundary_data"), dtype=np.float64)
@@ -292,7 +307,7 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
             raise ValueError("sensor.mask must be a binary grid or a dim x N Cartesian array")
         if info.time_rev:
             record = []
-            if tr_data.shape != (len(sens_idx), Nt):
+            if tr_data.shape != (n_sens, Nt):
                 raise ValueError("sensor.time_reversal_boundary_data must be [num_sensor_points x Nt]")
     info.record = record
     rec_flags = 0
@@ -324,15 +339,19 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
         mode = _get(source, mode_name, "additive")
         if mode not in _abi.SRC_MODE:
             raise ValueError(f"source.{mode_name} must be set to 'additive', 'additive-no-correction', or 'dirichlet'.")
-        uniq = np.unique(m)
-        labelled = not (uniq.size <= 2 and uniq.sum() == 1)
+        # distinct values of the mask (inputChecking.m:850-887), from its
+        # nonzero entries plus 0 when any cell is outside the source
+        vals = m[np.unravel_index(pos, shape, order="F")]
+        uniq_nz = np.unique(vals)
+        n_uniq = uniq_nz.size + (1 if len(pos) < m.size else 0)
+        labelled = not (n_uniq <= 2 and float(uniq_nz.sum()) == 1)
         sig_index = None
         if labelled:
-            sig_index = np.asarray(m).ravel(order="F")[pos].astype(np.uint64)
+            sig_index = np.asarray(vals).astype(np.uint64)
         sigs = {nm: np.atleast_2d(np.asarray(_get(source, nm), dtype=np.float64)).copy()
                 for nm in present}
         for nm, s in sigs.items():
-            if labelled and s.shape[0] != len(uniq) - 1:
+            if labelled and s.shape[0] != n_uniq - 1:
                 raise ValueError("The number of time series must match the number of labelled source elements.")
             if not labelled and s.shape[0] > 1 and s.shape[0] != len(pos):
                 raise ValueError("The number of time series must match the number of source elements.")
@@ -352,7 +371,8 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
 
     # -- source scaling (kspaceFirstOrder_scaleSourceTerms.m:43-391) ----------
     scale = bool(opts.get("ScaleSourceTerms", True))
-    c_flat = c.ravel(order="F") if c.size > 1 else None
+    # sound speed at the source points (column-major linear indices)
+    c_at = (lambda pos: c[np.unravel_index(pos, c.shape, order="F")]) if c.size > 1 else None
     if src_p is not None and scale:
         s = src_p.sigs["p"]
         if src_p.mode == "dirichlet":
@@ -360,13 +380,13 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
                 s = s / (dim * float(c) ** 2)
             else:
                 for i in range(s.shape[0]):  # row i scaled by source point i (reference behaviour)
-                    s[i] = s[i] / (dim * c_flat[src_p.pos[i]] ** 2)
+                    s[i] = s[i] / (dim * c_at(src_p.pos[i]) ** 2)
         else:
             if c.size == 1:
                 s = s * (2 * dt / (dim * float(c) * kgrid.dx))
             else:
                 for i in range(s.shape[0]):
-                    s[i] = s[i] * (2 * dt / (dim * c_flat[src_p.pos[i]] * kgrid.dx))
+                    s[i] = s[i] * (2 * dt / (dim * c_at(src_p.pos[i]) * kgrid.dx))
         src_p.sigs["p"] = s
     if src_u is not None and scale and src_u.mode != "dirichlet":
         for ax, nm in enumerate(u_names):
@@ -378,18 +398,41 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
                 s = s * (2 * float(c) * dt / dd)
             else:
                 for i in range(s.shape[0]):
-                    s[i] = s[i] * (2 * c_flat[src_u.pos[i]] * dt / dd)
+                    s[i] = s[i] * (2 * c_at(src_u.pos[i]) * dt / dd)
             src_u.sigs[nm] = s
 
     # -- staggered density (kspaceFirstOrder3D.m:585-607) ----------------------
     use_sg = bool(opts.get("UseSG", True))
     rho_sg = [None] * 3
     if rho.size > 1 and use_sg:
+        # computed on the column-major copy, so every array is contiguous and
+        # its flattening free
+        rho_f = _abi.fortran_flat(rho).reshape(shape, order="F")
+        last = dim - 1
         for ax in range(dim):
-            r = np.moveaxis(rho, ax, 0)
-            sg = r.copy()
-            sg[:-1] = r[:-1] + 0.5 * (r[1:] - r[:-1])
-            rho_sg[ax] = _abi.fortran_flat(np.moveaxis(sg, 0, ax))
+            sg = np.empty(shape, dtype=np.float64, order="F")
+
+            def part(z0, z1, ax=ax, sg=sg):
+                # slab z0:z1 of the last axis; along the last axis the
+                # neighbour plane may lie in the next slab (read only)
+                a = rho_f[..., z0:z1]
+                o = sg[..., z0:z1]
+                o[...] = a
+                if ax == last:
+                    z1i = min(z1, shape[last] - 1)
+                    if z1i > z0:
+                        lo = rho_f[..., z0:z1i]
+                        o[..., :z1i - z0] = lo + 0.5 * (rho_f[..., z0 + 1:z1i + 1] - lo)
+                else:
+                    sl_lo = [slice(None)] * dim
+                    sl_hi = [slice(None)] * dim
+                    sl_lo[ax] = slice(0, -1)
+                    sl_hi[ax] = slice(1, None)
+                    sl_lo, sl_hi = tuple(sl_lo), tuple(sl_hi)
+                    o[sl_lo] = a[sl_lo] + 0.5 * (a[sl_hi] - a[sl_lo])
+
+            _abi.by_slabs(part, shape[last])
+            rho_sg[ax] = sg.ravel(order="F")
 
     # -- fill the kw_problem ---------------------------------------------------
     pr.dim = dim
@@ -465,15 +508,15 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
                 fill_source(pr.u_source[ax], src_u, nm)
 
     if info.use_sensor:
-        full = len(sens_idx) == ncell and np.array_equal(sens_idx, np.arange(ncell))
-        pr.sensor_num_points = len(sens_idx)
+        full = sens_full or (n_sens == ncell and np.array_equal(sens_idx, np.arange(ncell)))
+        pr.sensor_num_points = n_sens
         if not full:
             pr.sensor_mask_index = _abi.uptr(pp.hold(np.ascontiguousarray(sens_idx.astype(np.uint64) + 1)))
         pr.record_flags = rec_flags
         pr.record_start_index = record_start
         if info.time_rev:
             pr.time_reversal_boundary_data = _abi.dptr(pp.hold(np.ascontiguousarray(tr_data.ravel(order="F"))))
-    info.n_sensor = len(sens_idx) if info.use_sensor else 0
+    info.n_sensor = n_sens if info.use_sensor else 0
     info.n_rec = (Nt - record_start + 1) if (info.use_sensor and not info.time_rev) else 0
     info.data_recast = bool(opts.get("DataRecast", False))
     info.backend = backend
