@@ -146,30 +146,31 @@ def batched_bench(n_problems, seed0=0):
                       f"{cell_steps / 1e9:.1f} G cell-steps"}
 
 
-def served_bench(n_req, rate, steps_lo, steps_hi, seed=7):
-    """A Poisson request stream of forward simulations (2D 256^2 - 1024^2,
-    3D 96^3 - 256^3, heterogeneous absorbing nonlinear medium, tone-burst
-    disc source, whole-grid p_max/p_rms): latency from arrival to result for
-    the engine, against a plain server that runs the same requests one at a
-    time in arrival order (its latencies follow from the measured service
-    time of each request and the same arrival times).  Each request is
-    submitted by a handler thread of its own at its arrival time, as a
-    server's connection handlers would (the engine is thread-safe), so the
-    host-side pre-processing of one request does not delay the next."""
+def _served_requests(n_req, steps_lo, steps_hi, seed):
+    import numpy as np
+
+    from kwhip import workloads
+
+    rng = np.random.default_rng(seed)
+    gaps = rng.exponential(1.0, size=n_req)  # arrival gaps at 1 request/s
+    steps = rng.integers(steps_lo, steps_hi + 1, size=n_req)
+    reqs = [workloads.served_problem(seed * 1000 + i, int(steps[i])) for i in range(n_req)]
+    return reqs, gaps
+
+
+def _served_engine(reqs, arrivals):
+    """Completion times (seconds after the first arrival's origin) of the
+    requests served by the engine.  Each request is submitted by a handler
+    thread of its own at its arrival time, as a server's connection
+    handlers would (the engine is thread-safe), so the host-side
+    pre-processing of one request does not delay the next."""
     from concurrent.futures import ThreadPoolExecutor
 
     import numpy as np
 
     import kwhip
-    from kwhip import workloads
 
-    rng = np.random.default_rng(seed)
-    arrivals = np.cumsum(rng.exponential(1.0 / rate, size=n_req))
-    steps = rng.integers(steps_lo, steps_hi + 1, size=n_req)
-    reqs = [workloads.served_problem(seed * 1000 + i, int(steps[i])) for i in range(n_req)]
-    cell_steps = float(sum(kg.total_grid_points * kg.Nt for kg, *_ in reqs))
-    fn = {2: kwhip.kspaceFirstOrder2D, 3: kwhip.kspaceFirstOrder3D}
-
+    n_req = len(reqs)
     with kwhip.Engine() as eng, ThreadPoolExecutor(max_workers=8) as handlers:
         t0 = time.perf_counter()
 
@@ -190,33 +191,84 @@ def served_bench(n_req, rate, steps_lo, steps_hi, seed=7):
         for i, (offset, t) in enumerate(tickets):
             _, info = eng.wait(t, with_info=True)
             done_at[i] = info["done_s"] + offset
-    # each request's own completion time (engine clock moved to wall clock)
-    lat = done_at - arrivals
-    eng_tp = cell_steps / float(np.max(done_at)) / 1e6
+    return done_at
 
-    service = np.zeros(n_req)
+
+def _served_plain_service(reqs):
+    """Service time of each request run alone through kspaceFirstOrder2D/3D
+    (host buffers in and out), the plain per-simulation GPU path."""
+    import numpy as np
+
+    import kwhip
+
+    fn = {2: kwhip.kspaceFirstOrder2D, 3: kwhip.kspaceFirstOrder3D}
+    service = np.zeros(len(reqs))
     for i, (kg, med, src, sens, opts) in enumerate(reqs):
         t1 = time.perf_counter()
         fn[kg.dim](kg, med, src, sens, **opts)
         service[i] = time.perf_counter() - t1
+    return service
+
+
+def _served_summary(reqs, arrivals, done_at, service, rate, steps_lo, steps_hi, seed):
+    import numpy as np
+
+    n_req = len(reqs)
+    cell_steps = float(sum(kg.total_grid_points * kg.Nt for kg, *_ in reqs))
+    lat = done_at - arrivals
+    eng_tp = cell_steps / float(np.max(done_at)) / 1e6
+    # the plain server runs the same requests one at a time in arrival order
     finish = np.zeros(n_req)
     tfree = 0.0
     for i in range(n_req):
         tfree = max(tfree, arrivals[i]) + service[i]
         finish[i] = tfree
     lat_plain = finish - arrivals
-    plain_tp = cell_steps / float(np.sum(service)) / 1e6
+    plain_tp = cell_steps / float(np.max(finish)) / 1e6
     pct = lambda a, q: round(float(np.percentile(a, q)), 3)  # noqa: E731
     log(f"[bench] served: {n_req} requests at {rate}/s: p50 {pct(lat, 50)} s p99 {pct(lat, 99)} s, "
         f"{eng_tp:.0f} Mcell-steps/s; one at a time: p50 {pct(lat_plain, 50)} s "
         f"p99 {pct(lat_plain, 99)} s, {plain_tp:.0f} Mcell-steps/s")
-    return {"value": round(eng_tp, 1), "unit": "Mcell-steps/s",
+    return {"rate_per_s": rate, "value": round(eng_tp, 1), "unit": "Mcell-steps/s",
             "p50_s": pct(lat, 50), "p99_s": pct(lat, 99),
             "plain_value": round(plain_tp, 1), "plain_p50_s": pct(lat_plain, 50),
             "plain_p99_s": pct(lat_plain, 99),
             "speedup_vs_plain": round(eng_tp / plain_tp, 2),
             "sample": f"{n_req} requests, Poisson at {rate}/s, {steps_lo}-{steps_hi} steps each "
                       f"(seed {seed}), {cell_steps / 1e9:.1f} G cell-steps"}
+
+
+def served_bench(n_req, rate, steps_lo, steps_hi, seed=7):
+    """A Poisson request stream of forward simulations (2D 256^2 - 1024^2,
+    3D 96^3 - 256^3, heterogeneous absorbing nonlinear medium, tone-burst
+    disc source, whole-grid p_max/p_rms): latency from arrival to result and
+    throughput (cell-steps over the time to the last result) for the engine,
+    against a plain server that runs the same requests one at a time in
+    arrival order (its latencies follow from the measured service time of
+    each request and the same arrival times)."""
+    import numpy as np
+
+    reqs, gaps = _served_requests(n_req, steps_lo, steps_hi, seed)
+    arrivals = np.cumsum(gaps / rate)
+    done_at = _served_engine(reqs, arrivals)
+    service = _served_plain_service(reqs)
+    return _served_summary(reqs, arrivals, done_at, service, rate, steps_lo, steps_hi, seed)
+
+
+def served_sweep(rates, n_req, steps_lo, steps_hi, seed=7):
+    """The served workload at several arrival rates (the same requests in the
+    same order; the plain server's service times are measured once)."""
+    import numpy as np
+
+    reqs, gaps = _served_requests(n_req, steps_lo, steps_hi, seed)
+    service = _served_plain_service(reqs)
+    out = []
+    for rate in rates:
+        arrivals = np.cumsum(gaps / rate)
+        done_at = _served_engine(reqs, arrivals)
+        out.append(_served_summary(reqs, arrivals, done_at, service, rate, steps_lo, steps_hi,
+                                   seed))
+    return out
 
 
 COPY_BYTES = 1 << 30  # calibration copy: 1 GiB read + 1 GiB written per dispatch
@@ -301,6 +353,8 @@ def main():
     ap.add_argument("--served", type=int, default=24,
                     help="requests in the served leg (0: skip)")
     ap.add_argument("--served-rate", type=float, default=10.0)
+    ap.add_argument("--served-sweep", type=float, nargs="+", metavar="RATE",
+                    help="run only the served workload at these arrival rates (one JSON line)")
     ap.add_argument("--served-steps", type=int, nargs=2, default=[200, 600])
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only the calibration copy (under rocprofv3 --pmc)")
@@ -314,6 +368,11 @@ def main():
         return
     if args.reduce_traffic:
         reduce_traffic(*args.reduce_traffic, args.traffic_kernel, args.traffic_variant, args.N)
+        return
+    if args.served_sweep:
+        n = args.served if args.served > 0 else 24
+        print(json.dumps({"served_sweep": served_sweep(args.served_sweep, n, *args.served_steps)}),
+              flush=True)
         return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on one GPU only: this build has no multi-GPU path "
