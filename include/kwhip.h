@@ -54,6 +54,11 @@ extern "C" {
 #define KW_REC_U_MAX_ALL (1u << 11)
 #define KW_REC_U_MIN_ALL (1u << 12)
 #define KW_REC_U_FINAL (1u << 13)
+/* particle velocity shifted onto the non-staggered grid, time series at the
+ * sensor points (kspaceFirstOrder_extractSensorData.m:36-57,111-122); the
+ * intensities 'I' and 'I_avg' (kspaceFirstOrder_saveIntensity.m) are host
+ * post-processing of it and of p */
+#define KW_REC_U_NON_STAGGERED (1u << 14)
 
 /* source modes, numbered as the reference's p_source_mode / u_source_mode
  * (kspaceFirstOrder_saveToDisk.m:225-231,262-268) */
@@ -145,6 +150,7 @@ typedef struct kw_outputs {
   double* u_max_all[3];
   double* u_min_all[3];
   float* u_final[3];
+  float* u_non_staggered[3]; /* [points x recorded steps] per component */
 } kw_outputs;
 
 typedef struct kw_sim kw_sim;

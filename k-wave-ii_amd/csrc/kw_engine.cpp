@@ -159,7 +159,7 @@ int64_t estimate_bytes(const kw_problem& p) {
 // host staging of a finished simulation's outputs
 // ---------------------------------------------------------------------------
 struct OutputStage {
-  std::vector<float> p, p_final, u[3], u_final[3];
+  std::vector<float> p, p_final, u[3], u_final[3], u_ns[3];
   std::vector<double> p_max, p_min, p_rms, p_max_all, p_min_all;
   std::vector<double> u_max[3], u_min[3], u_rms[3], u_max_all[3], u_min_all[3];
 
@@ -192,6 +192,7 @@ struct OutputStage {
       o.u_max_all[a] = f64(u_max_all[a], nc, r & KW_REC_U_MAX_ALL);
       o.u_min_all[a] = f64(u_min_all[a], nc, r & KW_REC_U_MIN_ALL);
       o.u_final[a] = f32(u_final[a], nc, r & KW_REC_U_FINAL);
+      o.u_non_staggered[a] = f32(u_ns[a], ns * nr, (r & KW_REC_U_NON_STAGGERED) && ns > 0);
     }
     read_outputs(S, &o);
   }
@@ -218,6 +219,7 @@ struct OutputStage {
       c(o->u_max_all[a], u_max_all[a]);
       c(o->u_min_all[a], u_min_all[a]);
       c(o->u_final[a], u_final[a]);
+      c(o->u_non_staggered[a], u_ns[a]);
     }
   }
 };
@@ -415,6 +417,13 @@ void group_step(Group& g) {
     e.rec = kRecAuto;
     g.ew(e, GCELL);
   }
+  // non-staggered velocity: per member, on the group's stream (its gather
+  // decides on the device whether the step is recorded)
+  for (Sim* s : g.sims)
+    if (s->record & KW_REC_U_NON_STAGGERED) {
+      record_uns(*s);
+      g.launches += (g.dim == 3 ? 6 : 4) * g.dim;
+    }
   EwArgs adv{};
   adv.kind = EW_ADVANCE;
   g.ew(adv, GONE);

@@ -386,6 +386,22 @@ KW_BODY void body_spec(Ctx& c, const Batch& B, const PassArgs& A) {
       auto st = [&](int, int n, cf v, int) { KW_ST32(S, addr(n), v); };
       LS::template fft<1, true>(c, pl, g, G, active, src, st, lw);
     }
+  } else if (A.mode == SM_SHIFT) {
+    // S <- ushift_d .* S for the velocity on the non-staggered grid
+    // (kspaceFirstOrder_extractSensorData.m:47-56): a per-pencil constant for
+    // x and y (3-D), varying along the pencil for the last axis
+    const int d = A.flags;
+    cf* S = P.S[A.buf[0]];
+    const cf* sh = P.ushift[d];
+    auto ld = [&](int, int n) -> cf { return KW_LD32(S, addr(n)); };
+    LS::template fft<1, false>(c, pl, g, G, active, ld, to_a, la);
+    const bool along = d == last;
+    const cf sc = (along || !active) ? cf{0.f, 0.f} : KW_LD(sh + (d == 0 ? kx : ky));
+    auto src = [&](int, int n) -> cf {
+      return cmul(along ? KW_LD32(sh, (uint32_t)n) : sc, la.get(0, n));
+    };
+    auto st = [&](int, int n, cf v, int) { KW_ST32(S, addr(n), v); };
+    LS::template fft<1, true>(c, pl, g, G, active, src, st, lw);
   } else {
     // SM_ABS: S0 <- nabla1 .* S0, S1 <- nabla2 .* S1;  SM_FILTER: one table
     const int nb = A.mode == SM_ABS ? 2 : 1;
@@ -843,6 +859,7 @@ enum EwKind : int32_t {
   EW_RECORD_GRID,    // whole-grid statistics (*_all) not fused elsewhere
   EW_ADVANCE,        // t_index += 1
   EW_FILL,           // zero a float buffer (graph-safe memset)
+  EW_RECORD_UNS,     // non-staggered velocity component `which` at the sensor points
 };
 
 // EwArgs.rec value for "the problem's own statistics" (launches shared by
@@ -964,6 +981,15 @@ KW_BODY void body_ew(Ctx& c, const Batch& B, const EwArgs& A) {
           KW_ST(P.u_min_all[d] + i, ud < m ? ud : m);
         }
       }
+      return;
+    }
+    case EW_RECORD_UNS: {
+      // kspaceFirstOrder_extractSensorData.m:111-122: component `which` of
+      // the shifted velocity (in P.uns) at the sensor points
+      if (i >= P.nsens || t < P.record_start || P.uns == nullptr) return;
+      const int fi = t - P.record_start + 1;
+      const int64_t cell = P.sens_idx ? (int64_t)KW_LD(P.sens_idx + i) : i;
+      KW_ST(P.uns_ts[A.which] + i + (int64_t)(fi - 1) * P.nsens, KW_LD(P.uns + cell));
       return;
     }
     case EW_ADVANCE: {

@@ -132,6 +132,13 @@ struct ProbDev {
   float* u_ts[3];
   double *u_max[3], *u_min[3], *u_rms2[3];
   double *u_max_all[3], *u_min_all[3];
+  // velocity on the non-staggered grid: 1-D shift operators per axis,
+  // ifftshift(exp(-i k d/2)) with the 1/(Nx Ny Nz) of the round trip and the
+  // real part at the Nyquist bin (what real(ifft(.)) keeps of it); the x
+  // operator holds floor(Nx/2)+1 entries, y and z the full length
+  const cf* ushift[3];
+  float* uns;           // [ncell] shifted component (scratch)
+  float* uns_ts[3];     // [nsens x n_rec]
   // time reversal
   int32_t time_rev;
   const int32_t* tr_map;  // per cell: sensor point or -1
@@ -188,6 +195,7 @@ enum SpecMode : int32_t {
   SM_DIV = 1,    // S[d] -> D-_d kappa S[d], d < dim
   SM_ABS = 2,    // S[0] -> nabla1 S[0], S[1] -> nabla2 S[1]
   SM_FILTER = 3, // S[buf] -> table .* S[buf]
+  SM_SHIFT = 4,  // S[buf] -> ushift_d .* S[buf], d = PassArgs.flags
 };
 
 struct PassArgs {

@@ -381,6 +381,27 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr, void* stream) {
     if (r & KW_REC_P_RMS) P.p_rms2 = S->alloc_zero<double>(ns);
     if (r & KW_REC_P_MAX_ALL) P.p_max_all = S->alloc_zero<double>(n);
     if (r & KW_REC_P_MIN_ALL) P.p_min_all = S->alloc_zero<double>(n);
+    if (r & KW_REC_U_NON_STAGGERED) {
+      // kspaceFirstOrder_createStorageVariables.m:86-114: shift operators
+      // ifftshift(exp(-i k d/2)) (1 without a staggered grid); here with the
+      // 1/(Nx Ny Nz) of the round trip through the spectrum and the real
+      // part at the Nyquist bin of even lengths, which is all that
+      // real(ifft(.)) keeps of that bin
+      P.uns = S->alloc<float>(n);
+      const double Ntot = (double)n;
+      for (int a = 0; a < dim; ++a) {
+        P.uns_ts[a] = S->alloc_zero<float>((size_t)ns * S->n_rec);
+        const int len = a == 0 ? G.Nx / 2 + 1 : Nd[a];
+        std::vector<cf> sh(len);
+        for (int i = 0; i < len; ++i) {
+          std::complex<double> v(1.0, 0.0);
+          if (pr->use_sg) v = std::exp(std::complex<double>(0.0, -kvec(i, Nd[a], d[a]) * d[a] / 2));
+          if (Nd[a] % 2 == 0 && i == Nd[a] / 2) v = std::complex<double>(v.real(), 0.0);
+          sh[i] = cf{(float)(v.real() / Ntot), (float)(v.imag() / Ntot)};
+        }
+        P.ushift[a] = S->upload(sh);
+      }
+    }
     for (int a = 0; a < dim; ++a) {
       if (r & KW_REC_U) P.u_ts[a] = S->alloc_zero<float>((size_t)ns * S->n_rec);
       if (r & KW_REC_U_MAX) P.u_max[a] = S->alloc_zero<double>(ns);
@@ -468,6 +489,34 @@ void src_grid(Sim& S, int which, int t) {
 static const uint32_t kPointP = KW_REC_P | KW_REC_P_MAX | KW_REC_P_MIN | KW_REC_P_RMS;
 static const uint32_t kPointU = KW_REC_U | KW_REC_U_MAX | KW_REC_U_MIN | KW_REC_U_RMS;
 
+// Velocity components on the non-staggered grid at the sensor points
+// (kspaceFirstOrder_extractSensorData.m:36-57,111-122): per component, a
+// round trip through the spectrum in S[1] (free at the end of a step) with
+// the half-cell shift along the component's axis, then a gather.  The
+// gather kernel checks the step on the device, so the passes may run at
+// every step (the engine's captured rounds do).
+void record_uns(Sim& S) {
+  const int dim = S.G.dim;
+  for (int a = 0; a < dim; ++a) {
+    PassArgs r = bufs(1, 1);
+    r.real[0] = S.P.u[a];
+    S.pass(PK_X_R2C, r);
+    if (dim == 3) S.pass(PK_YFWD, r);
+    PassArgs sh = r;
+    sh.mode = SM_SHIFT;
+    sh.flags = a;
+    S.pass(PK_SPEC, sh);
+    if (dim == 3) S.pass(PK_YINV, r);
+    PassArgs c2r = r;
+    c2r.real[0] = S.P.uns;
+    S.pass(PK_X_C2R, c2r);
+    EwArgs e{};
+    e.kind = EW_RECORD_UNS;
+    e.which = a;
+    S.ew(e, S.lsens.nb);
+  }
+}
+
 static void record_step(Sim& S, int t, bool init) {
   if (S.time_rev || S.nsens == 0 || t < S.record_start) return;
   uint32_t pts = S.record & (kPointP | kPointU);
@@ -478,6 +527,7 @@ static void record_step(Sim& S, int t, bool init) {
     e.rec = pts;
     S.ew(e, S.lsens.nb);
   }
+  if (S.record & KW_REC_U_NON_STAGGERED) record_uns(S);
   uint32_t grid = S.record & (KW_REC_U_MAX_ALL | KW_REC_U_MIN_ALL);
   if (init) grid |= S.record & (KW_REC_P_MAX_ALL | KW_REC_P_MIN_ALL);
   if (grid) {
@@ -585,6 +635,7 @@ void read_outputs(Sim& S, kw_outputs* o) {
     get(o->u_max_all[a], P.u_max_all[a], n * sizeof(double));
     get(o->u_min_all[a], P.u_min_all[a], n * sizeof(double));
     if (o->u_final[a]) get(o->u_final[a], P.u[a], n * sizeof(float));
+    get(o->u_non_staggered[a], P.uns_ts[a], (size_t)ns * S.n_rec * sizeof(float));
   }
 }
 

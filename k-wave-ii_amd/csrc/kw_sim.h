@@ -284,6 +284,7 @@ struct Sim {
         if (a.mode == SM_GRAD) return sp + dim * sp + tab;
         if (a.mode == SM_DIV) return 2.0 * dim * sp + tab;
         if (a.mode == SM_ABS) return 4.0 * sp + 2.0 * tab;
+        if (a.mode == SM_SHIFT) return 2.0 * sp;
         return 2.0 * sp + tab;
       case PK_X_R2C:
         return a.nbuf * (4.0 * nc + sp);
@@ -448,6 +449,7 @@ struct Sim {
 std::unique_ptr<Sim> create_sim(const kw_problem* pr, void* stream = nullptr);
 PassArgs bufs(int nb, int b0 = 0, int b1 = 1, int b2 = 2);
 void src_grid(Sim& S, int which, int t);
+void record_uns(Sim& S);
 void step_once(Sim& S);
 void step_n(Sim& S, int64_t nsteps);
 int64_t recorded(const Sim& S);

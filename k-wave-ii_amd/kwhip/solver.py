@@ -44,7 +44,8 @@ _UNSUPPORTED = {"SaveToDisk", "StreamToDisk", "SaveToDiskExit", "HDFCompressionL
                 "PMLRange", "MultiAxialPMLRatio", "UseFD"}
 
 RECORD_ALLOWED = ("p", "p_max", "p_min", "p_rms", "p_max_all", "p_min_all", "p_final",
-                  "u", "u_max", "u_min", "u_rms", "u_max_all", "u_min_all", "u_final")
+                  "u", "u_max", "u_min", "u_rms", "u_max_all", "u_min_all", "u_final",
+                  "u_non_staggered", "I", "I_avg")
 
 
 def _get(obj, name, default=None):
@@ -310,8 +311,14 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
             if tr_data.shape != (n_sens, Nt):
                 raise ValueError("sensor.time_reversal_boundary_data must be [num_sensor_points x Nt]")
     info.record = record
+    # what the device records: the intensities are computed afterwards from
+    # p and the non-staggered velocity (kspaceFirstOrder_saveIntensity.m)
+    dev_record = set(record) - {"I", "I_avg"}
+    if "I" in record or "I_avg" in record:
+        dev_record |= {"p", "u_non_staggered"}
+    info.dev_record = dev_record
     rec_flags = 0
-    for r in record:
+    for r in dev_record:
         rec_flags |= _abi.REC[r]
     record_start = int(_get(sensor, "record_start_index", 1)) if info.use_sensor else 1
     if info.use_sensor and not info.time_rev and record_start > Nt:
@@ -534,7 +541,7 @@ def alloc_outputs(pp: PreparedProblem):
     dim, ncell, ns, nr = info.dim, int(np.prod(info.shape)), info.n_sensor, info.n_rec
     out = _abi.Outputs()
     arrays = {}
-    rec = set(info.record)
+    rec = set(info.dev_record)
     if info.time_rev:
         rec = {"p_final"}
 
@@ -567,6 +574,9 @@ def alloc_outputs(pp: PreparedProblem):
             if f"u_{stat}" in rec:
                 arrays[f"{u}_{stat}"] = f64(ns)
                 getattr(out, f"u_{stat}")[a] = _abi.dptr(arrays[f"{u}_{stat}"])
+        if "u_non_staggered" in rec:
+            arrays[f"{u}_non_staggered"] = f32(ns * nr)
+            out.u_non_staggered[a] = _abi.fptr(arrays[f"{u}_non_staggered"])
         for stat in ("max_all", "min_all"):
             if f"u_{stat}" in rec:
                 arrays[f"{u}_{stat}"] = f64(ncell)
@@ -585,19 +595,21 @@ def shape_outputs(pp: PreparedProblem, arrays: dict):
     ns, nr, shape = info.n_sensor, info.n_rec, info.shape
     sd = {}
     for k, v in arrays.items():
-        if k in ("p", "ux", "uy", "uz"):
+        if k in ("p", "ux", "uy", "uz") or k.endswith("_non_staggered"):
             sd[k] = v.reshape((ns, nr), order="F")
         elif k.endswith("_all") or k.endswith("_final"):
             sd[k] = v.reshape(shape, order="F")
         else:
             sd[k] = v
+    if info.data_recast:
+        sd = {k: np.asarray(v, dtype=np.float64) for k, v in sd.items()}
+    if info.use_sensor and not info.time_rev and ("I" in info.record or "I_avg" in info.record):
+        _save_intensity(sd, info)
     if info.reorder_index is not None and not info.time_rev:
         order = np.argsort(info.reorder_index, kind="stable")
         for k in list(sd):
             if not (k.endswith("_all") or k.endswith("_final")):
                 sd[k] = sd[k][order]
-    if info.data_recast:
-        sd = {k: np.asarray(v, dtype=np.float64) for k, v in sd.items()}
     if not info.use_sensor:
         return None
     if info.time_rev:
@@ -605,6 +617,50 @@ def shape_outputs(pp: PreparedProblem, arrays: dict):
     if not info.record_given:
         return sd["p"]
     return sd
+
+
+def fourier_shift(data, shift, axis=None):
+    """Shift sampled data by `shift` samples with the Fourier shift theorem
+    (+kwave/+legacy/fourierShift.m:73-114: by default along the last dimension, or
+    the first for a column)."""
+    data = np.asarray(data)
+    if axis is None:
+        axis = data.ndim - 1
+        if data.ndim == 2 and data.shape[1] == 1:
+            axis = 0
+    N = data.shape[axis]
+    if N % 2 == 0:
+        k = 2 * np.pi * np.arange(-N // 2, N // 2) / N
+    else:
+        k = 2 * np.pi * np.arange(-(N - 1) // 2, (N - 1) // 2 + 1) / N
+    k[N // 2] = 0.0
+    ph = np.fft.ifftshift(np.exp(1j * k * shift))
+    sh = [1] * data.ndim
+    sh[axis] = N
+    out = np.real(np.fft.ifft(ph.reshape(sh) * np.fft.fft(data, axis=axis), axis=axis))
+    return out.astype(data.dtype, copy=False)
+
+
+def _save_intensity(sd: dict, info):
+    """Intensity from p and the non-staggered velocity, after the time loop
+    (kspaceFirstOrder_saveIntensity.m:44-96): u is moved forward by half a time
+    step with a Fourier shift, I = p u, I_avg = its mean over time; the
+    velocity and p are dropped again when they were not asked for."""
+    comps = ["x", "y", "z"][:info.dim]
+    for c in comps:
+        u_sgt = fourier_shift(sd[f"u{c}_non_staggered"], 0.5)
+        ic = sd["p"] * u_sgt
+        sd[f"I{c}"] = ic
+        if "I_avg" in info.record:
+            sd[f"I{c}_avg"] = np.mean(ic, axis=ic.ndim - 1)
+    if "u_non_staggered" not in info.record:
+        for c in comps:
+            del sd[f"u{c}_non_staggered"]
+    if "I" not in info.record:
+        for c in comps:
+            del sd[f"I{c}"]
+    if "p" not in info.record:
+        del sd["p"]
 
 
 def run_prepared(pp: PreparedProblem):

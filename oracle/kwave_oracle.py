@@ -37,6 +37,11 @@ How each part is pinned (tests/test_oracle_*.py)
   LP/kspaceFirstOrder_createAbsorptionVariables.m).
 * heterogeneous c/rho: normal-incidence reflection coefficient
   (Z2-Z1)/(Z2+Z1) of a plane wave at a planar interface.
+* velocity on the non-staggered grid and the intensities (LP/
+  kspaceFirstOrder_extractSensorData.m:36-57, LP/kspaceFirstOrder_saveIntensity.m,
+  L/fourierShift.m): the exact periodic standing wave, whose non-staggered
+  velocity and intensity have closed forms; fourierShift against the shifted
+  closed form of sampled sinusoids.
 * nonlinearity (BonA + nonlinear mass conservation): second-harmonic growth
   of a plane wave vs the Fubini small-distance law beta*w*p0^2*x/(2 rho c^3).
 * sources: additive vs additive-no-correction vs dirichlet bookkeeping,
@@ -320,9 +325,50 @@ def _along(v, ax, dim):
 
 RECORD_ALLOWED = {"p", "p_max", "p_min", "p_rms", "p_max_all", "p_min_all", "p_final",
                   "u", "u_max", "u_min", "u_rms", "u_max_all", "u_min_all", "u_final",
-                  "u_non_staggered"}
-# 'I' and 'I_avg' (LP/kspaceFirstOrder_saveIntensity.m, a Fourier time shift
-# of u_non_staggered) are not restated.
+                  "u_non_staggered", "I", "I_avg"}
+
+
+def fourier_shift(data, shift, dim=None):
+    """L/fourierShift.m:73-114: shift sampled data by `shift` samples along
+    dimension `dim` (0-based here; default the last one, or the first for a
+    column vector) with real(ifft(ifftshift(exp(i k shift)) .* fft(data)))."""
+    data = np.asarray(data)
+    if dim is None:
+        dim = data.ndim - 1
+        if data.ndim == 2 and data.shape[1] == 1:
+            dim = 0
+    N = data.shape[dim]
+    if N % 2 == 0:
+        k_vec = (2 * np.pi) * (np.arange(-N / 2, N / 2) / N)
+    else:
+        k_vec = (2 * np.pi) * (np.arange(-(N - 1) / 2, (N - 1) / 2 + 1) / N)
+    k_vec[int(np.floor(N / 2))] = 0.0
+    shape = [1] * data.ndim
+    shape[dim] = N
+    ph = np.fft.ifftshift(np.exp(1j * k_vec * shift)).reshape(shape)
+    return np.real(np.fft.ifft(ph * np.fft.fft(data, axis=dim), axis=dim))
+
+
+def save_intensity(sd, dim, record):
+    """LP/kspaceFirstOrder_saveIntensity.m:44-96: I = p .* fourierShift(u
+    on the non-staggered grid, 1/2) per component, I_avg its mean over the
+    last dimension; u_non_staggered and p are removed when not requested."""
+    comps = ["x", "y", "z"][:dim]
+    for c in comps:
+        u_sgt = fourier_shift(sd[f"u{c}_non_staggered"], 0.5)
+        ic = (sd["p"] * u_sgt).astype(sd["p"].dtype)
+        sd[f"I{c}"] = ic
+        if "I_avg" in record:
+            sd[f"I{c}_avg"] = np.mean(ic, axis=ic.ndim - 1).astype(ic.dtype)
+    if "u_non_staggered" not in record:
+        for c in comps:
+            del sd[f"u{c}_non_staggered"]
+    if "I" not in record:
+        for c in comps:
+            del sd[f"I{c}"]
+    if "p" not in record:
+        del sd["p"]
+    return sd
 
 
 def _get(obj, name, default=None):
@@ -660,7 +706,10 @@ def kspace_first_order(kgrid: KGrid, medium, source, sensor, *, data_cast="off",
     n_rec = Nt - record_start_index + 1
     if use_sensor and not time_rev:
         ns = len(sensor_idx)
-        if "p" in record:
+        intensity = bool(record & {"I", "I_avg"})
+        # p and u_non_staggered are also recorded for the intensities
+        # (LP/kspaceFirstOrder_createStorageVariables.m:148,309)
+        if "p" in record or intensity:
             sd["p"] = np.zeros((ns, n_rec), dtype=rdt)
         for nm in ("p_max", "p_min", "p_rms"):
             if nm in record:
@@ -669,7 +718,7 @@ def kspace_first_order(kgrid: KGrid, medium, source, sensor, *, data_cast="off",
         if "u" in record:
             for u in ucomp:
                 sd[u] = np.zeros((ns, n_rec), dtype=rdt)
-        if "u_non_staggered" in record:
+        if "u_non_staggered" in record or intensity:
             for u in ucomp:
                 sd[u + "_non_staggered"] = np.zeros((ns, n_rec), dtype=rdt)
         for stat in ("max", "min", "rms"):
@@ -747,7 +796,7 @@ def kspace_first_order(kgrid: KGrid, medium, source, sensor, *, data_cast="off",
                     key = f"{u}_{stat}_all"
                     sd[key] = us[a].copy() if fi == 1 else op(sd[key], us[a])
 
-    u_ns_needed = "u_non_staggered" in record
+    u_ns_needed = bool(record & {"u_non_staggered", "I", "I_avg"})
     dim_c = rdt(dim)
 
     # -- time loop (kspaceFirstOrder3D.m:769-1208) ---------------------------
@@ -839,13 +888,16 @@ def kspace_first_order(kgrid: KGrid, medium, source, sensor, *, data_cast="off",
     if "u_final" in record:
         for a, u in enumerate(["ux", "uy", "uz"][:dim]):
             sd[u + "_final"] = us[a].copy()
+    # dataRecast, then saveIntensity, then reorderCartData (:1327-1343)
+    if data_recast:
+        sd = {k: v.astype(np.float64) for k, v in sd.items()}
+    if use_sensor and not time_rev and record & {"I", "I_avg"}:
+        sd = save_intensity(sd, dim, record)
     if reorder_index is not None and use_sensor and not time_rev:
         for key in list(sd.keys()):
             if key.endswith("_all") or key.endswith("_final"):
                 continue
             sd[key] = reorder_binary_sensor_data(sd[key], reorder_index)
-    if data_recast:
-        sd = {k: v.astype(np.float64) for k, v in sd.items()}
     if return_state:
         return sd, SimpleNamespace(p=p, rho=rhos, u=us)
     if not use_sensor:

@@ -152,6 +152,22 @@ def case(name, dim=2, N=None):
         if data.shape[1] < Nt:
             data = np.pad(data, ((0, 0), (0, Nt - data.shape[1])))
         sens = {"mask": m, "time_reversal_boundary_data": data + 0.01 * rng.standard_normal((npts, Nt))}
+    elif name == "intensity":
+        # the intensities alone: p and u_non_staggered are recorded for them
+        # and dropped again (kspaceFirstOrder_saveIntensity.m)
+        c, rho = hetero_medium(shape, 11)
+        med = {"sound_speed": c, "density": rho, "alpha_coeff": 0.75, "alpha_power": 1.5}
+        src["p0"] = disc(shape, centre, 3)
+        sens = {"mask": _sparse_mask(shape, rng), "record": ["I", "I_avg", "p_max"]}
+    elif name == "u_nonstaggered":
+        c, rho = hetero_medium(shape, 13)
+        med = {"sound_speed": c, "density": rho}
+        m = np.zeros(shape)
+        m[tuple(slice(n // 2 - 2, n // 2 + 2) for n in shape)] = 1
+        src["p"] = tone(Nt, dt, 2e6, 2)
+        src["p_mask"] = m
+        sens = {"mask": _sparse_mask(shape, rng), "record": ["p", "u", "u_non_staggered", "I"],
+                "record_start_index": 4}
     elif name == "cartesian_nearest":
         src["p0"] = disc(shape, centre, 3)
         npt = 12
@@ -177,7 +193,7 @@ def _sparse_mask(shape, rng, frac=0.1):
 CASES_2D = ["p0_lossless", "p0_hetero_absorbing", "p0_nonlinear_absorbing", "p0_nonlinear_lossless",
             "stokes", "no_dispersion", "no_absorption", "psrc_add", "psrc_dir", "psrc_nocorr",
             "psrc_many", "psrc_label", "psrc_many_dir", "usrc_add", "usrc_dir", "usrc_nocorr",
-            "usrc_many", "time_reversal", "cartesian_nearest"]
+            "usrc_many", "time_reversal", "cartesian_nearest", "intensity", "u_nonstaggered"]
 CASES_3D = CASES_2D
 
 

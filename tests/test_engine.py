@@ -56,6 +56,9 @@ MIX = [
     (2, "psrc_add", (64, 64)),
     (2, "p0_lossless", (75, 75)),
     (2, "time_reversal", (75, 75)),
+    # per-member passes inside a group: velocity on the non-staggered grid
+    (2, "u_nonstaggered", (30, 27)),
+    (3, "intensity", (16, 15, 18)),
 ]
 
 

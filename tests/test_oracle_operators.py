@@ -147,3 +147,23 @@ def test_find_column_major():
     m = np.zeros((3, 4))
     m[2, 0] = m[0, 1] = m[1, 3] = 1
     assert list(ko.find_F(m)) == [2, 3, 10]
+
+
+@pytest.mark.parametrize("N", [16, 17])
+def test_fourier_shift_closed_form(N):
+    """fourierShift.m moves a periodic band-limited signal forward by a
+    fraction of a sample: x_n = cos(2 pi f n / N + phi) shifted by s is
+    cos(2 pi f (n + s) / N + phi) (f below Nyquist); along the last axis of
+    a [points x time] array, or the first of a column.  The oracle and the
+    host post-processing of kwhip restate it separately; both are held to
+    the closed form."""
+    from kwhip.solver import fourier_shift as fs_product
+
+    n = np.arange(N)
+    rows = np.stack([np.cos(2 * np.pi * f * n / N + 0.3 * f) for f in (1, 3, 5)])
+    for s in (0.5, -0.25):
+        want = np.stack([np.cos(2 * np.pi * f * (n + s) / N + 0.3 * f) for f in (1, 3, 5)])
+        for fs in (ko.fourier_shift, fs_product):
+            assert np.max(np.abs(fs(rows, s) - want)) < 1e-12
+            col = rows[1][:, None]
+            assert np.max(np.abs(fs(col, s)[:, 0] - want[1])) < 1e-12

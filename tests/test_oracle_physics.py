@@ -522,3 +522,46 @@ def test_pml_absorbs_outgoing_energy():
     sd0 = run(kg, {"sound_speed": 1500.0, "density": 1000.0}, {"p0": p0},
               {"mask": np.ones(shape), "record": ["p_final"]}, pml_size=12, pml_alpha=0.0)
     assert np.max(np.abs(sd0["p_final"])) > 0.03 * np.max(p0)
+
+
+@pytest.mark.parametrize("dim,axis", [(2, 0), (2, 1), (3, 2)])
+def test_standing_wave_velocity_and_intensity(dim, axis):
+    """Periodic lossless standing wave p0 = cos(k x_a): the k-space scheme
+    is exact, so p = cos(k x) cos(w t_n) with t_n = (n-1) dt, the velocity
+    on the non-staggered grid is sin(k x) sin(w (t_n - dt/2)) / (rho c)
+    (at the grid points, half a step behind p), and after the half-step
+    Fourier shift in time of kspaceFirstOrder_saveIntensity.m the intensity
+    is I = p u = cos(k x) sin(k x) cos(w t) sin(w t) / (rho c); two whole
+    periods are recorded, so I_avg = 0.  The other components vanish."""
+    N, dx, c, rho, m = 32, 1e-4, 1500.0, 1000.0, 3
+    k = 2 * math.pi * m / (N * dx)
+    w = c * k
+    Nt = 40
+    dt = 2 * math.pi / w / 20
+    shape = [4] * dim
+    shape[axis] = N
+    args = []
+    for n in shape:
+        args += [n, dx]
+    kg = ko.KGrid(*args)
+    kg.set_time(Nt, dt)
+    x = np.arange(N) * dx
+    sh = [1] * dim
+    sh[axis] = N
+    p0 = np.broadcast_to(np.cos(k * x).reshape(sh), shape).copy()
+    sd = run(kg, {"sound_speed": c, "density": rho}, {"p0": p0},
+             {"mask": np.ones(shape), "record": ["p", "u_non_staggered", "I", "I_avg"]},
+             pml_size=2, pml_alpha=0.0)
+    xa = np.broadcast_to(x.reshape(sh), shape).ravel(order="F")[:, None]
+    t = (np.arange(Nt) * dt)[None, :]
+    comp = "xyz"[axis]
+    assert np.max(np.abs(sd["p"] - np.cos(k * xa) * np.cos(w * t))) < 1e-12
+    u = np.sin(k * xa) * np.sin(w * (t - dt / 2)) / (rho * c)
+    assert np.max(np.abs(sd[f"u{comp}_non_staggered"] - u)) < 1e-12 * np.max(np.abs(u))
+    intensity = np.cos(k * xa) * np.sin(k * xa) * np.cos(w * t) * np.sin(w * t) / (rho * c)
+    assert np.max(np.abs(sd[f"I{comp}"] - intensity)) < 1e-12 * np.max(np.abs(intensity))
+    assert np.max(np.abs(sd[f"I{comp}_avg"])) < 1e-12 * np.max(np.abs(intensity))
+    for other in "xyz"[:dim]:
+        if other != comp:
+            assert np.max(np.abs(sd[f"u{other}_non_staggered"])) < 1e-12 * np.max(np.abs(u))
+            assert np.max(np.abs(sd[f"I{other}"])) < 1e-12 * np.max(np.abs(intensity))
