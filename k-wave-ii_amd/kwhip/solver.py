@@ -196,19 +196,102 @@ def _source_series(src, nm, n_points):
     return s
 
 
+def _expand(a, sizes, edge):
+    """expandMatrix.m: pad by sizes[d] on both sides of axis d, replicating
+    the edges (edge=True) or with zeros."""
+    a = np.asarray(a)
+    pad = [(int(s), int(s)) for s in sizes]
+    if edge:
+        return np.pad(a, pad, mode="edge")
+    return np.pad(a, pad, mode="constant", constant_values=0)
+
+
+def _pml_outside(kgrid, medium, source, sensor, dim, opts):
+    """'PMLInside', false (kspaceFirstOrder_inputChecking.m:1501-1560,
+    kspaceFirstOrder_expandGridMatrices.m): the time axis is set on the
+    user's grid, p0 is smoothed there, a Cartesian sensor becomes a binary
+    mask there; then the grid grows by the PML size on both sides of every
+    axis, heterogeneous medium maps are extended by their edge values and
+    masks and p0 by zeros.  Returns the expanded inputs, the options for
+    them, the slices of the user's grid in the expanded one (the _all and
+    _final outputs are cut back to it) and the Cartesian reorder index."""
+    pml_size = opts.get("PMLSize", 20 if dim == 2 else 10)
+    sizes = [int(v) for v in np.broadcast_to(np.asarray(pml_size, dtype=np.int64), (dim,))]
+    backend = opts.get("Backend", "hip")
+    if isinstance(kgrid.Nt, str) or isinstance(kgrid.dt, str):
+        if sensor is not None and _has(sensor, "time_reversal_boundary_data"):
+            raise ValueError("kgrid.t_array must be explicitly defined for time reversal")
+        kgrid.makeTime(np.asarray(_get(medium, "sound_speed"), dtype=np.float64), 0.3)
+    args = []
+    for n, d, s in zip(kgrid.shape, kgrid.spacings, sizes):
+        args += [n + 2 * s, d]
+    kg = kWaveGrid(*args)
+    kg.setTime(kgrid.Nt, kgrid.dt)
+    opts = dict(opts)
+    opts["PMLInside"] = True
+    sm = np.atleast_1d(np.asarray(opts.get("Smooth", [True, False, False]), dtype=bool))
+    med = dict(medium)
+    for nm in ("sound_speed", "density", "alpha_coeff", "BonA"):
+        if nm in med and np.asarray(med[nm]).size > 1:
+            med[nm] = _expand(np.asarray(med[nm], dtype=np.float64), sizes, True)
+    src = None
+    if source is not None:
+        src = dict(source)
+        if "p0" in src:
+            p0 = np.asarray(src["p0"], dtype=np.float64)
+            if p0.shape != kgrid.shape:
+                raise ValueError("source.p0 must be the same size as the computational grid.")
+            if sm[0]:
+                p0 = smooth(p0, True, backend=backend)  # before the expansion
+            src["p0"] = _expand(p0, sizes, False)
+        for nm in ("p_mask", "u_mask"):
+            if nm in src:
+                src[nm] = _expand(src[nm], sizes, False)
+    # p0 is already smoothed; c and rho are smoothed on the expanded grid
+    sm_new = np.array(sm)
+    sm_new[0] = False
+    opts["Smooth"] = list(sm_new) if sm_new.size > 1 else [False]
+    if sm.size == 1 and bool(sm[0]):
+        opts["Smooth"] = [False, True, True]
+    sens = sensor
+    reorder_index = None
+    if sensor is not None and _has(sensor, "mask"):
+        sens = dict(sensor)
+        mask = np.asarray(sens["mask"])
+        if mask.shape != kgrid.shape and mask.ndim == 2 and mask.shape[0] == dim:
+            if opts.get("CartInterp", "linear") != "nearest" and "time_reversal_boundary_data" not in sens:
+                raise NotImplementedError(
+                    "CartInterp 'linear' (Delaunay interpolation) is not supported; use 'nearest'")
+            mask, order_index, reorder_index = cart2grid(kgrid, mask)
+            if "time_reversal_boundary_data" in sens:
+                d = np.asarray(sens["time_reversal_boundary_data"], dtype=np.float64)
+                sens["time_reversal_boundary_data"] = d[np.argsort(order_index, kind="stable")]
+        sens["mask"] = _expand(mask, sizes, False)
+    inside = tuple(slice(s, s + n) for s, n in zip(sizes, kgrid.shape))
+    return kg, med, src, sens, opts, inside, reorder_index
+
+
 def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> PreparedProblem:
     """Host precompute of kspaceFirstOrder{2D,3D} up to the native-offload
     variable set (kspaceFirstOrder_inputChecking.m, kspaceFirstOrder3D.m:585-
     607, kspaceFirstOrder_scaleSourceTerms.m)."""
     if kgrid.dim != dim:
         raise ValueError(f"kgrid must be {dim}D for kspaceFirstOrder{dim}D")
+    if not bool(opts.get("PMLInside", True)):
+        if medium is None or not _has(medium, "sound_speed"):
+            raise ValueError("The field medium.sound_speed must be defined.")
+        kg, med, src, sens, o, inside, reorder_index = _pml_outside(
+            kgrid, medium, source, sensor, dim, opts)
+        pp = prepare(kg, med, src, sens, dim, o)
+        pp.info.inside = inside
+        if reorder_index is not None:
+            pp.info.reorder_index = reorder_index
+        return pp
     backend = opts.get("Backend", "hip")
     pp = PreparedProblem()
     pr = pp.prob
     shape = kgrid.shape
     ncell = kgrid.total_grid_points
-    if not bool(opts.get("PMLInside", True)):
-        raise NotImplementedError("'PMLInside', false is not supported by kwhip")
 
     # -- medium (inputChecking.m:148-281) -----------------------------------
     if medium is None or not _has(medium, "sound_speed"):
@@ -269,6 +352,7 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
     info.dim, info.shape, info.Nt = dim, shape, Nt
     info.record_given = False
     info.reorder_index = None
+    info.inside = None  # user's grid inside the expanded one ('PMLInside', false)
     info.time_rev = False
     info.use_sensor = sensor is not None and _has(sensor, "mask")
     record = ["p"]
@@ -599,6 +683,9 @@ def shape_outputs(pp: PreparedProblem, arrays: dict):
             sd[k] = v.reshape((ns, nr), order="F")
         elif k.endswith("_all") or k.endswith("_final"):
             sd[k] = v.reshape(shape, order="F")
+            if info.inside is not None:
+                # kspaceFirstOrder3D.m:1306-1313, extractSensorData.m:464-569
+                sd[k] = sd[k][info.inside]
         else:
             sd[k] = v
     if info.data_recast:

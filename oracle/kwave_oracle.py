@@ -387,6 +387,94 @@ def _has(obj, name):
     return getattr(obj, name, None) is not None
 
 
+def expand_matrix(mat, sizes, edge_val=None):
+    """L/expandMatrix.m: grow by sizes[d] elements on both sides of every
+    axis d; without edge_val the new elements repeat the nearest edge value
+    (corners take the corner value), with edge_val they hold it."""
+    mat = np.asarray(mat)
+    out_shape = tuple(n + 2 * s for n, s in zip(mat.shape, sizes))
+    inner = tuple(slice(s, s + n) for s, n in zip(sizes, mat.shape))
+    if edge_val is not None:
+        out = np.full(out_shape, edge_val, dtype=mat.dtype)
+        out[inner] = mat
+        return out
+    # each new element takes the value of the nearest element of mat
+    idx = [np.clip(np.arange(m) - s, 0, n - 1) for m, s, n in zip(out_shape, sizes, mat.shape)]
+    return mat[np.ix_(*idx)]
+
+
+def _pml_outside(kgrid, medium, source, sensor, *, smooth, cart_interp, **kw):
+    """'PMLInside', false: LP/kspaceFirstOrder_inputChecking.m:1501-1560 and
+    LP/kspaceFirstOrder_expandGridMatrices.m.  The time axis is set and p0
+    smoothed on the user's grid, a Cartesian sensor becomes a binary mask
+    there; then the grid grows by the PML size on both sides of every axis
+    (heterogeneous medium maps by edge replication, p0 and masks by zeros)
+    and the simulation runs on it; the _final and _all grids are cut back to
+    the user's grid (kspaceFirstOrder3D.m:1306-1313,
+    LP/kspaceFirstOrder_extractSensorData.m:464-569) and Cartesian data
+    reordered as usual."""
+    dim = kgrid.dim
+    pml_size = kw.get("pml_size")
+    if pml_size is None:
+        pml_size = 20 if dim == 2 else 10
+    sizes = [int(v) for v in np.broadcast_to(np.asarray(pml_size, dtype=np.int64), (dim,))]
+    if kgrid.dt is None:
+        kgrid.make_time(np.asarray(_get(medium, "sound_speed"), dtype=np.float64), 0.3)
+    args = []
+    for n, d, s in zip(kgrid.shape, kgrid.spacings, sizes):
+        args += [n + 2 * s, d]
+    kg = KGrid(*args)
+    kg.set_time(kgrid.Nt, kgrid.dt)
+    sm = np.atleast_1d(np.asarray(smooth, dtype=bool))
+    smooth_p0, smooth_c0, smooth_rho0 = bool(sm[0]), bool(sm[(len(sm) + 1) // 2 - 1]), bool(sm[-1])
+    med = {k: _get(medium, k) for k in ("sound_speed", "density", "alpha_coeff", "alpha_power",
+                                        "alpha_mode", "alpha_sign", "BonA", "sound_speed_ref")
+           if _has(medium, k)}
+    for k in ("sound_speed", "density", "alpha_coeff", "BonA"):
+        if k in med and np.asarray(med[k]).size > 1:
+            med[k] = expand_matrix(np.asarray(med[k], dtype=np.float64), sizes)
+    src = None
+    if source is not None:
+        src = dict(source) if isinstance(source, dict) else dict(vars(source))
+        if "p0" in src:
+            p0 = np.asarray(src["p0"], dtype=np.float64)
+            if smooth_p0:
+                p0 = smooth_mat(p0, True)
+            src["p0"] = expand_matrix(p0, sizes, 0.0)
+        for k in ("p_mask", "u_mask"):
+            if k in src:
+                src[k] = expand_matrix(np.asarray(src[k]), sizes, 0)
+    sens = sensor
+    reorder_index = None
+    if sensor is not None and _has(sensor, "mask"):
+        sens = dict(sensor) if isinstance(sensor, dict) else dict(vars(sensor))
+        mask = np.asarray(sens["mask"])
+        if mask.shape != kgrid.shape:
+            time_rev = "time_reversal_boundary_data" in sens
+            if cart_interp != "nearest" and not time_rev:
+                raise NotImplementedError("Cartesian 'linear' interpolation is parity unpinned")
+            mask, order_index, reorder_index = cart2grid(kgrid, mask)
+            if time_rev:
+                d = np.asarray(sens["time_reversal_boundary_data"], dtype=np.float64)
+                sens["time_reversal_boundary_data"] = d[np.argsort(order_index, kind="stable")]
+        sens["mask"] = expand_matrix(mask, sizes, 0)
+    out = kspace_first_order(kg, med, src, sens, pml_inside=True,
+                             smooth=(False, smooth_c0, smooth_rho0), cart_interp=cart_interp, **kw)
+    inside = tuple(slice(s, s + n) for s, n in zip(sizes, kgrid.shape))
+    if out is None:
+        return None
+    if not isinstance(out, dict):
+        if sensor is not None and _has(sensor, "time_reversal_boundary_data"):
+            return out[inside]  # p_final of a time reversal
+        return reorder_binary_sensor_data(out, reorder_index) if reorder_index is not None else out
+    for k in list(out):
+        if k.endswith("_all") or k.endswith("_final"):
+            out[k] = out[k][inside]
+        elif reorder_index is not None:
+            out[k] = reorder_binary_sensor_data(out[k], reorder_index)
+    return out
+
+
 def kspace_first_order(kgrid: KGrid, medium, source, sensor, *, data_cast="off",
                        pml_size=None, pml_alpha=None, pml_inside=True, smooth=(True, False, False),
                        cart_interp="linear", use_kspace=True, use_sg=True,
@@ -403,7 +491,11 @@ def kspace_first_order(kgrid: KGrid, medium, source, sensor, *, data_cast="off",
     dim = kgrid.dim
     shape = kgrid.shape
     if not pml_inside:
-        raise NotImplementedError("PMLInside=false (grid expansion) is not restated")
+        return _pml_outside(kgrid, medium, source, sensor, data_cast=data_cast,
+                            pml_size=pml_size, pml_alpha=pml_alpha, smooth=smooth,
+                            cart_interp=cart_interp, use_kspace=use_kspace, use_sg=use_sg,
+                            scale_source_terms=scale_source_terms, data_recast=data_recast,
+                            fft_workers=fft_workers)
     if data_cast in ("off", "double"):
         rdt, cdt = np.float64, np.complex128
     elif data_cast == "single":

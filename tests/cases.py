@@ -55,6 +55,9 @@ def case(name, dim=2, N=None):
     rng = np.random.default_rng(zlib.crc32(name.encode()))
     if N is None:
         N = (32, 30) if dim == 2 else (16, 18, 20)
+        if name.startswith("pml_outside"):
+            # the grid grows by the PML on both sides: FFT-friendly after it
+            N = (24, 40) if dim == 2 else (12, 18, 10)
     shape = tuple(N)
     dx = 1e-4
     c0 = 1500.0
@@ -168,6 +171,28 @@ def case(name, dim=2, N=None):
         src["p_mask"] = m
         sens = {"mask": _sparse_mask(shape, rng), "record": ["p", "u", "u_non_staggered", "I"],
                 "record_start_index": 4}
+    elif name == "pml_outside":
+        # 'PMLInside', false: heterogeneous maps grow by edge replication,
+        # p0 is smoothed before the expansion, _all/_final cut back
+        c, rho = hetero_medium(shape, 17)
+        med = {"sound_speed": c, "density": rho, "alpha_coeff": 0.75, "alpha_power": 1.5}
+        src["p0"] = disc(shape, centre, 3)
+        opts["PMLInside"] = False
+        opts["Smooth"] = True
+        sens = {"mask": _sparse_mask(shape, rng),
+                "record": ["p", "p_final", "p_max_all", "u_final", "p_rms", "u_non_staggered"]}
+    elif name == "pml_outside_source":
+        m = np.zeros(shape)
+        m[tuple(slice(1, 3) for _ in shape)] = 1   # at the user's grid edge
+        src["p"] = tone(Nt, dt, 2e6, 2)
+        src["p_mask"] = m
+        opts["PMLInside"] = False
+        # Cartesian points near the source, given out of grid order (1-based
+        # grid index i sits at x = (i - floor(N/2) - 1) dx)
+        idx = np.array([[6, 3, 8, 4, 5, 7, 3, 9], [3, 6, 4, 8, 5, 2, 7, 3], [4, 2, 5, 3, 6, 4, 2, 5]])
+        pts = [(idx[a] - shape[a] // 2 - 1) * dx for a in range(dim)]
+        sens = {"mask": np.stack(pts), "record": ["p", "u", "p_min_all"]}
+        opts["CartInterp"] = "nearest"
     elif name == "cartesian_nearest":
         src["p0"] = disc(shape, centre, 3)
         npt = 12
@@ -193,7 +218,8 @@ def _sparse_mask(shape, rng, frac=0.1):
 CASES_2D = ["p0_lossless", "p0_hetero_absorbing", "p0_nonlinear_absorbing", "p0_nonlinear_lossless",
             "stokes", "no_dispersion", "no_absorption", "psrc_add", "psrc_dir", "psrc_nocorr",
             "psrc_many", "psrc_label", "psrc_many_dir", "usrc_add", "usrc_dir", "usrc_nocorr",
-            "usrc_many", "time_reversal", "cartesian_nearest", "intensity", "u_nonstaggered"]
+            "usrc_many", "time_reversal", "cartesian_nearest", "intensity", "u_nonstaggered",
+            "pml_outside", "pml_outside_source"]
 CASES_3D = CASES_2D
 
 

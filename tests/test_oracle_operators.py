@@ -167,3 +167,20 @@ def test_fourier_shift_closed_form(N):
             assert np.max(np.abs(fs(rows, s) - want)) < 1e-12
             col = rows[1][:, None]
             assert np.max(np.abs(fs(col, s)[:, 0] - want[1])) < 1e-12
+
+
+def test_expand_matrix_known_answer():
+    """expandMatrix.m: without an edge value the new elements repeat the
+    nearest edge element (corners take the corner value); with one they
+    hold it."""
+    m = np.arange(6.0).reshape(2, 3)
+    want_edge = np.array([[0, 0, 0, 1, 2, 2, 2],
+                          [0, 0, 0, 1, 2, 2, 2],
+                          [3, 3, 3, 4, 5, 5, 5],
+                          [3, 3, 3, 4, 5, 5, 5]], dtype=float)
+    assert np.array_equal(ko.expand_matrix(m, [1, 2]), want_edge)
+    want_zero = np.zeros((4, 5))
+    want_zero[1:3, 1:4] = m
+    assert np.array_equal(ko.expand_matrix(m, [1, 1], 0), want_zero)
+    m3 = np.arange(24.0).reshape(2, 3, 4)
+    assert np.array_equal(ko.expand_matrix(m3, [1, 2, 1]), np.pad(m3, [(1, 1), (2, 2), (1, 1)], mode="edge"))

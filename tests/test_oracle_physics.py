@@ -565,3 +565,45 @@ def test_standing_wave_velocity_and_intensity(dim, axis):
         if other != comp:
             assert np.max(np.abs(sd[f"u{other}_non_staggered"])) < 1e-12 * np.max(np.abs(u))
             assert np.max(np.abs(sd[f"I{other}"])) < 1e-12 * np.max(np.abs(intensity))
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_pml_outside_equals_expanded_problem(dim):
+    """'PMLInside', false is the simulation of the problem grown by the PML
+    on every side (medium maps by their edge values, p0 smoothed on the
+    user's grid and the masks by zeros), with the _final and _all grids cut
+    back to the user's grid and the sensor series unchanged; the expanded
+    problem is built here with numpy.pad."""
+    shape = (24, 40) if dim == 2 else (12, 18, 10)
+    pml = 4 if dim == 2 else 3
+    dx, Nt, dt = 1e-4, 20, 1.5e-8
+    rng = np.random.default_rng(dim)
+    c = 1500 + 100 * rng.uniform(size=shape)
+    rho = 1000 + 100 * rng.uniform(size=shape)
+    p0 = np.zeros(shape)
+    p0[tuple(slice(n // 2 - 2, n // 2 + 2) for n in shape)] = 1.0
+    mask = rng.uniform(size=shape) < 0.2
+    rec = ["p", "p_final", "p_max_all", "u_final"]
+    args = []
+    for n in shape:
+        args += [n, dx]
+    kg = ko.KGrid(*args)
+    kg.set_time(Nt, dt)
+    med = {"sound_speed": c, "density": rho, "alpha_coeff": 0.5, "alpha_power": 1.5}
+    got = run(kg, med, {"p0": p0}, {"mask": mask, "record": rec}, pml_size=pml,
+              pml_inside=False, smooth=(True, False, False))
+    pad = [(pml, pml)] * dim
+    args = []
+    for n in shape:
+        args += [n + 2 * pml, dx]
+    kge = ko.KGrid(*args)
+    kge.set_time(Nt, dt)
+    mede = dict(med, sound_speed=np.pad(c, pad, mode="edge"), density=np.pad(rho, pad, mode="edge"))
+    p0e = np.pad(ko.smooth_mat(p0, True), pad)
+    want = run(kge, mede, {"p0": p0e}, {"mask": np.pad(mask, pad), "record": rec},
+               pml_size=pml, smooth=(False, False, False))
+    inside = tuple(slice(pml, pml + n) for n in shape)
+    assert np.array_equal(got["p"], want["p"])
+    for k in ("p_final", "p_max_all", "ux_final"):
+        assert got[k].shape == shape
+        assert np.array_equal(got[k], want[k][inside])
