@@ -59,6 +59,10 @@ extern "C" {
  * intensities 'I' and 'I_avg' (kspaceFirstOrder_saveIntensity.m) are host
  * post-processing of it and of p */
 #define KW_REC_U_NON_STAGGERED (1u << 14)
+/* compressional and shear parts of the particle velocity at the sensor
+ * points, time series (kspaceFirstOrder_extractSensorData.m:124-292; binary
+ * sensor masks only, kspaceFirstOrder_inputChecking.m:1363-1366) */
+#define KW_REC_U_SPLIT_FIELD (1u << 15)
 
 /* source modes, numbered as the reference's p_source_mode / u_source_mode
  * (kspaceFirstOrder_saveToDisk.m:225-231,262-268) */
@@ -151,6 +155,8 @@ typedef struct kw_outputs {
   double* u_min_all[3];
   float* u_final[3];
   float* u_non_staggered[3]; /* [points x recorded steps] per component */
+  float* u_split_p[3];       /* [points x recorded steps] per component */
+  float* u_split_s[3];
 } kw_outputs;
 
 typedef struct kw_sim kw_sim;

@@ -159,7 +159,7 @@ int64_t estimate_bytes(const kw_problem& p) {
 // host staging of a finished simulation's outputs
 // ---------------------------------------------------------------------------
 struct OutputStage {
-  std::vector<float> p, p_final, u[3], u_final[3], u_ns[3];
+  std::vector<float> p, p_final, u[3], u_final[3], u_ns[3], u_sp[3], u_ss[3];
   std::vector<double> p_max, p_min, p_rms, p_max_all, p_min_all;
   std::vector<double> u_max[3], u_min[3], u_rms[3], u_max_all[3], u_min_all[3];
 
@@ -193,6 +193,8 @@ struct OutputStage {
       o.u_min_all[a] = f64(u_min_all[a], nc, r & KW_REC_U_MIN_ALL);
       o.u_final[a] = f32(u_final[a], nc, r & KW_REC_U_FINAL);
       o.u_non_staggered[a] = f32(u_ns[a], ns * nr, (r & KW_REC_U_NON_STAGGERED) && ns > 0);
+      o.u_split_p[a] = f32(u_sp[a], ns * nr, (r & KW_REC_U_SPLIT_FIELD) && ns > 0);
+      o.u_split_s[a] = f32(u_ss[a], ns * nr, (r & KW_REC_U_SPLIT_FIELD) && ns > 0);
     }
     read_outputs(S, &o);
   }
@@ -220,6 +222,8 @@ struct OutputStage {
       c(o->u_min_all[a], u_min_all[a]);
       c(o->u_final[a], u_final[a]);
       c(o->u_non_staggered[a], u_ns[a]);
+      c(o->u_split_p[a], u_sp[a]);
+      c(o->u_split_s[a], u_ss[a]);
     }
   }
 };
@@ -420,9 +424,10 @@ void group_step(Group& g) {
   // non-staggered velocity: per member, on the group's stream (its gather
   // decides on the device whether the step is recorded)
   for (Sim* s : g.sims)
-    if (s->record & KW_REC_U_NON_STAGGERED) {
-      record_uns(*s);
+    if (s->record & (KW_REC_U_NON_STAGGERED | KW_REC_U_SPLIT_FIELD)) {
+      record_velocity(*s);
       g.launches += (g.dim == 3 ? 6 : 4) * g.dim;
+      if (s->record & KW_REC_U_SPLIT_FIELD) g.launches += g.dim == 3 ? 9 : 5;
     }
   EwArgs adv{};
   adv.kind = EW_ADVANCE;

@@ -402,6 +402,44 @@ KW_BODY void body_spec(Ctx& c, const Batch& B, const PassArgs& A) {
     };
     auto st = [&](int, int n, cf v, int) { KW_ST32(S, addr(n), v); };
     LS::template fft<1, true>(c, pl, g, G, active, src, st, lw);
+  } else if (A.mode == SM_SPLIT) {
+    // compressional part of the velocity (kspaceFirstOrder_extractSensorData.m:
+    // 124-292): D = sum_b khat_b shift_b U_b is accumulated in lb (each thread
+    // the elements it produces, the same for every component), then
+    // component a is ifft(khat_a D).  real(ifftn(.)) keeps nothing of the
+    // bins where k_a is the Nyquist wavenumber (the product there is
+    // anti-Hermitian), so khat_a is taken as 0 there and every transform
+    // stays Hermitian.
+    const int nc = P.dim;
+    const bool nyq_n = (N & 1) == 0;
+    for (int b = 0; b < nc; ++b) {
+      const cf* Sb = P.S[A.buf[b]];
+      auto ld = [&](int, int n) -> cf { return KW_LD32(Sb, addr(n)); };
+      const cf* sh = P.sshift[b];
+      const bool along = b == last;
+      const cf sc = (along || !active) ? cf{0.f, 0.f} : KW_LD(sh + (b == 0 ? kx : ky));
+      const float* kh = P.khat[b];
+      auto acc = [&](int, int n, cf v, int) {
+        const cf s = along ? KW_LD32(sh, (uint32_t)n) : sc;
+        const cf t = cscale(cmul(s, v), KW_LD32(kh, taddr(n)));
+        lb.set(0, n, b == 0 ? t : cadd(lb.get(0, n), t));
+      };
+      LS::template fft<1, false>(c, pl, g, G, active, ld, acc, la);
+    }
+    for (int a = 0; a < nc; ++a) {
+      cf* Sa = P.S[A.buf[a]];
+      const float* kh = P.khat[a];
+      // is this pencil at the Nyquist wavenumber of axis a (for a != last)
+      const bool pen_nyq = a == 0 ? (P.nyq && kx == P.Nx / 2 && tile >= tH)
+                                  : (a != last && (P.Ny & 1) == 0 && ky == P.Ny / 2);
+      auto src = [&](int, int n) -> cf {
+        const bool zero = a == last ? (nyq_n && n == N / 2) : pen_nyq;
+        const float k = zero ? 0.f : KW_LD32(kh, taddr(n));
+        return cscale(lb.get(0, n), k);
+      };
+      auto st = [&](int, int n, cf v, int) { KW_ST32(Sa, addr(n), v); };
+      LS::template fft<1, true>(c, pl, g, G, active, src, st, la);
+    }
   } else {
     // SM_ABS: S0 <- nabla1 .* S0, S1 <- nabla2 .* S1;  SM_FILTER: one table
     const int nb = A.mode == SM_ABS ? 2 : 1;
@@ -860,6 +898,7 @@ enum EwKind : int32_t {
   EW_ADVANCE,        // t_index += 1
   EW_FILL,           // zero a float buffer (graph-safe memset)
   EW_RECORD_UNS,     // non-staggered velocity component `which` at the sensor points
+  EW_RECORD_SPLIT,   // compressional / shear parts of component `which`
 };
 
 // EwArgs.rec value for "the problem's own statistics" (launches shared by
@@ -990,6 +1029,20 @@ KW_BODY void body_ew(Ctx& c, const Batch& B, const EwArgs& A) {
       const int fi = t - P.record_start + 1;
       const int64_t cell = P.sens_idx ? (int64_t)KW_LD(P.sens_idx + i) : i;
       KW_ST(P.uns_ts[A.which] + i + (int64_t)(fi - 1) * P.nsens, KW_LD(P.uns + cell));
+      return;
+    }
+    case EW_RECORD_SPLIT: {
+      // part p of component `which` is in P.usp, the component on the
+      // non-staggered grid in P.uns; part s is their difference
+      if (i >= P.nsens || t < P.record_start || P.usp == nullptr) return;
+      const int fi = t - P.record_start + 1;
+      const int64_t cell = P.sens_idx ? (int64_t)KW_LD(P.sens_idx + i) : i;
+      const int64_t o = i + (int64_t)(fi - 1) * P.nsens;
+      const float up = KW_LD(P.usp + cell);
+      const float un = KW_LD(P.uns + cell);
+      KW_ST(P.usplit_p[A.which] + o, up);
+      KW_ST(P.usplit_s[A.which] + o, un - up);
+      if (P.uns_ts[A.which] != nullptr) KW_ST(P.uns_ts[A.which] + o, un);
       return;
     }
     case EW_ADVANCE: {

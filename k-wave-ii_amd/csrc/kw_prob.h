@@ -115,7 +115,8 @@ struct ProbDev {
   float* u[3];
   float* rho[3];
   float* rsum;  // sum of the split densities (absorbing EOS input)
-  cf* S[3];     // spectral work buffers; S[0] carries fft(p) between steps
+  cf* S[4];     // spectral work buffers; S[0] carries fft(p) between steps;
+                // S[1..dim] are distinct when the velocity split is recorded
   // sources
   TimeSource srcp;
   TimeSource srcu[3];
@@ -139,6 +140,14 @@ struct ProbDev {
   const cf* ushift[3];
   float* uns;           // [ncell] shifted component (scratch)
   float* uns_ts[3];     // [nsens x n_rec]
+  // velocity split into compressional and shear parts: tables of
+  // k_d / |k| (0 at k = 0) in the last-axis pass's order, the shift
+  // operators with their value at the Nyquist bin, and 1/(Nx Ny Nz)
+  const float* khat[3];
+  const cf* sshift[3];
+  float* usp;           // [ncell] compressional part of one component (scratch)
+  float* usplit_p[3];   // [nsens x n_rec]
+  float* usplit_s[3];
   // time reversal
   int32_t time_rev;
   const int32_t* tr_map;  // per cell: sensor point or -1
@@ -196,6 +205,7 @@ enum SpecMode : int32_t {
   SM_ABS = 2,    // S[0] -> nabla1 S[0], S[1] -> nabla2 S[1]
   SM_FILTER = 3, // S[buf] -> table .* S[buf]
   SM_SHIFT = 4,  // S[buf] -> ushift_d .* S[buf], d = PassArgs.flags
+  SM_SPLIT = 5,  // S[buf[a]] -> khat_a' .* sum_b khat_b sshift_b S[buf[b]]
 };
 
 struct PassArgs {

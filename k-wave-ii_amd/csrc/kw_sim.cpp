@@ -330,9 +330,13 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr, void* stream) {
     P.rho[a] = S->alloc_zero<float>(n);
   }
   if (P.eos == EOS_ABSORBING) P.rsum = S->alloc_zero<float>(n);
-  const int nS = std::max(dim, 2);
+  // S[1..dim] distinct for the velocity split (S[0] holds the next step's
+  // pressure spectrum when the sensors record); unused entries alias
+  const bool split = pr->sensor_num_points > 0 && pr->time_reversal_boundary_data == nullptr &&
+                     (pr->record_flags & KW_REC_U_SPLIT_FIELD);
+  const int nS = split ? dim + 1 : std::max(dim, 2);
   for (int b = 0; b < nS; ++b) P.S[b] = S->alloc_zero<cf>(G.nspec);
-  for (int b = nS; b < 3; ++b) P.S[b] = P.S[nS - 1];
+  for (int b = nS; b < 4; ++b) P.S[b] = P.S[nS - 1];
 
   // step counter
   int32_t* dt_index = S->alloc<int32_t>(1);
@@ -381,7 +385,7 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr, void* stream) {
     if (r & KW_REC_P_RMS) P.p_rms2 = S->alloc_zero<double>(ns);
     if (r & KW_REC_P_MAX_ALL) P.p_max_all = S->alloc_zero<double>(n);
     if (r & KW_REC_P_MIN_ALL) P.p_min_all = S->alloc_zero<double>(n);
-    if (r & KW_REC_U_NON_STAGGERED) {
+    if (r & (KW_REC_U_NON_STAGGERED | KW_REC_U_SPLIT_FIELD)) {
       // kspaceFirstOrder_createStorageVariables.m:86-114: shift operators
       // ifftshift(exp(-i k d/2)) (1 without a staggered grid); here with the
       // 1/(Nx Ny Nz) of the round trip through the spectrum and the real
@@ -390,7 +394,7 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr, void* stream) {
       P.uns = S->alloc<float>(n);
       const double Ntot = (double)n;
       for (int a = 0; a < dim; ++a) {
-        P.uns_ts[a] = S->alloc_zero<float>((size_t)ns * S->n_rec);
+        if (r & KW_REC_U_NON_STAGGERED) P.uns_ts[a] = S->alloc_zero<float>((size_t)ns * S->n_rec);
         const int len = a == 0 ? G.Nx / 2 + 1 : Nd[a];
         std::vector<cf> sh(len);
         for (int i = 0; i < len; ++i) {
@@ -400,6 +404,31 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr, void* stream) {
           sh[i] = cf{(float)(v.real() / Ntot), (float)(v.imag() / Ntot)};
         }
         P.ushift[a] = S->upload(sh);
+      }
+    }
+    if (r & KW_REC_U_SPLIT_FIELD) {
+      // kspaceFirstOrder_createStorageVariables.m:118-137: k_a / |k| (0 at
+      // k = 0); the shifts keep their Nyquist value here (the pass zeroes the
+      // anti-Hermitian products instead) and carry the 1/(Nx Ny Nz)
+      P.usp = S->alloc<float>(n);
+      const double Ntot = (double)n;
+      for (int a = 0; a < dim; ++a) {
+        P.usplit_p[a] = S->alloc_zero<float>((size_t)ns * S->n_rec);
+        P.usplit_s[a] = S->alloc_zero<float>((size_t)ns * S->n_rec);
+        P.khat[a] = S->spectral_table([&, a](int ix, int iy, int iz) {
+          const double km = kmag(ix, iy, iz);
+          if (km == 0.0) return 0.0;
+          const int idx[3] = {ix, iy, iz};
+          return kvec(idx[a], Nd[a], d[a]) / km;
+        });
+        const int len = a == 0 ? G.Nx / 2 + 1 : Nd[a];
+        std::vector<cf> sh(len);
+        for (int i = 0; i < len; ++i) {
+          std::complex<double> v(1.0, 0.0);
+          if (pr->use_sg) v = std::exp(std::complex<double>(0.0, -kvec(i, Nd[a], d[a]) * d[a] / 2));
+          sh[i] = cf{(float)(v.real() / Ntot), (float)(v.imag() / Ntot)};
+        }
+        P.sshift[a] = S->upload(sh);
       }
     }
     for (int a = 0; a < dim; ++a) {
@@ -489,30 +518,62 @@ void src_grid(Sim& S, int which, int t) {
 static const uint32_t kPointP = KW_REC_P | KW_REC_P_MAX | KW_REC_P_MIN | KW_REC_P_RMS;
 static const uint32_t kPointU = KW_REC_U | KW_REC_U_MAX | KW_REC_U_MIN | KW_REC_U_RMS;
 
-// Velocity components on the non-staggered grid at the sensor points
-// (kspaceFirstOrder_extractSensorData.m:36-57,111-122): per component, a
-// round trip through the spectrum in S[1] (free at the end of a step) with
-// the half-cell shift along the component's axis, then a gather.  The
-// gather kernel checks the step on the device, so the passes may run at
-// every step (the engine's captured rounds do).
-void record_uns(Sim& S) {
+// Component a of the velocity on the non-staggered grid into P.uns
+// (kspaceFirstOrder_extractSensorData.m:36-57): a round trip through the
+// spectrum in S[1] with the half-cell shift along axis a.
+static void velocity_unstaggered(Sim& S, int a) {
   const int dim = S.G.dim;
+  PassArgs r = bufs(1, 1);
+  r.real[0] = S.P.u[a];
+  S.pass(PK_X_R2C, r);
+  if (dim == 3) S.pass(PK_YFWD, r);
+  PassArgs sh = r;
+  sh.mode = SM_SHIFT;
+  sh.flags = a;
+  S.pass(PK_SPEC, sh);
+  if (dim == 3) S.pass(PK_YINV, r);
+  PassArgs c2r = r;
+  c2r.real[0] = S.P.uns;
+  S.pass(PK_X_C2R, c2r);
+}
+
+// Velocity records that need transforms, at the sensor points: the
+// non-staggered components (:111-122) and their split into compressional
+// and shear parts (:124-292).  The split transforms all components into
+// S[1..dim] (free at the end of a step, and distinct when it is recorded),
+// forms the compressional part of each in one last-axis pass, and the shear
+// part is the non-staggered component minus it.  The gather kernels check
+// the step on the device, so the passes may run at every step (the
+// engine's captured rounds do).
+void record_velocity(Sim& S) {
+  const int dim = S.G.dim;
+  const bool split = S.record & KW_REC_U_SPLIT_FIELD;
+  if (split) {
+    PassArgs all = bufs(dim, 1, 2, 3);
+    for (int b = 0; b < dim; ++b) {
+      PassArgs r = bufs(1, 1 + b);
+      r.real[0] = S.P.u[b];
+      S.pass(PK_X_R2C, r);
+    }
+    if (dim == 3) S.pass(PK_YFWD, all);
+    PassArgs sp = all;
+    sp.mode = SM_SPLIT;
+    S.pass(PK_SPEC, sp);
+    if (dim == 3) S.pass(PK_YINV, all);
+  }
   for (int a = 0; a < dim; ++a) {
-    PassArgs r = bufs(1, 1);
-    r.real[0] = S.P.u[a];
-    S.pass(PK_X_R2C, r);
-    if (dim == 3) S.pass(PK_YFWD, r);
-    PassArgs sh = r;
-    sh.mode = SM_SHIFT;
-    sh.flags = a;
-    S.pass(PK_SPEC, sh);
-    if (dim == 3) S.pass(PK_YINV, r);
-    PassArgs c2r = r;
-    c2r.real[0] = S.P.uns;
-    S.pass(PK_X_C2R, c2r);
     EwArgs e{};
-    e.kind = EW_RECORD_UNS;
     e.which = a;
+    if (split) {
+      // the compressional part first: S[1] is needed again below
+      PassArgs c2r = bufs(1, 1 + a);
+      c2r.real[0] = S.P.usp;
+      S.pass(PK_X_C2R, c2r);
+      e.kind = EW_RECORD_SPLIT;
+    } else {
+      e.kind = EW_RECORD_UNS;
+    }
+    velocity_unstaggered(S, a);
     S.ew(e, S.lsens.nb);
   }
 }
@@ -527,7 +588,7 @@ static void record_step(Sim& S, int t, bool init) {
     e.rec = pts;
     S.ew(e, S.lsens.nb);
   }
-  if (S.record & KW_REC_U_NON_STAGGERED) record_uns(S);
+  if (S.record & (KW_REC_U_NON_STAGGERED | KW_REC_U_SPLIT_FIELD)) record_velocity(S);
   uint32_t grid = S.record & (KW_REC_U_MAX_ALL | KW_REC_U_MIN_ALL);
   if (init) grid |= S.record & (KW_REC_P_MAX_ALL | KW_REC_P_MIN_ALL);
   if (grid) {
@@ -636,6 +697,8 @@ void read_outputs(Sim& S, kw_outputs* o) {
     get(o->u_min_all[a], P.u_min_all[a], n * sizeof(double));
     if (o->u_final[a]) get(o->u_final[a], P.u[a], n * sizeof(float));
     get(o->u_non_staggered[a], P.uns_ts[a], (size_t)ns * S.n_rec * sizeof(float));
+    get(o->u_split_p[a], P.usplit_p[a], (size_t)ns * S.n_rec * sizeof(float));
+    get(o->u_split_s[a], P.usplit_s[a], (size_t)ns * S.n_rec * sizeof(float));
   }
 }
 

@@ -130,9 +130,9 @@ struct LaunchGeom {
 // last-axis pass holds two pencil buffers per pencil (la, lb); with a
 // fixed-length plan only the gradient uses lb (its two inverse transforms
 // both read la), so the other modes launch with la alone and fit twice as
-// many blocks per CU.
+// many blocks per CU (the velocity split keeps its sum in lb too).
 inline size_t launch_lds(PassKind k, int fixed, const PassArgs& a, size_t full) {
-  return (k == PK_SPEC && fixed && a.mode != SM_GRAD) ? full / 2 : full;
+  return (k == PK_SPEC && fixed && a.mode != SM_GRAD && a.mode != SM_SPLIT) ? full / 2 : full;
 }
 
 inline int max_pencils_env() {
@@ -285,6 +285,7 @@ struct Sim {
         if (a.mode == SM_DIV) return 2.0 * dim * sp + tab;
         if (a.mode == SM_ABS) return 4.0 * sp + 2.0 * tab;
         if (a.mode == SM_SHIFT) return 2.0 * sp;
+        if (a.mode == SM_SPLIT) return 2.0 * dim * sp + dim * tab;
         return 2.0 * sp + tab;
       case PK_X_R2C:
         return a.nbuf * (4.0 * nc + sp);
@@ -449,7 +450,7 @@ struct Sim {
 std::unique_ptr<Sim> create_sim(const kw_problem* pr, void* stream = nullptr);
 PassArgs bufs(int nb, int b0 = 0, int b1 = 1, int b2 = 2);
 void src_grid(Sim& S, int which, int t);
-void record_uns(Sim& S);
+void record_velocity(Sim& S);
 void step_once(Sim& S);
 void step_n(Sim& S, int64_t nsteps);
 int64_t recorded(const Sim& S);

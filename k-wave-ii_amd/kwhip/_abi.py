@@ -29,7 +29,7 @@ REC = {
     "p_max_all": 1 << 4, "p_min_all": 1 << 5, "p_final": 1 << 6,
     "u": 1 << 7, "u_max": 1 << 8, "u_min": 1 << 9, "u_rms": 1 << 10,
     "u_max_all": 1 << 11, "u_min_all": 1 << 12, "u_final": 1 << 13,
-    "u_non_staggered": 1 << 14,
+    "u_non_staggered": 1 << 14, "u_split_field": 1 << 15,
 }
 SRC_MODE = {"dirichlet": 0, "additive-no-correction": 1, "additive": 2}
 NKINDS = 9
@@ -95,6 +95,7 @@ class Outputs(C.Structure):
         ("u", c_float_p * 3), ("u_max", c_double_p * 3), ("u_min", c_double_p * 3),
         ("u_rms", c_double_p * 3), ("u_max_all", c_double_p * 3), ("u_min_all", c_double_p * 3),
         ("u_final", c_float_p * 3), ("u_non_staggered", c_float_p * 3),
+        ("u_split_p", c_float_p * 3), ("u_split_s", c_float_p * 3),
     ]
 
 

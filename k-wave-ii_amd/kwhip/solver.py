@@ -45,7 +45,7 @@ _UNSUPPORTED = {"SaveToDisk", "StreamToDisk", "SaveToDiskExit", "HDFCompressionL
 
 RECORD_ALLOWED = ("p", "p_max", "p_min", "p_rms", "p_max_all", "p_min_all", "p_final",
                   "u", "u_max", "u_min", "u_rms", "u_max_all", "u_min_all", "u_final",
-                  "u_non_staggered", "I", "I_avg")
+                  "u_non_staggered", "I", "I_avg", "u_split_field")
 
 
 def _get(obj, name, default=None):
@@ -381,6 +381,10 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
             if cart_interp != "nearest" and not info.time_rev:
                 raise NotImplementedError(
                     "CartInterp 'linear' (Delaunay interpolation) is not supported; use 'nearest'")
+            if "u_split_field" in record:
+                # kspaceFirstOrder_inputChecking.m:1363-1366
+                raise ValueError("The option sensor.record = {'u_split_field'} is only "
+                                 "compatible with a binary sensor mask.")
             gmask, order_index, reorder_index = cart2grid(kgrid, mask)
             sens_idx = _find(gmask)
             n_sens = len(sens_idx)
@@ -661,6 +665,11 @@ def alloc_outputs(pp: PreparedProblem):
         if "u_non_staggered" in rec:
             arrays[f"{u}_non_staggered"] = f32(ns * nr)
             out.u_non_staggered[a] = _abi.fptr(arrays[f"{u}_non_staggered"])
+        if "u_split_field" in rec:
+            # kspaceFirstOrder_createStorageVariables.m:329-349
+            for part in ("p", "s"):
+                arrays[f"{u}_split_{part}"] = f32(ns * nr)
+                getattr(out, f"u_split_{part}")[a] = _abi.fptr(arrays[f"{u}_split_{part}"])
         for stat in ("max_all", "min_all"):
             if f"u_{stat}" in rec:
                 arrays[f"{u}_{stat}"] = f64(ncell)
@@ -679,7 +688,7 @@ def shape_outputs(pp: PreparedProblem, arrays: dict):
     ns, nr, shape = info.n_sensor, info.n_rec, info.shape
     sd = {}
     for k, v in arrays.items():
-        if k in ("p", "ux", "uy", "uz") or k.endswith("_non_staggered"):
+        if k in ("p", "ux", "uy", "uz") or k.endswith(("_non_staggered", "_split_p", "_split_s")):
             sd[k] = v.reshape((ns, nr), order="F")
         elif k.endswith("_all") or k.endswith("_final"):
             sd[k] = v.reshape(shape, order="F")

@@ -49,8 +49,14 @@ How each part is pinned (tests/test_oracle_*.py)
 * time reversal: ordering of the boundary enforcement, final assignment.
 * sensor statistics: p_max/p_min/p_rms against the recorded time series.
 * cart2grid 'nearest': rounding and duplicate handling on hand-made inputs.
-Cartesian 'linear' interpolation (MATLAB delaunayTriangulation) is parity
-unpinned and not restated here.
+* Cartesian 'linear' (grid_data_linear): affine fields are reproduced
+  exactly at any point (true of every triangulation of the grid), the
+  barycentric coordinates rebuild the point, points on grid nodes reduce to
+  the node, and hand-made points fall in the declared simplex.  MATLAB's
+  delaunayTriangulation picks one of the many Delaunay triangulations of a
+  regular grid in an implementation-defined way; the oracle and the product
+  use the declared one (see grid_data_linear), so fields that are not affine
+  inside a cell can differ from a MATLAB run by the choice of diagonal.
 """
 
 from __future__ import annotations
@@ -281,6 +287,75 @@ def cart2grid(kgrid: KGrid, cart_data):
     return mask, order_index, reorder_index
 
 
+def grid_data_linear(kgrid: KGrid, cart_data):
+    """Linear interpolation of grid values at Cartesian points
+    (LP/kspaceFirstOrder_createStorageVariables.m:376-398 with
+    LP/gridDataFast2D.m / gridDataFast3D.m: the enclosing simplex of a
+    Delaunay triangulation of the grid points and the barycentric
+    coordinates of the point in it; LP/kspaceFirstOrder_extractSensorData.m:
+    296-... takes sum(f(tri) .* bc, 2)).
+
+    Every Delaunay triangulation of a regular grid splits each cell into
+    simplices of its own corners; the declared one here is the Kuhn
+    subdivision.  With the cell's lower corner i and the point's local
+    coordinates f in [0, 1]^dim sorted as f[s1] >= f[s2] >= ... (ties by
+    axis order), the simplex is v0 = i, v_m = v_(m-1) + e_(s_m), with weights
+    1 - f[s1], f[s1] - f[s2], ..., f[s_dim].  In 2-D this cuts every cell
+    along its diagonal from (i, j) to (i+1, j+1).
+
+    Grid point n (0-based) of axis a lies at (n - floor(N_a/2)) d_a.
+    Returns (tri, bc): [npts, dim+1] 0-based column-major linear indices and
+    float64 weights."""
+    cart = np.asarray(cart_data, dtype=np.float64)
+    dim = kgrid.dim
+    npts = cart.shape[1]
+    corner = np.zeros((dim, npts), dtype=np.int64)
+    frac = np.zeros((dim, npts))
+    for a in range(dim):
+        N = kgrid.shape[a]
+        u = cart[a] / kgrid.spacings[a] + math.floor(N / 2)
+        if np.any(u < -1e-9) or np.any(u > N - 1 + 1e-9):
+            raise ValueError("Cartesian points must lie within the k-space grid defined by kgrid.")
+        i0 = np.clip(np.floor(u), 0, max(N - 2, 0)).astype(np.int64)
+        corner[a] = i0
+        frac[a] = np.clip(u - i0, 0.0, 1.0)
+    strides = np.cumprod([1] + list(kgrid.shape[:-1]))
+    tri = np.zeros((npts, dim + 1), dtype=np.int64)
+    bc = np.zeros((npts, dim + 1))
+    for q in range(npts):
+        order = sorted(range(dim), key=lambda a: (-frac[a, q], a))
+        v = corner[:, q].copy()
+        tri[q, 0] = int(np.dot(v, strides))
+        prev = 1.0
+        for m, a in enumerate(order):
+            bc[q, m] = prev - frac[a, q]
+            prev = frac[a, q]
+            v[a] += 1
+            tri[q, m + 1] = int(np.dot(v, strides))
+        bc[q, dim] = prev
+    return tri, bc
+
+
+class _Interp:
+    """Sensor values at Cartesian points by linear interpolation:
+    sum(f(tri) .* bc, 2) in the run's precision, summed over the simplex
+    vertices in order (LP/kspaceFirstOrder_extractSensorData.m:296-...)."""
+
+    def __init__(self, tri, bc, rdt):
+        self.tri = np.asarray(tri)
+        self.bc = np.asarray(bc).astype(rdt)
+
+    def get(self, a):
+        flat = np.asarray(a).ravel(order="F")
+        acc = flat[self.tri[:, 0]] * self.bc[:, 0]
+        for k in range(1, self.tri.shape[1]):
+            acc = acc + flat[self.tri[:, k]] * self.bc[:, k]
+        return acc
+
+    def __len__(self):
+        return self.tri.shape[0]
+
+
 def reorder_binary_sensor_data(data, reorder_index):
     """L/reorderBinarySensorData.m: sortrows on the appended reorder column."""
     order = np.argsort(np.asarray(reorder_index), kind="stable")
@@ -325,7 +400,7 @@ def _along(v, ax, dim):
 
 RECORD_ALLOWED = {"p", "p_max", "p_min", "p_rms", "p_max_all", "p_min_all", "p_final",
                   "u", "u_max", "u_min", "u_rms", "u_max_all", "u_min_all", "u_final",
-                  "u_non_staggered", "I", "I_avg"}
+                  "u_non_staggered", "I", "I_avg", "u_split_field"}
 
 
 def fourier_shift(data, shift, dim=None):
@@ -385,6 +460,39 @@ def _has(obj, name):
     if isinstance(obj, dict):
         return name in obj and obj[name] is not None
     return getattr(obj, name, None) is not None
+
+
+def velocity_split(us, kgrid, use_sg=True, rdt=np.float64, workers=None):
+    """Compressional and shear parts of the staggered velocity components
+    `us` (LP/kspaceFirstOrder_createStorageVariables.m:86-137 for the
+    operators, LP/kspaceFirstOrder_extractSensorData.m:124-292): u_k,a =
+    shift_a .* fftn(u_a) with shift_a = ifftshift(exp(-i k_a d_a/2)),
+    k_norm_a = ifftshift(k_a / |k|) (0 at k = 0); part p of component a is
+    real(ifftn(sum_b k_norm_a k_norm_b u_k,b)), part s is
+    real(ifftn((1 - k_norm_a^2) u_k,a - sum_{b != a} k_norm_a k_norm_b u_k,b))."""
+    dim = kgrid.dim
+    shape = kgrid.shape
+    cdt = np.complex64 if rdt == np.float32 else np.complex128
+    kv = kgrid.k_vecs()
+    kc = kgrid.k()
+    k_norm = []
+    for a in range(dim):
+        ka = np.broadcast_to(_along(kv[a], a, dim), shape)
+        with np.errstate(invalid="ignore", divide="ignore"):
+            kn = ka / kc
+        kn[kc == 0] = 0.0
+        k_norm.append(np.fft.ifftshift(kn).astype(rdt))
+    shift = [_along(np.fft.ifftshift(np.exp(-1j * kv[a] * kgrid.spacings[a] / 2)).astype(cdt)
+                    if use_sg else np.ones(shape[a], dtype=cdt), a, dim) for a in range(dim)]
+    uk = [shift[a] * sfft.fftn(np.asarray(us[a], dtype=rdt), workers=workers) for a in range(dim)]
+    comp, shear = [], []
+    for a in range(dim):
+        cp = sum(k_norm[a] * k_norm[b] * uk[b] for b in range(dim))
+        sh = sum(((1 - k_norm[a] ** 2) if b == a else -k_norm[a] * k_norm[b]) * uk[b]
+                 for b in range(dim))
+        comp.append(np.real(sfft.ifftn(cp, workers=workers)).astype(rdt))
+        shear.append(np.real(sfft.ifftn(sh, workers=workers)).astype(rdt))
+    return comp, shear
 
 
 def expand_matrix(mat, sizes, edge_val=None):
@@ -451,20 +559,34 @@ def _pml_outside(kgrid, medium, source, sensor, *, smooth, cart_interp, **kw):
         mask = np.asarray(sens["mask"])
         if mask.shape != kgrid.shape:
             time_rev = "time_reversal_boundary_data" in sens
-            if cart_interp != "nearest" and not time_rev:
-                raise NotImplementedError("Cartesian 'linear' interpolation is parity unpinned")
             mask, order_index, reorder_index = cart2grid(kgrid, mask)
+            if cart_interp == "linear" and not time_rev:
+                # the points keep their coordinates, which the grown grid
+                # (centred as the user's) interpolates
+                # (LP/kspaceFirstOrder_createStorageVariables.m:376-398 runs
+                # after LP/kspaceFirstOrder_expandGridMatrices.m)
+                out = kspace_first_order(kg, med, src, sens, pml_inside=True,
+                                         smooth=(False, smooth_c0, smooth_rho0),
+                                         cart_interp=cart_interp, **kw)
+                return _cut_back(out, sizes, kgrid)
             if time_rev:
                 d = np.asarray(sens["time_reversal_boundary_data"], dtype=np.float64)
                 sens["time_reversal_boundary_data"] = d[np.argsort(order_index, kind="stable")]
         sens["mask"] = expand_matrix(mask, sizes, 0)
     out = kspace_first_order(kg, med, src, sens, pml_inside=True,
                              smooth=(False, smooth_c0, smooth_rho0), cart_interp=cart_interp, **kw)
+    time_rev = sensor is not None and _has(sensor, "time_reversal_boundary_data")
+    return _cut_back(out, sizes, kgrid, reorder_index, time_rev)
+
+
+def _cut_back(out, sizes, kgrid, reorder_index=None, time_rev=False):
+    """The _final and _all grids of a run on the grown grid cut back to the
+    user's grid; Cartesian sensor rows reordered."""
     inside = tuple(slice(s, s + n) for s, n in zip(sizes, kgrid.shape))
     if out is None:
         return None
     if not isinstance(out, dict):
-        if sensor is not None and _has(sensor, "time_reversal_boundary_data"):
+        if time_rev:
             return out[inside]  # p_final of a time reversal
         return reorder_binary_sensor_data(out, reorder_index) if reorder_index is not None else out
     for k in list(out):
@@ -569,14 +691,24 @@ def kspace_first_order(kgrid: KGrid, medium, source, sensor, *, data_cast="off",
         raise ValueError("sensor.record_start_index must be less than or equal to kgrid.Nt")
     reorder_index = None
     tr_data = None
+    interp = None
     if use_sensor:
         mask = np.asarray(_get(sensor, "mask"))
         if mask.shape == shape:
             sensor_mask = mask
         elif mask.ndim == 2 and mask.shape[0] == dim:
-            if cart_interp != "nearest" and not time_rev:
-                raise NotImplementedError("Cartesian 'linear' interpolation is parity unpinned")
+            if "u_split_field" in record:
+                # LP/kspaceFirstOrder_inputChecking.m:1363-1366
+                raise ValueError("The option sensor.record = {'u_split_field'} is only "
+                                 "compatible with a binary sensor mask.")
+            # the nearest-grid mask is computed (and its checks applied) for
+            # 'linear' too (LP/kspaceFirstOrder_inputChecking.m:612-618)
             sensor_mask, order_index, reorder_index = cart2grid(kgrid, mask)
+            if cart_interp == "linear" and not time_rev:
+                # LP/kspaceFirstOrder_createStorageVariables.m:36-46,376-398:
+                # no reordering, one output row per Cartesian point
+                interp = grid_data_linear(kgrid, mask)
+                reorder_index = None
             if time_rev:
                 d = np.asarray(_get(sensor, "time_reversal_boundary_data"), dtype=np.float64)
                 tr_data = d[np.argsort(order_index, kind="stable")]
@@ -584,7 +716,7 @@ def kspace_first_order(kgrid: KGrid, medium, source, sensor, *, data_cast="off",
             raise ValueError("sensor.mask must be a binary grid or a dim x N Cartesian array")
         if time_rev and tr_data is None:
             tr_data = np.asarray(_get(sensor, "time_reversal_boundary_data"), dtype=np.float64)
-        sensor_idx = _Idx(find_F(sensor_mask), shape)
+        sensor_idx = _Interp(*interp, rdt) if interp is not None else _Idx(find_F(sensor_mask), shape)
     if time_rev:
         # fliplr (kspaceFirstOrder3D.m:721-728)
         tr_data = tr_data[:, ::-1]
@@ -813,12 +945,18 @@ def kspace_first_order(kgrid: KGrid, medium, source, sensor, *, data_cast="off",
         if "u_non_staggered" in record or intensity:
             for u in ucomp:
                 sd[u + "_non_staggered"] = np.zeros((ns, n_rec), dtype=rdt)
+        if "u_split_field" in record:
+            # LP/kspaceFirstOrder_createStorageVariables.m:329-349
+            for u in ucomp:
+                sd[u + "_split_p"] = np.zeros((ns, n_rec), dtype=rdt)
+                sd[u + "_split_s"] = np.zeros((ns, n_rec), dtype=rdt)
         for stat in ("max", "min", "rms"):
             if "u_" + stat in record:
                 for u in ucomp:
                     sd[f"{u}_{stat}"] = np.zeros(ns, dtype=rdt)
     shift_neg = [_along(C(np.fft.ifftshift(np.exp(-1j * kv[a] * kgrid.spacings[a] / 2))
                          if use_sg else np.ones(shape[a])), a, dim) for a in range(dim)]
+    split_field = "u_split_field" in record
 
     def tsig(src, name, t):
         """source.X(sig_index, t) for one time column."""
@@ -864,6 +1002,11 @@ def kspace_first_order(kgrid: KGrid, medium, source, sensor, *, data_cast="off",
                 sh = np.real(sfft.ifft(shift_neg[a] * sfft.fft(us[a], axis=a, workers=W),
                                        axis=a, workers=W)).astype(rdt)
                 sd[u + "_non_staggered"][:, fi - 1] = sensor_idx.get(sh)
+        if split_field:
+            comp, shear = velocity_split(us, kgrid, use_sg, rdt, W)
+            for a, u in enumerate(ucomp):
+                sd[u + "_split_p"][:, fi - 1] = sensor_idx.get(comp[a])
+                sd[u + "_split_s"][:, fi - 1] = sensor_idx.get(shear[a])
         for stat in ("max", "min"):
             if "u_" + stat in record:
                 for a, u in enumerate(ucomp):

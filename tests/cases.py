@@ -171,6 +171,25 @@ def case(name, dim=2, N=None):
         src["p_mask"] = m
         sens = {"mask": _sparse_mask(shape, rng), "record": ["p", "u", "u_non_staggered", "I"],
                 "record_start_index": 4}
+    elif name == "u_split":
+        # compressional and shear parts of the velocity: a velocity source
+        # whose components vary along the source plane drives a rotational
+        # field, and the density gradients add vorticity
+        c, rho = hetero_medium(shape, 19)
+        med = {"sound_speed": c, "density": rho}
+        m = np.zeros(shape)
+        sl = tuple([slice(pml + 3, pml + 5)] + [slice(pml + 2, n - pml - 2) for n in shape[1:]])
+        m[sl] = 1
+        npts = int(m.sum())
+        sig = tone(Nt, dt, 2e6, 2, amp=1e-3)
+        src["ux"] = np.stack([sig * np.cos(0.7 * i) for i in range(npts)])
+        src["uy"] = np.stack([sig * np.sin(0.4 * i + 0.3) for i in range(npts)])
+        if dim == 3:
+            src["uz"] = np.stack([0.5 * sig * np.cos(0.3 * i) for i in range(npts)])
+        src["u_mask"] = m
+        src["u_mode"] = "additive"
+        sens = {"mask": _sparse_mask(shape, rng), "record": ["p", "u_split_field", "u_non_staggered"],
+                "record_start_index": 3}
     elif name == "pml_outside":
         # 'PMLInside', false: heterogeneous maps grow by edge replication,
         # p0 is smoothed before the expansion, _all/_final cut back
@@ -218,7 +237,7 @@ def _sparse_mask(shape, rng, frac=0.1):
 CASES_2D = ["p0_lossless", "p0_hetero_absorbing", "p0_nonlinear_absorbing", "p0_nonlinear_lossless",
             "stokes", "no_dispersion", "no_absorption", "psrc_add", "psrc_dir", "psrc_nocorr",
             "psrc_many", "psrc_label", "psrc_many_dir", "usrc_add", "usrc_dir", "usrc_nocorr",
-            "usrc_many", "time_reversal", "cartesian_nearest", "intensity", "u_nonstaggered",
+            "usrc_many", "time_reversal", "cartesian_nearest", "intensity", "u_nonstaggered", "u_split",
             "pml_outside", "pml_outside_source"]
 CASES_3D = CASES_2D
 

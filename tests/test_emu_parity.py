@@ -60,6 +60,9 @@ def test_emu_parity_3d(name):
     (3, "p0_nonlinear_absorbing", (12, 10, 135)),
     (2, "psrc_label", (270, 16)),
     (2, "stokes", (16, 270)),
+    # the velocity split's last-axis pass on fixed-length kernels
+    (2, "u_split", (30, 64)),
+    (3, "u_split", (12, 16, 64)),
 ])
 def test_emu_parity_fixed_lengths(dim, name, shape):
     _check(cases.case(name, dim, shape))

@@ -59,6 +59,8 @@ MIX = [
     # per-member passes inside a group: velocity on the non-staggered grid
     (2, "u_nonstaggered", (30, 27)),
     (3, "intensity", (16, 15, 18)),
+    (2, "u_split", (27, 30)),
+    (3, "u_split", (15, 18, 16)),
 ]
 
 

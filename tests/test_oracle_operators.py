@@ -184,3 +184,38 @@ def test_expand_matrix_known_answer():
     assert np.array_equal(ko.expand_matrix(m, [1, 1], 0), want_zero)
     m3 = np.arange(24.0).reshape(2, 3, 4)
     assert np.array_equal(ko.expand_matrix(m3, [1, 2, 1]), np.pad(m3, [(1, 1), (2, 2), (1, 1)], mode="edge"))
+
+
+@pytest.mark.parametrize("shape", [(16, 12), (12, 10, 8), (15, 9)])
+def test_velocity_split_of_gradient_and_rotational_fields(shape):
+    """velocity_split on single-mode fields sampled at the staggered points
+    (x_a + d_a/2 for component a): the gradient of phi = sin(k.x) is purely
+    compressional (part p = the field at the grid points, part s = 0), a
+    rotational field (k_y f, -k_x f, 0) with f = sin(k.x) purely shear; the
+    parts sum to the non-staggered velocity."""
+    dim = len(shape)
+    d = [1e-3 * (1 + 0.1 * a) for a in range(dim)]
+    args = []
+    for n, dd in zip(shape, d):
+        args += [n, dd]
+    kg = ko.KGrid(*args)
+    m = [1, 2, 1][:dim]
+    kvec = [2 * np.pi * m[a] / (shape[a] * d[a]) for a in range(dim)]
+    grids = np.meshgrid(*[np.arange(n) * dd for n, dd in zip(shape, d)], indexing="ij")
+
+    def at(shift_axis):
+        # phase k.x at the grid points moved by half a cell along one axis
+        return sum(kvec[a] * (grids[a] + (d[a] / 2 if a == shift_axis else 0.0)) for a in range(dim))
+
+    grad = [kvec[a] * np.cos(at(a)) for a in range(dim)]
+    grad_ns = [kvec[a] * np.cos(at(-1)) for a in range(dim)]
+    comp, shear = ko.velocity_split(grad, kg)
+    for a in range(dim):
+        assert np.max(np.abs(comp[a] - grad_ns[a])) < 1e-10 * max(kvec)
+        assert np.max(np.abs(shear[a])) < 1e-10 * max(kvec)
+    rot = [kvec[1] * np.sin(at(0)), -kvec[0] * np.sin(at(1))] + ([np.zeros(shape)] if dim == 3 else [])
+    rot_ns = [kvec[1] * np.sin(at(-1)), -kvec[0] * np.sin(at(-1))] + ([np.zeros(shape)] if dim == 3 else [])
+    comp, shear = ko.velocity_split(rot, kg)
+    for a in range(dim):
+        assert np.max(np.abs(comp[a])) < 1e-10 * max(kvec)
+        assert np.max(np.abs(shear[a] - rot_ns[a])) < 1e-10 * max(kvec)

@@ -607,3 +607,25 @@ def test_pml_outside_equals_expanded_problem(dim):
     for k in ("p_final", "p_max_all", "ux_final"):
         assert got[k].shape == shape
         assert np.array_equal(got[k], want[k][inside])
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_split_field_of_irrotational_velocity(dim):
+    """In a homogeneous periodic medium the velocity is the gradient of a
+    scalar at every step (u_k = -dt/rho0 i k kappa exp(i k d/2) P summed
+    over steps), so its shear part vanishes and its compressional part is
+    the velocity on the non-staggered grid."""
+    shape = (16, 12) if dim == 2 else (12, 10, 8)
+    args = []
+    for n in shape:
+        args += [n, 1e-4]
+    kg = ko.KGrid(*args)
+    kg.set_time(12, 2e-8)
+    rng = np.random.default_rng(5)
+    sd = run(kg, {"sound_speed": 1500.0, "density": 1000.0}, {"p0": rng.standard_normal(shape)},
+             {"mask": np.ones(shape), "record": ["u_non_staggered", "u_split_field"]},
+             pml_size=2, pml_alpha=0.0)
+    for u in ["ux", "uy", "uz"][:dim]:
+        scale = np.max(np.abs(sd[u + "_non_staggered"]))
+        assert np.max(np.abs(sd[u + "_split_s"])) < 1e-12 * scale
+        assert np.max(np.abs(sd[u + "_split_p"] - sd[u + "_non_staggered"])) < 1e-12 * scale
