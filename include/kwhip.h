@@ -128,6 +128,17 @@ typedef struct kw_problem {
   /* time reversal: [sensor_num_points x Nt] in the original (not reversed)
    * time order, or NULL */
   const double* time_reversal_boundary_data;
+  /* Cartesian sensor points with 'CartInterp' 'linear'
+   * (kspaceFirstOrder_createStorageVariables.m:376-398,
+   * kspaceFirstOrder_extractSensorData.m:301-457): for each of the
+   * sensor_num_points points the dim+1 grid points of the simplex that holds
+   * it (record.tri: 1-based column-major grid indices) and its barycentric
+   * coordinates there (record.bc), both [sensor_num_points x (dim+1)]
+   * column-major; a sensor value is sum(f(tri) .* bc, 2).  sensor_mask_index
+   * must then be NULL.  NULL for binary masks; not valid with time
+   * reversal or KW_REC_U_SPLIT_FIELD. */
+  const uint64_t* sensor_tri;
+  const double* sensor_bc;
 } kw_problem;
 
 typedef struct kw_sizes {

@@ -39,8 +39,8 @@ using Clock = std::chrono::steady_clock;
 // ---------------------------------------------------------------------------
 struct ProblemCopy {
   kw_problem pb{};
-  std::vector<double> c0, rho0, rho0_sg[3], bona, alpha, p0, tr;
-  std::vector<uint64_t> sens;
+  std::vector<double> c0, rho0, rho0_sg[3], bona, alpha, p0, tr, bc;
+  std::vector<uint64_t> sens, tri;
   struct Src {
     std::vector<uint64_t> index, sig_index;
     std::vector<double> input;
@@ -67,6 +67,15 @@ struct ProblemCopy {
     if (in.sensor_mask_index && in.sensor_num_points > 0) {
       sens.assign(in.sensor_mask_index, in.sensor_mask_index + in.sensor_num_points);
       pb.sensor_mask_index = sens.data();
+    }
+    pb.sensor_tri = nullptr;
+    pb.sensor_bc = nullptr;
+    if (in.sensor_tri && in.sensor_bc && in.sensor_num_points > 0) {
+      const int64_t m = in.sensor_num_points * (in.dim + 1);
+      tri.assign(in.sensor_tri, in.sensor_tri + m);
+      bc.assign(in.sensor_bc, in.sensor_bc + m);
+      pb.sensor_tri = tri.data();
+      pb.sensor_bc = bc.data();
     }
     pb.time_reversal_boundary_data = nullptr;
     if (in.time_reversal_boundary_data && in.sensor_num_points > 0) {

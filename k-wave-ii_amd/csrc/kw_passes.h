@@ -914,6 +914,18 @@ struct EwArgs {
   uint32_t rec;          // EW_RECORD_*: which statistics to update
 };
 
+// Field f at sensor point i: the grid point of a binary mask (`cell`), or
+// for Cartesian points interpolated linearly sum(f(tri) .* bc, 2) over the
+// simplex vertices in order (kspaceFirstOrder_extractSensorData.m:301-457)
+KW_BODY float sensor_value(const ProbDev& P, const float* f, int64_t i, int64_t cell) {
+  if (P.sens_tri == nullptr) return KW_LD(f + cell);
+  const int K = P.dim + 1;
+  const int64_t o = i * K;
+  float s = KW_LD(f + KW_LD(P.sens_tri + o)) * KW_LD(P.sens_bc + o);
+  for (int k = 1; k < K; ++k) s += KW_LD(f + KW_LD(P.sens_tri + o + k)) * KW_LD(P.sens_bc + o + k);
+  return s;
+}
+
 template <class Ctx>
 KW_BODY void body_ew(Ctx& c, const Batch& B, const EwArgs& A) {
   int tile;
@@ -959,7 +971,7 @@ KW_BODY void body_ew(Ctx& c, const Batch& B, const EwArgs& A) {
       const int fi = t - P.record_start + 1;
       const int64_t cell = P.sens_idx ? (int64_t)KW_LD(P.sens_idx + i) : i;
       const uint32_t rec = A.rec == kRecAuto ? P.rec_points : A.rec;
-      const float pv = KW_LD(P.p + cell);
+      const float pv = sensor_value(P, P.p, i, cell);
       const double pd = (double)pv;
       if (rec & REC_P) KW_ST(P.p_ts + i + (int64_t)(fi - 1) * P.nsens, pv);
       if (rec & REC_P_MAX) {
@@ -975,7 +987,7 @@ KW_BODY void body_ew(Ctx& c, const Batch& B, const EwArgs& A) {
         KW_ST(P.p_rms2 + i, s + pd * pd);
       }
       for (int d = 0; d < P.dim; ++d) {
-        const float uv = KW_LD(P.u[d] + cell);
+        const float uv = sensor_value(P, P.u[d], i, cell);
         const double ud = (double)uv;
         if (rec & REC_U) KW_ST(P.u_ts[d] + i + (int64_t)(fi - 1) * P.nsens, uv);
         if (rec & REC_U_MAX) {
@@ -1028,7 +1040,7 @@ KW_BODY void body_ew(Ctx& c, const Batch& B, const EwArgs& A) {
       if (i >= P.nsens || t < P.record_start || P.uns == nullptr) return;
       const int fi = t - P.record_start + 1;
       const int64_t cell = P.sens_idx ? (int64_t)KW_LD(P.sens_idx + i) : i;
-      KW_ST(P.uns_ts[A.which] + i + (int64_t)(fi - 1) * P.nsens, KW_LD(P.uns + cell));
+      KW_ST(P.uns_ts[A.which] + i + (int64_t)(fi - 1) * P.nsens, sensor_value(P, P.uns, i, cell));
       return;
     }
     case EW_RECORD_SPLIT: {

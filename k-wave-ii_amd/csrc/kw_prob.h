@@ -127,6 +127,11 @@ struct ProbDev {
   int32_t full_mask;     // sensor mask covers every cell in natural order
   int32_t nsens;
   const int32_t* sens_idx;
+  // Cartesian points interpolated linearly: per point the dim+1 simplex
+  // vertices (0-based cells) and weights, point-major ([nsens x (dim+1)]);
+  // nullptr for binary masks
+  const int32_t* sens_tri;
+  const float* sens_bc;
   float* p_ts;           // [nsens x n_rec]
   double *p_max, *p_min, *p_rms2;  // [nsens]; p_rms2 accumulates p^2
   double *p_max_all, *p_min_all;   // [ncell]

@@ -165,7 +165,20 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr, void* stream) {
   P.n_rec = (int32_t)S->n_rec;
   P.nsens = (int32_t)S->nsens;
   P.time_rev = S->time_rev ? 1 : 0;
-  if (S->nsens > 0) {
+  const bool interp = pr->sensor_tri != nullptr || pr->sensor_bc != nullptr;
+  if (interp) {
+    if (!pr->sensor_tri || !pr->sensor_bc)
+      fail(KW_EINVAL, "sensor_tri and sensor_bc must be given together");
+    if (pr->sensor_mask_index)
+      fail(KW_EINVAL, "sensor_mask_index must be NULL for interpolated Cartesian sensor points");
+    if (S->time_rev) fail(KW_EINVAL, "time reversal takes a binary sensor mask");
+    if (S->record & KW_REC_U_SPLIT_FIELD)
+      fail(KW_EINVAL,
+           "The option sensor.record = {'u_split_field'} is only compatible with a binary "
+           "sensor mask.");
+    check_index(pr->sensor_tri, S->nsens * (dim + 1), G.ncell, "sensor_tri");
+    S->full_mask = false;
+  } else if (S->nsens > 0) {
     if (pr->sensor_mask_index) {
       check_index(pr->sensor_mask_index, S->nsens, G.ncell, "sensor.mask");
       bool identity = S->nsens == G.ncell;
@@ -372,7 +385,20 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr, void* stream) {
   }
   // sensor outputs
   if (S->nsens > 0 && !S->time_rev) {
-    if (!S->full_mask) {
+    if (interp) {
+      // record.tri / record.bc, column-major [points x (dim+1)], stored
+      // point-major so that a point's vertices are adjacent
+      const int K = dim + 1;
+      std::vector<int32_t> tri((size_t)S->nsens * K);
+      std::vector<float> bc((size_t)S->nsens * K);
+      for (int64_t i = 0; i < S->nsens; ++i)
+        for (int k = 0; k < K; ++k) {
+          tri[(size_t)i * K + k] = (int32_t)(pr->sensor_tri[i + k * S->nsens] - 1);
+          bc[(size_t)i * K + k] = (float)pr->sensor_bc[i + k * S->nsens];
+        }
+      P.sens_tri = S->upload(tri);
+      P.sens_bc = S->upload(bc);
+    } else if (!S->full_mask) {
       std::vector<int32_t> idx(S->nsens);
       for (int64_t i = 0; i < S->nsens; ++i) idx[i] = (int32_t)(pr->sensor_mask_index[i] - 1);
       P.sens_idx = S->upload(idx);

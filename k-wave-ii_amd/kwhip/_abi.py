@@ -80,6 +80,7 @@ class Problem(C.Structure):
         ("record_flags", C.c_uint32),
         ("record_start_index", C.c_int64),
         ("time_reversal_boundary_data", c_double_p),
+        ("sensor_tri", c_u64_p), ("sensor_bc", c_double_p),
     ]
 
 

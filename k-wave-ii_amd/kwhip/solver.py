@@ -160,6 +160,44 @@ def cart2grid(kgrid: kWaveGrid, cart_data):
     return (grid != 0), order_index, reorder_index
 
 
+def grid_data_linear(kgrid: kWaveGrid, cart_data):
+    """record.tri / record.bc for 'CartInterp' 'linear'
+    (kspaceFirstOrder_createStorageVariables.m:376-398 with gridDataFast2D.m /
+    gridDataFast3D.m): for each Cartesian point the simplex of a Delaunay
+    triangulation of the grid points that holds it, and the point's
+    barycentric coordinates there.
+
+    A regular grid has many Delaunay triangulations (the corners of every
+    cell are co-spherical) and MATLAB's choice among them is
+    implementation-defined; this one is declared: the Kuhn subdivision of
+    each cell.  For a point with cell corner i and local coordinates f, with
+    the axes ordered by decreasing f (ties in axis order), the simplex walks
+    from i one unit step along each axis in that order, and the weights are
+    1 - f[s1], f[s1] - f[s2], ..., f[s_dim] (in 2-D: cells cut along the
+    diagonal from (i, j) to (i+1, j+1)).
+
+    Returns tri ([points, dim+1], 1-based column-major grid indices) and bc
+    (same shape, float64)."""
+    cart = np.asarray(cart_data, dtype=np.float64)
+    dim, shape = kgrid.dim, kgrid.shape
+    npts = cart.shape[1]
+    # grid point n (0-based) of axis a sits at (n - floor(N_a/2)) d_a
+    u = np.stack([cart[a] / kgrid.spacings[a] + shape[a] // 2 for a in range(dim)])
+    top = np.array(shape, dtype=np.float64)[:, None] - 1
+    if np.any(u < -1e-9) or np.any(u > top + 1e-9):
+        raise ValueError("Cartesian points must lie within the k-space grid defined by kgrid.")
+    corner = np.clip(np.floor(u), 0, np.maximum(top - 1, 0)).astype(np.int64)
+    f = np.clip(u - corner, 0.0, 1.0)
+    order = np.argsort(-f, axis=0, kind="stable")
+    levels = np.vstack([np.ones((1, npts)), np.take_along_axis(f, order, axis=0),
+                        np.zeros((1, npts))])
+    bc = (levels[:-1] - levels[1:]).T
+    strides = np.cumprod([1] + list(shape[:-1])).astype(np.int64)
+    first = (corner * strides[:, None]).sum(axis=0)
+    tri = np.vstack([first, first + np.cumsum(strides[order], axis=0)]).T
+    return tri + 1, bc
+
+
 def _find(mask) -> np.ndarray:
     """Column-major linear indices of the nonzero cells of a grid (MATLAB's
     find).  Sparse masks are searched in memory order and the indices
@@ -259,10 +297,15 @@ def _pml_outside(kgrid, medium, source, sensor, dim, opts):
         sens = dict(sensor)
         mask = np.asarray(sens["mask"])
         if mask.shape != kgrid.shape and mask.ndim == 2 and mask.shape[0] == dim:
-            if opts.get("CartInterp", "linear") != "nearest" and "time_reversal_boundary_data" not in sens:
-                raise NotImplementedError(
-                    "CartInterp 'linear' (Delaunay interpolation) is not supported; use 'nearest'")
+            cart = mask
             mask, order_index, reorder_index = cart2grid(kgrid, mask)
+            if opts.get("CartInterp", "linear") == "linear" and "time_reversal_boundary_data" not in sens:
+                # the points keep their coordinates, which the grown grid
+                # (centred as the user's) interpolates: the triangulation is
+                # computed after kspaceFirstOrder_expandGridMatrices.m
+                inside = tuple(slice(s, s + n) for s, n in zip(sizes, kgrid.shape))
+                sens["mask"] = cart
+                return kg, med, src, sens, opts, inside, None
             if "time_reversal_boundary_data" in sens:
                 d = np.asarray(sens["time_reversal_boundary_data"], dtype=np.float64)
                 sens["time_reversal_boundary_data"] = d[np.argsort(order_index, kind="stable")]
@@ -352,6 +395,7 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
     info.dim, info.shape, info.Nt = dim, shape, Nt
     info.record_given = False
     info.reorder_index = None
+    info.interp = None
     info.inside = None  # user's grid inside the expanded one ('PMLInside', false)
     info.time_rev = False
     info.use_sensor = sensor is not None and _has(sensor, "mask")
@@ -378,17 +422,22 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
             if info.time_rev:
                 tr_data = np.asarray(_get(sensor, "time_reversal_boundary_data"), dtype=np.float64)
         elif mask.ndim == 2 and mask.shape[0] == dim:
-            if cart_interp != "nearest" and not info.time_rev:
-                raise NotImplementedError(
-                    "CartInterp 'linear' (Delaunay interpolation) is not supported; use 'nearest'")
             if "u_split_field" in record:
                 # kspaceFirstOrder_inputChecking.m:1363-1366
                 raise ValueError("The option sensor.record = {'u_split_field'} is only "
                                  "compatible with a binary sensor mask.")
+            # the nearest-grid mask and its checks apply to 'linear' too
+            # (kspaceFirstOrder_inputChecking.m:612-618)
             gmask, order_index, reorder_index = cart2grid(kgrid, mask)
-            sens_idx = _find(gmask)
-            n_sens = len(sens_idx)
-            info.reorder_index = reorder_index
+            if cart_interp == "linear" and not info.time_rev:
+                # one output row per Cartesian point, in the given order
+                # (kspaceFirstOrder_createStorageVariables.m:36-46,63-67)
+                info.interp = grid_data_linear(kgrid, mask)
+                n_sens = mask.shape[1]
+            else:
+                sens_idx = _find(gmask)
+                n_sens = len(sens_idx)
+                info.reorder_index = reorder_index
             if info.time_rev:
                 d = np.asarray(_get(sensor, "time_reversal_boundary_data"), dtype=np.float64)
                 tr_data = d[np.argsort(order_index, kind="stable")]
@@ -603,9 +652,14 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
                 fill_source(pr.u_source[ax], src_u, nm)
 
     if info.use_sensor:
-        full = sens_full or (n_sens == ncell and np.array_equal(sens_idx, np.arange(ncell)))
+        full = info.interp is None and (
+            sens_full or (n_sens == ncell and np.array_equal(sens_idx, np.arange(ncell))))
         pr.sensor_num_points = n_sens
-        if not full:
+        if info.interp is not None:
+            tri, bc = info.interp
+            pr.sensor_tri = _abi.uptr(pp.hold(np.ascontiguousarray(tri.ravel(order="F").astype(np.uint64))))
+            pr.sensor_bc = _abi.dptr(pp.hold(np.ascontiguousarray(bc.ravel(order="F"))))
+        elif not full:
             pr.sensor_mask_index = _abi.uptr(pp.hold(np.ascontiguousarray(sens_idx.astype(np.uint64) + 1)))
         pr.record_flags = rec_flags
         pr.record_start_index = record_start

@@ -55,7 +55,7 @@ def case(name, dim=2, N=None):
     rng = np.random.default_rng(zlib.crc32(name.encode()))
     if N is None:
         N = (32, 30) if dim == 2 else (16, 18, 20)
-        if name.startswith("pml_outside"):
+        if "pml_outside" in name:
             # the grid grows by the PML on both sides: FFT-friendly after it
             N = (24, 40) if dim == 2 else (12, 18, 10)
     shape = tuple(N)
@@ -212,6 +212,27 @@ def case(name, dim=2, N=None):
         pts = [(idx[a] - shape[a] // 2 - 1) * dx for a in range(dim)]
         sens = {"mask": np.stack(pts), "record": ["p", "u", "p_min_all"]}
         opts["CartInterp"] = "nearest"
+    elif name in ("cartesian_linear", "cartesian_linear_pml_outside"):
+        # points between grid nodes, interpolated on the simplices of the
+        # declared triangulation; the second grows the grid by the PML
+        c, rho = hetero_medium(shape, 23)
+        med = {"sound_speed": c, "density": rho, "alpha_coeff": 0.75, "alpha_power": 1.5}
+        src["p0"] = disc(shape, centre, 3)
+        npt = 11
+        ang = np.arange(npt) * 2 * math.pi / npt + 0.3
+        r = (min(shape) // 2 - pml - 1.37) * dx
+        pts = [r * np.cos(ang), -r * np.sin(ang)]
+        if dim == 3:
+            pts.append(0.43 * dx * np.sin(2 * ang))
+        pts = np.stack(pts)
+        pts[:, 0] = [(n - 1 - n // 2) * dx for n in shape]   # the last grid node
+        rec = ["p", "p_max", "p_min", "p_rms", "u", "u_rms", "u_non_staggered", "I_avg"]
+        if name == "cartesian_linear_pml_outside":
+            opts["PMLInside"] = False
+            pts[:, 1] = [(n - 1 - n // 2 + 0.4) * dx for n in shape]   # between the user's edge and the PML
+            rec = ["p", "p_max", "u", "p_final"]
+        sens = {"mask": pts, "record": rec}
+        opts["CartInterp"] = "linear"
     elif name == "cartesian_nearest":
         src["p0"] = disc(shape, centre, 3)
         npt = 12
@@ -238,7 +259,7 @@ CASES_2D = ["p0_lossless", "p0_hetero_absorbing", "p0_nonlinear_absorbing", "p0_
             "stokes", "no_dispersion", "no_absorption", "psrc_add", "psrc_dir", "psrc_nocorr",
             "psrc_many", "psrc_label", "psrc_many_dir", "usrc_add", "usrc_dir", "usrc_nocorr",
             "usrc_many", "time_reversal", "cartesian_nearest", "intensity", "u_nonstaggered", "u_split",
-            "pml_outside", "pml_outside_source"]
+            "pml_outside", "pml_outside_source", "cartesian_linear", "cartesian_linear_pml_outside"]
 CASES_3D = CASES_2D
 
 

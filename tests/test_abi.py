@@ -97,3 +97,46 @@ def test_errors_are_reported_not_raised():
     prob.dt, prob.c_ref, prob.c0_scalar, prob.rho0_scalar = 1e-8, 1500.0, 1500.0, 1000.0
     rc = lib.kw_sim_create(C.byref(prob), C.byref(h), buf, 256)
     assert rc == _abi.KW_EUNSUP
+
+
+def test_interpolated_sensor_points_are_checked():
+    """Cartesian points interpolated linearly: simplex vertices and weights
+    come together, exclude a mask index list, stay inside the grid, and are
+    refused with time reversal and the split velocity."""
+    import numpy as np
+    from kwhip import _abi
+
+    lib = _abi.load("emu")
+    prob = _abi.Problem()
+    prob.dim, prob.Nx, prob.Ny, prob.Nz, prob.Nt = 2, 16, 16, 1, 3
+    prob.dx = prob.dy = 1e-4
+    prob.dt, prob.c_ref, prob.c0_scalar, prob.rho0_scalar = 1e-8, 1500.0, 1500.0, 1000.0
+    prob.pml_size[0] = prob.pml_size[1] = 2
+    prob.use_kspace = prob.use_sg = 1
+    prob.sensor_num_points = 2
+    prob.record_flags = _abi.REC["p"]
+    tri = np.array([1, 5, 2, 6, 18, 22], dtype=np.uint64)     # [2 x 3] column-major
+    bc = np.array([0.5, 0.25, 0.25, 0.5, 0.25, 0.25])
+    bad = np.array([1, 5, 2, 6, 18, 257], dtype=np.uint64)
+    h = C.c_void_p()
+    buf = C.create_string_buffer(256)
+
+    def create():
+        rc = lib.kw_sim_create(C.byref(prob), C.byref(h), buf, 256)
+        if rc == 0:
+            lib.kw_sim_destroy(h)
+        return rc
+
+    prob.sensor_tri = _abi.uptr(tri)
+    assert create() == _abi.KW_EINVAL and b"together" in buf.value
+    prob.sensor_bc = _abi.dptr(bc)
+    assert create() == 0
+    prob.sensor_tri = _abi.uptr(bad)
+    assert create() == _abi.KW_EINVAL and b"range" in buf.value
+    prob.sensor_tri = _abi.uptr(tri)
+    prob.record_flags = _abi.REC["u_split_field"]
+    assert create() == _abi.KW_EINVAL and b"binary sensor mask" in buf.value
+    prob.record_flags = _abi.REC["p"]
+    idx = np.array([1, 2], dtype=np.uint64)
+    prob.sensor_mask_index = _abi.uptr(idx)
+    assert create() == _abi.KW_EINVAL and b"NULL" in buf.value

@@ -61,6 +61,8 @@ MIX = [
     (3, "intensity", (16, 15, 18)),
     (2, "u_split", (27, 30)),
     (3, "u_split", (15, 18, 16)),
+    # Cartesian points interpolated on simplices
+    (2, "cartesian_linear", (30, 32)),
 ]
 
 

@@ -143,6 +143,61 @@ def test_cart2grid_rounding_and_duplicates():
         ko.cart2grid(kg, np.array([[20.0], [0.0]]))
 
 
+@pytest.mark.parametrize("shape", [(10, 9), (7, 8, 6)])
+def test_grid_data_linear_reproduces_affine_fields(shape):
+    """Linear interpolation over any triangulation of the grid reproduces an
+    affine field exactly at every point inside the grid; the barycentric
+    weights are non-negative, sum to one and rebuild the point from the
+    simplex vertices."""
+    dim = len(shape)
+    d = [0.5 + 0.25 * a for a in range(dim)]
+    args = []
+    for n, dd in zip(shape, d):
+        args += [n, dd]
+    kg = ko.KGrid(*args)
+    rng = np.random.default_rng(3)
+    lo = [-math.floor(n / 2) * dd for n, dd in zip(shape, d)]
+    hi = [(n - 1 - math.floor(n / 2)) * dd for n, dd in zip(shape, d)]
+    pts = np.stack([rng.uniform(l, h, 200) for l, h in zip(lo, hi)])
+    pts[:, :2 ** dim] = np.array(np.meshgrid(*[[l, h] for l, h in zip(lo, hi)],
+                                             indexing="ij")).reshape(dim, -1)  # the corners
+    tri, bc = ko.grid_data_linear(kg, pts)
+    assert np.all(bc >= -1e-12) and np.allclose(bc.sum(axis=1), 1.0, atol=1e-12)
+    coords = np.meshgrid(*[(np.arange(n) - math.floor(n / 2)) * dd for n, dd in zip(shape, d)],
+                         indexing="ij")
+    for a in range(dim):
+        xa = coords[a].ravel(order="F")
+        assert np.allclose((xa[tri] * bc).sum(axis=1), pts[a], atol=1e-12)
+    coef = [1.3, -0.7, 2.1][:dim]
+    f = 0.4 + sum(c * x for c, x in zip(coef, coords))
+    interp = ko._Interp(tri, bc, np.float64)
+    assert np.allclose(interp.get(f), 0.4 + sum(c * x for c, x in zip(coef, pts)), atol=1e-12)
+
+
+def test_grid_data_linear_declared_triangulation():
+    """The declared (Kuhn) triangulation on hand-made points: in 2-D cells
+    are cut along the diagonal from (i, j) to (i+1, j+1); a grid node takes
+    weight 1; points outside the grid raise the reference's error."""
+    kg = ko.KGrid(6, 1.0, 6, 1.0)   # nodes at -3 .. 2 on both axes
+    # (0.25, 0.75) from node (3, 3) (0-based): below the diagonal side fy > fx,
+    # simplex (3,3) -> (3,4) -> (4,4) with weights 1-0.75, 0.75-0.25, 0.25
+    tri, bc = ko.grid_data_linear(kg, np.array([[0.25], [0.75]]))
+    assert list(tri[0]) == [3 + 6 * 3, 3 + 6 * 4, 4 + 6 * 4]
+    assert np.allclose(bc[0], [0.25, 0.5, 0.25])
+    tri, bc = ko.grid_data_linear(kg, np.array([[0.75], [0.25]]))
+    assert list(tri[0]) == [3 + 6 * 3, 4 + 6 * 3, 4 + 6 * 4]
+    assert np.allclose(bc[0], [0.25, 0.5, 0.25])
+    tri, bc = ko.grid_data_linear(kg, np.array([[1.0, 2.0], [-3.0, 2.0]]))
+    flat = (tri * (bc > 0.5)).sum(axis=1)
+    assert list(flat) == [4 + 6 * 0, 5 + 6 * 5] and np.allclose(bc.max(axis=1), 1.0)
+    kg3 = ko.KGrid(4, 1.0, 4, 1.0, 4, 1.0)
+    tri, bc = ko.grid_data_linear(kg3, np.array([[0.5], [0.25], [0.125]]))
+    # f = (0.5, 0.25, 0.125) from node (2, 2, 2): x, then y, then z
+    assert list(tri[0]) == [42, 43, 47, 63] and np.allclose(bc[0], [0.5, 0.25, 0.125, 0.125])
+    with pytest.raises(ValueError, match="k-space grid"):
+        ko.grid_data_linear(kg, np.array([[2.2], [0.0]]))
+
+
 def test_find_column_major():
     m = np.zeros((3, 4))
     m[2, 0] = m[0, 1] = m[1, 3] = 1

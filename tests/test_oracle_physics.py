@@ -629,3 +629,44 @@ def test_split_field_of_irrotational_velocity(dim):
         scale = np.max(np.abs(sd[u + "_non_staggered"]))
         assert np.max(np.abs(sd[u + "_split_s"])) < 1e-12 * scale
         assert np.max(np.abs(sd[u + "_split_p"] - sd[u + "_non_staggered"])) < 1e-12 * scale
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+def test_cartesian_linear_at_grid_nodes_and_midpoints(dim):
+    """'CartInterp' 'linear' on points that are grid nodes returns what
+    'nearest' returns for them (in the points' order), and at the midpoint
+    of a grid edge the mean of the two nodes' series: both hold for every
+    triangulation of the grid."""
+    shape = (14, 12) if dim == 2 else (10, 12, 8)
+    args = []
+    for n in shape:
+        args += [n, 1e-4]
+    kg = ko.KGrid(*args)
+    kg.set_time(10, 2e-8)
+    rng = np.random.default_rng(11)
+    p0 = np.zeros(shape)
+    p0[tuple(slice(n // 2 - 2, n // 2 + 2) for n in shape)] = rng.uniform(0.5, 1.5, (4,) * dim)
+    nodes = np.array([[2, -3, 0, 1][:4], [-1, 2, 0, -3][:4], [1, 0, -2, 2][:4]][:dim]) * 1e-4
+    rec = ["p", "p_max", "u", "u_non_staggered"]
+    med = {"sound_speed": 1500.0, "density": 1000.0}
+    lin = run(kg, med, {"p0": p0}, {"mask": nodes, "record": rec}, pml_size=2, cart_interp="linear")
+    near = run(kg, med, {"p0": p0}, {"mask": nodes, "record": rec}, pml_size=2, cart_interp="nearest")
+    for k in near:
+        assert np.allclose(lin[k], near[k], rtol=0, atol=1e-12 * np.max(np.abs(near[k]))), k
+    # midpoints of the edges between node 0 and its neighbour along each axis
+    mids = []
+    for a in range(dim):
+        m = nodes[:, 0].copy()
+        m[a] += 0.5e-4
+        mids.append(m)
+    ends = []
+    for a in range(dim):
+        e = nodes[:, 0].copy()
+        e[a] += 1e-4
+        ends.append(e)
+    pts = np.stack(mids + ends + [nodes[:, 0]], axis=1)
+    lin = run(kg, med, {"p0": p0}, {"mask": pts, "record": ["p"]}, pml_size=2, cart_interp="linear")
+    p = lin["p"]
+    for a in range(dim):
+        want = 0.5 * (p[-1] + p[dim + a])
+        assert np.allclose(p[a], want, rtol=0, atol=1e-12 * np.max(np.abs(p)))
