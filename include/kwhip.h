@@ -85,8 +85,8 @@ typedef struct kw_time_source {
 } kw_time_source;
 
 typedef struct kw_problem {
-  int32_t dim;               /* 2 or 3 */
-  int64_t Nx, Ny, Nz;        /* Nz = 1 in 2D */
+  int32_t dim;               /* 1, 2 or 3 (1: kspaceFirstOrder1D.m) */
+  int64_t Nx, Ny, Nz;        /* Nz = 1 in 2D; Ny = Nz = 1 in 1D */
   int64_t Nt;                /* number of time steps */
   double dx, dy, dz, dt;
   double c_ref;              /* reference sound speed (setSoundSpeedRef.m) */
@@ -287,8 +287,8 @@ int kw_engine_get_stats(kw_engine* engine, kw_engine_stats* stats, char* err, si
 /* Cancels queued and running simulations, stops the threads, frees all. */
 int kw_engine_destroy(kw_engine* engine);
 
-/* Smoothing filter: out = real(ifftn(fftn(in) .* ifftshift(win))) on a 2D or
- * 3D grid, computed on the device.  `win` is the centred window of
+/* Smoothing filter: out = real(ifftn(fftn(in) .* ifftshift(win))) on a 1D,
+ * 2D or 3D grid, computed on the device.  `win` is the centred window of
  * getWin (not shifted).  Replaces the transform of +kwave/+legacy/smooth.m;
  * the caller restores the maximum. */
 int kw_smooth(int32_t dim, const int64_t N[3], const double* in, const double* win,

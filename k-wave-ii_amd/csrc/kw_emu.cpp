@@ -305,7 +305,12 @@ static void emu_pass(PassKind kind, int dim, int nblocks, int threads, size_t ld
   switch (kind) {
     case PK_YFWD: f = [&](EmuCtx& c) { body_ypass<false, Sel>(c, B, A); }; break;
     case PK_YINV: f = [&](EmuCtx& c) { body_ypass<true, Sel>(c, B, A); }; break;
-    case PK_SPEC: f = [&](EmuCtx& c) { body_spec<Sel>(c, B, A); }; break;
+    case PK_SPEC:
+      if (spec_is_aux(A))
+        f = [&](EmuCtx& c) { body_spec<Sel, true>(c, B, A); };
+      else
+        f = [&](EmuCtx& c) { body_spec<Sel, false>(c, B, A); };
+      break;
     case PK_X_R2C: f = [&](EmuCtx& c) { body_x_r2c<SelX>(c, B, A); }; break;
     case PK_X_C2R: f = [&](EmuCtx& c) { body_x_c2r<SelX>(c, B, A); }; break;
     case PK_X_GRAD:

@@ -111,8 +111,9 @@ struct ProblemCopy {
 // the sizes kw_submit must trust to copy the arrays
 void check_shape(const kw_problem* p) {
   if (!p) fail(KW_EINVAL, "problem is NULL");
-  if (p->dim != 2 && p->dim != 3) fail(KW_EINVAL, "dim must be 2 or 3");
-  if (p->Nx < 2 || p->Ny < 2 || (p->dim == 3 && p->Nz < 2) || (p->dim == 2 && p->Nz != 1))
+  if (p->dim < 1 || p->dim > 3) fail(KW_EINVAL, "dim must be 1, 2 or 3");
+  if (p->Nx < 2 || (p->dim == 1 && (p->Ny != 1 || p->Nz != 1)) ||
+      (p->dim >= 2 && p->Ny < 2) || (p->dim == 3 && p->Nz < 2) || (p->dim == 2 && p->Nz != 1))
     fail(KW_EINVAL, "invalid grid size");
   if ((double)p->Nx * (double)p->Ny * (double)(p->dim == 3 ? p->Nz : 1) > 4.0e9)
     fail(KW_EUNSUP, "grid too large");
@@ -127,9 +128,10 @@ void check_shape(const kw_problem* p) {
 // device bytes a problem will hold (the allocations of create_sim, with
 // margin); used to admit simulations before they are created
 int64_t estimate_bytes(const kw_problem& p) {
-  const Geometry G = make_geometry(p.dim, p.Nx, p.Ny, p.Nz);
+  // a 1-D problem runs as a one-row 2-D grid
+  const int dim = std::max(p.dim, 2);
+  const Geometry G = make_geometry(dim, p.Nx, p.Ny, p.Nz);
   const double nc = (double)G.ncell, ns = (double)G.nspec;
-  const int dim = p.dim;
   double b = 4.0 * nc * (2 + 2 * dim) + 8.0 * ns * 3 + 4.0 * ns * 3;
   auto grid = [&](const void* a) { return a ? 4.0 * nc : 0.0; };
   b += grid(p.c0) + grid(p.rho0) + grid(p.BonA) + grid(p.p0);

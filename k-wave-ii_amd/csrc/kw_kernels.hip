@@ -82,8 +82,16 @@ __global__ __launch_bounds__(512) void kern(KW_KPARAMS, PassArgs A) {
 __global__ __launch_bounds__(512) void kern(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
   KW_KBATCH;
-  body_spec<Sel>(c, B, A);
+  body_spec<Sel, false>(c, B, A);
 }
+// the sensor-record modes of the last-axis pass
+#define kern_aux KW_CAT(kern, _rec)
+__global__ __launch_bounds__(512) void kern_aux(KW_KPARAMS, PassArgs A) {
+  GpuCtx c;
+  KW_KBATCH;
+  body_spec<Sel, true>(c, B, A);
+}
+#define KW_HAS_AUX 1
 #elif KW_KIND == 3
 __global__ __launch_bounds__(256) void kern(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
@@ -141,6 +149,13 @@ void KW_CAT(KW_CAT(KW_CAT(launch_v_, KW_KIND), _), KW_SIZE)(int dim, int nb, int
   }
 #endif
   (void)dim;
+#ifdef KW_HAS_AUX
+  if (spec_is_aux(A)) {
+    hipLaunchKernelGGL(kern_aux, dim3(nb), dim3(T), lds, (hipStream_t)s, B.probs, B.tile_start,
+                       B.nprob, A);
+    return;
+  }
+#endif
   hipLaunchKernelGGL(kern, dim3(nb), dim3(T), lds, (hipStream_t)s, B.probs, B.tile_start,
                      B.nprob, A);
 }

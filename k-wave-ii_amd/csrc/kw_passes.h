@@ -286,7 +286,75 @@ KW_BODY void body_ypass(Ctx& c, const Batch& B, const PassArgs& A) {
 // transforms read la.  Operators (kspaceFirstOrder3D.m:629-661, 792-803,
 // 907-909, 977-978; createAbsorptionVariables.m:29-84).
 // ---------------------------------------------------------------------------
-template <class LS, class Ctx>
+// Sensor-record modes of the last-axis pass (see body_spec).
+template <class LS, class Ctx, class Addr, class TAddr, class ToA>
+KW_BODY void spec_aux(Ctx& c, const ProbDev& P, const PassArgs& A, const FftPlan& pl, int N, int G,
+                      int last, int g, bool active, int tile, int64_t tH, int kx, int ky,
+                      const LdsPencils& la, const LdsPencils& lb, const LdsPencils& lw, Addr& addr,
+                      TAddr& taddr, ToA& to_a) {
+  if (A.mode == SM_SHIFT) {
+    // S <- ushift_d .* S for the velocity on the non-staggered grid
+    // (kspaceFirstOrder_extractSensorData.m:47-56): a per-pencil constant for
+    // x and y (3-D), varying along the pencil for the last axis
+    const int d = A.flags;
+    cf* S = P.S[A.buf[0]];
+    const cf* sh = P.ushift[d];
+    auto ld = [&](int, int n) -> cf { return KW_LD32(S, addr(n)); };
+    LS::template fft<1, false>(c, pl, g, G, active, ld, to_a, la);
+    const bool along = d == last;
+    const cf sc = (along || !active) ? cf{0.f, 0.f} : KW_LD(sh + (d == 0 ? kx : ky));
+    auto src = [&](int, int n) -> cf {
+      return cmul(along ? KW_LD32(sh, (uint32_t)n) : sc, la.get(0, n));
+    };
+    auto st = [&](int, int n, cf v, int) { KW_ST32(S, addr(n), v); };
+    LS::template fft<1, true>(c, pl, g, G, active, src, st, lw);
+  } else if (A.mode == SM_SPLIT) {
+    // compressional part of the velocity (kspaceFirstOrder_extractSensorData.m:
+    // 124-292): D = sum_b khat_b shift_b U_b is accumulated in lb (each thread
+    // the elements it produces, the same for every component), then
+    // component a is ifft(khat_a D).  real(ifftn(.)) keeps nothing of the
+    // bins where k_a is the Nyquist wavenumber (the product there is
+    // anti-Hermitian), so khat_a is taken as 0 there and every transform
+    // stays Hermitian.
+    const int nc = P.dim;
+    const bool nyq_n = (N & 1) == 0;
+    for (int b = 0; b < nc; ++b) {
+      const cf* Sb = P.S[A.buf[b]];
+      auto ld = [&](int, int n) -> cf { return KW_LD32(Sb, addr(n)); };
+      const cf* sh = P.sshift[b];
+      const bool along = b == last;
+      const cf sc = (along || !active) ? cf{0.f, 0.f} : KW_LD(sh + (b == 0 ? kx : ky));
+      const float* kh = P.khat[b];
+      auto acc = [&](int, int n, cf v, int) {
+        const cf s = along ? KW_LD32(sh, (uint32_t)n) : sc;
+        const cf t = cscale(cmul(s, v), KW_LD32(kh, taddr(n)));
+        lb.set(0, n, b == 0 ? t : cadd(lb.get(0, n), t));
+      };
+      LS::template fft<1, false>(c, pl, g, G, active, ld, acc, la);
+    }
+    for (int a = 0; a < nc; ++a) {
+      cf* Sa = P.S[A.buf[a]];
+      const float* kh = P.khat[a];
+      // is this pencil at the Nyquist wavenumber of axis a (for a != last)
+      const bool pen_nyq = a == 0 ? (P.nyq && kx == P.Nx / 2 && tile >= tH)
+                                  : (a != last && (P.Ny & 1) == 0 && ky == P.Ny / 2);
+      auto src = [&](int, int n) -> cf {
+        const bool zero = a == last ? (nyq_n && n == N / 2) : pen_nyq;
+        const float k = zero ? 0.f : KW_LD32(kh, taddr(n));
+        return cscale(lb.get(0, n), k);
+      };
+      auto st = [&](int, int n, cf v, int) { KW_ST32(Sa, addr(n), v); };
+      LS::template fft<1, true>(c, pl, g, G, active, src, st, la);
+    }
+  }
+}
+
+KW_HOSTDEV bool spec_is_aux(const PassArgs& A) { return A.mode == SM_SHIFT || A.mode == SM_SPLIT; }
+
+// AUX selects the sensor-record modes (SM_SHIFT, SM_SPLIT), built as a
+// kernel of their own so that the time step's kernel carries none of their
+// code.
+template <class LS, bool AUX, class Ctx>
 KW_BODY void body_spec(Ctx& c, const Batch& B, const PassArgs& A) {
   int tile;
   const ProbDev& P = batch_prob(c, B, tile);
@@ -351,7 +419,9 @@ KW_BODY void body_spec(Ctx& c, const Batch& B, const PassArgs& A) {
     la.set(0, n, cscale(v, KW_LD32(P.kappa, taddr(n))));
   };
 
-  if (A.mode == SM_GRAD) {
+  if constexpr (AUX) {
+    spec_aux<LS>(c, P, A, pl, N, G, last, g, active, tile, tH, kx, ky, la, lb, lw, addr, taddr, to_a);
+  } else if (A.mode == SM_GRAD) {
     cf* S0 = P.S[0];
     auto ld = [&](int, int n) -> cf { return KW_LD32(S0, addr(n)); };
     LS::template fft<1, false>(c, pl, g, G, active, ld, to_ak, la);
@@ -385,60 +455,6 @@ KW_BODY void body_spec(Ctx& c, const Batch& B, const PassArgs& A) {
       };
       auto st = [&](int, int n, cf v, int) { KW_ST32(S, addr(n), v); };
       LS::template fft<1, true>(c, pl, g, G, active, src, st, lw);
-    }
-  } else if (A.mode == SM_SHIFT) {
-    // S <- ushift_d .* S for the velocity on the non-staggered grid
-    // (kspaceFirstOrder_extractSensorData.m:47-56): a per-pencil constant for
-    // x and y (3-D), varying along the pencil for the last axis
-    const int d = A.flags;
-    cf* S = P.S[A.buf[0]];
-    const cf* sh = P.ushift[d];
-    auto ld = [&](int, int n) -> cf { return KW_LD32(S, addr(n)); };
-    LS::template fft<1, false>(c, pl, g, G, active, ld, to_a, la);
-    const bool along = d == last;
-    const cf sc = (along || !active) ? cf{0.f, 0.f} : KW_LD(sh + (d == 0 ? kx : ky));
-    auto src = [&](int, int n) -> cf {
-      return cmul(along ? KW_LD32(sh, (uint32_t)n) : sc, la.get(0, n));
-    };
-    auto st = [&](int, int n, cf v, int) { KW_ST32(S, addr(n), v); };
-    LS::template fft<1, true>(c, pl, g, G, active, src, st, lw);
-  } else if (A.mode == SM_SPLIT) {
-    // compressional part of the velocity (kspaceFirstOrder_extractSensorData.m:
-    // 124-292): D = sum_b khat_b shift_b U_b is accumulated in lb (each thread
-    // the elements it produces, the same for every component), then
-    // component a is ifft(khat_a D).  real(ifftn(.)) keeps nothing of the
-    // bins where k_a is the Nyquist wavenumber (the product there is
-    // anti-Hermitian), so khat_a is taken as 0 there and every transform
-    // stays Hermitian.
-    const int nc = P.dim;
-    const bool nyq_n = (N & 1) == 0;
-    for (int b = 0; b < nc; ++b) {
-      const cf* Sb = P.S[A.buf[b]];
-      auto ld = [&](int, int n) -> cf { return KW_LD32(Sb, addr(n)); };
-      const cf* sh = P.sshift[b];
-      const bool along = b == last;
-      const cf sc = (along || !active) ? cf{0.f, 0.f} : KW_LD(sh + (b == 0 ? kx : ky));
-      const float* kh = P.khat[b];
-      auto acc = [&](int, int n, cf v, int) {
-        const cf s = along ? KW_LD32(sh, (uint32_t)n) : sc;
-        const cf t = cscale(cmul(s, v), KW_LD32(kh, taddr(n)));
-        lb.set(0, n, b == 0 ? t : cadd(lb.get(0, n), t));
-      };
-      LS::template fft<1, false>(c, pl, g, G, active, ld, acc, la);
-    }
-    for (int a = 0; a < nc; ++a) {
-      cf* Sa = P.S[A.buf[a]];
-      const float* kh = P.khat[a];
-      // is this pencil at the Nyquist wavenumber of axis a (for a != last)
-      const bool pen_nyq = a == 0 ? (P.nyq && kx == P.Nx / 2 && tile >= tH)
-                                  : (a != last && (P.Ny & 1) == 0 && ky == P.Ny / 2);
-      auto src = [&](int, int n) -> cf {
-        const bool zero = a == last ? (nyq_n && n == N / 2) : pen_nyq;
-        const float k = zero ? 0.f : KW_LD32(kh, taddr(n));
-        return cscale(lb.get(0, n), k);
-      };
-      auto st = [&](int, int n, cf v, int) { KW_ST32(Sa, addr(n), v); };
-      LS::template fft<1, true>(c, pl, g, G, active, src, st, la);
     }
   } else {
     // SM_ABS: S0 <- nabla1 .* S0, S1 <- nabla2 .* S1;  SM_FILTER: one table
@@ -620,6 +636,13 @@ KW_BODY void with_eos(const ProbDev& P, F&& f) {
   });
 }
 
+// Density component set on the sensor cells by time reversal: p / (dim c0^2)
+// (kspaceFirstOrder3D.m:772-788, kspaceFirstOrder2D.m:715-729,
+// kspaceFirstOrder1D.m:597-611)
+KW_BODY float tr_rho(const ProbDev& P, float p, float c2) {
+  return P.dim == 3 ? p / (3.f * c2) : P.dim == 2 ? (0.5f * p) / c2 : p / c2;
+}
+
 // New p of the cells of this thread's slots (pv[slot] = rows r0, r1): time
 // reversal, sensors and the optional store, each as one batched phase over
 // the slots; finally the values go to LDS pencil 0 for fft_x(p).
@@ -640,7 +663,7 @@ KW_BODY void x_finish_p(Ctx& c, const ProbDev& P, const XTile& T, cf* pv, int t,
             KW_LD(P.tr_data + (int64_t)(i < 0 ? 0 : i) + (int64_t)(P.Nt - t - 1) * P.nsens);
         const float c0 = P.c0.at(cells[h]);
         const float c2 = c0 * c0;
-        rv[slot][h] = P.dim == 3 ? p / (3.f * c2) : (0.5f * p) / c2;
+        rv[slot][h] = tr_rho(P, p, c2);
         iv[slot][h] = i;
         float& pp = h == 0 ? pv[slot].x : pv[slot].y;
         pp = i >= 0 ? p : pp;
@@ -778,8 +801,10 @@ KW_BODY void body_x_div(Ctx& cb, const Batch& B, const PassArgs& A) {
         du1[slot] = du1[slot] + g.y;
       };
       XS::last_slots(P.fx, T.g, T.G, T.active, f);
-      // pressure source (kspaceFirstOrder3D.m:930-962) as a phase of its own
-      if (psrc) x_source<XS>(P, T, P.srcp, t, av, bv);
+      // pressure source (kspaceFirstOrder3D.m:930-962) as a phase of its own;
+      // a 1-D problem has its density in rho_x alone
+      // (kspaceFirstOrder1D.m:716-743)
+      if (psrc && d < P.dim) x_source<XS>(P, T, P.srcp, t, av, bv);
       auto acc = [&](int slot, int) {
         rs0[slot] = rs0[slot] + av[slot];
         rs1[slot] = rs1[slot] + bv[slot];
@@ -952,7 +977,7 @@ KW_BODY void body_ew(Ctx& c, const Batch& B, const EwArgs& A) {
         p = KW_LD(P.tr_data + (int64_t)k + (int64_t)(P.Nt - 1) * P.nsens);
         const float c0 = P.c0.at(i);
         const float c2 = c0 * c0;
-        const float r = P.dim == 3 ? p / (3.f * c2) : (0.5f * p) / c2;
+        const float r = tr_rho(P, p, c2);
         for (int d = 0; d < P.dim; ++d) KW_ST(P.rho[d] + i, r);
       }
       KW_ST(P.p + i, p);

@@ -98,12 +98,32 @@ static float* source_km(Sim& S, const kw_time_source& src) {
   return km;
 }
 
-std::unique_ptr<Sim> create_sim(const kw_problem* pr, void* stream) {
-  if (!pr) fail(KW_EINVAL, "problem is NULL");
-  if (pr->dim != 2 && pr->dim != 3) fail(KW_EINVAL, "dim must be 2 or 3");
-  if (pr->Nx < 2 || pr->Ny < 2 || (pr->dim == 3 && pr->Nz < 2))
+std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream) {
+  if (!pr_in) fail(KW_EINVAL, "problem is NULL");
+  const int pdim = pr_in->dim;
+  if (pdim < 1 || pdim > 3) fail(KW_EINVAL, "dim must be 1, 2 or 3");
+  // A 1-D problem (kspaceFirstOrder1D.m) runs as a 2-D grid of one row:
+  // with Ny = 1 every y wavenumber is 0, so the y derivatives vanish, uy and
+  // rho_y stay zero, and the 2-D update is the 1-D one; the kernels see
+  // P.dim = 1 where the density enters as a single component (p0, time
+  // reversal) and in the sensor records.
+  kw_problem row;
+  const kw_problem* pr = pr_in;
+  if (pdim == 1) {
+    if (pr_in->Ny != 1 || pr_in->Nz != 1) fail(KW_EINVAL, "Ny and Nz must be 1 in 1D");
+    if (pr_in->u_source[1].flag > 0 || pr_in->u_source[2].flag > 0)
+      fail(KW_EINVAL, "a 1D simulation takes the velocity source ux only");
+    row = *pr_in;
+    row.dim = 2;
+    row.dy = pr_in->dx;
+    row.pml_size[1] = row.pml_size[2] = 0;
+    row.pml_alpha[1] = row.pml_alpha[2] = 0.0;
+    row.rho0_sg[1] = row.rho0_sg[2] = nullptr;
+    pr = &row;
+  }
+  if (pr->Nx < 2 || (pdim >= 2 && pr->Ny < 2) || (pdim == 3 && pr->Nz < 2))
     fail(KW_EINVAL, "grid sizes must be at least 2");
-  if (pr->dim == 2 && pr->Nz != 1) fail(KW_EINVAL, "Nz must be 1 in 2D");
+  if (pdim == 2 && pr->Nz != 1) fail(KW_EINVAL, "Nz must be 1 in 2D");
   if (pr->Nt < 1) fail(KW_EINVAL, "Nt must be positive");
   if (!(pr->dt > 0) || !(pr->dx > 0) || !(pr->dy > 0) || (pr->dim == 3 && !(pr->dz > 0)))
     fail(KW_EINVAL, "dt and grid spacings must be positive");
@@ -123,7 +143,7 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr, void* stream) {
   S->G = make_geometry(pr->dim, pr->Nx, pr->Ny, pr->Nz);
   const Geometry& G = S->G;
   ProbDev& P = S->P;
-  P.dim = G.dim;
+  P.dim = pdim;
   P.Nx = G.Nx;
   P.Ny = G.Ny;
   P.Nz = G.Nz;
@@ -176,7 +196,7 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr, void* stream) {
       fail(KW_EINVAL,
            "The option sensor.record = {'u_split_field'} is only compatible with a binary "
            "sensor mask.");
-    check_index(pr->sensor_tri, S->nsens * (dim + 1), G.ncell, "sensor_tri");
+    check_index(pr->sensor_tri, S->nsens * (pdim + 1), G.ncell, "sensor_tri");
     S->full_mask = false;
   } else if (S->nsens > 0) {
     if (pr->sensor_mask_index) {
@@ -347,6 +367,7 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr, void* stream) {
   // pressure spectrum when the sensors record); unused entries alias
   const bool split = pr->sensor_num_points > 0 && pr->time_reversal_boundary_data == nullptr &&
                      (pr->record_flags & KW_REC_U_SPLIT_FIELD);
+  if (split && pdim == 1) fail(KW_EINVAL, "u_split_field needs a 2D or 3D simulation");
   const int nS = split ? dim + 1 : std::max(dim, 2);
   for (int b = 0; b < nS; ++b) P.S[b] = S->alloc_zero<cf>(G.nspec);
   for (int b = nS; b < 4; ++b) P.S[b] = P.S[nS - 1];
@@ -388,7 +409,7 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr, void* stream) {
     if (interp) {
       // record.tri / record.bc, column-major [points x (dim+1)], stored
       // point-major so that a point's vertices are adjacent
-      const int K = dim + 1;
+      const int K = pdim + 1;
       std::vector<int32_t> tri((size_t)S->nsens * K);
       std::vector<float> bc((size_t)S->nsens * K);
       for (int64_t i = 0; i < S->nsens; ++i)
@@ -572,7 +593,7 @@ static void velocity_unstaggered(Sim& S, int a) {
 // the step on the device, so the passes may run at every step (the
 // engine's captured rounds do).
 void record_velocity(Sim& S) {
-  const int dim = S.G.dim;
+  const int dim = S.P.dim;
   const bool split = S.record & KW_REC_U_SPLIT_FIELD;
   if (split) {
     PassArgs all = bufs(dim, 1, 2, 3);
@@ -733,7 +754,7 @@ static void smooth(int dim, const int64_t N[3], const double* in, const double* 
   kw_problem pr{};
   pr.dim = dim;
   pr.Nx = N[0];
-  pr.Ny = N[1];
+  pr.Ny = dim >= 2 ? N[1] : 1;
   pr.Nz = dim == 3 ? N[2] : 1;
   pr.Nt = 1;
   pr.dx = pr.dy = pr.dz = 1.0;
@@ -904,7 +925,7 @@ int kw_smooth(int32_t dim, const int64_t N[3], const double* in, const double* w
               char* err, size_t errlen) {
   return kw_guarded(err, errlen, [&]() {
     if (!N || !in || !win || !out) kw::fail(KW_EINVAL, "NULL argument");
-    if (dim != 2 && dim != 3) kw::fail(KW_EINVAL, "dim must be 2 or 3");
+    if (dim < 1 || dim > 3) kw::fail(KW_EINVAL, "dim must be 1, 2 or 3");
     kw::smooth(dim, N, in, win, out);
   });
 }

@@ -52,7 +52,9 @@ inline bool factorise(int N, std::vector<int>& r) {
     r.push_back(5);
     n /= 5;
   }
-  return n == 1 && !r.empty() && (int)r.size() <= KW_MAX_STAGES;
+  // N = 1 (the y axis of a 1-D problem): no stages, the transform is the
+  // identity
+  return n == 1 && (N == 1 || !r.empty()) && (int)r.size() <= KW_MAX_STAGES;
 }
 
 // wavenumber of ifftshifted index i (kWaveGrid.m:779-797 then ifftshift):

@@ -1,11 +1,13 @@
 """kwhip: MI355X-native k-space pseudospectral time stepping with the call
-signature of k-Wave's kspaceFirstOrder2D / kspaceFirstOrder3D."""
+signature of k-Wave's kspaceFirstOrder1D / kspaceFirstOrder2D /
+kspaceFirstOrder3D."""
 
 from ._abi import KWError, load
 from .engine import Engine, Ticket
 from .kgrid import kWaveGrid
 from .sim import Simulation
-from .solver import cart2grid, kspaceFirstOrder2D, kspaceFirstOrder3D, prepare, smooth
+from .solver import (cart2grid, kspaceFirstOrder1D, kspaceFirstOrder2D, kspaceFirstOrder3D, prepare,
+                     smooth)
 
-__all__ = ["kWaveGrid", "kspaceFirstOrder2D", "kspaceFirstOrder3D", "Simulation", "Engine",
-           "Ticket", "prepare", "smooth", "cart2grid", "KWError", "load"]
+__all__ = ["kWaveGrid", "kspaceFirstOrder1D", "kspaceFirstOrder2D", "kspaceFirstOrder3D",
+           "Simulation", "Engine", "Ticket", "prepare", "smooth", "cart2grid", "KWError", "load"]

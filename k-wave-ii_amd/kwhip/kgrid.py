@@ -25,21 +25,25 @@ def _make_dim(N: int, d: float) -> np.ndarray:
 
 
 class kWaveGrid:  # noqa: N801 - reference class name
-    """kWaveGrid(Nx, dx, Ny, dy) or kWaveGrid(Nx, dx, Ny, dy, Nz, dz)."""
+    """kWaveGrid(Nx, dx), kWaveGrid(Nx, dx, Ny, dy) or
+    kWaveGrid(Nx, dx, Ny, dy, Nz, dz).  1-D grid fields are arrays of shape
+    (Nx,)."""
 
     def __init__(self, Nx, dx, Ny=None, dy=None, Nz=None, dz=None):
-        if Ny is None or dy is None:
-            raise ValueError("only 2D and 3D grids are supported (kWaveGrid(Nx, dx, Ny, dy[, Nz, dz]))")
         self.Nx, self.dx = int(Nx), float(dx)
-        self.Ny, self.dy = int(Ny), float(dy)
-        if Nz is not None:
+        if Ny is None or dy is None:
+            self.Ny, self.dy = 1, 0.0
+            self.dim = 1
+        else:
+            self.Ny, self.dy = int(Ny), float(dy)
+            self.dim = 2
+        if Nz is not None and self.dim == 2:
             self.Nz, self.dz = int(Nz), float(dz)
             self.dim = 3
         else:
             self.Nz, self.dz = 1, 0.0
-            self.dim = 2
         self.kx_vec = _make_dim(self.Nx, self.dx)
-        self.ky_vec = _make_dim(self.Ny, self.dy)
+        self.ky_vec = _make_dim(self.Ny, self.dy) if self.dim >= 2 else np.zeros(1)
         self.kz_vec = _make_dim(self.Nz, self.dz) if self.dim == 3 else np.zeros(1)
         self.Nt = "auto"
         self.dt = "auto"
@@ -47,7 +51,7 @@ class kWaveGrid:  # noqa: N801 - reference class name
     # -- sizes --------------------------------------------------------------
     @property
     def shape(self):
-        return (self.Nx, self.Ny) if self.dim == 2 else (self.Nx, self.Ny, self.Nz)
+        return ((self.Nx,), (self.Nx, self.Ny), (self.Nx, self.Ny, self.Nz))[self.dim - 1]
 
     @property
     def total_grid_points(self):
@@ -55,7 +59,7 @@ class kWaveGrid:  # noqa: N801 - reference class name
 
     @property
     def spacings(self):
-        return (self.dx, self.dy) if self.dim == 2 else (self.dx, self.dy, self.dz)
+        return (self.dx, self.dy, self.dz)[:self.dim]
 
     @property
     def x_size(self):
@@ -83,7 +87,7 @@ class kWaveGrid:  # noqa: N801 - reference class name
 
     @property
     def k_max(self):
-        vecs = [self.kx_vec, self.ky_vec] + ([self.kz_vec] if self.dim == 3 else [])
+        vecs = [self.kx_vec, self.ky_vec, self.kz_vec][:self.dim]
         return min(float(np.max(np.abs(v))) for v in vecs)
 
     # -- time ---------------------------------------------------------------

@@ -1,10 +1,11 @@
-"""Drop-in kspaceFirstOrder2D / kspaceFirstOrder3D over the kwhip C ABI.
+"""Drop-in kspaceFirstOrder1D / 2D / 3D over the kwhip C ABI.
 
 Mirrors the reference call signature
 
     sensor_data = kspaceFirstOrder2D(kgrid, medium, source, sensor, 'Name', value, ...)
 
-(+kwave/+legacy/kspaceFirstOrder2D.m:1, kspaceFirstOrder3D.m:1).  ``medium``,
+(+kwave/+legacy/kspaceFirstOrder1D.m:1, kspaceFirstOrder2D.m:1,
+kspaceFirstOrder3D.m:1).  ``medium``,
 ``source`` and ``sensor`` are dicts (or attribute objects) with the
 reference's field names; name-value options are given as keyword arguments
 or as trailing ('Name', value) pairs.
@@ -99,6 +100,9 @@ def blackman_window(grid_size) -> np.ndarray:
     N = size + (~symmetric).astype(np.int64)
     L = int(N.max())
     lin = _blackman(L)
+    if len(size) == 1:
+        # 1-D: no rotation, the window trimmed to the grid (getWin.m:240-256)
+        return np.abs(lin[:int(size[0])])
     radius = (L - 1) / 2.0
     ref = np.linspace(-radius, radius, L)
     axes = [np.linspace(-radius, radius, int(n)) for n in N]
@@ -119,8 +123,8 @@ def smooth(mat, restore_max=False, backend="hip"):
     k-space (on the device), optionally restoring the maximum magnitude."""
     lib = _abi.load(backend)
     mat = np.asarray(mat, dtype=np.float64)
-    if mat.ndim not in (2, 3):
-        raise ValueError("smooth supports 2D and 3D grids")
+    if mat.ndim not in (1, 2, 3):
+        raise ValueError("smooth supports 1D, 2D and 3D grids")
     win = blackman_window(mat.shape)
     N = (C.c_int64 * 3)(*(list(mat.shape) + [1] * (3 - mat.ndim)))
     src = _abi.fortran_flat(mat)
@@ -158,6 +162,53 @@ def cart2grid(kgrid: kWaveGrid, cart_data):
     reorder_index = flat[flat != 0]
     order_index = np.argsort(reorder_index, kind="stable") + 1
     return (grid != 0), order_index, reorder_index
+
+
+def cuboid_points(kgrid: kWaveGrid, corners):
+    """Sensor points of a cuboid-corners mask: columns [x1, (y1, (z1,)) x2,
+    (y2, (z2))], 1-based and inclusive; the points of each cuboid in
+    column-major order, cuboid after cuboid (overlaps repeat points)
+    (kspaceFirstOrder_inputChecking.m:519-576 checks, :1610-1639 indices).
+    Returns the 0-based indices and the integer corners."""
+    cor = np.asarray(corners, dtype=np.float64)
+    dim, shape = kgrid.dim, kgrid.shape
+    if not np.all(np.mod(cor, 1) == 0):
+        raise ValueError("sensor.mask cuboid corner indices must be integers.")
+    cor = cor.astype(np.int64)
+    if np.any(cor[dim:] < cor[:dim]):
+        raise ValueError("sensor.mask cuboid corners must be defined as [x1, ..., x2, ...].' "
+                         "where x2 => x1, etc.")
+    top = np.array(shape, dtype=np.int64)[:, None]
+    if np.any(cor < 1) or np.any(cor[:dim] > top) or np.any(cor[dim:] > top):
+        raise ValueError("sensor.mask cuboid corners must be within the grid.")
+    strides = np.cumprod([1] + list(shape[:-1])).astype(np.int64)
+    parts = []
+    for q in range(cor.shape[1]):
+        # column-major enumeration of the box: x fastest
+        axes = [np.arange(cor[a, q] - 1, cor[a + dim, q]) * strides[a] for a in range(dim)]
+        grids = np.meshgrid(*axes, indexing="ij")
+        parts.append(sum(grids).ravel(order="F"))
+    return np.concatenate(parts).astype(np.int64), cor
+
+
+def _split_cuboids(sd: dict, corners, dim):
+    """kspaceFirstOrder_reorderCuboidCorners.m: one record per cuboid, the
+    sensor rows reshaped (column-major) to the cuboid, time last; the
+    whole-grid outputs in the first record."""
+    out = []
+    start = 0
+    for q in range(corners.shape[1]):
+        size = tuple(int(corners[a + dim, q] - corners[a, q] + 1) for a in range(dim))
+        n = int(np.prod(size))
+        d = {}
+        for k, v in sd.items():
+            if not (k.endswith("_all") or k.endswith("_final")):
+                v = np.asarray(v)
+                d[k] = v[start:start + n].reshape(size + v.shape[1:], order="F")
+        out.append(d)
+        start += n
+    out[0].update({k: v for k, v in sd.items() if k.endswith("_all") or k.endswith("_final")})
+    return out
 
 
 def grid_data_linear(kgrid: kWaveGrid, cart_data):
@@ -253,7 +304,7 @@ def _pml_outside(kgrid, medium, source, sensor, dim, opts):
     masks and p0 by zeros.  Returns the expanded inputs, the options for
     them, the slices of the user's grid in the expanded one (the _all and
     _final outputs are cut back to it) and the Cartesian reorder index."""
-    pml_size = opts.get("PMLSize", 20 if dim == 2 else 10)
+    pml_size = opts.get("PMLSize", 20 if dim <= 2 else 10)
     sizes = [int(v) for v in np.broadcast_to(np.asarray(pml_size, dtype=np.int64), (dim,))]
     backend = opts.get("Backend", "hip")
     if isinstance(kgrid.Nt, str) or isinstance(kgrid.dt, str):
@@ -296,6 +347,16 @@ def _pml_outside(kgrid, medium, source, sensor, dim, opts):
     if sensor is not None and _has(sensor, "mask"):
         sens = dict(sensor)
         mask = np.asarray(sens["mask"])
+        if mask.shape != kgrid.shape and mask.ndim == 2 and mask.shape[0] == 2 * dim:
+            # cuboid corners, checked on the user's grid and shifted by the
+            # PML (kspaceFirstOrder_expandGridMatrices.m:99-112)
+            cuboid_points(kgrid, mask)
+            cor = np.asarray(mask, dtype=np.float64).copy()
+            for a in range(dim):
+                cor[[a, a + dim]] += sizes[a]
+            sens["mask"] = cor
+            inside = tuple(slice(s, s + n) for s, n in zip(sizes, kgrid.shape))
+            return kg, med, src, sens, opts, inside, None
         if mask.shape != kgrid.shape and mask.ndim == 2 and mask.shape[0] == dim:
             cart = mask
             mask, order_index, reorder_index = cart2grid(kgrid, mask)
@@ -379,7 +440,7 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
     Nt, dt = int(kgrid.Nt), float(kgrid.dt)
 
     # -- options --------------------------------------------------------------
-    pml_size = opts.get("PMLSize", 20 if dim == 2 else 10)
+    pml_size = opts.get("PMLSize", 20 if dim <= 2 else 10)
     pml_alpha = opts.get("PMLAlpha", 2.0)
     pml_size = np.broadcast_to(np.asarray(pml_size, dtype=np.int64), (dim,))
     pml_alpha = np.broadcast_to(np.asarray(pml_alpha, dtype=np.float64), (dim,))
@@ -396,6 +457,7 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
     info.record_given = False
     info.reorder_index = None
     info.interp = None
+    info.cuboids = None
     info.inside = None  # user's grid inside the expanded one ('PMLInside', false)
     info.time_rev = False
     info.use_sensor = sensor is not None and _has(sensor, "mask")
@@ -421,6 +483,15 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
                 n_sens = len(sens_idx)
             if info.time_rev:
                 tr_data = np.asarray(_get(sensor, "time_reversal_boundary_data"), dtype=np.float64)
+        elif mask.ndim == 2 and mask.shape[0] == 2 * dim:
+            # cuboid corners (kspaceFirstOrder_inputChecking.m:516-576)
+            if info.time_rev:
+                raise NotImplementedError("time reversal with a cuboid-corners sensor mask")
+            if "u_split_field" in record:
+                raise ValueError("The option sensor.record = {'u_split_field'} is only "
+                                 "compatible with a binary sensor mask.")
+            sens_idx, info.cuboids = cuboid_points(kgrid, mask)
+            n_sens = len(sens_idx)
         elif mask.ndim == 2 and mask.shape[0] == dim:
             if "u_split_field" in record:
                 # kspaceFirstOrder_inputChecking.m:1363-1366
@@ -764,6 +835,10 @@ def shape_outputs(pp: PreparedProblem, arrays: dict):
         return None
     if info.time_rev:
         return sd["p_final"]
+    if info.cuboids is not None:
+        # a record per cuboid, also without sensor.record
+        # (kspaceFirstOrder2D.m:1109-1130)
+        return _split_cuboids(sd, info.cuboids, info.dim)
     if not info.record_given:
         return sd["p"]
     return sd
@@ -824,6 +899,34 @@ def _solve(dim, kgrid, medium, source, sensor, args, kwargs):
     opts = _parse_options(args, kwargs)
     pp = prepare(kgrid, medium, source, sensor, dim, opts)
     return run_prepared(pp)
+
+
+def _column_to_1d(obj, names, n):
+    """1-D inputs given as MATLAB column vectors (Nx x 1) become (Nx,)."""
+    if obj is None:
+        return obj
+    out = dict(obj) if isinstance(obj, dict) else dict(vars(obj))
+    for nm in names:
+        if nm in out and out[nm] is not None:
+            v = np.asarray(out[nm])
+            if v.ndim == 2 and v.shape == (n, 1):
+                out[nm] = v[:, 0]
+    return out
+
+
+def kspaceFirstOrder1D(kgrid, medium, source, sensor, *args, **kwargs):  # noqa: N802
+    """1D time-domain simulation (reference +kwave/+legacy/kspaceFirstOrder1D.m).
+    Fields are arrays of shape (Nx,) (Nx x 1 columns are accepted); a
+    Cartesian sensor mask is a 1 x N array of x positions."""
+    n = kgrid.Nx
+    medium = _column_to_1d(medium, ("sound_speed", "density", "alpha_coeff", "BonA"), n)
+    source = _column_to_1d(source, ("p0", "p_mask", "u_mask"), n)
+    sensor = _column_to_1d(sensor, ("mask",), n)
+    if source is not None and any(nm in source for nm in ("uy", "uz")):
+        raise ValueError("a 1D simulation takes the velocity source ux only")
+    if sensor is not None and "u_split_field" in list(sensor.get("record", [])):
+        raise ValueError("sensor.record 'u_split_field' needs a 2D or 3D simulation")
+    return _solve(1, kgrid, medium, source, sensor, args, kwargs)
 
 
 def kspaceFirstOrder2D(kgrid, medium, source, sensor, *args, **kwargs):  # noqa: N802

@@ -86,25 +86,24 @@ def make_dim(N: int, d: float) -> np.ndarray:
 
 
 class KGrid:
-    """Uniform 2D/3D grid.  L/kWaveGrid.m:342-436 (constructor) and
-    :494-530 (x_vec, t_array), :612-673 (makeTime)."""
+    """Uniform 1D/2D/3D grid.  L/kWaveGrid.m:342-436 (constructor) and
+    :494-530 (x_vec, t_array), :612-673 (makeTime).  1-D fields are arrays
+    of shape (Nx,)."""
 
     def __init__(self, Nx, dx, Ny=None, dy=None, Nz=None, dz=None):
         self.Nx, self.dx = int(Nx), float(dx)
         self.Ny, self.dy = (int(Ny), float(dy)) if Ny is not None else (1, 0.0)
         self.Nz, self.dz = (int(Nz), float(dz)) if Nz is not None else (1, 0.0)
         self.dim = 1 + (Ny is not None) + (Nz is not None)
-        if self.dim not in (2, 3):
-            raise ValueError("only 2D and 3D grids are supported")
         self.kx_vec = make_dim(self.Nx, self.dx)
-        self.ky_vec = make_dim(self.Ny, self.dy)
+        self.ky_vec = make_dim(self.Ny, self.dy) if self.dim >= 2 else np.zeros(1)
         self.kz_vec = make_dim(self.Nz, self.dz) if self.dim == 3 else np.zeros(1)
         self.Nt = None
         self.dt = None
 
     @property
     def shape(self):
-        return (self.Nx, self.Ny) if self.dim == 2 else (self.Nx, self.Ny, self.Nz)
+        return ((self.Nx,), (self.Nx, self.Ny), (self.Nx, self.Ny, self.Nz))[self.dim - 1]
 
     @property
     def N(self):
@@ -112,10 +111,10 @@ class KGrid:
 
     @property
     def spacings(self):
-        return (self.dx, self.dy) if self.dim == 2 else (self.dx, self.dy, self.dz)
+        return (self.dx, self.dy, self.dz)[:self.dim]
 
     def k_vecs(self):
-        return [self.kx_vec, self.ky_vec] + ([self.kz_vec] if self.dim == 3 else [])
+        return [self.kx_vec, self.ky_vec, self.kz_vec][:self.dim]
 
     def k(self) -> np.ndarray:
         """|k| in centred order, accumulated as in :399-403 / :417-422."""
@@ -220,7 +219,8 @@ def blackman_1d(N, param=0.16):
 
 
 def get_win_rotated(grid_size, symmetric):
-    """getWin.m:180-347 for type 'Blackman', 'Rotation' true, 2D/3D.
+    """getWin.m:180-347 for type 'Blackman', 'Rotation' true (1-D: the
+    window of :240-256 with no rotation).
 
     N is enlarged by one where the window is not symmetric (:188), the 1-D
     window of length max(N) is rotated radially with linear interpolation
@@ -229,6 +229,9 @@ def get_win_rotated(grid_size, symmetric):
     N = np.array(grid_size) + (~np.asarray(symmetric)).astype(int)
     L = int(N.max())
     win_lin = blackman_1d(L)
+    if len(N) == 1:
+        # 1-D: the window itself, trimmed (:240-256)
+        return win_lin[:int(N[0]) - (0 if symmetric[0] else 1)]
     radius = (L - 1) / 2.0
     ll = np.linspace(-radius, radius, L)
     axes = [np.linspace(-radius, radius, int(n)) for n in N]
@@ -354,6 +357,55 @@ class _Interp:
 
     def __len__(self):
         return self.tri.shape[0]
+
+
+def cuboid_corners_index(kgrid: KGrid, corners):
+    """Sensor points of a cuboid-corners mask, [x1, (y1, (z1,)) x2, (y2,
+    (z2))] per column, 1-based and inclusive
+    (LP/kspaceFirstOrder_inputChecking.m:519-576 checks,
+    :1610-1639 indices): the column-major indices of each cuboid in turn,
+    concatenated (overlapping cuboids repeat points).  0-based."""
+    cor = np.asarray(corners, dtype=np.float64)
+    dim = kgrid.dim
+    if not np.all(np.mod(cor, 1) == 0):
+        raise ValueError("sensor.mask cuboid corner indices must be integers.")
+    cor = cor.astype(np.int64)
+    if np.any(cor[dim:] - cor[:dim] < 0):
+        raise ValueError("sensor.mask cuboid corners must be defined as [x1, ..., x2, ...].' "
+                         "where x2 => x1, etc.")
+    if np.any(cor < 1) or any(np.any(cor[[a, a + dim]] > kgrid.shape[a]) for a in range(dim)):
+        raise ValueError("sensor.mask cuboid corners must be within the grid.")
+    idx = []
+    for q in range(cor.shape[1]):
+        m = np.zeros(kgrid.shape, dtype=bool)
+        m[tuple(slice(cor[a, q] - 1, cor[a + dim, q]) for a in range(dim))] = True
+        idx.append(find_F(m))
+    return np.concatenate(idx), cor
+
+
+def reorder_cuboid_corners(sd, corners, dim):
+    """LP/kspaceFirstOrder_reorderCuboidCorners.m: the sensor rows split per
+    cuboid and reshaped (column-major) to the cuboid's size, time last; the
+    whole-grid outputs go to the first cuboid.  Returns a list of dicts, one
+    per cuboid (the reference's struct array)."""
+    out = []
+    start = 0
+    for q in range(corners.shape[1]):
+        size = tuple(int(corners[a + dim, q] - corners[a, q] + 1) for a in range(dim))
+        n = int(np.prod(size))
+        d = {}
+        for k, v in sd.items():
+            if k.endswith("_all") or k.endswith("_final"):
+                continue
+            v = np.asarray(v)
+            part = v[start:start + n]
+            d[k] = part.reshape(size + v.shape[1:], order="F")
+        out.append(d)
+        start += n
+    for k, v in sd.items():
+        if k.endswith("_all") or k.endswith("_final"):
+            out[0][k] = v
+    return out
 
 
 def reorder_binary_sensor_data(data, reorder_index):
@@ -524,7 +576,7 @@ def _pml_outside(kgrid, medium, source, sensor, *, smooth, cart_interp, **kw):
     dim = kgrid.dim
     pml_size = kw.get("pml_size")
     if pml_size is None:
-        pml_size = 20 if dim == 2 else 10
+        pml_size = 20 if dim <= 2 else 10
     sizes = [int(v) for v in np.broadcast_to(np.asarray(pml_size, dtype=np.int64), (dim,))]
     if kgrid.dt is None:
         kgrid.make_time(np.asarray(_get(medium, "sound_speed"), dtype=np.float64), 0.3)
@@ -557,6 +609,17 @@ def _pml_outside(kgrid, medium, source, sensor, *, smooth, cart_interp, **kw):
     if sensor is not None and _has(sensor, "mask"):
         sens = dict(sensor) if isinstance(sensor, dict) else dict(vars(sensor))
         mask = np.asarray(sens["mask"])
+        if mask.shape != kgrid.shape and mask.ndim == 2 and mask.shape[0] == 2 * dim:
+            # cuboid corners: checked on the user's grid, then shifted by the
+            # PML (LP/kspaceFirstOrder_expandGridMatrices.m:99-112)
+            cuboid_corners_index(kgrid, mask)
+            shifted = np.asarray(mask, dtype=np.float64).copy()
+            for a in range(dim):
+                shifted[[a, a + dim]] += sizes[a]
+            sens["mask"] = shifted
+            out = kspace_first_order(kg, med, src, sens, pml_inside=True,
+                                     smooth=(False, smooth_c0, smooth_rho0), cart_interp=cart_interp, **kw)
+            return _cut_back(out, sizes, kgrid)
         if mask.shape != kgrid.shape:
             time_rev = "time_reversal_boundary_data" in sens
             mask, order_index, reorder_index = cart2grid(kgrid, mask)
@@ -585,6 +648,12 @@ def _cut_back(out, sizes, kgrid, reorder_index=None, time_rev=False):
     inside = tuple(slice(s, s + n) for s, n in zip(sizes, kgrid.shape))
     if out is None:
         return None
+    if isinstance(out, list):
+        # cuboid corners: the grids sit in the first cuboid's record
+        for k in list(out[0]):
+            if k.endswith("_all") or k.endswith("_final"):
+                out[0][k] = out[0][k][inside]
+        return out
     if not isinstance(out, dict):
         if time_rev:
             return out[inside]  # p_final of a time reversal
@@ -692,10 +761,18 @@ def kspace_first_order(kgrid: KGrid, medium, source, sensor, *, data_cast="off",
     reorder_index = None
     tr_data = None
     interp = None
+    cuboids = None
     if use_sensor:
         mask = np.asarray(_get(sensor, "mask"))
         if mask.shape == shape:
             sensor_mask = mask
+        elif mask.ndim == 2 and mask.shape[0] == 2 * dim:
+            if time_rev:
+                raise NotImplementedError("time reversal with a cuboid-corners sensor mask")
+            if "u_split_field" in record:
+                raise ValueError("The option sensor.record = {'u_split_field'} is only "
+                                 "compatible with a binary sensor mask.")
+            cub_idx, cuboids = cuboid_corners_index(kgrid, mask)
         elif mask.ndim == 2 and mask.shape[0] == dim:
             if "u_split_field" in record:
                 # LP/kspaceFirstOrder_inputChecking.m:1363-1366
@@ -716,7 +793,12 @@ def kspace_first_order(kgrid: KGrid, medium, source, sensor, *, data_cast="off",
             raise ValueError("sensor.mask must be a binary grid or a dim x N Cartesian array")
         if time_rev and tr_data is None:
             tr_data = np.asarray(_get(sensor, "time_reversal_boundary_data"), dtype=np.float64)
-        sensor_idx = _Interp(*interp, rdt) if interp is not None else _Idx(find_F(sensor_mask), shape)
+        if interp is not None:
+            sensor_idx = _Interp(*interp, rdt)
+        elif cuboids is not None:
+            sensor_idx = _Idx(cub_idx, shape)
+        else:
+            sensor_idx = _Idx(find_F(sensor_mask), shape)
     if time_rev:
         # fliplr (kspaceFirstOrder3D.m:721-728)
         tr_data = tr_data[:, ::-1]
@@ -781,7 +863,7 @@ def kspace_first_order(kgrid: KGrid, medium, source, sensor, *, data_cast="off",
 
     # -- PML settings (setDefaults.m:151-170) --------------------------------
     if pml_size is None:
-        pml_size = 20 if dim == 2 else 10
+        pml_size = 20 if dim <= 2 else 10
     if pml_alpha is None:
         pml_alpha = 2.0
     pml_sizes = list(np.broadcast_to(np.asarray(pml_size), (dim,)).astype(int))
@@ -1046,8 +1128,10 @@ def kspace_first_order(kgrid: KGrid, medium, source, sensor, *, data_cast="off",
             p_k = fftn(p)
             if dim == 3:
                 rmod = p / (rdt(3) * c0 ** 2)
-            else:
+            elif dim == 2:
                 rmod = rdt(0.5) * p / (c0 ** 2)
+            else:
+                rmod = p / (c0 ** 2)   # kspaceFirstOrder1D.m:607
             rmod = np.broadcast_to(rmod, shape)
             for a in range(dim):
                 sensor_idx.set(rhos[a], sensor_idx.get(rmod))
@@ -1139,6 +1223,10 @@ def kspace_first_order(kgrid: KGrid, medium, source, sensor, *, data_cast="off",
         return None
     if time_rev:
         return sd["p_final"]
+    if cuboids is not None:
+        # kspaceFirstOrder2D.m:1109-1130: a struct array even without
+        # sensor.record
+        return reorder_cuboid_corners(sd, cuboids, dim)
     if not record_given:
         return sd["p"]
     return sd

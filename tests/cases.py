@@ -54,10 +54,10 @@ def case(name, dim=2, N=None):
     source, sensor, opts."""
     rng = np.random.default_rng(zlib.crc32(name.encode()))
     if N is None:
-        N = (32, 30) if dim == 2 else (16, 18, 20)
+        N = {1: (96,), 2: (32, 30), 3: (16, 18, 20)}[dim]
         if "pml_outside" in name:
             # the grid grows by the PML on both sides: FFT-friendly after it
-            N = (24, 40) if dim == 2 else (12, 18, 10)
+            N = {1: (40,), 2: (24, 40), 3: (12, 18, 10)}[dim]
     shape = tuple(N)
     dx = 1e-4
     c0 = 1500.0
@@ -65,7 +65,7 @@ def case(name, dim=2, N=None):
     Nt = 24
     med = {"sound_speed": c0, "density": 1000.0}
     src = {}
-    pml = 4 if dim == 2 else 3
+    pml = 4 if dim <= 2 else 3
     opts = {"PMLSize": pml, "PMLAlpha": 2.0, "Smooth": False}
     centre = [n // 2 + 1 for n in shape]
     sens = {"mask": np.ones(shape), "record": ["p", "p_final", "p_max", "p_min", "p_rms"]}
@@ -104,7 +104,9 @@ def case(name, dim=2, N=None):
         c, rho = hetero_medium(shape, 11)
         med = {"sound_speed": c, "density": rho}
         m = np.zeros(shape)
-        sl = tuple([slice(pml + 2, pml + 3)] + [slice(pml + 2, n - pml - 2) for n in shape[1:]])
+        # a plane of points (a few points in 1-D)
+        sl = tuple([slice(pml + 2, pml + (3 if dim > 1 else 6))] +
+                   [slice(pml + 2, n - pml - 2) for n in shape[1:]])
         m[sl] = 1
         npts = int(m.sum())
         sig = tone(Nt, dt, 2e6, 3)
@@ -129,16 +131,19 @@ def case(name, dim=2, N=None):
         mode = {"usrc_add": "additive", "usrc_dir": "dirichlet",
                 "usrc_nocorr": "additive-no-correction", "usrc_many": "additive"}[name]
         m = np.zeros(shape)
-        sl = tuple([slice(pml + 3, pml + 4)] + [slice(pml + 2, n - pml - 2) for n in shape[1:]])
+        sl = tuple([slice(pml + 3, pml + (4 if dim > 1 else 7))] +
+                   [slice(pml + 2, n - pml - 2) for n in shape[1:]])
         m[sl] = 1
         npts = int(m.sum())
         sig = tone(Nt, dt, 2e6, 2, amp=1e-3)
         if name == "usrc_many":
             src["ux"] = np.stack([sig * (1 + 0.05 * i) for i in range(npts)])
-            src["uy"] = np.stack([0.3 * sig * (1 - 0.05 * i) for i in range(npts)])
+            if dim >= 2:
+                src["uy"] = np.stack([0.3 * sig * (1 - 0.05 * i) for i in range(npts)])
         else:
             src["ux"] = sig
-            src["uy"] = 0.5 * sig
+            if dim >= 2:
+                src["uy"] = 0.5 * sig
         if dim == 3:
             src["uz"] = -0.25 * (src["ux"])
         src["u_mask"] = m
@@ -221,7 +226,7 @@ def case(name, dim=2, N=None):
         npt = 11
         ang = np.arange(npt) * 2 * math.pi / npt + 0.3
         r = (min(shape) // 2 - pml - 1.37) * dx
-        pts = [r * np.cos(ang), -r * np.sin(ang)]
+        pts = [r * np.cos(ang), -r * np.sin(ang)][:dim]
         if dim == 3:
             pts.append(0.43 * dx * np.sin(2 * ang))
         pts = np.stack(pts)
@@ -233,12 +238,27 @@ def case(name, dim=2, N=None):
             rec = ["p", "p_max", "u", "p_final"]
         sens = {"mask": pts, "record": rec}
         opts["CartInterp"] = "linear"
+    elif name in ("cuboid_corners", "cuboid_corners_pml_outside"):
+        # two overlapping cuboids of sensor points (1-based, inclusive)
+        c, rho = hetero_medium(shape, 29)
+        med = {"sound_speed": c, "density": rho}
+        src["p0"] = disc(shape, centre, 3)
+        lo1 = [n // 2 - 4 for n in shape]
+        hi1 = [n // 2 + 1 for n in shape]
+        lo2 = [n // 2 - 1 for n in shape]
+        hi2 = [min(n, n // 2 + 5) for n in shape]
+        corners = np.array([lo1 + hi1, lo2 + hi2], dtype=float).T
+        rec = ["p", "p_max", "p_rms", "u", "u_non_staggered", "p_final", "u_max_all", "I_avg"]
+        if name == "cuboid_corners_pml_outside":
+            opts["PMLInside"] = False
+            rec = ["p", "u_max", "p_max_all"]
+        sens = {"mask": corners, "record": rec}
     elif name == "cartesian_nearest":
         src["p0"] = disc(shape, centre, 3)
         npt = 12
         ang = np.arange(npt) * 2 * math.pi / npt + math.pi / 2
         r = (min(shape) // 2 - pml - 1) * dx
-        pts = [r * np.cos(ang), -r * np.sin(ang)]
+        pts = [r * np.cos(ang), -r * np.sin(ang)][:dim]
         if dim == 3:
             pts.append(np.zeros(npt))
         sens = {"mask": np.stack(pts), "record": ["p", "p_max", "p_rms"]}
@@ -252,6 +272,10 @@ def case(name, dim=2, N=None):
 def _sparse_mask(shape, rng, frac=0.1):
     m = (rng.uniform(size=shape) < frac).astype(np.float64)
     m.ravel()[0] = 1
+    if len(shape) == 1:
+        # 1-D: also every other point near the sources, which sit a few
+        # points inside the PML, so the sensors see more than rounding
+        m[6:22:2] = 1
     return m
 
 
@@ -259,8 +283,15 @@ CASES_2D = ["p0_lossless", "p0_hetero_absorbing", "p0_nonlinear_absorbing", "p0_
             "stokes", "no_dispersion", "no_absorption", "psrc_add", "psrc_dir", "psrc_nocorr",
             "psrc_many", "psrc_label", "psrc_many_dir", "usrc_add", "usrc_dir", "usrc_nocorr",
             "usrc_many", "time_reversal", "cartesian_nearest", "intensity", "u_nonstaggered", "u_split",
-            "pml_outside", "pml_outside_source", "cartesian_linear", "cartesian_linear_pml_outside"]
+            "pml_outside", "pml_outside_source", "cartesian_linear", "cartesian_linear_pml_outside",
+            "cuboid_corners", "cuboid_corners_pml_outside"]
 CASES_3D = CASES_2D
+# kspaceFirstOrder1D: every case whose inputs exist in 1-D
+CASES_1D = ["p0_lossless", "p0_hetero_absorbing", "p0_nonlinear_absorbing", "p0_nonlinear_lossless",
+            "stokes", "no_dispersion", "psrc_add", "psrc_dir", "psrc_nocorr", "psrc_many",
+            "psrc_label", "usrc_add", "usrc_dir", "usrc_many", "time_reversal", "cartesian_nearest",
+            "cartesian_linear", "intensity", "u_nonstaggered", "pml_outside", "cartesian_linear_pml_outside",
+            "cuboid_corners", "cuboid_corners_pml_outside"]
 
 
 def oracle_opts(opts):
@@ -291,13 +322,23 @@ def run_kwhip(cs, backend):
         args += [n, dx]
     kg = kwhip.kWaveGrid(*args)
     kg.setTime(cs["Nt"], cs["dt"])
-    fn = kwhip.kspaceFirstOrder2D if cs["dim"] == 2 else kwhip.kspaceFirstOrder3D
+    fn = {1: kwhip.kspaceFirstOrder1D, 2: kwhip.kspaceFirstOrder2D, 3: kwhip.kspaceFirstOrder3D}[cs["dim"]]
     return fn(kg, cs["medium"], cs["source"], cs["sensor"], Backend=backend, **cs["opts"])
 
 
 def compare(got, ref):
     """Relative L-inf per output field; returns {field: err}."""
     from oracle import kwave_oracle as ko
+    if isinstance(ref, list):
+        # cuboid corners: one record per cuboid
+        assert isinstance(got, list) and len(got) == len(ref)
+        errs = {}
+        for q, (g, r) in enumerate(zip(got, ref)):
+            assert set(g) == set(r), (q, sorted(g), sorted(r))
+            for k, e in compare(g, r).items():
+                assert np.shape(g[k]) == np.shape(r[k]), (q, k)
+                errs[f"{q}.{k}"] = e
+        return errs
     if not isinstance(ref, dict):
         got, ref = {"p": got}, {"p": ref}
     errs = {}

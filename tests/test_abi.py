@@ -97,6 +97,13 @@ def test_errors_are_reported_not_raised():
     prob.dt, prob.c_ref, prob.c0_scalar, prob.rho0_scalar = 1e-8, 1500.0, 1500.0, 1000.0
     rc = lib.kw_sim_create(C.byref(prob), C.byref(h), buf, 256)
     assert rc == _abi.KW_EUNSUP
+    prob.dim, prob.Nx = 1, 16     # 1-D: Ny and Nz must be 1
+    rc = lib.kw_sim_create(C.byref(prob), C.byref(h), buf, 256)
+    assert rc == _abi.KW_EINVAL and b"1D" in buf.value
+    prob.Ny = 1
+    prob.u_source[1].flag = 3      # no uy source in 1-D
+    rc = lib.kw_sim_create(C.byref(prob), C.byref(h), buf, 256)
+    assert rc == _abi.KW_EINVAL and b"ux only" in buf.value
 
 
 def test_interpolated_sensor_points_are_checked():

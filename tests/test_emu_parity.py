@@ -32,6 +32,11 @@ def _check(cs):
         assert e64[k] <= TOL64, (k, e64[k])
 
 
+@pytest.mark.parametrize("name", cases.CASES_1D)
+def test_emu_parity_1d(name):
+    _check(cases.case(name, 1))
+
+
 @pytest.mark.parametrize("name", cases.CASES_2D)
 def test_emu_parity_2d(name):
     _check(cases.case(name, 2))
@@ -60,6 +65,10 @@ def test_emu_parity_3d(name):
     (3, "p0_nonlinear_absorbing", (12, 10, 135)),
     (2, "psrc_label", (270, 16)),
     (2, "stokes", (16, 270)),
+    # 1-D grids on fixed-length x kernels
+    (1, "p0_hetero_absorbing", (75,)),
+    (1, "psrc_many", (128,)),
+    (1, "time_reversal", (270,)),
     # the velocity split's last-axis pass on fixed-length kernels
     (2, "u_split", (30, 64)),
     (3, "u_split", (12, 16, 64)),

@@ -26,6 +26,19 @@ def _check(cs, hip_lib):
         assert e64[k] <= TOL64, (k, e64[k])
 
 
+@pytest.mark.parametrize("name", cases.CASES_1D)
+def test_parity_1d(name, hip_lib):
+    _check(cases.case(name, 1), hip_lib)
+
+
+@pytest.mark.parametrize("name,shape", [("p0_hetero_absorbing", (75,)),
+                                        ("psrc_many", (128,)),
+                                        ("time_reversal", (1024,)),
+                                        ("usrc_add", (2000,))])
+def test_parity_1d_sizes(name, shape, hip_lib):
+    _check(cases.case(name, 1, shape), hip_lib)
+
+
 @pytest.mark.parametrize("name", cases.CASES_2D)
 def test_parity_2d(name, hip_lib):
     _check(cases.case(name, 2), hip_lib)

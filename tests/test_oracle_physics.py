@@ -42,7 +42,8 @@ def kmag(shape, spacings):
 # homogeneous lossless: exact solutions
 # ---------------------------------------------------------------------------
 
-@pytest.mark.parametrize("shape,sp", [((24, 20), (1e-4, 1.2e-4)),
+@pytest.mark.parametrize("shape,sp", [((48,), (1e-4,)), ((45,), (1.1e-4,)),
+                                      ((24, 20), (1e-4, 1.2e-4)),
                                       ((15, 16), (1e-4, 1e-4)),
                                       ((12, 10, 9), (1e-4, 0.9e-4, 1.1e-4))])
 def test_periodic_initial_value_is_exact(shape, sp):
@@ -395,13 +396,13 @@ def test_time_reversal_equals_shifted_dirichlet_source():
     assert np.max(np.abs(tr - sd["p_final"])) < 1e-12 * np.max(np.abs(tr))
 
 
-@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("dim", [1, 2, 3])
 def test_additive_source_is_filtered_no_correction_source(dim):
     """An 'additive' p source equals an 'additive-no-correction' source on
     every cell whose signal is the source mask filtered by
     cos(c_ref k dt / 2) in k-space times the time signal; both are scaled by
     2 dt / (dim c dx)."""
-    shape = (16, 14) if dim == 2 else (10, 12, 9)
+    shape = {1: (40,), 2: (16, 14), 3: (10, 12, 9)}[dim]
     sp = (1e-4, 1.1e-4, 0.9e-4)[:dim]
     args = [v for pair in zip(shape, sp) for v in pair]
     rng = np.random.default_rng(9)
@@ -670,3 +671,111 @@ def test_cartesian_linear_at_grid_nodes_and_midpoints(dim):
     for a in range(dim):
         want = 0.5 * (p[-1] + p[dim + a])
         assert np.allclose(p[a], want, rtol=0, atol=1e-12 * np.max(np.abs(p)))
+
+
+@pytest.mark.parametrize("kind", ["p0", "p_no_correction", "p_dirichlet", "ux_additive",
+                                  "time_reversal"])
+def test_1d_equals_y_invariant_2d(kind):
+    """kspaceFirstOrder1D against the 2-D loop (pinned above) on a problem
+    that does not vary along y, with no PML along y: for linear problems,
+    the 2-D densities add up to the 1-D one (the y part holds its initial
+    share and changes only where a p source feeds it), the p sources scaled
+    by 1/dim go into each component, and a sensor row of the 2-D run equals
+    the 1-D sensors.  Heterogeneous c and rho, power-law absorption, PML
+    along x.  (The k-space filtered 'additive' p source also reaches the
+    PML cells, where the split PML damps only the x part of the 2-D density,
+    so it is pinned on its own by the filtered-source test above.)"""
+    Nx, Ny, dx, Nt, dt = 64, 6, 1e-4, 40, 2e-8
+    rng = np.random.default_rng(7)
+    c1 = 1500 + 60 * np.sin(np.arange(Nx) / 6.0)
+    r1 = 1000 + 50 * np.cos(np.arange(Nx) / 4.0)
+    med1 = {"sound_speed": c1, "density": r1, "alpha_coeff": 0.6, "alpha_power": 1.4}
+    med2 = {"sound_speed": np.repeat(c1[:, None], Ny, 1), "density": np.repeat(r1[:, None], Ny, 1),
+            "alpha_coeff": 0.6, "alpha_power": 1.4}
+    sens_x = np.array([20, 31, 40, 47])
+    m1 = np.zeros(Nx)
+    m1[sens_x] = 1
+    m2 = np.zeros((Nx, Ny))
+    m2[sens_x, 0] = 1
+    sig = np.sin(2 * np.pi * 2e6 * np.arange(Nt) * dt) * np.hanning(Nt)
+    src1, src2 = {}, {}
+    if kind == "p0":
+        p0 = np.zeros(Nx)
+        p0[28:36] = rng.uniform(0.5, 1.5, 8)
+        src1["p0"], src2["p0"] = p0, np.repeat(p0[:, None], Ny, 1)
+    elif kind.startswith("p_") or kind == "ux_additive":
+        sm1 = np.zeros(Nx)
+        sm1[30:33] = 1
+        key = "p" if kind.startswith("p_") else "ux"
+        mode = {"p_dirichlet": "dirichlet", "p_no_correction": "additive-no-correction"}.get(
+            kind, "additive")
+        pre = "p" if key == "p" else "u"
+        src1 = {key: sig, pre + "_mask": sm1, pre + "_mode": mode}
+        src2 = {key: sig, pre + "_mask": np.repeat(sm1[:, None], Ny, 1), pre + "_mode": mode}
+    kw = dict(pml_alpha=2.0)
+    if kind == "time_reversal":
+        data = rng.standard_normal((4, Nt))
+        kg1 = ko.KGrid(Nx, dx)
+        kg1.set_time(Nt, dt)
+        kg2 = ko.KGrid(Nx, dx, Ny, dx)
+        kg2.set_time(Nt, dt)
+        got1 = run(kg1, med1, {}, {"mask": m1, "time_reversal_boundary_data": data}, pml_size=8, **kw)
+        # the boundary data on every row (column-major order: row by row)
+        m2all = np.repeat(m1[:, None], Ny, 1)
+        got2 = run(kg2, med2, {}, {"mask": m2all, "time_reversal_boundary_data": np.tile(data, (Ny, 1))},
+                   pml_size=[8, 0], **kw)
+        assert np.allclose(got1, got2[:, 0], rtol=0, atol=1e-10 * np.max(np.abs(got2)))
+        return
+    kg1 = ko.KGrid(Nx, dx)
+    kg1.set_time(Nt, dt)
+    kg2 = ko.KGrid(Nx, dx, Ny, dx)
+    kg2.set_time(Nt, dt)
+    rec = ["p", "u", "p_final"]
+    got1 = run(kg1, med1, src1, {"mask": m1, "record": rec}, pml_size=8, **kw)
+    got2 = run(kg2, med2, src2, {"mask": m2, "record": rec}, pml_size=[8, 0], **kw)
+    for k1, k2 in (("p", "p"), ("ux", "ux"), ("p_final", "p_final")):
+        a, b = np.asarray(got1[k1]), np.asarray(got2[k2])
+        if k1 == "p_final":
+            b = b[:, 0]
+        assert np.allclose(a, b, rtol=0, atol=1e-10 * np.max(np.abs(b))), k1
+    assert np.max(np.abs(got2["uy"])) < 1e-12 * np.max(np.abs(got2["ux"]))
+
+
+@pytest.mark.parametrize("dim", [1, 2, 3])
+def test_cuboid_corners_equal_full_grid_record(dim):
+    """A cuboid-corners sensor returns, per cuboid, the full-grid record at
+    the cuboid's points reshaped to the cuboid (time last); overlapping
+    cuboids both see their shared points; whole-grid outputs go to the
+    first cuboid; invalid corners raise the reference's errors."""
+    shape = {1: (40,), 2: (20, 18), 3: (12, 10, 9)}[dim]
+    args = []
+    for n in shape:
+        args += [n, 1e-4]
+    kg = ko.KGrid(*args)
+    kg.set_time(12, 2e-8)
+    rng = np.random.default_rng(13)
+    p0 = np.zeros(shape)
+    p0[tuple(slice(n // 2 - 2, n // 2 + 2) for n in shape)] = rng.uniform(0.5, 1.5, (4,) * dim)
+    med = {"sound_speed": 1500.0, "density": 1000.0}
+    lo1, hi1 = [3, 4, 2][:dim], [9, 8, 6][:dim]
+    lo2, hi2 = [6, 2, 3][:dim], [12, 5, 7][:dim]
+    corners = np.array([lo1 + hi1, lo2 + hi2], dtype=float).T
+    rec = ["p", "p_max", "u", "p_max_all"]
+    cub = run(kg, med, {"p0": p0}, {"mask": corners, "record": rec}, pml_size=2)
+    full = run(kg, med, {"p0": p0}, {"mask": np.ones(shape), "record": rec}, pml_size=2)
+    assert isinstance(cub, list) and len(cub) == 2
+    for q, (lo, hi) in enumerate(((lo1, hi1), (lo2, hi2))):
+        sl = tuple(slice(a - 1, b) for a, b in zip(lo, hi))
+        for k in ("p", "ux", "p_max"):
+            g = np.asarray(full[k]).reshape(shape + np.asarray(full[k]).shape[1:], order="F")
+            assert np.array_equal(cub[q][k], g[sl]), (q, k)
+    assert np.array_equal(cub[0]["p_max_all"], full["p_max_all"]) and "p_max_all" not in cub[1]
+    with pytest.raises(ValueError, match="integers"):
+        run(kg, med, {"p0": p0}, {"mask": corners + 0.5}, pml_size=2)
+    with pytest.raises(ValueError, match="x2 => x1"):
+        run(kg, med, {"p0": p0}, {"mask": corners[list(range(dim, 2 * dim)) + list(range(dim))]},
+            pml_size=2)
+    bad = corners.copy()
+    bad[dim, 0] = shape[0] + 1
+    with pytest.raises(ValueError, match="within the grid"):
+        run(kg, med, {"p0": p0}, {"mask": bad}, pml_size=2)
