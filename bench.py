@@ -319,8 +319,9 @@ def reduce_traffic(fetch_dir, write_dir, cal_fetch_dir, cal_write_dir, kernel, v
     (KiB per dispatch), each scaled by factor = known bytes / counter bytes of
     the calibration copy; written to profiles/traffic_<kernel>_cfg2_<N>.json,
     which the bench reports as roofline.traffic."""
-    f_kib, nf = _counter_means(fetch_dir, "FETCH_SIZE", variant)
-    w_kib, nw = _counter_means(write_dir, "WRITE_SIZE", variant)
+    # "(" ends the kernel name: the record variant (<variant>_rec) is not it
+    f_kib, nf = _counter_means(fetch_dir, "FETCH_SIZE", variant + "(")
+    w_kib, nw = _counter_means(write_dir, "WRITE_SIZE", variant + "(")
     # the calibration copy runs as the runtime's copy kernel (copyBuffer)
     cf_kib, _ = _counter_means(cal_fetch_dir, "FETCH_SIZE", "copyBuffer")
     cw_kib, _ = _counter_means(cal_write_dir, "WRITE_SIZE", "copyBuffer")

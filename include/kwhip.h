@@ -64,6 +64,10 @@ extern "C" {
  * sensor masks only, kspaceFirstOrder_inputChecking.m:1363-1366) */
 #define KW_REC_U_SPLIT_FIELD (1u << 15)
 
+/* directivity patterns (sensor.directivity_pattern) */
+#define KW_DIR_PRESSURE 0
+#define KW_DIR_GRADIENT 1
+
 /* source modes, numbered as the reference's p_source_mode / u_source_mode
  * (kspaceFirstOrder_saveToDisk.m:225-231,262-268) */
 #define KW_SRC_DIRICHLET 0
@@ -139,6 +143,15 @@ typedef struct kw_problem {
    * reversal or KW_REC_U_SPLIT_FIELD. */
   const uint64_t* sensor_tri;
   const double* sensor_bc;
+  /* 2-D sensor directivity (sensor.directivity_angle, _pattern, _size;
+   * kspaceFirstOrder2D.m:945-947, directionalResponse.m): the angle of each
+   * sensor point in radians [sensor_num_points], or NULL for none.  The
+   * recorded p time series is then, per unique angle, the pressure filtered
+   * in k-space by the pattern (KW_DIR_*) of element size directivity_size;
+   * the statistics stay those of p.  2D binary masks only. */
+  const double* directivity_angle;
+  int32_t directivity_pattern;
+  double directivity_size;
 } kw_problem;
 
 typedef struct kw_sizes {

@@ -39,7 +39,7 @@ using Clock = std::chrono::steady_clock;
 // ---------------------------------------------------------------------------
 struct ProblemCopy {
   kw_problem pb{};
-  std::vector<double> c0, rho0, rho0_sg[3], bona, alpha, p0, tr, bc;
+  std::vector<double> c0, rho0, rho0_sg[3], bona, alpha, p0, tr, bc, dir_angle;
   std::vector<uint64_t> sens, tri;
   struct Src {
     std::vector<uint64_t> index, sig_index;
@@ -76,6 +76,11 @@ struct ProblemCopy {
       bc.assign(in.sensor_bc, in.sensor_bc + m);
       pb.sensor_tri = tri.data();
       pb.sensor_bc = bc.data();
+    }
+    pb.directivity_angle = nullptr;
+    if (in.directivity_angle && in.sensor_num_points > 0) {
+      dir_angle.assign(in.directivity_angle, in.directivity_angle + in.sensor_num_points);
+      pb.directivity_angle = dir_angle.data();
     }
     pb.time_reversal_boundary_data = nullptr;
     if (in.time_reversal_boundary_data && in.sensor_num_points > 0) {
@@ -439,6 +444,11 @@ void group_step(Group& g) {
       record_velocity(*s);
       g.launches += (g.dim == 3 ? 6 : 4) * g.dim;
       if (s->record & KW_REC_U_SPLIT_FIELD) g.launches += g.dim == 3 ? 9 : 5;
+    }
+  for (Sim* s : g.sims)
+    if (!s->dir_tab.empty()) {
+      record_directivity(*s);
+      g.launches += 4 * (int64_t)s->dir_tab.size();
     }
   EwArgs adv{};
   adv.kind = EW_ADVANCE;

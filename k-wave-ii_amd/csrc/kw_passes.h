@@ -924,6 +924,7 @@ enum EwKind : int32_t {
   EW_FILL,           // zero a float buffer (graph-safe memset)
   EW_RECORD_UNS,     // non-staggered velocity component `which` at the sensor points
   EW_RECORD_SPLIT,   // compressional / shear parts of component `which`
+  EW_RECORD_DIR,     // directional p of the sensor points of angle `which`
 };
 
 // EwArgs.rec value for "the problem's own statistics" (launches shared by
@@ -1080,6 +1081,16 @@ KW_BODY void body_ew(Ctx& c, const Batch& B, const EwArgs& A) {
       KW_ST(P.usplit_p[A.which] + o, up);
       KW_ST(P.usplit_s[A.which] + o, un - up);
       if (P.uns_ts[A.which] != nullptr) KW_ST(P.uns_ts[A.which] + o, un);
+      return;
+    }
+    case EW_RECORD_DIR: {
+      // kspaceFirstOrder2D.m:945-947: the p time series of the points whose
+      // angle is number `which`, from P.dirp (p filtered for that angle)
+      if (i >= P.nsens || t < P.record_start || P.dir_id == nullptr) return;
+      if (KW_LD(P.dir_id + i) != A.which) return;
+      const int fi = t - P.record_start + 1;
+      const int64_t cell = P.sens_idx ? (int64_t)KW_LD(P.sens_idx + i) : i;
+      KW_ST(P.p_ts + i + (int64_t)(fi - 1) * P.nsens, KW_LD(P.dirp + cell));
       return;
     }
     case EW_ADVANCE: {

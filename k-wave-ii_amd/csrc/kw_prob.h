@@ -132,6 +132,10 @@ struct ProbDev {
   // nullptr for binary masks
   const int32_t* sens_tri;
   const float* sens_bc;
+  // 2-D directivity: the unique-angle number of each sensor point and the
+  // pressure filtered for one angle (scratch grid)
+  const int32_t* dir_id;
+  float* dirp;
   float* p_ts;           // [nsens x n_rec]
   double *p_max, *p_min, *p_rms2;  // [nsens]; p_rms2 accumulates p^2
   double *p_max_all, *p_min_all;   // [ncell]

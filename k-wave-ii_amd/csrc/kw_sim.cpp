@@ -186,6 +186,14 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream) {
   P.nsens = (int32_t)S->nsens;
   P.time_rev = S->time_rev ? 1 : 0;
   const bool interp = pr->sensor_tri != nullptr || pr->sensor_bc != nullptr;
+  // 2-D sensor directivity (ignored by time reversal, as the reference does)
+  const bool directivity = pr->directivity_angle != nullptr && S->nsens > 0 && !S->time_rev;
+  if (directivity) {
+    if (pdim != 2) fail(KW_EINVAL, "sensor directivity is only supported in 2D");
+    if (interp) fail(KW_EINVAL, "sensor directivity needs a binary sensor mask");
+    if (pr->directivity_pattern != KW_DIR_PRESSURE && pr->directivity_pattern != KW_DIR_GRADIENT)
+      fail(KW_EINVAL, "Unsupported directivity pattern.");
+  }
   if (interp) {
     if (!pr->sensor_tri || !pr->sensor_bc)
       fail(KW_EINVAL, "sensor_tri and sensor_bc must be given together");
@@ -210,6 +218,9 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream) {
         fail(KW_EINVAL, "sensor_mask_index is NULL but num_points is not the grid size");
       S->full_mask = true;
     }
+    // the directional p is gathered per angle, so the p record of a full
+    // mask is not fused into the p producer
+    if (directivity) S->full_mask = false;
   }
   P.full_mask = S->full_mask ? 1 : 0;
   {
@@ -220,8 +231,10 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream) {
     const bool sparse_p = S->nsens > 0 && !S->full_mask && !S->time_rev && (S->record & pts_p);
     P.p_store_from = sparse_p ? (int32_t)S->record_start : INT32_MAX;
     P.p_store_at = (S->record & KW_REC_P_FINAL) ? (int32_t)S->Nt_eff : -1;
-    P.rec_points =
-        S->time_rev ? 0u : (S->record & (pts_p | pts_u)) & ~(S->full_mask ? pts_p : 0u);
+    P.rec_points = S->time_rev ? 0u
+                               : (S->record & (pts_p | pts_u)) & ~(S->full_mask ? pts_p : 0u) &
+                                     ~(directivity ? (uint32_t)KW_REC_P : 0u);
+    S->directivity = directivity;
     P.rec_grid = S->time_rev ? 0u : S->record & (KW_REC_U_MAX_ALL | KW_REC_U_MIN_ALL);
   }
 
@@ -421,8 +434,44 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream) {
       P.sens_bc = S->upload(bc);
     } else if (!S->full_mask) {
       std::vector<int32_t> idx(S->nsens);
-      for (int64_t i = 0; i < S->nsens; ++i) idx[i] = (int32_t)(pr->sensor_mask_index[i] - 1);
+      for (int64_t i = 0; i < S->nsens; ++i)
+        idx[i] = pr->sensor_mask_index ? (int32_t)(pr->sensor_mask_index[i] - 1) : (int32_t)i;
       P.sens_idx = S->upload(idx);
+    }
+    if (directivity && (S->record & KW_REC_P)) {
+      // directionalResponse.m: one k-space filter per unique angle, built on
+      // the centred wavenumber grids and moved to FFT order with fftshift
+      // (for an odd length, one bin off ifftshift: FFT bin i reads the
+      // centred index (i - floor(N/2)) mod N).  real(ifft2(.)) keeps the
+      // even part of the filter, (D(k) + D(-k))/2, which the half-spectrum
+      // transforms apply exactly; 1/(Nx Ny) of the round trip folded in.
+      std::vector<double> ang(pr->directivity_angle, pr->directivity_angle + S->nsens);
+      std::vector<double> uniq = ang;
+      std::sort(uniq.begin(), uniq.end());
+      uniq.erase(std::unique(uniq.begin(), uniq.end()), uniq.end());
+      std::vector<int32_t> id(S->nsens);
+      for (int64_t i = 0; i < S->nsens; ++i)
+        id[i] = (int32_t)(std::lower_bound(uniq.begin(), uniq.end(), ang[i]) - uniq.begin());
+      P.dir_id = S->upload(id);
+      P.dirp = S->alloc<float>(G.ncell);
+      auto kq = [](int i, int N, double d) {
+        const int j = ((i - N / 2) % N + N) % N;
+        return (2.0 * kPi / (N * d)) * (double)(j - N / 2);
+      };
+      const double size = pr->directivity_size, Ntot = (double)G.ncell;
+      const int pattern = pr->directivity_pattern;
+      for (double th : uniq) {
+        auto D = [&](int ix, int iy) {
+          const double kx = kq(ix, G.Nx, pr->dx), ky = kq(iy, G.Ny, pr->dy);
+          if (pattern == KW_DIR_PRESSURE) return sinc(std::fabs(std::cos(th) * ky - std::sin(th) * kx) * size / 2);
+          const double k = std::sqrt(kx * kx + ky * ky);
+          return k == 0.0 ? 0.0 : std::fabs(std::sin(th) * ky + std::cos(th) * kx) / k;
+        };
+        S->dir_tab.push_back(S->spectral_table([&](int ix, int iy, int) {
+          const int mx = (G.Nx - ix) % G.Nx, my = (G.Ny - iy) % G.Ny;
+          return 0.5 * (D(ix, iy) + D(mx, my)) / Ntot;
+        }));
+      }
     }
     const int64_t ns = S->nsens;
     const uint32_t r = S->record;
@@ -625,10 +674,34 @@ void record_velocity(Sim& S) {
   }
 }
 
+// The p time series of a sensor with directivity
+// (kspaceFirstOrder2D.m:945-947, directionalResponse.m): per unique angle,
+// p filtered in k-space through S[1] (free at the end of a step) into
+// P.dirp, then a gather at the points of that angle.
+void record_directivity(Sim& S) {
+  for (size_t a = 0; a < S.dir_tab.size(); ++a) {
+    PassArgs r = bufs(1, 1);
+    r.real[0] = S.P.p;
+    S.pass(PK_X_R2C, r);
+    PassArgs f = r;
+    f.mode = SM_FILTER;
+    f.table = S.dir_tab[a];
+    S.pass(PK_SPEC, f);
+    PassArgs c2r = r;
+    c2r.real[0] = S.P.dirp;
+    S.pass(PK_X_C2R, c2r);
+    EwArgs e{};
+    e.kind = EW_RECORD_DIR;
+    e.which = (int32_t)a;
+    S.ew(e, S.lsens.nb);
+  }
+}
+
 static void record_step(Sim& S, int t, bool init) {
   if (S.time_rev || S.nsens == 0 || t < S.record_start) return;
   uint32_t pts = S.record & (kPointP | kPointU);
   if (S.full_mask && !init) pts &= ~kPointP;  // fused into the p producer
+  if (S.directivity) pts &= ~(uint32_t)KW_REC_P;  // the directional p below
   if (pts) {
     EwArgs e{};
     e.kind = EW_RECORD_POINTS;
@@ -636,6 +709,7 @@ static void record_step(Sim& S, int t, bool init) {
     S.ew(e, S.lsens.nb);
   }
   if (S.record & (KW_REC_U_NON_STAGGERED | KW_REC_U_SPLIT_FIELD)) record_velocity(S);
+  if (!S.dir_tab.empty()) record_directivity(S);
   uint32_t grid = S.record & (KW_REC_U_MAX_ALL | KW_REC_U_MIN_ALL);
   if (init) grid |= S.record & (KW_REC_P_MAX_ALL | KW_REC_P_MIN_ALL);
   if (grid) {

@@ -155,11 +155,12 @@ struct Sim {
   int64_t bytes = 0;
   int64_t steps_done = 0;
   int64_t Nt = 0, Nt_eff = 0, n_rec = 0, nsens = 0, record_start = 1;
-  bool has_p0 = false, time_rev = false, full_mask = false;
+  bool has_p0 = false, time_rev = false, full_mask = false, directivity = false;
   uint32_t record = 0;
   bool p_grid = false, u_grid[3] = {false, false, false};
   float* skappa = nullptr;
   float* p0 = nullptr;
+  std::vector<float*> dir_tab;  // k-space filter per unique directivity angle
   LaunchGeom lx, ly, ll, lcell, lsens;
   // per-kernel event timing
   bool timing = false;
@@ -453,6 +454,7 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr, void* stream = nullptr);
 PassArgs bufs(int nb, int b0 = 0, int b1 = 1, int b2 = 2);
 void src_grid(Sim& S, int which, int t);
 void record_velocity(Sim& S);
+void record_directivity(Sim& S);
 void step_once(Sim& S);
 void step_n(Sim& S, int64_t nsteps);
 int64_t recorded(const Sim& S);

@@ -81,6 +81,8 @@ class Problem(C.Structure):
         ("record_start_index", C.c_int64),
         ("time_reversal_boundary_data", c_double_p),
         ("sensor_tri", c_u64_p), ("sensor_bc", c_double_p),
+        ("directivity_angle", c_double_p), ("directivity_pattern", C.c_int32),
+        ("directivity_size", C.c_double),
     ]
 
 

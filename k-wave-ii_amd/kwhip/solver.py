@@ -458,6 +458,8 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
     info.reorder_index = None
     info.interp = None
     info.cuboids = None
+    info.freq_response = None
+    info.dt = dt
     info.inside = None  # user's grid inside the expanded one ('PMLInside', false)
     info.time_rev = False
     info.use_sensor = sensor is not None and _has(sensor, "mask")
@@ -468,6 +470,11 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
     sens_full = False  # the mask is the whole grid (no index list is built)
     if info.use_sensor:
         info.time_rev = _has(sensor, "time_reversal_boundary_data")
+        if _has(sensor, "frequency_response") and not info.time_rev:
+            fr = np.asarray(_get(sensor, "frequency_response"), dtype=np.float64).ravel()
+            if fr.size != 2:
+                raise ValueError("sensor.frequency_response must be [center_freq, bandwidth]")
+            info.freq_response = fr
         if _has(sensor, "record"):
             record = list(_get(sensor, "record"))
             info.record_given = True
@@ -732,6 +739,22 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
             pr.sensor_bc = _abi.dptr(pp.hold(np.ascontiguousarray(bc.ravel(order="F"))))
         elif not full:
             pr.sensor_mask_index = _abi.uptr(pp.hold(np.ascontiguousarray(sens_idx.astype(np.uint64) + 1)))
+        if _has(sensor, "directivity_angle") and not info.time_rev:
+            # kspaceFirstOrder_inputChecking.m:312-343 (time reversal ignores
+            # it, :703-705)
+            if dim != 2:
+                raise ValueError("sensor.directivity_angle is only supported in 2D")
+            ang = np.asarray(_get(sensor, "directivity_angle"), dtype=np.float64)
+            if ang.shape != shape or info.interp is not None or info.reorder_index is not None \
+                    or info.cuboids is not None:
+                raise ValueError("sensor.directivity_angle and sensor.mask must be the same size.")
+            pattern = _get(sensor, "directivity_pattern", "pressure")
+            if pattern not in ("pressure", "gradient"):
+                raise ValueError("Unsupported directivity pattern.")
+            at = ang.ravel(order="F") if full else ang.ravel(order="F")[sens_idx]
+            pr.directivity_angle = _abi.dptr(pp.hold(np.ascontiguousarray(at)))
+            pr.directivity_pattern = 0 if pattern == "pressure" else 1
+            pr.directivity_size = float(_get(sensor, "directivity_size", 10 * max(kgrid.dx, kgrid.dy)))
         pr.record_flags = rec_flags
         pr.record_start_index = record_start
         if info.time_rev:
@@ -833,6 +856,10 @@ def shape_outputs(pp: PreparedProblem, arrays: dict):
                 sd[k] = sd[k][order]
     if not info.use_sensor:
         return None
+    if info.freq_response is not None and "p" in sd:
+        # kspaceFirstOrder3D.m:1346-1348
+        sd["p"] = gaussian_filter(sd["p"], 1.0 / info.dt, info.freq_response[0],
+                                  info.freq_response[1])
     if info.time_rev:
         return sd["p_final"]
     if info.cuboids is not None:
@@ -842,6 +869,23 @@ def shape_outputs(pp: PreparedProblem, arrays: dict):
     if not info.record_given:
         return sd["p"]
     return sd
+
+
+def gaussian_filter(signal, Fs, freq, bandwidth):
+    """+kwave/+legacy/gaussianFilter.m: every row of `signal` filtered in the
+    frequency domain by the larger of two unit-height Gaussians at +freq and
+    -freq whose full width at half maximum is bandwidth percent of freq
+    (gaussian.m)."""
+    signal = np.asarray(signal)
+    N = signal.shape[1]
+    f = (np.arange(N) - N // 2) * (Fs / N)
+    sigma2 = (bandwidth / 100.0 * freq / (2.0 * math.sqrt(2.0 * math.log(2.0)))) ** 2
+    gain = np.maximum(np.exp(-((f - freq) ** 2) / (2.0 * sigma2)),
+                      np.exp(-((f + freq) ** 2) / (2.0 * sigma2)))
+    # the gain is centred: move it to FFT order rather than the spectrum
+    gain = np.fft.ifftshift(gain)
+    out = np.real(np.fft.ifft(np.fft.fft(signal, axis=1) * gain[None, :], axis=1))
+    return out.astype(signal.dtype, copy=False)
 
 
 def fourier_shift(data, shift, axis=None):

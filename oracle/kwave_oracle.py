@@ -359,6 +359,52 @@ class _Interp:
         return self.tri.shape[0]
 
 
+def directivity_filters(kgrid: KGrid, unique_angles, pattern, size):
+    """The k-space filter of each unique sensor angle
+    (LP/directionalResponse.m with the wavenumbers of
+    LP/kspaceFirstOrder_inputChecking.m:342): 'pressure': sinc(|k_t| size/2)
+    with the tangential wavenumber k_t = cos(theta) ky - sin(theta) kx;
+    'gradient': |k_n / k| with k_n = sin(theta) ky + cos(theta) kx (0 at
+    k = 0).  Built on the centred wavenumber grids and moved to FFT order
+    with fftshift, as the reference does (for an odd length this is one bin
+    off ifftshift)."""
+    kx = kgrid.kx_vec[:, None] * np.ones((1, kgrid.Ny))
+    ky = np.ones((kgrid.Nx, 1)) * kgrid.ky_vec[None, :]
+    k = np.sqrt(kx ** 2 + ky ** 2)
+    out = []
+    for th in unique_angles:
+        if pattern == "pressure":
+            kt = np.abs(math.cos(th) * ky - math.sin(th) * kx)
+            d = sinc(kt * size / 2)
+        elif pattern == "gradient":
+            kn = math.sin(th) * ky + math.cos(th) * kx
+            with np.errstate(invalid="ignore", divide="ignore"):
+                d = np.abs(kn / k)
+            d[k == 0] = 0.0
+        else:
+            raise ValueError("Unsupported directivity pattern.")
+        out.append(np.fft.fftshift(d))
+    return out
+
+
+def gaussian_filter(signal, Fs, freq, bandwidth):
+    """L/gaussianFilter.m: each row filtered in the frequency domain by the
+    larger of two Gaussians of unit height centred on +freq and -freq, with
+    FWHM bandwidth% of freq (variance (bw/100 freq / (2 sqrt(2 ln 2)))^2,
+    L/gaussian.m)."""
+    signal = np.asarray(signal)
+    N = signal.shape[1]
+    if N % 2 == 0:
+        f = np.arange(-N // 2, N // 2) * Fs / N
+    else:
+        f = np.arange(-(N - 1) // 2, (N - 1) // 2 + 1) * Fs / N
+    var = (bandwidth / 100 * freq / (2 * math.sqrt(2 * math.log(2)))) ** 2
+    g = np.maximum(np.exp(-(f - freq) ** 2 / (2 * var)), np.exp(-(f + freq) ** 2 / (2 * var)))
+    spec = np.fft.fftshift(np.fft.fft(signal, axis=1), axes=1)
+    out = np.real(np.fft.ifft(np.fft.ifftshift(g[None, :] * spec, axes=1), axis=1))
+    return out.astype(signal.dtype)
+
+
 def cuboid_corners_index(kgrid: KGrid, corners):
     """Sensor points of a cuboid-corners mask, [x1, (y1, (z1,)) x2, (y2,
     (z2))] per column, 1-based and inclusive
@@ -1069,6 +1115,12 @@ def kspace_first_order(kgrid: KGrid, medium, source, sensor, *, data_cast="off",
         pv = sensor_idx.get(p)
         if "p" in sd:
             sd["p"][:, fi - 1] = pv
+            if dir_filters is not None:
+                # kspaceFirstOrder2D.m:945-947, LP/directionalResponse.m: the
+                # time series (only) from p_k filtered per sensor angle
+                for filt, sel in dir_filters:
+                    pd = np.real(sfft.ifftn(p_k * filt, workers=W)).astype(rdt)
+                    sd["p"][sel, fi - 1] = sensor_idx.get(pd)[sel]
         if "p_max" in record:
             sd["p_max"] = pv.copy() if fi == 1 else np.maximum(sd["p_max"], pv)
         if "p_min" in record:
@@ -1115,6 +1167,20 @@ def kspace_first_order(kgrid: KGrid, medium, source, sensor, *, data_cast="off",
 
     u_ns_needed = bool(record & {"u_non_staggered", "I", "I_avg"})
     dim_c = rdt(dim)
+    dir_filters = None
+    if use_sensor and not time_rev and _has(sensor, "directivity_angle"):
+        # LP/kspaceFirstOrder_inputChecking.m:312-343 (2-D, binary masks)
+        if dim != 2:
+            raise ValueError("sensor.directivity_angle is only supported in 2D")
+        ang = np.asarray(_get(sensor, "directivity_angle"), dtype=np.float64)
+        if ang.shape != shape or interp is not None or reorder_index is not None or cuboids is not None:
+            raise ValueError("sensor.directivity_angle and sensor.mask must be the same size.")
+        pattern = _get(sensor, "directivity_pattern", "pressure")
+        dsize = float(_get(sensor, "directivity_size", 10 * max(kgrid.dx, kgrid.dy)))
+        at_sensors = sensor_idx.get(ang)
+        uniq = np.unique(at_sensors)
+        dir_filters = [(f.astype(rdt), at_sensors == th)
+                       for f, th in zip(directivity_filters(kgrid, uniq, pattern, dsize), uniq)]
 
     # -- time loop (kspaceFirstOrder3D.m:769-1208) ---------------------------
     index_end = Nt - 1 if time_rev else Nt
@@ -1217,6 +1283,10 @@ def kspace_first_order(kgrid: KGrid, medium, source, sensor, *, data_cast="off",
             if key.endswith("_all") or key.endswith("_final"):
                 continue
             sd[key] = reorder_binary_sensor_data(sd[key], reorder_index)
+    if use_sensor and not time_rev and _has(sensor, "frequency_response") and "p" in sd:
+        # kspaceFirstOrder3D.m:1346-1348
+        fr = np.asarray(_get(sensor, "frequency_response"), dtype=np.float64).ravel()
+        sd["p"] = gaussian_filter(sd["p"], 1.0 / dt, fr[0], fr[1])
     if return_state:
         return sd, SimpleNamespace(p=p, rho=rhos, u=us)
     if not use_sensor:

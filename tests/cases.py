@@ -238,6 +238,31 @@ def case(name, dim=2, N=None):
             rec = ["p", "p_max", "u", "p_final"]
         sens = {"mask": pts, "record": rec}
         opts["CartInterp"] = "linear"
+    elif name in ("directivity_pressure", "directivity_gradient"):
+        # 2-D sensor directivity: a line of sensors facing the source at a
+        # few angles, a full-grid sensor with one angle per column
+        c, rho = hetero_medium(shape, 37)
+        med = {"sound_speed": c, "density": rho}
+        src["p0"] = disc(shape, centre, 3)
+        pattern = name.split("_")[1]
+        if pattern == "pressure":
+            m = _sparse_mask(shape, rng, 0.2)
+            ang = np.zeros(shape)
+            ang[...] = (np.arange(shape[0]) % 4)[:, None] * (np.pi / 5)
+            sens = {"mask": m, "record": ["p", "p_max", "I_avg"], "directivity_angle": ang,
+                    "directivity_size": 4 * dx}
+        else:
+            ang = np.repeat((np.arange(shape[1]) % 3)[None, :] * 0.7, shape[0], 0)
+            sens = {"mask": np.ones(shape), "record": ["p", "p_rms"], "directivity_angle": ang,
+                    "directivity_pattern": "gradient"}
+    elif name == "freq_response":
+        # the recorded p filtered by a Gaussian around 2 MHz (intensity and
+        # p_max are taken from the unfiltered p)
+        c, rho = hetero_medium(shape, 31)
+        med = {"sound_speed": c, "density": rho}
+        src["p0"] = disc(shape, centre, 2)
+        sens = {"mask": _sparse_mask(shape, rng), "record": ["p", "p_max", "I_avg"],
+                "frequency_response": [2e6, 60]}
     elif name in ("cuboid_corners", "cuboid_corners_pml_outside"):
         # two overlapping cuboids of sensor points (1-based, inclusive)
         c, rho = hetero_medium(shape, 29)
@@ -284,14 +309,16 @@ CASES_2D = ["p0_lossless", "p0_hetero_absorbing", "p0_nonlinear_absorbing", "p0_
             "psrc_many", "psrc_label", "psrc_many_dir", "usrc_add", "usrc_dir", "usrc_nocorr",
             "usrc_many", "time_reversal", "cartesian_nearest", "intensity", "u_nonstaggered", "u_split",
             "pml_outside", "pml_outside_source", "cartesian_linear", "cartesian_linear_pml_outside",
-            "cuboid_corners", "cuboid_corners_pml_outside"]
-CASES_3D = CASES_2D
+            "cuboid_corners", "cuboid_corners_pml_outside", "freq_response",
+            "directivity_pressure", "directivity_gradient"]
+# sensor directivity exists in 2-D only
+CASES_3D = [c for c in CASES_2D if not c.startswith("directivity")]
 # kspaceFirstOrder1D: every case whose inputs exist in 1-D
 CASES_1D = ["p0_lossless", "p0_hetero_absorbing", "p0_nonlinear_absorbing", "p0_nonlinear_lossless",
             "stokes", "no_dispersion", "psrc_add", "psrc_dir", "psrc_nocorr", "psrc_many",
             "psrc_label", "usrc_add", "usrc_dir", "usrc_many", "time_reversal", "cartesian_nearest",
             "cartesian_linear", "intensity", "u_nonstaggered", "pml_outside", "cartesian_linear_pml_outside",
-            "cuboid_corners", "cuboid_corners_pml_outside"]
+            "cuboid_corners", "cuboid_corners_pml_outside", "freq_response"]
 
 
 def oracle_opts(opts):

@@ -141,6 +141,10 @@ def test_interpolated_sensor_points_are_checked():
     prob.sensor_tri = _abi.uptr(bad)
     assert create() == _abi.KW_EINVAL and b"range" in buf.value
     prob.sensor_tri = _abi.uptr(tri)
+    ang = np.zeros(2)
+    prob.directivity_angle = _abi.dptr(ang)
+    assert create() == _abi.KW_EINVAL and b"binary sensor mask" in buf.value
+    prob.directivity_angle = None
     prob.record_flags = _abi.REC["u_split_field"]
     assert create() == _abi.KW_EINVAL and b"binary sensor mask" in buf.value
     prob.record_flags = _abi.REC["p"]

@@ -63,6 +63,8 @@ MIX = [
     (3, "u_split", (15, 18, 16)),
     # Cartesian points interpolated on simplices
     (2, "cartesian_linear", (30, 32)),
+    # per-member directivity filters
+    (2, "directivity_gradient", (30, 32)),
     # 1-D problems (one-row grids) in the same engine
     (1, "psrc_add", (96,)),
     (1, "time_reversal", (75,)),

@@ -274,3 +274,26 @@ def test_velocity_split_of_gradient_and_rotational_fields(shape):
     for a in range(dim):
         assert np.max(np.abs(comp[a])) < 1e-10 * max(kvec)
         assert np.max(np.abs(shear[a] - rot_ns[a])) < 1e-10 * max(kvec)
+
+
+@pytest.mark.parametrize("N", [64, 63])
+def test_gaussian_filter_gain_on_tones(N):
+    """gaussianFilter on sampled tones at FFT bin frequencies: the gain is
+    the larger of the two Gaussians at +-freq, exp(-(f - freq)^2 / (2 s^2))
+    with FWHM = bandwidth% of freq, i.e. 1 at the centre frequency and 1/2
+    at half the bandwidth away from it."""
+    Fs = 64e6
+    df = Fs / N
+    freq = 10 * df
+    bw = 40.0
+    s2 = (bw / 100 * freq / (2 * math.sqrt(2 * math.log(2)))) ** 2
+    t = np.arange(N) / Fs
+    for m, ph in ((10, 0.3), (12, 1.1), (7, -0.4), (13, 0.0)):
+        f = m * df
+        sig = np.cos(2 * np.pi * f * t + ph)[None, :]
+        out = ko.gaussian_filter(sig, Fs, freq, bw)
+        gain = math.exp(-(f - freq) ** 2 / (2 * s2))
+        assert np.allclose(out, gain * sig, atol=1e-12), m
+    # FWHM: a tone bw/2 % away from the centre keeps half its amplitude
+    half = math.exp(-((bw / 200 * freq) ** 2) / (2 * s2))
+    assert abs(half - 0.5) < 1e-12

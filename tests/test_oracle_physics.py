@@ -779,3 +779,56 @@ def test_cuboid_corners_equal_full_grid_record(dim):
     bad[dim, 0] = shape[0] + 1
     with pytest.raises(ValueError, match="within the grid"):
         run(kg, med, {"p0": p0}, {"mask": bad}, pml_size=2)
+
+
+@pytest.mark.parametrize("shape", [(24, 20), (25, 21)])
+def test_directivity_on_a_plane_wave(shape):
+    """2-D sensor directivity on a periodic standing wave cos(kx0 x): the
+    field stays one mode (+-kx0, 0), so every recorded p is the raw p times
+    the mean of the filter at the two bins.  Even sizes: 'gradient' gives
+    |cos(theta)|, 'pressure' sinc(|sin(theta)| kx0 size/2), and size 0 the
+    raw p.  Odd sizes: the reference moves the centred filter to FFT order
+    with fftshift, which reads the filter one bin off, at kx = (m+1) dk and
+    (1-m) dk and ky = dky; the factor is the mean of |kx|/|k| there."""
+    Nx, Ny = shape
+    dx = dy = 1e-4
+    kg = ko.KGrid(Nx, dx, Ny, dy)
+    kg.set_time(15, 2e-8)
+    m = 3
+    x = np.arange(Nx) * dx
+    p0 = np.repeat(np.cos(2 * np.pi * m * x / (Nx * dx))[:, None], Ny, 1)
+    mask = np.zeros(shape)
+    pts = [(2, 3), (7, 5), (11, 9), (16, 14)]
+    for i, j in pts:
+        mask[i, j] = 1
+    thetas = [0.0, np.pi / 6, np.pi / 2, 2.0]
+    ang = np.zeros(shape)
+    for (i, j), th in zip(pts, thetas):
+        ang[i, j] = th
+    med = {"sound_speed": 1500.0, "density": 1000.0}
+    raw = run(kg, med, {"p0": p0}, {"mask": mask, "record": ["p"]}, pml_size=2, pml_alpha=0.0)["p"]
+    order = np.argsort([i + Nx * j for i, j in pts])   # sensor rows in column-major order
+    th_rows = np.array(thetas)[order]
+    kx0 = 2 * np.pi * m / (Nx * dx)
+    dkx, dky = 2 * np.pi / (Nx * dx), 2 * np.pi / (Ny * dy)
+
+    def directional(pattern, size):
+        sens = {"mask": mask, "record": ["p"], "directivity_angle": ang,
+                "directivity_pattern": pattern, "directivity_size": size}
+        return run(kg, med, {"p0": p0}, sens, pml_size=2, pml_alpha=0.0)["p"]
+
+    scale = np.max(np.abs(raw))
+    got = directional("gradient", 1e-3)
+    if Nx % 2 == 0 and Ny % 2 == 0:
+        want = np.abs(np.cos(th_rows))
+    else:
+        def g(kx, ky, th):
+            return abs(np.sin(th) * ky + np.cos(th) * kx) / np.hypot(kx, ky)
+        want = np.array([(g((m + 1) * dkx, dky, th) + g((1 - m) * dkx, dky, th)) / 2 for th in th_rows])
+    assert np.allclose(got, want[:, None] * raw, rtol=0, atol=1e-9 * scale)
+    if Nx % 2 == 0 and Ny % 2 == 0:
+        size = 3 * dx
+        got = directional("pressure", size)
+        want = np.array([ko.sinc(abs(np.sin(th)) * kx0 * size / 2) for th in th_rows])
+        assert np.allclose(got, want[:, None] * raw, rtol=0, atol=1e-9 * scale)
+        assert np.allclose(directional("pressure", 0.0), raw, rtol=0, atol=1e-12 * scale)
