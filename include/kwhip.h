@@ -152,6 +152,10 @@ typedef struct kw_problem {
   const double* directivity_angle;
   int32_t directivity_pattern;
   double directivity_size;
+  /* medium.alpha_filter: a grid of Nx*Ny*Nz values in centred (fftshifted)
+   * k-space order multiplying both power-law absorption operators
+   * (kspaceFirstOrder_createAbsorptionVariables.m:56-76), or NULL */
+  const double* alpha_filter;
 } kw_problem;
 
 typedef struct kw_sizes {

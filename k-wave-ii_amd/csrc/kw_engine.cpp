@@ -39,7 +39,7 @@ using Clock = std::chrono::steady_clock;
 // ---------------------------------------------------------------------------
 struct ProblemCopy {
   kw_problem pb{};
-  std::vector<double> c0, rho0, rho0_sg[3], bona, alpha, p0, tr, bc, dir_angle;
+  std::vector<double> c0, rho0, rho0_sg[3], bona, alpha, alpha_filter, p0, tr, bc, dir_angle;
   std::vector<uint64_t> sens, tri;
   struct Src {
     std::vector<uint64_t> index, sig_index;
@@ -62,6 +62,7 @@ struct ProblemCopy {
     for (int d = 0; d < 3; ++d) pb.rho0_sg[d] = grid(in.rho0_sg[d], rho0_sg[d]);
     pb.BonA = grid(in.BonA, bona);
     pb.alpha_coeff = grid(in.alpha_coeff, alpha);
+    pb.alpha_filter = grid(in.alpha_filter, alpha_filter);
     pb.p0 = grid(in.p0, p0);
     pb.sensor_mask_index = nullptr;
     if (in.sensor_mask_index && in.sensor_num_points > 0) {

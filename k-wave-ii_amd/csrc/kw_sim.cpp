@@ -357,15 +357,30 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream) {
   });
   if (P.eos == EOS_ABSORBING) {
     const bool no_abs = pr->alpha_mode == 1, no_disp = pr->alpha_mode == 2;
+    // medium.alpha_filter (createAbsorptionVariables.m:56-76): both
+    // operators times the centred filter, FFT bin i of an axis of length N
+    // at centred index (i + floor(N/2)) mod N; the even part, which is what
+    // real(ifftn(.)) keeps of the product
+    const double* af = pr->alpha_filter;
+    auto filt = [&](int ix, int iy, int iz) {
+      if (!af) return 1.0;
+      auto at = [&](int jx, int jy, int jz) {
+        const int64_t cx = (jx + G.Nx / 2) % G.Nx, cy = (jy + G.Ny / 2) % G.Ny;
+        const int64_t cz = dim == 3 ? (jz + G.Nz / 2) % G.Nz : 0;
+        return af[cx + (int64_t)G.Nx * (cy + (int64_t)G.Ny * cz)];
+      };
+      const int mz = dim == 3 ? (G.Nz - iz) % G.Nz : 0;
+      return 0.5 * (at(ix, iy, iz) + at((G.Nx - ix) % G.Nx, (G.Ny - iy) % G.Ny, mz));
+    };
     P.nabla1 = S->spectral_table([&](int ix, int iy, int iz) {
       if (no_abs) return 0.0;
       const double v = std::pow(kmag(ix, iy, iz), y - 2);
-      return std::isinf(v) ? 0.0 : v / Ntot;
+      return std::isinf(v) ? 0.0 : v * filt(ix, iy, iz) / Ntot;
     });
     P.nabla2 = S->spectral_table([&](int ix, int iy, int iz) {
       if (no_disp) return 0.0;
       const double v = std::pow(kmag(ix, iy, iz), y - 1);
-      return std::isinf(v) ? 0.0 : v / Ntot;
+      return std::isinf(v) ? 0.0 : v * filt(ix, iy, iz) / Ntot;
     });
   }
 

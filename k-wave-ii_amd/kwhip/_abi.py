@@ -82,7 +82,7 @@ class Problem(C.Structure):
         ("time_reversal_boundary_data", c_double_p),
         ("sensor_tri", c_u64_p), ("sensor_bc", c_double_p),
         ("directivity_angle", c_double_p), ("directivity_pattern", C.c_int32),
-        ("directivity_size", C.c_double),
+        ("directivity_size", C.c_double), ("alpha_filter", c_double_p),
     ]
 
 

@@ -415,7 +415,12 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
     if alpha_mode is not None and alpha_mode not in ("no_absorption", "no_dispersion", "stokes"):
         raise ValueError("medium.alpha_mode must be set to 'no_absorption', 'no_dispersion', or 'stokes'.")
     if _has(medium, "alpha_filter"):
-        raise NotImplementedError("medium.alpha_filter is not supported by kwhip")
+        # kspaceFirstOrder_inputChecking.m:198-201,240-241
+        if alpha_mode == "stokes":
+            raise ValueError("Input option medium.alpha_filter is not supported with the "
+                             "axisymmetric code or medium.alpha_mode = 'stokes'.")
+        if np.asarray(_get(medium, "alpha_filter")).shape != shape:
+            raise ValueError("medium.alpha_filter must be the same size as the computational grid.")
     nonlinear = _has(medium, "BonA")
     if nonlinear and not _has(medium, "density"):
         raise ValueError("medium.density must be defined for nonlinear simulations")
@@ -475,6 +480,10 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
             if fr.size != 2:
                 raise ValueError("sensor.frequency_response must be [center_freq, bandwidth]")
             info.freq_response = fr
+        if _has(sensor, "record_mode"):
+            # kspaceFirstOrder_inputChecking.m:360-363
+            raise ValueError("sensor.record_mode input has been deprecated. Please use the syntax "
+                             "sensor.record = {'p_rms', 'p_max', ...} to set recorded fields.")
         if _has(sensor, "record"):
             record = list(_get(sensor, "record"))
             info.record_given = True
@@ -692,6 +701,9 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
                 raise ValueError("medium.alpha_power must be scalar.")
             pr.alpha_power = float(y)
             pr.alpha_mode = {"no_absorption": 1, "no_dispersion": 2}.get(alpha_mode, 0)
+            if _has(medium, "alpha_filter"):
+                af = np.asarray(_get(medium, "alpha_filter"), dtype=np.float64)
+                pr.alpha_filter = _abi.dptr(pp.hold(_abi.fortran_flat(af)))
         a, s_ = grid_or_scalar(np.asarray(_get(medium, "alpha_coeff"), dtype=np.float64))
         pr.alpha_coeff, pr.alpha_coeff_scalar = _abi.dptr(a), s_
         if _has(medium, "alpha_sign"):

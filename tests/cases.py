@@ -255,6 +255,15 @@ def case(name, dim=2, N=None):
             ang = np.repeat((np.arange(shape[1]) % 3)[None, :] * 0.7, shape[0], 0)
             sens = {"mask": np.ones(shape), "record": ["p", "p_rms"], "directivity_angle": ang,
                     "directivity_pattern": "gradient"}
+    elif name == "alpha_filter":
+        # absorption compensation: the power-law operators times a centred
+        # k-space window (not symmetric for odd sizes)
+        c, rho = hetero_medium(shape, 41)
+        g = np.meshgrid(*[np.linspace(-1, 1, n) for n in shape], indexing="ij")
+        af = np.clip(1.2 - sum(v * v for v in g), 0, 1) * (1 + 0.1 * g[0])
+        med = {"sound_speed": c, "density": rho, "alpha_coeff": 3.0, "alpha_power": 1.5,
+               "alpha_filter": af}
+        src["p0"] = disc(shape, centre, 3)
     elif name == "freq_response":
         # the recorded p filtered by a Gaussian around 2 MHz (intensity and
         # p_max are taken from the unfiltered p)
@@ -310,7 +319,7 @@ CASES_2D = ["p0_lossless", "p0_hetero_absorbing", "p0_nonlinear_absorbing", "p0_
             "usrc_many", "time_reversal", "cartesian_nearest", "intensity", "u_nonstaggered", "u_split",
             "pml_outside", "pml_outside_source", "cartesian_linear", "cartesian_linear_pml_outside",
             "cuboid_corners", "cuboid_corners_pml_outside", "freq_response",
-            "directivity_pressure", "directivity_gradient"]
+            "directivity_pressure", "directivity_gradient", "alpha_filter"]
 # sensor directivity exists in 2-D only
 CASES_3D = [c for c in CASES_2D if not c.startswith("directivity")]
 # kspaceFirstOrder1D: every case whose inputs exist in 1-D
@@ -318,7 +327,7 @@ CASES_1D = ["p0_lossless", "p0_hetero_absorbing", "p0_nonlinear_absorbing", "p0_
             "stokes", "no_dispersion", "psrc_add", "psrc_dir", "psrc_nocorr", "psrc_many",
             "psrc_label", "usrc_add", "usrc_dir", "usrc_many", "time_reversal", "cartesian_nearest",
             "cartesian_linear", "intensity", "u_nonstaggered", "pml_outside", "cartesian_linear_pml_outside",
-            "cuboid_corners", "cuboid_corners_pml_outside", "freq_response"]
+            "cuboid_corners", "cuboid_corners_pml_outside", "freq_response", "alpha_filter"]
 
 
 def oracle_opts(opts):

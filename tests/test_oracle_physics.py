@@ -832,3 +832,31 @@ def test_directivity_on_a_plane_wave(shape):
         want = np.array([ko.sinc(abs(np.sin(th)) * kx0 * size / 2) for th in th_rows])
         assert np.allclose(got, want[:, None] * raw, rtol=0, atol=1e-9 * scale)
         assert np.allclose(directional("pressure", 0.0), raw, rtol=0, atol=1e-12 * scale)
+
+
+def test_alpha_filter_selects_absorbed_modes():
+    """medium.alpha_filter (centred k-space order) multiplies both absorption
+    operators: on a periodic grid, a filter that is 1 at the centred bins of
+    +-k1 and 0 elsewhere absorbs mode k1 as without a filter and leaves mode
+    k2 lossless (by linearity the run is the sum of the two); a filter of
+    ones changes nothing."""
+    N, dx, Nt, dt = 48, 1e-4, 40, 2e-8
+    kg = ko.KGrid(N, dx)
+    kg.set_time(Nt, dt)
+    x = np.arange(N) * dx
+    m1, m2 = 5, 9
+    f1 = np.cos(2 * np.pi * m1 * x / (N * dx))
+    f2 = np.cos(2 * np.pi * m2 * x / (N * dx))
+    absorbing = {"sound_speed": 1500.0, "density": 1000.0, "alpha_coeff": 40.0, "alpha_power": 1.5}
+    lossless = {"sound_speed": 1500.0, "density": 1000.0}
+    sens = {"mask": np.ones(N), "record": ["p"]}
+    kw = dict(pml_size=2, pml_alpha=0.0)
+    af = np.zeros(N)
+    af[N // 2 + m1] = af[N // 2 - m1] = 1.0     # centred index of +-k1
+    got = run(kg, dict(absorbing, alpha_filter=af), {"p0": f1 + f2}, sens, **kw)["p"]
+    want = run(kg, absorbing, {"p0": f1}, sens, **kw)["p"] + run(kg, lossless, {"p0": f2}, sens, **kw)["p"]
+    assert np.max(np.abs(got - want)) < 1e-11 * np.max(np.abs(want))
+    plain = run(kg, absorbing, {"p0": f1 + f2}, sens, **kw)["p"]
+    assert np.max(np.abs(got - plain)) > 1e-3 * np.max(np.abs(plain))   # mode k2 was absorbed there
+    ones = run(kg, dict(absorbing, alpha_filter=np.ones(N)), {"p0": f1 + f2}, sens, **kw)["p"]
+    assert np.max(np.abs(ones - plain)) < 1e-12 * np.max(np.abs(plain))
