@@ -49,6 +49,31 @@ RECORD_ALLOWED = ("p", "p_max", "p_min", "p_rms", "p_max_all", "p_min_all", "p_f
                   "u_non_staggered", "I", "I_avg", "u_split_field")
 
 
+# allowed fields of the input structures (kspaceFirstOrder_inputChecking.m:
+# 148-149, 302-350, 729-741), checked as checkFieldNames.m does
+_MEDIUM_FIELDS = {"sound_speed", "sound_speed_ref", "density", "alpha_coeff", "alpha_power",
+                  "alpha_mode", "alpha_filter", "alpha_sign", "BonA"}
+_SENSOR_FIELDS = {"mask", "time_reversal_boundary_data", "frequency_response", "record_mode",
+                  "record", "record_start_index"}
+_DIRECTIVITY_FIELDS = {"directivity_pattern", "directivity_angle", "directivity_size"}
+
+
+def _check_fields(obj, allowed, what):
+    if obj is None:
+        return
+    names = obj.keys() if isinstance(obj, dict) else vars(obj).keys()
+    for nm in names:
+        if nm not in allowed:
+            raise ValueError(f"{nm} is not a valid field for the structure {what}.")
+
+
+def _check_inputs(dim, medium, source, sensor):
+    _check_fields(medium, _MEDIUM_FIELDS, "medium")
+    src = {"p0", "p", "p_mask", "p_mode", "p_frequency_ref", "u_mask", "u_mode",
+           "u_frequency_ref"} | set(["ux", "uy", "uz"][:dim])
+    _check_fields(source, src, "source")
+    _check_fields(sensor, _SENSOR_FIELDS | (_DIRECTIVITY_FIELDS if dim == 2 else set()), "sensor")
+
 def _get(obj, name, default=None):
     if obj is None:
         return default
@@ -397,6 +422,8 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
     shape = kgrid.shape
     ncell = kgrid.total_grid_points
 
+    _check_inputs(dim, medium, source, sensor)
+
     # -- medium (inputChecking.m:148-281) -----------------------------------
     if medium is None or not _has(medium, "sound_speed"):
         raise ValueError("The field medium.sound_speed must be defined.")
@@ -591,6 +618,21 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
     src_p = time_source(["p"], "p_mask", "p_mode")
     u_names = ["ux", "uy", "uz"][:dim]
     src_u = time_source(u_names, "u_mask", "u_mode")
+    # the k-space source correction at one frequency instead of the k-space
+    # filter: signals times cos(2 pi f_ref dt/2), injected without
+    # correction (kspaceFirstOrder_inputChecking.m:819-836,922-939,
+    # kspaceFirstOrder_scaleSourceTerms.m:42-45,268-282)
+    for sx, key in ((src_p, "p_frequency_ref"), (src_u, "u_frequency_ref")):
+        if sx is None or not _has(source, key):
+            continue
+        f_ref = float(_get(source, key))
+        if f_ref < 0:
+            raise ValueError(f"source.{key} must be nonnegative")
+        if f_ref > kgrid.k_max * float(np.min(c)) / (2 * math.pi):
+            raise ValueError(f"source.{key} is higher than the maximum frequency supported by the spatial grid.")
+        sx.mode = "additive-no-correction"
+        w = math.cos(math.pi * f_ref * dt)
+        sx.sigs = {nm: s * w for nm, s in sx.sigs.items()}
 
     # -- smoothing (inputChecking.m:1501-1600) ---------------------------------
     if p0 is not None and smooth_p0:
@@ -978,8 +1020,6 @@ def kspaceFirstOrder1D(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
     medium = _column_to_1d(medium, ("sound_speed", "density", "alpha_coeff", "BonA"), n)
     source = _column_to_1d(source, ("p0", "p_mask", "u_mask"), n)
     sensor = _column_to_1d(sensor, ("mask",), n)
-    if source is not None and any(nm in source for nm in ("uy", "uz")):
-        raise ValueError("a 1D simulation takes the velocity source ux only")
     if sensor is not None and "u_split_field" in list(sensor.get("record", [])):
         raise ValueError("sensor.record 'u_split_field' needs a 2D or 3D simulation")
     return _solve(1, kgrid, medium, source, sensor, args, kwargs)

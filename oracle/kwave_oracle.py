@@ -486,6 +486,16 @@ class _Idx:
         return len(self.lin)
 
 
+def _check_fields(obj, allowed, what):
+    """LP/checkFieldNames.m: every field of the structure must be allowed
+    (lists of LP/kspaceFirstOrder_inputChecking.m:148-149,302-350,729-741)."""
+    if obj is None:
+        return
+    for nm in (obj.keys() if isinstance(obj, dict) else vars(obj).keys()):
+        if nm not in allowed:
+            raise ValueError(f"{nm} is not a valid field for the structure {what}.")
+
+
 def _along(v, ax, dim):
     sh = [1] * dim
     sh[ax] = -1
@@ -746,6 +756,16 @@ def kspace_first_order(kgrid: KGrid, medium, source, sensor, *, data_cast="off",
     def ifftn_real(a):
         return np.real(sfft.ifftn(a, workers=W))
 
+    _check_fields(medium, {"sound_speed", "sound_speed_ref", "density", "alpha_coeff",
+                           "alpha_power", "alpha_mode", "alpha_filter", "alpha_sign", "BonA"},
+                  "medium")
+    _check_fields(source, {"p0", "p", "p_mask", "p_mode", "p_frequency_ref", "u_mask", "u_mode",
+                           "u_frequency_ref"} | set(["ux", "uy", "uz"][:dim]), "source")
+    _check_fields(sensor, {"mask", "time_reversal_boundary_data", "frequency_response",
+                           "record_mode", "record", "record_start_index"}
+                  | ({"directivity_pattern", "directivity_angle", "directivity_size"}
+                     if dim == 2 else set()), "sensor")
+
     # -- medium (LP/kspaceFirstOrder_inputChecking.m:148-281) --------------
     c = np.asarray(_get(medium, "sound_speed"), dtype=np.float64)
     if _has(medium, "density"):
@@ -794,6 +814,10 @@ def kspace_first_order(kgrid: KGrid, medium, source, sensor, *, data_cast="off",
     time_rev = use_sensor and _has(sensor, "time_reversal_boundary_data")
     record = set(["p"])
     record_given = use_sensor and _has(sensor, "record")
+    if use_sensor and _has(sensor, "record_mode"):
+        # LP/kspaceFirstOrder_inputChecking.m:360-363
+        raise ValueError("sensor.record_mode input has been deprecated. Please use the syntax "
+                         "sensor.record = {'p_rms', 'p_max', ...} to set recorded fields.")
     if record_given:
         record = set(_get(sensor, "record"))
         bad = record - RECORD_ALLOWED
@@ -892,6 +916,21 @@ def kspace_first_order(kgrid: KGrid, medium, source, sensor, *, data_cast="off",
     src_p = parse_time_source(["p"], "p_mask", "p_mode")
     u_names = ["ux", "uy", "uz"][:dim]
     src_u = parse_time_source(u_names, "u_mask", "u_mode")
+    # k-space source correction as a function of frequency
+    # (LP/kspaceFirstOrder_inputChecking.m:819-836,922-939,
+    # LP/kspaceFirstOrder_scaleSourceTerms.m:42-45,268-282): the signals
+    # times cos(2 pi f_ref dt / 2), injected without the k-space correction
+    for srcx, key in ((src_p, "p_frequency_ref"), (src_u, "u_frequency_ref")):
+        if srcx is not None and _has(source, key):
+            f_ref = float(_get(source, key))
+            if f_ref < 0:
+                raise ValueError(f"source.{key} must be nonnegative")
+            if f_ref > kgrid.k_max * float(np.min(c)) / (2 * math.pi):
+                raise ValueError(f"source.{key} is higher than the maximum frequency supported "
+                                 "by the spatial grid.")
+            srcx.mode = "additive-no-correction"
+            for nm in srcx.sigs:
+                srcx.sigs[nm] = srcx.sigs[nm] * math.cos(2 * math.pi * f_ref * dt / 2)
 
     # -- smoothing (inputChecking.m:1501-1543 for p0 with the maximum
     # restored, :1578-1600 for c and rho; 'Smooth' holds 1-3 flags assigned

@@ -255,6 +255,17 @@ def case(name, dim=2, N=None):
             ang = np.repeat((np.arange(shape[1]) % 3)[None, :] * 0.7, shape[0], 0)
             sens = {"mask": np.ones(shape), "record": ["p", "p_rms"], "directivity_angle": ang,
                     "directivity_pattern": "gradient"}
+    elif name == "source_frequency_ref":
+        # the k-space source correction at one frequency (a scaled
+        # 'additive-no-correction' source), for p and u sources
+        c, rho = hetero_medium(shape, 43)
+        med = {"sound_speed": c, "density": rho}
+        m = np.zeros(shape)
+        m[tuple(slice(n // 2 - 1, n // 2 + 1) for n in shape)] = 1
+        sig = tone(Nt, dt, 2e6, 2)
+        src = {"p": sig, "p_mask": m, "p_frequency_ref": 2e6,
+               "ux": 1e-3 * sig, "u_mask": m, "u_frequency_ref": 1.5e6}
+        sens = {"mask": _sparse_mask(shape, rng, 0.3), "record": ["p", "u_max"]}
     elif name == "alpha_filter":
         # absorption compensation: the power-law operators times a centred
         # k-space window (not symmetric for odd sizes)
@@ -319,7 +330,7 @@ CASES_2D = ["p0_lossless", "p0_hetero_absorbing", "p0_nonlinear_absorbing", "p0_
             "usrc_many", "time_reversal", "cartesian_nearest", "intensity", "u_nonstaggered", "u_split",
             "pml_outside", "pml_outside_source", "cartesian_linear", "cartesian_linear_pml_outside",
             "cuboid_corners", "cuboid_corners_pml_outside", "freq_response",
-            "directivity_pressure", "directivity_gradient", "alpha_filter"]
+            "directivity_pressure", "directivity_gradient", "alpha_filter", "source_frequency_ref"]
 # sensor directivity exists in 2-D only
 CASES_3D = [c for c in CASES_2D if not c.startswith("directivity")]
 # kspaceFirstOrder1D: every case whose inputs exist in 1-D
@@ -327,7 +338,8 @@ CASES_1D = ["p0_lossless", "p0_hetero_absorbing", "p0_nonlinear_absorbing", "p0_
             "stokes", "no_dispersion", "psrc_add", "psrc_dir", "psrc_nocorr", "psrc_many",
             "psrc_label", "usrc_add", "usrc_dir", "usrc_many", "time_reversal", "cartesian_nearest",
             "cartesian_linear", "intensity", "u_nonstaggered", "pml_outside", "cartesian_linear_pml_outside",
-            "cuboid_corners", "cuboid_corners_pml_outside", "freq_response", "alpha_filter"]
+            "cuboid_corners", "cuboid_corners_pml_outside", "freq_response", "alpha_filter",
+            "source_frequency_ref"]
 
 
 def oracle_opts(opts):

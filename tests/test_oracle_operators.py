@@ -297,3 +297,25 @@ def test_gaussian_filter_gain_on_tones(N):
     # FWHM: a tone bw/2 % away from the centre keeps half its amplitude
     half = math.exp(-((bw / 200 * freq) ** 2) / (2 * s2))
     assert abs(half - 0.5) < 1e-12
+
+
+def test_input_field_names_are_checked():
+    """checkFieldNames: a field outside the allowed list of each structure
+    raises the reference's error; sensor directivity fields exist in 2-D
+    only; sensor.record_mode is refused as deprecated."""
+    kg = ko.KGrid(24, 1e-4)
+    kg.set_time(5, 2e-8)
+    med = {"sound_speed": 1500.0, "density": 1000.0}
+    p0 = np.zeros(24)
+    p0[10] = 1
+    sens = {"mask": np.ones(24)}
+    with pytest.raises(ValueError, match="speed is not a valid field for the structure medium"):
+        ko.kspace_first_order(kg, dict(med, speed=1.0), {"p0": p0}, sens, smooth=(False,))
+    with pytest.raises(ValueError, match="uy is not a valid field for the structure source"):
+        ko.kspace_first_order(kg, med, {"p0": p0, "uy": np.zeros(5)}, sens, smooth=(False,))
+    with pytest.raises(ValueError, match="directivity_angle is not a valid field for the structure sensor"):
+        ko.kspace_first_order(kg, med, {"p0": p0}, dict(sens, directivity_angle=np.zeros(24)),
+                              smooth=(False,))
+    with pytest.raises(ValueError, match="record_mode input has been deprecated"):
+        ko.kspace_first_order(kg, med, {"p0": p0}, dict(sens, record_mode="statistics"),
+                              smooth=(False,))

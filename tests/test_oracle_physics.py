@@ -860,3 +860,44 @@ def test_alpha_filter_selects_absorbed_modes():
     assert np.max(np.abs(got - plain)) > 1e-3 * np.max(np.abs(plain))   # mode k2 was absorbed there
     ones = run(kg, dict(absorbing, alpha_filter=np.ones(N)), {"p0": f1 + f2}, sens, **kw)["p"]
     assert np.max(np.abs(ones - plain)) < 1e-12 * np.max(np.abs(plain))
+
+
+@pytest.mark.parametrize("kind", ["p", "ux"])
+def test_frequency_ref_source_matches_kspace_corrected_tone(kind):
+    """source.p_frequency_ref / u_frequency_ref replace the k-space source
+    filter cos(c k dt/2) by its value cos(w dt/2) at one frequency: for a
+    tone at that frequency the radiated wave matches the k-space corrected
+    ('additive') source to a small fraction of the 5 % the correction
+    itself makes at this coarse time step."""
+    N, dx = 256, 1e-4
+    c = 1500.0
+    dt = 0.5 * dx / c
+    Nt = 400
+    kg = ko.KGrid(N, dx)
+    kg.set_time(Nt, dt)
+    f0 = 3e6
+    t = np.arange(Nt) * dt
+    ramp = np.clip(t / (8 / f0), 0, 1)
+    sig = ramp * np.sin(2 * np.pi * f0 * t)
+    if kind == "ux":
+        sig = sig * 1e-6
+    m = np.zeros(N)
+    m[40] = 1
+    sens = {"mask": (np.arange(N) == 200).astype(float), "record": ["p"]}
+    med = {"sound_speed": c, "density": 1000.0}
+    pre = "p" if kind == "p" else "u"
+
+    def go(mode, fref=None):
+        src = {kind: sig, pre + "_mask": m, pre + "_mode": mode}
+        if fref is not None:
+            src[pre + "_frequency_ref"] = fref
+        return run(kg, med, src, sens, pml_size=20)["p"][0, -100:]
+
+    ref = go("additive")
+    amp = np.max(np.abs(ref))
+    plain = go("additive-no-correction")
+    wcorr = go("additive", f0)
+    assert np.max(np.abs(plain - ref)) > 0.03 * amp
+    assert np.max(np.abs(wcorr - ref)) < 0.1 * np.max(np.abs(plain - ref))
+    with pytest.raises(ValueError, match="maximum frequency"):
+        go("additive", 10 * f0)
