@@ -39,10 +39,10 @@ from .kgrid import kWaveGrid
 _VISUAL_OPTIONS = {"PlotSim", "PlotLayout", "PlotPML", "PlotScale", "PlotFreq", "DisplayMask",
                    "LogScale", "RecordMovie", "MovieName", "MovieArgs", "MovieProfile",
                    "CreateLog", "MeshPlot"}
-_OPTIONS = {"PMLSize", "PMLAlpha", "PMLInside", "Smooth", "DataCast", "DataRecast",
+_OPTIONS = {"PMLSize", "PMLAlpha", "PMLInside", "PMLRange", "Smooth", "DataCast", "DataRecast",
             "CartInterp", "UsekSpace", "UseSG", "ScaleSourceTerms", "Backend"} | _VISUAL_OPTIONS
 _UNSUPPORTED = {"SaveToDisk", "StreamToDisk", "SaveToDiskExit", "HDFCompressionLevel",
-                "PMLRange", "MultiAxialPMLRatio", "UseFD"}
+                "MultiAxialPMLRatio", "UseFD"}
 
 RECORD_ALLOWED = ("p", "p_max", "p_min", "p_rms", "p_max_all", "p_min_all", "p_final",
                   "u", "u_max", "u_min", "u_rms", "u_max_all", "u_min_all", "u_final",
@@ -187,6 +187,33 @@ def cart2grid(kgrid: kWaveGrid, cart_data):
     reorder_index = flat[flat != 0]
     order_index = np.argsort(reorder_index, kind="stable") + 1
     return (grid != 0), order_index, reorder_index
+
+
+def optimal_pml_size(grid_size, pml_range=(10, 40)):
+    """getOptimalPMLSize.m: for each axis the PML size in the inclusive range
+    whose grown length N + 2 pml has the smallest largest prime factor (the
+    smallest such size)."""
+    lo, hi = (int(round(v)) for v in pml_range)
+    if lo < 0 or hi < 0:
+        raise ValueError("Optional input pml_range must be positive.")
+    if hi <= lo:
+        raise ValueError("The second value for pml_range must be greater than the first.")
+
+    def largest_prime_factor(n):
+        f, last = 2, 1
+        while n > 1 and f * f <= n:
+            if n % f == 0:
+                last = f
+                n //= f
+            else:
+                f += 1
+        return max(last, n)
+
+    sizes = []
+    for N in grid_size:
+        best = min(range(lo, hi + 1), key=lambda p: (largest_prime_factor(N + 2 * p), p))
+        sizes.append(best)
+    return sizes
 
 
 def cuboid_points(kgrid: kWaveGrid, corners):
@@ -406,6 +433,14 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
     607, kspaceFirstOrder_scaleSourceTerms.m)."""
     if kgrid.dim != dim:
         raise ValueError(f"kgrid must be {dim}D for kspaceFirstOrder{dim}D")
+    if isinstance(opts.get("PMLSize"), str):
+        # kspaceFirstOrder_checkOptionalInputs.m:297-305,484-512
+        if opts["PMLSize"] != "auto":
+            raise ValueError(f"Optional input 'PMLSize' must be a 1 or {dim} element numerical array, "
+                             "or set to 'auto'.")
+        if bool(opts.get("PMLInside", True)):
+            raise ValueError("'PMLSize' set to 'auto' is only supported with 'PMLInside' set to false.")
+        opts = dict(opts, PMLSize=optimal_pml_size(kgrid.shape, opts.get("PMLRange", [10, 40])))
     if not bool(opts.get("PMLInside", True)):
         if medium is None or not _has(medium, "sound_speed"):
             raise ValueError("The field medium.sound_speed must be defined.")

@@ -619,6 +619,31 @@ def expand_matrix(mat, sizes, edge_val=None):
     return mat[np.ix_(*idx)]
 
 
+def optimal_pml_size(grid_size, pml_range=(10, 40)):
+    """L/getOptimalPMLSize.m: per axis, the PML size in pml_range (inclusive)
+    whose grown length N + 2 pml has the smallest largest prime factor (the
+    first such size)."""
+    lo, hi = int(round(pml_range[0])), int(round(pml_range[1]))
+    if lo < 0 or hi < 0:
+        raise ValueError("Optional input pml_range must be positive.")
+    if hi <= lo:
+        raise ValueError("The second value for pml_range must be greater than the first.")
+
+    def largest_factor(n):
+        best, f = 1, 2
+        while f * f <= n:
+            while n % f == 0:
+                best, n = f, n // f
+            f += 1
+        return max(best, n) if n > 1 else best
+
+    out = []
+    for N in grid_size:
+        facs = [largest_factor(N + 2 * p) for p in range(lo, hi + 1)]
+        out.append(lo + int(np.argmin(facs)))
+    return out
+
+
 def _pml_outside(kgrid, medium, source, sensor, *, smooth, cart_interp, **kw):
     """'PMLInside', false: LP/kspaceFirstOrder_inputChecking.m:1501-1560 and
     LP/kspaceFirstOrder_expandGridMatrices.m.  The time axis is set and p0
@@ -724,6 +749,7 @@ def _cut_back(out, sizes, kgrid, reorder_index=None, time_rev=False):
 
 def kspace_first_order(kgrid: KGrid, medium, source, sensor, *, data_cast="off",
                        pml_size=None, pml_alpha=None, pml_inside=True, smooth=(True, False, False),
+                       pml_range=(10, 40),
                        cart_interp="linear", use_kspace=True, use_sg=True,
                        scale_source_terms=True, data_recast=False,
                        return_state=False, fft_workers=None, step_times=None):
@@ -737,6 +763,14 @@ def kspace_first_order(kgrid: KGrid, medium, source, sensor, *, data_cast="off",
     W = FFT_WORKERS if fft_workers is None else fft_workers
     dim = kgrid.dim
     shape = kgrid.shape
+    if isinstance(pml_size, str):
+        # LP/kspaceFirstOrder_checkOptionalInputs.m:297-305,484-512
+        if pml_size != "auto":
+            raise ValueError("Optional input 'PMLSize' must be a 1 or dim element numerical "
+                             "array, or set to 'auto'.")
+        if pml_inside:
+            raise ValueError("'PMLSize' set to 'auto' is only supported with 'PMLInside' set to false.")
+        pml_size = optimal_pml_size(shape, pml_range)
     if not pml_inside:
         return _pml_outside(kgrid, medium, source, sensor, data_cast=data_cast,
                             pml_size=pml_size, pml_alpha=pml_alpha, smooth=smooth,

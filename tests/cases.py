@@ -58,6 +58,10 @@ def case(name, dim=2, N=None):
         if "pml_outside" in name:
             # the grid grows by the PML on both sides: FFT-friendly after it
             N = {1: (40,), 2: (24, 40), 3: (12, 18, 10)}[dim]
+        if name == "pml_auto":
+            # sizes with factors above 5: 'PMLSize' 'auto' picks PMLs that
+            # make the grown grid FFT-friendly
+            N = {1: (53,), 2: (37, 29), 3: (13, 11, 17)}[dim]
     shape = tuple(N)
     dx = 1e-4
     c0 = 1500.0
@@ -266,6 +270,14 @@ def case(name, dim=2, N=None):
         src = {"p": sig, "p_mask": m, "p_frequency_ref": 2e6,
                "ux": 1e-3 * sig, "u_mask": m, "u_frequency_ref": 1.5e6}
         sens = {"mask": _sparse_mask(shape, rng, 0.3), "record": ["p", "u_max"]}
+    elif name == "pml_auto":
+        c, rho = hetero_medium(shape, 47)
+        med = {"sound_speed": c, "density": rho, "alpha_coeff": 0.5, "alpha_power": 1.5}
+        src["p0"] = disc(shape, centre, 3)
+        opts["PMLInside"] = False
+        opts["PMLSize"] = "auto"
+        opts["PMLRange"] = [4, 14]
+        sens = {"mask": _sparse_mask(shape, rng, 0.3), "record": ["p", "p_final", "p_max_all"]}
     elif name == "alpha_filter":
         # absorption compensation: the power-law operators times a centred
         # k-space window (not symmetric for odd sizes)
@@ -330,7 +342,8 @@ CASES_2D = ["p0_lossless", "p0_hetero_absorbing", "p0_nonlinear_absorbing", "p0_
             "usrc_many", "time_reversal", "cartesian_nearest", "intensity", "u_nonstaggered", "u_split",
             "pml_outside", "pml_outside_source", "cartesian_linear", "cartesian_linear_pml_outside",
             "cuboid_corners", "cuboid_corners_pml_outside", "freq_response",
-            "directivity_pressure", "directivity_gradient", "alpha_filter", "source_frequency_ref"]
+            "directivity_pressure", "directivity_gradient", "alpha_filter", "source_frequency_ref",
+            "pml_auto"]
 # sensor directivity exists in 2-D only
 CASES_3D = [c for c in CASES_2D if not c.startswith("directivity")]
 # kspaceFirstOrder1D: every case whose inputs exist in 1-D
@@ -339,12 +352,12 @@ CASES_1D = ["p0_lossless", "p0_hetero_absorbing", "p0_nonlinear_absorbing", "p0_
             "psrc_label", "usrc_add", "usrc_dir", "usrc_many", "time_reversal", "cartesian_nearest",
             "cartesian_linear", "intensity", "u_nonstaggered", "pml_outside", "cartesian_linear_pml_outside",
             "cuboid_corners", "cuboid_corners_pml_outside", "freq_response", "alpha_filter",
-            "source_frequency_ref"]
+            "source_frequency_ref", "pml_auto"]
 
 
 def oracle_opts(opts):
     """Map reference option names to the oracle's keyword names."""
-    m = {"PMLSize": "pml_size", "PMLAlpha": "pml_alpha", "Smooth": "smooth",
+    m = {"PMLSize": "pml_size", "PMLAlpha": "pml_alpha", "Smooth": "smooth", "PMLRange": "pml_range",
          "CartInterp": "cart_interp", "PMLInside": "pml_inside", "UsekSpace": "use_kspace",
          "UseSG": "use_sg"}
     return {m[k]: v for k, v in opts.items() if k in m}

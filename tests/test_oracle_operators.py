@@ -319,3 +319,15 @@ def test_input_field_names_are_checked():
     with pytest.raises(ValueError, match="record_mode input has been deprecated"):
         ko.kspace_first_order(kg, med, {"p0": p0}, dict(sens, record_mode="statistics"),
                               smooth=(False,))
+
+
+def test_optimal_pml_size_known_answers():
+    """getOptimalPMLSize: N + 2 pml with the smallest largest prime factor
+    in the range, the first such size: 100 + 28 = 128 = 2^7 -> 14 (120 =
+    2^3 3 5 at 10 loses); 101: 121 = 11^2, 123 = 3 41, 125 = 5^3 -> 12 (odd
+    lengths cannot reach 3 before 243); 97: 117, 119, 121, 123, 125 -> 14;
+    101 in 13..20: 127, 129 = 3 43, 131, 133 = 7 19, 135 = 3^3 5 -> 17."""
+    assert ko.optimal_pml_size([100, 101, 97]) == [14, 12, 14]
+    assert ko.optimal_pml_size([101], (13, 20)) == [17]
+    with pytest.raises(ValueError):
+        ko.optimal_pml_size([64], (20, 10))
