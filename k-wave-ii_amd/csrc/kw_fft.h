@@ -75,6 +75,12 @@ template <int R>
 struct MaxB {
   static constexpr int v = (KW_E + R - 1) / R;
 };
+// lengths with a factor 7 run with G = ceil(N/7) threads (see kw_sim.h), so
+// a radix-7 stage has one butterfly per thread
+template <>
+struct MaxB<7> {
+  static constexpr int v = 1;
+};
 // largest number of last-stage outputs one thread can own with G = ceil(N/KW_E)
 constexpr int kMaxSlots = 10;
 static_assert(MaxB<2>::v * 2 <= kMaxSlots && MaxB<3>::v * 3 <= kMaxSlots &&
@@ -166,6 +172,43 @@ template <>
 KW_BODY void dft<5, false>(cf* v) { dft5<false>(v); }
 template <>
 KW_BODY void dft<5, true>(cf* v) { dft5<true>(v); }
+
+// radix 7 from the pairs a_k = v_k + v_(7-k), b_k = v_k - v_(7-k):
+// X_j = v_0 + sum_k cos(2 pi jk/7) a_k -+ i sum_k sin(2 pi jk/7) b_k
+template <bool INV>
+KW_BODY void dft7(cf* v) {
+  const float c1 = 0.6234898018587336f, c2 = -0.22252093395631434f, c3 = -0.9009688679024191f;
+  const float s1 = 0.7818314824680298f, s2 = 0.9749279121818236f, s3 = 0.43388373911755823f;
+  const cf a1 = cadd(v[1], v[6]), b1 = csub(v[1], v[6]);
+  const cf a2 = cadd(v[2], v[5]), b2 = csub(v[2], v[5]);
+  const cf a3 = cadd(v[3], v[4]), b3 = csub(v[3], v[4]);
+  const cf x0 = v[0];
+  auto lin = [](cf z, float p, cf a, float q, cf b, float r, cf c) {
+    return cf{z.x + p * a.x + q * b.x + r * c.x, z.y + p * a.y + q * b.y + r * c.y};
+  };
+  const cf zero{0.f, 0.f};
+  const cf m1 = lin(x0, c1, a1, c2, a2, c3, a3);
+  const cf m2 = lin(x0, c2, a1, c3, a2, c1, a3);
+  const cf m3 = lin(x0, c3, a1, c1, a2, c2, a3);
+  const cf n1 = lin(zero, s1, b1, s2, b2, s3, b3);
+  const cf n2 = lin(zero, s2, b1, -s3, b2, -s1, b3);
+  const cf n3 = lin(zero, s3, b1, -s1, b2, s2, b3);
+  // forward: X_j = m_j - i n_j, X_(7-j) = m_j + i n_j
+  const cf i1 = INV ? cmul_pi(n1) : cmul_mi(n1);
+  const cf i2 = INV ? cmul_pi(n2) : cmul_mi(n2);
+  const cf i3 = INV ? cmul_pi(n3) : cmul_mi(n3);
+  v[0] = cf{x0.x + a1.x + a2.x + a3.x, x0.y + a1.y + a2.y + a3.y};
+  v[1] = cadd(m1, i1);
+  v[6] = csub(m1, i1);
+  v[2] = cadd(m2, i2);
+  v[5] = csub(m2, i2);
+  v[3] = cadd(m3, i3);
+  v[4] = csub(m3, i3);
+}
+template <>
+KW_BODY void dft<7, false>(cf* v) { dft7<false>(v); }
+template <>
+KW_BODY void dft<7, true>(cf* v) { dft7<true>(v); }
 
 template <bool INV>
 KW_BODY void dft8(cf* v) {
@@ -265,7 +308,7 @@ struct LdsPencils {
 };
 
 // ---- runtime-length engine --------------------------------------------------
-// Any N = 2^a 3^b 5^c.  The transform is staged entirely in LDS: a strided,
+// Any N = 2^a 3^b 5^c 7^d.  The transform is staged entirely in LDS: a strided,
 // coalesced pass loads the pencil into LDS through the caller's functor, the
 // Stockham stages run from LDS to LDS, and a strided pass hands element
 // n = g + i*G to the caller as slot i.  The caller's functors are
@@ -291,6 +334,7 @@ KW_BODY void fft_stages_rt(Ctx& c, const FftPlan& pl, int g, int G, bool active,
       case 3: stage<3, NP, INV, MaxB<3>::v>(c, pl.N, Ns, G, g, active, pl.tw, lsrc, ldst); break;
       case 4: stage<4, NP, INV, MaxB<4>::v>(c, pl.N, Ns, G, g, active, pl.tw, lsrc, ldst); break;
       case 5: stage<5, NP, INV, MaxB<5>::v>(c, pl.N, Ns, G, g, active, pl.tw, lsrc, ldst); break;
+      case 7: stage<7, NP, INV, MaxB<7>::v>(c, pl.N, Ns, G, g, active, pl.tw, lsrc, ldst); break;
       default: stage<8, NP, INV, MaxB<8>::v>(c, pl.N, Ns, G, g, active, pl.tw, lsrc, ldst); break;
     }
     Ns *= R;

@@ -52,6 +52,12 @@ inline bool factorise(int N, std::vector<int>& r) {
     r.push_back(5);
     n /= 5;
   }
+  // radix 7 in the runtime engine only (pencils of such lengths run with
+  // G = ceil(N/7) threads: one radix-7 butterfly each)
+  while (n % 7 == 0) {
+    r.push_back(7);
+    n /= 7;
+  }
   // N = 1 (the y axis of a 1-D problem): no stages, the transform is the
   // identity
   return n == 1 && (N == 1 || !r.empty()) && (int)r.size() <= KW_MAX_STAGES;
@@ -339,7 +345,9 @@ struct Sim {
   void plan_launches() {
     const size_t maxl = dev_max_lds();
     const int capP = max_pencils_env();
-    auto Gof = [](int N) { return (N + KW_E - 1) / KW_E; };
+    // ceil(N/7) for lengths with a factor 7 (one radix-7 butterfly per
+    // thread; every other stage then has at most its usual count)
+    auto Gof = [](int N) { return N % 7 == 0 ? (N + 6) / 7 : (N + KW_E - 1) / KW_E; };
     // x pass: <= 256 threads, LDS P * kNPX padded pencils; the fixed-length
     // x kernels run x_threads(Nx) threads per pencil
     {
@@ -395,7 +403,7 @@ struct Sim {
     std::vector<int> r;
     if (!factorise(N, r))
       fail(KW_EUNSUP, "grid size " + std::to_string(N) +
-                          " is not a product of 2, 3 and 5 (supported FFT radices)");
+                          " is not a product of 2, 3, 5 and 7 (supported FFT radices)");
     pl.N = N;
     pl.nst = (int)r.size();
     for (size_t i = 0; i < r.size(); ++i) pl.radix[i] = r[i];

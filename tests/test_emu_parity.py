@@ -69,6 +69,10 @@ def test_emu_parity_3d(name):
     (1, "p0_hetero_absorbing", (75,)),
     (1, "psrc_many", (128,)),
     (1, "time_reversal", (270,)),
+    # lengths with a factor 7 (radix-7 stages of the runtime engine)
+    (1, "p0_hetero_absorbing", (98,)),
+    (2, "time_reversal", (49, 30)),
+    (3, "p0_nonlinear_absorbing", (14, 21, 28)),
     # the velocity split's last-axis pass on fixed-length kernels
     (2, "u_split", (30, 64)),
     (3, "u_split", (12, 16, 64)),

@@ -51,13 +51,18 @@ def test_parity_3d(name, hip_lib):
 
 @pytest.mark.parametrize("name,shape", [("p0_hetero_absorbing", (27, 25)),
                                         ("psrc_many", (45, 50)),
-                                        ("time_reversal", (64, 48))])
+                                        ("time_reversal", (64, 48)),
+                                        ("p0_hetero_absorbing", (56, 42)),
+                                        ("u_split", (28, 35)),
+                                        ("psrc_add", (112, 98))])
 def test_parity_2d_sizes(name, shape, hip_lib):
     _check(cases.case(name, 2, shape), hip_lib)
 
 
 @pytest.mark.parametrize("name,shape", [("p0_hetero_absorbing", (15, 9, 25)),
                                         ("psrc_add", (24, 30, 32)),
-                                        ("p0_nonlinear_absorbing", (32, 32, 32))])
+                                        ("p0_nonlinear_absorbing", (32, 32, 32)),
+                                        ("p0_nonlinear_absorbing", (14, 21, 28)),
+                                        ("psrc_dir", (21, 16, 14))])
 def test_parity_3d_sizes(name, shape, hip_lib):
     _check(cases.case(name, 3, shape), hip_lib)
