@@ -57,6 +57,17 @@ How each part is pinned (tests/test_oracle_*.py)
   regular grid in an implementation-defined way; the oracle and the product
   use the declared one (see grid_data_linear), so fields that are not affine
   inside a cell can differ from a MATLAB run by the choice of diagonal.
+* velocity_split: single-mode gradient fields are all compressional and
+  rotational fields all shear; an irrotational run has zero shear.
+* 1-D (kspaceFirstOrder1D): the periodic exact solution, and equality with
+  y-invariant 2-D runs for linear problems.
+* cuboid corners: per-cuboid records equal the full-grid record there.
+* gaussian_filter: gains on sampled tones (closed form).
+* directivity_filters: plane-wave closed forms, including the one-bin
+  offset of the reference's fftshift for odd sizes.
+* alpha_filter: mode selection on a periodic grid (linearity).
+* p/u_frequency_ref: a coarse-step tone against the k-space corrected source.
+* optimal_pml_size: known answers.
 """
 
 from __future__ import annotations
