@@ -27,6 +27,12 @@ def _submit(eng, cs):
 
 
 def _same(a, b):
+    if isinstance(a, list):
+        # cuboid corners: one record per cuboid
+        assert isinstance(b, list) and len(a) == len(b)
+        for x, y in zip(a, b):
+            _same(x, y)
+        return
     if not isinstance(a, dict):
         a, b = {"p": a}, {"p": b}
     assert sorted(a) == sorted(b)
@@ -63,8 +69,11 @@ MIX = [
     (3, "u_split", (15, 18, 16)),
     # Cartesian points interpolated on simplices
     (2, "cartesian_linear", (30, 32)),
-    # per-member directivity filters
+    # per-member directivity filters; host-side sensor forms
     (2, "directivity_gradient", (30, 32)),
+    (3, "cuboid_corners", (16, 15, 18)),
+    (2, "freq_response", (32, 30)),
+    (2, "alpha_filter", (27, 30)),
     # 1-D problems (one-row grids) in the same engine
     (1, "psrc_add", (96,)),
     (1, "time_reversal", (75,)),
