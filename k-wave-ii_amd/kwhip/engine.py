@@ -10,7 +10,7 @@ include/kwhip.h.
     sensor_data = eng.wait(t)           # as kspaceFirstOrder3D returns it
     eng.close()
 
-Each result equals what kspaceFirstOrder2D/3D returns for the same inputs.
+Each result equals what kspaceFirstOrder1D/2D/3D returns for the same inputs.
 """
 
 from __future__ import annotations
@@ -50,7 +50,7 @@ class Engine:
         _abi.call(self.lib.kw_engine_create, C.byref(cfg), C.byref(self.h))
 
     def submit(self, kgrid, medium, source, sensor, dim: int, *args, **kwargs) -> Ticket:
-        """Validate and pre-process as kspaceFirstOrder2D/3D, copy the problem
+        """Validate and pre-process as kspaceFirstOrder1D/2D/3D, copy the problem
         into the engine and return at once."""
         opts = _parse_options(args, kwargs)
         opts["Backend"] = self.backend

@@ -1,4 +1,4 @@
-"""kWaveGrid: the grid object of the reference, for uniform 2D/3D grids.
+"""kWaveGrid: the grid object of the reference, for uniform 1D/2D/3D grids.
 
 Mirrors +kwave/+legacy/kWaveGrid.m: constructor (:342-436), wavenumber
 vectors (makeDim, :779-797), position vectors (:494-506), t_array
