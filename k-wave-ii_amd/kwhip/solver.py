@@ -39,10 +39,10 @@ from .kgrid import kWaveGrid
 _VISUAL_OPTIONS = {"PlotSim", "PlotLayout", "PlotPML", "PlotScale", "PlotFreq", "DisplayMask",
                    "LogScale", "RecordMovie", "MovieName", "MovieArgs", "MovieProfile",
                    "CreateLog", "MeshPlot"}
-_OPTIONS = {"PMLSize", "PMLAlpha", "PMLInside", "PMLRange", "Smooth", "DataCast", "DataRecast",
+_OPTIONS = {"PMLSize", "PMLAlpha", "PMLInside", "PMLRange", "MultiAxialPMLRatio", "UseFD",
+            "Smooth", "DataCast", "DataRecast",
             "CartInterp", "UsekSpace", "UseSG", "ScaleSourceTerms", "Backend"} | _VISUAL_OPTIONS
-_UNSUPPORTED = {"SaveToDisk", "StreamToDisk", "SaveToDiskExit", "HDFCompressionLevel",
-                "MultiAxialPMLRatio", "UseFD"}
+_UNSUPPORTED = {"SaveToDisk", "StreamToDisk", "SaveToDiskExit", "HDFCompressionLevel"}
 
 RECORD_ALLOWED = ("p", "p_max", "p_min", "p_rms", "p_max_all", "p_min_all", "p_final",
                   "u", "u_max", "u_min", "u_rms", "u_max_all", "u_min_all", "u_final",
@@ -99,7 +99,28 @@ def _parse_options(args, kwargs):
             raise NotImplementedError(f"option '{k}' is not supported by kwhip")
         if k not in _OPTIONS:
             raise ValueError(f"Unknown optional input '{k}'.")
+    if "MultiAxialPMLRatio" in opts:
+        # kspaceFirstOrder_checkOptionalInputs.m:195-203: checked, used by the
+        # elastic solvers only
+        r = np.asarray(opts["MultiAxialPMLRatio"])
+        if r.size != 1 or not np.issubdtype(r.dtype, np.number) or float(r) < 0:
+            raise ValueError("Optional input 'MultiAxialPMLRatio' must be a single positive value.")
     return opts
+
+
+def _check_use_fd(dim, opts):
+    """kspaceFirstOrder_checkOptionalInputs.m:440-456: 'UseFD' exists for the
+    1-D fluid code only; false keeps the spectral gradients."""
+    if "UseFD" not in opts:
+        return
+    if dim != 1:
+        raise ValueError("Optional input 'UseFD' only supported in 1D.")
+    v = opts["UseFD"]
+    if isinstance(v, (bool, np.bool_)) and not v:
+        return
+    if v in (2, 4):
+        raise NotImplementedError("finite-difference gradients ('UseFD' 2 or 4) are not supported by kwhip")
+    raise ValueError("Optional input 'UseFD' must be set to 2, 4, or false.")
 
 
 # ---------------------------------------------------------------------------
@@ -1030,6 +1051,7 @@ def run_prepared(pp: PreparedProblem):
 
 def _solve(dim, kgrid, medium, source, sensor, args, kwargs):
     opts = _parse_options(args, kwargs)
+    _check_use_fd(dim, opts)
     pp = prepare(kgrid, medium, source, sensor, dim, opts)
     return run_prepared(pp)
 

@@ -61,3 +61,27 @@ def test_cuboid_corner_errors():
         with pytest.raises(ValueError, match=msg):
             kwhip.kspaceFirstOrder2D(kg, med, {"p0": p0}, {"mask": np.array(corners, dtype=float)},
                                      Smooth=False, Backend="emu")
+
+
+def test_options_of_other_solvers_are_checked():
+    """'MultiAxialPMLRatio' is checked and has no effect on the fluid code;
+    'UseFD' exists in 1-D only (finite differences are not built)."""
+    import kwhip
+    med = {"sound_speed": 1500.0, "density": 1000.0}
+    p0 = np.zeros(24)
+    p0[10] = 1
+    sens = {"mask": np.ones(24)}
+    a = kwhip.kspaceFirstOrder1D(_grid(), med, {"p0": p0}, sens, Smooth=False, Backend="emu",
+                                 PMLSize=4)
+    b = kwhip.kspaceFirstOrder1D(_grid(), med, {"p0": p0}, sens, Smooth=False, Backend="emu",
+                                 PMLSize=4, MultiAxialPMLRatio=0.1, UseFD=False)
+    assert np.array_equal(a, b)
+    with pytest.raises(ValueError, match="single positive value"):
+        kwhip.kspaceFirstOrder1D(_grid(), med, {"p0": p0}, sens, Backend="emu", MultiAxialPMLRatio=-1)
+    with pytest.raises(NotImplementedError):
+        kwhip.kspaceFirstOrder1D(_grid(), med, {"p0": p0}, sens, Backend="emu", UseFD=4)
+    kg = kwhip.kWaveGrid(16, 1e-4, 16, 1e-4)
+    kg.setTime(5, 2e-8)
+    with pytest.raises(ValueError, match="only supported in 1D"):
+        kwhip.kspaceFirstOrder2D(kg, med, {"p0": np.ones((16, 16))}, {"mask": np.ones((16, 16))},
+                                 Backend="emu", UseFD=2)
