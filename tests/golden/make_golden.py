@@ -11,6 +11,7 @@ of their flattened (column-major) form, with k stored as ``stride_<name>``,
 to keep the files small.
 
 Run from the repository root:  python tests/golden/make_golden.py
+(--missing writes only the fixtures that do not exist yet)
 """
 
 import sys
@@ -34,6 +35,12 @@ GOLDEN = [
     (3, "psrc_label", None),
     (3, "usrc_many", None),
     (3, "stokes", (16, 12, 20)),
+    (1, "p0_hetero_absorbing", None),
+    (1, "psrc_add", None),
+    (1, "time_reversal", None),
+    (2, "directivity_pressure", None),
+    (2, "cartesian_linear", None),
+    (3, "u_split", None),
 ]
 
 
@@ -61,7 +68,10 @@ def fixture_name(dim, name, shape):
 
 
 def main():
+    missing_only = "--missing" in sys.argv[1:]
     for dim, name, shape in GOLDEN:
+        if missing_only and (HERE / fixture_name(dim, name, shape)).exists():
+            continue
         cs = cases.case(name, dim, shape)
         out = cases.run_oracle(cs, "off")
         if not isinstance(out, dict):
