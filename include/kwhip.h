@@ -156,6 +156,15 @@ typedef struct kw_problem {
    * k-space order multiplying both power-law absorption operators
    * (kspaceFirstOrder_createAbsorptionVariables.m:56-76), or NULL */
   const double* alpha_filter;
+  /* cuboid-corners sensor (the native contract's sensor_mask_corners,
+   * kspaceFirstOrder_saveToDisk.m, kspaceFirstOrder_inputChecking.m:
+   * 1610-1639): sensor_num_cuboids columns of [x1 y1 z1 x2 y2 z2] (2*dim
+   * rows, 1-based, inclusive), column-major; the sensor points are the
+   * cuboids' grid points in column-major order, cuboid after cuboid, and
+   * sensor_num_points must equal their total.  sensor_mask_index must then
+   * be NULL. */
+  const uint64_t* sensor_mask_corners;
+  int64_t sensor_num_cuboids;
 } kw_problem;
 
 typedef struct kw_sizes {

@@ -40,7 +40,7 @@ using Clock = std::chrono::steady_clock;
 struct ProblemCopy {
   kw_problem pb{};
   std::vector<double> c0, rho0, rho0_sg[3], bona, alpha, alpha_filter, p0, tr, bc, dir_angle;
-  std::vector<uint64_t> sens, tri;
+  std::vector<uint64_t> sens, tri, corners;
   struct Src {
     std::vector<uint64_t> index, sig_index;
     std::vector<double> input;
@@ -68,6 +68,12 @@ struct ProblemCopy {
     if (in.sensor_mask_index && in.sensor_num_points > 0) {
       sens.assign(in.sensor_mask_index, in.sensor_mask_index + in.sensor_num_points);
       pb.sensor_mask_index = sens.data();
+    }
+    pb.sensor_mask_corners = nullptr;
+    if (in.sensor_mask_corners && in.sensor_num_cuboids > 0) {
+      corners.assign(in.sensor_mask_corners,
+                     in.sensor_mask_corners + in.sensor_num_cuboids * 2 * in.dim);
+      pb.sensor_mask_corners = corners.data();
     }
     pb.sensor_tri = nullptr;
     pb.sensor_bc = nullptr;

@@ -107,13 +107,36 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream) {
   // rho_y stay zero, and the 2-D update is the 1-D one; the kernels see
   // P.dim = 1 where the density enters as a single component (p0, time
   // reversal) and in the sensor records.
-  kw_problem row;
+  kw_problem row = *pr_in;
   const kw_problem* pr = pr_in;
+  // cuboid corners become the index list of their points
+  std::vector<uint64_t> cuboid_idx;
+  if (pr_in->sensor_mask_corners && pr_in->sensor_num_cuboids > 0) {
+    if (pr_in->sensor_mask_index) fail(KW_EINVAL, "give sensor_mask_index or sensor_mask_corners");
+    const int64_t Nd[3] = {pr_in->Nx, pdim >= 2 ? pr_in->Ny : 1, pdim == 3 ? pr_in->Nz : 1};
+    const uint64_t* c = pr_in->sensor_mask_corners;
+    for (int64_t q = 0; q < pr_in->sensor_num_cuboids; ++q) {
+      int64_t lo[3] = {1, 1, 1}, hi[3] = {1, 1, 1};
+      for (int a = 0; a < pdim; ++a) {
+        lo[a] = (int64_t)c[q * 2 * pdim + a];
+        hi[a] = (int64_t)c[q * 2 * pdim + pdim + a];
+        if (hi[a] < lo[a]) fail(KW_EINVAL, "sensor.mask cuboid corners must be defined as [x1, ..., x2, ...].' where x2 => x1, etc.");
+        if (lo[a] < 1 || hi[a] > Nd[a]) fail(KW_EINVAL, "sensor.mask cuboid corners must be within the grid.");
+      }
+      for (int64_t z = lo[2]; z <= hi[2]; ++z)
+        for (int64_t y = lo[1]; y <= hi[1]; ++y)
+          for (int64_t x = lo[0]; x <= hi[0]; ++x)
+            cuboid_idx.push_back((uint64_t)(x + Nd[0] * ((y - 1) + Nd[1] * (z - 1))));
+    }
+    if ((int64_t)cuboid_idx.size() != pr_in->sensor_num_points)
+      fail(KW_EINVAL, "sensor_num_points must equal the number of points of the cuboids");
+    row.sensor_mask_index = cuboid_idx.data();
+    pr = &row;
+  }
   if (pdim == 1) {
     if (pr_in->Ny != 1 || pr_in->Nz != 1) fail(KW_EINVAL, "Ny and Nz must be 1 in 1D");
     if (pr_in->u_source[1].flag > 0 || pr_in->u_source[2].flag > 0)
       fail(KW_EINVAL, "a 1D simulation takes the velocity source ux only");
-    row = *pr_in;
     row.dim = 2;
     row.dy = pr_in->dx;
     row.pml_size[1] = row.pml_size[2] = 0;

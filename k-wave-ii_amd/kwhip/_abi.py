@@ -83,6 +83,7 @@ class Problem(C.Structure):
         ("sensor_tri", c_u64_p), ("sensor_bc", c_double_p),
         ("directivity_angle", c_double_p), ("directivity_pattern", C.c_int32),
         ("directivity_size", C.c_double), ("alpha_filter", c_double_p),
+        ("sensor_mask_corners", c_u64_p), ("sensor_num_cuboids", C.c_int64),
     ]
 
 

@@ -847,6 +847,11 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
             tri, bc = info.interp
             pr.sensor_tri = _abi.uptr(pp.hold(np.ascontiguousarray(tri.ravel(order="F").astype(np.uint64))))
             pr.sensor_bc = _abi.dptr(pp.hold(np.ascontiguousarray(bc.ravel(order="F"))))
+        elif info.cuboids is not None:
+            # the corners themselves: the library enumerates their points
+            cor = np.ascontiguousarray(info.cuboids.ravel(order="F").astype(np.uint64))
+            pr.sensor_mask_corners = _abi.uptr(pp.hold(cor))
+            pr.sensor_num_cuboids = info.cuboids.shape[1]
         elif not full:
             pr.sensor_mask_index = _abi.uptr(pp.hold(np.ascontiguousarray(sens_idx.astype(np.uint64) + 1)))
         if _has(sensor, "directivity_angle") and not info.time_rev:

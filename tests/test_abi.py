@@ -151,3 +151,42 @@ def test_interpolated_sensor_points_are_checked():
     idx = np.array([1, 2], dtype=np.uint64)
     prob.sensor_mask_index = _abi.uptr(idx)
     assert create() == _abi.KW_EINVAL and b"NULL" in buf.value
+
+
+def test_cuboid_corners_are_checked():
+    """sensor_mask_corners: the point count must match the cuboids, the
+    corners lie in the grid with x2 >= x1, and an index list is not given
+    as well."""
+    import numpy as np
+    from kwhip import _abi
+
+    lib = _abi.load("emu")
+    prob = _abi.Problem()
+    prob.dim, prob.Nx, prob.Ny, prob.Nz, prob.Nt = 2, 16, 16, 1, 3
+    prob.dx = prob.dy = 1e-4
+    prob.dt, prob.c_ref, prob.c0_scalar, prob.rho0_scalar = 1e-8, 1500.0, 1500.0, 1000.0
+    prob.pml_size[0] = prob.pml_size[1] = 2
+    prob.use_kspace = prob.use_sg = 1
+    prob.record_flags = _abi.REC["p"]
+    cor = np.array([3, 4, 5, 6, 8, 8, 9, 9], dtype=np.uint64)   # 3x3 and 2x2 points
+    prob.sensor_mask_corners = _abi.uptr(cor)
+    prob.sensor_num_cuboids = 2
+    h = C.c_void_p()
+    buf = C.create_string_buffer(256)
+
+    def create():
+        rc = lib.kw_sim_create(C.byref(prob), C.byref(h), buf, 256)
+        if rc == 0:
+            lib.kw_sim_destroy(h)
+        return rc
+
+    prob.sensor_num_points = 12
+    assert create() == _abi.KW_EINVAL and b"number of points" in buf.value
+    prob.sensor_num_points = 13
+    assert create() == 0
+    bad = np.array([5, 4, 3, 6, 8, 8, 9, 9], dtype=np.uint64)
+    prob.sensor_mask_corners = _abi.uptr(bad)
+    assert create() == _abi.KW_EINVAL and b"x2 => x1" in buf.value
+    out = np.array([3, 4, 5, 17, 8, 8, 9, 9], dtype=np.uint64)
+    prob.sensor_mask_corners = _abi.uptr(out)
+    assert create() == _abi.KW_EINVAL and b"within the grid" in buf.value
