@@ -28,7 +28,7 @@
 extern "C" {
 #endif
 
-#define KW_ABI_VERSION 1
+#define KW_ABI_VERSION 2
 
 /* error codes */
 #define KW_OK 0
@@ -111,7 +111,12 @@ typedef struct kw_problem {
   double alpha_coeff_scalar;
   double alpha_power;
   int32_t alpha_mode;        /* 0 both terms, 1 'no_absorption', 2 'no_dispersion' */
-  double alpha_sign[2];      /* medium.alpha_sign; 0 means +1 */
+  /* medium.alpha_sign (kspaceFirstOrder_createAbsorptionVariables.m:78-81):
+   * with alpha_sign_set = 1, tau is multiplied by sign(alpha_sign[0]) and
+   * eta by sign(alpha_sign[1]), MATLAB's sign (-1, 0 or +1, so a 0 removes
+   * the term); with alpha_sign_set = 0 both multipliers are +1 */
+  double alpha_sign[2];
+  int32_t alpha_sign_set;
 
   int64_t pml_size[3];
   double pml_alpha[3];

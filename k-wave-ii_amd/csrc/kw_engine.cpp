@@ -131,6 +131,7 @@ void check_shape(const kw_problem* p) {
     fail(KW_EUNSUP, "grid too large");
   if (p->Nt < 1) fail(KW_EINVAL, "Nt must be positive");
   if (p->sensor_num_points < 0) fail(KW_EINVAL, "negative number of sensor points");
+  if (p->sensor_num_cuboids < 0) fail(KW_EINVAL, "negative number of sensor cuboids");
   const kw_time_source* s[4] = {&p->p_source, &p->u_source[0], &p->u_source[1], &p->u_source[2]};
   for (auto* t : s)
     if (t->flag > 0 && (t->num_points < 0 || t->num_series < 0))

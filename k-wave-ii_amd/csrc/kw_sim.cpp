@@ -111,25 +111,50 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream) {
   const kw_problem* pr = pr_in;
   // cuboid corners become the index list of their points
   std::vector<uint64_t> cuboid_idx;
+  if (pr_in->sensor_num_cuboids < 0) fail(KW_EINVAL, "negative number of sensor cuboids");
   if (pr_in->sensor_mask_corners && pr_in->sensor_num_cuboids > 0) {
     if (pr_in->sensor_mask_index) fail(KW_EINVAL, "give sensor_mask_index or sensor_mask_corners");
+    // the reference takes cuboid corners for forward simulations with
+    // uninterpolated records only (kspaceFirstOrder_inputChecking.m:516-576)
+    if (pr_in->time_reversal_boundary_data)
+      fail(KW_EINVAL, "time reversal takes a binary sensor mask, not cuboid corners");
+    if (pr_in->sensor_tri || pr_in->sensor_bc)
+      fail(KW_EINVAL, "cuboid corners cannot be combined with interpolated Cartesian points");
+    if (pr_in->directivity_angle)
+      fail(KW_EINVAL, "sensor directivity needs a binary sensor mask");
     const int64_t Nd[3] = {pr_in->Nx, pdim >= 2 ? pr_in->Ny : 1, pdim == 3 ? pr_in->Nz : 1};
     const uint64_t* c = pr_in->sensor_mask_corners;
+    // check every cuboid and count its points before enumerating any, so a
+    // corner list that does not match sensor_num_points fails without a
+    // large host allocation
+    int64_t total = 0;
+    for (int64_t q = 0; q < pr_in->sensor_num_cuboids; ++q) {
+      int64_t cnt = 1;
+      for (int a = 0; a < pdim; ++a) {
+        const int64_t lo = (int64_t)c[q * 2 * pdim + a];
+        const int64_t hi = (int64_t)c[q * 2 * pdim + pdim + a];
+        if (hi < lo) fail(KW_EINVAL, "sensor.mask cuboid corners must be defined as [x1, ..., x2, ...].' where x2 => x1, etc.");
+        if (lo < 1 || hi > Nd[a]) fail(KW_EINVAL, "sensor.mask cuboid corners must be within the grid.");
+        cnt *= hi - lo + 1;
+      }
+      total += cnt;
+      if (total > pr_in->sensor_num_points)
+        fail(KW_EINVAL, "sensor_num_points must equal the number of points of the cuboids");
+    }
+    if (total != pr_in->sensor_num_points)
+      fail(KW_EINVAL, "sensor_num_points must equal the number of points of the cuboids");
+    cuboid_idx.reserve((size_t)total);
     for (int64_t q = 0; q < pr_in->sensor_num_cuboids; ++q) {
       int64_t lo[3] = {1, 1, 1}, hi[3] = {1, 1, 1};
       for (int a = 0; a < pdim; ++a) {
         lo[a] = (int64_t)c[q * 2 * pdim + a];
         hi[a] = (int64_t)c[q * 2 * pdim + pdim + a];
-        if (hi[a] < lo[a]) fail(KW_EINVAL, "sensor.mask cuboid corners must be defined as [x1, ..., x2, ...].' where x2 => x1, etc.");
-        if (lo[a] < 1 || hi[a] > Nd[a]) fail(KW_EINVAL, "sensor.mask cuboid corners must be within the grid.");
       }
       for (int64_t z = lo[2]; z <= hi[2]; ++z)
         for (int64_t y = lo[1]; y <= hi[1]; ++y)
           for (int64_t x = lo[0]; x <= hi[0]; ++x)
             cuboid_idx.push_back((uint64_t)(x + Nd[0] * ((y - 1) + Nd[1] * (z - 1))));
     }
-    if ((int64_t)cuboid_idx.size() != pr_in->sensor_num_points)
-      fail(KW_EINVAL, "sensor_num_points must equal the number of points of the cuboids");
     row.sensor_mask_index = cuboid_idx.data();
     pr = &row;
   }
@@ -297,8 +322,10 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream) {
   }
   if (P.eos != EOS_LOSSLESS) {
     // createAbsorptionVariables.m:29-100 with db2neper.m
-    const double s_tau = pr->alpha_sign[0] < 0 ? -1.0 : 1.0;
-    const double s_eta = pr->alpha_sign[1] < 0 ? -1.0 : 1.0;
+    // MATLAB sign(): -1, 0 or +1 (a zero sign removes the term)
+    auto msign = [](double v) { return v > 0 ? 1.0 : (v < 0 ? -1.0 : 0.0); };
+    const double s_tau = pr->alpha_sign_set ? msign(pr->alpha_sign[0]) : 1.0;
+    const double s_eta = pr->alpha_sign_set ? msign(pr->alpha_sign[1]) : 1.0;
     const bool no_abs = P.eos == EOS_ABSORBING && pr->alpha_mode == 1;
     const bool no_disp = P.eos == EOS_ABSORBING && pr->alpha_mode == 2;
     const double conv = 100.0 * std::pow(1e-6 / (2.0 * kPi), y) / (20.0 * std::log10(std::exp(1.0)));
@@ -353,6 +380,15 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream) {
       if (use_sg) {
         p = ik * std::exp(std::complex<double>(0.0, k * d[a] / 2));
         m = ik * std::exp(std::complex<double>(0.0, -k * d[a] / 2));
+      }
+      if (Nd[a] % 2 == 0 && i == Nd[a] / 2) {
+        // the Nyquist bin is its own mirror: real(ifftn(D .* F)) keeps only
+        // the real part of D there (the Hermitian part of the operator), and
+        // the half-spectrum transforms, which assume Hermitian spectra, are
+        // exact with it.  Staggered operators are real there already; the
+        // unstaggered i k ('UseSG' false) becomes 0.
+        p = std::complex<double>(p.real(), 0.0);
+        m = std::complex<double>(m.real(), 0.0);
       }
       dp[i] = cf{(float)p.real(), (float)p.imag()};
       dn[i] = cf{(float)m.real(), (float)m.imag()};

@@ -1,13 +1,15 @@
 """ctypes view of the C ABI in include/kwhip.h.
 
 The product library is ``libkwhip.so`` (HIP, gfx950).  ``libkwhip_emu.so`` is
-the same host code over the CPU lane emulator; it is loaded only when a caller
-asks for it explicitly (``backend="emu"``), which only the test-suite does.
-There is no silent fallback: if the HIP library is missing, loading fails.
+the same host code over the CPU lane emulator.  The public API always runs
+the HIP library; the test-suite alone switches to the emulator, with the
+private ``_backend_override`` context.  There is no silent fallback: if the
+HIP library is missing, loading fails.
 """
 
 from __future__ import annotations
 
+import contextlib
 import ctypes as C
 import os
 from pathlib import Path
@@ -68,6 +70,7 @@ class Problem(C.Structure):
         ("alpha_power", C.c_double),
         ("alpha_mode", C.c_int32),
         ("alpha_sign", C.c_double * 2),
+        ("alpha_sign_set", C.c_int32),
         ("pml_size", C.c_int64 * 3),
         ("pml_alpha", C.c_double * 3),
         ("use_kspace", C.c_int32),
@@ -137,6 +140,32 @@ class KWError(RuntimeError):
 
 
 _libs: dict[str, C.CDLL] = {}
+_active = "hip"
+
+
+def active_backend() -> str:
+    """The library the public API runs: the HIP library, unless the
+    test-suite has switched to the emulator build."""
+    return _active
+
+
+@contextlib.contextmanager
+def _backend_override(name: str):
+    """Test harness only: run the public API on another build of the library
+    (``"emu"``: the CPU lane emulator) inside the ``with`` block."""
+    global _active
+    if name not in LIB_NAMES:
+        raise ValueError(name)
+    prev, _active = _active, name
+    try:
+        yield
+    finally:
+        _active = prev
+
+
+def host_threads() -> int:
+    """Host threads for the host-side precompute (FFT workers)."""
+    return max(1, min(16, os.cpu_count() or 1))
 
 
 def library_path(backend: str = "hip") -> Path:
@@ -178,7 +207,7 @@ def load(backend: str = "hip") -> C.CDLL:
     lib.kw_cancel.argtypes = [C.c_void_p, C.c_int64] + err
     lib.kw_engine_get_stats.argtypes = [C.c_void_p, P(EngineStats)] + err
     lib.kw_engine_destroy.argtypes = [C.c_void_p]
-    if lib.kw_abi_version() != 1:
+    if lib.kw_abi_version() != 2:
         raise ImportError("kwhip ABI version mismatch")
     _libs[backend] = lib
     return lib

@@ -41,10 +41,8 @@ class Ticket:
 
 
 class Engine:
-    def __init__(self, device_bytes: int = 0, steps_per_round: int = 0, backend: str = "hip",
-                 admit_threads: int = 0):
-        self.backend = backend
-        self.lib = _abi.load(backend)
+    def __init__(self, device_bytes: int = 0, steps_per_round: int = 0, admit_threads: int = 0):
+        self.lib = _abi.load(_abi.active_backend())
         cfg = _abi.EngineConfig(int(device_bytes), int(steps_per_round), int(admit_threads))
         self.h = C.c_void_p()
         _abi.call(self.lib.kw_engine_create, C.byref(cfg), C.byref(self.h))
@@ -53,7 +51,6 @@ class Engine:
         """Validate and pre-process as kspaceFirstOrder1D/2D/3D, copy the problem
         into the engine and return at once."""
         opts = _parse_options(args, kwargs)
-        opts["Backend"] = self.backend
         pp = prepare(kgrid, medium, source, sensor, dim, opts)
         tid = C.c_int64()
         _abi.call(self.lib.kw_submit, self.h, C.byref(pp.prob), C.byref(tid))

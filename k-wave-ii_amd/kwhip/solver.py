@@ -28,6 +28,7 @@ from __future__ import annotations
 
 import ctypes as C
 import math
+import warnings
 from types import SimpleNamespace
 
 import numpy as np
@@ -41,7 +42,7 @@ _VISUAL_OPTIONS = {"PlotSim", "PlotLayout", "PlotPML", "PlotScale", "PlotFreq", 
                    "CreateLog", "MeshPlot"}
 _OPTIONS = {"PMLSize", "PMLAlpha", "PMLInside", "PMLRange", "MultiAxialPMLRatio", "UseFD",
             "Smooth", "DataCast", "DataRecast",
-            "CartInterp", "UsekSpace", "UseSG", "ScaleSourceTerms", "Backend"} | _VISUAL_OPTIONS
+            "CartInterp", "UsekSpace", "UseSG", "ScaleSourceTerms"} | _VISUAL_OPTIONS
 _UNSUPPORTED = {"SaveToDisk", "StreamToDisk", "SaveToDiskExit", "HDFCompressionLevel"}
 
 RECORD_ALLOWED = ("p", "p_max", "p_min", "p_rms", "p_max_all", "p_min_all", "p_final",
@@ -164,20 +165,25 @@ def blackman_window(grid_size) -> np.ndarray:
     return np.abs(win[sl])
 
 
-def smooth(mat, restore_max=False, backend="hip"):
-    """+kwave/+legacy/smooth.m: filter with the rotated Blackman window in
-    k-space (on the device), optionally restoring the maximum magnitude."""
-    lib = _abi.load(backend)
+def smooth(mat, restore_max=False):
+    """+kwave/+legacy/smooth.m: real(ifftn(fftn(mat) .* ifftshift(win))) with
+    the rotated Blackman window, optionally restoring the maximum magnitude.
+
+    This is input preparation, done in double precision on the host as the
+    reference's MATLAB precompute does before it hands the smoothed p0 (and
+    c, rho) to its native engines (kspaceFirstOrder_inputChecking.m:
+    1501-1543); the values are rounded to single once, when the device
+    copies them.  ``kw_smooth`` in the C ABI is the single-precision device
+    version of the same filter for callers of the library."""
+    import scipy.fft as sfft
+
     mat = np.asarray(mat, dtype=np.float64)
     if mat.ndim not in (1, 2, 3):
         raise ValueError("smooth supports 1D, 2D and 3D grids")
     win = blackman_window(mat.shape)
-    N = (C.c_int64 * 3)(*(list(mat.shape) + [1] * (3 - mat.ndim)))
-    src = _abi.fortran_flat(mat)
-    w = _abi.fortran_flat(win)
-    out = np.empty_like(src)
-    _abi.call(lib.kw_smooth, C.c_int32(mat.ndim), N, _abi.dptr(src), _abi.dptr(w), _abi.dptr(out))
-    res = out.reshape(mat.shape, order="F")
+    workers = _abi.host_threads()
+    res = np.real(sfft.ifftn(sfft.fftn(mat, workers=workers) * np.fft.ifftshift(win),
+                             workers=workers))
     if restore_max:
         res = (np.max(np.abs(mat)) / np.max(np.abs(res))) * res
     return res
@@ -214,7 +220,8 @@ def optimal_pml_size(grid_size, pml_range=(10, 40)):
     """getOptimalPMLSize.m: for each axis the PML size in the inclusive range
     whose grown length N + 2 pml has the smallest largest prime factor (the
     smallest such size)."""
-    lo, hi = (int(round(v)) for v in pml_range)
+    # MATLAB round: halves away from zero (10.5 -> 11)
+    lo, hi = (int(math.copysign(math.floor(abs(float(v)) + 0.5), float(v))) for v in pml_range)
     if lo < 0 or hi < 0:
         raise ValueError("Optional input pml_range must be positive.")
     if hi <= lo:
@@ -322,6 +329,51 @@ def grid_data_linear(kgrid: kWaveGrid, cart_data):
     return tri + 1, bc
 
 
+def check_stability(kgrid: kWaveGrid, c, c_ref, medium):
+    """+kwave/+legacy/checkStability.m:60-144: the largest time step for
+    which the scheme is stable.  Lossless media: unconditionally stable
+    (inf) when c_ref >= max(c), else 2/(c_ref k_max) asin(c_ref/max(c)).
+    Absorbing media: the estimate with kappa = 1, refined by the fixed-point
+    iteration on kappa(dt) = sinc(c_ref k_max dt/2).  k_max is the largest
+    wavenumber magnitude on the grid."""
+    vecs = [kgrid.kx_vec, kgrid.ky_vec, kgrid.kz_vec][:kgrid.dim]
+    kmax = math.sqrt(sum(float(np.max(np.abs(v))) ** 2 for v in vecs))
+    c_max = float(np.max(c))
+    alpha = _get(medium, "alpha_coeff")
+    if alpha is None or np.all(np.asarray(alpha) == 0):
+        if c_ref >= c_max:
+            return math.inf
+        return 2.0 / (c_ref * kmax) * math.asin(c_ref / c_max)
+    mode = _get(medium, "alpha_mode")
+    y = 2.0 if mode == "stokes" else float(np.asarray(_get(medium, "alpha_power")))
+    a_np = 100.0 * np.asarray(alpha, dtype=np.float64) * (1e-6 / (2 * math.pi)) ** y / \
+        (20.0 * math.log10(math.e))
+    tau = 0.0 if mode == "no_absorption" else -2.0 * a_np * c ** (y - 1)
+    eta = 0.0 if mode == "no_dispersion" else 2.0 * a_np * c ** y * math.tan(math.pi * y / 2)
+    tau_min, eta_min = float(np.min(tau)), float(np.min(eta))
+    temp1 = c_max * tau_min * kmax ** (y - 1)
+    temp2 = 1.0 - eta_min * kmax ** (y - 1)
+    dt = (temp1 + math.sqrt(temp1 ** 2 + 4 * temp2)) / (temp2 * kmax * c_max)
+
+    def kappa(t):
+        x = c_ref * kmax * t / 2
+        return 1.0 if x == 0 else math.sin(x) / x
+
+    def func(t):
+        t3 = c_max * tau_min * kappa(t) * kmax ** (y - 1)
+        return (t3 + math.sqrt(t3 ** 2 + 4 * temp2)) / (temp2 * kmax * kappa(t) * c_max)
+
+    # the reference iterates until the step changes by at most 1e-12 s (the
+    # iteration converges slowly when absorption is weak); the cap only
+    # guards against a non-converging input
+    dt_old = 0.0
+    for _ in range(1000000):
+        if abs(dt - dt_old) <= 1e-12:
+            break
+        dt_old, dt = dt, func(dt)
+    return dt
+
+
 def _find(mask) -> np.ndarray:
     """Column-major linear indices of the nonzero cells of a grid (MATLAB's
     find).  Sparse masks are searched in memory order and the indices
@@ -379,7 +431,6 @@ def _pml_outside(kgrid, medium, source, sensor, dim, opts):
     _final outputs are cut back to it) and the Cartesian reorder index."""
     pml_size = opts.get("PMLSize", 20 if dim <= 2 else 10)
     sizes = [int(v) for v in np.broadcast_to(np.asarray(pml_size, dtype=np.int64), (dim,))]
-    backend = opts.get("Backend", "hip")
     if isinstance(kgrid.Nt, str) or isinstance(kgrid.dt, str):
         if sensor is not None and _has(sensor, "time_reversal_boundary_data"):
             raise ValueError("kgrid.t_array must be explicitly defined for time reversal")
@@ -404,7 +455,7 @@ def _pml_outside(kgrid, medium, source, sensor, dim, opts):
             if p0.shape != kgrid.shape:
                 raise ValueError("source.p0 must be the same size as the computational grid.")
             if sm[0]:
-                p0 = smooth(p0, True, backend=backend)  # before the expansion
+                p0 = smooth(p0, True)  # before the expansion
             src["p0"] = _expand(p0, sizes, False)
         for nm in ("p_mask", "u_mask"):
             if nm in src:
@@ -472,7 +523,6 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
         if reorder_index is not None:
             pp.info.reorder_index = reorder_index
         return pp
-    backend = opts.get("Backend", "hip")
     pp = PreparedProblem()
     pr = pp.prob
     shape = kgrid.shape
@@ -526,6 +576,10 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
             raise ValueError("kgrid.t_array must be explicitly defined for time reversal")
         kgrid.makeTime(c, 0.3)
     Nt, dt = int(kgrid.Nt), float(kgrid.dt)
+    # kspaceFirstOrder_inputChecking.m:1270-1281: a warning, not an error
+    if dt > check_stability(kgrid, c, c_ref, medium):
+        warnings.warn("WARNING: time step may be too large for a stable simulation.",
+                      RuntimeWarning, stacklevel=3)
 
     # -- options --------------------------------------------------------------
     pml_size = opts.get("PMLSize", 20 if dim <= 2 else 10)
@@ -692,11 +746,11 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
 
     # -- smoothing (inputChecking.m:1501-1600) ---------------------------------
     if p0 is not None and smooth_p0:
-        p0 = smooth(p0, True, backend=backend)
+        p0 = smooth(p0, True)
     if smooth_c0 and c.size > 1:
-        c = smooth(c, False, backend=backend)
+        c = smooth(c, False)
     if smooth_rho0 and rho.size > 1:
-        rho = smooth(rho, False, backend=backend)
+        rho = smooth(rho, False)
 
     # -- source scaling (kspaceFirstOrder_scaleSourceTerms.m:43-391) ----------
     scale = bool(opts.get("ScaleSourceTerms", True))
@@ -808,9 +862,10 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
             sg = np.asarray(_get(medium, "alpha_sign"), dtype=np.float64).ravel()
             if sg.size != 2:
                 raise ValueError("medium.alpha_sign must be given as a 2 element numerical array")
-            pr.alpha_sign[0], pr.alpha_sign[1] = float(np.sign(sg[0]) or 1), float(np.sign(sg[1]) or 1)
-        else:
-            pr.alpha_sign[0], pr.alpha_sign[1] = 1.0, 1.0
+            # sign(alpha_sign) multiplies tau and eta, MATLAB's sign(0) = 0
+            # included (kspaceFirstOrder_createAbsorptionVariables.m:78-81)
+            pr.alpha_sign[0], pr.alpha_sign[1] = float(np.sign(sg[0])), float(np.sign(sg[1]))
+            pr.alpha_sign_set = 1
     for ax in range(dim):
         pr.pml_size[ax] = int(pml_size[ax])
         pr.pml_alpha[ax] = float(pml_alpha[ax])
@@ -877,7 +932,7 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
     info.n_sensor = n_sens if info.use_sensor else 0
     info.n_rec = (Nt - record_start + 1) if (info.use_sensor and not info.time_rev) else 0
     info.data_recast = bool(opts.get("DataRecast", False))
-    info.backend = backend
+    info.backend = _abi.active_backend()
     return pp
 
 

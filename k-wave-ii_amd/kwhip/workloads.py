@@ -11,6 +11,12 @@
   (seeds 0 and 1), power-law absorption 0.75 dB/(MHz^1.5 cm), PML 10, a 1 MHz
   10-cycle Hann-windowed 1 Pa tone burst on a disc of radius 30 points in the
   plane z = PML + 2, CFL 0.3, 1000 steps, full-grid p_max and p_rms.
+* ``cfg3``: 3D N^3 (N = 384 in the precision study), the cfg2 medium recipe
+  with B/A = 6, a 0.5 MHz 5-cycle Hann-windowed 1 MPa tone burst on a
+  spherical-cap bowl (radius of curvature 150 points, aperture 120 points
+  at N = 384, scaled with N) whose rear sits on the axis at z = PML + 2,
+  facing +z, CFL 0.3, 2000 steps; the sensor is a 64 x 64 plane through the
+  bowl's geometric focus, recording p (and p_max, p_rms).
 * ``batch_tr_problem``: one problem of the batched photoacoustic workload:
   a 2D grid from 256^2 to 1024^2 or a 3D grid from 64^3 to 192^3 (sizes
   2^a 3^b 5^c), homogeneous c0 ~ U[1450, 1600] m/s, p0 = 5-20 random discs
@@ -96,6 +102,69 @@ def cfg2(N=256, Nt=1000, pml=10, src_radius=30, seeds=(0, 1)):
     sensor = {"mask": np.ones(shape, dtype=bool), "record": ["p_max", "p_rms"]}
     opts = {"PMLSize": pml, "PMLAlpha": 2.0}
     return kg, medium, source, sensor, opts
+
+
+def bowl_mask(shape, apex, radius, diameter, focus):
+    """Binary spherical cap ("bowl") on a grid: the cells within half a cell
+    of the sphere of the given radius of curvature whose centre lies at
+    `radius` from `apex` towards `focus`, limited to the cap of aperture
+    `diameter` around the apex (0-based cell coordinates, lengths in
+    cells)."""
+    apex = np.asarray(apex, dtype=np.float64)
+    axis = np.asarray(focus, dtype=np.float64) - apex
+    axis = axis / np.linalg.norm(axis)
+    centre = apex + radius * axis
+    half = diameter / 2.0
+    if half > radius:
+        raise ValueError("the aperture must not exceed twice the radius of curvature")
+    # the cap: points of the sphere at most `depth` from the apex along the axis
+    depth = radius - math.sqrt(radius * radius - half * half)
+    lo = [max(0, int(math.floor(min(apex[a], centre[a]) - radius - 1))) for a in range(3)]
+    hi = [min(shape[a], int(math.ceil(max(apex[a], centre[a]) + radius + 2))) for a in range(3)]
+    g = np.meshgrid(*[np.arange(lo[a], hi[a], dtype=np.float64) for a in range(3)], indexing="ij")
+    d = [g[a] - centre[a] for a in range(3)]
+    r = np.sqrt(d[0] ** 2 + d[1] ** 2 + d[2] ** 2)
+    along = sum((g[a] - apex[a]) * axis[a] for a in range(3))
+    sel = (np.abs(r - radius) <= 0.5) & (along <= depth + 0.5)
+    mask = np.zeros(shape)
+    mask[lo[0]:hi[0], lo[1]:hi[1], lo[2]:hi[2]] = sel
+    return mask
+
+
+def cfg3(N=384, Nt=2000, pml=10, seeds=(0, 1), plane=64):
+    """Precision-study workload (nonlinear absorbing, bowl source); returns
+    (kgrid, medium, source, sensor, opts) and the 0-based z index of the
+    sensor plane."""
+    dx = 1e-4
+    shape = (N, N, N)
+    c = np.clip(1500.0 + 100.0 * smoothed_noise(shape, seeds[0], 4.0), 1400.0, 1600.0)
+    rho = np.clip(1000.0 + 100.0 * smoothed_noise(shape, seeds[1], 4.0), 900.0, 1100.0)
+    kg = kWaveGrid(N, dx, N, dx, N, dx)
+    dt = 0.3 * dx / float(c.max())
+    kg.setTime(Nt, dt)
+    medium = {"sound_speed": c, "density": rho, "alpha_coeff": 0.75, "alpha_power": 1.5,
+              "BonA": 6.0}
+    f0, cycles = 0.5e6, 5
+    n_sig = int(math.floor(cycles / f0 / dt)) + 1
+    t = np.arange(n_sig) * dt
+    sig = 1e6 * np.sin(2 * math.pi * f0 * t) * np.hanning(n_sig)
+    # radius of curvature and aperture (150 and 120 points at N = 384),
+    # rounded half away from zero
+    radius = math.floor(150.0 * N / 384 + 0.5)
+    diameter = math.floor(120.0 * N / 384 + 0.5)
+    cx = N // 2
+    apex = (cx, cx, pml + 1)
+    mask = bowl_mask(shape, apex, radius, diameter, (cx, cx, N - 1))
+    source = {"p": sig[None, :], "p_mask": mask, "p_mode": "additive"}
+    zf = pml + 1 + radius
+    if zf >= N - pml:
+        raise ValueError("grid too small for the bowl")
+    m = np.zeros(shape, dtype=bool)
+    h = plane // 2
+    m[cx - h:cx - h + plane, cx - h:cx - h + plane, zf] = True
+    sensor = {"mask": m, "record": ["p", "p_max", "p_rms"]}
+    opts = {"PMLSize": pml, "PMLAlpha": 2.0}
+    return kg, medium, source, sensor, opts, zf
 
 
 def fft_friendly(lo, hi):

@@ -634,7 +634,8 @@ def optimal_pml_size(grid_size, pml_range=(10, 40)):
     """L/getOptimalPMLSize.m: per axis, the PML size in pml_range (inclusive)
     whose grown length N + 2 pml has the smallest largest prime factor (the
     first such size)."""
-    lo, hi = int(round(pml_range[0])), int(round(pml_range[1]))
+    # MATLAB round (halves away from zero), not Python's round-half-even
+    lo, hi = (int(matlab_round(float(v))) for v in pml_range)
     if lo < 0 or hi < 0:
         raise ValueError("Optional input pml_range must be positive.")
     if hi <= lo:

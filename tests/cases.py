@@ -51,7 +51,22 @@ def tone(Nt, dt, f0, cycles, amp=1.0):
 
 def case(name, dim=2, N=None):
     """Named parity cases.  Returns a dict with keys grid, Nt, dt, medium,
-    source, sensor, opts."""
+    source, sensor, opts.
+
+    A name ending in "_off" is the named source case with every signal cut
+    to SIGNAL_OFF_LEN time points, fewer than Nt: the source is applied at
+    steps 1..SIGNAL_OFF_LEN and the remaining steps run with it switched off
+    (kspaceFirstOrder3D.m:806,930: flags.source_p >= t_index)."""
+    if name.endswith("_off"):
+        cs = case(name[:-4], dim, N)
+        src = dict(cs["source"])
+        for nm in ("p", "ux", "uy", "uz"):
+            if nm in src:
+                s = np.atleast_2d(np.asarray(src[nm]))
+                assert s.shape[1] > SIGNAL_OFF_LEN
+                src[nm] = s[:, :SIGNAL_OFF_LEN] if np.ndim(cs["source"][nm]) == 2 else s[0, :SIGNAL_OFF_LEN]
+        cs["source"] = src
+        return cs
     rng = np.random.default_rng(zlib.crc32(name.encode()))
     if N is None:
         N = {1: (96,), 2: (32, 30), 3: (16, 18, 20)}[dim]
@@ -310,6 +325,39 @@ def case(name, dim=2, N=None):
             opts["PMLInside"] = False
             rec = ["p", "u_max", "p_max_all"]
         sens = {"mask": corners, "record": rec}
+    elif name.startswith("cref_"):
+        # medium.sound_speed_ref other than the default maximum
+        # (kspaceFirstOrder_setSoundSpeedRef.m:30-50): kappa, the source
+        # correction and the PML all follow c_ref
+        c, rho = hetero_medium(shape, 53)
+        ref = {"cref_min": "min", "cref_mean": "mean", "cref_max": "max", "cref_scalar": 1530.0}[name]
+        med = {"sound_speed": c, "density": rho, "alpha_coeff": 0.75, "alpha_power": 1.5,
+               "sound_speed_ref": ref}
+        m = np.zeros(shape)
+        m[tuple(slice(n // 2 - 1, n // 2 + 1) for n in shape)] = 1
+        src = {"p": tone(Nt, dt, 2e6, 3), "p_mask": m}
+        sens = {"mask": _sparse_mask(shape, rng, 0.3), "record": ["p", "p_max", "u_rms", "p_final"]}
+    elif name.startswith("alpha_sign"):
+        # medium.alpha_sign (kspaceFirstOrder_createAbsorptionVariables.m:
+        # 78-81): sign(alpha_sign) multiplies tau and eta, sign(0) = 0
+        c, rho = hetero_medium(shape, 59)
+        sign = {"alpha_sign": [-1.0, 1.0], "alpha_sign_zero": [2.0, 0.0]}[name]
+        med = {"sound_speed": c, "density": rho, "alpha_coeff": 3.0, "alpha_power": 1.5,
+               "alpha_sign": sign}
+        src["p0"] = disc(shape, centre, 3)
+    elif name in ("no_kspace", "no_sg"):
+        # 'UsekSpace' false: kappa = 1 and no source correction filter;
+        # 'UseSG' false: the gradients at the grid points and rho0 not
+        # interpolated (kspaceFirstOrder3D.m:585-607,629-661)
+        c, rho = hetero_medium(shape, 61)
+        med = {"sound_speed": c, "density": rho, "alpha_coeff": 0.75, "alpha_power": 1.5}
+        m = np.zeros(shape)
+        m[tuple(slice(n // 2 - 1, n // 2 + 1) for n in shape)] = 1
+        src = {"p": tone(Nt, dt, 2e6, 3), "p_mask": m,
+               "ux": 1e-3 * tone(Nt, dt, 1.5e6, 2), "u_mask": m}
+        opts["UsekSpace" if name == "no_kspace" else "UseSG"] = False
+        sens = {"mask": _sparse_mask(shape, rng, 0.3),
+                "record": ["p", "p_max", "u", "u_non_staggered", "p_final"]}
     elif name == "cartesian_nearest":
         src["p0"] = disc(shape, centre, 3)
         npt = 12
@@ -336,6 +384,15 @@ def _sparse_mask(shape, rng, frac=0.1):
     return m
 
 
+SIGNAL_OFF_LEN = 10
+
+# source cases whose signals end before Nt
+OFF_CASES = ["psrc_add_off", "psrc_dir_off", "psrc_nocorr_off", "psrc_many_off", "psrc_label_off",
+             "psrc_many_dir_off", "usrc_add_off", "usrc_dir_off", "usrc_nocorr_off", "usrc_many_off"]
+# options that change the operator tables
+OPTION_CASES = ["cref_min", "cref_mean", "cref_scalar", "alpha_sign", "alpha_sign_zero",
+                "no_kspace", "no_sg"]
+
 CASES_2D = ["p0_lossless", "p0_hetero_absorbing", "p0_nonlinear_absorbing", "p0_nonlinear_lossless",
             "stokes", "no_dispersion", "no_absorption", "psrc_add", "psrc_dir", "psrc_nocorr",
             "psrc_many", "psrc_label", "psrc_many_dir", "usrc_add", "usrc_dir", "usrc_nocorr",
@@ -343,7 +400,7 @@ CASES_2D = ["p0_lossless", "p0_hetero_absorbing", "p0_nonlinear_absorbing", "p0_
             "pml_outside", "pml_outside_source", "cartesian_linear", "cartesian_linear_pml_outside",
             "cuboid_corners", "cuboid_corners_pml_outside", "freq_response",
             "directivity_pressure", "directivity_gradient", "alpha_filter", "source_frequency_ref",
-            "pml_auto"]
+            "pml_auto"] + OFF_CASES + OPTION_CASES
 # sensor directivity exists in 2-D only
 CASES_3D = [c for c in CASES_2D if not c.startswith("directivity")]
 # kspaceFirstOrder1D: every case whose inputs exist in 1-D
@@ -352,7 +409,9 @@ CASES_1D = ["p0_lossless", "p0_hetero_absorbing", "p0_nonlinear_absorbing", "p0_
             "psrc_label", "usrc_add", "usrc_dir", "usrc_many", "time_reversal", "cartesian_nearest",
             "cartesian_linear", "intensity", "u_nonstaggered", "pml_outside", "cartesian_linear_pml_outside",
             "cuboid_corners", "cuboid_corners_pml_outside", "freq_response", "alpha_filter",
-            "source_frequency_ref", "pml_auto"]
+            "source_frequency_ref", "pml_auto"] + \
+    [c for c in OFF_CASES if c[:-4] in ("psrc_add", "psrc_dir", "psrc_nocorr", "psrc_many", "psrc_label",
+                                        "usrc_add", "usrc_dir", "usrc_many")] + OPTION_CASES
 
 
 def oracle_opts(opts):
@@ -376,7 +435,10 @@ def run_oracle(cs, data_cast="single"):
 
 
 def run_kwhip(cs, backend):
+    """The product on the inputs of a case; backend "emu" runs the same host
+    code over the CPU lane emulator (test harness only)."""
     import kwhip
+    from kwhip import _abi
     shape, dx = cs["shape"], cs["dx"]
     args = []
     for n in shape:
@@ -384,7 +446,8 @@ def run_kwhip(cs, backend):
     kg = kwhip.kWaveGrid(*args)
     kg.setTime(cs["Nt"], cs["dt"])
     fn = {1: kwhip.kspaceFirstOrder1D, 2: kwhip.kspaceFirstOrder2D, 3: kwhip.kspaceFirstOrder3D}[cs["dim"]]
-    return fn(kg, cs["medium"], cs["source"], cs["sensor"], Backend=backend, **cs["opts"])
+    with _abi._backend_override(backend):
+        return fn(kg, cs["medium"], cs["source"], cs["sensor"], **cs["opts"])
 
 
 def compare(got, ref):

@@ -1,0 +1,106 @@
+"""Writes the long-run golden fixtures: the benchmark recipes run for their
+full length through the oracle, in single precision (the reference's
+'DataCast','single') and in double precision.
+
+* cfg2_long_<N>.npz: the 256^3 benchmark recipe (heterogeneous c and rho,
+  power-law absorption y = 1.5, additive tone-burst disc source) at N^3 for
+  1000 steps, the source on for the first 529-534 steps and off for the
+  rest; p_max, p_rms and p_final on every `stride`-th cell along each axis.
+* cfg3_<N>.npz: the nonlinear precision-study recipe (B/A = 6, 1 MPa
+  0.5 MHz burst on a bowl) at N^3 for 2000 steps; the sensor is the 64 x 64
+  plane through the focus: p_max and p_rms at every plane point and the p
+  time series at every 4th point along x and y, every 4th step.
+
+The oracle takes minutes to tens of minutes per fixture, so the GPU tests
+(tests/test_gpu_longrun.py) compare against these files instead of running
+it.  Each fixture stores the sizes, the subsampling and a CRC of the inputs.
+
+Run from the repository root:
+    python tests/golden/make_longrun.py cfg2 64
+    python tests/golden/make_longrun.py cfg2 128
+    python tests/golden/make_longrun.py cfg3 96
+"""
+
+import sys
+import time
+import zlib
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+ROOT = HERE.parent.parent
+sys.path[:0] = [str(ROOT), str(ROOT / "k-wave-ii_amd")]
+
+CFG2_STEPS = 1000
+CFG3_STEPS = 2000
+CFG3_SUB = 4      # plane points along x and y, and time steps, of the p series
+
+
+def cfg2_inputs(N, Nt=CFG2_STEPS):
+    from kwhip import workloads
+    kg, med, src, sens, opts = workloads.cfg2(N, Nt=Nt, pml=10, src_radius=max(4, N // 8))
+    sens = {"mask": sens["mask"], "record": ["p_max", "p_rms", "p_final"]}
+    return kg, med, src, sens, opts
+
+
+def cfg3_inputs(N, Nt=CFG3_STEPS):
+    from kwhip import workloads
+    kg, med, src, sens, opts, zf = workloads.cfg3(N, Nt=Nt)
+    return kg, med, src, sens, opts
+
+
+def cfg2_stride(N):
+    return max(1, N // 32)
+
+
+def input_crc(med, src):
+    crc = 0
+    for d in (med, src):
+        for k in sorted(d):
+            v = d[k]
+            b = v.encode() if isinstance(v, str) else np.ascontiguousarray(
+                np.asarray(v, dtype=np.float64)).tobytes()
+            crc = zlib.crc32(k.encode() + b, crc)
+    return crc
+
+
+def cfg3_p_sub(p):
+    """The fixture's subset of the plane's p series [4096 x Nt] (plane points
+    in column-major order): every CFG3_SUB-th point along x and y, every
+    CFG3_SUB-th step."""
+    p = np.asarray(p).reshape(64, 64, -1, order="F")
+    return p[::CFG3_SUB, ::CFG3_SUB, ::CFG3_SUB]
+
+
+def run(kind, N):
+    from oracle import kwave_oracle as ko
+    kg, med, src, sens, opts = (cfg2_inputs if kind == "cfg2" else cfg3_inputs)(N)
+    kgo = ko.KGrid(N, kg.dx, N, kg.dy, N, kg.dz)
+    kgo.set_time(kg.Nt, kg.dt)
+    arrays = {}
+    for cast, tag in (("single", "fp32"), ("off", "fp64")):
+        t0 = time.time()
+        ref = ko.kspace_first_order(kgo, med, src, sens, data_cast=cast, pml_size=opts["PMLSize"],
+                                    pml_alpha=opts["PMLAlpha"])
+        print(f"{kind} {N}^3 {tag}: {time.time() - t0:.0f} s", flush=True)
+        if kind == "cfg2":
+            s = cfg2_stride(N)
+            for k in ("p_max", "p_rms", "p_final"):
+                v = np.asarray(ref[k], dtype=np.float64)
+                arrays[f"{tag}_{k}"] = v[::s, ::s, ::s].astype(np.float32)
+                arrays[f"{tag}_{k}_absmax"] = np.float64(np.max(np.abs(v)))
+        else:
+            for k in ("p_max", "p_rms"):
+                arrays[f"{tag}_{k}"] = np.asarray(ref[k], dtype=np.float32).ravel()
+            arrays[f"{tag}_p"] = cfg3_p_sub(ref["p"]).astype(np.float32)
+            arrays[f"{tag}_p_absmax"] = np.float64(np.max(np.abs(ref["p"])))
+            arrays[f"{tag}_p_l2"] = np.float64(np.sqrt(np.sum(np.asarray(ref["p"], np.float64) ** 2)))
+    out = HERE / f"{kind}_{'long_' if kind == 'cfg2' else ''}{N}.npz"
+    np.savez_compressed(out, N=N, Nt=kg.Nt, crc=np.uint32(input_crc(med, src)),
+                        stride=np.int64(cfg2_stride(N) if kind == "cfg2" else CFG3_SUB), **arrays)
+    print(out, flush=True)
+
+
+if __name__ == "__main__":
+    run(sys.argv[1], int(sys.argv[2]))

@@ -40,7 +40,7 @@ def test_emulator_reports_its_backend():
     from kwhip import _abi
 
     lib = _abi.load("emu")
-    assert lib.kw_abi_version() == 1
+    assert lib.kw_abi_version() == 2
     assert lib.kw_backend().decode() == "emulator"
 
 

@@ -83,8 +83,9 @@ MIX = [
 def _run_mix(backend, mix, steps_per_round=0, device_bytes=0):
     import kwhip
     css = [cases.case(name, dim, shape) for dim, name, shape in mix]
-    with kwhip.Engine(device_bytes=device_bytes, steps_per_round=steps_per_round,
-                      backend=backend) as eng:
+    from kwhip import _abi
+    with _abi._backend_override(backend), \
+            kwhip.Engine(device_bytes=device_bytes, steps_per_round=steps_per_round) as eng:
         tickets = [_submit(eng, cs) for cs in css]
         got = [eng.wait(t) for t in tickets]
         st = eng.stats()
@@ -118,7 +119,8 @@ def test_engine_small_budget_admits_in_turn_emu(few_pencils):
 def test_engine_poll_cancel_and_errors_emu(few_pencils):
     import kwhip
     cs = cases.case("p0_hetero_absorbing", 2, (32, 30))
-    with kwhip.Engine(backend="emu", device_bytes=1) as eng:
+    from kwhip import _abi
+    with _abi._backend_override("emu"), kwhip.Engine(device_bytes=1) as eng:
         t1 = _submit(eng, cs)
         t2 = _submit(eng, cs)
         t2.cancel()

@@ -66,6 +66,45 @@ def test_blackman_coefficients(N):
     assert np.allclose(ko.blackman_1d(N), np.blackman(N), rtol=0, atol=1e-14)
 
 
+def _blackman_closed(n, N):
+    # the textbook Blackman window 0.42 - 0.5 cos(2 pi n/(N-1)) + 0.08 cos(4 pi n/(N-1))
+    return 0.42 - 0.5 * math.cos(2 * math.pi * n / (N - 1)) + 0.08 * math.cos(4 * math.pi * n / (N - 1))
+
+
+def _window_fns():
+    from kwhip.solver import blackman_window
+    return [("oracle", lambda shape: np.abs(ko.get_win_rotated(shape, [bool(n % 2) for n in shape]))),
+            ("product", blackman_window)]
+
+
+@pytest.mark.parametrize("which", [0, 1])
+def test_rotated_window_closed_forms(which):
+    """getWin.m:260-347 ('Rotation' true) as smooth.m calls it, for the oracle
+    and for the product's host copy: on an odd grid the central row and
+    column are the 1-D Blackman window; the window depends on the radius only
+    (transposes and mirrors agree); off the axes it is the linear
+    interpolation of the 1-D window at the radius (hand-computed at r = 5,
+    an integer radius on a diagonal, and at r = sqrt 2); beyond the radius
+    it is clipped to the window's end value; an even grid is the window of
+    the next odd size with its last row and column dropped."""
+    name, win = _window_fns()[which]
+    N, c = 17, 8
+    w = win((N, N))
+    b = np.array([_blackman_closed(n, N) for n in range(N)])
+    assert np.allclose(w[:, c], b, rtol=0, atol=1e-15), name
+    assert np.allclose(w[c, :], b, rtol=0, atol=1e-15), name
+    assert np.allclose(w, w.T, rtol=0, atol=1e-15) and np.allclose(w, w[::-1, :], rtol=0, atol=1e-15)
+    assert w[c + 3, c + 4] == pytest.approx(b[c + 5], abs=1e-15)   # r = 5
+    r = math.sqrt(2.0)
+    assert w[c + 1, c + 1] == pytest.approx(b[c + 1] + (r - 1) * (b[c + 2] - b[c + 1]), abs=1e-15)
+    assert w[0, 0] == pytest.approx(abs(b[N - 1]), abs=1e-15)      # clipped to the radius
+    w3 = win((N, N, N))
+    assert np.allclose(w3[c, c, :], b, rtol=0, atol=1e-15)
+    assert w3[c + 2, c + 3, c + 6] == pytest.approx(b[c + 7], abs=1e-15)  # r = 7
+    we = win((16, 16))
+    assert we.shape == (16, 16) and np.allclose(we, w[:16, :16], rtol=0, atol=1e-15)
+
+
 @pytest.mark.parametrize("shape", [(32, 32), (31, 33), (16, 18, 15)])
 def test_smooth_preserves_constants_and_restores_max(shape):
     c = np.full(shape, 3.5)
@@ -329,5 +368,9 @@ def test_optimal_pml_size_known_answers():
     101 in 13..20: 127, 129 = 3 43, 131, 133 = 7 19, 135 = 3^3 5 -> 17."""
     assert ko.optimal_pml_size([100, 101, 97]) == [14, 12, 14]
     assert ko.optimal_pml_size([101], (13, 20)) == [17]
+    # 'PMLRange' is rounded as MATLAB's round: 12.5 -> 13 (Python's
+    # round-half-even would give 12 and the answer 12)
+    assert ko.optimal_pml_size([101], (12.5, 20)) == [17]
+    assert ko.optimal_pml_size([101], (12.4, 20)) == [12]
     with pytest.raises(ValueError):
         ko.optimal_pml_size([64], (20, 10))
