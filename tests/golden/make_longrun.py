@@ -54,6 +54,13 @@ def cfg2_stride(N):
     return max(1, N // 32)
 
 
+def as_grid(v, N):
+    """A full-grid sensor record as an N^3 grid: the statistics of a binary
+    mask come back as one value per sensor point, in column-major order."""
+    v = np.asarray(v, dtype=np.float64)
+    return v.reshape((N, N, N), order="F") if v.ndim == 1 else v
+
+
 def input_crc(med, src):
     crc = 0
     for d in (med, src):
@@ -87,7 +94,7 @@ def run(kind, N):
         if kind == "cfg2":
             s = cfg2_stride(N)
             for k in ("p_max", "p_rms", "p_final"):
-                v = np.asarray(ref[k], dtype=np.float64)
+                v = as_grid(ref[k], N)
                 arrays[f"{tag}_{k}"] = v[::s, ::s, ::s].astype(np.float32)
                 arrays[f"{tag}_{k}_absmax"] = np.float64(np.max(np.abs(v)))
         else:

@@ -7,8 +7,13 @@ oracle needs minutes to tens of minutes per run).
   (kspaceFirstOrder3D.m:930: flags.source_p >= t_index), so both regimes
   and the FP32 drift over the whole run are covered.  p_max, p_rms and
   p_final on a subsampled grid: relative L-inf <= 1e-5 against the oracle in
-  single precision, <= 1e-4 against double precision (normalised by the
-  largest magnitude over the whole grid).
+  single precision, <= 1e-4 against double precision, normalised by the
+  largest magnitude over the whole grid; for p_final that is the run's peak
+  pressure (the largest p_max).  After 1000 steps the wave has left the grid
+  and p_final is about 3e-6 of the peak, at the single-precision rounding
+  floor of a field that carried the peak: the oracle's own single- and
+  double-precision runs differ by 8e-4 of p_final's own largest value, so
+  that error is printed but not gated.
 * The nonlinear precision-study recipe (B/A = 6, 1 MPa bowl source) at
   96^3 for 2000 steps, sensor p on the 64 x 64 focal plane: p_max and p_rms
   at every plane point and the p series on a subsampled set of points and
@@ -53,10 +58,13 @@ def test_cfg2_recipe_full_length(N, hip_lib):
     s = int(fx["stride"])
     for tag, tol in (("fp32", TOL32), ("fp64", TOL64)):
         for k in ("p_max", "p_rms", "p_final"):
-            g = np.asarray(got[k], dtype=np.float64)[::s, ::s, ::s]
+            g = ml.as_grid(got[k], N)[::s, ::s, ::s]
             r = fx[f"{tag}_{k}"].astype(np.float64)
-            err = np.max(np.abs(g - r)) / float(fx[f"{tag}_{k}_absmax"])
-            print(f"[longrun] {N}^3 {k} vs {tag} oracle: rel L-inf {err:.2e}", flush=True)
+            own = np.max(np.abs(g - r)) / float(fx[f"{tag}_{k}_absmax"])
+            scale = "p_max" if k == "p_final" else k
+            err = np.max(np.abs(g - r)) / float(fx[f"{tag}_{scale}_absmax"])
+            print(f"[longrun] {N}^3 {k} vs {tag} oracle: rel L-inf {err:.2e} "
+                  f"(of its own largest value {own:.2e})", flush=True)
             assert err <= tol, (N, tag, k, err)
 
 
