@@ -318,6 +318,22 @@ int kw_engine_get_stats(kw_engine* engine, kw_engine_stats* stats, char* err, si
 /* Cancels queued and running simulations, stops the threads, frees all. */
 int kw_engine_destroy(kw_engine* engine);
 
+/* kw_run without any device allocation.  It has no counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * running while the surrounding application captures a HIP graph, serving
+ * loops and harnesses that forbid hipMalloc).  Every device buffer of the
+ * simulation (fields, spectra, operators, the staged inputs and recorded
+ * outputs) is carved out of a static scratch array of KW_NOALLOC_MAX_BYTES
+ * bytes that each device holds from the library's load until the process
+ * ends; a problem whose device footprint (kw_sim_device_bytes of the same
+ * problem, plus 256-byte alignment per buffer) exceeds it fails with
+ * KW_ENOMEM.  Calls in one process are serialised by a lock in the
+ * library, since they share the array; the array keeps the last call's
+ * contents, and each call clears what it needs.  Runs on the null stream
+ * of the current device.  Outputs are those of kw_run. */
+#define KW_NOALLOC_MAX_BYTES (128ll << 20)
+int kw_run_noalloc(const kw_problem* prob, kw_outputs* out, char* err, size_t errlen);
+
 /* Smoothing filter: out = real(ifftn(fftn(in) .* ifftshift(win))) on a 1D,
  * 2D or 3D grid, computed on the device.  `win` is the centred window of
  * getWin (not shifted).  Replaces the transform of +kwave/+legacy/smooth.m;

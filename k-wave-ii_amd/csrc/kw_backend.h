@@ -56,6 +56,12 @@ void dev_graph_destroy(void* exec);
 // largest dynamic LDS per block the kernels may request
 size_t dev_max_lds();
 
+// The library's static scratch region on the current device (a file-scope
+// device array of *bytes bytes, the same array on every call, held until the
+// process ends); its device address is looked up once per device.  Used by
+// the entry point that allocates nothing.
+void* dev_scratch(size_t* bytes);
+
 // free and total device memory in bytes
 void dev_mem_info(size_t* free_bytes, size_t* total_bytes);
 

@@ -38,6 +38,7 @@
 
 #include "kw_backend.h"
 #include "kw_variants.h"
+#include "kwhip.h"
 
 namespace kw {
 
@@ -96,6 +97,18 @@ void dev_free(void* p) {
     g_allocs.erase((uintptr_t)p);
   }
   std::free(p);
+}
+
+// the emulator's stand-in for the device scratch array: one host block of
+// the same size, made on first use, registered as one allocation and kept
+// until the process ends
+void* dev_scratch(size_t* bytes) {
+  static void* block = nullptr;
+  static std::mutex mu;
+  std::lock_guard<std::mutex> lk(mu);
+  if (!block) block = dev_alloc((size_t)KW_NOALLOC_MAX_BYTES);
+  if (bytes) *bytes = (size_t)KW_NOALLOC_MAX_BYTES;
+  return block;
 }
 
 void* dev_stream_create() { return nullptr; }

@@ -3,10 +3,12 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdio>
+#include <mutex>
 #include <string>
 
 #include "kw_backend.h"
 #include "kw_variants.h"
+#include "kwhip.h"
 
 namespace kw {
 
@@ -66,6 +68,33 @@ void dev_zero(void* dst, size_t bytes, void* s) {
 }
 
 size_t dev_max_lds() { return 65536; }
+
+// Scratch of kw_run_noalloc: every device buffer of one simulation (fields,
+// spectra, operators, staging of the inputs) is carved out of this array, so
+// that call makes no device allocation.  In host code `scratch` is only a
+// host-side handle of the symbol; its address on a device comes from
+// hipGetSymbolAddress, once per device, with that device current.
+static __device__ __align__(256) unsigned char scratch[KW_NOALLOC_MAX_BYTES];
+
+namespace {
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[64] = {};
+}  // namespace
+
+void* dev_scratch(size_t* bytes) {
+  int dev = 0;
+  check(hipGetDevice(&dev), "hipGetDevice");
+  if (dev < 0 || dev >= 64) throw DeviceError(-3, "device ordinal out of range");
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    check(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "hipGetSymbolAddress(scratch)");
+    if (!p) throw DeviceError(-3, "hipGetSymbolAddress(scratch) returned no address");
+    g_scratch_ptr[dev] = p;
+  }
+  if (bytes) *bytes = (size_t)KW_NOALLOC_MAX_BYTES;
+  return g_scratch_ptr[dev];
+}
 
 void dev_mem_info(size_t* free_bytes, size_t* total_bytes) {
   size_t f = 0, t = 0;

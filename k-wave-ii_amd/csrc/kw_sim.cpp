@@ -5,6 +5,8 @@
 // implements the single-simulation C ABI declared in include/kwhip.h.
 #include "kw_sim.h"
 
+#include <mutex>
+
 namespace kw {
 
 // ---------------------------------------------------------------------------
@@ -98,7 +100,7 @@ static float* source_km(Sim& S, const kw_time_source& src) {
   return km;
 }
 
-std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream) {
+std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream, Arena* arena) {
   if (!pr_in) fail(KW_EINVAL, "problem is NULL");
   const int pdim = pr_in->dim;
   if (pdim < 1 || pdim > 3) fail(KW_EINVAL, "dim must be 1, 2 or 3");
@@ -186,8 +188,9 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream) {
   auto S = std::make_unique<Sim>();
   // a simulation owns its stream unless the caller hands one in (the engine
   // runs every simulation on its own stream)
-  S->own_stream = stream == nullptr;
-  S->stream = stream ? stream : dev_stream_create();
+  S->arena = arena;
+  S->own_stream = stream == nullptr && arena == nullptr;
+  S->stream = (stream || arena) ? stream : dev_stream_create();
   S->G = make_geometry(pr->dim, pr->Nx, pr->Ny, pr->Nz);
   const Geometry& G = S->G;
   ProbDev& P = S->P;
@@ -1064,6 +1067,21 @@ int kw_sim_step_timed(kw_sim* sim, int64_t nsteps, double* ms, kw_kernel_stats* 
 int kw_run(const kw_problem* prob, kw_outputs* out, char* err, size_t errlen) {
   return kw_guarded(err, errlen, [&]() {
     auto s = kw::create_sim(prob);
+    kw::step_n(*s, s->Nt_eff);
+    kw::read_outputs(*s, out);
+  });
+}
+
+// One call at a time per process: every call carves its buffers from the
+// same scratch array of the current device.
+static std::mutex g_noalloc_mu;
+
+int kw_run_noalloc(const kw_problem* prob, kw_outputs* out, char* err, size_t errlen) {
+  return kw_guarded(err, errlen, [&]() {
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    kw::Arena arena;
+    arena.base = (char*)kw::dev_scratch(&arena.size);
+    auto s = kw::create_sim(prob, nullptr, &arena);
     kw::step_n(*s, s->Nt_eff);
     kw::read_outputs(*s, out);
   });

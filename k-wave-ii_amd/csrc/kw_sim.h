@@ -150,6 +150,15 @@ inline int max_pencils_env() {
   return v > 0 ? v : 1 << 30;
 }
 
+// Bump allocator over a device region the caller owns (the static scratch
+// array of the entry point that allocates nothing): 256-byte aligned
+// buffers, released all at once when the region is reused.
+struct Arena {
+  char* base = nullptr;
+  size_t size = 0;
+  size_t used = 0;
+};
+
 struct Sim {
   Geometry G{};
   ProbDev P{};
@@ -157,6 +166,7 @@ struct Sim {
   int32_t* d_tile0 = nullptr;
   void* stream = nullptr;
   bool own_stream = true;
+  Arena* arena = nullptr;  // set: buffers come from the arena, not dev_alloc
   std::vector<void*> allocs;
   int64_t bytes = 0;
   int64_t steps_done = 0;
@@ -187,9 +197,23 @@ struct Sim {
 
   template <class T>
   T* alloc(size_t n) {
-    void* p = dev_alloc(n * sizeof(T));
-    allocs.push_back(p);
-    bytes += (int64_t)(n * sizeof(T));
+    const size_t nb = n * sizeof(T);
+    void* p;
+    if (arena) {
+      const size_t need = ((nb ? nb : 1) + 255) / 256 * 256;
+      if (need > arena->size - arena->used)
+        fail(KW_ENOMEM, "the problem needs more device memory than the " +
+                            std::to_string(arena->size >> 20) +
+                            " MiB scratch of the no-allocation entry point (at least " +
+                            std::to_string((arena->used + need + (1 << 20) - 1) >> 20) +
+                            " MiB so far)");
+      p = arena->base + arena->used;
+      arena->used += need;
+    } else {
+      p = dev_alloc(nb);
+      allocs.push_back(p);
+    }
+    bytes += (int64_t)nb;
     return (T*)p;
   }
   template <class T>
@@ -458,7 +482,10 @@ struct Sim {
   }
 };
 
-std::unique_ptr<Sim> create_sim(const kw_problem* pr, void* stream = nullptr);
+// `arena` set: every device buffer comes from it and the simulation runs on
+// the device's null stream (no stream is created)
+std::unique_ptr<Sim> create_sim(const kw_problem* pr, void* stream = nullptr,
+                                Arena* arena = nullptr);
 PassArgs bufs(int nb, int b0 = 0, int b1 = 1, int b2 = 2);
 void src_grid(Sim& S, int which, int t);
 void record_velocity(Sim& S);

@@ -35,6 +35,7 @@ REC = {
 }
 SRC_MODE = {"dirichlet": 0, "additive-no-correction": 1, "additive": 2}
 NKINDS = 9
+NOALLOC_MAX_BYTES = 128 << 20  # KW_NOALLOC_MAX_BYTES
 
 c_double_p = C.POINTER(C.c_double)
 c_float_p = C.POINTER(C.c_float)
@@ -189,6 +190,7 @@ def load(backend: str = "hip") -> C.CDLL:
     err = [C.c_char_p, C.c_size_t]
     lib.kw_output_sizes.argtypes = [P(Problem), P(Sizes)] + err
     lib.kw_run.argtypes = [P(Problem), P(Outputs)] + err
+    lib.kw_run_noalloc.argtypes = [P(Problem), P(Outputs)] + err
     lib.kw_sim_create.argtypes = [P(Problem), P(C.c_void_p)] + err
     lib.kw_sim_step.argtypes = [C.c_void_p, C.c_int64] + err
     lib.kw_sim_sync.argtypes = [C.c_void_p] + err

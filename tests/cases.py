@@ -470,3 +470,29 @@ def compare(got, ref):
         g = got[k]
         errs[k] = ko.rel_linf(np.asarray(g, dtype=np.float64), np.asarray(r, dtype=np.float64))
     return errs
+
+
+def prepared(cs):
+    """The kw_problem of a case (the host precompute of kspaceFirstOrder1D/
+    2D/3D), for calls straight through the C ABI."""
+    import kwhip
+    from kwhip import solver
+    shape, dx = cs["shape"], cs["dx"]
+    args = []
+    for n in shape:
+        args += [n, dx]
+    kg = kwhip.kWaveGrid(*args)
+    kg.setTime(cs["Nt"], cs["dt"])
+    opts = solver._parse_options((), dict(cs["opts"]))
+    return solver.prepare(kg, cs["medium"], cs["source"], cs["sensor"], cs["dim"], opts)
+
+
+def run_entry(lib, fn_name, pp):
+    """Run a prepared problem through the C-ABI entry point `fn_name`
+    (kw_run or kw_run_noalloc) of `lib`; outputs as a dict of fields."""
+    import ctypes as C
+    from kwhip import _abi, solver
+    out, arrays = solver.alloc_outputs(pp)
+    _abi.call(getattr(lib, fn_name), C.byref(pp.prob), C.byref(out))
+    res = solver.shape_outputs(pp, arrays)
+    return res if isinstance(res, dict) else {"p": res}
