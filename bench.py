@@ -33,9 +33,28 @@ for p in (ROOT, ROOT / "k-wave-ii_amd"):
 
 HBM_PEAK_GBS = 8000.0
 # whole-step algorithmic bytes per cell-step of this workload (per-axis-pass
-# model: 14 FFTs x 3 passes x 8 B, fused elementwise operands, FP64 p_max and
-# p_rms read+write): 536 B while the source is on, 488 B after it
-B_ALG_SRC_ON, B_ALG_SRC_OFF = 536.0, 488.0
+# model: 14 FFTs x 3 axis passes x 8 B, fused elementwise operands, FP64
+# p_max and p_rms read+write): 488 B; while the source is on the step adds
+# the 4 B read of the precomputed source field K_M = ifftn(source_kappa .*
+# fftn(mask)) (the two FFTs per step it replaces are an algorithmic saving,
+# not counted)
+B_ALG_SRC_ON, B_ALG_SRC_OFF = 492.0, 488.0
+# The same model per kernel kind, bytes per cell over one step (sum of the
+# kind's launches): 8 B per cell for every FFT axis pass a kind performs, 4 B
+# per FP32 grid operand and 8 B per FP64 one it must read or write.
+#   y_fwd    : y passes of 1 + 3 + 2 forward transforms            48
+#   y_inv    : y passes of 3 + 3 + 2 inverse transforms            64
+#   spectral : z passes of 4 + 6 + 4 transforms                   112
+#   x_grad   : 3 inverse + 3 forward x passes; u r/w, 1/rho0_sg    84
+#   x_div    : 3 inverse + 2 forward x passes; rho r/w, rho0,
+#              sum(rho) write (+ K_M while the source is on)     72 (+4)
+#   x_eos    : 2 inverse + 1 forward x passes; sum(rho), c0, tau,
+#              eta; p_max and p_rms r/w in FP64                   72
+# (452 B source off: the kinds omit the intermediate p write/read and a few
+# operands the survey's per-step total counts.)
+KIND_B_STEP = {"y_fwd": 48.0, "y_inv": 64.0, "spectral": 112.0, "x_grad": 84.0,
+               "x_div": 72.0, "x_eos": 72.0}
+KIND_B_SRC_ON = {"x_div": 4.0}
 METRIC = "Mcell-steps/s 256³ FP32 vs plain GPU; served p50/p99 latency at ≤1e-4 rel err"
 
 
@@ -43,23 +62,50 @@ def log(msg):
     print(msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(N, nsteps, cores):
-    """Oracle (float32, scipy.fft workers = cores) on the first nsteps steps
-    of the same workload; returns Mcell-steps/s from the median step time of
-    the steps after the first two."""
-    import numpy as np
+def cpu_model():
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_phase(N, nsteps, cores, cast, source_on):
+    """Median wall time of one step of the oracle (scipy.fft workers =
+    cores) over steps 3..2+nsteps of the 256^3 workload, with the source on
+    throughout (source_on) or switched off after step 1 (a one-sample
+    signal): the two regimes of the 1000-step run."""
     from kwhip import workloads
     from oracle import kwave_oracle as ko
 
-    kg, med, src, sens, opts = workloads.cfg2(N, Nt=nsteps)
+    Nt = 2 + nsteps
+    kg, med, src, sens, opts = workloads.cfg2(N, Nt=Nt)
+    if not source_on:
+        src = dict(src, p=src["p"][:, :1])
     kgo = ko.KGrid(N, kg.dx, N, kg.dy, N, kg.dz)
-    kgo.set_time(nsteps, kg.dt)
+    kgo.set_time(Nt, kg.dt)
     times = []
-    ko.kspace_first_order(kgo, med, src, sens, data_cast="single", pml_size=opts["PMLSize"],
+    ko.kspace_first_order(kgo, med, src, sens, data_cast=cast, pml_size=opts["PMLSize"],
                           pml_alpha=opts["PMLAlpha"], fft_workers=cores, step_times=times)
-    tail = sorted(times[2:]) if len(times) > 2 else sorted(times)
-    med_t = tail[len(tail) // 2]
-    return N ** 3 / med_t / 1e6, med_t, len(times)
+    tail = sorted(times[2:])
+    return tail[len(tail) // 2]
+
+
+def cpu_baseline(N, nsteps, cores, casts=("single",), n_on=534, n_total=1000):
+    """CPU restatement of the reference algorithm (oracle/) on the host
+    cores: per precision, the step time of each regime (2 warm-up + nsteps
+    timed steps each), weighted by the 1000-step run's n_on source-on and
+    n_total - n_on source-off steps; Mcell-steps/s of the weighted step."""
+    out = {}
+    for cast in casts:
+        t_on = cpu_phase(N, nsteps, cores, cast, True)
+        t_off = cpu_phase(N, nsteps, cores, cast, False)
+        t = (n_on * t_on + (n_total - n_on) * t_off) / n_total
+        out["fp32" if cast == "single" else "fp64"] = {
+            "value": N ** 3 / t / 1e6, "s_per_step_on": t_on, "s_per_step_off": t_off}
+    return out
 
 
 def accuracy_check(N=64, Nt=300):
@@ -79,6 +125,88 @@ def accuracy_check(N=64, Nt=300):
                                     pml_alpha=opts["PMLAlpha"])
         res["fp64" if cast == "off" else "fp32"] = max(ko.rel_linf(got[k], ref[k]) for k in ("p_max", "p_rms"))
     return res
+
+
+def cfg1_leg():
+    """The 2-D 128^2 lossless workload (Cartesian circle of 50 points,
+    906 steps) in full: the GPU against the CPU restatement of the reference
+    algorithm in double precision (the reference's default), both timed, and
+    the GPU result's relative L-inf error against it."""
+    import kwhip
+    from kwhip import workloads
+    from oracle import kwave_oracle as ko
+
+    kg, med, src, sens, opts = workloads.cfg1()
+    kwhip.kspaceFirstOrder2D(kg, med, src, sens, **opts)  # warm-up (host set-up paths)
+    t0 = time.perf_counter()
+    got = kwhip.kspaceFirstOrder2D(kg, med, src, sens, **opts)
+    t_gpu = time.perf_counter() - t0
+    kgo = ko.KGrid(kg.Nx, kg.dx, kg.Ny, kg.dy)
+    kgo.set_time(kg.Nt, kg.dt)
+    t0 = time.perf_counter()
+    ref = ko.kspace_first_order(kgo, med, src, sens, data_cast="off", pml_size=opts["PMLSize"],
+                                pml_alpha=opts["PMLAlpha"], cart_interp=opts["CartInterp"])
+    t_cpu = time.perf_counter() - t0
+    cells = kg.Nx * kg.Ny * kg.Nt
+    return {"steps": int(kg.Nt), "gpu_s": round(t_gpu, 3), "cpu_fp64_s": round(t_cpu, 3),
+            "gpu_Mcell_steps_s": round(cells / t_gpu / 1e6, 1),
+            "cpu_fp64_Mcell_steps_s": round(cells / t_cpu / 1e6, 2),
+            "rel_linf_vs_fp64": ko.rel_linf(got, ref),
+            "note": "whole call (host set-up, transfers, 906 steps) on both sides"}
+
+
+def cfg3_leg(N=384, Nt=2000, steps=None):
+    """The nonlinear precision-study workload (B/A = 6, power-law absorption,
+    1 MPa 0.5 MHz bowl source, 64 x 64 sensor plane recording p, p_max and
+    p_rms): throughput of the whole run at N^3 on the GPU (inputs resident),
+    and its error against the oracle from the committed 2000-step fixtures
+    of the same recipe at smaller sizes (tests/golden/cfg3_<n>.npz: single-
+    and double-precision oracle runs, p on a subsampled set of plane points
+    and steps, p_max and p_rms on the whole plane)."""
+    import numpy as np
+
+    import kwhip
+    from kwhip import workloads
+
+    steps = Nt if steps is None else steps
+    kg, med, src, sens, opts, zf = workloads.cfg3(N, Nt=Nt)
+    sim = kwhip.Simulation(kg, med, src, sens, 3, **opts)
+    sim.sync()
+    t0 = time.perf_counter()
+    ev_ms, _ = sim.step_timed(steps)
+    sim.sync()
+    seconds = max(time.perf_counter() - t0, ev_ms / 1e3)
+    out = {"grid": [N, N, N], "steps": steps, "ms_per_step": round(seconds * 1e3 / steps, 4),
+           "value": round(N ** 3 * steps / seconds / 1e6, 1), "unit": "Mcell-steps/s",
+           "device_gb": round(sim.device_bytes / 1e9, 3)}
+    if steps == Nt:
+        res = sim.read()
+        pk = float(np.max(np.abs(res["p"])))
+        out["p_peak_pa"] = round(pk, 1)
+    sim.close()
+    sys.path.insert(0, str(ROOT / "tests" / "golden"))
+    import make_longrun as ml
+    errs = {}
+    for n in (96, 192):
+        path = ROOT / "tests" / "golden" / f"cfg3_{n}.npz"
+        if not path.exists():
+            continue
+        fx = np.load(path)
+        kgs, meds, srcs, senss, optss = ml.cfg3_inputs(n)
+        got = kwhip.kspaceFirstOrder3D(kgs, meds, srcs, senss, **optss)
+        gp = ml.cfg3_p_sub(got["p"]).astype(np.float64)
+        e = {}
+        for tag in ("fp32", "fp64"):
+            r = fx[f"{tag}_p"].astype(np.float64)
+            e[f"p_linf_vs_{tag}"] = float(np.max(np.abs(gp - r)) / float(fx[f"{tag}_p_absmax"]))
+            e[f"p_l2_vs_{tag}"] = float(np.sqrt(np.sum((gp - r) ** 2) / np.sum(r ** 2)))
+            for k in ("p_max", "p_rms"):
+                g = np.asarray(got[k], dtype=np.float64).ravel()
+                rr = fx[f"{tag}_{k}"].astype(np.float64)
+                e[f"{k}_linf_vs_{tag}"] = float(np.max(np.abs(g - rr)) / np.max(np.abs(rr)))
+        errs[f"{n}^3"] = e
+    out["rel_err_2000_steps"] = errs
+    return out
 
 
 def _tr_sensor(sens, data):
@@ -339,16 +467,97 @@ def reduce_traffic(fetch_dir, write_dir, cal_fetch_dir, cal_write_dir, kernel, v
     print(json.dumps(out))
 
 
+COUNTER_KERNELS = (("kv_0_256", "y_fwd"), ("kv_1_256", "y_inv"), ("kv_2_256", "spectral"),
+                   ("kv_5_256", "x_grad"), ("kv_6_256", "x_div"), ("kv_7_256", "x_eos"))
+
+
+def _all_counter_means(run_dir, name_part):
+    """Mean per-dispatch value of every counter of a rocprofv3 --pmc run (CSV
+    output) over the dispatches of the kernels whose name contains
+    name_part."""
+    import csv
+
+    per = {}
+    for f in Path(run_dir).rglob("*counter_collection.csv"):
+        for r in csv.DictReader(open(f)):
+            if name_part not in r.get("Kernel_Name", ""):
+                continue
+            key = (r["Counter_Name"], str(f), r.get("Dispatch_Id"))
+            per[key] = per.get(key, 0.0) + float(r["Counter_Value"])
+    sums, counts = {}, {}
+    for (c, _, _), v in per.items():
+        sums[c] = sums.get(c, 0.0) + v
+        counts[c] = counts.get(c, 0) + 1
+    return {c: sums[c] / counts[c] for c in sums}
+
+
+def reduce_counters(dirs):
+    """Markdown table of the instruction, wait and LDS counters of the step
+    kernels from several rocprofv3 --pmc runs (one counter set per run), with
+    the ratios that locate where a kernel's time goes: the share of its
+    wave-cycles spent waiting for anything, for LDS and for vector memory,
+    the share issuing VALU and LDS instructions, the LDS bank-conflict rate,
+    and the waves in flight per sampled cycle."""
+    rows = []
+    for variant, kind in COUNTER_KERNELS:
+        c = {}
+        for d in dirs:
+            c.update(_all_counter_means(d, variant + "("))
+        if not c:
+            continue
+
+        def ratio(a, b):
+            return c[a] / c[b] if a in c and b in c and c[b] else None
+
+        rows.append((kind, variant, {
+            "waves": c.get("SQ_WAVES"),
+            "wait_any": ratio("SQ_WAIT_ANY", "SQ_WAVE_CYCLES"),
+            "wait_inst_any": ratio("SQ_WAIT_INST_ANY", "SQ_WAVE_CYCLES"),
+            "wait_lds": ratio("SQ_WAIT_INST_LDS", "SQ_WAVE_CYCLES"),
+            "active_valu": ratio("SQ_ACTIVE_INST_VALU", "SQ_WAVE_CYCLES"),
+            "active_lds": ratio("SQ_ACTIVE_INST_LDS", "SQ_WAVE_CYCLES"),
+            "active_vmem": ratio("SQ_ACTIVE_INST_VMEM", "SQ_WAVE_CYCLES"),
+            "lds_bank_conflict": ratio("SQ_LDS_BANK_CONFLICT", "SQ_LDS_IDX_ACTIVE"),
+            # waves in flight per sampled cycle, summed over the SQ instances
+            # the counters aggregate (a relative occupancy measure)
+            "waves_level": ratio("SQ_LEVEL_WAVES", "SQ_CYCLES"),
+            "valu_per_wave": ratio("SQ_INSTS_VALU", "SQ_WAVES"),
+            "lds_per_wave": ratio("SQ_INSTS_LDS", "SQ_WAVES"),
+            "vmem_rd_per_wave": ratio("SQ_INSTS_VMEM_RD", "SQ_WAVES"),
+            "vmem_wr_per_wave": ratio("SQ_INSTS_VMEM_WR", "SQ_WAVES"),
+            "raw": c}))
+    cols = ["waves", "wait_any", "wait_inst_any", "wait_lds", "active_valu", "active_lds",
+            "active_vmem", "lds_bank_conflict", "waves_level", "valu_per_wave", "lds_per_wave",
+            "vmem_rd_per_wave", "vmem_wr_per_wave"]
+    print("| kernel | " + " | ".join(cols) + " |")
+    print("|---" * (len(cols) + 1) + "|")
+    for kind, variant, r in rows:
+        vals = []
+        for k in cols:
+            v = r[k]
+            vals.append("-" if v is None else (f"{v:.0f}" if abs(v) >= 100 else f"{v:.3g}"))
+        print(f"| {variant} ({kind}) | " + " | ".join(vals) + " |")
+    print()
+    print(json.dumps({variant: r["raw"] for _, variant, r in rows}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--N", type=int, default=256)
-    ap.add_argument("--cpu-steps", type=int, default=5)
+    ap.add_argument("--cpu-steps", type=int, default=5,
+                    help="timed oracle steps per regime (source on, off) after 2 warm-up steps")
+    ap.add_argument("--cpu-fp64", action="store_true",
+                    help="also time the oracle in double precision")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
+    ap.add_argument("--no-cfg1", action="store_true")
+    ap.add_argument("--cfg3", type=int, default=384,
+                    help="grid size of the nonlinear precision-study leg (0: skip)")
+    ap.add_argument("--cfg3-steps", type=int, default=2000)
     ap.add_argument("--batched", type=int, default=24,
                     help="problems in the batched time-reversal leg (0: skip)")
     ap.add_argument("--served", type=int, default=24,
@@ -361,11 +570,16 @@ def main():
                     help="run only the calibration copy (under rocprofv3 --pmc)")
     ap.add_argument("--reduce-traffic", nargs=4, metavar="DIR",
                     help="FETCH_SIZE run, WRITE_SIZE run, and the two calibration runs")
+    ap.add_argument("--reduce-counters", nargs="+", metavar="DIR",
+                    help="rocprofv3 --pmc runs of the step kernels (one counter set each)")
     ap.add_argument("--traffic-kernel", default="spectral")
     ap.add_argument("--traffic-variant", default="kv_2_256")
     args = ap.parse_args()
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.reduce_counters:
+        reduce_counters(args.reduce_counters)
         return
     if args.reduce_traffic:
         reduce_traffic(*args.reduce_traffic, args.traffic_kernel, args.traffic_variant, args.N)
@@ -420,38 +634,68 @@ def main():
         sim = kwhip.Simulation(kg, med, src, sens, 3, **opts)
         _, kern = sim.step_timed(K, per_kernel=True)
         sim.close()
-        kernels = {k: {"ms_total": round(v["ms"], 4), "launches": v["launches"],
-                       "avg_us": round(1e3 * v["ms"] / v["launches"], 3) if v["launches"] else None,
-                       "GBps": round(v["bytes"] / (v["ms"] * 1e-3) / 1e9, 1) if v["ms"] > 0 else None}
-                   for k, v in kern.items() if v["launches"]}
-        dom = max(kern, key=lambda k: kern[k]["ms"])
-        d = kern[dom]
-        avg_s = d["ms"] * 1e-3 / d["launches"]
-        bytes_per_launch = d["bytes"] / d["launches"]
-        achieved = bytes_per_launch / avg_s / 1e9
+        # per kind: algorithmic bytes of the model above over the K steps,
+        # and the compulsory bytes the library counts (every array a launch
+        # touches, once per launch)
+        kernels = {}
+        for k, v in kern.items():
+            if not v["launches"] or k not in KIND_B_STEP:
+                continue
+            alg = ncell * (K * KIND_B_STEP[k] + on * KIND_B_SRC_ON.get(k, 0.0))
+            sec = v["ms"] * 1e-3
+            kernels[k] = {"ms_total": round(v["ms"], 4), "launches": v["launches"],
+                          "avg_us": round(1e3 * v["ms"] / v["launches"], 3),
+                          "alg_bytes_per_launch": round(alg / v["launches"]),
+                          "alg_GBps": round(alg / sec / 1e9, 1),
+                          "frac": round(alg / sec / 1e9 / HBM_PEAK_GBS, 4),
+                          "compulsory_GBps": round(v["bytes"] / sec / 1e9, 1)}
+        dom = max(kernels, key=lambda k: kernels[k]["ms_total"])
+        d = kernels[dom]
+        avg_s = d["avg_us"] * 1e-6
+        achieved = d["alg_bytes_per_launch"] / avg_s / 1e9
         traffic = None
         tf = ROOT / "profiles" / f"traffic_{dom}_cfg2_{N}.json"
         if tf.exists():
             traffic = json.loads(tf.read_text()).get("bytes_per_launch")
         roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1),
                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4),
-                    "traffic": traffic, "bytes_per_launch": bytes_per_launch,
-                    "avg_launch_us": round(avg_s * 1e6, 3)}
-        log(f"[bench] dominant kernel {dom}: {achieved:.0f} GB/s ({achieved / HBM_PEAK_GBS:.1%} of peak)")
+                    "traffic": traffic, "alg_bytes_per_launch": d["alg_bytes_per_launch"],
+                    "avg_launch_us": d["avg_us"]}
+        log(f"[bench] dominant kernel {dom}: {achieved:.0f} GB/s algorithmic "
+            f"({achieved / HBM_PEAK_GBS:.1%} of peak)")
 
     cpu = None
     if not args.no_cpu:
         cores = min(os.cpu_count() or 1, int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
-        log(f"[bench] CPU baseline: oracle float32, {args.cpu_steps} steps, {cores} threads")
-        cval, step_s, n = cpu_baseline(N, args.cpu_steps, cores)
-        cpu = {"value": round(cval, 3), "unit": "Mcell-steps/s", "cores": cores, "kind": "port",
-               "sample": f"{N}^3 workload, first {n} steps (source on), median of steps 3..{n} "
-                         f"({step_s:.2f} s/step), NumPy/scipy.fft float32 restatement"}
+        casts = ("single", "off") if args.cpu_fp64 else ("single",)
+        log(f"[bench] CPU baseline: oracle {'/'.join(casts)}, 2 + {args.cpu_steps} steps per "
+            f"regime, {cores} threads")
+        cb = cpu_baseline(N, args.cpu_steps, cores, casts, n_src_on, Nt)
+        f32 = cb["fp32"]
+        cpu = {"value": round(f32["value"], 3), "unit": "Mcell-steps/s", "cores": cores,
+               "kind": "port", "cpu_model": cpu_model(),
+               "sample": f"{N}^3 workload, CPU restatement of the reference algorithm (NumPy/"
+                         f"scipy.fft, single precision): median step of 2 warm-up + "
+                         f"{args.cpu_steps} steps with the source on "
+                         f"({f32['s_per_step_on']:.2f} s) and off ({f32['s_per_step_off']:.2f} s), "
+                         f"weighted {n_src_on}/{Nt - n_src_on} as in the {Nt}-step run"}
+        if "fp64" in cb:
+            cpu["fp64_value"] = round(cb["fp64"]["value"], 3)
 
     acc = None
     if not args.no_accuracy:
         log("[bench] accuracy: 64^3 x 300 steps of the same recipe vs the FP64/FP32 oracle")
         acc = accuracy_check()
+
+    c1 = None
+    if not args.no_cfg1:
+        log("[bench] 2-D 128^2 lossless workload in full, GPU and CPU (double precision)")
+        c1 = cfg1_leg()
+    c3 = None
+    if args.cfg3 > 0:
+        log(f"[bench] nonlinear bowl workload {args.cfg3}^3 x {args.cfg3_steps} steps")
+        c3 = cfg3_leg(args.cfg3, 2000, args.cfg3_steps)
+        log(f"[bench] nonlinear bowl workload: {c3['value']} Mcell-steps/s")
 
     batched = batched_bench(args.batched) if args.batched > 0 else None
     served = (served_bench(args.served, args.served_rate, *args.served_steps)
@@ -470,6 +714,8 @@ def main():
                           "model": "per-axis-pass algorithmic bytes / 8 TB/s"},
         "cpu_baseline": cpu,
         "rel_err_64cubed_300steps": acc,
+        "cfg1_2d_128": c1,
+        "cfg3_nonlinear": c3,
         "device_gb": round(dev_gb, 3),
         "kernels": kernels,
         "batched": batched,

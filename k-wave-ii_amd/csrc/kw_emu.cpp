@@ -126,7 +126,7 @@ void dev_zero(void* dst, size_t bytes, void*) {
   emu_check(dst, bytes, "zero");
   std::memset(dst, 0, bytes);
 }
-size_t dev_max_lds() { return 65536; }
+size_t dev_max_lds() { return 163840; }  // gfx950: 160 KiB per block
 
 // the emulator's "device" is host memory; report a fixed 16 GiB
 void dev_mem_info(size_t* free_bytes, size_t* total_bytes) {

@@ -255,7 +255,7 @@ struct OutputStage {
 // ---------------------------------------------------------------------------
 // groups: one launch per pass for all members
 // ---------------------------------------------------------------------------
-enum GeoKind { GX = 0, GY, GL, GCELL, GSENS, GONE, NGEO };
+enum GeoKind { GX = 0, GY, GL, GCELL, GSENS, NGEO };
 
 struct Ticket;
 
@@ -307,14 +307,12 @@ struct Group {
   }
 
   static const LaunchGeom& geom_of(const Sim& S, int k) {
-    static const LaunchGeom one{1, EW_T, 0};
     switch (k) {
       case GX: return S.lx;
       case GY: return S.ly;
       case GL: return S.ll;
       case GCELL: return S.lcell;
-      case GSENS: return S.lsens;
-      default: return one;
+      default: return S.lsens;
     }
   }
 
@@ -413,9 +411,13 @@ void group_step(Group& g) {
       if (s->u_grid[a]) src_grid(*s, a + 1, t);
   }
   const int dim = g.dim;
-  if (dim == 3) g.pass(PK_YFWD, bufs(1, 0));
+  // the first pass of the step advances every member's step number
+  PassArgs y = bufs(1, 0);
+  y.advance = 1;
+  if (dim == 3) g.pass(PK_YFWD, y);
   PassArgs gr{};
   gr.mode = SM_GRAD;
+  gr.advance = dim == 3 ? 0 : 1;
   g.pass(PK_SPEC, gr);
   if (dim == 3) g.pass(PK_YINV, bufs(3));
   g.pass(PK_X_GRAD, PassArgs{});
@@ -458,9 +460,6 @@ void group_step(Group& g) {
       record_directivity(*s);
       g.launches += 4 * (int64_t)s->dir_tab.size();
     }
-  EwArgs adv{};
-  adv.kind = EW_ADVANCE;
-  g.ew(adv, GONE);
   for (Sim* s : g.sims) s->steps_done++;
 }
 

@@ -67,7 +67,16 @@ void dev_zero(void* dst, size_t bytes, void* s) {
   check(hipStreamSynchronize((hipStream_t)s), "memset sync");
 }
 
-size_t dev_max_lds() { return 65536; }
+size_t dev_max_lds() {
+  static const size_t v = []() {
+    int dev = 0, b = 0;
+    check(hipGetDevice(&dev), "hipGetDevice");
+    check(hipDeviceGetAttribute(&b, hipDeviceAttributeMaxSharedMemoryPerBlock, dev),
+          "hipDeviceGetAttribute");
+    return (size_t)b;
+  }();
+  return v;
+}
 
 // Scratch of kw_run_noalloc: every device buffer of one simulation (fields,
 // spectra, operators, staging of the inputs) is carved out of this array, so

@@ -5,6 +5,9 @@
 // kernels and one host launcher.
 #include <hip/hip_runtime.h>
 
+#include <atomic>
+#include <string>
+
 #include "kw_backend.h"
 #include "kw_variants.h"
 
@@ -65,28 +68,28 @@ using SelX = PlanCtX<KW_SIZE>;
 #endif
 
 // x passes run at most 256 threads per block; strided and last-axis passes
-// at most 512
+// at most 1024 (32 pencils of 32 threads)
 #if KW_KIND == 0
-__global__ __launch_bounds__(512) void kern(KW_KPARAMS, PassArgs A) {
+__global__ __launch_bounds__(1024) void kern(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
   KW_KBATCH;
   body_ypass<false, Sel>(c, B, A);
 }
 #elif KW_KIND == 1
-__global__ __launch_bounds__(512) void kern(KW_KPARAMS, PassArgs A) {
+__global__ __launch_bounds__(1024) void kern(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
   KW_KBATCH;
   body_ypass<true, Sel>(c, B, A);
 }
 #elif KW_KIND == 2
-__global__ __launch_bounds__(512) void kern(KW_KPARAMS, PassArgs A) {
+__global__ __launch_bounds__(1024) void kern(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
   KW_KBATCH;
   body_spec<Sel, false>(c, B, A);
 }
 // the sensor-record modes of the last-axis pass
 #define kern_aux KW_CAT(kern, _rec)
-__global__ __launch_bounds__(512) void kern_aux(KW_KPARAMS, PassArgs A) {
+__global__ __launch_bounds__(1024) void kern_aux(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
   KW_KBATCH;
   body_spec<Sel, true>(c, B, A);
@@ -138,9 +141,30 @@ __global__ __launch_bounds__(256) void kern(KW_KPARAMS, PassArgs A) {
 #error "KW_KIND must be 0..8"
 #endif
 
+// dynamic LDS above 64 KiB per block (gfx950 has 160 KiB) is requested per
+// kernel once, before its first such launch
+static void lds_limit(const void* f, std::atomic<int>& done, size_t lds) {
+  if (lds <= 65536 || done.load(std::memory_order_acquire)) return;
+  const hipError_t e = hipFuncSetAttribute(f, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                           (int)dev_max_lds());
+  if (e != hipSuccess) throw DeviceError(-3, std::string("hipFuncSetAttribute: ") +
+                                                 hipGetErrorString(e));
+  done.store(1, std::memory_order_release);
+}
+
 void KW_CAT(KW_CAT(KW_CAT(launch_v_, KW_KIND), _), KW_SIZE)(int dim, int nb, int T, size_t lds,
                                                             void* s, const Batch& B,
                                                             const PassArgs& A) {
+  static std::atomic<int> done_main{0};
+#ifdef KW_HAS_DIM2
+  static std::atomic<int> done_d2{0};
+  lds_limit((const void*)kern2, done_d2, lds);
+#endif
+#ifdef KW_HAS_AUX
+  static std::atomic<int> done_aux{0};
+  lds_limit((const void*)kern_aux, done_aux, lds);
+#endif
+  lds_limit((const void*)kern, done_main, lds);
 #ifdef KW_HAS_DIM2
   if (dim == 2) {
     hipLaunchKernelGGL(kern2, dim3(nb), dim3(T), lds, (hipStream_t)s, B.probs, B.tile_start,

@@ -237,10 +237,17 @@ KW_BODY const ProbDev& batch_prob(Ctx& c, const Batch& B, int& tile) {
 // ---------------------------------------------------------------------------
 // strided pass along y (3D only), forward or inverse, in place
 // ---------------------------------------------------------------------------
+// the first pass of a step advances the problem's step number (no block of
+// that pass reads it; every later kernel of the step does)
+KW_BODY void advance_step(const ProbDev& P, const PassArgs& A, int tile, int tid) {
+  if (A.advance && tile == 0 && tid == 0) KW_ST(P.t_index, KW_LD(P.t_index) + 1);
+}
+
 template <bool INV, class YS, class Ctx>
 KW_BODY void body_ypass(Ctx& c, const Batch& B, const PassArgs& A) {
   int tile;
   const ProbDev& P = batch_prob(c, B, tile);
+  advance_step(P, A, tile, c.tid());
   const int N = P.Ny, G = P.Gy, Pn = P.Py;
   const int64_t nH = (int64_t)P.Wm * P.Nz;
   const int64_t tH = (nH + Pn - 1) / Pn;
@@ -358,6 +365,7 @@ template <class LS, bool AUX, class Ctx>
 KW_BODY void body_spec(Ctx& c, const Batch& B, const PassArgs& A) {
   int tile;
   const ProbDev& P = batch_prob(c, B, tile);
+  advance_step(P, A, tile, c.tid());
   const bool d3 = P.dim == 3;
   const FftPlan& pl = d3 ? P.fz : P.fy;
   const int N = pl.N, G = P.Gl, Pn = P.Pl;
@@ -920,7 +928,6 @@ enum EwKind : int32_t {
   EW_SRC_GRID,       // per-step scatter grid of a many-series additive source
   EW_RECORD_POINTS,  // sensor statistics at sensor points (general mask)
   EW_RECORD_GRID,    // whole-grid statistics (*_all) not fused elsewhere
-  EW_ADVANCE,        // t_index += 1
   EW_FILL,           // zero a float buffer (graph-safe memset)
   EW_RECORD_UNS,     // non-staggered velocity component `which` at the sensor points
   EW_RECORD_SPLIT,   // compressional / shear parts of component `which`
@@ -985,10 +992,13 @@ KW_BODY void body_ew(Ctx& c, const Batch& B, const EwArgs& A) {
       return;
     }
     case EW_SRC_GRID: {
+      // launched before the step's first pass: t_index still holds the
+      // previous step
       if (i >= P.ncell) return;
       const TimeSource& s = A.which == 0 ? P.srcp : P.srcu[A.which - 1];
       const int32_t k = KW_LD(s.map + i);
-      const float v = (k >= 0 && s.len >= t) ? s.value(k, t - 1) : 0.f;
+      const int ts = t + 1;
+      const float v = (k >= 0 && s.len >= ts) ? s.value(k, ts - 1) : 0.f;
       KW_ST((float*)s.grid + i, v);
       return;
     }
@@ -1091,10 +1101,6 @@ KW_BODY void body_ew(Ctx& c, const Batch& B, const EwArgs& A) {
       const int fi = t - P.record_start + 1;
       const int64_t cell = P.sens_idx ? (int64_t)KW_LD(P.sens_idx + i) : i;
       KW_ST(P.p_ts + i + (int64_t)(fi - 1) * P.nsens, KW_LD(P.dirp + cell));
-      return;
-    }
-    case EW_ADVANCE: {
-      if (tile == 0 && c.tid() == 0) KW_ST(P.t_index, t + 1);
       return;
     }
     case EW_FILL: {

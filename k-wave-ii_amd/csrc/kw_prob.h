@@ -161,7 +161,10 @@ struct ProbDev {
   int32_t time_rev;
   const int32_t* tr_map;  // per cell: sensor point or -1
   const float* tr_data;   // [nsens x Nt], original (not reversed) column order
-  // current step (1-based), advanced on the device at the end of each step
+  // step number on the device: the number of steps done until the first
+  // pass of a step advances it to the step being computed (1-based), so
+  // kernels after that pass read the current step and the kernels before it
+  // (the per-step source grids) read the step before
   int32_t* t_index;
   // per-step decisions taken on the device from t_index, so that one launch
   // can serve problems that are at different steps: p is stored to HBM at
@@ -224,6 +227,10 @@ struct PassArgs {
   float* real[3];   // real grids for r2c/c2r (per problem when nprob == 1)
   const float* table;  // filter table for SM_FILTER (nprob == 1)
   int32_t flags;
+  // set on the first pass of every time step (the gradient's y pass in 3D,
+  // its last-axis pass in 2D), which reads no step number: one thread per
+  // problem advances t_index to the step being computed
+  int32_t advance;
 };
 
 enum : int32_t {

@@ -462,13 +462,12 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream, Arena* ar
   for (int b = 0; b < nS; ++b) P.S[b] = S->alloc_zero<cf>(G.nspec);
   for (int b = nS; b < 4; ++b) P.S[b] = P.S[nS - 1];
 
-  // step counter
+  // step counter: steps done (the first pass of each step advances it)
   int32_t* dt_index = S->alloc<int32_t>(1);
-  const std::vector<int32_t> one{1};
-  dev_h2d(dt_index, one.data(), sizeof(int32_t), S->stream);
+  const std::vector<int32_t> zero{0};
+  dev_h2d(dt_index, zero.data(), sizeof(int32_t), S->stream);
   P.t_index = dt_index;
   S->d_tile0 = S->alloc<int32_t>(1);
-  const std::vector<int32_t> zero{0};
   dev_h2d(S->d_tile0, zero.data(), sizeof(int32_t), S->stream);
 
   // sources
@@ -651,11 +650,15 @@ PassArgs bufs(int nb, int b0, int b1, int b2) {
   return a;
 }
 
+// the gradient opens every step: its first pass advances the step number
 static void gradient(Sim& S, int xflags) {
   const int dim = S.G.dim;
-  if (dim == 3) S.pass(PK_YFWD, bufs(1, 0));
+  PassArgs y = bufs(1, 0);
+  y.advance = 1;
+  if (dim == 3) S.pass(PK_YFWD, y);
   PassArgs g{};
   g.mode = SM_GRAD;
+  g.advance = dim == 3 ? 0 : 1;
   S.pass(PK_SPEC, g);
   if (dim == 3) S.pass(PK_YINV, bufs(3));
   PassArgs x{};
@@ -847,9 +850,6 @@ void step_once(Sim& S) {
     }
     record_step(S, t, false);
   }
-  EwArgs adv{};
-  adv.kind = EW_ADVANCE;
-  S.ew(adv, 1);
   S.steps_done++;
 }
 

@@ -150,6 +150,15 @@ inline int max_pencils_env() {
   return v > 0 ? v : 1 << 30;
 }
 
+// pencils per block of the strided y pass and of the last-axis pass
+// (each pencil row of a block is P consecutive complex values in HBM)
+inline int pencils_env(const char* name, int dflt) {
+  const char* e = std::getenv(name);
+  if (!e) return dflt;
+  const int v = std::atoi(e);
+  return v > 0 ? v : dflt;
+}
+
 // Bump allocator over a device region the caller owns (the static scratch
 // array of the entry point that allocates nothing): 256-byte aligned
 // buffers, released all at once when the region is reused.
@@ -388,10 +397,10 @@ struct Sim {
       lx.lds = pencil * (size_t)P;
       if (Gx > 256 || lx.lds > maxl) fail(KW_EUNSUP, "grid size Nx too large for the x pass");
     }
-    // strided y pass (3D): <= 512 threads, LDS P*(Ny+1)
+    // strided y pass (3D): <= 1024 threads, LDS P*(Ny+1)
     if (G.dim == 3) {
       const int Gy = Gof(G.Ny);
-      int P = std::max(1, std::min(16, 512 / Gy));
+      int P = std::max(1, std::min(pencils_env("KW_PENCILS_Y", 16), 1024 / Gy));
       P = std::min<int64_t>(P, (int64_t)(maxl / (sizeof(cf) * (G.Ny + 1))));
       P = std::max(1, std::min(P, capP));
       this->P.Gy = Gy;
@@ -400,13 +409,13 @@ struct Sim {
       ly.nb = (int)((nH + P - 1) / P + (G.nyq ? (G.Nz + P - 1) / P : 0));
       ly.T = P * Gy;
       ly.lds = sizeof(cf) * (size_t)P * (G.Ny + 1);
-      if (Gy > 512 || ly.lds > maxl) fail(KW_EUNSUP, "grid size Ny too large for the y pass");
+      if (Gy > 1024 || ly.lds > maxl) fail(KW_EUNSUP, "grid size Ny too large for the y pass");
     }
-    // last-axis pass: <= 512 threads, LDS 2*P*N
+    // last-axis pass: <= 1024 threads, LDS 2*P*N
     {
       const int Nl = G.dim == 3 ? G.Nz : G.Ny;
       const int Gl = Gof(Nl);
-      int P = std::max(1, std::min(16, 512 / Gl));
+      int P = std::max(1, std::min(pencils_env("KW_PENCILS_L", 16), 1024 / Gl));
       P = std::min<int64_t>(P, (int64_t)(maxl / (2 * sizeof(cf) * Nl)));
       P = std::max(1, std::min(P, capP));
       this->P.Gl = Gl;
@@ -416,7 +425,7 @@ struct Sim {
       ll.nb = (int)((nH + P - 1) / P + (G.nyq ? (nQ + P - 1) / P : 0));
       ll.T = P * Gl;
       ll.lds = 2 * sizeof(cf) * (size_t)P * Nl;
-      if (Gl > 512 || ll.lds > maxl) fail(KW_EUNSUP, "grid size too large for the last-axis pass");
+      if (Gl > 1024 || ll.lds > maxl) fail(KW_EUNSUP, "grid size too large for the last-axis pass");
     }
     lcell.nb = (int)((G.ncell + EW_T - 1) / EW_T);
     lsens.nb = (int)((std::max<int64_t>(nsens, 1) + EW_T - 1) / EW_T);
