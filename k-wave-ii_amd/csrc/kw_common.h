@@ -202,6 +202,7 @@ struct PencilCtx {
   bool wave;
   KW_BODY int tid() const { return c.tid(); }
   KW_BODY int bid() const { return c.bid(); }
+  KW_BODY int nthreads() const { return c.nthreads(); }
   KW_BODY cf* lds() const { return c.lds(); }
   KW_BODY void sync() const {
     if (wave)

@@ -149,10 +149,11 @@ thread_local Runner* tl_runner = nullptr;
 
 struct EmuCtx {
   Runner* R;
-  int t, b;
+  int t, b, nt;
   cf* L;
   int tid() const { return t; }
   int bid() const { return b; }
+  int nthreads() const { return nt; }
   cf* lds() const { return L; }
   void sync() const;
   void wsync() const;
@@ -177,6 +178,7 @@ struct Runner {
   std::vector<long> lw_sync, lw_tick;
   int cur = 0;
   int bid = 0;
+  int nthr = 0;
   const std::function<void(EmuCtx&)>* body = nullptr;
 
   void yield(int kind) {
@@ -192,13 +194,14 @@ struct Runner {
   static void trampoline() {
     Runner* R = tl_runner;
     const int i = R->cur;
-    EmuCtx c{R, i, R->bid, R->lds.data()};
+    EmuCtx c{R, i, R->bid, R->nthr, R->lds.data()};
     (*R->body)(c);
     R->lanes[i].done = true;
   }
 
   void run_block(int b, int T) {
     bid = b;
+    nthr = T;
     if ((int)lanes.size() < T) lanes.resize(T);
     const float qnan = std::nanf("");
     for (auto& v : lds) v = cf{qnan, qnan};

@@ -88,11 +88,36 @@ static_assert(MaxB<2>::v * 2 <= kMaxSlots && MaxB<3>::v * 3 <= kMaxSlots &&
                   MaxB<8>::v * 8 <= kMaxSlots,
               "slot bound");
 
+// Twiddle table of a plan as the stages read it: in HBM (each read a vector
+// memory load through the same address path as the data streams), or
+// copied once per block into LDS (stage_twiddles).
+struct TwG {
+  const cf* p;
+};
+struct TwL {
+  const cf* p;
+};
+
 template <bool INV>
-KW_BODY cf twiddle(const cf* tw, int idx) {
-  cf w = KW_LD32(tw, (uint32_t)idx);
+KW_BODY cf twiddle(const TwG& tw, int idx) {
+  cf w = KW_LD32(tw.p, (uint32_t)idx);
   if (INV) w.y = -w.y;
   return w;
+}
+template <bool INV>
+KW_BODY cf twiddle(const TwL& tw, int idx) {
+  cf w = lds_ld(tw.p + idx);
+  if (INV) w.y = -w.y;
+  return w;
+}
+
+// the N twiddles of a plan copied into the block's LDS at `dst` by all
+// threads of the block, followed by a block barrier
+template <class Ctx>
+KW_BODY TwL stage_twiddles(Ctx& c, const FftPlan& pl, cf* dst) {
+  for (int i = c.tid(); i < pl.N; i += c.nthreads()) lds_st(dst + i, KW_LD32(pl.tw, (uint32_t)i));
+  c.sync();
+  return TwL{dst};
 }
 
 // radix-R DFT in place; forward uses exp(-2 pi i rs/R), inverse exp(+...)
@@ -244,8 +269,8 @@ KW_BODY int mod_ns(int b, int Ns) {
 // number of butterflies per thread (exact for the fixed-length engine, an
 // upper bound for the runtime engine).  `src` must be safe to call for
 // every thread, including threads with active == false.
-template <int R, int NP, bool INV, int MB, class Ctx, class Src, class Dst>
-KW_BODY void stage(Ctx& c, int N, int Ns, int G, int g, bool active, const cf* tw, Src& src,
+template <int R, int NP, bool INV, int MB, class Ctx, class TW, class Src, class Dst>
+KW_BODY void stage(Ctx& c, int N, int Ns, int G, int g, bool active, const TW& tw, Src& src,
                    Dst& dst) {
   const int nb = N / R;
   const int twstep = N / (Ns * R);
@@ -321,8 +346,8 @@ static_assert(kRtSlots <= kMaxSlots, "slot bound");
 // Threads without a pencil (active == false) may share another pencil's LDS
 // to keep their loads in bounds, so they never store, here as in the
 // fixed-length engine.
-template <int NP, bool INV, class Ctx>
-KW_BODY void fft_stages_rt(Ctx& c, const FftPlan& pl, int g, int G, bool active,
+template <int NP, bool INV, class Ctx, class TW>
+KW_BODY void fft_stages_rt(Ctx& c, const FftPlan& pl, const TW& tw, int g, int G, bool active,
                            const LdsPencils& lds) {
   auto lsrc = [&](int j, int n) -> cf { return lds.get(j, n); };
   auto ldst = [&](int j, int n, cf v, int) { lds.set(j, n, v); };
@@ -330,20 +355,20 @@ KW_BODY void fft_stages_rt(Ctx& c, const FftPlan& pl, int g, int G, bool active,
   for (int s = 0; s < pl.nst; ++s) {
     const int R = pl.radix[s];
     switch (R) {
-      case 2: stage<2, NP, INV, MaxB<2>::v>(c, pl.N, Ns, G, g, active, pl.tw, lsrc, ldst); break;
-      case 3: stage<3, NP, INV, MaxB<3>::v>(c, pl.N, Ns, G, g, active, pl.tw, lsrc, ldst); break;
-      case 4: stage<4, NP, INV, MaxB<4>::v>(c, pl.N, Ns, G, g, active, pl.tw, lsrc, ldst); break;
-      case 5: stage<5, NP, INV, MaxB<5>::v>(c, pl.N, Ns, G, g, active, pl.tw, lsrc, ldst); break;
-      case 7: stage<7, NP, INV, MaxB<7>::v>(c, pl.N, Ns, G, g, active, pl.tw, lsrc, ldst); break;
-      default: stage<8, NP, INV, MaxB<8>::v>(c, pl.N, Ns, G, g, active, pl.tw, lsrc, ldst); break;
+      case 2: stage<2, NP, INV, MaxB<2>::v>(c, pl.N, Ns, G, g, active, tw, lsrc, ldst); break;
+      case 3: stage<3, NP, INV, MaxB<3>::v>(c, pl.N, Ns, G, g, active, tw, lsrc, ldst); break;
+      case 4: stage<4, NP, INV, MaxB<4>::v>(c, pl.N, Ns, G, g, active, tw, lsrc, ldst); break;
+      case 5: stage<5, NP, INV, MaxB<5>::v>(c, pl.N, Ns, G, g, active, tw, lsrc, ldst); break;
+      case 7: stage<7, NP, INV, MaxB<7>::v>(c, pl.N, Ns, G, g, active, tw, lsrc, ldst); break;
+      default: stage<8, NP, INV, MaxB<8>::v>(c, pl.N, Ns, G, g, active, tw, lsrc, ldst); break;
     }
     Ns *= R;
   }
 }
 
-template <int NP, bool INV, class Ctx, class Src, class Dst>
-KW_BODY void fft_rt(Ctx& c, const FftPlan& pl, int g, int G, bool active, Src& src, Dst& dst,
-                    const LdsPencils& lds) {
+template <int NP, bool INV, class Ctx, class TW, class Src, class Dst>
+KW_BODY void fft_rt(Ctx& c, const FftPlan& pl, const TW& tw, int g, int G, bool active, Src& src,
+                    Dst& dst, const LdsPencils& lds) {
   const int N = pl.N;
   g = kw_fresh(g);
 #pragma unroll
@@ -358,7 +383,7 @@ KW_BODY void fft_rt(Ctx& c, const FftPlan& pl, int g, int G, bool active, Src& s
     }
   }
   c.sync();
-  fft_stages_rt<NP, INV>(c, pl, g, G, active, lds);
+  fft_stages_rt<NP, INV>(c, pl, tw, g, G, active, lds);
 #pragma unroll
   for (int i = 0; i < kRtSlots; ++i) {
     const int n = g + i * G;
@@ -388,8 +413,8 @@ struct CtStages {
   static constexpr int R = ct_radix(N, S);
   static constexpr int Ns = ct_ns(N, S);
   static constexpr int MB = (N / R + G - 1) / G;
-  template <class Ctx, class Src, class Dst, class LSrc, class LDst>
-  KW_BODY static void run(Ctx& c, int g, bool active, const cf* tw, Src& src0, Dst& dstN,
+  template <class Ctx, class TW, class Src, class Dst, class LSrc, class LDst>
+  KW_BODY static void run(Ctx& c, int g, bool active, const TW& tw, Src& src0, Dst& dstN,
                           LSrc& lsrc, LDst& ldst) {
     if constexpr (S == 0 && NST == 1) {
       stage<R, NP, INV, MB>(c, N, Ns, G, g, active, tw, src0, dstN);
@@ -405,8 +430,8 @@ struct CtStages {
   }
 };
 
-template <int N, int G, int NP, bool INV, class Ctx, class Src, class Dst>
-KW_BODY void fft_ct(Ctx& c, const cf* tw, int g, bool active, Src& src0, Dst& dstN,
+template <int N, int G, int NP, bool INV, class Ctx, class TW, class Src, class Dst>
+KW_BODY void fft_ct(Ctx& c, const TW& tw, int g, bool active, Src& src0, Dst& dstN,
                     const LdsPencils& lds) {
   static_assert(ct_nst(N) > 0, "length must factor into 2, 3 and 5");
   g = kw_fresh(g);
@@ -445,7 +470,13 @@ struct PlanRt {
   template <int NP, bool INV, class Ctx, class Src, class Dst>
   KW_BODY static void fft(Ctx& c, const FftPlan& pl, int g, int G, bool active, Src& src,
                           Dst& dst, const LdsPencils& lds) {
-    fft_rt<NP, INV>(c, pl, g, G, active, src, dst, lds);
+    fft_rt<NP, INV>(c, pl, TwG{pl.tw}, g, G, active, src, dst, lds);
+  }
+  // the same with the twiddles read from `tw` (for example staged in LDS)
+  template <int NP, bool INV, class Ctx, class TW, class Src, class Dst>
+  KW_BODY static void fft_tw(Ctx& c, const FftPlan& pl, const TW& tw, int g, int G, bool active,
+                             Src& src, Dst& dst, const LdsPencils& lds) {
+    fft_rt<NP, INV>(c, pl, tw, g, G, active, src, dst, lds);
   }
   template <class F>
   KW_BODY static void last_slots(const FftPlan& pl, int g, int G, bool active, F& f) {
@@ -479,7 +510,12 @@ struct PlanCt {
   template <int NP, bool INV, class Ctx, class Src, class Dst>
   KW_BODY static void fft(Ctx& c, const FftPlan& pl, int g, int, bool active, Src& src, Dst& dst,
                           const LdsPencils& lds) {
-    fft_ct<N_, G, NP, INV>(c, pl.tw, g, active, src, dst, lds);
+    fft_ct<N_, G, NP, INV>(c, TwG{pl.tw}, g, active, src, dst, lds);
+  }
+  template <int NP, bool INV, class Ctx, class TW, class Src, class Dst>
+  KW_BODY static void fft_tw(Ctx& c, const FftPlan&, const TW& tw, int g, int, bool active,
+                             Src& src, Dst& dst, const LdsPencils& lds) {
+    fft_ct<N_, G, NP, INV>(c, tw, g, active, src, dst, lds);
   }
   template <class F>
   KW_BODY static void last_slots(const FftPlan&, int g, int, bool active, F& f) {

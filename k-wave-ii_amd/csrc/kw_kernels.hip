@@ -18,6 +18,7 @@ namespace kw {
 struct GpuCtx {
   __device__ __forceinline__ int tid() const { return (int)threadIdx.x; }
   __device__ __forceinline__ int bid() const { return (int)blockIdx.x; }
+  __device__ __forceinline__ int nthreads() const { return (int)blockDim.x; }
   __device__ __forceinline__ void sync() const { __syncthreads(); }
   // wavefront barrier: LDS writes of the wavefront before it are visible to
   // its LDS reads after it (a wavefront's LDS operations execute in order;

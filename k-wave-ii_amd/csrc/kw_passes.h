@@ -63,6 +63,7 @@ struct XTile {
   // (row * Wm), Nyquist entry (nspec_main + row); y and z of each row
   uint32_t c0, c1, h0, h1, q0, q1, y0, y1, z0, z1;
   LdsPencils lds;
+  TwL tw;  // the x twiddles, staged at the start of the block's LDS
 };
 
 template <class XS, class Ctx>
@@ -95,8 +96,10 @@ KW_BODY XTile x_tile(Ctx& c, const ProbDev& P, int tile) {
   T.z0 = r0 / (uint32_t)P.Ny;
   T.y1 = r1 % (uint32_t)P.Ny;
   T.z1 = r1 / (uint32_t)P.Ny;
+  // LDS: the Nx twiddles (staged by the whole block), then the pencils
+  T.tw = stage_twiddles(c, P.fx, c.lds());
   const int stride = x_lds_stride(P.Nx);
-  T.lds = LdsPencils{c.lds() + (int64_t)p * kNPX * stride, stride, 1, true};
+  T.lds = LdsPencils{c.lds() + P.Nx + (int64_t)p * kNPX * stride, stride, 1, true};
   return T;
 }
 
@@ -128,7 +131,7 @@ template <class XS, int NP, class Ctx>
 KW_BODY void x_fwd_from_lds(Ctx& c, const ProbDev& P, const XTile& T, cf* const* S) {
   auto src = [&](int j, int n) -> cf { return T.lds.get(j, n); };
   auto dst = [&](int j, int n, cf v, int) { T.lds.set(j, n, v); };
-  XS::template fft<NP, false>(c, P.fx, T.g, T.G, T.active, src, dst, T.lds);
+  XS::template fft_tw<NP, false>(c, P.fx, T.tw, T.g, T.G, T.active, src, dst, T.lds);
   x_split_store<NP>(c, P, T, S);
 }
 
@@ -249,6 +252,9 @@ KW_BODY void body_ypass(Ctx& c, const Batch& B, const PassArgs& A) {
   const ProbDev& P = batch_prob(c, B, tile);
   advance_step(P, A, tile, c.tid());
   const int N = P.Ny, G = P.Gy, Pn = P.Py;
+  // LDS: the N twiddles, then the pencils
+  const TwL tw = stage_twiddles(c, P.fy, c.lds());
+  cf* const lp = c.lds() + N;
   const int64_t nH = (int64_t)P.Wm * P.Nz;
   const int64_t tH = (nH + Pn - 1) / Pn;
   const int tid = c.tid();
@@ -265,7 +271,7 @@ KW_BODY void body_ypass(Ctx& c, const Batch& B, const PassArgs& A) {
     if (!active) q = 0;     // read a valid pencil, never store
     base = (uint32_t)((q / P.Wm) * ((int64_t)P.Wm * P.Ny) + (q % P.Wm));
     stride = (uint32_t)P.Wm;
-    lds = LdsPencils{c.lds() + p, 0, Pn, false};
+    lds = LdsPencils{lp + p, 0, Pn, false};
   } else {
     p = tid / G;
     g = tid % G;
@@ -277,13 +283,13 @@ KW_BODY void body_ypass(Ctx& c, const Batch& B, const PassArgs& A) {
     }
     base = (uint32_t)(P.nspec_main + q * P.Ny);
     stride = 1;
-    lds = LdsPencils{c.lds() + (int64_t)p * (N + 1), 0, 1, false};
+    lds = LdsPencils{lp + (int64_t)p * (N + 1), 0, 1, false};
   }
   for (int b = 0; b < A.nbuf; ++b) {
     cf* S = P.S[A.buf[b]];
     auto ld = [&](int, int n) -> cf { return KW_LD32(S, base + (uint32_t)n * stride); };
     auto st = [&](int, int n, cf v, int) { KW_ST32(S, base + (uint32_t)n * stride, v); };
-    YS::template fft<1, INV>(c, P.fy, g, G, active, ld, st, lds);
+    YS::template fft_tw<1, INV>(c, P.fy, tw, g, G, active, ld, st, lds);
   }
 }
 
@@ -295,7 +301,8 @@ KW_BODY void body_ypass(Ctx& c, const Batch& B, const PassArgs& A) {
 // ---------------------------------------------------------------------------
 // Sensor-record modes of the last-axis pass (see body_spec).
 template <class LS, class Ctx, class Addr, class TAddr, class ToA>
-KW_BODY void spec_aux(Ctx& c, const ProbDev& P, const PassArgs& A, const FftPlan& pl, int N, int G,
+KW_BODY void spec_aux(Ctx& c, const ProbDev& P, const PassArgs& A, const FftPlan& pl,
+                      const TwL& tw, int N, int G,
                       int last, int g, bool active, int tile, int64_t tH, int kx, int ky,
                       const LdsPencils& la, const LdsPencils& lb, const LdsPencils& lw, Addr& addr,
                       TAddr& taddr, ToA& to_a) {
@@ -307,14 +314,14 @@ KW_BODY void spec_aux(Ctx& c, const ProbDev& P, const PassArgs& A, const FftPlan
     cf* S = P.S[A.buf[0]];
     const cf* sh = P.ushift[d];
     auto ld = [&](int, int n) -> cf { return KW_LD32(S, addr(n)); };
-    LS::template fft<1, false>(c, pl, g, G, active, ld, to_a, la);
+    LS::template fft_tw<1, false>(c, pl, tw, g, G, active, ld, to_a, la);
     const bool along = d == last;
     const cf sc = (along || !active) ? cf{0.f, 0.f} : KW_LD(sh + (d == 0 ? kx : ky));
     auto src = [&](int, int n) -> cf {
       return cmul(along ? KW_LD32(sh, (uint32_t)n) : sc, la.get(0, n));
     };
     auto st = [&](int, int n, cf v, int) { KW_ST32(S, addr(n), v); };
-    LS::template fft<1, true>(c, pl, g, G, active, src, st, lw);
+    LS::template fft_tw<1, true>(c, pl, tw, g, G, active, src, st, lw);
   } else if (A.mode == SM_SPLIT) {
     // compressional part of the velocity (kspaceFirstOrder_extractSensorData.m:
     // 124-292): D = sum_b khat_b shift_b U_b is accumulated in lb (each thread
@@ -337,7 +344,7 @@ KW_BODY void spec_aux(Ctx& c, const ProbDev& P, const PassArgs& A, const FftPlan
         const cf t = cscale(cmul(s, v), KW_LD32(kh, taddr(n)));
         lb.set(0, n, b == 0 ? t : cadd(lb.get(0, n), t));
       };
-      LS::template fft<1, false>(c, pl, g, G, active, ld, acc, la);
+      LS::template fft_tw<1, false>(c, pl, tw, g, G, active, ld, acc, la);
     }
     for (int a = 0; a < nc; ++a) {
       cf* Sa = P.S[A.buf[a]];
@@ -351,7 +358,7 @@ KW_BODY void spec_aux(Ctx& c, const ProbDev& P, const PassArgs& A, const FftPlan
         return cscale(lb.get(0, n), k);
       };
       auto st = [&](int, int n, cf v, int) { KW_ST32(Sa, addr(n), v); };
-      LS::template fft<1, true>(c, pl, g, G, active, src, st, la);
+      LS::template fft_tw<1, true>(c, pl, tw, g, G, active, src, st, la);
     }
   }
 }
@@ -403,8 +410,11 @@ KW_BODY void body_spec(Ctx& c, const Batch& B, const PassArgs& A) {
     kx = 0;
     ky = d3 ? 0 : -1;
   }
-  LdsPencils la{c.lds() + p, 0, Pn, false};
-  LdsPencils lb{c.lds() + (int64_t)Pn * N + p, 0, Pn, false};
+  // LDS: the N twiddles, then pencil buffers la and lb
+  const TwL tw = stage_twiddles(c, pl, c.lds());
+  cf* const lp = c.lds() + N;
+  LdsPencils la{lp + p, 0, Pn, false};
+  LdsPencils lb{lp + (int64_t)Pn * N + p, 0, Pn, false};
   // workspace of the inverse transforms that read la: la itself for a
   // fixed-length plan (every thread of a block then has a pencil or a
   // pencil of its own), lb for runtime plans, whose launches may be shared
@@ -428,11 +438,12 @@ KW_BODY void body_spec(Ctx& c, const Batch& B, const PassArgs& A) {
   };
 
   if constexpr (AUX) {
-    spec_aux<LS>(c, P, A, pl, N, G, last, g, active, tile, tH, kx, ky, la, lb, lw, addr, taddr, to_a);
+    spec_aux<LS>(c, P, A, pl, tw, N, G, last, g, active, tile, tH, kx, ky, la, lb, lw, addr, taddr,
+                 to_a);
   } else if (A.mode == SM_GRAD) {
     cf* S0 = P.S[0];
     auto ld = [&](int, int n) -> cf { return KW_LD32(S0, addr(n)); };
-    LS::template fft<1, false>(c, pl, g, G, active, ld, to_ak, la);
+    LS::template fft_tw<1, false>(c, pl, tw, g, G, active, ld, to_ak, la);
     // C = ifft(kappa .* S); x (and in 3D y) derivatives are per-pencil
     // constants times C
     const cf dx = active ? KW_LD(P.dpos[0] + kx) : cf{0.f, 0.f};
@@ -442,18 +453,18 @@ KW_BODY void body_spec(Ctx& c, const Batch& B, const PassArgs& A) {
       KW_ST32(P.S[0], addr(n), cmul(dx, v));
       if (d3) KW_ST32(P.S[1], addr(n), cmul(dy, v));
     };
-    LS::template fft<1, true>(c, pl, g, G, active, src1, st1, lb);
+    LS::template fft_tw<1, true>(c, pl, tw, g, G, active, src1, st1, lb);
     // derivative along the pencil axis
     const cf* dl = P.dpos[last];
     auto src2 = [&](int, int n) -> cf { return cmul(KW_LD32(dl, (uint32_t)n), la.get(0, n)); };
     cf* So = P.S[last];
     auto st2 = [&](int, int n, cf v, int) { KW_ST32(So, addr(n), v); };
-    LS::template fft<1, true>(c, pl, g, G, active, src2, st2, lb);
+    LS::template fft_tw<1, true>(c, pl, tw, g, G, active, src2, st2, lb);
   } else if (A.mode == SM_DIV) {
     for (int d = 0; d < P.dim; ++d) {
       cf* S = P.S[d];
       auto ld = [&](int, int n) -> cf { return KW_LD32(S, addr(n)); };
-      LS::template fft<1, false>(c, pl, g, G, active, ld, to_ak, la);
+      LS::template fft_tw<1, false>(c, pl, tw, g, G, active, ld, to_ak, la);
       const cf dc = (d == last || !active) ? cf{0.f, 0.f}
                                            : KW_LD(P.dneg[d] + (d == 0 ? kx : ky));
       const cf* dl = P.dneg[d];
@@ -462,7 +473,7 @@ KW_BODY void body_spec(Ctx& c, const Batch& B, const PassArgs& A) {
         return cmul(along ? KW_LD32(dl, (uint32_t)n) : dc, la.get(0, n));
       };
       auto st = [&](int, int n, cf v, int) { KW_ST32(S, addr(n), v); };
-      LS::template fft<1, true>(c, pl, g, G, active, src, st, lw);
+      LS::template fft_tw<1, true>(c, pl, tw, g, G, active, src, st, lw);
     }
   } else {
     // SM_ABS: S0 <- nabla1 .* S0, S1 <- nabla2 .* S1;  SM_FILTER: one table
@@ -471,10 +482,10 @@ KW_BODY void body_spec(Ctx& c, const Batch& B, const PassArgs& A) {
       cf* S = P.S[A.mode == SM_ABS ? j : A.buf[0]];
       const float* tab = A.mode == SM_ABS ? (j == 0 ? P.nabla1 : P.nabla2) : A.table;
       auto ld = [&](int, int n) -> cf { return KW_LD32(S, addr(n)); };
-      LS::template fft<1, false>(c, pl, g, G, active, ld, to_a, la);
+      LS::template fft_tw<1, false>(c, pl, tw, g, G, active, ld, to_a, la);
       auto src = [&](int, int n) -> cf { return cscale(la.get(0, n), KW_LD32(tab, taddr(n))); };
       auto st = [&](int, int n, cf v, int) { KW_ST32(S, addr(n), v); };
-      LS::template fft<1, true>(c, pl, g, G, active, src, st, lw);
+      LS::template fft_tw<1, true>(c, pl, tw, g, G, active, src, st, lw);
     }
   }
 }
@@ -498,7 +509,7 @@ KW_BODY void body_x_r2c(Ctx& cb, const Batch& B, const PassArgs& A) {
       return cf{a, T.has1 ? bb : 0.f};
     };
     auto dst = [&](int j, int n, cf v, int) { T.lds.set(j, n, v); };
-    XS::template fft<1, false>(c, P.fx, T.g, T.G, T.active, src, dst, T.lds);
+    XS::template fft_tw<1, false>(c, P.fx, T.tw, T.g, T.G, T.active, src, dst, T.lds);
     cf* S[1] = {P.S[A.buf[b]]};
     x_split_store<1>(c, P, T, S);
   }
@@ -522,7 +533,7 @@ KW_BODY void body_x_c2r(Ctx& cb, const Batch& B, const PassArgs& A) {
       KW_ST32(f, i1, o1 + v.y);
       KW_ST32(f, i0, o0 + v.x);
     };
-    XS::template fft<1, true>(c, P.fx, T.g, T.G, T.active, src, dst, T.lds);
+    XS::template fft_tw<1, true>(c, P.fx, T.tw, T.g, T.G, T.active, src, dst, T.lds);
   }
 }
 
@@ -533,7 +544,7 @@ KW_BODY void x_inverse(PC& c, const ProbDev& P, const XTile& T, const cf* S, cf*
   HalfPairLoad<1> src{&P, {S}, T.h0, T.h1, T.q0, T.q1, T.has1};
   auto dst = [&](int, int, cf v, int slot) { gv[slot] = v; };
   KW_PHASE_FENCE();
-  XS::template fft<1, true>(c, P.fx, T.g, T.G, T.active, src, dst, T.lds);
+  XS::template fft_tw<1, true>(c, P.fx, T.tw, T.g, T.G, T.active, src, dst, T.lds);
   KW_PHASE_FENCE();
 }
 

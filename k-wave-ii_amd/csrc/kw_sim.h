@@ -135,12 +135,18 @@ struct LaunchGeom {
 };
 
 // LDS of a launch of pass k, whose geometry reserves `full` bytes.  The
-// last-axis pass holds two pencil buffers per pencil (la, lb); with a
+// strided and last-axis passes start their LDS with the N twiddles of the
+// transform, then the pencils.  The last-axis pass holds two pencil buffers
+// per pencil (la, lb); with a
 // fixed-length plan only the gradient uses lb (its two inverse transforms
 // both read la), so the other modes launch with la alone and fit twice as
 // many blocks per CU (the velocity split keeps its sum in lb too).
 inline size_t launch_lds(PassKind k, int fixed, const PassArgs& a, size_t full) {
-  return (k == PK_SPEC && fixed && a.mode != SM_GRAD && a.mode != SM_SPLIT) ? full / 2 : full;
+  // `full` holds the N twiddles and the two pencil buffers
+  const size_t tw = sizeof(cf) * (size_t)fixed;
+  return (k == PK_SPEC && fixed && a.mode != SM_GRAD && a.mode != SM_SPLIT)
+             ? (full - tw) / 2 + tw
+             : full;
 }
 
 inline int max_pencils_env() {
@@ -381,42 +387,44 @@ struct Sim {
     // ceil(N/7) for lengths with a factor 7 (one radix-7 butterfly per
     // thread; every other stage then has at most its usual count)
     auto Gof = [](int N) { return N % 7 == 0 ? (N + 6) / 7 : (N + KW_E - 1) / KW_E; };
-    // x pass: <= 256 threads, LDS P * kNPX padded pencils; the fixed-length
-    // x kernels run x_threads(Nx) threads per pencil
+    // x pass: <= 256 threads, LDS Nx twiddles + P * kNPX padded pencils; the
+    // fixed-length x kernels run x_threads(Nx) threads per pencil
     {
       const int Gx = fixed_len(PK_X_GRAD) ? x_threads(G.Nx) : Gof(G.Nx);
       const size_t pencil = sizeof(cf) * (size_t)kNPX * x_lds_stride(G.Nx);
+      const size_t twb = sizeof(cf) * (size_t)G.Nx;
       int P = std::max(1, 256 / Gx);
-      P = std::min<int64_t>(P, (int64_t)(maxl / pencil));
+      P = std::min<int64_t>(P, (int64_t)((maxl - twb) / pencil));
       P = std::max(1, std::min(P, capP));
       this->P.Gx = Gx;
       this->P.Px = P;
       const int64_t npairs = (G.nrows + 1) / 2;
       lx.nb = (int)((npairs + P - 1) / P);
       lx.T = P * Gx;
-      lx.lds = pencil * (size_t)P;
+      lx.lds = pencil * (size_t)P + twb;
       if (Gx > 256 || lx.lds > maxl) fail(KW_EUNSUP, "grid size Nx too large for the x pass");
     }
-    // strided y pass (3D): <= 1024 threads, LDS P*(Ny+1)
+    // strided y pass (3D): <= 1024 threads, LDS Ny twiddles + P*(Ny+1)
     if (G.dim == 3) {
       const int Gy = Gof(G.Ny);
       int P = std::max(1, std::min(pencils_env("KW_PENCILS_Y", 16), 1024 / Gy));
-      P = std::min<int64_t>(P, (int64_t)(maxl / (sizeof(cf) * (G.Ny + 1))));
+      P = std::min<int64_t>(
+          P, (int64_t)((maxl - sizeof(cf) * G.Ny) / (sizeof(cf) * (G.Ny + 1))));
       P = std::max(1, std::min(P, capP));
       this->P.Gy = Gy;
       this->P.Py = P;
       const int64_t nH = (int64_t)G.Wm * G.Nz;
       ly.nb = (int)((nH + P - 1) / P + (G.nyq ? (G.Nz + P - 1) / P : 0));
       ly.T = P * Gy;
-      ly.lds = sizeof(cf) * (size_t)P * (G.Ny + 1);
+      ly.lds = sizeof(cf) * ((size_t)P * (G.Ny + 1) + G.Ny);
       if (Gy > 1024 || ly.lds > maxl) fail(KW_EUNSUP, "grid size Ny too large for the y pass");
     }
-    // last-axis pass: <= 1024 threads, LDS 2*P*N
+    // last-axis pass: <= 1024 threads, LDS N twiddles + 2*P*N
     {
       const int Nl = G.dim == 3 ? G.Nz : G.Ny;
       const int Gl = Gof(Nl);
       int P = std::max(1, std::min(pencils_env("KW_PENCILS_L", 16), 1024 / Gl));
-      P = std::min<int64_t>(P, (int64_t)(maxl / (2 * sizeof(cf) * Nl)));
+      P = std::min<int64_t>(P, (int64_t)((maxl - sizeof(cf) * Nl) / (2 * sizeof(cf) * Nl)));
       P = std::max(1, std::min(P, capP));
       this->P.Gl = Gl;
       this->P.Pl = P;
@@ -424,7 +432,7 @@ struct Sim {
       const int64_t nQ = G.dim == 3 ? G.Ny : 1;
       ll.nb = (int)((nH + P - 1) / P + (G.nyq ? (nQ + P - 1) / P : 0));
       ll.T = P * Gl;
-      ll.lds = 2 * sizeof(cf) * (size_t)P * Nl;
+      ll.lds = sizeof(cf) * (2 * (size_t)P * Nl + Nl);
       if (Gl > 1024 || ll.lds > maxl) fail(KW_EUNSUP, "grid size too large for the last-axis pass");
     }
     lcell.nb = (int)((G.ncell + EW_T - 1) / EW_T);
