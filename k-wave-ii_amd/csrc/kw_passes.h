@@ -783,6 +783,20 @@ KW_BODY void body_x_div(Ctx& cb, const Batch& B, const PassArgs& A) {
   // kept in registers across the components
   float du0[kMaxSlots], du1[kMaxSlots], rs0[kMaxSlots], rs1[kMaxSlots];
   float k0[kMaxSlots], k1[kMaxSlots];
+  // a single-series additive pressure source adds s(t) K_M to every density
+  // component: K_M is read once here and kept in registers for all of them
+  float sk0[kMaxSlots], sk1[kMaxSlots];
+  const bool km_once = psrc && src_form(P.srcp) == 0;
+  if (km_once) {
+    KW_PHASE_FENCE();
+    const float s0 = src_value0<0>(P.srcp, t);
+    auto f = [&](int slot, int n) {
+      sk0[slot] = s0 * KW_LD32(P.srcp.km, T.c0 + n);
+      sk1[slot] = s0 * KW_LD32(P.srcp.km, T.c1 + n);
+    };
+    XS::last_slots(P.fx, T.g, T.G, T.active, f);
+    KW_PHASE_FENCE();
+  }
   const bool nonlin = P.nonlinear != 0;
   with_flag(nonlin, [&](auto NL) {
     constexpr bool nl = decltype(NL)::value;
@@ -823,7 +837,15 @@ KW_BODY void body_x_div(Ctx& cb, const Batch& B, const PassArgs& A) {
       // pressure source (kspaceFirstOrder3D.m:930-962) as a phase of its own;
       // a 1-D problem has its density in rho_x alone
       // (kspaceFirstOrder1D.m:716-743)
-      if (psrc && d < P.dim) x_source<XS>(P, T, P.srcp, t, av, bv);
+      if (km_once && d < P.dim) {
+        auto f = [&](int slot, int) {
+          av[slot] = av[slot] + sk0[slot];
+          bv[slot] = bv[slot] + sk1[slot];
+        };
+        XS::last_slots(P.fx, T.g, T.G, T.active, f);
+      } else if (psrc && d < P.dim) {
+        x_source<XS>(P, T, P.srcp, t, av, bv);
+      }
       auto acc = [&](int slot, int) {
         rs0[slot] = rs0[slot] + av[slot];
         rs1[slot] = rs1[slot] + bv[slot];
