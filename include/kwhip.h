@@ -37,6 +37,8 @@ extern "C" {
 #define KW_EDEVICE (-3)  /* HIP runtime error */
 #define KW_ENOMEM (-4)   /* device allocation failed */
 #define KW_ESTATE (-5)   /* call not valid in the handle's current state */
+#define KW_ENUMERIC (-6) /* engine: the simulation's pressure became non-finite
+                            (an unstable time step); only that request fails */
 
 /* sensor.record flags (reference record names,
  * +kwave/+legacy/private/kspaceFirstOrder_inputChecking.m:399-402) */
@@ -311,7 +313,12 @@ int kw_submit(kw_engine* engine, const kw_problem* prob, int64_t* ticket, char* 
 int kw_poll(kw_engine* engine, int64_t ticket, kw_ticket_info* info, char* err, size_t errlen);
 /* Blocks until the ticket has finished, copies its outputs (as kw_run) into
  * `out` (NULL: discard them) and releases the ticket.  Returns the
- * simulation's error code if it failed, KW_ESTATE if it was cancelled. */
+ * simulation's error code if it failed, KW_ESTATE if it was cancelled.
+ * A simulation whose pressure becomes non-finite (NaN or Inf, flagged on
+ * the device as the pressure is formed and checked after every round of
+ * steps) fails alone with KW_ENUMERIC and leaves its group; the other
+ * simulations run on, unaffected.  A HIP error while a group's round is
+ * being launched fails the members of that group. */
 int kw_wait(kw_engine* engine, int64_t ticket, kw_outputs* out, char* err, size_t errlen);
 int kw_cancel(kw_engine* engine, int64_t ticket, char* err, size_t errlen);
 int kw_engine_get_stats(kw_engine* engine, kw_engine_stats* stats, char* err, size_t errlen);

@@ -18,6 +18,10 @@ struct DeviceError : std::runtime_error {
 const char* backend_name();
 void* dev_alloc(size_t bytes);  // throws DeviceError(KW_ENOMEM) on failure
 void dev_free(void* p);
+// pinned host memory the device reads and writes directly (the engine's
+// per-simulation failure flags); freed with dev_host_free
+void* dev_host_alloc(size_t bytes);
+void dev_host_free(void* p);
 void* dev_stream_create();
 void dev_stream_destroy(void* s);
 void dev_h2d(void* dst, const void* src, size_t bytes, void* stream);

@@ -111,6 +111,8 @@ void* dev_scratch(size_t* bytes) {
   return block;
 }
 
+void* dev_host_alloc(size_t bytes) { return dev_alloc(bytes); }
+void dev_host_free(void* p) { dev_free(p); }
 void* dev_stream_create() { return nullptr; }
 void dev_stream_destroy(void*) {}
 void dev_h2d(void* dst, const void* src, size_t bytes, void*) {

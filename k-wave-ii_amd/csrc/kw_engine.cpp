@@ -476,6 +476,7 @@ struct Ticket {
   double submit_s = 0, start_s = -1, done_s = -1;
   int64_t est_bytes = 0, bytes = 0, steps_total = 0, steps_done = 0, cells = 0;
   bool cancel = false;
+  int flag = -1;             // slot of its failure flag while on the device
   std::unique_ptr<Sim> sim;  // owned by the stepping thread while running
   OutputStage out;
 };
@@ -506,8 +507,24 @@ struct Engine {
   int64_t launches = 0;
   std::vector<std::thread> admit_thr;
   std::thread step_thr;
+  // per-simulation failure flags in pinned host memory: the kernels that
+  // form the pressure write the first step with a non-finite value into the
+  // simulation's slot, and the stepping thread reads it after each round
+  // without a device copy
+  static constexpr int kFlagSlots = 1 << 16;
+  int32_t* flags = nullptr;
+  std::vector<int> free_flags;  // under mu
 
   double now() const { return std::chrono::duration<double>(Clock::now() - t0).count(); }
+
+  int32_t flag_of(const Ticket* t) const {
+    return t->flag < 0 ? 0 : __atomic_load_n(&flags[t->flag], __ATOMIC_ACQUIRE);
+  }
+  void release_flag(Ticket* t) {
+    // under mu
+    if (t->flag >= 0) free_flags.push_back(t->flag);
+    t->flag = -1;
+  }
 
   void finish(Ticket* t, int32_t status, int32_t err, const std::string& m) {
     // under mu
@@ -539,6 +556,12 @@ struct Engine {
         finish(t, KW_TICKET_CANCELLED, KW_ESTATE, "cancelled");
         continue;
       }
+      if (free_flags.empty()) {
+        finish(t, KW_TICKET_FAILED, KW_ENOMEM, "too many simulations on the device at once");
+        continue;
+      }
+      t->flag = free_flags.back();
+      free_flags.pop_back();
       held += t->est_bytes;
       t->status = KW_TICKET_RUNNING;
       t->start_s = now();
@@ -549,6 +572,10 @@ struct Engine {
       std::string msg;
       try {
         sim = create_sim(&t->prob->pb, admit_stream);
+        // the failure flag lives in the engine's pinned slot
+        __atomic_store_n(&flags[t->flag], 0, __ATOMIC_RELEASE);
+        sim->P.bad = flags + t->flag;
+        dev_h2d(sim->dP, &sim->P, sizeof(ProbDev), admit_stream);
         if (sim->Nt_eff > 0) step_once(*sim);
         dev_sync(admit_stream);
       } catch (const DeviceError& e) {
@@ -569,6 +596,7 @@ struct Engine {
         sim.reset();
         held -= t->est_bytes;
         --n_running;
+        release_flag(t);
         finish(t, KW_TICKET_FAILED, code, msg);
         continue;
       }
@@ -607,12 +635,29 @@ struct Engine {
     lk.lock();
     held -= b;
     --n_running;
+    release_flag(t);
     if (cancelled)
       finish(t, KW_TICKET_CANCELLED, KW_ESTATE, "cancelled");
     else if (code != KW_OK)
       finish(t, KW_TICKET_FAILED, code, msg);
     else
       finish(t, KW_TICKET_DONE, KW_OK, "");
+    cv_admit.notify_one();
+    lk.unlock();
+  }
+
+  // a simulation whose pressure became non-finite at step `bad` fails alone
+  // (called without mu held; its group's round has ended)
+  void drop_diverged(Ticket* t, int32_t bad, std::unique_lock<std::mutex>& lk) {
+    const int64_t b = t->bytes;
+    t->sim.reset();
+    lk.lock();
+    held -= b;
+    --n_running;
+    release_flag(t);
+    finish(t, KW_TICKET_FAILED, KW_ENUMERIC,
+           "the pressure became non-finite at time step " + std::to_string(bad) +
+               " (the simulation is unstable: check the time step against the stability limit)");
     cv_admit.notify_one();
     lk.unlock();
   }
@@ -652,6 +697,10 @@ struct Engine {
       ready.clear();
       lk.unlock();
       for (Ticket* t : joining) {
+        if (const int32_t bad = flag_of(t)) {
+          drop_diverged(t, bad, lk);
+          continue;
+        }
         if (t->sim->steps_done >= t->sim->Nt_eff) {
           retire(t, lk, false);  // finished in its first step
           continue;
@@ -687,7 +736,10 @@ struct Engine {
           lk.lock();
           const bool cancelled = t->cancel;
           lk.unlock();
-          if (cancelled || t->sim->steps_done >= t->sim->Nt_eff)
+          const int32_t bad = cancelled ? 0 : flag_of(t);
+          if (bad)
+            drop_diverged(t, bad, lk);
+          else if (cancelled || t->sim->steps_done >= t->sim->Nt_eff)
             retire(t, lk, cancelled);
           else
             keep.push_back(t);
@@ -784,6 +836,7 @@ struct Engine {
             lk.lock();
             held -= b;
             --n_running;
+            release_flag(t);
             finish(t, KW_TICKET_FAILED, code, err);
             lk.unlock();
           }
@@ -833,6 +886,7 @@ struct Engine {
       t->sim.reset();
       held -= t->bytes;
       --n_running;
+      release_flag(t);
       finish(t, KW_TICKET_CANCELLED, KW_ESTATE, "engine destroyed");
     }
   }
@@ -857,6 +911,10 @@ int kw_engine_create(const kw_engine_config* cfg, kw_engine** engine, char* err,
     E.budget = (cfg && cfg->device_bytes > 0) ? cfg->device_bytes : (int64_t)((double)fr * 0.9);
     E.steps_per_round = (cfg && cfg->steps_per_round > 0) ? cfg->steps_per_round : 32;
     E.use_graphs = std::getenv("KW_NO_GRAPHS") == nullptr;
+    E.flags = (int32_t*)kw::dev_host_alloc(sizeof(int32_t) * kw::Engine::kFlagSlots);
+    std::memset(E.flags, 0, sizeof(int32_t) * kw::Engine::kFlagSlots);
+    E.free_flags.reserve(kw::Engine::kFlagSlots);
+    for (int i = kw::Engine::kFlagSlots - 1; i >= 0; --i) E.free_flags.push_back(i);
     // as many compute streams as the device has hardware queues for this
     // process by default (4), so that groups overlap without oversubscribing
     for (int i = 0; i < 4; ++i) E.streams.push_back(kw::dev_stream_create());
@@ -995,6 +1053,7 @@ int kw_engine_destroy(kw_engine* engine) {
   if (E.step_thr.joinable()) E.step_thr.join();
   for (void* st : E.streams) kw::dev_stream_destroy(st);
   kw::dev_stream_destroy(E.admit_stream);
+  kw::dev_host_free(E.flags);
   delete engine;
   return KW_OK;
 }

@@ -45,6 +45,14 @@ void* dev_alloc(size_t bytes) {
 void dev_free(void* p) {
   if (p) (void)hipFree(p);
 }
+void* dev_host_alloc(size_t bytes) {
+  void* p = nullptr;
+  check(hipHostMalloc(&p, bytes ? bytes : 1, hipHostMallocDefault), "hipHostMalloc");
+  return p;
+}
+void dev_host_free(void* p) {
+  if (p) (void)hipHostFree(p);
+}
 void* dev_stream_create() {
   hipStream_t s;
   check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");

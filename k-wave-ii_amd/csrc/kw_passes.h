@@ -668,6 +668,17 @@ KW_BODY float tr_rho(const ProbDev& P, float p, float c2) {
 template <class XS, class Ctx>
 KW_BODY void x_finish_p(Ctx& c, const ProbDev& P, const XTile& T, cf* pv, int t, bool store) {
   const int N = P.Nx;
+  {
+    // non-finite pressure (NaN fails every comparison): the first step it
+    // appears at is flagged for the engine's per-request failure check
+    bool fin = true;
+    auto f = [&](int slot, int) {
+      fin = fin && (__builtin_fabsf(pv[slot].x) <= 3.402823466e38f) &&
+            (__builtin_fabsf(pv[slot].y) <= 3.402823466e38f);
+    };
+    XS::last_slots(P.fx, T.g, T.G, T.active, f);
+    if (!fin && T.active && KW_LD(P.bad) == 0) KW_ST(P.bad, (int32_t)t);
+  }
   if (P.time_rev) {
     // cells on the sensor take the boundary data of step t+1 of the reversed
     // series (original column Nt - t) and rho_d = p / (dim c0^2)

@@ -175,6 +175,10 @@ struct ProbDev {
   int32_t p_store_at;
   uint32_t rec_points;
   uint32_t rec_grid;
+  // first step at which the pressure held a non-finite value (0: none),
+  // written by the kernels that form p; device memory, or pinned host memory
+  // the engine reads after each round without a copy
+  int32_t* bad;
 };
 
 // per-launch mapping of blocks to problems: problem i owns blocks

@@ -467,6 +467,7 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream, Arena* ar
   const std::vector<int32_t> zero{0};
   dev_h2d(dt_index, zero.data(), sizeof(int32_t), S->stream);
   P.t_index = dt_index;
+  P.bad = S->alloc_zero<int32_t>(1);
   S->d_tile0 = S->alloc<int32_t>(1);
   dev_h2d(S->d_tile0, zero.data(), sizeof(int32_t), S->stream);
 

@@ -25,6 +25,7 @@ KW_EUNSUP = -2
 KW_EDEVICE = -3
 KW_ENOMEM = -4
 KW_ESTATE = -5
+KW_ENUMERIC = -6
 
 REC = {
     "p": 1 << 0, "p_max": 1 << 1, "p_min": 1 << 2, "p_rms": 1 << 3,

@@ -149,3 +149,61 @@ def test_engine_mix_equals_alone_hip(hip_lib):
     assert st["completed"] == len(MIX) and st["failed"] == 0
     for cs, g in zip(css, got):
         _same(g, _alone(cs, "hip"))
+
+
+def _unstable_case(N=32, Nt=200):
+    """A 2-D problem that becomes non-finite within Nt steps: half the grid at
+    4500 m/s with the reference sound speed set to 1500 m/s and dt at the
+    1500 m/s CFL limit of 1 (the k-space correction then under-corrects the
+    fast half; the oracle in single precision goes to NaN within the same
+    200 steps)."""
+    dx = 1e-4
+    c = np.full((N, N), 1500.0)
+    c[N // 2:, :] = 4500.0
+    p0 = np.zeros((N, N))
+    p0[8:12, 8:12] = 1.0
+    return {"dim": 2, "shape": (N, N), "dx": dx, "Nt": Nt, "dt": dx / 1500.0,
+            "medium": {"sound_speed": c, "density": 1000.0, "sound_speed_ref": 1500.0},
+            "source": {"p0": p0},
+            "sensor": {"mask": np.ones((N, N)), "record": ["p_max", "p_final"]},
+            "opts": {"PMLSize": 4, "Smooth": False}}
+
+
+def _isolation(backend):
+    """One request diverges in a group with two stable neighbours of the
+    same size and equation of state: it alone fails, with KW_ENUMERIC, and
+    the neighbours' results equal their runs alone."""
+    import warnings
+
+    import kwhip
+    from kwhip import _abi
+    good = [cases.case("p0_lossless", 2, (32, 32)), cases.case("p0_hetero_absorbing", 2, (32, 32))]
+    for cs in good:
+        cs["opts"] = dict(cs["opts"])
+    bad = _unstable_case()
+    bad["Nt"] = max(bad["Nt"], max(cs["Nt"] for cs in good))
+    with warnings.catch_warnings():
+        warnings.simplefilter("ignore")  # the unstable request's stability warning
+        with _abi._backend_override(backend), kwhip.Engine(steps_per_round=8) as eng:
+            tg0 = _submit(eng, good[0])
+            tb = _submit(eng, bad)
+            tg1 = _submit(eng, good[1])
+            with pytest.raises(kwhip.KWError) as ei:
+                eng.wait(tb)
+            r0, r1 = eng.wait(tg0), eng.wait(tg1)
+            st = eng.stats()
+    assert ei.value.code == _abi.KW_ENUMERIC, ei.value
+    assert "non-finite" in str(ei.value)
+    assert st["failed"] == 1 and st["completed"] == 2
+    _same(r0, _alone(good[0], backend))
+    _same(r1, _alone(good[1], backend))
+
+
+def test_engine_isolates_diverging_request_emu(few_pencils):
+    _isolation("emu")
+
+
+@pytest.mark.gpu
+def test_engine_isolates_diverging_request_hip(hip_lib):
+    print("[engine] one diverging request between two stable ones", flush=True)
+    _isolation("hip")
