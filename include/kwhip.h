@@ -341,6 +341,32 @@ int kw_engine_destroy(kw_engine* engine);
 #define KW_NOALLOC_MAX_BYTES (128ll << 20)
 int kw_run_noalloc(const kw_problem* prob, kw_outputs* out, char* err, size_t errlen);
 
+/* The native engines' file contract.
+ *
+ * kw_save_problem writes the HDF5 input file of a problem: the variables,
+ * names and storage of +kwave/+legacy/private/kspaceFirstOrder_saveToDisk.m
+ * (uint64 sizes, flags and 1-based index lists; single-precision reals;
+ * MATLAB array sizes; writeMatrix.m's domain_type / data_type attributes;
+ * writeAttributes.m's file attributes, version 1.2, file_type 'input').
+ * Replaces the 'SaveToDisk' branch of kspaceFirstOrder3D.m:668-686.  2-D
+ * and 3-D forward problems with binary or cuboid-corner sensor masks only,
+ * as the contract (KW_EUNSUP otherwise).
+ *
+ * kw_run_file runs such an input file (written by the reference's MATLAB
+ * code or by kw_save_problem) and writes the output file the reference's
+ * MATLAB side reads back (kspaceFirstOrder3DC.m:571-777): /p, /p_max, /p_min,
+ * /p_rms, /p_max_all, /p_min_all, /p_final, /ux ... /uz_non_staggered in
+ * single precision with MATLAB array sizes, per-cuboid groups (/p/1, ...) for
+ * cuboid-corner sensors, and /Nx, /Ny, /Nz, /Nt, /pml_*_size.  record_flags
+ * are KW_REC_* (the command-line flags --p_raw, --p_max, ...), and
+ * record_start_index is the -s option.  Replaces the native binary call of
+ * kspaceFirstOrder3DC.m:546 (the executable kspaceFirstOrder-HIP wraps it).
+ * Errors in the file (missing or malformed variables, unsupported HDF5
+ * features or simulation types) return KW_EINVAL or KW_EUNSUP. */
+int kw_save_problem(const kw_problem* prob, const char* path, char* err, size_t errlen);
+int kw_run_file(const char* input_file, const char* output_file, uint32_t record_flags,
+                int64_t record_start_index, char* err, size_t errlen);
+
 /* Smoothing filter: out = real(ifftn(fftn(in) .* ifftshift(win))) on a 1D,
  * 2D or 3D grid, computed on the device.  `win` is the centred window of
  * getWin (not shifted).  Replaces the transform of +kwave/+legacy/smooth.m;

@@ -203,6 +203,8 @@ def load(backend: str = "hip") -> C.CDLL:
     lib.kw_sim_destroy.argtypes = [C.c_void_p]
     lib.kw_sim_step_timed.argtypes = [C.c_void_p, C.c_int64, P(C.c_double), P(KernelStats)] + err
     lib.kw_smooth.argtypes = [C.c_int32, P(C.c_int64), c_double_p, c_double_p, c_double_p] + err
+    lib.kw_save_problem.argtypes = [P(Problem), C.c_char_p] + err
+    lib.kw_run_file.argtypes = [C.c_char_p, C.c_char_p, C.c_uint32, C.c_int64] + err
     lib.kw_engine_create.argtypes = [P(EngineConfig), P(C.c_void_p)] + err
     lib.kw_submit.argtypes = [C.c_void_p, P(Problem), P(C.c_int64)] + err
     lib.kw_poll.argtypes = [C.c_void_p, C.c_int64, P(TicketInfo)] + err

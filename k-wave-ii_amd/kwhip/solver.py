@@ -42,8 +42,11 @@ _VISUAL_OPTIONS = {"PlotSim", "PlotLayout", "PlotPML", "PlotScale", "PlotFreq", 
                    "CreateLog", "MeshPlot"}
 _OPTIONS = {"PMLSize", "PMLAlpha", "PMLInside", "PMLRange", "MultiAxialPMLRatio", "UseFD",
             "Smooth", "DataCast", "DataRecast",
-            "CartInterp", "UsekSpace", "UseSG", "ScaleSourceTerms"} | _VISUAL_OPTIONS
-_UNSUPPORTED = {"SaveToDisk", "StreamToDisk", "SaveToDiskExit", "HDFCompressionLevel"}
+            "CartInterp", "UsekSpace", "UseSG", "ScaleSourceTerms", "SaveToDisk",
+            "SaveToDiskExit", "HDFCompressionLevel"} | _VISUAL_OPTIONS
+_UNSUPPORTED = {"StreamToDisk"}
+# kspaceFirstOrder_setDefaults.m:196
+SAVE_TO_DISK_FILENAME_DEF = "kwave_input_data.h5"
 
 RECORD_ALLOWED = ("p", "p_max", "p_min", "p_rms", "p_max_all", "p_min_all", "p_final",
                   "u", "u_max", "u_min", "u_rms", "u_max_all", "u_min_all", "u_final",
@@ -604,6 +607,7 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
     info.dt = dt
     info.inside = None  # user's grid inside the expanded one ('PMLInside', false)
     info.time_rev = False
+    info.cartesian = False
     info.use_sensor = sensor is not None and _has(sensor, "mask")
     record = ["p"]
     tr_data = None
@@ -646,6 +650,7 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
             sens_idx, info.cuboids = cuboid_points(kgrid, mask)
             n_sens = len(sens_idx)
         elif mask.ndim == 2 and mask.shape[0] == dim:
+            info.cartesian = True
             if "u_split_field" in record:
                 # kspaceFirstOrder_inputChecking.m:1363-1366
                 raise ValueError("The option sensor.record = {'u_split_field'} is only "
@@ -1109,10 +1114,66 @@ def run_prepared(pp: PreparedProblem):
     return shape_outputs(pp, arrays)
 
 
+def _save_to_disk_options(dim, opts):
+    """'SaveToDisk', 'SaveToDiskExit' and 'HDFCompressionLevel'
+    (kspaceFirstOrder_checkOptionalInputs.m:125-135,376-414); returns the
+    file name or None."""
+    for k in ("SaveToDisk", "SaveToDiskExit"):
+        if k in opts and dim == 1:
+            raise ValueError("Optional input 'SaveToDisk' is not compatible with 1D simulations.")
+    if "HDFCompressionLevel" in opts:
+        lv = opts["HDFCompressionLevel"]
+        if not np.isscalar(lv) or round(lv) != lv or lv > 9 or lv < 0:
+            raise ValueError("Optional input 'HDFCompressionLevel' must be an integer between 0 and 9.")
+    if "SaveToDiskExit" in opts and not isinstance(opts["SaveToDiskExit"], (bool, np.bool_)):
+        raise ValueError("Optional input 'SaveToDiskExit' must be Boolean.")
+    v = opts.get("SaveToDisk", False)
+    if not isinstance(v, (bool, np.bool_, str)):
+        raise ValueError("Optional input 'SaveToDisk' must be Boolean or a String.")
+    if v is False or (isinstance(v, (bool, np.bool_)) and not v):
+        return None
+    return SAVE_TO_DISK_FILENAME_DEF if isinstance(v, (bool, np.bool_)) else v
+
+
+def save_to_disk(pp: PreparedProblem, filename: str):
+    """Write the prepared problem as the native engines' HDF5 input file
+    (kspaceFirstOrder_saveToDisk.m: the variable set, names and sizes, in
+    single precision with uint64 sizes, flags and indices), through
+    kw_save_problem."""
+    import os
+    info = pp.info
+    if info.dim == 3 and (not info.use_sensor or info.cartesian or info.time_rev):
+        # kspaceFirstOrder_inputChecking.m:1393-1395
+        raise ValueError("The optional input 'SaveToDisk' is currently only compatible with "
+                         "forward simulations using a non-zero binary sensor mask.")
+    if info.cartesian or not info.use_sensor:
+        # kspaceFirstOrder_saveToDisk.m:43-46
+        raise ValueError("Optional input 'SaveToDisk' only supported for sensor masks defined as "
+                         "a binary matrix or the opposing corners of a rectangle (2D) or cuboid (3D).")
+    ext = os.path.splitext(filename)[1]
+    if ext == "":
+        filename = filename + ".h5"
+    elif ext == ".mat":
+        raise NotImplementedError("'SaveToDisk' to a MATLAB .mat file is not supported by kwhip")
+    elif ext != ".h5":
+        raise ValueError("unknown file extension for 'SaveToDisk' filename")
+    if os.path.exists(filename):
+        os.remove(filename)
+    lib = _abi.load(info.backend)
+    _abi.call(lib.kw_save_problem, C.byref(pp.prob), filename.encode())
+    return filename
+
+
 def _solve(dim, kgrid, medium, source, sensor, args, kwargs):
     opts = _parse_options(args, kwargs)
     _check_use_fd(dim, opts)
+    save = _save_to_disk_options(dim, opts)
     pp = prepare(kgrid, medium, source, sensor, dim, opts)
+    if save is not None:
+        save_to_disk(pp, save)
+        if opts.get("SaveToDiskExit", True):
+            # kspaceFirstOrder3D.m:681-684
+            return []
     return run_prepared(pp)
 
 
