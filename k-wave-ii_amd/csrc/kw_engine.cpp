@@ -356,9 +356,9 @@ struct Group {
         const Sim& S = *sims[i];
         size_t l = geom_of(S, k).lds;
         if (k == GX)
-          l = sizeof(cf) * ((size_t)kNPX * (size_t)x_lds_stride(S.G.Nx) *
+          l = sizeof(cf) * ((size_t)kNPX * (size_t)x_lds_stride(S.P.fx.L) *
                                 (size_t)((T + S.P.Gx - 1) / S.P.Gx) +
-                            (size_t)S.G.Nx);
+                            (size_t)S.P.fx.L);
         lds = std::max(lds, l);
       }
       if (lds > dev_max_lds()) fail(KW_EUNSUP, "group launch needs too much LDS");

@@ -36,11 +36,21 @@
 
 namespace kw {
 
+// A transform of length N.  For N = 2^a 3^b 5^c 7^d the stages run at
+// length L = N.  Any other N runs as Bluestein's chirp-z transform: with the
+// chirp w_n = exp(-i pi n^2 / N), X_k = w_k sum_n (x_n w_n) conj(w_(k-n)),
+// a circular convolution of length L >= 2N - 1 (L = 2^a 3^b 5^c) computed
+// with two length-L transforms in LDS; `chirp` (N entries) and `bspec`
+// (L entries: the length-L transform of the chirp filter conj(w_m),
+// m = -(N-1) .. N-1, wrapped) are set, and the inverse uses their conjugates.
 struct FftPlan {
-  int32_t N;
+  int32_t N;   // length of the transform
+  int32_t L;   // length the stages run at (N, or the chirp-z length)
   int32_t nst;
   int32_t radix[KW_MAX_STAGES];
-  const cf* tw;  // N entries
+  const cf* tw;     // L twiddles exp(-2 pi i j / L)
+  const cf* chirp;  // Bluestein only (else NULL): w_n, n < N
+  const cf* bspec;  // Bluestein only: transform of the chirp filter, L entries
 };
 
 // factorisation rule shared by the host planner and the compile-time plans:
@@ -111,11 +121,11 @@ KW_BODY cf twiddle(const TwL& tw, int idx) {
   return w;
 }
 
-// the N twiddles of a plan copied into the block's LDS at `dst` by all
+// the L twiddles of a plan copied into the block's LDS at `dst` by all
 // threads of the block, followed by a block barrier
 template <class Ctx>
 KW_BODY TwL stage_twiddles(Ctx& c, const FftPlan& pl, cf* dst) {
-  for (int i = c.tid(); i < pl.N; i += c.nthreads()) lds_st(dst + i, KW_LD32(pl.tw, (uint32_t)i));
+  for (int i = c.tid(); i < pl.L; i += c.nthreads()) lds_st(dst + i, KW_LD32(pl.tw, (uint32_t)i));
   c.sync();
   return TwL{dst};
 }
@@ -352,18 +362,73 @@ KW_BODY void fft_stages_rt(Ctx& c, const FftPlan& pl, const TW& tw, int g, int G
   auto lsrc = [&](int j, int n) -> cf { return lds.get(j, n); };
   auto ldst = [&](int j, int n, cf v, int) { lds.set(j, n, v); };
   int Ns = 1;
+  const int L = pl.L;
   for (int s = 0; s < pl.nst; ++s) {
     const int R = pl.radix[s];
     switch (R) {
-      case 2: stage<2, NP, INV, MaxB<2>::v>(c, pl.N, Ns, G, g, active, tw, lsrc, ldst); break;
-      case 3: stage<3, NP, INV, MaxB<3>::v>(c, pl.N, Ns, G, g, active, tw, lsrc, ldst); break;
-      case 4: stage<4, NP, INV, MaxB<4>::v>(c, pl.N, Ns, G, g, active, tw, lsrc, ldst); break;
-      case 5: stage<5, NP, INV, MaxB<5>::v>(c, pl.N, Ns, G, g, active, tw, lsrc, ldst); break;
-      case 7: stage<7, NP, INV, MaxB<7>::v>(c, pl.N, Ns, G, g, active, tw, lsrc, ldst); break;
-      default: stage<8, NP, INV, MaxB<8>::v>(c, pl.N, Ns, G, g, active, tw, lsrc, ldst); break;
+      case 2: stage<2, NP, INV, MaxB<2>::v>(c, L, Ns, G, g, active, tw, lsrc, ldst); break;
+      case 3: stage<3, NP, INV, MaxB<3>::v>(c, L, Ns, G, g, active, tw, lsrc, ldst); break;
+      case 4: stage<4, NP, INV, MaxB<4>::v>(c, L, Ns, G, g, active, tw, lsrc, ldst); break;
+      case 5: stage<5, NP, INV, MaxB<5>::v>(c, L, Ns, G, g, active, tw, lsrc, ldst); break;
+      case 7: stage<7, NP, INV, MaxB<7>::v>(c, L, Ns, G, g, active, tw, lsrc, ldst); break;
+      default: stage<8, NP, INV, MaxB<8>::v>(c, L, Ns, G, g, active, tw, lsrc, ldst); break;
     }
     Ns *= R;
   }
+}
+
+// Bluestein's transform of length N (see FftPlan) over the length-L stages:
+// the chirped input zero-padded to L, a forward transform, the product with
+// the filter's transform, an inverse transform, and the chirped outputs
+// scaled by 1/L; the inverse transform conjugates the chirp and the filter.
+// Each thread reads and writes only its own elements n = g + i G between
+// barriers, as the plain runtime engine does.
+template <int NP, bool INV, class Ctx, class TW, class Src, class Dst>
+KW_BODY void fft_bluestein(Ctx& c, const FftPlan& pl, const TW& tw, int g, int G, bool active,
+                           Src& src, Dst& dst, const LdsPencils& lds) {
+  const int N = pl.N, L = pl.L;
+  auto chirp = [&](int n) -> cf {
+    cf w = KW_LD32(pl.chirp, (uint32_t)n);
+    if (INV) w.y = -w.y;
+    return w;
+  };
+#pragma unroll
+  for (int i = 0; i < kRtSlots; ++i) {
+    const int n = g + i * G;
+    if (n < L) {
+      const cf w = n < N ? chirp(n) : cf{0.f, 0.f};
+#pragma unroll
+      for (int j = 0; j < NP; ++j) {
+        const cf v = n < N ? cmul(src(j, n), w) : cf{0.f, 0.f};
+        if (active) lds.set(j, n, v);
+      }
+    }
+  }
+  c.sync();
+  fft_stages_rt<NP, false>(c, pl, tw, g, G, active, lds);
+#pragma unroll
+  for (int i = 0; i < kRtSlots; ++i) {
+    const int n = g + i * G;
+    if (active && n < L) {
+      cf b = KW_LD32(pl.bspec, (uint32_t)n);
+      if (INV) b.y = -b.y;
+#pragma unroll
+      for (int j = 0; j < NP; ++j) lds.set(j, n, cmul(lds.get(j, n), b));
+    }
+  }
+  c.sync();
+  fft_stages_rt<NP, true>(c, pl, tw, g, G, active, lds);
+  const float inv_l = 1.0f / (float)L;
+#pragma unroll
+  for (int i = 0; i < kRtSlots; ++i) {
+    const int n = g + i * G;
+    if (active && n < N) {
+      const cf w = cscale(chirp(n), inv_l);
+#pragma unroll
+      for (int j = 0; j < NP; ++j) dst(j, n, cmul(lds.get(j, n), w), i);
+    }
+  }
+  c.sync();
 }
 
 template <int NP, bool INV, class Ctx, class TW, class Src, class Dst>
@@ -371,6 +436,10 @@ KW_BODY void fft_rt(Ctx& c, const FftPlan& pl, const TW& tw, int g, int G, bool 
                     Dst& dst, const LdsPencils& lds) {
   const int N = pl.N;
   g = kw_fresh(g);
+  if (pl.chirp) {
+    fft_bluestein<NP, INV>(c, pl, tw, g, G, active, src, dst, lds);
+    return;
+  }
 #pragma unroll
   for (int i = 0; i < kRtSlots; ++i) {
     const int n = g + i * G;

@@ -97,43 +97,43 @@ __global__ __launch_bounds__(1024) void kern_aux(KW_KPARAMS, PassArgs A) {
 }
 #define KW_HAS_AUX 1
 #elif KW_KIND == 3
-__global__ __launch_bounds__(256) void kern(KW_KPARAMS, PassArgs A) {
+__global__ __launch_bounds__(512) void kern(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
   KW_KBATCH;
   body_x_r2c<SelX>(c, B, A);
 }
 #elif KW_KIND == 4
-__global__ __launch_bounds__(256) void kern(KW_KPARAMS, PassArgs A) {
+__global__ __launch_bounds__(512) void kern(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
   KW_KBATCH;
   body_x_c2r<SelX>(c, B, A);
 }
 #elif KW_KIND == 5
-__global__ __launch_bounds__(256) void kern2(KW_KPARAMS, PassArgs A) {
+__global__ __launch_bounds__(512) void kern2(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
   KW_KBATCH;
   body_x_grad<SelX, 2>(c, B, A);
 }
-__global__ __launch_bounds__(256) void kern(KW_KPARAMS, PassArgs A) {
+__global__ __launch_bounds__(512) void kern(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
   KW_KBATCH;
   body_x_grad<SelX, 3>(c, B, A);
 }
 #define KW_HAS_DIM2 1
 #elif KW_KIND == 6
-__global__ __launch_bounds__(256) void kern2(KW_KPARAMS, PassArgs A) {
+__global__ __launch_bounds__(512) void kern2(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
   KW_KBATCH;
   body_x_div<SelX, 2>(c, B, A);
 }
-__global__ __launch_bounds__(256) void kern(KW_KPARAMS, PassArgs A) {
+__global__ __launch_bounds__(512) void kern(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
   KW_KBATCH;
   body_x_div<SelX, 3>(c, B, A);
 }
 #define KW_HAS_DIM2 1
 #elif KW_KIND == 7
-__global__ __launch_bounds__(256) void kern(KW_KPARAMS, PassArgs A) {
+__global__ __launch_bounds__(512) void kern(KW_KPARAMS, PassArgs A) {
   GpuCtx c;
   KW_KBATCH;
   body_x_eos<SelX>(c, B, A);

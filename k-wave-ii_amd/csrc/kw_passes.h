@@ -96,10 +96,11 @@ KW_BODY XTile x_tile(Ctx& c, const ProbDev& P, int tile) {
   T.z0 = r0 / (uint32_t)P.Ny;
   T.y1 = r1 % (uint32_t)P.Ny;
   T.z1 = r1 / (uint32_t)P.Ny;
-  // LDS: the Nx twiddles (staged by the whole block), then the pencils
+  // LDS: the L twiddles of the x transform (staged by the whole block), then
+  // the pencils (L elements each, padded)
   T.tw = stage_twiddles(c, P.fx, c.lds());
-  const int stride = x_lds_stride(P.Nx);
-  T.lds = LdsPencils{c.lds() + P.Nx + (int64_t)p * kNPX * stride, stride, 1, true};
+  const int stride = x_lds_stride(P.fx.L);
+  T.lds = LdsPencils{c.lds() + P.fx.L + (int64_t)p * kNPX * stride, stride, 1, true};
   return T;
 }
 
@@ -252,9 +253,11 @@ KW_BODY void body_ypass(Ctx& c, const Batch& B, const PassArgs& A) {
   const ProbDev& P = batch_prob(c, B, tile);
   advance_step(P, A, tile, c.tid());
   const int N = P.Ny, G = P.Gy, Pn = P.Py;
-  // LDS: the N twiddles, then the pencils
+  // LDS: the L twiddles, then the pencils (L elements each: the length the
+  // stages run at)
   const TwL tw = stage_twiddles(c, P.fy, c.lds());
-  cf* const lp = c.lds() + N;
+  const int L = P.fy.L;
+  cf* const lp = c.lds() + L;
   const int64_t nH = (int64_t)P.Wm * P.Nz;
   const int64_t tH = (nH + Pn - 1) / Pn;
   const int tid = c.tid();
@@ -283,7 +286,7 @@ KW_BODY void body_ypass(Ctx& c, const Batch& B, const PassArgs& A) {
     }
     base = (uint32_t)(P.nspec_main + q * P.Ny);
     stride = 1;
-    lds = LdsPencils{lp + (int64_t)p * (N + 1), 0, 1, false};
+    lds = LdsPencils{lp + (int64_t)p * (L + 1), 0, 1, false};
   }
   for (int b = 0; b < A.nbuf; ++b) {
     cf* S = P.S[A.buf[b]];
@@ -410,11 +413,11 @@ KW_BODY void body_spec(Ctx& c, const Batch& B, const PassArgs& A) {
     kx = 0;
     ky = d3 ? 0 : -1;
   }
-  // LDS: the N twiddles, then pencil buffers la and lb
+  // LDS: the L twiddles, then pencil buffers la and lb (L elements each)
   const TwL tw = stage_twiddles(c, pl, c.lds());
-  cf* const lp = c.lds() + N;
+  cf* const lp = c.lds() + pl.L;
   LdsPencils la{lp + p, 0, Pn, false};
-  LdsPencils lb{lp + (int64_t)Pn * N + p, 0, Pn, false};
+  LdsPencils lb{lp + (int64_t)Pn * pl.L + p, 0, Pn, false};
   // workspace of the inverse transforms that read la: la itself for a
   // fixed-length plan (every thread of a block then has a pencil or a
   // pencil of its own), lb for runtime plans, whose launches may be shared

@@ -63,6 +63,23 @@ inline bool factorise(int N, std::vector<int>& r) {
   return n == 1 && (N == 1 || !r.empty()) && (int)r.size() <= KW_MAX_STAGES;
 }
 
+// length of the chirp-z convolution for a transform of length N that does
+// not factor into 2, 3, 5 and 7: the smallest 2^a 3^b 5^c >= 2N - 1
+inline int chirp_length(int N) {
+  for (int L = std::max(1, 2 * N - 1);; ++L) {
+    int m = L;
+    for (int p : {2, 3, 5})
+      while (m % p == 0) m /= p;
+    if (m == 1) return L;
+  }
+}
+
+// length the FFT stages of an axis of length N run at
+inline int stage_length(int N) {
+  std::vector<int> r;
+  return factorise(N, r) ? N : chirp_length(N);
+}
+
 // wavenumber of ifftshifted index i (kWaveGrid.m:779-797 then ifftshift):
 // (2 pi/d) * (n/N) with n = i for i < ceil(N/2), else i - N
 inline double kvec(int i, int N, double d) {
@@ -384,15 +401,20 @@ struct Sim {
   void plan_launches() {
     const size_t maxl = dev_max_lds();
     const int capP = max_pencils_env();
-    // ceil(N/7) for lengths with a factor 7 (one radix-7 butterfly per
-    // thread; every other stage then has at most its usual count)
-    auto Gof = [](int N) { return N % 7 == 0 ? (N + 6) / 7 : (N + KW_E - 1) / KW_E; };
+    // threads per pencil of the runtime engine, from the length its stages
+    // run at: ceil(L/7) for lengths with a factor 7 (one radix-7 butterfly
+    // per thread; every other stage then has at most its usual count)
+    auto Gof = [](int N) {
+      const int L = stage_length(N);
+      return L % 7 == 0 ? (L + 6) / 7 : (L + KW_E - 1) / KW_E;
+    };
     // x pass: <= 256 threads, LDS Nx twiddles + P * kNPX padded pencils; the
     // fixed-length x kernels run x_threads(Nx) threads per pencil
     {
       const int Gx = fixed_len(PK_X_GRAD) ? x_threads(G.Nx) : Gof(G.Nx);
-      const size_t pencil = sizeof(cf) * (size_t)kNPX * x_lds_stride(G.Nx);
-      const size_t twb = sizeof(cf) * (size_t)G.Nx;
+      const int Lx = stage_length(G.Nx);
+      const size_t pencil = sizeof(cf) * (size_t)kNPX * x_lds_stride(Lx);
+      const size_t twb = sizeof(cf) * (size_t)Lx;
       int P = std::max(1, 256 / Gx);
       P = std::min<int64_t>(P, (int64_t)((maxl - twb) / pencil));
       P = std::max(1, std::min(P, capP));
@@ -402,29 +424,30 @@ struct Sim {
       lx.nb = (int)((npairs + P - 1) / P);
       lx.T = P * Gx;
       lx.lds = pencil * (size_t)P + twb;
-      if (Gx > 256 || lx.lds > maxl) fail(KW_EUNSUP, "grid size Nx too large for the x pass");
+      if (Gx > 512 || lx.lds > maxl) fail(KW_EUNSUP, "grid size Nx too large for the x pass");
     }
     // strided y pass (3D): <= 1024 threads, LDS Ny twiddles + P*(Ny+1)
     if (G.dim == 3) {
       const int Gy = Gof(G.Ny);
+      const int Ly = stage_length(G.Ny);
       int P = std::max(1, std::min(pencils_env("KW_PENCILS_Y", 16), 1024 / Gy));
-      P = std::min<int64_t>(
-          P, (int64_t)((maxl - sizeof(cf) * G.Ny) / (sizeof(cf) * (G.Ny + 1))));
+      P = std::min<int64_t>(P, (int64_t)((maxl - sizeof(cf) * Ly) / (sizeof(cf) * (Ly + 1))));
       P = std::max(1, std::min(P, capP));
       this->P.Gy = Gy;
       this->P.Py = P;
       const int64_t nH = (int64_t)G.Wm * G.Nz;
       ly.nb = (int)((nH + P - 1) / P + (G.nyq ? (G.Nz + P - 1) / P : 0));
       ly.T = P * Gy;
-      ly.lds = sizeof(cf) * ((size_t)P * (G.Ny + 1) + G.Ny);
+      ly.lds = sizeof(cf) * ((size_t)P * (Ly + 1) + Ly);
       if (Gy > 1024 || ly.lds > maxl) fail(KW_EUNSUP, "grid size Ny too large for the y pass");
     }
     // last-axis pass: <= 1024 threads, LDS N twiddles + 2*P*N
     {
       const int Nl = G.dim == 3 ? G.Nz : G.Ny;
       const int Gl = Gof(Nl);
+      const int Ll = stage_length(Nl);
       int P = std::max(1, std::min(pencils_env("KW_PENCILS_L", 16), 1024 / Gl));
-      P = std::min<int64_t>(P, (int64_t)((maxl - sizeof(cf) * Nl) / (2 * sizeof(cf) * Nl)));
+      P = std::min<int64_t>(P, (int64_t)((maxl - sizeof(cf) * Ll) / (2 * sizeof(cf) * Ll)));
       P = std::max(1, std::min(P, capP));
       this->P.Gl = Gl;
       this->P.Pl = P;
@@ -432,28 +455,58 @@ struct Sim {
       const int64_t nQ = G.dim == 3 ? G.Ny : 1;
       ll.nb = (int)((nH + P - 1) / P + (G.nyq ? (nQ + P - 1) / P : 0));
       ll.T = P * Gl;
-      ll.lds = sizeof(cf) * (2 * (size_t)P * Nl + Nl);
+      ll.lds = sizeof(cf) * (2 * (size_t)P * Ll + Ll);
       if (Gl > 1024 || ll.lds > maxl) fail(KW_EUNSUP, "grid size too large for the last-axis pass");
     }
     lcell.nb = (int)((G.ncell + EW_T - 1) / EW_T);
     lsens.nb = (int)((std::max<int64_t>(nsens, 1) + EW_T - 1) / EW_T);
   }
 
+  // FFT plan of an axis of length N: stages of the radices of N, or for N
+  // with a prime factor above 7 Bluestein's chirp-z transform (kw_fft.h),
+  // whose chirp and filter transform are computed here in double precision
+  // and rounded once
   FftPlan make_plan(int N) {
     FftPlan pl{};
     std::vector<int> r;
-    if (!factorise(N, r))
-      fail(KW_EUNSUP, "grid size " + std::to_string(N) +
-                          " is not a product of 2, 3, 5 and 7 (supported FFT radices)");
+    const bool direct = factorise(N, r);
+    const int L = direct ? N : chirp_length(N);
+    if (!direct && !factorise(L, r)) fail(KW_EUNSUP, "no FFT plan for length " + std::to_string(N));
     pl.N = N;
+    pl.L = L;
     pl.nst = (int)r.size();
     for (size_t i = 0; i < r.size(); ++i) pl.radix[i] = r[i];
-    std::vector<cf> tw(N);
-    for (int j = 0; j < N; ++j) {
-      const double a = 2.0 * kPi * (double)j / (double)N;
+    std::vector<cf> tw(L);
+    for (int j = 0; j < L; ++j) {
+      const double a = 2.0 * kPi * (double)j / (double)L;
       tw[j] = cf{(float)std::cos(a), (float)-std::sin(a)};
     }
     pl.tw = upload(tw);
+    if (!direct) {
+      // w_n = exp(-i pi n^2 / N), with n^2 reduced modulo 2N
+      std::vector<std::complex<double>> w(N);
+      for (int n = 0; n < N; ++n) {
+        const int64_t q = ((int64_t)n * n) % (2 * (int64_t)N);
+        const double a = kPi * (double)q / (double)N;
+        w[n] = std::complex<double>(std::cos(a), -std::sin(a));
+      }
+      // filter b_m = conj(w_|m|) for |m| < N, wrapped to length L (symmetric),
+      // and its transform B_k = b_0 + 2 sum_m b_m cos(2 pi m k / L)
+      std::vector<double> cosl(L);
+      for (int j = 0; j < L; ++j) cosl[j] = std::cos(2.0 * kPi * (double)j / (double)L);
+      std::vector<std::complex<double>> B(L);
+      parallel_for(L, [&](int64_t k) {
+        std::complex<double> acc = std::conj(w[0]);
+        for (int m = 1; m < N; ++m)
+          acc += 2.0 * cosl[(int64_t)m * k % L] * std::conj(w[m]);
+        B[k] = acc;
+      }, N);
+      std::vector<cf> ch(N), bs(L);
+      for (int n = 0; n < N; ++n) ch[n] = cf{(float)w[n].real(), (float)w[n].imag()};
+      for (int k = 0; k < L; ++k) bs[k] = cf{(float)B[k].real(), (float)B[k].imag()};
+      pl.chirp = upload(ch);
+      pl.bspec = upload(bs);
+    }
     return pl;
   }
 

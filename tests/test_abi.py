@@ -92,7 +92,7 @@ def test_errors_are_reported_not_raised():
     rc = lib.kw_sim_create(C.byref(prob), C.byref(h), buf, 256)
     assert rc == _abi.KW_EINVAL
     assert b"dim" in buf.value
-    prob.dim, prob.Nx, prob.Ny, prob.Nz, prob.Nt = 2, 22, 16, 1, 3   # 22 = 2 x 11: radix 11 unsupported
+    prob.dim, prob.Nx, prob.Ny, prob.Nz, prob.Nt = 2, 4099, 16, 1, 3   # prime: chirp-z length 8200 > 4096
     prob.dx = prob.dy = 1e-4
     prob.dt, prob.c_ref, prob.c0_scalar, prob.rho0_scalar = 1e-8, 1500.0, 1500.0, 1000.0
     rc = lib.kw_sim_create(C.byref(prob), C.byref(h), buf, 256)

@@ -76,6 +76,19 @@ def test_emu_parity_3d(name):
     # the velocity split's last-axis pass on fixed-length kernels
     (2, "u_split", (30, 64)),
     (3, "u_split", (12, 16, 64)),
+    # lengths with a prime factor above 7 (Bluestein's chirp-z transform on
+    # every axis kind): 22 = 2 x 11, 26 = 2 x 13, 34 = 2 x 17, 31 prime
+    (1, "p0_hetero_absorbing", (22,)),
+    (1, "time_reversal", (31,)),
+    (2, "p0_hetero_absorbing", (22, 26)),
+    (2, "psrc_add", (34, 20)),
+    (2, "time_reversal", (26, 31)),
+    (3, "p0_nonlinear_absorbing", (22, 26, 17)),
+    (3, "psrc_dir", (13, 20, 34)),
+    (3, "u_split", (22, 13, 16)),
+    # the longest x axes: 512 threads per pencil
+    (2, "p0_lossless", (4096, 10)),
+    (2, "p0_lossless", (2039, 10)),
 ])
 def test_emu_parity_fixed_lengths(dim, name, shape):
     _check(cases.case(name, dim, shape))

@@ -66,3 +66,20 @@ def test_parity_2d_sizes(name, shape, hip_lib):
                                         ("psrc_dir", (21, 16, 14))])
 def test_parity_3d_sizes(name, shape, hip_lib):
     _check(cases.case(name, 3, shape), hip_lib)
+
+
+@pytest.mark.parametrize("dim,name,shape", [
+    # lengths with a prime factor above 7: Bluestein's chirp-z transform
+    (1, "p0_hetero_absorbing", (22,)),
+    (2, "p0_hetero_absorbing", (22, 26)),
+    (2, "time_reversal", (34, 31)),
+    (3, "p0_nonlinear_absorbing", (22, 26, 34)),
+    (3, "psrc_dir", (13, 20, 34)),
+    (3, "u_split", (22, 13, 16)),
+    # long x axes: 4096 (512 threads per pencil) and the prime 2039
+    # (chirp-z length 4096)
+    (2, "p0_hetero_absorbing", (4096, 64)),
+    (2, "psrc_add", (2039, 24)),
+])
+def test_parity_any_length(dim, name, shape, hip_lib):
+    _check(cases.case(name, dim, shape), hip_lib)
