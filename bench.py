@@ -209,6 +209,43 @@ def cfg3_leg(N=384, Nt=2000, steps=None):
     return out
 
 
+def b_alg_cell_step(dim, absorbing, hetero, nonlinear, full_stats):
+    """Algorithmic bytes per cell-step of one simulation under the per-axis-
+    pass model: 8 B per cell for every FFT axis pass (gradient 1 forward +
+    dim inverse transforms, divergence dim + dim, absorbing equation of state
+    2 + 2), 8 B for each velocity and density component read and written,
+    8 B for the pressure the equation of state writes and the next gradient
+    reads, 4 B per FP32 medium grid (1/rho0_sg_d, rho0, c0; with absorption
+    tau, eta and sum(rho) written and read; B/A), and 16 B for each FP64
+    whole-grid statistic read and written.  2-D lossless homogeneous: 152 B;
+    3-D: 296 B."""
+    n_fft = (1 + dim) + 2 * dim + (4 if absorbing else 0)
+    b = n_fft * dim * 8.0 + 2 * dim * 8.0 + 8.0
+    if hetero:
+        b += 4.0 * (dim + 2)
+    if absorbing:
+        b += 8.0 + (8.0 if hetero else 0.0)
+    if nonlinear and hetero:
+        b += 4.0
+    return b + 16.0 * full_stats
+
+
+def _problem_bytes(kg, med, sens, steps):
+    """Algorithmic bytes of `steps` steps of one problem of the batched or
+    served workloads (b_alg_cell_step)."""
+    import numpy as np
+    hetero = np.ndim(med.get("sound_speed", 0.0)) > 0 or np.ndim(med.get("density", 0.0)) > 0
+    absorbing = "alpha_coeff" in med
+    nonlinear = "BonA" in med and np.ndim(med["BonA"]) > 0
+    full = sum(1 for r in sens.get("record", []) if r in ("p_max", "p_min", "p_rms")) \
+        if np.size(sens.get("mask", 0)) == kg.total_grid_points else 0
+    return kg.total_grid_points * steps * b_alg_cell_step(kg.dim, absorbing, hetero, nonlinear, full)
+
+
+def _roofline(nbytes, seconds):
+    return round(nbytes / seconds / (HBM_PEAK_GBS * 1e9), 4)
+
+
 def _tr_sensor(sens, data):
     return {"mask": sens["mask"], "time_reversal_boundary_data": data}
 
@@ -264,10 +301,15 @@ def batched_bench(n_problems, seed0=0):
     diff = max(float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-30))
                for a, b in zip(rec_eng, rec_plain))
     n2 = sum(1 for kg, *_ in probs if kg.dim == 2)
+    nbytes = sum(_problem_bytes(kg, med, sens, 2 * kg.Nt - 1) for kg, med, src, sens, _ in probs)
     return {"value": round(cell_steps / t_eng / 1e6, 1), "unit": "Mcell-steps/s",
             "plain_value": round(cell_steps / t_plain / 1e6, 1),
             "speedup_vs_plain": round(t_plain / t_eng, 2),
             "seconds": round(t_eng, 3), "plain_seconds": round(t_plain, 3),
+            "roofline_frac": _roofline(nbytes, t_eng),
+            "roofline_frac_busy": _roofline(nbytes, st["busy_s"]),
+            "plain_roofline_frac": _roofline(nbytes, t_plain),
+            "alg_bytes": nbytes,
             "rel_linf_vs_plain": diff,
             "sample": f"{n_problems} problems (seeds {seed0}..{seed0 + n_problems - 1}: {n2} 2D, "
                       f"{n_problems - n2} 3D), forward + time reversal each, "
@@ -353,6 +395,7 @@ def _served_summary(reqs, arrivals, done_at, service, rate, steps_lo, steps_hi, 
         finish[i] = tfree
     lat_plain = finish - arrivals
     plain_tp = cell_steps / float(np.max(finish)) / 1e6
+    nbytes = sum(_problem_bytes(kg, med, sens, kg.Nt) for kg, med, src, sens, _ in reqs)
     pct = lambda a, q: round(float(np.percentile(a, q)), 3)  # noqa: E731
     log(f"[bench] served: {n_req} requests at {rate}/s: p50 {pct(lat, 50)} s p99 {pct(lat, 99)} s, "
         f"{eng_tp:.0f} Mcell-steps/s; one at a time: p50 {pct(lat_plain, 50)} s "
@@ -362,6 +405,8 @@ def _served_summary(reqs, arrivals, done_at, service, rate, steps_lo, steps_hi, 
             "plain_value": round(plain_tp, 1), "plain_p50_s": pct(lat_plain, 50),
             "plain_p99_s": pct(lat_plain, 99),
             "speedup_vs_plain": round(eng_tp / plain_tp, 2),
+            "roofline_frac": _roofline(nbytes, float(np.max(done_at))),
+            "plain_roofline_frac": _roofline(nbytes, float(np.max(finish))),
             "sample": f"{n_req} requests, Poisson at {rate}/s, {steps_lo}-{steps_hi} steps each "
                       f"(seed {seed}), {cell_steps / 1e9:.1f} G cell-steps"}
 
@@ -558,14 +603,14 @@ def main():
     ap.add_argument("--cfg3", type=int, default=384,
                     help="grid size of the nonlinear precision-study leg (0: skip)")
     ap.add_argument("--cfg3-steps", type=int, default=2000)
-    ap.add_argument("--batched", type=int, default=24,
+    ap.add_argument("--batched", type=int, default=96,
                     help="problems in the batched time-reversal leg (0: skip)")
     ap.add_argument("--served", type=int, default=24,
                     help="requests in the served leg (0: skip)")
     ap.add_argument("--served-rate", type=float, default=10.0)
     ap.add_argument("--served-sweep", type=float, nargs="+", metavar="RATE",
                     help="run only the served workload at these arrival rates (one JSON line)")
-    ap.add_argument("--served-steps", type=int, nargs=2, default=[200, 600])
+    ap.add_argument("--served-steps", type=int, nargs=2, default=[500, 3000])
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only the calibration copy (under rocprofv3 --pmc)")
     ap.add_argument("--reduce-traffic", nargs=4, metavar="DIR",

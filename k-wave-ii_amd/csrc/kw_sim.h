@@ -415,7 +415,7 @@ struct Sim {
       const int Lx = stage_length(G.Nx);
       const size_t pencil = sizeof(cf) * (size_t)kNPX * x_lds_stride(Lx);
       const size_t twb = sizeof(cf) * (size_t)Lx;
-      int P = std::max(1, 256 / Gx);
+      int P = std::max(1, std::min(pencils_env("KW_PENCILS_X", 256 / Gx), 512 / Gx));
       P = std::min<int64_t>(P, (int64_t)((maxl - twb) / pencil));
       P = std::max(1, std::min(P, capP));
       this->P.Gx = Gx;

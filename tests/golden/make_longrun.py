@@ -19,6 +19,7 @@ Run from the repository root:
     python tests/golden/make_longrun.py cfg2 64
     python tests/golden/make_longrun.py cfg2 128
     python tests/golden/make_longrun.py cfg3 96
+    python tests/golden/make_longrun.py cfg3 192   (about 2 h on 8 cores)
 """
 
 import sys
