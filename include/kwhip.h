@@ -247,6 +247,19 @@ typedef struct kw_kernel_stats {
  * headline time). */
 int kw_sim_step_timed(kw_sim* sim, int64_t nsteps, double* ms, kw_kernel_stats* stats,
                       char* err, size_t errlen);
+/* Pencils per block the library launches for a problem's grid, for the x,
+ * strided y and last-axis pass families (tiles[1] is 0 in 1-D and 2-D).
+ * They come from a tile autotuner: the first time a grid shape and equation
+ * of state (absorbing, nonlinear) is seen, a scratch simulation of that
+ * shape steps under each candidate with every launch timed by device
+ * events, and each family takes its fastest candidate.  Choices are cached
+ * per process and, when the environment variable KW_TUNE_FILE names a text
+ * file, in that file for later processes; grids of 2^24 cells and more, the
+ * no-allocation entry point (which only reads the cache) and KW_TUNE=0 keep
+ * the built-in tiles.  Tiles never change results.  No counterpart in the
+ * reference, whose native engines fix their launch shapes. */
+int kw_tiles(const kw_problem* prob, int32_t tiles[3], char* err, size_t errlen);
+
 /* Name of kernel kind k (0 <= k < KW_NKINDS). */
 const char* kw_kind_name(int k);
 

@@ -497,6 +497,9 @@ struct Engine {
   int64_t budget = 0, held = 0;  // device bytes: budget, held (estimates until created)
   int32_t steps_per_round = 32;
   bool use_graphs = true;
+  bool sjf_only = false;  // one lane: least remaining work first
+  // cells per launch above which a group's launches fill the device
+  static constexpr double kBigCells = double(1 << 22);
   bool stop = false;
   std::vector<void*> streams;  // compute streams, one per group (round robin)
   void* admit_stream = nullptr;
@@ -571,7 +574,12 @@ struct Engine {
       int code = KW_OK;
       std::string msg;
       try {
-        sim = create_sim(&t->prob->pb, admit_stream);
+        {
+          // tiles come from the autotuner's cache (or the built-in rule):
+          // a measurement here would hold up every admission behind it
+          TuneCacheOnly cache_only;
+          sim = create_sim(&t->prob->pb, admit_stream);
+        }
         // the failure flag lives in the engine's pinned slot
         __atomic_store_n(&flags[t->flag], 0, __ATOMIC_RELEASE);
         sim->P.bad = flags + t->flag;
@@ -680,9 +688,10 @@ struct Engine {
   // A group has at most one round (steps_per_round steps of all members) in
   // flight, on whichever stream of the pool is free when the round is
   // issued; rounds of different groups overlap on the device.  Members join
-  // and leave a group only between its rounds.  Free streams go to the
-  // groups with the least remaining work first, so short simulations are not
-  // queued behind the rounds of long ones.
+  // and leave a group only between its rounds.  Free streams go to groups in
+  // two lanes (large launches by most remaining work, small ones by least),
+  // so short simulations are not queued behind the rounds of long ones and
+  // large groups keep the device busy while small ones wait on latency.
   void step_loop() {
     std::map<std::array<int, 5>, std::unique_ptr<Group>> groups;
     std::vector<char> stream_busy(streams.size(), 0);
@@ -758,8 +767,22 @@ struct Engine {
                 [](const std::pair<double, Group*>& a, const std::pair<double, Group*>& b) {
                   return a.first < b.first;
                 });
+      // two lanes: the first half of the streams serves groups whose
+      // launches fill the device (largest remaining work first), the other
+      // half the latency-bound small groups (least remaining work first);
+      // a lane with nothing of its own kind takes the other kind, so small
+      // rounds run beside large ones instead of queueing in front of them
+      std::vector<Group*> big, small;
       for (auto& wg : idle) {
-        Group& g = *wg.second;
+        double cells = 0;
+        for (Ticket* t : wg.second->members) cells += (double)t->sim->G.ncell;
+        (sjf_only || cells < kBigCells ? small : big).push_back(wg.second);
+      }
+      std::reverse(big.begin(), big.end());
+      size_t ib = 0, is = 0;
+      const size_t nbig_lanes = sjf_only ? 0 : (streams.size() + 1) / 2;
+      for (;;) {
+        if (ib == big.size() && is == small.size()) break;
         int slot = -1;
         for (size_t k = 0; k < streams.size() && slot < 0; ++k)
           if (!stream_busy[k]) slot = (int)k;
@@ -767,6 +790,13 @@ struct Engine {
           busy = true;  // every stream has a round in flight
           break;
         }
+        const bool big_lane = (size_t)slot < nbig_lanes;
+        Group* pick;
+        if (big_lane ? ib < big.size() : is == small.size())
+          pick = big[ib++];
+        else
+          pick = small[is++];
+        Group& g = *pick;
         int code = KW_OK;
         std::string err;
         try {
@@ -911,6 +941,7 @@ int kw_engine_create(const kw_engine_config* cfg, kw_engine** engine, char* err,
     E.budget = (cfg && cfg->device_bytes > 0) ? cfg->device_bytes : (int64_t)((double)fr * 0.9);
     E.steps_per_round = (cfg && cfg->steps_per_round > 0) ? cfg->steps_per_round : 32;
     E.use_graphs = std::getenv("KW_NO_GRAPHS") == nullptr;
+    E.sjf_only = std::getenv("KW_ENGINE_SJF") != nullptr;
     E.flags = (int32_t*)kw::dev_host_alloc(sizeof(int32_t) * kw::Engine::kFlagSlots);
     std::memset(E.flags, 0, sizeof(int32_t) * kw::Engine::kFlagSlots);
     E.free_flags.reserve(kw::Engine::kFlagSlots);

@@ -5,7 +5,10 @@
 // implements the single-simulation C ABI declared in include/kwhip.h.
 #include "kw_sim.h"
 
+#include <cstring>
+#include <map>
 #include <mutex>
+#include <tuple>
 
 namespace kw {
 
@@ -289,6 +292,7 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream, Arena* ar
     P.rec_grid = S->time_rev ? 0u : S->record & (KW_REC_U_MAX_ALL | KW_REC_U_MIN_ALL);
   }
 
+  S->tiles = choose_tiles(*pr, S->stream, arena == nullptr);
   S->plan_launches();
   P.fx = S->make_plan(G.Nx);
   P.fy = S->make_plan(G.Ny);
@@ -901,8 +905,193 @@ void read_outputs(Sim& S, kw_outputs* o) {
   }
 }
 
+// ---------------------------------------------------------------------------
+// tile autotuner
+// ---------------------------------------------------------------------------
+// The pencils per block of each pass family are chosen by measurement the
+// first time a grid shape and equation of state is seen: a scratch
+// simulation of that shape (scalar medium, no sources or sensors, so every
+// field stays zero and no record is written) steps under each candidate
+// tile with every launch bracketed by device events, and each family takes
+// the candidate with the least time of its own kernels (the families are
+// independent knobs).  The built-in rule stays unless a candidate beats it
+// by more than 2 %.  Choices are cached for the process and, with
+// KW_TUNE_FILE set, in that text file (one line per grid:
+// "dim Nx Ny Nz absorbing nonlinear x y l"), which later processes read
+// instead of measuring again.  KW_TUNE=0 keeps the built-in rule; grids of
+// 2^24 cells and more keep it too (their launches fill the device with the
+// rule's tiles; it is the best of the sweeps at 256^3 and 384^3); the
+// emulator measures only with KW_TUNE=1.
+
+namespace {
+
+struct TuneKey {
+  int dim, Nx, Ny, Nz, absorbing, nonlinear;
+  bool operator<(const TuneKey& o) const {
+    return std::tie(dim, Nx, Ny, Nz, absorbing, nonlinear) <
+           std::tie(o.dim, o.Nx, o.Ny, o.Nz, o.absorbing, o.nonlinear);
+  }
+};
+
+std::mutex g_tune_mu;
+std::map<TuneKey, Tiles> g_tune;
+bool g_tune_file_read = false;
+thread_local bool t_no_tune = false;        // inside a scratch simulation
+thread_local const Tiles* t_force = nullptr;  // a candidate of the measurement
+thread_local bool t_cache_only = false;       // read the cache, never measure
+
+struct NoTune {
+  bool prev;
+  NoTune() : prev(t_no_tune) { t_no_tune = true; }
+  ~NoTune() { t_no_tune = prev; }
+};
+
+void read_tune_file() {
+  g_tune_file_read = true;
+  const char* path = std::getenv("KW_TUNE_FILE");
+  if (!path) return;
+  FILE* f = std::fopen(path, "r");
+  if (!f) return;
+  TuneKey k{};
+  Tiles t;
+  while (std::fscanf(f, "%d %d %d %d %d %d %d %d %d", &k.dim, &k.Nx, &k.Ny, &k.Nz, &k.absorbing,
+                     &k.nonlinear, &t.x, &t.y, &t.l) == 9)
+    if (t.x > 0 && t.y > 0 && t.l > 0) g_tune[k] = t;
+  std::fclose(f);
+}
+
+// the measurement (g_tune_mu held).  Every candidate gets a simulation of
+// its own: the spectral tables are laid out by the last-axis tile when a
+// simulation is created, so a simulation never changes its tiles.
+Tiles measure_tiles(const kw_problem& pr, void* stream) {
+  NoTune guard;
+  kw_problem tp{};
+  tp.dim = pr.dim;
+  tp.Nx = pr.Nx;
+  tp.Ny = pr.Ny;
+  tp.Nz = pr.Nz;
+  tp.Nt = 16;
+  tp.dx = pr.dx;
+  tp.dy = pr.dy;
+  tp.dz = pr.dz;
+  tp.dt = pr.dt;
+  tp.c_ref = pr.c_ref;
+  tp.c0_scalar = pr.c_ref;
+  tp.rho0_scalar = 1000.0;
+  tp.nonlinear_flag = pr.nonlinear_flag;
+  tp.BonA_scalar = 6.0;
+  tp.absorbing_flag = pr.absorbing_flag;
+  tp.alpha_coeff_scalar = 0.5;
+  tp.alpha_power = pr.alpha_power;
+  tp.alpha_mode = pr.alpha_mode;
+  for (int a = 0; a < 3; ++a) {
+    tp.pml_size[a] = pr.pml_size[a];
+    tp.pml_alpha[a] = pr.pml_alpha[a];
+  }
+  tp.use_kspace = pr.use_kspace;
+  tp.use_sg = pr.use_sg;
+  // candidates: the built-in rule first, then powers of two (each family
+  // clamps a candidate to what its threads and LDS allow)
+  std::vector<Tiles> cand{Tiles{}};
+  for (int c = 1; c <= 64; c *= 2) cand.push_back(Tiles{c, c, c});
+  struct Eff {
+    int x, y, l;
+    double tx, ty, tl;
+  };
+  std::vector<Eff> eff;
+  for (const Tiles& c : cand) {
+    t_force = &c;
+    std::unique_ptr<Sim> S;
+    try {
+      S = create_sim(&tp, stream);
+    } catch (...) {
+      t_force = nullptr;
+      throw;
+    }
+    t_force = nullptr;
+    const Eff e0{S->P.Px, S->P.Py, S->P.Pl, 0.0, 0.0, 0.0};
+    bool seen = false;
+    for (const Eff& e : eff) seen = seen || (e.x == e0.x && e.y == e0.y && e.l == e0.l);
+    if (seen) continue;
+    step_n(*S, 2);  // the first step takes its own path; one warm-up step
+    dev_sync(S->stream);
+    S->timing = true;
+    step_n(*S, 8);
+    S->timing = false;
+    double t[KW_NKINDS] = {};
+    for (size_t k = 0; k < S->ev_used.size(); ++k) {
+      t[S->ev_kind[k]] += dev_event_elapsed_ms(S->ev_used[k].first, S->ev_used[k].second);
+      S->ev_free.push_back(S->ev_used[k].first);
+      S->ev_free.push_back(S->ev_used[k].second);
+    }
+    S->ev_used.clear();
+    S->ev_kind.clear();
+    S->ev_bytes.clear();
+    Eff e = e0;
+    e.ty = t[PK_YFWD] + t[PK_YINV];
+    e.tl = t[PK_SPEC];
+    e.tx = t[PK_X_R2C] + t[PK_X_C2R] + t[PK_X_GRAD] + t[PK_X_DIV] + t[PK_X_EOS];
+    eff.push_back(e);
+    dev_sync(S->stream);
+  }
+  // each family: the fastest candidate, the rule's tile unless beaten by 2 %
+  auto pick = [&](auto time, auto tile) {
+    const Eff& rule = eff[0];
+    int best = tile(rule);
+    double bt = time(rule);
+    for (const Eff& e : eff)
+      if (time(e) < bt && time(e) < 0.98 * time(rule)) {
+        bt = time(e);
+        best = tile(e);
+      }
+    return best;
+  };
+  Tiles out;
+  out.x = pick([](const Eff& e) { return e.tx; }, [](const Eff& e) { return e.x; });
+  out.y = pr.dim == 3 ? pick([](const Eff& e) { return e.ty; }, [](const Eff& e) { return e.y; })
+                      : 0;
+  out.l = pick([](const Eff& e) { return e.tl; }, [](const Eff& e) { return e.l; });
+  return out;
+}
+
+}  // namespace
+
+TuneCacheOnly::TuneCacheOnly() : prev(t_cache_only) { t_cache_only = true; }
+TuneCacheOnly::~TuneCacheOnly() { t_cache_only = prev; }
+
+Tiles choose_tiles(const kw_problem& pr, void* stream, bool may_tune) {
+  if (t_force) return *t_force;
+  if (t_no_tune) return Tiles{};
+  const char* env = std::getenv("KW_TUNE");
+  if (env && std::strcmp(env, "0") == 0) return Tiles{};
+  const TuneKey key{pr.dim,          (int)pr.Nx,     (int)pr.Ny,
+                    (int)pr.Nz,      pr.absorbing_flag != 0 ? 1 : 0,
+                    pr.nonlinear_flag != 0 ? 1 : 0};
+  std::lock_guard<std::mutex> lk(g_tune_mu);
+  if (!g_tune_file_read) read_tune_file();
+  auto it = g_tune.find(key);
+  if (it != g_tune.end()) return it->second;
+  const bool emulator = std::strcmp(backend_name(), "emulator") == 0;
+  const double cells = (double)pr.Nx * (double)pr.Ny * (double)(pr.dim == 3 ? pr.Nz : 1);
+  const bool forced = env && std::strcmp(env, "1") == 0;
+  if (!may_tune || t_cache_only || pr.Ny < 2 || (emulator && !forced) ||
+      (!forced && cells >= 16777216.0))
+    return Tiles{};
+  const Tiles t = measure_tiles(pr, stream);
+  g_tune[key] = t;
+  if (const char* path = std::getenv("KW_TUNE_FILE")) {
+    if (FILE* f = std::fopen(path, "a")) {
+      std::fprintf(f, "%d %d %d %d %d %d %d %d %d\n", key.dim, key.Nx, key.Ny, key.Nz,
+                   key.absorbing, key.nonlinear, t.x, t.y, t.l);
+      std::fclose(f);
+    }
+  }
+  return t;
+}
+
 // kw_smooth: one filter pass on a scratch problem
 static void smooth(int dim, const int64_t N[3], const double* in, const double* win, double* out) {
+  NoTune guard;
   kw_problem pr{};
   pr.dim = dim;
   pr.Nx = N[0];
@@ -973,6 +1162,33 @@ int kw_output_sizes(const kw_problem* prob, kw_sizes* sizes, char* err, size_t e
     sizes->num_recorded_time_points =
         (prob->sensor_num_points > 0 && !tr) ? std::max<int64_t>(0, prob->Nt - rs + 1) : 0;
     sizes->grid_cells = prob->Nx * prob->Ny * (prob->dim == 3 ? prob->Nz : 1);
+  });
+}
+
+int kw_tiles(const kw_problem* prob, int32_t tiles[3], char* err, size_t errlen) {
+  return kw_guarded(err, errlen, [&]() {
+    if (!prob || !tiles) kw::fail(KW_EINVAL, "NULL argument");
+    if (prob->dim < 1 || prob->dim > 3) kw::fail(KW_EINVAL, "dim must be 1, 2 or 3");
+    kw_problem row = *prob;
+    if (row.dim == 1) {  // a 1-D problem runs as a 2-D grid of one row
+      row.dim = 2;
+      row.Ny = row.Nz = 1;
+      row.dy = row.dx;
+    }
+    if (row.dim == 2) row.Nz = 1;
+    if (row.Nx < 2 || (prob->dim >= 2 && row.Ny < 2) || (row.dim == 3 && row.Nz < 2))
+      kw::fail(KW_EINVAL, "grid sizes must be at least 2");
+    if (!(row.dt > 0) || !(row.dx > 0) || !(row.dy > 0) || (row.dim == 3 && !(row.dz > 0)) ||
+        !(row.c_ref > 0))
+      kw::fail(KW_EINVAL, "dt, grid spacings and c_ref must be positive");
+    kw::Sim S;
+    S.own_stream = false;
+    S.G = kw::make_geometry(row.dim, row.Nx, row.Ny, row.Nz);
+    S.tiles = kw::choose_tiles(row, nullptr, true);
+    S.plan_launches();
+    tiles[0] = S.P.Px;
+    tiles[1] = row.dim == 3 ? S.P.Py : 0;
+    tiles[2] = S.P.Pl;
   });
 }
 

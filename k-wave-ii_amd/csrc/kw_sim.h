@@ -182,6 +182,15 @@ inline int pencils_env(const char* name, int dflt) {
   return v > 0 ? v : dflt;
 }
 
+// Pencils per block of the three pass families (x passes, strided y
+// passes, last-axis passes); 0 takes the built-in rule (256 threads per x
+// block, 16 pencils per y and last-axis block).  The tile changes which
+// block handles a pencil, never the arithmetic on it, so results do not
+// depend on it.
+struct Tiles {
+  int x = 0, y = 0, l = 0;
+};
+
 // Bump allocator over a device region the caller owns (the static scratch
 // array of the entry point that allocates nothing): 256-byte aligned
 // buffers, released all at once when the region is reused.
@@ -196,6 +205,7 @@ struct Sim {
   ProbDev P{};
   ProbDev* dP = nullptr;
   int32_t* d_tile0 = nullptr;
+  Tiles tiles;  // pencils per block chosen for this grid (plan_launches)
   void* stream = nullptr;
   bool own_stream = true;
   Arena* arena = nullptr;  // set: buffers come from the arena, not dev_alloc
@@ -415,7 +425,8 @@ struct Sim {
       const int Lx = stage_length(G.Nx);
       const size_t pencil = sizeof(cf) * (size_t)kNPX * x_lds_stride(Lx);
       const size_t twb = sizeof(cf) * (size_t)Lx;
-      int P = std::max(1, std::min(pencils_env("KW_PENCILS_X", 256 / Gx), 512 / Gx));
+      int P = std::max(1, std::min(pencils_env("KW_PENCILS_X", tiles.x ? tiles.x : 256 / Gx),
+                                   512 / Gx));
       P = std::min<int64_t>(P, (int64_t)((maxl - twb) / pencil));
       P = std::max(1, std::min(P, capP));
       this->P.Gx = Gx;
@@ -430,7 +441,7 @@ struct Sim {
     if (G.dim == 3) {
       const int Gy = Gof(G.Ny);
       const int Ly = stage_length(G.Ny);
-      int P = std::max(1, std::min(pencils_env("KW_PENCILS_Y", 16), 1024 / Gy));
+      int P = std::max(1, std::min(pencils_env("KW_PENCILS_Y", tiles.y ? tiles.y : 16), 1024 / Gy));
       P = std::min<int64_t>(P, (int64_t)((maxl - sizeof(cf) * Ly) / (sizeof(cf) * (Ly + 1))));
       P = std::max(1, std::min(P, capP));
       this->P.Gy = Gy;
@@ -446,7 +457,7 @@ struct Sim {
       const int Nl = G.dim == 3 ? G.Nz : G.Ny;
       const int Gl = Gof(Nl);
       const int Ll = stage_length(Nl);
-      int P = std::max(1, std::min(pencils_env("KW_PENCILS_L", 16), 1024 / Gl));
+      int P = std::max(1, std::min(pencils_env("KW_PENCILS_L", tiles.l ? tiles.l : 16), 1024 / Gl));
       P = std::min<int64_t>(P, (int64_t)((maxl - sizeof(cf) * Ll) / (2 * sizeof(cf) * Ll)));
       P = std::max(1, std::min(P, capP));
       this->P.Gl = Gl;
@@ -562,6 +573,16 @@ void record_velocity(Sim& S);
 void record_directivity(Sim& S);
 void step_once(Sim& S);
 void step_n(Sim& S, int64_t nsteps);
+// pencils per block for a problem's grid: the autotuner's measurement,
+// cached per grid shape and equation of state (see kw_tiles in kwhip.h)
+Tiles choose_tiles(const kw_problem& pr, void* stream, bool may_tune);
+// while one lives, choose_tiles on this thread only reads the cache (the
+// engine's admission threads: a measurement there would hold up admission)
+struct TuneCacheOnly {
+  bool prev;
+  TuneCacheOnly();
+  ~TuneCacheOnly();
+};
 int64_t recorded(const Sim& S);
 void read_outputs(Sim& S, kw_outputs* o);
 

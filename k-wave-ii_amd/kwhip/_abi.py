@@ -193,6 +193,7 @@ def load(backend: str = "hip") -> C.CDLL:
     lib.kw_run.argtypes = [P(Problem), P(Outputs)] + err
     lib.kw_run_noalloc.argtypes = [P(Problem), P(Outputs)] + err
     lib.kw_sim_create.argtypes = [P(Problem), P(C.c_void_p)] + err
+    lib.kw_tiles.argtypes = [P(Problem), P(C.c_int32)] + err
     lib.kw_sim_step.argtypes = [C.c_void_p, C.c_int64] + err
     lib.kw_sim_sync.argtypes = [C.c_void_p] + err
     lib.kw_sim_read.argtypes = [C.c_void_p, P(Outputs)] + err

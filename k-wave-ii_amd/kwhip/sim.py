@@ -14,6 +14,18 @@ from . import _abi
 from .solver import _parse_options, alloc_outputs, prepare, shape_outputs
 
 
+def launch_tiles(kgrid, medium, source, sensor, dim, *args, **kwargs) -> tuple:
+    """Pencils per block (x, y, last axis) the library launches for this
+    problem's grid (kw_tiles: measured by the tile autotuner on first use of
+    a grid shape, cached afterwards; y is 0 in 1-D and 2-D)."""
+    opts = _parse_options(args, kwargs)
+    pp = prepare(kgrid, medium, source, sensor, dim, opts)
+    lib = _abi.load(pp.info.backend)
+    tiles = (C.c_int32 * 3)()
+    _abi.call(lib.kw_tiles, C.byref(pp.prob), tiles)
+    return tuple(int(v) for v in tiles)
+
+
 class Simulation:
     """Simulation(kgrid, medium, source, sensor, dim, **options): inputs are
     copied to the device and all operators built on construction."""
