@@ -65,6 +65,11 @@ extern "C" {
  * points, time series (kspaceFirstOrder_extractSensorData.m:124-292; binary
  * sensor masks only, kspaceFirstOrder_inputChecking.m:1363-1366) */
 #define KW_REC_U_SPLIT_FIELD (1u << 15)
+/* the summed split density sum_d rho_d at the sensor points, time series
+ * (the new toolbox's AcousticSensor densitySensor 'on',
+ * +kwave/+toolbox/@AcousticSensor/AcousticSensor.m:153-173 with
+ * @Sensor/Sensor.m ProcessSensorData 'sum') */
+#define KW_REC_RHO (1u << 16)
 
 /* directivity patterns (sensor.directivity_pattern) */
 #define KW_DIR_PRESSURE 0
@@ -201,6 +206,7 @@ typedef struct kw_outputs {
   float* u_non_staggered[3]; /* [points x recorded steps] per component */
   float* u_split_p[3];       /* [points x recorded steps] per component */
   float* u_split_s[3];
+  float* rho;                /* KW_REC_RHO: [points x recorded steps] */
 } kw_outputs;
 
 typedef struct kw_sim kw_sim;
@@ -229,6 +235,41 @@ int kw_sim_read(kw_sim* sim, kw_outputs* out, char* err, size_t errlen);
 int64_t kw_sim_steps_done(const kw_sim* sim);
 int64_t kw_sim_device_bytes(const kw_sim* sim);
 int kw_sim_destroy(kw_sim* sim);
+
+/* Entry points of the new toolbox's time-domain solver, whose run() may be
+ * called repeatedly on the same state, with a new time step each time
+ * (+kwave/+toolbox/@TimeDomainSolver/run.m:58-101,
+ * @AcousticSolver/executeTimeStep.m:35-260):
+ *
+ * kw_sim_extend: grow the simulation to Nt_total time steps (at least its
+ * current number); recorded time series keep their columns and gain the
+ * new ones (@AcousticSensor/AcousticSensor.m initialiseSensorData appends
+ * columns).  Not for time reversal or the no-allocation entry point.
+ *
+ * kw_sim_change_dt: the next step is the transition from time step dt_prev
+ * to dt_new (executeTimeStep.m:45-55,143-176): its velocity update uses the
+ * mean step (dt_prev + dt_new)/2 (in single precision, as the reference's
+ * single prevTimeStep makes it), the PML of that step, the two-step k-space
+ * correction kappa and the split correction term dt * kappaSplit(pml(u))
+ * (setkSpaceCorrection.m:36-57, kappaSplitCorrection.m:56-60); its density
+ * update and every later step use dt_new.  dt_prev = 0 gives the first step
+ * after an initial particle velocity.  A dt_new equal to dt_prev in single
+ * precision changes nothing.
+ *
+ * kw_sim_init_velocity: the first step as the initial condition with a
+ * given particle velocity (executeTimeStep.m:128-142): p = p0, rho_d =
+ * p0 / (dim c0^2), u_d = u[d] (already on the staggered grid; Nx*Ny*Nz
+ * values each, column-major), recorded like any first step.  Needs p0 and
+ * no step taken.
+ *
+ * kw_sim_get_fields: copy the current p, u_d and rho_d (the solver's
+ * pressure, velocity and densitySplit) to host arrays of Nx*Ny*Nz floats;
+ * NULL skips a field. */
+int kw_sim_extend(kw_sim* sim, int64_t Nt_total, char* err, size_t errlen);
+int kw_sim_change_dt(kw_sim* sim, double dt_prev, double dt_new, char* err, size_t errlen);
+int kw_sim_init_velocity(kw_sim* sim, const double* const u[3], char* err, size_t errlen);
+int kw_sim_get_fields(kw_sim* sim, float* p, float* const u[3], float* const rho[3], char* err,
+                      size_t errlen);
 
 /* Per-kernel-kind launch statistics of a timed run: summed device time
  * (each launch bracketed by events), launch count, and the compulsory HBM

@@ -979,6 +979,9 @@ enum EwKind : int32_t {
   EW_RECORD_UNS,     // non-staggered velocity component `which` at the sensor points
   EW_RECORD_SPLIT,   // compressional / shear parts of component `which`
   EW_RECORD_DIR,     // directional p of the sensor points of angle `which`
+  EW_AXIS_MUL,       // ptr = src * prof(axis `which`)
+  EW_AXIS_FOLD,      // ptr += alpha * src / prof(axis `which`)
+  EW_RECORD_RHO,     // sum_d rho_d at the sensor points
 };
 
 // EwArgs.rec value for "the problem's own statistics" (launches shared by
@@ -990,9 +993,18 @@ struct EwArgs {
   int32_t which;         // EW_SRC_GRID: 0 = p source, 1..3 = u source d-1
   int64_t n;             // EW_FILL length (in floats)
   float* ptr;            // EW_FILL target
-  const float* p0;       // EW_INIT_P0
+  const float* p0;       // EW_INIT_P0; EW_AXIS_*: the source grid
   uint32_t rec;          // EW_RECORD_*: which statistics to update
+  const float* prof;     // EW_AXIS_*: 1-D profile along axis `which`
+  float alpha;           // EW_AXIS_FOLD
 };
+
+// coordinate of cell i along axis a (column-major, x fastest)
+KW_BODY int64_t axis_coord(const ProbDev& P, int64_t i, int a) {
+  if (a == 0) return i % P.Nx;
+  if (a == 1) return (i / P.Nx) % P.Ny;
+  return i / ((int64_t)P.Nx * P.Ny);
+}
 
 // Field f at sensor point i: the grid point of a binary mask (`cell`), or
 // for Cartesian points interpolated linearly sum(f(tri) .* bc, 2) over the
@@ -1013,6 +1025,26 @@ KW_BODY void body_ew(Ctx& c, const Batch& B, const EwArgs& A) {
   const int64_t i = (int64_t)tile * EW_T + c.tid();
   const int t = KW_LD(P.t_index);
   switch (A.kind) {
+    case EW_AXIS_MUL: {
+      if (i >= P.ncell) return;
+      KW_ST(A.ptr + i, KW_LD(A.p0 + i) * KW_LD(A.prof + axis_coord(P, i, A.which)));
+      return;
+    }
+    case EW_AXIS_FOLD: {
+      if (i >= P.ncell) return;
+      const float f = KW_LD(A.prof + axis_coord(P, i, A.which));
+      KW_ST(A.ptr + i, KW_LD(A.ptr + i) + A.alpha * KW_LD(A.p0 + i) / f);
+      return;
+    }
+    case EW_RECORD_RHO: {
+      if (i >= P.nsens || t < P.record_start) return;
+      const int fi = t - P.record_start + 1;
+      const int64_t cell = P.sens_idx ? (int64_t)KW_LD(P.sens_idx + i) : i;
+      float r = sensor_value(P, P.rho[0], i, cell);
+      for (int d = 1; d < P.dim; ++d) r += sensor_value(P, P.rho[d], i, cell);
+      KW_ST(P.rho_ts + i + (int64_t)(fi - 1) * P.nsens, r);
+      return;
+    }
     case EW_INIT_P0: {
       if (i >= P.ncell) return;
       const float p0 = KW_LD(A.p0 + i);

@@ -157,6 +157,7 @@ struct ProbDev {
   float* usp;           // [ncell] compressional part of one component (scratch)
   float* usplit_p[3];   // [nsens x n_rec]
   float* usplit_s[3];
+  float* rho_ts;        // [nsens x n_rec] summed split density (KW_REC_RHO)
   // time reversal
   int32_t time_rev;
   const int32_t* tr_map;  // per cell: sensor point or -1

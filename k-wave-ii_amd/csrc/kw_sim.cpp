@@ -212,6 +212,14 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream, Arena* ar
   P.dt = (float)pr->dt;
   const int dim = G.dim;
   const double d[3] = {pr->dx, pr->dy, pr->dz};
+  for (int a = 0; a < 3; ++a) {
+    S->op.d[a] = d[a];
+    S->op.pml_size[a] = (int)pr->pml_size[a];
+    S->op.pml_alpha[a] = pr->pml_alpha[a];
+  }
+  S->op.c_ref = pr->c_ref;
+  S->op.use_sg = pr->use_sg != 0;
+  S->op.use_k = pr->use_kspace != 0;
   const int Nd[3] = {G.Nx, G.Ny, G.Nz};
 
   // sensor / time reversal / record bookkeeping
@@ -557,6 +565,7 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream, Arena* ar
     const int64_t ns = S->nsens;
     const uint32_t r = S->record;
     if (r & KW_REC_P) P.p_ts = S->alloc_zero<float>((size_t)ns * S->n_rec);
+    if (r & KW_REC_RHO) P.rho_ts = S->alloc_zero<float>((size_t)ns * S->n_rec);
     if (r & KW_REC_P_MAX) P.p_max = S->alloc_zero<double>(ns);
     if (r & KW_REC_P_MIN) P.p_min = S->alloc_zero<double>(ns);
     if (r & KW_REC_P_RMS) P.p_rms2 = S->alloc_zero<double>(ns);
@@ -782,6 +791,79 @@ void record_directivity(Sim& S) {
   }
 }
 
+// The velocity half of the new toolbox's transition step between time
+// steps (executeTimeStep.m:143-158 and :168-176):
+//   u_d = pmlm_d (pmlm_d u_d - dtm / rho_sg_d  D+_d κm p + dtm KS(pmlm_d u_d))
+// with the mean step dtm, its PML pmlm, the two-step κm and the split
+// correction KS (kappaSplitCorrection.m).  The correction is folded into
+// u first (u_d += dtm KS(pmlm_d u_d) / pmlm_d, so that pmlm_d u_d gains
+// dtm KS(.)), then the usual gradient pass runs with the transition's dt,
+// κ and PML; the divergence and everything after use the new step's.
+static void transition_gradient(Sim& S) {
+  Sim::DtChange& c = S.dtc;
+  const int dim = S.G.dim;
+  ProbDev& P = S.P;
+  if (c.ksplit) {
+    if (!S.scratch_grid) S.scratch_grid = S.alloc<float>((size_t)S.G.ncell);
+    const int nd = P.dim;  // 1-D problems carry one velocity component
+    for (int a = 0; a < nd; ++a) {
+      EwArgs m{};
+      m.kind = EW_AXIS_MUL;
+      m.which = a;
+      m.p0 = P.u[a];
+      m.ptr = S.scratch_grid;
+      m.prof = c.pml_sg_mean[a];
+      S.ew(m, S.lcell.nb);
+      // KS filter on spectral buffer 1 (buffer 0 carries fft(p))
+      PassArgs f = bufs(1, 1);
+      f.real[0] = S.scratch_grid;
+      S.pass(PK_X_R2C, f);
+      if (dim == 3) S.pass(PK_YFWD, f);
+      PassArgs fl = f;
+      fl.mode = SM_FILTER;
+      fl.table = c.ksplit;
+      S.pass(PK_SPEC, fl);
+      if (dim == 3) S.pass(PK_YINV, f);
+      S.pass(PK_X_C2R, f);
+      EwArgs fo{};
+      fo.kind = EW_AXIS_FOLD;
+      fo.which = a;
+      fo.p0 = S.scratch_grid;
+      fo.ptr = P.u[a];
+      fo.prof = c.pml_sg_mean[a];
+      fo.alpha = c.dt_mean;
+      S.ew(fo, S.lcell.nb);
+    }
+  }
+  // the velocity half with the mean step
+  const float* old_kappa = P.kappa;
+  const float* old_pml[3] = {P.pml[0], P.pml[1], P.pml[2]};
+  const float* old_sg[3] = {P.pml_sg[0], P.pml_sg[1], P.pml_sg[2]};
+  P.dt = c.dt_mean;
+  P.kappa = c.kappa_mix;
+  for (int a = 0; a < dim; ++a) P.pml_sg[a] = c.pml_sg_mean[a];
+  dev_h2d(S.dP, &P, sizeof(ProbDev), S.stream);
+  gradient(S, 0);
+  // the new step from here on (the copy waits for the gradient's kernels,
+  // so the old tables can go)
+  P.dt = c.dt_new;
+  P.kappa = c.kappa_new;
+  for (int a = 0; a < dim; ++a) {
+    P.pml[a] = c.pml_new[a];
+    P.pml_sg[a] = c.pml_sg_new[a];
+  }
+  dev_h2d(S.dP, &P, sizeof(ProbDev), S.stream);
+  S.release(old_kappa);
+  S.release(c.kappa_mix);
+  S.release(c.ksplit);
+  for (int a = 0; a < dim; ++a) {
+    S.release(old_pml[a]);
+    S.release(old_sg[a]);
+    S.release(c.pml_sg_mean[a]);
+  }
+  c = Sim::DtChange{};
+}
+
 static void record_step(Sim& S, int t, bool init) {
   if (S.time_rev || S.nsens == 0 || t < S.record_start) return;
   uint32_t pts = S.record & (kPointP | kPointU);
@@ -794,6 +876,11 @@ static void record_step(Sim& S, int t, bool init) {
     S.ew(e, S.lsens.nb);
   }
   if (S.record & (KW_REC_U_NON_STAGGERED | KW_REC_U_SPLIT_FIELD)) record_velocity(S);
+  if (S.record & KW_REC_RHO) {
+    EwArgs e{};
+    e.kind = EW_RECORD_RHO;
+    S.ew(e, S.lsens.nb);
+  }
   if (!S.dir_tab.empty()) record_directivity(S);
   uint32_t grid = S.record & (KW_REC_U_MAX_ALL | KW_REC_U_MIN_ALL);
   if (init) grid |= S.record & (KW_REC_P_MAX_ALL | KW_REC_P_MIN_ALL);
@@ -834,7 +921,10 @@ void step_once(Sim& S) {
     if (S.p_grid) src_grid(S, 0, t);
     for (int a = 0; a < dim; ++a)
       if (S.u_grid[a]) src_grid(S, a + 1, t);
-    gradient(S, 0);
+    if (S.dtc.armed)
+      transition_gradient(S);
+    else
+      gradient(S, 0);
     if (dim == 3) S.pass(PK_YFWD, bufs(3));
     PassArgs dv{};
     dv.mode = SM_DIV;
@@ -865,6 +955,141 @@ void step_n(Sim& S, int64_t nsteps) {
   for (int64_t i = 0; i < nsteps; ++i) step_once(S);
 }
 
+// ---------------------------------------------------------------------------
+// the new toolbox's repeated run(): more steps, a new time step, an initial
+// particle velocity
+// ---------------------------------------------------------------------------
+
+void extend(Sim& S, int64_t Nt_total) {
+  if (S.time_rev) fail(KW_EINVAL, "a time-reversal simulation cannot be extended");
+  if (S.arena) fail(KW_ESTATE, "the no-allocation entry point cannot extend a simulation");
+  if (Nt_total < S.Nt) fail(KW_EINVAL, "Nt_total is below the simulation's number of time steps");
+  if (Nt_total >= INT32_MAX) fail(KW_EUNSUP, "too many time steps");
+  if (Nt_total == S.Nt) return;
+  dev_sync(S.stream);
+  ProbDev& P = S.P;
+  const int64_t ns = S.nsens;
+  const int64_t n_old = S.n_rec;
+  const int64_t n_new = ns > 0 ? Nt_total - S.record_start + 1 : 0;
+  // time series [ns x n_rec], column-major: the old columns stay in front
+  auto grow = [&](float*& buf) {
+    if (!buf || n_new == n_old) return;
+    float* nb = S.alloc_zero<float>((size_t)(ns * n_new));
+    if (n_old > 0) {
+      std::vector<float> h((size_t)(ns * n_old));
+      dev_d2h(h.data(), buf, h.size() * sizeof(float), S.stream);
+      dev_h2d(nb, h.data(), h.size() * sizeof(float), S.stream);
+    }
+    S.release(buf);
+    buf = nb;
+  };
+  grow(P.p_ts);
+  grow(P.rho_ts);
+  for (int a = 0; a < 3; ++a) {
+    grow(P.u_ts[a]);
+    grow(P.uns_ts[a]);
+    grow(P.usplit_p[a]);
+    grow(P.usplit_s[a]);
+  }
+  S.Nt = S.Nt_eff = Nt_total;
+  S.n_rec = n_new;
+  P.Nt = (int32_t)Nt_total;
+  P.n_rec = (int32_t)n_new;
+  if (S.record & KW_REC_P_FINAL) P.p_store_at = (int32_t)S.Nt_eff;
+  dev_h2d(S.dP, &P, sizeof(ProbDev), S.stream);
+}
+
+void change_dt(Sim& S, double dt_prev, double dt_new) {
+  if (!(dt_new > 0) || !(dt_prev >= 0) || !std::isfinite(dt_new) || !std::isfinite(dt_prev))
+    fail(KW_EINVAL, "time steps must be finite, dt_new positive and dt_prev non-negative");
+  if (S.time_rev) fail(KW_EINVAL, "time reversal runs at one time step");
+  if (S.dtc.armed) fail(KW_ESTATE, "a time-step change is already pending");
+  const float f_prev = (float)dt_prev, f_new = (float)dt_new;
+  if (f_prev == f_new) return;
+  dev_sync(S.stream);
+  const Geometry& G = S.G;
+  const int dim = G.dim;
+  const Sim::OpParams& o = S.op;
+  // the reference's prevTimeStep is single: the mean is formed in single
+  const float f_mean = (f_new + f_prev) / 2.0f;
+  const double t1 = dt_prev, t2 = dt_new;
+  const int Nd[3] = {G.Nx, G.Ny, G.Nz};
+  const double Ntot = (double)G.ncell;
+  auto kmag = [&](int ix, int iy, int iz) {
+    const double kx = kvec(ix, G.Nx, o.d[0]), ky = kvec(iy, G.Ny, o.d[1]);
+    double acc = kx * kx + ky * ky;
+    if (dim == 3) {
+      const double kz = kvec(iz, G.Nz, o.d[2]);
+      acc += kz * kz;
+    }
+    return std::sqrt(acc);
+  };
+  Sim::DtChange c;
+  c.dt_mean = f_mean;
+  c.dt_new = f_new;
+  // setkSpaceCorrection.m:45-53 (NaN, at k = 0, becomes 1)
+  c.kappa_mix = S.spectral_table([&](int ix, int iy, int iz) {
+    if (!o.use_k) return 1.0 / Ntot;
+    const double ck = o.c_ref * kmag(ix, iy, iz);
+    double v = 2.0 * (std::sin(ck * t2 / 2) + std::sin(ck * t1 / 2) * (std::cos(ck * t2 / 2) /
+                                                                       std::cos(ck * t1 / 2))) /
+               (ck * (t1 + t2));
+    if (std::isnan(v)) v = 1.0;
+    return v / Ntot;
+  });
+  if (o.use_k)
+    c.ksplit = S.spectral_table([&](int ix, int iy, int iz) {
+      const double ck = o.c_ref * kmag(ix, iy, iz);
+      double v = 2.0 * (std::cos(ck * t2 / 2) / std::cos(ck * t1 / 2) - 1.0) / (t1 + t2);
+      if (std::isnan(v)) v = 1.0;
+      return v / Ntot;
+    });
+  c.kappa_new = S.spectral_table([&](int ix, int iy, int iz) {
+    return (o.use_k ? sinc(o.c_ref * kmag(ix, iy, iz) * t2 / 2) : 1.0) / Ntot;
+  });
+  // setupQuarticPML.m with the mean step (velocity half) and the new step
+  auto profile = [&](int a, double dt, bool staggered) {
+    const auto v = pml_profile(Nd[a], o.d[a], dt, o.c_ref, o.pml_size[a], o.pml_alpha[a],
+                               staggered);
+    std::vector<float> f(Nd[a]);
+    for (int i = 0; i < Nd[a]; ++i) f[i] = (float)v[i];
+    return S.upload(f);
+  };
+  for (int a = 0; a < dim; ++a) {
+    c.pml_sg_mean[a] = profile(a, (double)f_mean, o.use_sg);
+    c.pml_new[a] = profile(a, t2, false);
+    c.pml_sg_new[a] = profile(a, t2, o.use_sg);
+  }
+  c.armed = true;
+  S.dtc = c;
+}
+
+void init_velocity(Sim& S, const double* const u[3]) {
+  if (S.steps_done != 0) fail(KW_ESTATE, "the initial velocity is the first step");
+  if (!S.has_p0) fail(KW_EINVAL, "an initial velocity needs the initial pressure p0");
+  if (!u) fail(KW_EINVAL, "velocity is NULL");
+  const int nd = S.P.dim;
+  for (int a = 0; a < nd; ++a)
+    if (!u[a]) fail(KW_EINVAL, "a velocity component is NULL");
+  EwArgs e{};
+  e.kind = EW_INIT_P0;
+  e.p0 = S.p0;
+  S.ew(e, S.lcell.nb);
+  const int64_t n = S.G.ncell;
+  std::vector<float> h((size_t)n);
+  for (int a = 0; a < nd; ++a) {
+    parallel_for(n, [&](int64_t i) { h[(size_t)i] = (float)u[a][i]; });
+    dev_h2d(S.P.u[a], h.data(), (size_t)n * sizeof(float), S.stream);
+  }
+  // the step counter advances in the first pass of a step; the initial
+  // condition has none, so it is set here
+  const int32_t one = 1;
+  dev_h2d(S.P.t_index, &one, sizeof(int32_t), S.stream);
+  record_step(S, 1, true);
+  r2c_p(S);
+  S.steps_done = 1;
+}
+
 int64_t recorded(const Sim& S) {
   if (S.nsens == 0 || S.time_rev) return 0;
   return std::max<int64_t>(0, std::min<int64_t>(S.steps_done, S.Nt) - S.record_start + 1);
@@ -885,6 +1110,7 @@ void read_outputs(Sim& S, kw_outputs* o) {
     for (int64_t i = 0; i < ns; ++i) h[i] = nr > 0 ? std::sqrt(h[i] / (double)nr) : 0.0;
   };
   get(o->p, P.p_ts, (size_t)ns * S.n_rec * sizeof(float));
+  get(o->rho, P.rho_ts, (size_t)ns * S.n_rec * sizeof(float));
   get(o->p_max, P.p_max, ns * sizeof(double));
   get(o->p_min, P.p_min, ns * sizeof(double));
   rms(o->p_rms, P.p_rms2);
@@ -1225,6 +1451,42 @@ int kw_sim_read(kw_sim* sim, kw_outputs* out, char* err, size_t errlen) {
 int64_t kw_sim_steps_done(const kw_sim* sim) { return sim ? sim->s->steps_done : -1; }
 
 int64_t kw_sim_device_bytes(const kw_sim* sim) { return sim ? sim->s->bytes : -1; }
+
+int kw_sim_extend(kw_sim* sim, int64_t Nt_total, char* err, size_t errlen) {
+  return kw_guarded(err, errlen, [&]() {
+    if (!sim) kw::fail(KW_EINVAL, "NULL handle");
+    kw::extend(*sim->s, Nt_total);
+  });
+}
+
+int kw_sim_change_dt(kw_sim* sim, double dt_prev, double dt_new, char* err, size_t errlen) {
+  return kw_guarded(err, errlen, [&]() {
+    if (!sim) kw::fail(KW_EINVAL, "NULL handle");
+    kw::change_dt(*sim->s, dt_prev, dt_new);
+  });
+}
+
+int kw_sim_init_velocity(kw_sim* sim, const double* const u[3], char* err, size_t errlen) {
+  return kw_guarded(err, errlen, [&]() {
+    if (!sim) kw::fail(KW_EINVAL, "NULL handle");
+    kw::init_velocity(*sim->s, u);
+  });
+}
+
+int kw_sim_get_fields(kw_sim* sim, float* p, float* const u[3], float* const rho[3], char* err,
+                      size_t errlen) {
+  return kw_guarded(err, errlen, [&]() {
+    if (!sim) kw::fail(KW_EINVAL, "NULL handle");
+    kw::Sim& S = *sim->s;
+    kw::dev_sync(S.stream);
+    const size_t nb = (size_t)S.G.ncell * sizeof(float);
+    if (p) kw::dev_d2h(p, S.P.p, nb, S.stream);
+    for (int a = 0; a < S.P.dim; ++a) {
+      if (u && u[a]) kw::dev_d2h(u[a], S.P.u[a], nb, S.stream);
+      if (rho && rho[a]) kw::dev_d2h(rho[a], S.P.rho[a], nb, S.stream);
+    }
+  });
+}
 
 int kw_sim_destroy(kw_sim* sim) {
   if (!sim) return KW_OK;

@@ -220,6 +220,28 @@ struct Sim {
   float* p0 = nullptr;
   std::vector<float*> dir_tab;  // k-space filter per unique directivity angle
   LaunchGeom lx, ly, ll, lcell, lsens;
+  // what the dt-dependent operators are built from (a new time step
+  // rebuilds κ and the PML profiles: kw_sim_change_dt)
+  struct OpParams {
+    double d[3] = {0, 0, 0};
+    double c_ref = 0;
+    int pml_size[3] = {0, 0, 0};
+    double pml_alpha[3] = {0, 0, 0};
+    bool use_sg = true, use_k = true;
+  } op;
+  // a time-step change armed for the next step: the tables of the
+  // transition step's velocity half (mean step) and those of the new step
+  struct DtChange {
+    bool armed = false;
+    float dt_mean = 0.f, dt_new = 0.f;
+    float* kappa_mix = nullptr;  // two-step κ, velocity half
+    float* ksplit = nullptr;     // split correction (nullptr: none)
+    float* pml_sg_mean[3] = {nullptr, nullptr, nullptr};
+    float* kappa_new = nullptr;
+    float* pml_new[3] = {nullptr, nullptr, nullptr};
+    float* pml_sg_new[3] = {nullptr, nullptr, nullptr};
+  } dtc;
+  float* scratch_grid = nullptr;  // one real grid (the split correction)
   // per-kernel event timing
   bool timing = false;
   std::vector<std::pair<void*, void*>> ev_used;
@@ -257,6 +279,16 @@ struct Sim {
     }
     bytes += (int64_t)nb;
     return (T*)p;
+  }
+  // free a buffer of alloc() before the simulation ends (no-op in an arena)
+  void release(const void* q) {
+    if (!q || arena) return;
+    for (size_t i = 0; i < allocs.size(); ++i)
+      if (allocs[i] == q) {
+        dev_free(allocs[i]);
+        allocs.erase(allocs.begin() + (long)i);
+        return;
+      }
   }
   template <class T>
   T* alloc_zero(size_t n) {
@@ -573,6 +605,9 @@ void record_velocity(Sim& S);
 void record_directivity(Sim& S);
 void step_once(Sim& S);
 void step_n(Sim& S, int64_t nsteps);
+void extend(Sim& S, int64_t Nt_total);
+void change_dt(Sim& S, double dt_prev, double dt_new);
+void init_velocity(Sim& S, const double* const u[3]);
 // pencils per block for a problem's grid: the autotuner's measurement,
 // cached per grid shape and equation of state (see kw_tiles in kwhip.h)
 Tiles choose_tiles(const kw_problem& pr, void* stream, bool may_tune);

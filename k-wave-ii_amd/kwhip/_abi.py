@@ -97,6 +97,10 @@ class Sizes(C.Structure):
                 ("grid_cells", C.c_int64)]
 
 
+# summed split density at the sensor points (the new toolbox's density sensor)
+KW_REC_RHO = 1 << 16
+
+
 class Outputs(C.Structure):
     _fields_ = [
         ("p", c_float_p), ("p_max", c_double_p), ("p_min", c_double_p), ("p_rms", c_double_p),
@@ -104,7 +108,7 @@ class Outputs(C.Structure):
         ("u", c_float_p * 3), ("u_max", c_double_p * 3), ("u_min", c_double_p * 3),
         ("u_rms", c_double_p * 3), ("u_max_all", c_double_p * 3), ("u_min_all", c_double_p * 3),
         ("u_final", c_float_p * 3), ("u_non_staggered", c_float_p * 3),
-        ("u_split_p", c_float_p * 3), ("u_split_s", c_float_p * 3),
+        ("u_split_p", c_float_p * 3), ("u_split_s", c_float_p * 3), ("rho", c_float_p),
     ]
 
 
@@ -194,6 +198,10 @@ def load(backend: str = "hip") -> C.CDLL:
     lib.kw_run_noalloc.argtypes = [P(Problem), P(Outputs)] + err
     lib.kw_sim_create.argtypes = [P(Problem), P(C.c_void_p)] + err
     lib.kw_tiles.argtypes = [P(Problem), P(C.c_int32)] + err
+    lib.kw_sim_extend.argtypes = [C.c_void_p, C.c_int64] + err
+    lib.kw_sim_change_dt.argtypes = [C.c_void_p, C.c_double, C.c_double] + err
+    lib.kw_sim_init_velocity.argtypes = [C.c_void_p, c_double_p * 3] + err
+    lib.kw_sim_get_fields.argtypes = [C.c_void_p, c_float_p, c_float_p * 3, c_float_p * 3] + err
     lib.kw_sim_step.argtypes = [C.c_void_p, C.c_int64] + err
     lib.kw_sim_sync.argtypes = [C.c_void_p] + err
     lib.kw_sim_read.argtypes = [C.c_void_p, P(Outputs)] + err
