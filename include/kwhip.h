@@ -177,6 +177,18 @@ typedef struct kw_problem {
    * be NULL. */
   const uint64_t* sensor_mask_corners;
   int64_t sensor_num_cuboids;
+  /* kspaceFirstOrderAS (+kwave/+legacy/kspaceFirstOrderAS.m) with the
+   * 'WSWA-FFT' radial symmetry: 1 when this 2-D problem is the radially
+   * expanded grid of an axisymmetric one (x axial, y radial, Ny four times
+   * the radial size, every input laid out as kspaceFirstOrderAS.m:703-733
+   * lays out the fields: p-like grids [f, 0, -flip(f(2:end)), -f, 0,
+   * flip(f(2:end))] along y, the y-staggered ones [f, flip(f), -f,
+   * -flip(f)], material grids with the same layouts unsigned); pml_size[1]
+   * is the outer radial PML of the radial grid.  The library then builds
+   * the y PML on every copy (getPML.m with axisymmetric true) and adds the
+   * u_r / r term of the divergence (:954-969).  Lossless media, p0 and
+   * binary sensors only (KW_EUNSUP otherwise).  0 otherwise. */
+  int32_t axisymmetric;
 } kw_problem;
 
 typedef struct kw_sizes {

@@ -965,6 +965,8 @@ int kw_submit(kw_engine* engine, const kw_problem* prob, int64_t* ticket, char* 
   return kw_guarded(err, errlen, [&]() {
     if (!engine || !ticket) kw::fail(KW_EINVAL, "NULL argument");
     kw::check_shape(prob);
+    if (prob->axisymmetric)
+      kw::fail(KW_EUNSUP, "axisymmetric problems run through kw_run or kw_sim_*, not the engine");
     auto t = std::make_unique<kw::Ticket>();
     t->prob = std::make_unique<kw::ProblemCopy>(*prob);
     t->est_bytes = kw::estimate_bytes(*prob);

@@ -80,6 +80,8 @@ void save_problem(const kw_problem* pr, const char* path) {
     fail(KW_EUNSUP, "alpha_filter, alpha_mode and alpha_sign are not part of the file contract");
   if (!pr->use_kspace || !pr->use_sg)
     fail(KW_EUNSUP, "'UsekSpace' and 'UseSG' false are not part of the file contract");
+  if (pr->axisymmetric)
+    fail(KW_EUNSUP, "axisymmetric problems are not written to the file contract here");
   const int64_t Nx = pr->Nx, Ny = pr->Ny, Nz = dim == 3 ? pr->Nz : 1;
   const int64_t n = Nx * Ny * Nz;
   kwh5::Writer w;

@@ -982,6 +982,8 @@ enum EwKind : int32_t {
   EW_AXIS_MUL,       // ptr = src * prof(axis `which`)
   EW_AXIS_FOLD,      // ptr += alpha * src / prof(axis `which`)
   EW_RECORD_RHO,     // sum_d rho_d at the sensor points
+  EW_ADD,            // ptr[i] += src[i] for i < n (floats)
+  EW_AS_MIRROR,      // axisymmetric: copies 2-4 of a p-like field from copy 1
 };
 
 // EwArgs.rec value for "the problem's own statistics" (launches shared by
@@ -1034,6 +1036,31 @@ KW_BODY void body_ew(Ctx& c, const Batch& B, const EwArgs& A) {
       if (i >= P.ncell) return;
       const float f = KW_LD(A.prof + axis_coord(P, i, A.which));
       KW_ST(A.ptr + i, KW_LD(A.ptr + i) + A.alpha * KW_LD(A.p0 + i) / f);
+      return;
+    }
+    case EW_AS_MIRROR: {
+      // the expanded radial axis holds four copies of No points: a p-like
+      // field is [f, 0, -flip(f(2:end)), -f, 0, flip(f(2:end))]
+      // (kspaceFirstOrderAS.m:703-707); nonlinear terms do not keep that
+      // symmetry, so copies 2-4 are rewritten from copy 1 after each update
+      if (i >= P.ncell) return;
+      const int No = P.Ny / 4;
+      const int64_t x = i % P.Nx, y = i / P.Nx;
+      const int b = (int)(y / No), r = (int)(y % No);
+      if (b == 0) return;
+      float v = 0.f;
+      if (b == 2) {
+        v = -KW_LD(A.ptr + x + (int64_t)P.Nx * r);
+      } else if (r != 0) {
+        const float f = KW_LD(A.ptr + x + (int64_t)P.Nx * (No - r));
+        v = b == 1 ? -f : f;
+      }
+      KW_ST(A.ptr + i, v);
+      return;
+    }
+    case EW_ADD: {
+      if (i >= A.n) return;
+      KW_ST(A.ptr + i, KW_LD(A.ptr + i) + KW_LD(A.p0 + i));
       return;
     }
     case EW_RECORD_RHO: {

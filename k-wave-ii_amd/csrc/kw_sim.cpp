@@ -328,6 +328,21 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream, Arena* ar
   P.tau = scalar_field(*S, 0.f);
   P.eta = P.tau;
   P.eos = pr->absorbing_flag;
+  // kspaceFirstOrderAS on its radially expanded grid (see kw_problem)
+  S->axisym = pr->axisymmetric != 0;
+  if (pr->axisymmetric < 0 || pr->axisymmetric > 1) fail(KW_EINVAL, "axisymmetric must be 0 or 1");
+  if (S->axisym) {
+    if (pdim != 2) fail(KW_EINVAL, "an axisymmetric problem is a 2D grid");
+    if (G.Ny % 4 != 0) fail(KW_EINVAL, "the expanded radial grid of 'WSWA-FFT' has 4 x Ny points");
+    if (!pr->use_sg) fail(KW_EINVAL, "Optional input 'UseSG' is not supported for axisymmetric simulations.");
+    if (pr->absorbing_flag == 1)
+      fail(KW_EUNSUP, "the axisymmetric code takes lossless or Stokes media");
+    if (pr->p_source.flag > 0 || pr->u_source[0].flag > 0 || pr->u_source[1].flag > 0)
+      fail(KW_EUNSUP, "axisymmetric simulations here take source.p0 only");
+    if (S->time_rev) fail(KW_EUNSUP, "axisymmetric time reversal is not supported");
+    if (pr->sensor_tri || pr->directivity_angle || (pr->record_flags & KW_REC_U_SPLIT_FIELD))
+      fail(KW_EUNSUP, "axisymmetric simulations here take binary sensor masks");
+  }
   if (P.eos < 0 || P.eos > 2) fail(KW_EINVAL, "absorbing_flag must be 0, 1 or 2");
   const double y = P.eos == EOS_STOKES ? 2.0 : pr->alpha_power;
   if (P.eos == EOS_ABSORBING) {
@@ -377,8 +392,29 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream, Arena* ar
   const bool use_sg = pr->use_sg != 0;
   for (int a = 0; a < dim; ++a) {
     if (pr->pml_size[a] < 0 || 2 * pr->pml_size[a] > Nd[a]) fail(KW_EINVAL, "PML size larger than the grid");
-    const auto pr_ = pml_profile(Nd[a], d[a], pr->dt, pr->c_ref, (int)pr->pml_size[a], pr->pml_alpha[a], false);
-    const auto ps_ = pml_profile(Nd[a], d[a], pr->dt, pr->c_ref, (int)pr->pml_size[a], pr->pml_alpha[a], use_sg);
+    auto pr_ = pml_profile(Nd[a], d[a], pr->dt, pr->c_ref, (int)pr->pml_size[a], pr->pml_alpha[a], false);
+    auto ps_ = pml_profile(Nd[a], d[a], pr->dt, pr->c_ref, (int)pr->pml_size[a], pr->pml_alpha[a], use_sg);
+    if (S->axisym && a == 1) {
+      // getPML.m with axisymmetric true on the radial grid of No points (the
+      // outer PML only), laid out on the four copies of the expanded axis as
+      // the fields are: p-like [f, f(No), flip(f(2:No)), f, f(No),
+      // flip(f(2:No))] (the mirror columns hold p = 0), y-staggered [f,
+      // flip(f), f, flip(f)]
+      if (pr->pml_size[1] > Nd[1] / 4) fail(KW_EINVAL, "PML size larger than the grid");
+      const int No = Nd[1] / 4;
+      auto rp = pml_profile(No, d[1], pr->dt, pr->c_ref, (int)pr->pml_size[1], pr->pml_alpha[1], false);
+      auto rs = pml_profile(No, d[1], pr->dt, pr->c_ref, (int)pr->pml_size[1], pr->pml_alpha[1], true);
+      const int ps = (int)pr->pml_size[1];
+      if (2 * ps <= No)
+        for (int i = 0; i < ps; ++i) rp[i] = rs[i] = 1.0;
+      else
+        fail(KW_EUNSUP, "an axisymmetric radial PML wider than half the radial grid");
+      for (int j = 0; j < Nd[1]; ++j) {
+        const int b = j / No, r = j % No;
+        pr_[j] = (b & 1) == 0 ? rp[r] : (r == 0 ? rp[No - 1] : rp[No - r]);
+        ps_[j] = (b & 1) == 0 ? rs[r] : rs[No - 1 - r];
+      }
+    }
     std::vector<float> f1(Nd[a]), f2(Nd[a]);
     for (int i = 0; i < Nd[a]; ++i) {
       f1[i] = (float)pr_[i];
@@ -429,6 +465,22 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream, Arena* ar
   P.kappa = S->spectral_table([&](int ix, int iy, int iz) {
     return (use_k ? sinc(cref * kmag(ix, iy, iz) * dt / 2) : 1.0) / Ntot;
   });
+  if (S->axisym) {
+    // the radial divergence adds shift_y^- κ fft(u_y / r) (kspaceFirstOrderAS.m:
+    // 954-969): κ / Ny in a filter pass followed by the y shift (which
+    // carries the 1/(Nx Ny) of the round trip)
+    S->as_kappa = S->spectral_table([&](int ix, int iy, int iz) {
+      return (use_k ? sinc(cref * kmag(ix, iy, iz) * dt / 2) : 1.0) / (double)G.Ny;
+    });
+    const int No = G.Ny / 4;
+    std::vector<float> ir(G.Ny);
+    for (int j = 0; j < G.Ny; ++j) {
+      const int b = j / No, r = j % No;
+      const double rsg = ((b & 1) == 0 ? r : No - 1 - r) * d[1] + d[1] / 2;
+      ir[j] = (float)(((b & 1) == 0 ? 1.0 : -1.0) / rsg);
+    }
+    S->as_ir = S->upload(ir);
+  }
   if (P.eos == EOS_ABSORBING) {
     const bool no_abs = pr->alpha_mode == 1, no_disp = pr->alpha_mode == 2;
     // medium.alpha_filter (createAbsorptionVariables.m:56-76): both
@@ -470,7 +522,7 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream, Arena* ar
   const bool split = pr->sensor_num_points > 0 && pr->time_reversal_boundary_data == nullptr &&
                      (pr->record_flags & KW_REC_U_SPLIT_FIELD);
   if (split && pdim == 1) fail(KW_EINVAL, "u_split_field needs a 2D or 3D simulation");
-  const int nS = split ? dim + 1 : std::max(dim, 2);
+  const int nS = split ? dim + 1 : (S->axisym ? 3 : std::max(dim, 2));
   for (int b = 0; b < nS; ++b) P.S[b] = S->alloc_zero<cf>(G.nspec);
   for (int b = nS; b < 4; ++b) P.S[b] = P.S[nS - 1];
 
@@ -625,6 +677,19 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream, Arena* ar
       if (r & KW_REC_U_MAX_ALL) P.u_max_all[a] = S->alloc_zero<double>(n);
       if (r & KW_REC_U_MIN_ALL) P.u_min_all[a] = S->alloc_zero<double>(n);
     }
+  }
+
+  if (S->axisym && !P.ushift[1]) {
+    // the y shift of the u_r / r term: ifftshift(exp(-i ky dy/2)) with the
+    // 1/(Nx Ny) of the round trip and the real part at the Nyquist bin, as
+    // the non-staggered velocity record builds it
+    std::vector<cf> sh(G.Ny);
+    for (int i = 0; i < G.Ny; ++i) {
+      std::complex<double> v = std::exp(std::complex<double>(0.0, -kvec(i, G.Ny, d[1]) * d[1] / 2));
+      if (G.Ny % 2 == 0 && i == G.Ny / 2) v = std::complex<double>(v.real(), 0.0);
+      sh[i] = cf{(float)(v.real() / (double)n), (float)(v.imag() / (double)n)};
+    }
+    P.ushift[1] = S->upload(sh);
   }
 
   // device copy of the descriptor (needed before any pass runs)
@@ -864,6 +929,65 @@ static void transition_gradient(Sim& S) {
   c = Sim::DtChange{};
 }
 
+// The u_r / r term of an axisymmetric divergence (kspaceFirstOrderAS.m:
+// 954-969 with 'WSWA-FFT'): the radial divergence is
+// shift_y^- κ (i ky fft2(u_y) + fft2(ir .* u_y)) on the expanded grid, ir the
+// signed inverse radius of each copy.  The second part goes through the
+// spare spectral buffer 2: ir .* u_y, its x transform, a filter pass with
+// κ / Ny and the y shift pass (which carries the 1/(Nx Ny) of the round
+// trip), which leaves it scaled as the divergence pass leaves its output;
+// it is added to the y component after that pass (divergence_radial), so
+// the density update, the nonlinear coefficient and the equation of state
+// see the whole divergence.
+static void radial_term(Sim& S) {
+  ProbDev& P = S.P;
+  if (!S.scratch_grid) S.scratch_grid = S.alloc<float>((size_t)S.G.ncell);
+  EwArgs m{};
+  m.kind = EW_AXIS_MUL;
+  m.which = 1;
+  m.p0 = P.u[1];
+  m.ptr = S.scratch_grid;
+  m.prof = S.as_ir;
+  S.ew(m, S.lcell.nb);
+  PassArgs f = bufs(1, 2);
+  f.real[0] = S.scratch_grid;
+  S.pass(PK_X_R2C, f);
+  PassArgs fl = f;
+  fl.mode = SM_FILTER;
+  fl.table = S.as_kappa;
+  S.pass(PK_SPEC, fl);
+  PassArgs sh = f;
+  sh.mode = SM_SHIFT;
+  sh.flags = 1;
+  S.pass(PK_SPEC, sh);
+}
+
+// Nonlinear axisymmetric steps: the nonlinear mass-conservation coefficient
+// and equation of state are even in the density while the mirrored copies
+// hold -rho, so copies 2-4 of rho_x, rho_y and p are rewritten from copy 1
+// after the update (the reference mirrors its radial grid every step), and
+// the spectrum of p that the next step's gradient starts from is formed
+// again.  The velocities stay symmetric: their update is linear in p.
+static void mirror_copies(Sim& S) {
+  float* f[3] = {S.P.rho[0], S.P.rho[1], S.P.p};
+  for (float* q : f) {
+    EwArgs e{};
+    e.kind = EW_AS_MIRROR;
+    e.ptr = q;
+    S.ew(e, S.lcell.nb);
+  }
+  r2c_p(S);
+}
+
+static void divergence_radial(Sim& S) {
+  EwArgs e{};
+  e.kind = EW_ADD;
+  e.n = 2 * S.G.nspec;
+  e.ptr = (float*)S.P.S[1];
+  e.p0 = (const float*)S.P.S[2];
+  S.ew(e, (int)((e.n + EW_T - 1) / EW_T));
+}
+
 static void record_step(Sim& S, int t, bool init) {
   if (S.time_rev || S.nsens == 0 || t < S.record_start) return;
   uint32_t pts = S.record & (kPointP | kPointU);
@@ -917,7 +1041,9 @@ void step_once(Sim& S) {
     const bool need_p = (S.nsens > 0 && !S.full_mask && !S.time_rev && t >= S.record_start &&
                          (S.record & kPointP)) ||
                         (last && (S.record & KW_REC_P_FINAL));
-    const int pflag = need_p ? X_STORE_P : 0;
+    // a nonlinear axisymmetric step rewrites the mirrored copies of p from
+    // the stored field
+    const int pflag = (need_p || (S.axisym && S.P.nonlinear)) ? X_STORE_P : 0;
     if (S.p_grid) src_grid(S, 0, t);
     for (int a = 0; a < dim; ++a)
       if (S.u_grid[a]) src_grid(S, a + 1, t);
@@ -925,11 +1051,13 @@ void step_once(Sim& S) {
       transition_gradient(S);
     else
       gradient(S, 0);
+    if (S.axisym) radial_term(S);
     if (dim == 3) S.pass(PK_YFWD, bufs(3));
     PassArgs dv{};
     dv.mode = SM_DIV;
     S.pass(PK_SPEC, dv);
     if (dim == 3) S.pass(PK_YINV, bufs(3));
+    if (S.axisym) divergence_radial(S);
     PassArgs xd{};
     xd.flags = pflag;
     S.pass(PK_X_DIV, xd);
@@ -943,6 +1071,7 @@ void step_once(Sim& S) {
       xe.flags = pflag;
       S.pass(PK_X_EOS, xe);
     }
+    if (S.axisym && S.P.nonlinear) mirror_copies(S);
     record_step(S, t, false);
   }
   S.steps_done++;

@@ -242,6 +242,11 @@ struct Sim {
     float* pml_sg_new[3] = {nullptr, nullptr, nullptr};
   } dtc;
   float* scratch_grid = nullptr;  // one real grid (the split correction)
+  // axisymmetric (WSWA-FFT) problems: the signed inverse radius along the
+  // expanded y axis and κ / Ny for the u_r / r term
+  bool axisym = false;
+  float* as_ir = nullptr;
+  float* as_kappa = nullptr;
   // per-kernel event timing
   bool timing = false;
   std::vector<std::pair<void*, void*>> ev_used;

@@ -89,6 +89,7 @@ class Problem(C.Structure):
         ("directivity_angle", c_double_p), ("directivity_pattern", C.c_int32),
         ("directivity_size", C.c_double), ("alpha_filter", c_double_p),
         ("sensor_mask_corners", c_u64_p), ("sensor_num_cuboids", C.c_int64),
+        ("axisymmetric", C.c_int32),
     ]
 
 

@@ -43,7 +43,7 @@ _VISUAL_OPTIONS = {"PlotSim", "PlotLayout", "PlotPML", "PlotScale", "PlotFreq", 
 _OPTIONS = {"PMLSize", "PMLAlpha", "PMLInside", "PMLRange", "MultiAxialPMLRatio", "UseFD",
             "Smooth", "DataCast", "DataRecast",
             "CartInterp", "UsekSpace", "UseSG", "ScaleSourceTerms", "SaveToDisk",
-            "SaveToDiskExit", "HDFCompressionLevel"} | _VISUAL_OPTIONS
+            "SaveToDiskExit", "HDFCompressionLevel", "RadialSymmetry"} | _VISUAL_OPTIONS
 _UNSUPPORTED = {"StreamToDisk"}
 # kspaceFirstOrder_setDefaults.m:196
 SAVE_TO_DISK_FILENAME_DEF = "kwave_input_data.h5"
@@ -1166,6 +1166,8 @@ def save_to_disk(pp: PreparedProblem, filename: str):
 
 def _solve(dim, kgrid, medium, source, sensor, args, kwargs):
     opts = _parse_options(args, kwargs)
+    if "RadialSymmetry" in opts:
+        raise ValueError("Unknown optional input 'RadialSymmetry'.")
     _check_use_fd(dim, opts)
     save = _save_to_disk_options(dim, opts)
     pp = prepare(kgrid, medium, source, sensor, dim, opts)
@@ -1211,3 +1213,143 @@ def kspaceFirstOrder2D(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
 def kspaceFirstOrder3D(kgrid, medium, source, sensor, *args, **kwargs):  # noqa: N802
     """3D time-domain simulation (reference +kwave/+legacy/kspaceFirstOrder3D.m)."""
     return _solve(3, kgrid, medium, source, sensor, args, kwargs)
+
+
+# ---------------------------------------------------------------------------
+# kspaceFirstOrderAS
+# ---------------------------------------------------------------------------
+
+def _as_mirror(f, kind, signed):
+    """The layout of a radial-grid field on the expanded grid of the
+    'WSWA-FFT' symmetry (kspaceFirstOrderAS.m:703-707, 944-952): p-like
+    [f, 0, -flip(f(2:end)), -f, 0, flip(f(2:end))], y-staggered [f, flip(f),
+    -f, -flip(f)]; unsigned (material grids) every copy keeps its sign and
+    the mirror columns of p-like grids repeat their neighbour."""
+    f = np.asarray(f, dtype=np.float64)
+    nx, ny = f.shape
+    e = np.zeros((nx, 4 * ny))
+    sg = (1, -1, -1, 1) if kind == "p" else (1, 1, -1, -1)
+    if not signed:
+        sg = (1, 1, 1, 1)
+    for b in range(4):
+        blk = e[:, b * ny:(b + 1) * ny]
+        if kind == "p":
+            if b % 2 == 0:
+                blk[:] = sg[b] * f
+            else:
+                blk[:, 1:] = sg[b] * f[:, :0:-1]
+                blk[:, 0] = 0.0 if signed else f[:, -1]
+        else:
+            blk[:] = sg[b] * (f if b % 2 == 0 else f[:, ::-1])
+    return e
+
+
+def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa: N802
+    """Axisymmetric time-domain simulation (reference
+    +kwave/+legacy/kspaceFirstOrderAS.m): a 2-D grid with x the axial and y
+    the radial coordinate, the axis at the first y index.  Runs the 2-D
+    kernels on the radially expanded grid of the 'WSWA-FFT' symmetry (the
+    default; 'WSWA' gives the same field, the reference's discrete
+    trigonometric transforms agreeing with it to machine precision) with the
+    u_r / r term of the divergence added by the library
+    (kw_problem.axisymmetric).  Supported here: source.p0, lossless media
+    (with or without B/A), binary sensor masks; 'WSWS' symmetries, Stokes
+    absorption, time-varying sources, time reversal and 'PMLInside' false
+    are refused."""
+    opts = _parse_options(args, kwargs)
+    sym = opts.pop("RadialSymmetry", "WSWA-FFT")
+    if sym not in ("WSWA", "WSWS", "WSWA-FFT", "WSWS-FFT"):
+        raise ValueError("Optional input 'RadialSymmetry' must be set to 'WSWA', 'WSWS', "
+                         "'WSWA-FFT', 'WSWS-FFT'.")
+    if sym in ("WSWS", "WSWS-FFT"):
+        raise NotImplementedError("'RadialSymmetry' 'WSWS' and 'WSWS-FFT' are not supported by kwhip")
+    if getattr(kgrid, "dim", 2) != 2:
+        raise ValueError("kspaceFirstOrderAS takes a 2D kWaveGrid (x axial, y radial).")
+    if not opts.get("UseSG", True):
+        raise ValueError("Optional input 'UseSG' is not supported for axisymmetric simulations.")
+    if not opts.get("PMLInside", True):
+        raise NotImplementedError("'PMLInside' false is not supported by kwhip for axisymmetric "
+                                  "simulations")
+    for k in ("SaveToDisk", "SaveToDiskExit"):
+        if opts.get(k):
+            raise NotImplementedError(f"'{k}' is not supported by kwhip for axisymmetric simulations")
+    med = dict(medium) if isinstance(medium, dict) else dict(vars(medium))
+    src = dict(source) if isinstance(source, dict) else dict(vars(source))
+    sen = None if sensor is None else (dict(sensor) if isinstance(sensor, dict) else dict(vars(sensor)))
+    for k in ("alpha_mode", "alpha_filter"):
+        if k in med and not (k == "alpha_mode" and med[k] == "stokes"):
+            raise ValueError(f"Input option medium.{k} is not supported with the axisymmetric code "
+                             "or medium.alpha_mode = 'stokes'.")
+    if "alpha_coeff" in med:
+        # kspaceFirstOrder_inputChecking.m:173-190: the axisymmetric code is
+        # Stokes absorption, alpha_power = 2
+        if "alpha_power" in med and float(np.asarray(med["alpha_power"])) != 2:
+            print("  WARNING: the axisymmetric code and stokes absorption assume alpha_power = 2, "
+                  "user value ignored.")
+        med["alpha_mode"] = "stokes"
+        med["alpha_power"] = 2.0
+    if set(src) - {"p0"}:
+        raise NotImplementedError("axisymmetric simulations in kwhip take source.p0 only")
+    if "p0" not in src:
+        raise ValueError("source.p0 must be given for the axisymmetric simulations kwhip runs")
+    if sen is not None:
+        for k in sen:
+            if k not in ("mask", "frequency_response", "record_mode", "record",
+                         "record_start_index", "time_reversal_boundary_data"):
+                raise ValueError(f"'{k}' is not a valid field for sensor")
+        if "time_reversal_boundary_data" in sen:
+            raise NotImplementedError("axisymmetric time reversal is not supported by kwhip")
+        m = np.asarray(sen.get("mask"))
+        if m.shape != (kgrid.Nx, kgrid.Ny):
+            raise NotImplementedError("axisymmetric simulations in kwhip take a binary sensor mask")
+    Nx, Ny = kgrid.Nx, kgrid.Ny
+    # p0: smoothed on the expanded grid (kspaceFirstOrder_inputChecking.m:
+    # 1504-1535), trimmed back, then laid out again
+    p0 = np.asarray(src["p0"], dtype=np.float64)
+    sm = opts.pop("Smooth", True)
+    sm_p0 = bool(np.atleast_1d(sm)[0])
+    if sm_p0:
+        p0 = smooth(_as_mirror(p0, "p", True), True)[:, :Ny]
+    if np.size(np.atleast_1d(sm)) > 1 and any(np.atleast_1d(sm)[1:]):
+        raise NotImplementedError("smoothing c0 or rho0 is not supported by kwhip for "
+                                  "axisymmetric simulations")
+    kg = kWaveGrid(Nx, kgrid.dx, 4 * Ny, kgrid.dy)
+    kg.setTime(kgrid.Nt, kgrid.dt)
+    emed = {}
+    for k, v in med.items():
+        if k in ("sound_speed", "density", "BonA", "alpha_coeff") and np.size(v) > 1:
+            emed[k] = _as_mirror(v, "p", False)
+        else:
+            emed[k] = v
+    if "sound_speed_ref" not in emed:
+        emed["sound_speed_ref"] = float(np.max(np.asarray(med["sound_speed"])))
+    esen = None
+    if sen is not None:
+        esen = dict(sen)
+        mask = np.zeros((Nx, 4 * Ny), dtype=bool)
+        mask[:, :Ny] = np.asarray(sen["mask"]).astype(bool)
+        esen["mask"] = mask
+    pml = opts.get("PMLSize", 20)
+    pml = [int(v) for v in np.broadcast_to(np.atleast_1d(pml), (2,))]
+    if pml[1] > Ny:
+        raise ValueError("The size of the PML must be smaller than the size of the grid.")
+    opts["PMLSize"] = pml
+    opts["Smooth"] = False
+    pp = prepare(kg, emed, {"p0": _as_mirror(p0, "p", True)}, esen, 2, opts)
+    pp.prob.axisymmetric = 1
+    rho = np.asarray(med["density"], dtype=np.float64)
+    if rho.size > 1:
+        # kspaceFirstOrderAS.m:440-456 on the radial grid, then laid out
+        rsx = rho.copy()
+        rsx[:-1, :] = (rho[:-1, :] + rho[1:, :]) / 2
+        rsy = rho.copy()
+        rsy[:, :-1] = (rho[:, :-1] + rho[:, 1:]) / 2
+        ex = np.asfortranarray(_as_mirror(rsx, "p", False)).reshape(-1, order="F")
+        ey = np.asfortranarray(_as_mirror(rsy, "u", False)).reshape(-1, order="F")
+        ex, ey = np.ascontiguousarray(ex), np.ascontiguousarray(ey)
+        pp.keep += [ex, ey]
+        pp.prob.rho0_sg[0] = _abi.dptr(ex)
+        pp.prob.rho0_sg[1] = _abi.dptr(ey)
+    # grid outputs (_all, _final) come back on the radial grid
+    pp.info.inside = (slice(None), slice(0, Ny))
+    return run_prepared(pp)

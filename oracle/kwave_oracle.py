@@ -1393,3 +1393,180 @@ def rel_linf(a, b):
     b = np.asarray(b, dtype=np.float64)
     den = np.max(np.abs(b))
     return float(np.max(np.abs(a - b)) / (den if den > 0 else 1.0))
+
+
+# ---------------------------------------------------------------------------
+# Axisymmetric solver (kspaceFirstOrderAS.m), 'WSWA-FFT' radial symmetry
+# ---------------------------------------------------------------------------
+
+def as_mirror_p(f):
+    """kspaceFirstOrderAS.m:703-707: a p-like field of the radial grid on the
+    expanded grid, [f, 0, -flip(f(2:end)), -f, 0, flip(f(2:end))] along y."""
+    nx, ny = f.shape
+    e = np.zeros((nx, 4 * ny), dtype=f.dtype)
+    e[:, 0:ny] = f
+    e[:, ny + 1:2 * ny] = -f[:, :0:-1]
+    e[:, 2 * ny:3 * ny] = -f
+    e[:, 3 * ny + 1:4 * ny] = f[:, :0:-1]
+    return e
+
+
+def as_mirror_u(f, signs=(1, 1, -1, -1)):
+    """kspaceFirstOrderAS.m:944-948: a y-staggered field, [f, flip(f), -f,
+    -flip(f)] (signs per copy; the u_r / r field takes (1, -1, -1, 1))."""
+    nx, ny = f.shape
+    e = np.zeros((nx, 4 * ny), dtype=f.dtype)
+    for b, sg in enumerate(signs):
+        e[:, b * ny:(b + 1) * ny] = sg * (f if b % 2 == 0 else f[:, ::-1])
+    return e
+
+
+def kspace_first_order_as(kgrid: KGrid, medium, source, sensor, *, data_cast="off",
+                          pml_size=20, pml_alpha=2.0, smooth=True):
+    """kspaceFirstOrderAS.m:355-1435 with 'RadialSymmetry' 'WSWA-FFT' for the
+    cases the library runs: an initial pressure, lossless or Stokes media
+    (linear or with B/A; the axisymmetric code takes medium.alpha_coeff as
+    Stokes absorption with alpha_power 2, kspaceFirstOrder_inputChecking.m:
+    173-190, kspaceFirstOrderAS.m:1134-1163), a binary sensor mask recording
+    'p', 'p_max', 'p_min',
+    'p_rms', 'p_final', 'u' and 'u_final' (x axial, y radial from the
+    axis).  Returns the sensor data dict (p as [points x Nt])."""
+    rdt = np.float32 if data_cast == "single" else np.float64
+    cdt = np.complex64 if data_cast == "single" else np.complex128
+    Nx, Ny = kgrid.Nx, kgrid.Ny
+    dx, dy, dt, Nt = kgrid.dx, kgrid.dy, kgrid.dt, kgrid.Nt
+    px, py = (pml_size, pml_size) if np.ndim(pml_size) == 0 else pml_size
+    ax, ay = (pml_alpha, pml_alpha) if np.ndim(pml_alpha) == 0 else pml_alpha
+    c0 = np.asarray(_get(medium, "sound_speed"), dtype=np.float64)
+    rho0 = np.asarray(_get(medium, "density"), dtype=np.float64)
+    bona = _get(medium, "BonA")
+    nonlinear = bona is not None
+    c_ref = float(_get(medium, "sound_speed_ref", np.max(c0)))
+    stokes = _has(medium, "alpha_coeff")
+    if stokes:
+        tau = (-2.0 * db2neper(_get(medium, "alpha_coeff"), 2) * c0)
+    p0 = np.asarray(source["p0"], dtype=np.float64)
+    if smooth:
+        # kspaceFirstOrder_inputChecking.m:1504-1535: smoothed on the
+        # expanded grid (WS origin symmetry), trimmed back
+        p0 = smooth_mat(as_mirror_p(p0), restore_max=True)[:, :Ny]
+    # operators (kspaceFirstOrderAS.m:463-565)
+    pml_x = get_pml(Nx, dx, dt, c_ref, px, ax, False)
+    pml_x_sg = get_pml(Nx, dx, dt, c_ref, px, ax, True)
+    pml_y = get_pml(Ny, dy, dt, c_ref, py, ay, False)
+    pml_y_sg = get_pml(Ny, dy, dt, c_ref, py, ay, True)
+    pml_y[:py] = 1.0  # getPML.m: axisymmetric, no PML at the axis
+    pml_y_sg[:py] = 1.0
+    kx = make_dim(Nx, dx)
+    ky = make_dim(4 * Ny, dy)
+    k = np.sqrt(kx[:, None] ** 2 + ky[None, :] ** 2)
+    kappa = np.fft.ifftshift(sinc(c_ref * k * dt / 2)).astype(rdt)
+    ddx_pos = np.fft.ifftshift(1j * kx * np.exp(1j * kx * dx / 2))[:, None].astype(cdt)
+    ddx_neg = np.fft.ifftshift(1j * kx * np.exp(-1j * kx * dx / 2))[:, None].astype(cdt)
+    ddy_k = np.fft.ifftshift(1j * ky)[None, :].astype(cdt)
+    y_pos = np.fft.ifftshift(np.exp(1j * ky * dy / 2))[None, :].astype(cdt)
+    y_neg = np.fft.ifftshift(np.exp(-1j * ky * dy / 2))[None, :].astype(cdt)
+    y_sg = (np.arange(Ny) * dy + dy / 2)[None, :]
+    if rho0.ndim == 2:
+        # interpn '*linear' at x + dx/2 and y + dy/2, NaN past the last point
+        # replaced by rho0 (kspaceFirstOrderAS.m:440-456)
+        rsx = rho0.copy()
+        rsx[:-1, :] = (rho0[:-1, :] + rho0[1:, :]) / 2
+        rsy = rho0.copy()
+        rsy[:, :-1] = (rho0[:, :-1] + rho0[:, 1:]) / 2
+    else:
+        rsx = rsy = rho0
+    rsx_inv = (1.0 / rsx).astype(rdt)
+    rsy_inv = (1.0 / rsy).astype(rdt)
+    c2 = (c0 ** 2).astype(rdt)
+    rho0c = np.asarray(rho0, dtype=rdt)
+    ba = np.asarray(bona, dtype=rdt) if nonlinear else None
+    tau_c = np.asarray(tau, dtype=rdt) if stokes else None
+    pml_x, pml_x_sg = pml_x.astype(rdt)[:, None], pml_x_sg.astype(rdt)[:, None]
+    pml_y, pml_y_sg = pml_y.astype(rdt)[None, :], pml_y_sg.astype(rdt)[None, :]
+    inv_y = (1.0 / y_sg).astype(rdt)
+    dtc = rdt(dt)
+
+    mask = np.asarray(sensor["mask"], dtype=bool)
+    idx = np.flatnonzero(mask.reshape(-1, order="F"))
+    record = list(sensor.get("record", ["p"]))
+    start = int(sensor.get("record_start_index", 1))
+    out = {}
+    nrec = Nt - start + 1
+    if "p" in record:
+        out["p"] = np.zeros((idx.size, nrec))
+    if "u" in record:
+        out["ux"] = np.zeros((idx.size, nrec))
+        out["uy"] = np.zeros((idx.size, nrec))
+    pmax = np.full(idx.size, -np.inf)
+    pmin = np.full(idx.size, np.inf)
+    prms = np.zeros(idx.size)
+
+    fft2 = lambda a: sfft.fft2(a, workers=FFT_WORKERS)  # noqa: E731
+    ifft2 = lambda a: np.real(sfft.ifft2(a, workers=FFT_WORKERS)).astype(rdt)  # noqa: E731
+    p = np.zeros((Nx, Ny), dtype=rdt)
+    ux = np.zeros_like(p)
+    uy = np.zeros_like(p)
+    rx = np.zeros_like(p)
+    ry = np.zeros_like(p)
+    p0c = p0.astype(rdt)
+
+    def grad(pp):
+        pk = kappa * fft2(as_mirror_p(pp))
+        return ifft2(ddx_pos * pk)[:, :Ny], ifft2(ddy_k * y_pos * pk)[:, :Ny]
+
+    def pick(f):
+        return np.reshape(f, -1, order="F")[idx].astype(np.float64)
+
+    for t in range(1, Nt + 1):
+        dpdx, dpdy = grad(p)
+        ux = pml_x_sg * (pml_x_sg * ux - dtc * rsx_inv * dpdx)
+        uy = pml_y_sg * (pml_y_sg * uy - dtc * rsy_inv * dpdy)
+        duxdx = ifft2(ddx_neg * kappa * fft2(as_mirror_p(ux)))[:, :Ny]
+        u_on_y = as_mirror_u(inv_y * uy, (1, -1, -1, 1))
+        duydy = ifft2(kappa * y_neg * (ddy_k * fft2(as_mirror_u(uy)) + fft2(u_on_y)))[:, :Ny]
+        if nonlinear:
+            k_ = 2 * (rx + ry) + rho0c
+        else:
+            k_ = rho0c
+        rx = pml_x * (pml_x * rx - dtc * k_ * duxdx)
+        ry = pml_y * (pml_y * ry - dtc * k_ * duydy)
+        st = tau_c * rho0c * (duxdx + duydy) if stokes else 0.0
+        if nonlinear:
+            p = c2 * (rx + ry + st + ba * (rx + ry) ** 2 / (2 * rho0c))
+        else:
+            p = c2 * (rx + ry + st)
+        if t == 1:
+            p = p0c.copy()
+            rx = (p0c / (2 * c2)).astype(rdt)
+            ry = rx.copy()
+            dpdx, dpdy = grad(p)
+            ux = dtc * rsx_inv * dpdx / 2
+            uy = dtc * rsy_inv * dpdy / 2
+        if t >= start:
+            j = t - start
+            pv = pick(p)
+            if "p" in out:
+                out["p"][:, j] = pv
+            if "ux" in out:
+                out["ux"][:, j] = pick(ux)
+                out["uy"][:, j] = pick(uy)
+            pmax = np.maximum(pmax, pv)
+            pmin = np.minimum(pmin, pv)
+            prms += pv ** 2
+    res = {}
+    if "p" in record:
+        res["p"] = out["p"]
+    if "p_max" in record:
+        res["p_max"] = pmax
+    if "p_min" in record:
+        res["p_min"] = pmin
+    if "p_rms" in record:
+        res["p_rms"] = np.sqrt(prms / nrec)
+    if "p_final" in record:
+        res["p_final"] = p.astype(np.float64)
+    if "u" in record:
+        res["ux"], res["uy"] = out["ux"], out["uy"]
+    if "u_final" in record:
+        res["ux_final"], res["uy_final"] = ux.astype(np.float64), uy.astype(np.float64)
+    return res

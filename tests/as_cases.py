@@ -1,0 +1,85 @@
+"""Cases of kspaceFirstOrderAS (the axisymmetric solver), shared by the
+emulator and GPU tests: x axial, y radial from the axis."""
+
+import numpy as np
+
+
+def case(name, Nx=32, Ny=16, seed=2):
+    rng = np.random.default_rng(seed)
+    dx = 1e-4
+    x = (np.arange(Nx) - Nx // 2) * dx
+    r = np.arange(Ny) * dx
+    X, R = np.meshgrid(x, r, indexing="ij")
+    c = 1500 + 80 * rng.random((Nx, Ny))
+    rho = 1000 + 90 * rng.random((Nx, Ny))
+    cs = {"Nx": Nx, "Ny": Ny, "dx": dx, "Nt": 24,
+          "medium": {"sound_speed": c, "density": rho},
+          "p0": np.exp(-(X ** 2 + R ** 2) / (3 * dx) ** 2),
+          "sensor": {"mask": rng.random((Nx, Ny)) < 0.3,
+                     "record": ["p", "p_max", "p_rms", "p_final", "u", "u_final"]},
+          "pml": [6, 4], "smooth": False}
+    cs["dt"] = 0.25 * dx / c.max()
+    if name == "homogeneous":
+        cs["medium"] = {"sound_speed": 1500.0, "density": 1000.0}
+        cs["dt"] = 0.25 * dx / 1500.0
+    elif name == "nonlinear":
+        cs["medium"]["BonA"] = 6 + rng.random((Nx, Ny))
+        cs["p0"] = cs["p0"] * 1e6
+    elif name == "stokes":
+        cs["medium"]["alpha_coeff"] = 0.5 + rng.random((Nx, Ny))
+    elif name == "stokes_nonlinear":
+        cs["medium"]["alpha_coeff"] = 0.75
+        cs["medium"]["BonA"] = 5.0
+        cs["p0"] = cs["p0"] * 1e6
+    elif name == "smoothed":
+        cs["smooth"] = True
+        cs["p0"] = (np.sqrt(X ** 2 + R ** 2) < 4 * dx).astype(float)
+    elif name != "hetero":
+        raise KeyError(name)
+    return cs
+
+
+NAMES = ["homogeneous", "hetero", "nonlinear", "stokes", "stokes_nonlinear", "smoothed"]
+
+
+def run_kwhip(cs):
+    import kwhip
+    kg = kwhip.kWaveGrid(cs["Nx"], cs["dx"], cs["Ny"], cs["dx"])
+    kg.setTime(cs["Nt"], cs["dt"])
+    return kwhip.kspaceFirstOrderAS(kg, dict(cs["medium"]), {"p0": cs["p0"]}, dict(cs["sensor"]),
+                                    PMLSize=cs["pml"], Smooth=cs["smooth"])
+
+
+def run_oracle(cs, data_cast):
+    from oracle import kwave_oracle as ko
+    kg = ko.KGrid(cs["Nx"], cs["dx"], cs["Ny"], cs["dx"])
+    kg.set_time(cs["Nt"], cs["dt"])
+    return ko.kspace_first_order_as(kg, cs["medium"], {"p0": cs["p0"]}, cs["sensor"],
+                                    data_cast=data_cast, pml_size=cs["pml"], smooth=cs["smooth"])
+
+
+def compare(got, ref):
+    errs = {}
+    for k, r in ref.items():
+        g = np.asarray(got[k], dtype=np.float64)
+        r = np.asarray(r, dtype=np.float64)
+        if g.shape != r.shape:
+            g = g.reshape(r.shape, order="F")
+        errs[k] = float(np.max(np.abs(g - r)) / np.max(np.abs(r)))
+    return errs
+
+
+def spherical_wave(Nx, Ny, dx, c0, Nt, dt, width):
+    """p0 = exp(-R^2 / w^2) centred on the axis, zero velocity: the exact
+    pressure [(R - ct) g(R - ct) + (R + ct) g(R + ct)] / (2R) at t = (Nt - 1)
+    dt (the first step is the initial condition)."""
+    x = (np.arange(Nx) - Nx // 2) * dx
+    r = np.arange(Ny) * dx
+    X, Rr = np.meshgrid(x, r, indexing="ij")
+    R = np.sqrt(X ** 2 + Rr ** 2)
+    g = lambda s: np.exp(-s ** 2 / width ** 2)  # noqa: E731
+    T = (Nt - 1) * dt
+    with np.errstate(invalid="ignore", divide="ignore"):
+        ex = ((R - c0 * T) * g(R - c0 * T) + (R + c0 * T) * g(R + c0 * T)) / (2 * R)
+    ex[R == 0] = (1 - 2 * (c0 * T) ** 2 / width ** 2) * g(c0 * T)
+    return g(R), ex, (np.abs(X) < (Nx / 2 - 20) * dx) & (Rr < (Ny - 20) * dx)

@@ -1,0 +1,62 @@
+"""kspaceFirstOrderAS on the GPU: every case of tests/as_cases.py on a
+128 x 64 grid against the oracle (relative L-inf <= 1e-5 single, <= 1e-4
+double), the spherical wave of a pulse centred on the axis against its
+closed form (the scheme's discretisation error, as the oracle measures it),
+and the engine's refusal of axisymmetric problems."""
+
+import numpy as np
+import pytest
+
+import as_cases as ac
+
+pytestmark = pytest.mark.gpu
+
+TOL32, TOL64 = 1e-5, 1e-4
+
+
+@pytest.mark.parametrize("name", ac.NAMES)
+def test_gpu_as_matches_oracle(hip_lib, name):
+    cs = ac.case(name, Nx=128, Ny=64)
+    cs["Nt"] = 120
+    cs["pml"] = [12, 10]
+    got = ac.run_kwhip(cs)
+    e32 = ac.compare(got, ac.run_oracle(cs, "single"))
+    e64 = ac.compare(got, ac.run_oracle(cs, "off"))
+    print(f"[as] {name}: worst vs fp32 {max(e32.values()):.2e}, vs fp64 {max(e64.values()):.2e}",
+          flush=True)
+    for k in e32:
+        assert e32[k] <= TOL32, (k, e32[k])
+        assert e64[k] <= TOL64, (k, e64[k])
+
+
+def test_gpu_as_spherical_wave(hip_lib):
+    import kwhip
+    Nx, Ny, dx, c0, Nt = 128, 64, 1e-4, 1500.0, 60
+    dt = 0.2 * dx / c0
+    p0, exact, inside = ac.spherical_wave(Nx, Ny, dx, c0, Nt, dt, 6 * dx)
+    kg = kwhip.kWaveGrid(Nx, dx, Ny, dx)
+    kg.setTime(Nt, dt)
+    out = kwhip.kspaceFirstOrderAS(kg, {"sound_speed": c0, "density": 1000.0}, {"p0": p0},
+                                   {"mask": np.ones((Nx, Ny), bool), "record": ["p_final"]},
+                                   PMLSize=16, Smooth=False)
+    pf = np.asarray(out["p_final"]).reshape(Nx, Ny, order="F")
+    err = float(np.max(np.abs(pf - exact)[inside]) / np.max(np.abs(exact)))
+    print(f"[as] spherical wave, w = 6 points: rel L-inf {err:.2e}", flush=True)
+    assert err < 6e-4
+
+
+def test_gpu_engine_refuses_axisymmetric(hip_lib):
+    import kwhip
+    from kwhip.solver import prepare, _parse_options
+    cs = ac.case("homogeneous")
+    kg = kwhip.kWaveGrid(cs["Nx"], cs["dx"], 4 * cs["Ny"], cs["dx"])
+    kg.setTime(4, cs["dt"])
+    with kwhip.Engine() as eng:
+        pp = prepare(kg, cs["medium"], {"p0": np.ones((cs["Nx"], 4 * cs["Ny"]))}, None, 2,
+                     _parse_options((), {"Smooth": False, "PMLSize": 4}))
+        pp.prob.axisymmetric = 1
+        import ctypes as C
+        from kwhip import _abi
+        tid = C.c_int64()
+        with pytest.raises(kwhip.KWError, match="axisymmetric"):
+            _abi.call(eng.lib.kw_submit, eng.h, C.byref(pp.prob), C.byref(tid))
