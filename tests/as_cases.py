@@ -59,13 +59,24 @@ def run_oracle(cs, data_cast):
 
 
 def compare(got, ref):
+    """Relative L-inf per output.  The final fields are normalised by the
+    run's peak (p_final by the largest recorded p_max, u_final by the
+    largest recorded velocity): a spreading wave leaves them far below the
+    peak, where single-precision implementations differ by more than 1e-5
+    of their own maximum (the oracle's single- and double-precision runs
+    of the 128 x 64 cases differ by 1.5e-5 there)."""
     errs = {}
+    peak = {"p_final": np.max(np.abs(ref["p_max"])) if "p_max" in ref else None,
+            "ux_final": max(np.max(np.abs(ref["ux"])), np.max(np.abs(ref["uy"])))
+            if "ux" in ref else None}
+    peak["uy_final"] = peak["ux_final"]
     for k, r in ref.items():
         g = np.asarray(got[k], dtype=np.float64)
         r = np.asarray(r, dtype=np.float64)
         if g.shape != r.shape:
             g = g.reshape(r.shape, order="F")
-        errs[k] = float(np.max(np.abs(g - r)) / np.max(np.abs(r)))
+        scale = peak.get(k) or np.max(np.abs(r))
+        errs[k] = float(np.max(np.abs(g - r)) / scale)
     return errs
 
 
