@@ -388,11 +388,17 @@ struct Group {
 
   Batch batch(int k) const { return Batch{probs, tiles[k], (int32_t)sims.size()}; }
 
-  void pass(PassKind kind, const PassArgs& a) {
+  void pass(PassKind kind, const PassArgs& a0) {
     const int k = (kind == PK_YFWD || kind == PK_YINV) ? GY : (kind == PK_SPEC ? GL : GX);
     const int fx = fixed[k == GX ? 0 : (k == GY ? 1 : 2)];
-    launch_pass(kind, fx, dim, geo[k].nb, geo[k].T, launch_lds(kind, fx, a, geo[k].lds), batch(k),
-                a, stream);
+    PassArgs a = a0;
+    int nb = geo[k].nb;
+    if (k == GY && a.nbuf > 1 && ysplit_on()) {
+      a.ysplit = nb;
+      nb *= a.nbuf;
+    }
+    launch_pass(kind, fx, dim, nb, geo[k].T, launch_lds(kind, fx, a, geo[k].lds), batch(k), a,
+                stream);
     ++launches;
   }
 

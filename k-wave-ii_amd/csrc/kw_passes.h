@@ -249,9 +249,18 @@ KW_BODY void advance_step(const ProbDev& P, const PassArgs& A, int tile, int tid
 
 template <bool INV, class YS, class Ctx>
 KW_BODY void body_ypass(Ctx& c, const Batch& B, const PassArgs& A) {
-  int tile;
-  const ProbDev& P = batch_prob(c, B, tile);
-  advance_step(P, A, tile, c.tid());
+  // the launch's tiles, once per buffer when the buffers are split over
+  // blocks (ysplit), else every block loops over the buffers
+  int vb = c.bid(), b0 = 0, b1 = A.nbuf;
+  if (A.ysplit > 0) {
+    b0 = vb / A.ysplit;
+    b1 = b0 + 1;
+    vb -= b0 * A.ysplit;
+  }
+  const int pi = find_prob<Ctx>(B, vb);
+  const int tile = vb - KW_LD(B.tile_start + pi);
+  const ProbDev& P = B.probs[pi];
+  if (b0 == 0) advance_step(P, A, tile, c.tid());
   const int N = P.Ny, G = P.Gy, Pn = P.Py;
   // LDS: the L twiddles, then the pencils (L elements each: the length the
   // stages run at)
@@ -288,7 +297,7 @@ KW_BODY void body_ypass(Ctx& c, const Batch& B, const PassArgs& A) {
     stride = 1;
     lds = LdsPencils{lp + (int64_t)p * (L + 1), 0, 1, false};
   }
-  for (int b = 0; b < A.nbuf; ++b) {
+  for (int b = b0; b < b1; ++b) {
     cf* S = P.S[A.buf[b]];
     auto ld = [&](int, int n) -> cf { return KW_LD32(S, base + (uint32_t)n * stride); };
     auto st = [&](int, int n, cf v, int) { KW_ST32(S, base + (uint32_t)n * stride, v); };

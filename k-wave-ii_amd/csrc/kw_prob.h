@@ -236,6 +236,9 @@ struct PassArgs {
   // its last-axis pass in 2D), which reads no step number: one thread per
   // problem advances t_index to the step being computed
   int32_t advance;
+  // y passes over several buffers: when > 0, the launch has nbuf times this
+  // many blocks and block k transforms buffer k / ysplit only
+  int32_t ysplit;
 };
 
 enum : int32_t {

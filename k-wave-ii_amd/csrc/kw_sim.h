@@ -166,6 +166,16 @@ inline size_t launch_lds(PassKind k, int fixed, const PassArgs& a, size_t full) 
              : full;
 }
 
+// strided y passes over several buffers give each buffer blocks of its own
+// (KW_YSPLIT=0: one block loops over the buffers)
+inline bool ysplit_on() {
+  static const bool on = [] {
+    const char* e = std::getenv("KW_YSPLIT");
+    return !(e && std::strcmp(e, "0") == 0);
+  }();
+  return on;
+}
+
 inline int max_pencils_env() {
   const char* e = std::getenv("KW_MAX_PENCILS");
   if (!e) return 1 << 30;
@@ -344,16 +354,20 @@ struct Sim {
     return fixed_variant(N, k);
   }
 
-  void pass(PassKind k, const PassArgs& a) {
+  void pass(PassKind k, const PassArgs& a0) {
     const LaunchGeom* g = &lx;
     if (k == PK_YFWD || k == PK_YINV) g = &ly;
     if (k == PK_SPEC) g = &ll;
     const Batch B = batch();
     const int fx = fixed_len(k);
+    PassArgs a = a0;
+    int nb = g->nb;
+    if ((k == PK_YFWD || k == PK_YINV) && a.nbuf > 1 && ysplit_on()) {
+      a.ysplit = nb;
+      nb *= a.nbuf;
+    }
     timed((int)k, timing ? pass_bytes(k, a) : 0.0,
-          [&]() {
-            launch_pass(k, fx, G.dim, g->nb, g->T, launch_lds(k, fx, a, g->lds), B, a, stream);
-          });
+          [&]() { launch_pass(k, fx, G.dim, nb, g->T, launch_lds(k, fx, a, g->lds), B, a, stream); });
   }
 
   void ew(const EwArgs& a, int nblocks) {
