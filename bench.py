@@ -386,6 +386,11 @@ def _served_engine(reqs, arrivals):
     return done_at
 
 
+def _served_steps(kg, sens):
+    """Time steps of a served request (a reconstruction runs Nt - 1)."""
+    return kg.Nt - 1 if "time_reversal_boundary_data" in sens else kg.Nt
+
+
 def _served_plain_service(reqs):
     """Service time of each request run alone through kspaceFirstOrder2D/3D
     (host buffers in and out), the plain per-simulation GPU path."""
@@ -406,7 +411,9 @@ def _served_summary(reqs, arrivals, done_at, service, rate, steps_lo, steps_hi, 
     import numpy as np
 
     n_req = len(reqs)
-    cell_steps = float(sum(kg.total_grid_points * kg.Nt for kg, *_ in reqs))
+    n_tr = sum(1 for r in reqs if "time_reversal_boundary_data" in r[3])
+    cell_steps = float(sum(kg.total_grid_points * _served_steps(kg, sens)
+                           for kg, med, src, sens, _ in reqs))
     lat = done_at - arrivals
     eng_tp = cell_steps / float(np.max(done_at)) / 1e6
     # the plain server runs the same requests one at a time in arrival order
@@ -417,7 +424,8 @@ def _served_summary(reqs, arrivals, done_at, service, rate, steps_lo, steps_hi, 
         finish[i] = tfree
     lat_plain = finish - arrivals
     plain_tp = cell_steps / float(np.max(finish)) / 1e6
-    nbytes = sum(_problem_bytes(kg, med, sens, kg.Nt) for kg, med, src, sens, _ in reqs)
+    nbytes = sum(_problem_bytes(kg, med, sens, _served_steps(kg, sens))
+                 for kg, med, src, sens, _ in reqs)
     pct = lambda a, q: round(float(np.percentile(a, q)), 3)  # noqa: E731
     log(f"[bench] served: {n_req} requests at {rate}/s: p50 {pct(lat, 50)} s p99 {pct(lat, 99)} s, "
         f"{eng_tp:.0f} Mcell-steps/s; one at a time: p50 {pct(lat_plain, 50)} s "
@@ -429,14 +437,16 @@ def _served_summary(reqs, arrivals, done_at, service, rate, steps_lo, steps_hi, 
             "speedup_vs_plain": round(eng_tp / plain_tp, 2),
             "roofline_frac": _roofline(nbytes, float(np.max(done_at))),
             "plain_roofline_frac": _roofline(nbytes, float(np.max(finish))),
-            "sample": f"{n_req} requests, Poisson at {rate}/s, {steps_lo}-{steps_hi} steps each "
+            "sample": f"{n_req} requests ({n_tr} time-reversal reconstructions), Poisson at "
+                      f"{rate}/s, {steps_lo}-{steps_hi} steps each "
                       f"(seed {seed}), {cell_steps / 1e9:.1f} G cell-steps"}
 
 
 def served_bench(n_req, rate, steps_lo, steps_hi, seed=7):
     """A Poisson request stream of forward simulations (2D 256^2 - 1024^2,
     3D 96^3 - 256^3, heterogeneous absorbing nonlinear medium, tone-burst
-    disc source, whole-grid p_max/p_rms): latency from arrival to result and
+    disc source, whole-grid p_max/p_rms) and time-reversal reconstructions
+    (the disc re-emitting delayed bursts): latency from arrival to result and
     throughput (cell-steps over the time to the last result) for the engine,
     against a plain server that runs the same requests one at a time in
     arrival order (its latencies follow from the measured service time of

@@ -230,8 +230,12 @@ def batch_tr_problem(seed, dim=None):
     return kg, medium, source, sensor, opts
 
 
-def served_problem(seed, steps, dim=None):
-    """(kgrid, medium, source, sensor, opts) of one served request."""
+def served_problem(seed, steps, dim=None, kind=None):
+    """(kgrid, medium, source, sensor, opts) of one served request: a forward
+    simulation of a tone burst from a disc (line) source recording the
+    whole-grid p_max and p_rms, or (three requests in ten) a time-reversal
+    reconstruction re-emitting bursts from that disc; kind = "forward" or
+    "reconstruction" fixes which."""
     rng = np.random.default_rng(seed)
     if dim is None:
         dim = int(rng.integers(2, 4))
@@ -258,7 +262,21 @@ def served_problem(seed, steps, dim=None):
         mask[pml + 1, N // 2 - r:N // 2 + r + 1] = 1
     else:
         mask[:, :, pml + 1] = disc_mask(N, N, N // 2 + 1, N // 2 + 1, r)
+    opts = {"PMLSize": pml, "PMLAlpha": 2.0, "Smooth": False}
+    tr = rng.random() < 0.3
+    if kind is not None:
+        tr = kind == "reconstruction"
+    if tr:
+        # a time-reversal reconstruction: the disc (line) re-emits recorded
+        # bursts, one per point with its own delay
+        n_pts = int(np.count_nonzero(mask))
+        delays = rng.integers(0, max(1, int(steps) // 3), size=n_pts)
+        data = np.zeros((n_pts, int(steps)))
+        for i, dl in enumerate(delays):
+            m = min(n_sig, int(steps) - int(dl))
+            data[i, dl:dl + m] = 1e5 * sig[:m] / 1e6
+        sensor = {"mask": mask.astype(bool), "time_reversal_boundary_data": data}
+        return kg, medium, {}, sensor, opts
     source = {"p": sig[None, :], "p_mask": mask, "p_mode": "additive"}
     sensor = {"mask": np.ones(shape, dtype=bool), "record": ["p_max", "p_rms"]}
-    opts = {"PMLSize": pml, "PMLAlpha": 2.0, "Smooth": False}
     return kg, medium, source, sensor, opts

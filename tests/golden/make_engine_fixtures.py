@@ -85,7 +85,7 @@ def run_tr(seed):
 
 def served(seed, steps):
     from kwhip import workloads
-    return workloads.served_problem(seed, steps)
+    return workloads.served_problem(seed, steps, kind="forward")
 
 
 def run_served(seed, steps):

@@ -101,7 +101,7 @@ def test_served_request_beside_a_running_one(hip_lib):
     f = _fixture("served_34_500.npz")
     kg, med, src, sens, opts = mf.served(34, 500)
     assert int(f["crc"]) == input_crc(med, src)
-    other = workloads.served_problem(53, 2000)
+    other = workloads.served_problem(53, 2000, kind="forward")
     print(f"[engine-served] 256^2 x 500 beside {other[0].shape} x {other[0].Nt}", flush=True)
     with kwhip.Engine(steps_per_round=32) as eng:
         t_other = eng.submit(other[0], other[1], other[2], other[3], other[0].dim, **other[4])
