@@ -695,7 +695,7 @@ struct Engine {
   // flight, on whichever stream of the pool is free when the round is
   // issued; rounds of different groups overlap on the device.  Members join
   // and leave a group only between its rounds.  Free streams go to groups in
-  // two lanes (large launches by most remaining work, small ones by least),
+  // two lanes (large launches and small ones, each by least remaining work),
   // so short simulations are not queued behind the rounds of long ones and
   // large groups keep the device busy while small ones wait on latency.
   void step_loop() {
@@ -774,17 +774,16 @@ struct Engine {
                   return a.first < b.first;
                 });
       // two lanes: the first half of the streams serves groups whose
-      // launches fill the device (largest remaining work first), the other
-      // half the latency-bound small groups (least remaining work first);
-      // a lane with nothing of its own kind takes the other kind, so small
-      // rounds run beside large ones instead of queueing in front of them
+      // launches fill the device, the other half the latency-bound small
+      // groups, each by least remaining work first; a lane with nothing of
+      // its own kind takes the other kind, so small rounds run beside large
+      // ones instead of queueing in front of them
       std::vector<Group*> big, small;
       for (auto& wg : idle) {
         double cells = 0;
         for (Ticket* t : wg.second->members) cells += (double)t->sim->G.ncell;
         (sjf_only || cells < kBigCells ? small : big).push_back(wg.second);
       }
-      std::reverse(big.begin(), big.end());
       size_t ib = 0, is = 0;
       const size_t nbig_lanes = sjf_only ? 0 : (streams.size() + 1) / 2;
       for (;;) {
