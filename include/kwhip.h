@@ -186,8 +186,10 @@ typedef struct kw_problem {
    * -flip(f)], material grids with the same layouts unsigned); pml_size[1]
    * is the outer radial PML of the radial grid.  The library then builds
    * the y PML on every copy (getPML.m with axisymmetric true) and adds the
-   * u_r / r term of the divergence (:954-969).  Lossless media, p0 and
-   * binary sensors only (KW_EUNSUP otherwise).  0 otherwise. */
+   * u_r / r term of the divergence (:954-969).  Lossless or Stokes media
+   * (with or without B/A), p0, pressure sources and time-reversal data
+   * given on the expanded grid with the signs of the p-like layout, and
+   * binary sensors; velocity sources return KW_EUNSUP.  0 otherwise. */
   int32_t axisymmetric;
 } kw_problem;
 

@@ -337,9 +337,11 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream, Arena* ar
     if (!pr->use_sg) fail(KW_EINVAL, "Optional input 'UseSG' is not supported for axisymmetric simulations.");
     if (pr->absorbing_flag == 1)
       fail(KW_EUNSUP, "the axisymmetric code takes lossless or Stokes media");
-    if (pr->p_source.flag > 0 || pr->u_source[0].flag > 0 || pr->u_source[1].flag > 0)
-      fail(KW_EUNSUP, "axisymmetric simulations here take source.p0 only");
-    if (S->time_rev) fail(KW_EUNSUP, "axisymmetric time reversal is not supported");
+    // pressure sources and time-reversal data come laid out on the expanded
+    // grid with the signs of each copy, so the 2-D source and time-reversal
+    // code applies them as given
+    if (pr->u_source[0].flag > 0 || pr->u_source[1].flag > 0)
+      fail(KW_EUNSUP, "axisymmetric simulations here take no velocity sources");
     if (pr->sensor_tri || pr->directivity_angle || (pr->record_flags & KW_REC_U_SPLIT_FIELD))
       fail(KW_EUNSUP, "axisymmetric simulations here take binary sensor masks");
   }

@@ -1244,6 +1244,87 @@ def _as_mirror(f, kind, signed):
     return e
 
 
+def _as_points(pos, nx, ny):
+    """Points of the radial grid (column-major linear indices) on the
+    expanded grid of the p-like 'WSWA-FFT' layout: each point (i, j) at
+    column j (sign +), 2 Ny - j (-), 2 Ny + j (-) and 4 Ny - j (+), the
+    second and fourth copies skipping j = 0, whose mirror column stays 0.
+    Returns the expanded indices in column-major order, the radial point
+    each came from and its sign."""
+    pos = np.asarray(pos, dtype=np.int64)
+    ix, iy = pos % nx, pos // nx
+    src = np.arange(pos.size)
+    cols, rows, signs = [iy, 2 * ny + iy], [src, src], [np.ones(pos.size), -np.ones(pos.size)]
+    off = iy >= 1
+    cols += [2 * ny - iy[off], 4 * ny - iy[off]]
+    rows += [src[off], src[off]]
+    signs += [-np.ones(off.sum()), np.ones(off.sum())]
+    lin = np.concatenate([ix[r] + nx * c for r, c in zip(rows, cols)])
+    row = np.concatenate(rows)
+    sign = np.concatenate(signs)
+    order = np.argsort(lin, kind="stable")
+    return lin[order], row[order], sign[order]
+
+
+def _as_pressure_source(src, med, kgrid, scale):
+    """source.p of an axisymmetric simulation on the expanded grid: one
+    series per expanded point, signed as the p-like layout, scaled first on
+    the radial grid as kspaceFirstOrder_scaleSourceTerms.m:52-129 scales
+    them for a 2-D grid (row i with the sound speed at source point i)."""
+    Nx, Ny = kgrid.Nx, kgrid.Ny
+    if "p_mask" not in src:
+        raise ValueError("source.p_mask must be defined with source.p")
+    m = np.asarray(src["p_mask"])
+    if m.shape != (Nx, Ny):
+        raise ValueError("source.p_mask must be the same size as the computational grid.")
+    pos = _find(m)
+    if pos.size == 0:
+        raise ValueError("source.p_mask must be a binary grid with at least one element set to 1.")
+    mode = src.get("p_mode", "additive")
+    if mode not in _abi.SRC_MODE:
+        raise ValueError("source.p_mode must be set to 'additive', 'additive-no-correction', or "
+                         "'dirichlet'.")
+    vals = m[np.unravel_index(pos, m.shape, order="F")]
+    uniq = np.unique(vals)
+    n_uniq = uniq.size + (1 if pos.size < m.size else 0)
+    labelled = not (n_uniq <= 2 and float(uniq.sum()) == 1)
+    sig = np.atleast_2d(np.asarray(src["p"], dtype=np.float64)).copy()
+    if labelled:
+        if sig.shape[0] != n_uniq - 1:
+            raise ValueError("The number of time series must match the number of labelled "
+                             "source elements.")
+        sig_idx = vals.astype(np.int64) - 1
+    elif sig.shape[0] == 1:
+        sig_idx = np.zeros(pos.size, dtype=np.int64)
+    elif sig.shape[0] == pos.size:
+        sig_idx = np.arange(pos.size)
+    else:
+        raise ValueError("The number of time series must match the number of source elements.")
+    dt = float(kgrid.dt)
+    c = np.asarray(med["sound_speed"], dtype=np.float64)
+    if "p_frequency_ref" in src:
+        # kspaceFirstOrder_scaleSourceTerms.m:42-45: the correction at one
+        # frequency, injected without the k-space filter
+        f_ref = float(src["p_frequency_ref"])
+        if f_ref < 0:
+            raise ValueError("source.p_frequency_ref must be nonnegative")
+        mode = "additive-no-correction"
+        sig *= math.cos(math.pi * f_ref * dt)
+    if scale:
+        c_at = (lambda i: float(c)) if c.size == 1 else (
+            lambda i: float(c.reshape(-1, order="F")[pos[i]]))
+        for i in range(sig.shape[0]):
+            if mode == "dirichlet":
+                sig[i] /= 2 * c_at(i) ** 2
+            else:
+                sig[i] *= 2 * dt / (2 * c_at(i) * kgrid.dx)
+    epos, row, sign = _as_points(pos, Nx, Ny)
+    emask = np.zeros(Nx * 4 * Ny, dtype=bool)
+    emask[epos] = True
+    emask = emask.reshape((Nx, 4 * Ny), order="F")
+    return {"p_mask": emask, "p": sign[:, None] * sig[sig_idx[row]], "p_mode": mode}
+
+
 def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa: N802
     """Axisymmetric time-domain simulation (reference
     +kwave/+legacy/kspaceFirstOrderAS.m): a 2-D grid with x the axial and y
@@ -1252,10 +1333,11 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
     default; 'WSWA' gives the same field, the reference's discrete
     trigonometric transforms agreeing with it to machine precision) with the
     u_r / r term of the divergence added by the library
-    (kw_problem.axisymmetric).  Supported here: source.p0, lossless media
-    (with or without B/A), binary sensor masks; 'WSWS' symmetries, Stokes
-    absorption, time-varying sources, time reversal and 'PMLInside' false
-    are refused."""
+    (kw_problem.axisymmetric).  Supported here: source.p0 or a pressure
+    source (every p_mode, laid out on the expanded grid with the signs of
+    the p-like layout), lossless or Stokes media (with or without B/A),
+    binary sensor masks and time reversal; 'WSWS' symmetries, velocity
+    sources and 'PMLInside' false are refused."""
     opts = _parse_options(args, kwargs)
     sym = opts.pop("RadialSymmetry", "WSWA-FFT")
     if sym not in ("WSWA", "WSWS", "WSWA-FFT", "WSWS-FFT"):
@@ -1288,28 +1370,32 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
                   "user value ignored.")
         med["alpha_mode"] = "stokes"
         med["alpha_power"] = 2.0
-    if set(src) - {"p0"}:
-        raise NotImplementedError("axisymmetric simulations in kwhip take source.p0 only")
-    if "p0" not in src:
-        raise ValueError("source.p0 must be given for the axisymmetric simulations kwhip runs")
+    if set(src) - {"p0", "p", "p_mask", "p_mode", "p_frequency_ref"}:
+        raise NotImplementedError("axisymmetric simulations in kwhip take source.p0 or a "
+                                  "pressure source (source.p, p_mask, p_mode, p_frequency_ref)")
+    time_rev = sen is not None and "time_reversal_boundary_data" in sen
     if sen is not None:
         for k in sen:
             if k not in ("mask", "frequency_response", "record_mode", "record",
                          "record_start_index", "time_reversal_boundary_data"):
                 raise ValueError(f"'{k}' is not a valid field for sensor")
-        if "time_reversal_boundary_data" in sen:
-            raise NotImplementedError("axisymmetric time reversal is not supported by kwhip")
         m = np.asarray(sen.get("mask"))
         if m.shape != (kgrid.Nx, kgrid.Ny):
             raise NotImplementedError("axisymmetric simulations in kwhip take a binary sensor mask")
     Nx, Ny = kgrid.Nx, kgrid.Ny
-    # p0: smoothed on the expanded grid (kspaceFirstOrder_inputChecking.m:
-    # 1504-1535), trimmed back, then laid out again
-    p0 = np.asarray(src["p0"], dtype=np.float64)
     sm = opts.pop("Smooth", True)
-    sm_p0 = bool(np.atleast_1d(sm)[0])
-    if sm_p0:
-        p0 = smooth(_as_mirror(p0, "p", True), True)[:, :Ny]
+    esrc = {}
+    if "p0" in src:
+        if "p" in src:
+            raise ValueError("source.p0 and source.p can't be defined in the same simulation.")
+        # p0: smoothed on the expanded grid (kspaceFirstOrder_inputChecking.m:
+        # 1504-1535), trimmed back, then laid out again
+        p0 = np.asarray(src["p0"], dtype=np.float64)
+        if bool(np.atleast_1d(sm)[0]):
+            p0 = smooth(_as_mirror(p0, "p", True), True)[:, :Ny]
+        esrc["p0"] = _as_mirror(p0, "p", True)
+    if "p" in src:
+        esrc.update(_as_pressure_source(src, med, kgrid, opts.get("ScaleSourceTerms", True)))
     if np.size(np.atleast_1d(sm)) > 1 and any(np.atleast_1d(sm)[1:]):
         raise NotImplementedError("smoothing c0 or rho0 is not supported by kwhip for "
                                   "axisymmetric simulations")
@@ -1326,8 +1412,20 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
     esen = None
     if sen is not None:
         esen = dict(sen)
-        mask = np.zeros((Nx, 4 * Ny), dtype=bool)
-        mask[:, :Ny] = np.asarray(sen["mask"]).astype(bool)
+        mask = np.zeros(Nx * 4 * Ny, dtype=bool)
+        if time_rev:
+            # the boundary data are enforced on every copy, with its sign
+            pos = _find(np.asarray(sen["mask"]))
+            epos, row, sign = _as_points(pos, Nx, Ny)
+            data = np.atleast_2d(np.asarray(sen["time_reversal_boundary_data"], dtype=np.float64))
+            if data.shape[0] != pos.size:
+                raise ValueError("The number of time series in sensor.time_reversal_boundary_data "
+                                 "must match the number of sensor points.")
+            mask[epos] = True
+            esen["time_reversal_boundary_data"] = sign[:, None] * data[row]
+        mask = mask.reshape((Nx, 4 * Ny), order="F")
+        if not time_rev:
+            mask[:, :Ny] = np.asarray(sen["mask"]).astype(bool)
         esen["mask"] = mask
     pml = opts.get("PMLSize", 20)
     pml = [int(v) for v in np.broadcast_to(np.atleast_1d(pml), (2,))]
@@ -1335,7 +1433,9 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
         raise ValueError("The size of the PML must be smaller than the size of the grid.")
     opts["PMLSize"] = pml
     opts["Smooth"] = False
-    pp = prepare(kg, emed, {"p0": _as_mirror(p0, "p", True)}, esen, 2, opts)
+    # the pressure signals come scaled with the radial grid's sound speed
+    opts["ScaleSourceTerms"] = False
+    pp = prepare(kg, emed, esrc, esen, 2, opts)
     pp.prob.axisymmetric = 1
     rho = np.asarray(med["density"], dtype=np.float64)
     if rho.size > 1:

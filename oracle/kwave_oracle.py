@@ -68,6 +68,13 @@ How each part is pinned (tests/test_oracle_*.py)
 * alpha_filter: mode selection on a periodic grid (linearity).
 * p/u_frequency_ref: a coarse-step tone against the k-space corrected source.
 * optimal_pml_size: known answers.
+* axisymmetric (kspace_first_order_as, tests/test_as_oracle.py): an on-axis
+  Gaussian pulse is the closed-form spherical wave; a planted sign error in
+  the u_r / r term is caught; an on-axis Gaussian pressure source ('additive'
+  and 'additive-no-correction') gives the field of the 3-D solver on a
+  Cartesian grid, and dropping the k-space source correction is caught;
+  'dirichlet' holds p at its points to the series at every step; time
+  reversal equals a 'dirichlet' run of the flipped data one step shifted.
 """
 
 from __future__ import annotations
@@ -1424,13 +1431,16 @@ def as_mirror_u(f, signs=(1, 1, -1, -1)):
 def kspace_first_order_as(kgrid: KGrid, medium, source, sensor, *, data_cast="off",
                           pml_size=20, pml_alpha=2.0, smooth=True):
     """kspaceFirstOrderAS.m:355-1435 with 'RadialSymmetry' 'WSWA-FFT' for the
-    cases the library runs: an initial pressure, lossless or Stokes media
+    cases the library runs: an initial pressure or a pressure source (every
+    p_mode, :1055-1128, and p_frequency_ref), lossless or Stokes media
     (linear or with B/A; the axisymmetric code takes medium.alpha_coeff as
     Stokes absorption with alpha_power 2, kspaceFirstOrder_inputChecking.m:
     173-190, kspaceFirstOrderAS.m:1134-1163), a binary sensor mask recording
     'p', 'p_max', 'p_min',
     'p_rms', 'p_final', 'u' and 'u_final' (x axial, y radial from the
-    axis).  Returns the sensor data dict (p as [points x Nt])."""
+    axis), or time reversal of sensor.time_reversal_boundary_data (:640-650,
+    690-702, 1343-1345).  Returns the sensor data dict (p as [points x Nt]);
+    {'p_final': p} for time reversal."""
     rdt = np.float32 if data_cast == "single" else np.float64
     cdt = np.complex64 if data_cast == "single" else np.complex128
     Nx, Ny = kgrid.Nx, kgrid.Ny
@@ -1445,11 +1455,38 @@ def kspace_first_order_as(kgrid: KGrid, medium, source, sensor, *, data_cast="of
     stokes = _has(medium, "alpha_coeff")
     if stokes:
         tau = (-2.0 * db2neper(_get(medium, "alpha_coeff"), 2) * c0)
-    p0 = np.asarray(source["p0"], dtype=np.float64)
-    if smooth:
+    p0 = source.get("p0")
+    p0 = None if p0 is None else np.asarray(p0, dtype=np.float64)
+    if p0 is not None and smooth:
         # kspaceFirstOrder_inputChecking.m:1504-1535: smoothed on the
         # expanded grid (WS origin symmetry), trimmed back
         p0 = smooth_mat(as_mirror_p(p0), restore_max=True)[:, :Ny]
+    # pressure source (kspaceFirstOrderAS.m:1055-1128) with the scaling of
+    # scaleSourceTerms.m for a 2-D grid (row i scaled with c0 at source
+    # point i, the reference's quirk)
+    psrc = None
+    if "p_mask" in source:
+        pmask = np.asarray(source["p_mask"])
+        pos = np.flatnonzero(pmask.reshape(-1, order="F"))
+        sig = np.atleast_2d(np.asarray(source["p"], dtype=np.float64)).copy()
+        mode = source.get("p_mode", "additive")
+        if "p_frequency_ref" in source:
+            # kspaceFirstOrder_inputChecking.m:819-836 and
+            # kspaceFirstOrder_scaleSourceTerms.m:42-45
+            sig = sig * np.cos(2 * np.pi * float(source["p_frequency_ref"]) * dt / 2)
+            mode = "additive-no-correction"
+        if sig.shape[0] == 1:
+            sig_idx = np.zeros(pos.size, dtype=int)
+        elif pmask.max() > 1:
+            sig_idx = pmask.reshape(-1, order="F")[pos].astype(int) - 1
+        else:
+            sig_idx = np.arange(pos.size)
+        cflat = np.broadcast_to(c0, (Nx, Ny)).reshape(-1, order="F")
+        for i in range(sig.shape[0]):
+            ci = cflat[pos[i]]
+            sig[i] = sig[i] / (2 * ci ** 2) if mode == "dirichlet" else sig[i] * (2 * dt / (2 * ci * dx))
+        psrc = (pos, sig, sig_idx, mode)
+    tr = sensor.get("time_reversal_boundary_data") if isinstance(sensor, dict) else None
     # operators (kspaceFirstOrderAS.m:463-565)
     pml_x = get_pml(Nx, dx, dt, c_ref, px, ax, False)
     pml_x_sg = get_pml(Nx, dx, dt, c_ref, px, ax, True)
@@ -1467,6 +1504,7 @@ def kspace_first_order_as(kgrid: KGrid, medium, source, sensor, *, data_cast="of
     y_pos = np.fft.ifftshift(np.exp(1j * ky * dy / 2))[None, :].astype(cdt)
     y_neg = np.fft.ifftshift(np.exp(-1j * ky * dy / 2))[None, :].astype(cdt)
     y_sg = (np.arange(Ny) * dy + dy / 2)[None, :]
+    source_kappa = np.fft.ifftshift(np.cos(c_ref * k * dt / 2)).astype(rdt)
     if rho0.ndim == 2:
         # interpn '*linear' at x + dx/2 and y + dy/2, NaN past the last point
         # replaced by rho0 (kspaceFirstOrderAS.m:440-456)
@@ -1509,7 +1547,23 @@ def kspace_first_order_as(kgrid: KGrid, medium, source, sensor, *, data_cast="of
     uy = np.zeros_like(p)
     rx = np.zeros_like(p)
     ry = np.zeros_like(p)
-    p0c = p0.astype(rdt)
+    p0c = None if p0 is None else p0.astype(rdt)
+    if tr is not None:
+        trd = np.asarray(tr, dtype=np.float64)[:, ::-1]   # fliplr (:640-650)
+        Nt_loop = Nt - 1
+    else:
+        Nt_loop = Nt
+    c2_flat = np.broadcast_to(c2, (Nx, Ny))
+
+    def flat_set(a, idx_, vals):
+        f = a.reshape(-1, order="F").copy()
+        f[idx_] = vals
+        return f.reshape(a.shape, order="F").astype(rdt)
+
+    def flat_add(a, idx_, vals):
+        f = a.reshape(-1, order="F").copy()
+        f[idx_] = f[idx_] + vals
+        return f.reshape(a.shape, order="F").astype(rdt)
 
     def grad(pp):
         pk = kappa * fft2(as_mirror_p(pp))
@@ -1518,7 +1572,12 @@ def kspace_first_order_as(kgrid: KGrid, medium, source, sensor, *, data_cast="of
     def pick(f):
         return np.reshape(f, -1, order="F")[idx].astype(np.float64)
 
-    for t in range(1, Nt + 1):
+    for t in range(1, Nt_loop + 1):
+        if tr is not None:
+            p = flat_set(p, idx, trd[:, t - 1])
+            rmod = (0.5 * p / c2_flat).reshape(-1, order="F")[idx]
+            rx = flat_set(rx, idx, rmod)
+            ry = flat_set(ry, idx, rmod)
         dpdx, dpdy = grad(p)
         ux = pml_x_sg * (pml_x_sg * ux - dtc * rsx_inv * dpdx)
         uy = pml_y_sg * (pml_y_sg * uy - dtc * rsy_inv * dpdy)
@@ -1531,19 +1590,35 @@ def kspace_first_order_as(kgrid: KGrid, medium, source, sensor, *, data_cast="of
             k_ = rho0c
         rx = pml_x * (pml_x * rx - dtc * k_ * duxdx)
         ry = pml_y * (pml_y * ry - dtc * k_ * duydy)
+        if psrc is not None and t <= psrc[1].shape[1]:
+            pos, sig, sig_idx, mode = psrc
+            v = sig[sig_idx, t - 1]
+            if mode == "dirichlet":
+                rx = flat_set(rx, pos, v)
+                ry = flat_set(ry, pos, v)
+            elif mode == "additive":
+                sm_ = np.zeros((Nx, Ny)).reshape(-1, order="F")
+                sm_[pos] = v
+                sm_ = sm_.reshape((Nx, Ny), order="F")
+                add = ifft2(source_kappa * fft2(as_mirror_p(sm_)))[:, :Ny]
+                rx = (rx + add).astype(rdt)
+                ry = (ry + add).astype(rdt)
+            else:
+                rx = flat_add(rx, pos, v)
+                ry = flat_add(ry, pos, v)
         st = tau_c * rho0c * (duxdx + duydy) if stokes else 0.0
         if nonlinear:
             p = c2 * (rx + ry + st + ba * (rx + ry) ** 2 / (2 * rho0c))
         else:
             p = c2 * (rx + ry + st)
-        if t == 1:
+        if t == 1 and p0c is not None:
             p = p0c.copy()
             rx = (p0c / (2 * c2)).astype(rdt)
             ry = rx.copy()
             dpdx, dpdy = grad(p)
             ux = dtc * rsx_inv * dpdx / 2
             uy = dtc * rsy_inv * dpdy / 2
-        if t >= start:
+        if tr is None and t >= start:
             j = t - start
             pv = pick(p)
             if "p" in out:
@@ -1554,6 +1629,10 @@ def kspace_first_order_as(kgrid: KGrid, medium, source, sensor, *, data_cast="of
             pmax = np.maximum(pmax, pv)
             pmin = np.minimum(pmin, pv)
             prms += pv ** 2
+    if tr is not None:
+        # :1343-1345: the final data column, then p_final
+        p = flat_set(p, idx, trd[:, Nt - 1])
+        return {"p_final": p.astype(np.float64)}
     res = {}
     if "p" in record:
         res["p"] = out["p"]

@@ -4,7 +4,7 @@ emulator and GPU tests: x axial, y radial from the axis."""
 import numpy as np
 
 
-def case(name, Nx=32, Ny=16, seed=2):
+def case(name, Nx=32, Ny=16, seed=2, Nt=24):
     rng = np.random.default_rng(seed)
     dx = 1e-4
     x = (np.arange(Nx) - Nx // 2) * dx
@@ -12,49 +12,80 @@ def case(name, Nx=32, Ny=16, seed=2):
     X, R = np.meshgrid(x, r, indexing="ij")
     c = 1500 + 80 * rng.random((Nx, Ny))
     rho = 1000 + 90 * rng.random((Nx, Ny))
-    cs = {"Nx": Nx, "Ny": Ny, "dx": dx, "Nt": 24,
+    cs = {"Nx": Nx, "Ny": Ny, "dx": dx, "Nt": Nt,
           "medium": {"sound_speed": c, "density": rho},
           "p0": np.exp(-(X ** 2 + R ** 2) / (3 * dx) ** 2),
           "sensor": {"mask": rng.random((Nx, Ny)) < 0.3,
                      "record": ["p", "p_max", "p_rms", "p_final", "u", "u_final"]},
           "pml": [6, 4], "smooth": False}
     cs["dt"] = 0.25 * dx / c.max()
+    cs["source"] = {"p0": cs.pop("p0")}
+    t = np.arange(cs["Nt"]) * cs["dt"]
+    burst = np.sin(2 * np.pi * t / (8 * cs["dt"])) * np.exp(-((t - 8 * cs["dt"]) / (5 * cs["dt"])) ** 2)
+    src_mask = rng.random((Nx, Ny)) < 0.06
+    src_mask[Nx // 2, 0] = src_mask[Nx // 3, 0] = True      # points on the axis
+    n_src = int(src_mask.sum())
     if name == "homogeneous":
         cs["medium"] = {"sound_speed": 1500.0, "density": 1000.0}
         cs["dt"] = 0.25 * dx / 1500.0
     elif name == "nonlinear":
         cs["medium"]["BonA"] = 6 + rng.random((Nx, Ny))
-        cs["p0"] = cs["p0"] * 1e6
+        cs["source"]["p0"] = cs["source"]["p0"] * 1e6
     elif name == "stokes":
         cs["medium"]["alpha_coeff"] = 0.5 + rng.random((Nx, Ny))
     elif name == "stokes_nonlinear":
         cs["medium"]["alpha_coeff"] = 0.75
         cs["medium"]["BonA"] = 5.0
-        cs["p0"] = cs["p0"] * 1e6
+        cs["source"]["p0"] = cs["source"]["p0"] * 1e6
     elif name == "smoothed":
         cs["smooth"] = True
-        cs["p0"] = (np.sqrt(X ** 2 + R ** 2) < 4 * dx).astype(float)
+        cs["source"]["p0"] = (np.sqrt(X ** 2 + R ** 2) < 4 * dx).astype(float)
+    elif name == "source_additive":
+        # one series per point
+        cs["source"] = {"p_mask": src_mask, "p_mode": "additive",
+                        "p": rng.standard_normal((n_src, 1)) * burst}
+    elif name == "source_dirichlet":
+        cs["source"] = {"p_mask": src_mask, "p_mode": "dirichlet", "p": burst[None, :12]}
+    elif name == "source_labelled":
+        lab = np.zeros((Nx, Ny))
+        lab[src_mask] = 1 + (rng.random(n_src) < 0.5)
+        cs["source"] = {"p_mask": lab, "p_mode": "additive-no-correction",
+                        "p": np.stack([burst, -0.5 * burst])}
+    elif name == "source_frequency_ref":
+        cs["source"] = {"p_mask": src_mask, "p": burst[None], "p_frequency_ref": 1e6}
+    elif name == "source_stokes_nonlinear":
+        cs["medium"]["alpha_coeff"] = 0.5 + rng.random((Nx, Ny))
+        cs["medium"]["BonA"] = 6.0
+        cs["source"] = {"p_mask": src_mask, "p": 1e6 * burst[None]}
+    elif name == "time_reversal":
+        sm = cs["sensor"]["mask"]
+        cs["source"] = {}
+        cs["sensor"] = {"mask": sm, "time_reversal_boundary_data":
+                        rng.standard_normal((int(sm.sum()), 1)) * burst}
     elif name != "hetero":
         raise KeyError(name)
     return cs
 
 
-NAMES = ["homogeneous", "hetero", "nonlinear", "stokes", "stokes_nonlinear", "smoothed"]
+NAMES = ["homogeneous", "hetero", "nonlinear", "stokes", "stokes_nonlinear", "smoothed",
+         "source_additive", "source_dirichlet", "source_labelled", "source_frequency_ref",
+         "source_stokes_nonlinear", "time_reversal"]
 
 
 def run_kwhip(cs):
     import kwhip
     kg = kwhip.kWaveGrid(cs["Nx"], cs["dx"], cs["Ny"], cs["dx"])
     kg.setTime(cs["Nt"], cs["dt"])
-    return kwhip.kspaceFirstOrderAS(kg, dict(cs["medium"]), {"p0": cs["p0"]}, dict(cs["sensor"]),
-                                    PMLSize=cs["pml"], Smooth=cs["smooth"])
+    out = kwhip.kspaceFirstOrderAS(kg, dict(cs["medium"]), dict(cs["source"]), dict(cs["sensor"]),
+                                   PMLSize=cs["pml"], Smooth=cs["smooth"])
+    return {"p_final": out} if "time_reversal_boundary_data" in cs["sensor"] else out
 
 
 def run_oracle(cs, data_cast):
     from oracle import kwave_oracle as ko
     kg = ko.KGrid(cs["Nx"], cs["dx"], cs["Ny"], cs["dx"])
     kg.set_time(cs["Nt"], cs["dt"])
-    return ko.kspace_first_order_as(kg, cs["medium"], {"p0": cs["p0"]}, cs["sensor"],
+    return ko.kspace_first_order_as(kg, cs["medium"], cs["source"], cs["sensor"],
                                     data_cast=data_cast, pml_size=cs["pml"], smooth=cs["smooth"])
 
 

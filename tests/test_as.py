@@ -40,7 +40,7 @@ def test_as_option_checks():
     cs = ac.case("homogeneous")
     kg = kwhip.kWaveGrid(cs["Nx"], cs["dx"], cs["Ny"], cs["dx"])
     kg.setTime(4, cs["dt"])
-    args = (kg, cs["medium"], {"p0": cs["p0"]}, cs["sensor"])
+    args = (kg, cs["medium"], dict(cs["source"]), cs["sensor"])
     with pytest.raises(ValueError, match="'RadialSymmetry' must be set to 'WSWA', 'WSWS'"):
         kwhip.kspaceFirstOrderAS(*args, RadialSymmetry="WSW")
     with pytest.raises(NotImplementedError):
@@ -49,10 +49,17 @@ def test_as_option_checks():
         kwhip.kspaceFirstOrderAS(*args, UseSG=False)
     with pytest.raises(ValueError, match="medium.alpha_mode is not supported with the axisymmetric"):
         kwhip.kspaceFirstOrderAS(kg, dict(cs["medium"], alpha_coeff=0.5, alpha_mode="no_dispersion"),
-                                 {"p0": cs["p0"]}, cs["sensor"])
-    with pytest.raises(NotImplementedError):
-        kwhip.kspaceFirstOrderAS(kg, cs["medium"], {"p0": cs["p0"], "p_mask": cs["p0"] > 0.5,
+                                 dict(cs["source"]), cs["sensor"])
+    p0 = cs["source"]["p0"]
+    with pytest.raises(ValueError, match="source.p0 and source.p can't be defined"):
+        kwhip.kspaceFirstOrderAS(kg, cs["medium"], {"p0": p0, "p_mask": p0 > 0.5,
                                                     "p": np.ones((1, 4))}, cs["sensor"])
+    with pytest.raises(NotImplementedError):
+        kwhip.kspaceFirstOrderAS(kg, cs["medium"], {"u_mask": p0 > 0.5, "ux": np.ones((1, 4))},
+                                 cs["sensor"])
+    with pytest.raises(ValueError, match="number of time series must match"):
+        kwhip.kspaceFirstOrderAS(kg, cs["medium"], {"p_mask": p0 > 0.5, "p": np.ones((2, 4))},
+                                 cs["sensor"])
     with pytest.raises(ValueError, match="Unknown optional input 'RadialSymmetry'"):
         kwhip.kspaceFirstOrder2D(*args, RadialSymmetry="WSWA")
     # 'WSWA' (discrete trigonometric transforms in the reference) gives the

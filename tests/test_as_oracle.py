@@ -6,10 +6,20 @@
   from the PML to the scheme's discretisation error, which falls as the
   pulse is better resolved (w = 4, 6, 8 points);
 * the same run with the sign of the u_r / r term flipped (a plausible
-  mistake, planted in the oracle here) is off by order one.
+  mistake, planted in the oracle here) is off by order one;
+* an on-axis Gaussian pressure source gives the field of the 3-D solver
+  (itself pinned by tests/test_oracle_*.py) on a Cartesian grid, to the
+  axisymmetric discretisation error, for 'additive' and
+  'additive-no-correction'; leaving out the k-space source correction is
+  three times that error;
+* 'dirichlet' holds p at its points to the series (and scales one series
+  with the sound speed at the first point, as the reference does);
+* time reversal equals a 'dirichlet' run of the flipped data one column
+  ahead; without the flip it is far off.
 """
 
 import numpy as np
+import pytest
 
 import as_cases as ac
 from oracle import kwave_oracle as ko
@@ -44,3 +54,122 @@ def test_planted_sign_error_is_caught(monkeypatch):
     monkeypatch.setattr(ko, "as_mirror_u", flipped)
     Nx, Ny, dx, c0 = 128, 64, 1e-4, 1500.0
     assert _run(Nx, Ny, dx, c0, 60, 0.2 * dx / c0, 6 * dx) > 0.1
+
+
+# -- pressure sources and time reversal ---------------------------------------
+
+def _pulse(Nt, dt, steps):
+    t = np.arange(Nt) * dt
+    T = steps * dt
+    return np.sin(2 * np.pi * t / T) * np.exp(-((t - 1.5 * T) / (0.6 * T)) ** 2)
+
+
+def _source_vs_3d(mode_as, mode_3d, Nx=64, Ny=32, Nt=40, w_pts=5.0, steps=20, pml=8):
+    """An on-axis Gaussian source g(R) s(t) (one series per point), run by
+    the axisymmetric solver and by the 3-D solver on an Nx x 2Ny x 2Ny
+    Cartesian grid centred on the axis; the relative L-inf difference of
+    the final pressure on the radial line, inside R < 12 dx (the wave has
+    not reached the PML)."""
+    dx, c0 = 1e-4, 1500.0
+    dt = 0.2 * dx / c0
+    s = _pulse(Nt, dt, steps)
+    w = w_pts * dx
+    x = (np.arange(Nx) - Nx // 2) * dx
+    X, Rr = np.meshgrid(x, np.arange(Ny) * dx, indexing="ij")
+    g = np.exp(-(X ** 2 + Rr ** 2) / w ** 2)
+    m = g > 1e-3
+    sig = g.reshape(-1, order="F")[np.flatnonzero(m.reshape(-1, order="F"))][:, None] * s
+    kg = ko.KGrid(Nx, dx, Ny, dx)
+    kg.set_time(Nt, dt)
+    med = {"sound_speed": c0, "density": 1000.0}
+    pa = ko.kspace_first_order_as(kg, med, {"p_mask": m, "p": sig, "p_mode": mode_as},
+                                  {"mask": np.ones((Nx, Ny), bool), "record": ["p_final"]},
+                                  pml_size=[pml, pml], smooth=False)["p_final"]
+    M = 2 * Ny
+    yv = (np.arange(M) - M // 2) * dx
+    X3, Y3, Z3 = np.meshgrid(x, yv, yv, indexing="ij")
+    g3 = np.exp(-(X3 ** 2 + Y3 ** 2 + Z3 ** 2) / w ** 2)
+    m3 = g3 > 1e-3
+    sig3 = g3.reshape(-1, order="F")[np.flatnonzero(m3.reshape(-1, order="F"))][:, None] * s
+    kg3 = ko.KGrid(Nx, dx, M, dx, M, dx)
+    kg3.set_time(Nt, dt)
+    p3 = ko.kspace_first_order(kg3, med, {"p_mask": m3, "p": sig3, "p_mode": mode_3d},
+                               {"mask": np.ones((Nx, M, M), bool), "record": ["p_final"]},
+                               pml_size=pml, smooth=False, data_cast="off")["p_final"]
+    p3 = np.asarray(p3).reshape((Nx, M, M), order="F")[:, M // 2:M // 2 + Ny, M // 2]
+    inside = np.sqrt(X ** 2 + Rr ** 2) < 12 * dx
+    return float(np.max(np.abs(pa - p3)[inside]) / np.max(np.abs(p3)[inside]))
+
+
+@pytest.mark.parametrize("mode", ["additive", "additive-no-correction"])
+def test_source_matches_3d(mode):
+    # the scheme's own axisymmetric discretisation error at this resolution
+    # is 6e-4; the source layout (signs of the mirrored copies, the split
+    # over two density components and its scaling) is pinned by it
+    assert _source_vs_3d(mode, mode) < 1e-3
+
+
+def test_dropped_source_correction_is_caught():
+    # injecting without the k-space correction cos(c k dt / 2) where the 3-D
+    # run applies it is three times the discretisation error
+    assert _source_vs_3d("additive-no-correction", "additive") > 1.2e-3
+
+
+def _hetero(Nx, Ny, seed):
+    rng = np.random.default_rng(seed)
+    return {"sound_speed": 1500 + 80 * rng.random((Nx, Ny)),
+            "density": 1000 + 90 * rng.random((Nx, Ny))}
+
+
+def test_dirichlet_holds_pressure():
+    Nx, Ny, dx, Nt = 32, 16, 1e-4, 20
+    med = _hetero(Nx, Ny, 4)
+    dt = 0.25 * dx / med["sound_speed"].max()
+    rng = np.random.default_rng(5)
+    m = rng.random((Nx, Ny)) < 0.08
+    m[:, 0] |= rng.random(Nx) < 0.2          # points on the axis too
+    d = rng.standard_normal((int(m.sum()), Nt))
+    kg = ko.KGrid(Nx, dx, Ny, dx)
+    kg.set_time(Nt, dt)
+    out = ko.kspace_first_order_as(kg, med, {"p_mask": m, "p": d, "p_mode": "dirichlet"},
+                                   {"mask": m, "record": ["p"]}, pml_size=[6, 4], smooth=False)
+    assert np.max(np.abs(out["p"] - d)) <= 1e-12 * np.max(np.abs(d))
+    # one series for every point, scaled with the sound speed at the first
+    # point (kspaceFirstOrder_scaleSourceTerms.m:64-66), sets p there to the
+    # series times c^2 / c_1^2
+    out1 = ko.kspace_first_order_as(kg, med, {"p_mask": m, "p": d[:1], "p_mode": "dirichlet"},
+                                    {"mask": m, "record": ["p"]}, pml_size=[6, 4], smooth=False)
+    c = med["sound_speed"].reshape(-1, order="F")[np.flatnonzero(m.reshape(-1, order="F"))]
+    want = d[:1] * (c ** 2 / c[0] ** 2)[:, None]
+    assert np.max(np.abs(out1["p"] - want)) <= 1e-12 * np.max(np.abs(want))
+
+
+def test_time_reversal_equals_dirichlet():
+    # time reversal enforces p = data(t) and rho = p / (2 c^2) at the start
+    # of step t over Nt - 1 steps and assigns the last column at the end; a
+    # dirichlet source sets the same density after the update of step t, so
+    # with a zero first column the two agree when the source runs the
+    # flipped data one column ahead for Nt - 1 steps
+    Nx, Ny, dx, Nt = 32, 16, 1e-4, 18
+    med = _hetero(Nx, Ny, 6)
+    dt = 0.25 * dx / med["sound_speed"].max()
+    rng = np.random.default_rng(7)
+    m = rng.random((Nx, Ny)) < 0.1
+    D = rng.standard_normal((int(m.sum()), Nt))
+    D[:, -1] = 0.0
+    kg = ko.KGrid(Nx, dx, Ny, dx)
+    kg.set_time(Nt, dt)
+    tr = ko.kspace_first_order_as(kg, med, {}, {"mask": m, "time_reversal_boundary_data": D},
+                                  pml_size=[6, 4], smooth=False)["p_final"]
+    kg2 = ko.KGrid(Nx, dx, Ny, dx)
+    kg2.set_time(Nt - 1, dt)
+    dr = ko.kspace_first_order_as(kg2, med, {"p_mask": m, "p": D[:, ::-1][:, 1:],
+                                             "p_mode": "dirichlet"},
+                                  {"mask": np.ones((Nx, Ny), bool), "record": ["p_final"]},
+                                  pml_size=[6, 4], smooth=False)["p_final"]
+    assert np.max(np.abs(tr - dr)) <= 1e-11 * np.max(np.abs(dr))
+    # the data run forwards (the flip left out) is far off
+    tr2 = ko.kspace_first_order_as(kg, med, {}, {"mask": m,
+                                                  "time_reversal_boundary_data": D[:, ::-1]},
+                                   pml_size=[6, 4], smooth=False)["p_final"]
+    assert np.max(np.abs(tr2 - dr)) > 0.1 * np.max(np.abs(dr))
