@@ -189,7 +189,12 @@ typedef struct kw_problem {
    * u_r / r term of the divergence (:954-969).  Lossless or Stokes media
    * (with or without B/A), p0, pressure sources and time-reversal data
    * given on the expanded grid with the signs of the p-like layout, and
-   * binary sensors; velocity sources return KW_EUNSUP.  0 otherwise. */
+   * binary sensors; velocity sources return KW_EUNSUP.  2: the radial grid
+   * itself (Ny the radial size, inputs as kspaceFirstOrderAS.m prepares
+   * them), the layout of the file contract: kw_save_problem writes it with
+   * axisymmetric_flag = 1 and kw_run_file lays such a file out on the
+   * expanded grid and writes the radial grid's outputs; kw_run refuses 2.
+   * 0 otherwise. */
   int32_t axisymmetric;
 } kw_problem;
 

@@ -80,8 +80,13 @@ void save_problem(const kw_problem* pr, const char* path) {
     fail(KW_EUNSUP, "alpha_filter, alpha_mode and alpha_sign are not part of the file contract");
   if (!pr->use_kspace || !pr->use_sg)
     fail(KW_EUNSUP, "'UsekSpace' and 'UseSG' false are not part of the file contract");
-  if (pr->axisymmetric)
-    fail(KW_EUNSUP, "axisymmetric problems are not written to the file contract here");
+  // an axisymmetric problem goes over on its radial grid (2), as
+  // kspaceFirstOrderAS.m writes it with 'SaveToDisk'; the expanded grid that
+  // kw_run steps (1) is not a file layout
+  if (pr->axisymmetric == 1)
+    fail(KW_EINVAL, "an axisymmetric problem is written on its radial grid (axisymmetric = 2)");
+  if (pr->axisymmetric != 0 && pr->axisymmetric != 2) fail(KW_EINVAL, "axisymmetric must be 0, 1 or 2");
+  if (pr->axisymmetric == 2 && dim != 2) fail(KW_EINVAL, "an axisymmetric problem is a 2D grid");
   const int64_t Nx = pr->Nx, Ny = pr->Ny, Nz = dim == 3 ? pr->Nz : 1;
   const int64_t n = Nx * Ny * Nz;
   kwh5::Writer w;
@@ -112,7 +117,7 @@ void save_problem(const kw_problem* pr, const char* path) {
   s.integer("nonlinear_flag", pr->nonlinear_flag ? 1 : 0);
   s.integer("absorbing_flag", (uint64_t)pr->absorbing_flag);
   s.integer("elastic_flag", 0);
-  s.integer("axisymmetric_flag", 0);
+  s.integer("axisymmetric_flag", pr->axisymmetric == 2 ? 1 : 0);
   const bool corners = pr->sensor_mask_corners && pr->sensor_num_cuboids > 0;
   s.integer("sensor_mask_type", corners ? 1 : 0);
   s.integer("pml_x_size", (uint64_t)pr->pml_size[0]);
@@ -206,6 +211,8 @@ struct Loaded {
   kw_problem pb{};
   std::vector<double> c0, rho0, sg[3], bona, alpha, p0, psig, usig[3];
   U64 pidx, uidx, sens, corners;
+  bool axisym = false;  // axisymmetric_flag: the radial grid of kspaceFirstOrderAS
+  int64_t Ny_radial = 0;
 };
 
 void load_problem(const kwh5::Reader& r, Loaded& L) {
@@ -224,8 +231,8 @@ void load_problem(const kwh5::Reader& r, Loaded& L) {
     if (!r.has(nm)) fail(KW_EINVAL, std::string("the input file has no '") + nm + "'");
     return r.scalar(nm);
   };
-  for (const char* nm : {"elastic_flag", "axisymmetric_flag", "transducer_source_flag",
-                         "nonuniform_grid_flag", "sxx_source_flag", "syy_source_flag",
+  L.axisym = uopt("axisymmetric_flag") != 0;
+  for (const char* nm : {"elastic_flag", "transducer_source_flag", "nonuniform_grid_flag", "sxx_source_flag", "syy_source_flag",
                          "szz_source_flag", "sxy_source_flag", "sxz_source_flag",
                          "syz_source_flag"})
     if (uopt(nm) != 0)
@@ -233,6 +240,7 @@ void load_problem(const kwh5::Reader& r, Loaded& L) {
                                         "simulations are supported");
   const int64_t Nx = (int64_t)u("Nx"), Ny = (int64_t)u("Ny"), Nz = (int64_t)u("Nz");
   const int dim = Nz > 1 ? 3 : 2;
+  if (L.axisym && dim != 2) fail(KW_EINVAL, "an axisymmetric input file holds a 2D grid");
   const int64_t n = Nx * Ny * Nz;
   p.dim = dim;
   p.Nx = Nx;
@@ -349,6 +357,120 @@ void load_problem(const kwh5::Reader& r, Loaded& L) {
 }
 
 // -------------------------------------------------------------------------
+// axisymmetric input files
+// -------------------------------------------------------------------------
+// kspaceFirstOrderAS.m writes its radial grid (x axial, y radial from the
+// axis) with the 'WSWA' symmetry that the reference's native code steps
+// with discrete trigonometric transforms.  The library steps the expanded
+// grid of 'WSWA-FFT' (the same field, kw_problem.axisymmetric = 1), four
+// times the radial size, so the radial inputs are laid out there:
+// p-like grids as [f, 0, -flip(f(2:end)), -f, 0, flip(f(2:end))] along y
+// (kspaceFirstOrderAS.m:703-707), the y-staggered density as [f, flip(f),
+// f, flip(f)], material grids unsigned with each mirror column repeating
+// its neighbour, and every pressure-source point (i, j) as the points
+// (i, j), (i, 2Ny - j), (i, 2Ny + j), (i, 4Ny - j) with signs +, -, -, +
+// (the second and fourth left out for j = 0) and a series of its own.
+// Sensor points keep their indices (the first copy is the radial grid).
+struct AxisymExpanded {
+  std::vector<double> c0, rho0, sg[2], bona, alpha, p0, psig;
+  U64 pidx;
+};
+
+// p-like layout of an Nx x Ny grid (signed: copies +, -, -, +, mirror
+// columns 0; unsigned: every copy +, mirror columns repeat their neighbour)
+std::vector<double> layout_p(const double* f, int64_t Nx, int64_t Ny, bool sign) {
+  std::vector<double> e((size_t)(Nx * 4 * Ny), 0.0);
+  const double sg[4] = {1.0, -1.0, -1.0, 1.0};
+  for (int b = 0; b < 4; ++b)
+    for (int64_t jj = 0; jj < Ny; ++jj) {
+      int64_t src;
+      if (b % 2 == 0)
+        src = jj;
+      else if (jj > 0)
+        src = Ny - jj;
+      else if (sign)
+        continue;  // the mirror column of a signed grid stays 0
+      else
+        src = Ny - 1;
+      const double k = sign ? sg[b] : 1.0;
+      double* o = e.data() + (b * Ny + jj) * Nx;
+      const double* a = f + src * Nx;
+      for (int64_t i = 0; i < Nx; ++i) o[i] = k * a[i];
+    }
+  return e;
+}
+
+// y-staggered layout, unsigned: [f, flip(f), f, flip(f)]
+std::vector<double> layout_u(const double* f, int64_t Nx, int64_t Ny) {
+  std::vector<double> e((size_t)(Nx * 4 * Ny));
+  for (int b = 0; b < 4; ++b)
+    for (int64_t jj = 0; jj < Ny; ++jj) {
+      const int64_t src = b % 2 == 0 ? jj : Ny - 1 - jj;
+      std::copy(f + src * Nx, f + (src + 1) * Nx, e.begin() + (b * Ny + jj) * Nx);
+    }
+  return e;
+}
+
+void expand_axisym(Loaded& L, AxisymExpanded& E) {
+  kw_problem& p = L.pb;
+  const int64_t Nx = p.Nx, Ny = p.Ny;
+  L.Ny_radial = Ny;
+  auto grid_p = [&](const double*& ptr, std::vector<double>& store, bool sign) {
+    if (ptr) {
+      store = layout_p(ptr, Nx, Ny, sign);
+      ptr = store.data();
+    }
+  };
+  grid_p(p.c0, E.c0, false);
+  grid_p(p.rho0, E.rho0, false);
+  grid_p(p.rho0_sg[0], E.sg[0], false);
+  if (p.rho0_sg[1]) {
+    E.sg[1] = layout_u(p.rho0_sg[1], Nx, Ny);
+    p.rho0_sg[1] = E.sg[1].data();
+  }
+  grid_p(p.BonA, E.bona, false);
+  grid_p(p.alpha_coeff, E.alpha, false);
+  grid_p(p.p0, E.p0, true);
+  kw_time_source& ps = p.p_source;
+  if (ps.flag > 0) {
+    struct Pt {
+      uint64_t lin;
+      int64_t row;
+      double sign;
+    };
+    std::vector<Pt> pts;
+    pts.reserve((size_t)(4 * ps.num_points));
+    for (int64_t k = 0; k < ps.num_points; ++k) {
+      const int64_t lin = (int64_t)ps.index[k] - 1;
+      if (lin < 0 || lin >= Nx * Ny) fail(KW_EINVAL, "p_source_index outside the grid");
+      const int64_t i = lin % Nx, j = lin / Nx;
+      const int64_t row = ps.num_series > 1 ? k : 0;
+      pts.push_back({(uint64_t)(i + Nx * j), row, 1.0});
+      pts.push_back({(uint64_t)(i + Nx * (2 * Ny + j)), row, -1.0});
+      if (j > 0) {
+        pts.push_back({(uint64_t)(i + Nx * (2 * Ny - j)), row, -1.0});
+        pts.push_back({(uint64_t)(i + Nx * (4 * Ny - j)), row, 1.0});
+      }
+    }
+    std::sort(pts.begin(), pts.end(), [](const Pt& a, const Pt& b) { return a.lin < b.lin; });
+    const int64_t ne = (int64_t)pts.size(), len = ps.flag, rows = ps.num_series;
+    E.pidx.resize((size_t)ne);
+    E.psig.resize((size_t)(ne * len));
+    for (int64_t e = 0; e < ne; ++e) {
+      E.pidx[e] = pts[e].lin + 1;
+      for (int64_t t = 0; t < len; ++t)
+        E.psig[e + ne * t] = pts[e].sign * ps.input[pts[e].row + rows * t];
+    }
+    ps.num_points = ne;
+    ps.index = E.pidx.data();
+    ps.num_series = ne;
+    ps.input = E.psig.data();
+  }
+  p.Ny = 4 * Ny;
+  p.axisymmetric = 1;
+}
+
+// -------------------------------------------------------------------------
 // outputs -> output file
 // -------------------------------------------------------------------------
 struct OutBufs {
@@ -370,6 +492,8 @@ void run_file(const char* in, const char* out, uint32_t rec, int64_t start) {
   kwh5::Reader r(in);
   Loaded L;
   load_problem(r, L);
+  AxisymExpanded E;
+  if (L.axisym) expand_axisym(L, E);
   kw_problem& p = L.pb;
   p.record_flags = rec;
   p.record_start_index = start > 0 ? start : 1;
@@ -402,17 +526,21 @@ void run_file(const char* in, const char* out, uint32_t rec, int64_t start) {
   rc = kw_run(&p, &o, err, sizeof err);
   if (rc != KW_OK) fail(rc, err);
 
+  // grid outputs of an axisymmetric file are the radial grid: the first
+  // copy, i.e. the first Nx * Ny cells in column-major order
+  const int64_t Ny_out = L.axisym ? L.Ny_radial : p.Ny;
+  const size_t nco = L.axisym ? (size_t)(p.Nx * Ny_out) : nc;
   kwh5::Writer w;
   Store s{w, {}, {}};
   s.integer("Nx", (uint64_t)p.Nx);
-  s.integer("Ny", (uint64_t)p.Ny);
+  s.integer("Ny", (uint64_t)Ny_out);
   s.integer("Nz", (uint64_t)(dim == 3 ? p.Nz : 1));
   s.integer("Nt", (uint64_t)p.Nt);
   s.integer("pml_x_size", (uint64_t)p.pml_size[0]);
   s.integer("pml_y_size", (uint64_t)p.pml_size[1]);
   s.integer("pml_z_size", (uint64_t)p.pml_size[2]);
   s.scalar("dt", p.dt);
-  const U64 grid = mdims((uint64_t)p.Nx, (uint64_t)p.Ny, (uint64_t)(dim == 3 ? p.Nz : 1));
+  const U64 grid = mdims((uint64_t)p.Nx, (uint64_t)Ny_out, (uint64_t)(dim == 3 ? p.Nz : 1));
   auto to_f = [](const double* a, size_t n) {
     std::vector<float> v(n);
     for (size_t i = 0; i < n; ++i) v[i] = (float)a[i];
@@ -459,10 +587,10 @@ void run_file(const char* in, const char* out, uint32_t rec, int64_t start) {
     }
   };
   auto gridf = [&](const std::string& nm, const float* a) {
-    if (a) s.real(nm, grid, std::vector<float>(a, a + nc));
+    if (a) s.real(nm, grid, std::vector<float>(a, a + nco));
   };
   auto gridd = [&](const std::string& nm, const double* a) {
-    if (a) s.real(nm, grid, to_f(a, nc));
+    if (a) s.real(nm, grid, to_f(a, nco));
   };
   series("p", o.p);
   stat("p_max", o.p_max);

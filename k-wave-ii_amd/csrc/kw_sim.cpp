@@ -330,6 +330,9 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream, Arena* ar
   P.eos = pr->absorbing_flag;
   // kspaceFirstOrderAS on its radially expanded grid (see kw_problem)
   S->axisym = pr->axisymmetric != 0;
+  if (pr->axisymmetric == 2)
+    fail(KW_EINVAL, "axisymmetric = 2 (the radial grid of the file contract) is written by "
+                    "kw_save_problem and run from the file by kw_run_file");
   if (pr->axisymmetric < 0 || pr->axisymmetric > 1) fail(KW_EINVAL, "axisymmetric must be 0 or 1");
   if (S->axisym) {
     if (pdim != 2) fail(KW_EINVAL, "an axisymmetric problem is a 2D grid");

@@ -1336,13 +1336,22 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
     (kw_problem.axisymmetric).  Supported here: source.p0 or a pressure
     source (every p_mode, laid out on the expanded grid with the signs of
     the p-like layout), lossless or Stokes media (with or without B/A),
-    binary sensor masks and time reversal; 'WSWS' symmetries, velocity
-    sources and 'PMLInside' false are refused."""
+    binary sensor masks and time reversal; 'SaveToDisk' writes the radial
+    grid's input file (axisymmetric_flag 1, 'WSWA'), which kw_run_file and
+    kspaceFirstOrder-HIP run; 'WSWS' symmetries, velocity sources and
+    'PMLInside' false are refused."""
     opts = _parse_options(args, kwargs)
     sym = opts.pop("RadialSymmetry", "WSWA-FFT")
     if sym not in ("WSWA", "WSWS", "WSWA-FFT", "WSWS-FFT"):
         raise ValueError("Optional input 'RadialSymmetry' must be set to 'WSWA', 'WSWS', "
                          "'WSWA-FFT', 'WSWS-FFT'.")
+    save = _save_to_disk_options(2, opts)
+    if save is not None and sym != "WSWA":
+        # kspaceFirstOrder_inputChecking.m:1402-1411: the file holds 'WSWA'
+        if sym.startswith("WSWS"):
+            print("  WARNING: Optional input 'RadialSymmetry' changed to 'WSWA' for compatibility "
+                  "with 'SaveToDisk'.")
+        sym = "WSWA"
     if sym in ("WSWS", "WSWS-FFT"):
         raise NotImplementedError("'RadialSymmetry' 'WSWS' and 'WSWS-FFT' are not supported by kwhip")
     if getattr(kgrid, "dim", 2) != 2:
@@ -1352,9 +1361,6 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
     if not opts.get("PMLInside", True):
         raise NotImplementedError("'PMLInside' false is not supported by kwhip for axisymmetric "
                                   "simulations")
-    for k in ("SaveToDisk", "SaveToDiskExit"):
-        if opts.get(k):
-            raise NotImplementedError(f"'{k}' is not supported by kwhip for axisymmetric simulations")
     med = dict(medium) if isinstance(medium, dict) else dict(vars(medium))
     src = dict(source) if isinstance(source, dict) else dict(vars(source))
     sen = None if sensor is None else (dict(sensor) if isinstance(sensor, dict) else dict(vars(sensor)))
@@ -1385,6 +1391,7 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
     Nx, Ny = kgrid.Nx, kgrid.Ny
     sm = opts.pop("Smooth", True)
     esrc = {}
+    p0 = None
     if "p0" in src:
         if "p" in src:
             raise ValueError("source.p0 and source.p can't be defined in the same simulation.")
@@ -1399,6 +1406,22 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
     if np.size(np.atleast_1d(sm)) > 1 and any(np.atleast_1d(sm)[1:]):
         raise NotImplementedError("smoothing c0 or rho0 is not supported by kwhip for "
                                   "axisymmetric simulations")
+    if save is not None:
+        # the radial grid as kspaceFirstOrderAS.m writes it with 'SaveToDisk'
+        # (kspaceFirstOrder_saveToDisk.m, axisymmetric_flag 1): a 2-D problem
+        # of the radial grid, the sources scaled as a 2-D grid scales them
+        rsrc = {k: v for k, v in src.items() if k != "p0"}
+        if p0 is not None:
+            rsrc["p0"] = p0
+        ropts = dict(opts, Smooth=False)
+        ropts.pop("SaveToDisk", None)
+        ropts.pop("SaveToDiskExit", None)
+        ropts.pop("HDFCompressionLevel", None)
+        rp = prepare(kgrid, med, rsrc, sen, 2, ropts)
+        rp.prob.axisymmetric = 2
+        save_to_disk(rp, save)
+        if opts.get("SaveToDiskExit", True):
+            return []
     kg = kWaveGrid(Nx, kgrid.dx, 4 * Ny, kgrid.dy)
     kg.setTime(kgrid.Nt, kgrid.dt)
     emed = {}

@@ -125,3 +125,48 @@ def spherical_wave(Nx, Ny, dx, c0, Nt, dt, width):
         ex = ((R - c0 * T) * g(R - c0 * T) + (R + c0 * T) * g(R + c0 * T)) / (2 * R)
     ex[R == 0] = (1 - 2 * (c0 * T) ** 2 / width ** 2) * g(c0 * T)
     return g(R), ex, (np.abs(X) < (Nx / 2 - 20) * dx) & (Rr < (Ny - 20) * dx)
+
+
+def save_and_run_file(cs, tmp_path, lib, record=("p", "p_max", "p_rms", "p_final", "u", "u_final"),
+                      matlab_style=False, **opts):
+    """Write the case with kspaceFirstOrderAS(..., 'SaveToDisk'), optionally
+    re-write it the way MATLAB writes it (chunked, deflated), run it with
+    kw_run_file and return the output file's records (h5lite) and the input
+    file's variables."""
+    import ctypes as C
+    import sys
+    from pathlib import Path
+    import kwhip
+    from kwhip import _abi
+    sys.path.insert(0, str(Path(__file__).resolve().parent / "golden"))
+    import h5lite
+    kg = kwhip.kWaveGrid(cs["Nx"], cs["dx"], cs["Ny"], cs["dx"])
+    kg.setTime(cs["Nt"], cs["dt"])
+    fn = tmp_path / "as_in.h5"
+    r = kwhip.kspaceFirstOrderAS(kg, dict(cs["medium"]), dict(cs["source"]), dict(cs["sensor"]),
+                                 PMLSize=cs["pml"], Smooth=cs["smooth"], SaveToDisk=str(fn), **opts)
+    assert r == []
+    v = h5lite.read(fn)
+    if matlab_style:
+        variables = {k: (a, "uint64" if a.dtype == np.uint64 else "single") for k, a in v.items()}
+        h5lite.write_input(tmp_path / "as_matlab.h5", variables, h5lite.read_attrs(fn), compression=4)
+        fn = tmp_path / "as_matlab.h5"
+    flags = 0
+    for k in record:
+        flags |= _abi.REC[k]
+    _abi.call(lib.kw_run_file, str(fn).encode(), str(tmp_path / "as_out.h5").encode(),
+              C.c_uint32(flags), C.c_int64(1))
+    return h5lite.read(tmp_path / "as_out.h5"), v
+
+
+def file_records(out, Nx, Ny):
+    """The output file's records under the names of the direct call."""
+    res = {}
+    for k, a in out.items():
+        if k in ("Nx", "Ny", "Nz", "Nt", "pml_x_size", "pml_y_size", "pml_z_size", "dt"):
+            continue
+        a = np.squeeze(np.asarray(a, dtype=np.float64))
+        if k.endswith("_final"):
+            a = a.reshape(Nx, Ny)
+        res[k] = a
+    return res

@@ -67,3 +67,66 @@ def test_as_option_checks():
     a = kwhip.kspaceFirstOrderAS(*args, PMLSize=[6, 4], Smooth=False)
     b = kwhip.kspaceFirstOrderAS(*args, PMLSize=[6, 4], Smooth=False, RadialSymmetry="WSWA")
     assert np.array_equal(np.asarray(a["p"]), np.asarray(b["p"]))
+
+
+@pytest.mark.parametrize("name", ["hetero", "stokes_nonlinear", "source_additive",
+                                  "source_dirichlet"])
+def test_as_file_contract(name, tmp_path):
+    """'SaveToDisk' writes the radial grid (axisymmetric_flag 1, the
+    variable set of a 2-D file), and kw_run_file runs it on the expanded
+    grid: the records match the direct call and the oracle."""
+    from kwhip import _abi
+    cs = ac.case(name)
+    out, v = ac.save_and_run_file(cs, tmp_path, _abi.load("emu"), matlab_style=name == "hetero")
+    assert int(v["axisymmetric_flag"].ravel()[0]) == 1
+    assert [int(v[k].ravel()[0]) for k in ("Nx", "Ny", "Nz")] == [cs["Nx"], cs["Ny"], 1]
+    assert "rho0_sgz" not in v and v["c0"].size in (1, cs["Nx"] * cs["Ny"])
+    assert [int(out[k].ravel()[0]) for k in ("Nx", "Ny")] == [cs["Nx"], cs["Ny"]]
+    got = ac.file_records(out, cs["Nx"], cs["Ny"])
+    direct = ac.run_kwhip(cs)
+    ref32 = ac.run_oracle(cs, "single")
+    for k in ("p", "p_max", "p_rms", "p_final", "ux", "uy", "ux_final", "uy_final"):
+        assert got[k].shape == np.squeeze(np.asarray(direct[k])).shape, k
+    # the file holds the inputs in single precision, so the operators built
+    # from them differ from the direct call's by single-precision rounding
+    e_direct = ac.compare(got, {k: np.squeeze(np.asarray(direct[k], dtype=np.float64))
+                                for k in got})
+    e32 = ac.compare(got, {k: np.squeeze(np.asarray(ref32[k], dtype=np.float64)) for k in got})
+    for k in got:
+        assert e_direct[k] <= TOL32, (k, e_direct[k])
+        assert e32[k] <= TOL32, (k, e32[k])
+
+
+def test_as_file_options(tmp_path, capsys):
+    import kwhip
+    from kwhip import _abi
+    cs = ac.case("homogeneous")
+    # 'WSWS' symmetries become 'WSWA' for the file (inputChecking.m:1402-1411)
+    out, v = ac.save_and_run_file(cs, tmp_path, _abi.load("emu"), record=("p",),
+                                  RadialSymmetry="WSWS")
+    assert "changed to 'WSWA'" in capsys.readouterr().out
+    ref = ac.run_oracle(dict(cs, sensor=dict(cs["sensor"], record=["p"])), "single")
+    got = np.squeeze(out["p"]).astype(np.float64)
+    assert np.max(np.abs(got - ref["p"])) <= TOL32 * np.max(np.abs(ref["p"]))
+    # SaveToDiskExit false runs the simulation as well
+    kg = kwhip.kWaveGrid(cs["Nx"], cs["dx"], cs["Ny"], cs["dx"])
+    kg.setTime(cs["Nt"], cs["dt"])
+    r = kwhip.kspaceFirstOrderAS(kg, cs["medium"], dict(cs["source"]), cs["sensor"],
+                                 PMLSize=cs["pml"], Smooth=False, SaveToDisk=str(tmp_path / "b.h5"),
+                                 SaveToDiskExit=False)
+    assert isinstance(r, dict) and (tmp_path / "b.h5").exists()
+    # the expanded layout is not a file layout, and the radial one is not run by kw_run
+    from kwhip.solver import prepare, _parse_options
+    pp = prepare(kg, cs["medium"], dict(cs["source"]), None, 2,
+                 _parse_options((), {"Smooth": False, "PMLSize": [6, 4]}))
+    pp.prob.axisymmetric = 1
+    lib = _abi.load("emu")
+    with pytest.raises(_abi.KWError, match="radial grid"):
+        import ctypes as C
+        _abi.call(lib.kw_save_problem, C.byref(pp.prob), str(tmp_path / "c.h5").encode())
+    pp.prob.axisymmetric = 2
+    out_arr = {}
+    with pytest.raises(_abi.KWError, match="kw_run_file"):
+        from kwhip.solver import run_prepared
+        out_arr = run_prepared(pp)
+    assert out_arr == {}

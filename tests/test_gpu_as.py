@@ -3,7 +3,8 @@ pressure, pressure sources in every mode, time reversal; lossless, Stokes,
 nonlinear) on a 128 x 64 grid against the oracle (relative L-inf <= 1e-5
 single, <= 1e-4 double), the spherical wave of a pulse centred on the axis against its
 closed form (the scheme's discretisation error, as the oracle measures it),
-and the engine's refusal of axisymmetric problems."""
+the engine's refusal of axisymmetric problems, and the file contract:
+'SaveToDisk' files run by kspaceFirstOrder-HIP against the oracle."""
 
 import numpy as np
 import pytest
@@ -60,3 +61,47 @@ def test_gpu_engine_refuses_axisymmetric(hip_lib):
         tid = C.c_int64()
         with pytest.raises(kwhip.KWError, match="axisymmetric"):
             _abi.call(eng.lib.kw_submit, eng.h, C.byref(pp.prob), C.byref(tid))
+
+
+@pytest.mark.parametrize("name", ["hetero", "source_additive", "stokes_nonlinear"])
+def test_gpu_as_command_line(hip_lib, tmp_path, name):
+    """kspaceFirstOrderAS(..., 'SaveToDisk') writes the radial grid's input
+    file (for one case re-written the way MATLAB writes it, chunked and
+    deflated), and kspaceFirstOrder-HIP runs it with the flags kspaceFirstOrder3DC.m passes;
+    the output file's records against the oracle (the kspaceFirstOrderASC
+    route)."""
+    import subprocess
+    import sys
+    from pathlib import Path
+    import kwhip
+    from kwhip import _abi
+    sys.path.insert(0, str(Path(__file__).resolve().parent / "golden"))
+    import h5lite
+    cs = ac.case(name, Nx=128, Ny=64, Nt=120)
+    cs["pml"] = [12, 10]
+    print(f"[as-cli] {name}", flush=True)
+    kg = kwhip.kWaveGrid(cs["Nx"], cs["dx"], cs["Ny"], cs["dx"])
+    kg.setTime(cs["Nt"], cs["dt"])
+    assert kwhip.kspaceFirstOrderAS(kg, dict(cs["medium"]), dict(cs["source"]), dict(cs["sensor"]),
+                                    PMLSize=cs["pml"], Smooth=False,
+                                    SaveToDisk=str(tmp_path / "in.h5")) == []
+    fn = tmp_path / "in.h5"
+    if name == "hetero":
+        # the test writer's chunk index holds one B-tree leaf: the grids of
+        # this case, not the long source signals of the others
+        v = h5lite.read(fn)
+        variables = {k: (a, "uint64" if a.dtype == np.uint64 else "single") for k, a in v.items()}
+        h5lite.write_input(tmp_path / "matlab.h5", variables, h5lite.read_attrs(fn), compression=4)
+        fn = tmp_path / "matlab.h5"
+    exe = _abi.LIB_DIR / "kspaceFirstOrder-HIP"
+    r = subprocess.run([str(exe), "-i", str(fn), "-o", str(tmp_path / "out.h5"),
+                        "--p_raw", "--p_max", "--p_rms", "--p_final", "--u_raw", "--u_final"],
+                       capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    got = ac.file_records(h5lite.read(tmp_path / "out.h5"), cs["Nx"], cs["Ny"])
+    for cast, tol in (("single", TOL32), ("off", TOL64)):
+        ref = ac.run_oracle(cs, cast)
+        errs = ac.compare(got, {k: np.squeeze(np.asarray(ref[k], dtype=np.float64)) for k in got})
+        print(f"[as-cli] {name} worst vs oracle ({cast}): {max(errs.values()):.2e}", flush=True)
+        for k, e in errs.items():
+            assert e <= tol, (k, cast, e)
