@@ -194,6 +194,10 @@ typedef struct kw_problem {
    * them), the layout of the file contract: kw_save_problem writes it with
    * axisymmetric_flag = 1 and kw_run_file lays such a file out on the
    * expanded grid and writes the radial grid's outputs; kw_run refuses 2.
+   * 3: the expanded grid of 'WSWS-FFT' (Ny = 2 x radial size - 2, p-like
+   * grids [f, flip(f(2:end-1))], y-staggered ones [f(1:end-1),
+   * flip(f(1:end-1))], kspaceFirstOrderAS.m:720-722, 966-977; sources and
+   * time-reversal points at columns j and 2Ny_r - 2 - j, both +).
    * 0 otherwise. */
   int32_t axisymmetric;
 } kw_problem;

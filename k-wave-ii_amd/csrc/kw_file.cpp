@@ -83,9 +83,9 @@ void save_problem(const kw_problem* pr, const char* path) {
   // an axisymmetric problem goes over on its radial grid (2), as
   // kspaceFirstOrderAS.m writes it with 'SaveToDisk'; the expanded grid that
   // kw_run steps (1) is not a file layout
-  if (pr->axisymmetric == 1)
+  if (pr->axisymmetric == 1 || pr->axisymmetric == 3)
     fail(KW_EINVAL, "an axisymmetric problem is written on its radial grid (axisymmetric = 2)");
-  if (pr->axisymmetric != 0 && pr->axisymmetric != 2) fail(KW_EINVAL, "axisymmetric must be 0, 1 or 2");
+  if (pr->axisymmetric != 0 && pr->axisymmetric != 2) fail(KW_EINVAL, "axisymmetric must be 0, 1, 2 or 3");
   if (pr->axisymmetric == 2 && dim != 2) fail(KW_EINVAL, "an axisymmetric problem is a 2D grid");
   const int64_t Nx = pr->Nx, Ny = pr->Ny, Nz = dim == 3 ? pr->Nz : 1;
   const int64_t n = Nx * Ny * Nz;

@@ -993,6 +993,7 @@ enum EwKind : int32_t {
   EW_RECORD_RHO,     // sum_d rho_d at the sensor points
   EW_ADD,            // ptr[i] += src[i] for i < n (floats)
   EW_AS_MIRROR,      // axisymmetric: copies 2-4 of a p-like field from copy 1
+  EW_AS_WS_COL,      // 'WSWS-FFT': the last radial column of u_y (which 0, 1, 2)
 };
 
 // EwArgs.rec value for "the problem's own statistics" (launches shared by
@@ -1007,7 +1008,8 @@ struct EwArgs {
   const float* p0;       // EW_INIT_P0; EW_AXIS_*: the source grid
   uint32_t rec;          // EW_RECORD_*: which statistics to update
   const float* prof;     // EW_AXIS_*: 1-D profile along axis `which`
-  float alpha;           // EW_AXIS_FOLD
+  float alpha;           // EW_AXIS_FOLD; EW_AS_WS_COL: the PML there
+  float beta;            // EW_AS_WS_COL: the PML of the column before
 };
 
 // coordinate of cell i along axis a (column-major, x fastest)
@@ -1065,6 +1067,36 @@ KW_BODY void body_ew(Ctx& c, const Batch& B, const EwArgs& A) {
         v = b == 1 ? -f : f;
       }
       KW_ST(A.ptr + i, v);
+      return;
+    }
+    case EW_AS_WS_COL: {
+      // 'WSWS-FFT' (kspaceFirstOrderAS.m:966-977) drops the last staggered
+      // radial point of u_y from its expanded layout (its place holds
+      // -u_y of the point before) but still updates and records it.  Row
+      // Ny/2 of the expanded grid holds the mirror value while the step's
+      // transforms read it, and the reference's value while it is recorded:
+      // which 0 puts the mirror value m back before a step; 2 (after the
+      // initial condition) and 1 (after an update) recover the gradient
+      // there from the mirror value's own update and advance the
+      // reference's value u with it, with the PML alpha (beta for the
+      // mirror) and prof = (1 / rho0_sgy there) / (1 / rho0_sgy of the
+      // mirror).  ptr = [u (Nx), m (Nx)].
+      if (i >= P.Nx) return;
+      float* row = P.u[1] + (int64_t)(P.Ny / 2) * P.Nx;
+      float* uc = A.ptr;
+      float* mc = A.ptr + P.Nx;
+      if (A.which == 0) {
+        KW_ST(row + i, KW_LD(mc + i));
+        return;
+      }
+      const float mn = KW_LD(row + i);
+      const float k = KW_LD(A.prof + i);
+      const float un = A.which == 2 ? k * mn
+                                    : A.alpha * (A.alpha * KW_LD(uc + i) -
+                                                 k * (A.beta * KW_LD(mc + i) - mn / A.beta));
+      KW_ST(mc + i, mn);
+      KW_ST(uc + i, un);
+      KW_ST(row + i, un);
       return;
     }
     case EW_ADD: {

@@ -329,14 +329,17 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream, Arena* ar
   P.eta = P.tau;
   P.eos = pr->absorbing_flag;
   // kspaceFirstOrderAS on its radially expanded grid (see kw_problem)
-  S->axisym = pr->axisymmetric != 0;
   if (pr->axisymmetric == 2)
     fail(KW_EINVAL, "axisymmetric = 2 (the radial grid of the file contract) is written by "
                     "kw_save_problem and run from the file by kw_run_file");
-  if (pr->axisymmetric < 0 || pr->axisymmetric > 1) fail(KW_EINVAL, "axisymmetric must be 0 or 1");
+  if (pr->axisymmetric < 0 || pr->axisymmetric > 3) fail(KW_EINVAL, "axisymmetric must be 0, 1 or 3");
+  S->axisym = pr->axisymmetric == 1 || pr->axisymmetric == 3;
+  S->as_ws = pr->axisymmetric == 3;
   if (S->axisym) {
     if (pdim != 2) fail(KW_EINVAL, "an axisymmetric problem is a 2D grid");
-    if (G.Ny % 4 != 0) fail(KW_EINVAL, "the expanded radial grid of 'WSWA-FFT' has 4 x Ny points");
+    if (!S->as_ws && G.Ny % 4 != 0) fail(KW_EINVAL, "the expanded radial grid of 'WSWA-FFT' has 4 x Ny points");
+    if (S->as_ws && (G.Ny % 2 != 0 || G.Ny < 4))
+      fail(KW_EINVAL, "the expanded radial grid of 'WSWS-FFT' has 2 x Ny - 2 points");
     if (!pr->use_sg) fail(KW_EINVAL, "Optional input 'UseSG' is not supported for axisymmetric simulations.");
     if (pr->absorbing_flag == 1)
       fail(KW_EUNSUP, "the axisymmetric code takes lossless or Stokes media");
@@ -405,8 +408,12 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream, Arena* ar
       // the fields are: p-like [f, f(No), flip(f(2:No)), f, f(No),
       // flip(f(2:No))] (the mirror columns hold p = 0), y-staggered [f,
       // flip(f), f, flip(f)]
-      if (pr->pml_size[1] > Nd[1] / 4) fail(KW_EINVAL, "PML size larger than the grid");
-      const int No = Nd[1] / 4;
+      // 'WSWS-FFT': [f, flip(f(2:No-1))] for p-like grids (every copy
+      // symmetric) and [f(1:No-1), flip(f(1:No-1))] for y-staggered ones
+      // (the last staggered point, which the reference updates but never
+      // feeds back, is the mirror of the one before)
+      const int No = S->as_ws ? Nd[1] / 2 + 1 : Nd[1] / 4;
+      if (pr->pml_size[1] > No) fail(KW_EINVAL, "PML size larger than the grid");
       auto rp = pml_profile(No, d[1], pr->dt, pr->c_ref, (int)pr->pml_size[1], pr->pml_alpha[1], false);
       auto rs = pml_profile(No, d[1], pr->dt, pr->c_ref, (int)pr->pml_size[1], pr->pml_alpha[1], true);
       const int ps = (int)pr->pml_size[1];
@@ -414,7 +421,16 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream, Arena* ar
         for (int i = 0; i < ps; ++i) rp[i] = rs[i] = 1.0;
       else
         fail(KW_EUNSUP, "an axisymmetric radial PML wider than half the radial grid");
+      if (S->as_ws) {
+        S->as_a = rs[No - 1];
+        S->as_ap = rs[No - 2];
+      }
       for (int j = 0; j < Nd[1]; ++j) {
+        if (S->as_ws) {
+          pr_[j] = j < No ? rp[j] : rp[2 * No - 2 - j];
+          ps_[j] = j < No - 1 ? rs[j] : rs[2 * No - 3 - j];
+          continue;
+        }
         const int b = j / No, r = j % No;
         pr_[j] = (b & 1) == 0 ? rp[r] : (r == 0 ? rp[No - 1] : rp[No - r]);
         ps_[j] = (b & 1) == 0 ? rs[r] : rs[No - 1 - r];
@@ -477,14 +493,38 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream, Arena* ar
     S->as_kappa = S->spectral_table([&](int ix, int iy, int iz) {
       return (use_k ? sinc(cref * kmag(ix, iy, iz) * dt / 2) : 1.0) / (double)G.Ny;
     });
-    const int No = G.Ny / 4;
+    // ir: + on the copies that hold u_r, - on the mirrored ones, so that
+    // ir .* u_r has the even layout of u_r / r (:966-977)
+    const int No = S->as_ws ? (int)G.Ny / 2 + 1 : (int)G.Ny / 4;
     std::vector<float> ir(G.Ny);
     for (int j = 0; j < G.Ny; ++j) {
-      const int b = j / No, r = j % No;
-      const double rsg = ((b & 1) == 0 ? r : No - 1 - r) * d[1] + d[1] / 2;
-      ir[j] = (float)(((b & 1) == 0 ? 1.0 : -1.0) / rsg);
+      int r;
+      bool first;
+      if (S->as_ws) {
+        first = j < No - 1;
+        r = first ? j : 2 * No - 3 - j;
+      } else {
+        first = ((j / No) & 1) == 0;
+        r = first ? j % No : No - 1 - j % No;
+      }
+      ir[j] = (float)((first ? 1.0 : -1.0) / (r * d[1] + d[1] / 2));
     }
     S->as_ir = S->upload(ir);
+    if (S->as_ws) {
+      // kspaceFirstOrderAS.m:440-456 leaves rho0_sgy of the last radial
+      // column at rho0 there (the interpolation past the grid is NaN); the
+      // expanded grid holds rho0_sgy of the column before in its place
+      const int64_t rl = (int64_t)(No - 1) * G.Nx, rp = (int64_t)(No - 2) * G.Nx;
+      std::vector<float> k(G.Nx);
+      for (int64_t x = 0; x < G.Nx; ++x) {
+        const double rho_last = pr->rho0 ? pr->rho0[x + rl] : pr->rho0_scalar;
+        const double sg_prev = pr->rho0_sg[1] ? pr->rho0_sg[1][x + rp]
+                               : pr->rho0 ? pr->rho0[x + rp] : pr->rho0_scalar;
+        k[x] = (float)(sg_prev / rho_last);
+      }
+      S->as_k = S->upload(k);
+      S->as_col = S->alloc_zero<float>(2 * (size_t)G.Nx);
+    }
   }
   if (P.eos == EOS_ABSORBING) {
     const bool no_abs = pr->alpha_mode == 1, no_disp = pr->alpha_mode == 2;
@@ -984,6 +1024,17 @@ static void mirror_copies(Sim& S) {
   r2c_p(S);
 }
 
+static void ws_column(Sim& S, int which) {
+  EwArgs e{};
+  e.kind = EW_AS_WS_COL;
+  e.which = which;
+  e.ptr = S.as_col;
+  e.prof = S.as_k;
+  e.alpha = (float)S.as_a;
+  e.beta = (float)S.as_ap;
+  S.ew(e, (int)((S.G.Nx + EW_T - 1) / EW_T));
+}
+
 static void divergence_radial(Sim& S) {
   EwArgs e{};
   e.kind = EW_ADD;
@@ -1025,6 +1076,7 @@ void step_once(Sim& S) {
   const int t = (int)(S.steps_done + 1);
   const int dim = S.G.dim;
   const bool last = S.steps_done + 1 == S.Nt_eff;
+  if (S.as_ws && S.steps_done > 0) ws_column(S, 0);
   if (t == 1 && S.has_p0) {
     // kspaceFirstOrder3D.m:1023-1037: the first step's update is replaced
     // by the initial condition; the sensor records p0
@@ -1034,6 +1086,7 @@ void step_once(Sim& S) {
     S.ew(e, S.lcell.nb);
     r2c_p(S);
     gradient(S, XG_INIT);
+    if (S.as_ws) ws_column(S, 2);
     record_step(S, t, true);
     r2c_p(S);
   } else {
@@ -1048,7 +1101,7 @@ void step_once(Sim& S) {
                         (last && (S.record & KW_REC_P_FINAL));
     // a nonlinear axisymmetric step rewrites the mirrored copies of p from
     // the stored field
-    const int pflag = (need_p || (S.axisym && S.P.nonlinear)) ? X_STORE_P : 0;
+    const int pflag = (need_p || (S.axisym && !S.as_ws && S.P.nonlinear)) ? X_STORE_P : 0;
     if (S.p_grid) src_grid(S, 0, t);
     for (int a = 0; a < dim; ++a)
       if (S.u_grid[a]) src_grid(S, a + 1, t);
@@ -1076,7 +1129,9 @@ void step_once(Sim& S) {
       xe.flags = pflag;
       S.pass(PK_X_EOS, xe);
     }
-    if (S.axisym && S.P.nonlinear) mirror_copies(S);
+    // 'WSWS-FFT' copies are all +rho, which the even nonlinear terms keep
+    if (S.axisym && !S.as_ws && S.P.nonlinear) mirror_copies(S);
+    if (S.as_ws) ws_column(S, 1);
     record_step(S, t, false);
   }
   S.steps_done++;

@@ -252,9 +252,17 @@ struct Sim {
     float* pml_sg_new[3] = {nullptr, nullptr, nullptr};
   } dtc;
   float* scratch_grid = nullptr;  // one real grid (the split correction)
-  // axisymmetric (WSWA-FFT) problems: the signed inverse radius along the
-  // expanded y axis and κ / Ny for the u_r / r term
+  // axisymmetric problems: the signed inverse radius along the expanded y
+  // axis and κ / Ny for the u_r / r term; as_ws: the 'WSWS-FFT' layout
+  // (2 No - 2 points, every copy symmetric) instead of 'WSWA-FFT' (4 No)
   bool axisym = false;
+  bool as_ws = false;
+  // 'WSWS-FFT': the reference's last radial column of u_y and the mirror
+  // value the transforms read ([2 Nx]), its coefficient per x, and the
+  // staggered radial PML at that column and the one before
+  float* as_col = nullptr;
+  float* as_k = nullptr;
+  double as_a = 1.0, as_ap = 1.0;
   float* as_ir = nullptr;
   float* as_kappa = nullptr;
   // per-kernel event timing

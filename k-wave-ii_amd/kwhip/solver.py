@@ -1219,14 +1219,21 @@ def kspaceFirstOrder3D(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
 # kspaceFirstOrderAS
 # ---------------------------------------------------------------------------
 
-def _as_mirror(f, kind, signed):
+def _as_mirror(f, kind, signed, ws=False):
     """The layout of a radial-grid field on the expanded grid of the
     'WSWA-FFT' symmetry (kspaceFirstOrderAS.m:703-707, 944-952): p-like
     [f, 0, -flip(f(2:end)), -f, 0, flip(f(2:end))], y-staggered [f, flip(f),
     -f, -flip(f)]; unsigned (material grids) every copy keeps its sign and
-    the mirror columns of p-like grids repeat their neighbour."""
+    the mirror columns of p-like grids repeat their neighbour.  ws: the
+    'WSWS-FFT' grid of 2 Ny - 2 points (:720-722, 966-977), p-like
+    [f, flip(f(2:end-1))], y-staggered [f(1:end-1), -flip(f(1:end-1))]
+    (unsigned: +flip)."""
     f = np.asarray(f, dtype=np.float64)
     nx, ny = f.shape
+    if ws:
+        if kind == "p":
+            return np.concatenate([f, f[:, -2:0:-1]], axis=1)
+        return np.concatenate([f[:, :-1], (-1.0 if signed else 1.0) * f[:, -2::-1]], axis=1)
     e = np.zeros((nx, 4 * ny))
     sg = (1, -1, -1, 1) if kind == "p" else (1, 1, -1, -1)
     if not signed:
@@ -1244,16 +1251,23 @@ def _as_mirror(f, kind, signed):
     return e
 
 
-def _as_points(pos, nx, ny):
+def _as_points(pos, nx, ny, ws=False):
     """Points of the radial grid (column-major linear indices) on the
     expanded grid of the p-like 'WSWA-FFT' layout: each point (i, j) at
     column j (sign +), 2 Ny - j (-), 2 Ny + j (-) and 4 Ny - j (+), the
-    second and fourth copies skipping j = 0, whose mirror column stays 0.
+    second and fourth copies skipping j = 0, whose mirror column stays 0
+    ('WSWS-FFT': column j and, for 0 < j < Ny - 1, 2 Ny - 2 - j, both +).
     Returns the expanded indices in column-major order, the radial point
     each came from and its sign."""
     pos = np.asarray(pos, dtype=np.int64)
     ix, iy = pos % nx, pos // nx
     src = np.arange(pos.size)
+    if ws:
+        off = (iy >= 1) & (iy <= ny - 2)
+        lin = np.concatenate([pos, ix[off] + nx * (2 * ny - 2 - iy[off])])
+        row = np.concatenate([src, src[off]])
+        order = np.argsort(lin, kind="stable")
+        return lin[order], row[order], np.ones(lin.size)
     cols, rows, signs = [iy, 2 * ny + iy], [src, src], [np.ones(pos.size), -np.ones(pos.size)]
     off = iy >= 1
     cols += [2 * ny - iy[off], 4 * ny - iy[off]]
@@ -1266,7 +1280,7 @@ def _as_points(pos, nx, ny):
     return lin[order], row[order], sign[order]
 
 
-def _as_pressure_source(src, med, kgrid, scale):
+def _as_pressure_source(src, med, kgrid, scale, ws=False):
     """source.p of an axisymmetric simulation on the expanded grid: one
     series per expanded point, signed as the p-like layout, scaled first on
     the radial grid as kspaceFirstOrder_scaleSourceTerms.m:52-129 scales
@@ -1318,10 +1332,11 @@ def _as_pressure_source(src, med, kgrid, scale):
                 sig[i] /= 2 * c_at(i) ** 2
             else:
                 sig[i] *= 2 * dt / (2 * c_at(i) * kgrid.dx)
-    epos, row, sign = _as_points(pos, Nx, Ny)
-    emask = np.zeros(Nx * 4 * Ny, dtype=bool)
+    epos, row, sign = _as_points(pos, Nx, Ny, ws)
+    M = 2 * Ny - 2 if ws else 4 * Ny
+    emask = np.zeros(Nx * M, dtype=bool)
     emask[epos] = True
-    emask = emask.reshape((Nx, 4 * Ny), order="F")
+    emask = emask.reshape((Nx, M), order="F")
     return {"p_mask": emask, "p": sign[:, None] * sig[sig_idx[row]], "p_mode": mode}
 
 
@@ -1331,15 +1346,16 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
     the radial coordinate, the axis at the first y index.  Runs the 2-D
     kernels on the radially expanded grid of the 'WSWA-FFT' symmetry (the
     default; 'WSWA' gives the same field, the reference's discrete
-    trigonometric transforms agreeing with it to machine precision) with the
-    u_r / r term of the divergence added by the library
-    (kw_problem.axisymmetric).  Supported here: source.p0 or a pressure
+    trigonometric transforms agreeing with it to machine precision) or of
+    'WSWS-FFT' ('WSWS' likewise) with the u_r / r term of the divergence
+    added by the library (kw_problem.axisymmetric 1 or 3).
+    Supported here: source.p0 or a pressure
     source (every p_mode, laid out on the expanded grid with the signs of
     the p-like layout), lossless or Stokes media (with or without B/A),
     binary sensor masks and time reversal; 'SaveToDisk' writes the radial
     grid's input file (axisymmetric_flag 1, 'WSWA'), which kw_run_file and
-    kspaceFirstOrder-HIP run; 'WSWS' symmetries, velocity sources and
-    'PMLInside' false are refused."""
+    kspaceFirstOrder-HIP run; velocity sources and 'PMLInside' false are
+    refused."""
     opts = _parse_options(args, kwargs)
     sym = opts.pop("RadialSymmetry", "WSWA-FFT")
     if sym not in ("WSWA", "WSWS", "WSWA-FFT", "WSWS-FFT"):
@@ -1352,8 +1368,7 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
             print("  WARNING: Optional input 'RadialSymmetry' changed to 'WSWA' for compatibility "
                   "with 'SaveToDisk'.")
         sym = "WSWA"
-    if sym in ("WSWS", "WSWS-FFT"):
-        raise NotImplementedError("'RadialSymmetry' 'WSWS' and 'WSWS-FFT' are not supported by kwhip")
+    ws = sym in ("WSWS", "WSWS-FFT")
     if getattr(kgrid, "dim", 2) != 2:
         raise ValueError("kspaceFirstOrderAS takes a 2D kWaveGrid (x axial, y radial).")
     if not opts.get("UseSG", True):
@@ -1389,6 +1404,9 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
         if m.shape != (kgrid.Nx, kgrid.Ny):
             raise NotImplementedError("axisymmetric simulations in kwhip take a binary sensor mask")
     Nx, Ny = kgrid.Nx, kgrid.Ny
+    if ws and Ny < 3:
+        raise ValueError("'WSWS' needs at least 3 radial grid points")
+    M = 2 * Ny - 2 if ws else 4 * Ny   # expanded radial size
     sm = opts.pop("Smooth", True)
     esrc = {}
     p0 = None
@@ -1399,10 +1417,10 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
         # 1504-1535), trimmed back, then laid out again
         p0 = np.asarray(src["p0"], dtype=np.float64)
         if bool(np.atleast_1d(sm)[0]):
-            p0 = smooth(_as_mirror(p0, "p", True), True)[:, :Ny]
-        esrc["p0"] = _as_mirror(p0, "p", True)
+            p0 = smooth(_as_mirror(p0, "p", True, ws), True)[:, :Ny]
+        esrc["p0"] = _as_mirror(p0, "p", True, ws)
     if "p" in src:
-        esrc.update(_as_pressure_source(src, med, kgrid, opts.get("ScaleSourceTerms", True)))
+        esrc.update(_as_pressure_source(src, med, kgrid, opts.get("ScaleSourceTerms", True), ws))
     if np.size(np.atleast_1d(sm)) > 1 and any(np.atleast_1d(sm)[1:]):
         raise NotImplementedError("smoothing c0 or rho0 is not supported by kwhip for "
                                   "axisymmetric simulations")
@@ -1422,12 +1440,12 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
         save_to_disk(rp, save)
         if opts.get("SaveToDiskExit", True):
             return []
-    kg = kWaveGrid(Nx, kgrid.dx, 4 * Ny, kgrid.dy)
+    kg = kWaveGrid(Nx, kgrid.dx, M, kgrid.dy)
     kg.setTime(kgrid.Nt, kgrid.dt)
     emed = {}
     for k, v in med.items():
         if k in ("sound_speed", "density", "BonA", "alpha_coeff") and np.size(v) > 1:
-            emed[k] = _as_mirror(v, "p", False)
+            emed[k] = _as_mirror(v, "p", False, ws)
         else:
             emed[k] = v
     if "sound_speed_ref" not in emed:
@@ -1435,18 +1453,18 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
     esen = None
     if sen is not None:
         esen = dict(sen)
-        mask = np.zeros(Nx * 4 * Ny, dtype=bool)
+        mask = np.zeros(Nx * M, dtype=bool)
         if time_rev:
             # the boundary data are enforced on every copy, with its sign
             pos = _find(np.asarray(sen["mask"]))
-            epos, row, sign = _as_points(pos, Nx, Ny)
+            epos, row, sign = _as_points(pos, Nx, Ny, ws)
             data = np.atleast_2d(np.asarray(sen["time_reversal_boundary_data"], dtype=np.float64))
             if data.shape[0] != pos.size:
                 raise ValueError("The number of time series in sensor.time_reversal_boundary_data "
                                  "must match the number of sensor points.")
             mask[epos] = True
             esen["time_reversal_boundary_data"] = sign[:, None] * data[row]
-        mask = mask.reshape((Nx, 4 * Ny), order="F")
+        mask = mask.reshape((Nx, M), order="F")
         if not time_rev:
             mask[:, :Ny] = np.asarray(sen["mask"]).astype(bool)
         esen["mask"] = mask
@@ -1459,7 +1477,7 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
     # the pressure signals come scaled with the radial grid's sound speed
     opts["ScaleSourceTerms"] = False
     pp = prepare(kg, emed, esrc, esen, 2, opts)
-    pp.prob.axisymmetric = 1
+    pp.prob.axisymmetric = 3 if ws else 1
     rho = np.asarray(med["density"], dtype=np.float64)
     if rho.size > 1:
         # kspaceFirstOrderAS.m:440-456 on the radial grid, then laid out
@@ -1467,8 +1485,8 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
         rsx[:-1, :] = (rho[:-1, :] + rho[1:, :]) / 2
         rsy = rho.copy()
         rsy[:, :-1] = (rho[:, :-1] + rho[:, 1:]) / 2
-        ex = np.asfortranarray(_as_mirror(rsx, "p", False)).reshape(-1, order="F")
-        ey = np.asfortranarray(_as_mirror(rsy, "u", False)).reshape(-1, order="F")
+        ex = np.asfortranarray(_as_mirror(rsx, "p", False, ws)).reshape(-1, order="F")
+        ey = np.asfortranarray(_as_mirror(rsy, "u", False, ws)).reshape(-1, order="F")
         ex, ey = np.ascontiguousarray(ex), np.ascontiguousarray(ey)
         pp.keep += [ex, ey]
         pp.prob.rho0_sg[0] = _abi.dptr(ex)

@@ -1428,9 +1428,23 @@ def as_mirror_u(f, signs=(1, 1, -1, -1)):
     return e
 
 
+def as_mirror_p_ws(f):
+    """kspaceFirstOrderAS.m:720-722 ('WSWS-FFT'): a p-like field on the
+    expanded grid of 2 Ny - 2 points, [f, flip(f(2:end-1))] along y."""
+    return np.concatenate([f, f[:, -2:0:-1]], axis=1)
+
+
+def as_mirror_u_ws(f, sign=-1):
+    """kspaceFirstOrderAS.m:966-977 ('WSWS-FFT'): a y-staggered field
+    without its last point, [f(1:end-1), sign flip(f(1:end-1))]; sign -1
+    for u_r (HAHA), +1 for u_r / r (HSHS)."""
+    return np.concatenate([f[:, :-1], sign * f[:, -2::-1]], axis=1)
+
+
 def kspace_first_order_as(kgrid: KGrid, medium, source, sensor, *, data_cast="off",
-                          pml_size=20, pml_alpha=2.0, smooth=True):
-    """kspaceFirstOrderAS.m:355-1435 with 'RadialSymmetry' 'WSWA-FFT' for the
+                          pml_size=20, pml_alpha=2.0, smooth=True, radial_symmetry="WSWA-FFT"):
+    """kspaceFirstOrderAS.m:355-1435 with 'RadialSymmetry' 'WSWA-FFT' (or
+    'WSWS-FFT', the expanded grid of 2 Ny - 2 points) for the
     cases the library runs: an initial pressure or a pressure source (every
     p_mode, :1055-1128, and p_frequency_ref), lossless or Stokes media
     (linear or with B/A; the axisymmetric code takes medium.alpha_coeff as
@@ -1457,10 +1471,21 @@ def kspace_first_order_as(kgrid: KGrid, medium, source, sensor, *, data_cast="of
         tau = (-2.0 * db2neper(_get(medium, "alpha_coeff"), 2) * c0)
     p0 = source.get("p0")
     p0 = None if p0 is None else np.asarray(p0, dtype=np.float64)
+    ws = radial_symmetry == "WSWS-FFT"
+    if not ws and radial_symmetry != "WSWA-FFT":
+        raise ValueError(radial_symmetry)
+    mirror_p = as_mirror_p_ws if ws else as_mirror_p
+    if ws:
+        mirror_u = lambda f: as_mirror_u_ws(f, -1)  # noqa: E731
+        mirror_u_on_y = lambda f: as_mirror_u_ws(f, 1)  # noqa: E731
+    else:
+        mirror_u = as_mirror_u
+        mirror_u_on_y = lambda f: as_mirror_u(f, (1, -1, -1, 1))  # noqa: E731
+    M = 2 * Ny - 2 if ws else 4 * Ny
     if p0 is not None and smooth:
         # kspaceFirstOrder_inputChecking.m:1504-1535: smoothed on the
         # expanded grid (WS origin symmetry), trimmed back
-        p0 = smooth_mat(as_mirror_p(p0), restore_max=True)[:, :Ny]
+        p0 = smooth_mat(mirror_p(p0), restore_max=True)[:, :Ny]
     # pressure source (kspaceFirstOrderAS.m:1055-1128) with the scaling of
     # scaleSourceTerms.m for a 2-D grid (row i scaled with c0 at source
     # point i, the reference's quirk)
@@ -1495,7 +1520,7 @@ def kspace_first_order_as(kgrid: KGrid, medium, source, sensor, *, data_cast="of
     pml_y[:py] = 1.0  # getPML.m: axisymmetric, no PML at the axis
     pml_y_sg[:py] = 1.0
     kx = make_dim(Nx, dx)
-    ky = make_dim(4 * Ny, dy)
+    ky = make_dim(M, dy)
     k = np.sqrt(kx[:, None] ** 2 + ky[None, :] ** 2)
     kappa = np.fft.ifftshift(sinc(c_ref * k * dt / 2)).astype(rdt)
     ddx_pos = np.fft.ifftshift(1j * kx * np.exp(1j * kx * dx / 2))[:, None].astype(cdt)
@@ -1566,7 +1591,7 @@ def kspace_first_order_as(kgrid: KGrid, medium, source, sensor, *, data_cast="of
         return f.reshape(a.shape, order="F").astype(rdt)
 
     def grad(pp):
-        pk = kappa * fft2(as_mirror_p(pp))
+        pk = kappa * fft2(mirror_p(pp))
         return ifft2(ddx_pos * pk)[:, :Ny], ifft2(ddy_k * y_pos * pk)[:, :Ny]
 
     def pick(f):
@@ -1581,9 +1606,9 @@ def kspace_first_order_as(kgrid: KGrid, medium, source, sensor, *, data_cast="of
         dpdx, dpdy = grad(p)
         ux = pml_x_sg * (pml_x_sg * ux - dtc * rsx_inv * dpdx)
         uy = pml_y_sg * (pml_y_sg * uy - dtc * rsy_inv * dpdy)
-        duxdx = ifft2(ddx_neg * kappa * fft2(as_mirror_p(ux)))[:, :Ny]
-        u_on_y = as_mirror_u(inv_y * uy, (1, -1, -1, 1))
-        duydy = ifft2(kappa * y_neg * (ddy_k * fft2(as_mirror_u(uy)) + fft2(u_on_y)))[:, :Ny]
+        duxdx = ifft2(ddx_neg * kappa * fft2(mirror_p(ux)))[:, :Ny]
+        u_on_y = mirror_u_on_y(inv_y * uy)
+        duydy = ifft2(kappa * y_neg * (ddy_k * fft2(mirror_u(uy)) + fft2(u_on_y)))[:, :Ny]
         if nonlinear:
             k_ = 2 * (rx + ry) + rho0c
         else:
@@ -1600,7 +1625,7 @@ def kspace_first_order_as(kgrid: KGrid, medium, source, sensor, *, data_cast="of
                 sm_ = np.zeros((Nx, Ny)).reshape(-1, order="F")
                 sm_[pos] = v
                 sm_ = sm_.reshape((Nx, Ny), order="F")
-                add = ifft2(source_kappa * fft2(as_mirror_p(sm_)))[:, :Ny]
+                add = ifft2(source_kappa * fft2(mirror_p(sm_)))[:, :Ny]
                 rx = (rx + add).astype(rdt)
                 ry = (ry + add).astype(rdt)
             else:

@@ -1,10 +1,11 @@
 """Cases of kspaceFirstOrderAS (the axisymmetric solver), shared by the
-emulator and GPU tests: x axial, y radial from the axis."""
+emulator and GPU tests: x axial, y radial from the axis; 'WSWA-FFT' unless
+a case says otherwise."""
 
 import numpy as np
 
 
-def case(name, Nx=32, Ny=16, seed=2, Nt=24):
+def case(name, Nx=32, Ny=16, seed=2, Nt=24, sym="WSWA-FFT"):
     rng = np.random.default_rng(seed)
     dx = 1e-4
     x = (np.arange(Nx) - Nx // 2) * dx
@@ -17,7 +18,7 @@ def case(name, Nx=32, Ny=16, seed=2, Nt=24):
           "p0": np.exp(-(X ** 2 + R ** 2) / (3 * dx) ** 2),
           "sensor": {"mask": rng.random((Nx, Ny)) < 0.3,
                      "record": ["p", "p_max", "p_rms", "p_final", "u", "u_final"]},
-          "pml": [6, 4], "smooth": False}
+          "pml": [6, 4], "smooth": False, "sym": sym}
     cs["dt"] = 0.25 * dx / c.max()
     cs["source"] = {"p0": cs.pop("p0")}
     t = np.arange(cs["Nt"]) * cs["dt"]
@@ -70,6 +71,9 @@ def case(name, Nx=32, Ny=16, seed=2, Nt=24):
 NAMES = ["homogeneous", "hetero", "nonlinear", "stokes", "stokes_nonlinear", "smoothed",
          "source_additive", "source_dirichlet", "source_labelled", "source_frequency_ref",
          "source_stokes_nonlinear", "time_reversal"]
+# the cases run with 'WSWS-FFT' as well
+WS_NAMES = ["hetero", "stokes_nonlinear", "smoothed", "source_additive", "source_dirichlet",
+            "time_reversal"]
 
 
 def run_kwhip(cs):
@@ -77,7 +81,8 @@ def run_kwhip(cs):
     kg = kwhip.kWaveGrid(cs["Nx"], cs["dx"], cs["Ny"], cs["dx"])
     kg.setTime(cs["Nt"], cs["dt"])
     out = kwhip.kspaceFirstOrderAS(kg, dict(cs["medium"]), dict(cs["source"]), dict(cs["sensor"]),
-                                   PMLSize=cs["pml"], Smooth=cs["smooth"])
+                                   PMLSize=cs["pml"], Smooth=cs["smooth"],
+                                   RadialSymmetry=cs.get("sym", "WSWA-FFT"))
     return {"p_final": out} if "time_reversal_boundary_data" in cs["sensor"] else out
 
 
@@ -85,8 +90,10 @@ def run_oracle(cs, data_cast):
     from oracle import kwave_oracle as ko
     kg = ko.KGrid(cs["Nx"], cs["dx"], cs["Ny"], cs["dx"])
     kg.set_time(cs["Nt"], cs["dt"])
+    sym = "WSWS-FFT" if cs.get("sym", "WSWA-FFT").startswith("WSWS") else "WSWA-FFT"
     return ko.kspace_first_order_as(kg, cs["medium"], cs["source"], cs["sensor"],
-                                    data_cast=data_cast, pml_size=cs["pml"], smooth=cs["smooth"])
+                                    data_cast=data_cast, pml_size=cs["pml"], smooth=cs["smooth"],
+                                    radial_symmetry=sym)
 
 
 def compare(got, ref):

@@ -35,6 +35,22 @@ def test_as_matches_oracle(name):
         assert e64[k] <= TOL64, (k, e64[k])
 
 
+@pytest.mark.parametrize("name", ac.WS_NAMES)
+def test_as_wsws_matches_oracle(name):
+    cs = ac.case(name, sym="WSWS-FFT")
+    got = ac.run_kwhip(cs)
+    e32 = ac.compare(got, ac.run_oracle(cs, "single"))
+    e64 = ac.compare(got, ac.run_oracle(cs, "off"))
+    for k in e32:
+        assert e32[k] <= TOL32, (k, e32[k])
+        assert e64[k] <= TOL64, (k, e64[k])
+    # 'WSWS' (the reference's discrete trigonometric transforms) gives the
+    # field of 'WSWS-FFT'
+    if name == "hetero":
+        again = ac.run_kwhip(dict(cs, sym="WSWS"))
+        assert np.array_equal(np.asarray(again["p"]), np.asarray(got["p"]))
+
+
 def test_as_option_checks():
     import kwhip
     cs = ac.case("homogeneous")
@@ -43,8 +59,6 @@ def test_as_option_checks():
     args = (kg, cs["medium"], dict(cs["source"]), cs["sensor"])
     with pytest.raises(ValueError, match="'RadialSymmetry' must be set to 'WSWA', 'WSWS'"):
         kwhip.kspaceFirstOrderAS(*args, RadialSymmetry="WSW")
-    with pytest.raises(NotImplementedError):
-        kwhip.kspaceFirstOrderAS(*args, RadialSymmetry="WSWS-FFT")
     with pytest.raises(ValueError, match="'UseSG' is not supported for axisymmetric"):
         kwhip.kspaceFirstOrderAS(*args, UseSG=False)
     with pytest.raises(ValueError, match="medium.alpha_mode is not supported with the axisymmetric"):

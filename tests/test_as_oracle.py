@@ -7,6 +7,9 @@
   pulse is better resolved (w = 4, 6, 8 points);
 * the same run with the sign of the u_r / r term flipped (a plausible
   mistake, planted in the oracle here) is off by order one;
+* 'WSWS-FFT' (the expanded grid of 2 Ny - 2 points) follows the same
+  spherical wave to the same error, and either of its velocity layouts with
+  the wrong parity is off by 40 to 500 times that error;
 * an on-axis Gaussian pressure source gives the field of the 3-D solver
   (itself pinned by tests/test_oracle_*.py) on a Cartesian grid, to the
   axisymmetric discretisation error, for 'additive' and
@@ -25,20 +28,21 @@ import as_cases as ac
 from oracle import kwave_oracle as ko
 
 
-def _run(Nx, Ny, dx, c0, Nt, dt, w):
+def _run(Nx, Ny, dx, c0, Nt, dt, w, sym="WSWA-FFT"):
     p0, exact, inside = ac.spherical_wave(Nx, Ny, dx, c0, Nt, dt, w)
     kg = ko.KGrid(Nx, dx, Ny, dx)
     kg.set_time(Nt, dt)
     pf = ko.kspace_first_order_as(kg, {"sound_speed": c0, "density": 1000.0}, {"p0": p0},
                                   {"mask": np.ones((Nx, Ny), bool), "record": ["p_final"]},
-                                  pml_size=16, smooth=False)["p_final"]
+                                  pml_size=16, smooth=False, radial_symmetry=sym)["p_final"]
     return float(np.max(np.abs(pf - exact)[inside]) / np.max(np.abs(exact)))
 
 
-def test_spherical_wave_converges():
+@pytest.mark.parametrize("sym", ["WSWA-FFT", "WSWS-FFT"])
+def test_spherical_wave_converges(sym):
     Nx, Ny, dx, c0 = 128, 64, 1e-4, 1500.0
     dt = 0.2 * dx / c0
-    errs = [_run(Nx, Ny, dx, c0, 60, dt, w * dx) for w in (4, 6, 8)]
+    errs = [_run(Nx, Ny, dx, c0, 60, dt, w * dx, sym) for w in (4, 6, 8)]
     assert errs[0] < 1.5e-3 and errs[1] < 6e-4 and errs[2] < 3e-4, errs
     assert errs[0] > errs[1] > errs[2], errs
 
@@ -54,6 +58,23 @@ def test_planted_sign_error_is_caught(monkeypatch):
     monkeypatch.setattr(ko, "as_mirror_u", flipped)
     Nx, Ny, dx, c0 = 128, 64, 1e-4, 1500.0
     assert _run(Nx, Ny, dx, c0, 60, 0.2 * dx / c0, 6 * dx) > 0.1
+
+
+@pytest.mark.parametrize("which", ["u_r", "u_r / r"])
+def test_planted_wsws_mirror_error_is_caught(monkeypatch, which):
+    # the 'WSWS-FFT' layouts of u_r (odd about the axis) and of u_r / r
+    # (even), each with the other parity planted
+    orig = ko.as_mirror_u_ws
+    flip = -1 if which == "u_r" else 1
+
+    def planted(f, sign=-1):
+        return orig(f, -sign if sign == flip else sign)
+
+    monkeypatch.setattr(ko, "as_mirror_u_ws", planted)
+    Nx, Ny, dx, c0 = 128, 64, 1e-4, 1500.0
+    # the scheme's own error at this width is 3.9e-4; u_r even about the
+    # axis is 1.6e-2 off, u_r / r odd 0.21
+    assert _run(Nx, Ny, dx, c0, 60, 0.2 * dx / c0, 6 * dx, "WSWS-FFT") > 5e-3
 
 
 # -- pressure sources and time reversal ---------------------------------------
