@@ -187,9 +187,10 @@ typedef struct kw_problem {
    * is the outer radial PML of the radial grid.  The library then builds
    * the y PML on every copy (getPML.m with axisymmetric true) and adds the
    * u_r / r term of the divergence (:954-969).  Lossless or Stokes media
-   * (with or without B/A), p0, pressure sources and time-reversal data
-   * given on the expanded grid with the signs of the p-like layout, and
-   * binary sensors; velocity sources return KW_EUNSUP.  2: the radial grid
+   * (with or without B/A), p0, pressure and axial velocity sources and
+   * time-reversal data given on the expanded grid with the signs of the
+   * p-like layout, radial velocity sources with those of the y-staggered
+   * layout, and binary sensors.  2: the radial grid
    * itself (Ny the radial size, inputs as kspaceFirstOrderAS.m prepares
    * them), the layout of the file contract: kw_save_problem writes it with
    * axisymmetric_flag = 1 and kw_run_file lays such a file out on the
@@ -197,8 +198,8 @@ typedef struct kw_problem {
    * 3: the expanded grid of 'WSWS-FFT' (Ny = 2 x radial size - 2, p-like
    * grids [f, flip(f(2:end-1))], y-staggered ones [f(1:end-1),
    * flip(f(1:end-1))], kspaceFirstOrderAS.m:720-722, 966-977; sources and
-   * time-reversal points at columns j and 2Ny_r - 2 - j, both +).
-   * 0 otherwise. */
+   * time-reversal points at columns j and 2Ny_r - 2 - j, both +; radial
+   * velocity sources return KW_EUNSUP).  0 otherwise. */
   int32_t axisymmetric;
 } kw_problem;
 

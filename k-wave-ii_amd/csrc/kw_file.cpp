@@ -367,13 +367,15 @@ void load_problem(const kwh5::Reader& r, Loaded& L) {
 // p-like grids as [f, 0, -flip(f(2:end)), -f, 0, flip(f(2:end))] along y
 // (kspaceFirstOrderAS.m:703-707), the y-staggered density as [f, flip(f),
 // f, flip(f)], material grids unsigned with each mirror column repeating
-// its neighbour, and every pressure-source point (i, j) as the points
-// (i, j), (i, 2Ny - j), (i, 2Ny + j), (i, 4Ny - j) with signs +, -, -, +
-// (the second and fourth left out for j = 0) and a series of its own.
+// its neighbour, and every pressure-source and axial-velocity-source point
+// (i, j) as the points (i, j), (i, 2Ny - j), (i, 2Ny + j), (i, 4Ny - j)
+// with signs +, -, -, + (the second and fourth left out for j = 0), a
+// radial-velocity-source point as (i, j), (i, 2Ny - 1 - j), (i, 2Ny + j),
+// (i, 4Ny - 1 - j) with +, +, -, -, each with a series of its own.
 // Sensor points keep their indices (the first copy is the radial grid).
 struct AxisymExpanded {
-  std::vector<double> c0, rho0, sg[2], bona, alpha, p0, psig;
-  U64 pidx;
+  std::vector<double> c0, rho0, sg[2], bona, alpha, p0, psig, usig[2];
+  U64 pidx, uidx[2];
 };
 
 // p-like layout of an Nx x Ny grid (signed: copies +, -, -, +, mirror
@@ -431,20 +433,31 @@ void expand_axisym(Loaded& L, AxisymExpanded& E) {
   grid_p(p.BonA, E.bona, false);
   grid_p(p.alpha_coeff, E.alpha, false);
   grid_p(p.p0, E.p0, true);
-  kw_time_source& ps = p.p_source;
-  if (ps.flag > 0) {
+  // a source on the expanded grid: one signed series per expanded point;
+  // p-like points (p, ux) at columns j, 2Ny - j, 2Ny + j, 4Ny - j with
+  // signs +, -, -, + (the second and fourth left out for j = 0), y-staggered
+  // ones (uy) at j, 2Ny - 1 - j, 2Ny + j, 4Ny - 1 - j with +, +, -, -
+  auto expand_source = [&](kw_time_source& ts, bool staggered, U64& idx, std::vector<double>& sig) {
+    if (ts.flag <= 0) return;
     struct Pt {
       uint64_t lin;
       int64_t row;
       double sign;
     };
     std::vector<Pt> pts;
-    pts.reserve((size_t)(4 * ps.num_points));
-    for (int64_t k = 0; k < ps.num_points; ++k) {
-      const int64_t lin = (int64_t)ps.index[k] - 1;
-      if (lin < 0 || lin >= Nx * Ny) fail(KW_EINVAL, "p_source_index outside the grid");
+    pts.reserve((size_t)(4 * ts.num_points));
+    for (int64_t k = 0; k < ts.num_points; ++k) {
+      const int64_t lin = (int64_t)ts.index[k] - 1;
+      if (lin < 0 || lin >= Nx * Ny) fail(KW_EINVAL, "source index outside the grid");
       const int64_t i = lin % Nx, j = lin / Nx;
-      const int64_t row = ps.num_series > 1 ? k : 0;
+      const int64_t row = ts.num_series > 1 ? k : 0;
+      if (staggered) {
+        pts.push_back({(uint64_t)(i + Nx * j), row, 1.0});
+        pts.push_back({(uint64_t)(i + Nx * (2 * Ny - 1 - j)), row, 1.0});
+        pts.push_back({(uint64_t)(i + Nx * (2 * Ny + j)), row, -1.0});
+        pts.push_back({(uint64_t)(i + Nx * (4 * Ny - 1 - j)), row, -1.0});
+        continue;
+      }
       pts.push_back({(uint64_t)(i + Nx * j), row, 1.0});
       pts.push_back({(uint64_t)(i + Nx * (2 * Ny + j)), row, -1.0});
       if (j > 0) {
@@ -453,19 +466,21 @@ void expand_axisym(Loaded& L, AxisymExpanded& E) {
       }
     }
     std::sort(pts.begin(), pts.end(), [](const Pt& a, const Pt& b) { return a.lin < b.lin; });
-    const int64_t ne = (int64_t)pts.size(), len = ps.flag, rows = ps.num_series;
-    E.pidx.resize((size_t)ne);
-    E.psig.resize((size_t)(ne * len));
+    const int64_t ne = (int64_t)pts.size(), len = ts.flag, rows = ts.num_series;
+    idx.resize((size_t)ne);
+    sig.resize((size_t)(ne * len));
     for (int64_t e = 0; e < ne; ++e) {
-      E.pidx[e] = pts[e].lin + 1;
-      for (int64_t t = 0; t < len; ++t)
-        E.psig[e + ne * t] = pts[e].sign * ps.input[pts[e].row + rows * t];
+      idx[e] = pts[e].lin + 1;
+      for (int64_t t = 0; t < len; ++t) sig[e + ne * t] = pts[e].sign * ts.input[pts[e].row + rows * t];
     }
-    ps.num_points = ne;
-    ps.index = E.pidx.data();
-    ps.num_series = ne;
-    ps.input = E.psig.data();
-  }
+    ts.num_points = ne;
+    ts.index = idx.data();
+    ts.num_series = ne;
+    ts.input = sig.data();
+  };
+  expand_source(p.p_source, false, E.pidx, E.psig);
+  expand_source(p.u_source[0], false, E.uidx[0], E.usig[0]);
+  expand_source(p.u_source[1], true, E.uidx[1], E.usig[1]);
   p.Ny = 4 * Ny;
   p.axisymmetric = 1;
 }

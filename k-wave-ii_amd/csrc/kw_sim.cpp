@@ -343,11 +343,13 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream, Arena* ar
     if (!pr->use_sg) fail(KW_EINVAL, "Optional input 'UseSG' is not supported for axisymmetric simulations.");
     if (pr->absorbing_flag == 1)
       fail(KW_EUNSUP, "the axisymmetric code takes lossless or Stokes media");
-    // pressure sources and time-reversal data come laid out on the expanded
-    // grid with the signs of each copy, so the 2-D source and time-reversal
-    // code applies them as given
-    if (pr->u_source[0].flag > 0 || pr->u_source[1].flag > 0)
-      fail(KW_EUNSUP, "axisymmetric simulations here take no velocity sources");
+    // sources and time-reversal data come laid out on the expanded grid with
+    // the signs of each copy, so the 2-D source and time-reversal code
+    // applies them as given.  A radial velocity source of 'WSWS-FFT' would
+    // also drive the mirror point that stands in for the last radial
+    // column, which the reference updates outside its layout.
+    if (S->as_ws && pr->u_source[1].flag > 0)
+      fail(KW_EUNSUP, "radial velocity sources (source.uy) with 'WSWS-FFT' are not supported here");
     if (pr->sensor_tri || pr->directivity_angle || (pr->record_flags & KW_REC_U_SPLIT_FIELD))
       fail(KW_EUNSUP, "axisymmetric simulations here take binary sensor masks");
   }

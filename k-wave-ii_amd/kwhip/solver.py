@@ -1280,6 +1280,90 @@ def _as_points(pos, nx, ny, ws=False):
     return lin[order], row[order], sign[order]
 
 
+def _as_points_u(pos, nx, ny):
+    """Points of the radial grid's y-staggered layout on the 'WSWA-FFT'
+    expanded grid ([f, flip(f), -f, -flip(f)], kspaceFirstOrderAS.m:
+    944-948): each point (i, j) at columns j (+), 2 Ny - 1 - j (+),
+    2 Ny + j (-) and 4 Ny - 1 - j (-).  Same returns as _as_points."""
+    pos = np.asarray(pos, dtype=np.int64)
+    ix, iy = pos % nx, pos // nx
+    src = np.arange(pos.size)
+    cols = [iy, 2 * ny - 1 - iy, 2 * ny + iy, 4 * ny - 1 - iy]
+    lin = np.concatenate([ix + nx * c for c in cols])
+    row = np.tile(src, 4)
+    sign = np.repeat([1.0, 1.0, -1.0, -1.0], pos.size)
+    order = np.argsort(lin, kind="stable")
+    return lin[order], row[order], sign[order]
+
+
+def _as_signal_rows(m, sig, pos):
+    """Row of the signal matrix per source point: one series, one per
+    point, or a labelled mask (kspaceFirstOrder_inputChecking.m:850-887)."""
+    vals = m[np.unravel_index(pos, m.shape, order="F")]
+    uniq = np.unique(vals)
+    n_uniq = uniq.size + (1 if pos.size < m.size else 0)
+    if not (n_uniq <= 2 and float(uniq.sum()) == 1):
+        if sig.shape[0] != n_uniq - 1:
+            raise ValueError("The number of time series must match the number of labelled "
+                             "source elements.")
+        return vals.astype(np.int64) - 1
+    if sig.shape[0] == 1:
+        return np.zeros(pos.size, dtype=np.int64)
+    if sig.shape[0] == pos.size:
+        return np.arange(pos.size)
+    raise ValueError("The number of time series must match the number of source elements.")
+
+
+def _as_velocity_sources(src, med, kgrid, scale, ws=False):
+    """source.ux (axial) and source.uy (radial) of an axisymmetric
+    simulation on the expanded grid: {axis: (expanded indices, series,
+    mode)}, one series per expanded point signed as the field's layout (ux
+    p-like, uy y-staggered), scaled first on the radial grid as
+    kspaceFirstOrder_scaleSourceTerms.m:284-350 scales them (2 c0 dt / dx,
+    2 c0 dt / dy; row i with the sound speed at source point i)."""
+    Nx, Ny = kgrid.Nx, kgrid.Ny
+    if "uz" in src:
+        raise ValueError("source.uz is not defined for axisymmetric simulations.")
+    if "u_mask" not in src:
+        raise ValueError("source.u_mask must be defined with source.ux or source.uy")
+    m = np.asarray(src["u_mask"])
+    if m.shape != (Nx, Ny):
+        raise ValueError("source.u_mask must be the same size as the computational grid.")
+    pos = _find(m)
+    if pos.size == 0:
+        raise ValueError("source.u_mask must be a binary grid with at least one element set to 1.")
+    mode = src.get("u_mode", "additive")
+    if mode not in _abi.SRC_MODE:
+        raise ValueError("source.u_mode must be set to 'additive', 'additive-no-correction', or "
+                         "'dirichlet'.")
+    dt = float(kgrid.dt)
+    w = 1.0
+    if "u_frequency_ref" in src:
+        f_ref = float(src["u_frequency_ref"])
+        if f_ref < 0:
+            raise ValueError("source.u_frequency_ref must be nonnegative")
+        mode = "additive-no-correction"
+        w = math.cos(math.pi * f_ref * dt)
+    c = np.asarray(med["sound_speed"], dtype=np.float64)
+    c_at = (lambda i: float(c)) if c.size == 1 else (
+        lambda i: float(c.reshape(-1, order="F")[pos[i]]))
+    out = {}
+    for ax, nm, dd in ((0, "ux", kgrid.dx), (1, "uy", kgrid.dy)):
+        if nm not in src:
+            continue
+        if nm == "uy" and ws:
+            raise NotImplementedError("radial velocity sources (source.uy) with 'WSWS' are not "
+                                      "supported by kwhip")
+        sig = w * np.atleast_2d(np.asarray(src[nm], dtype=np.float64))
+        rows = _as_signal_rows(m, sig, pos)
+        if scale and mode != "dirichlet":
+            for i in range(sig.shape[0]):
+                sig[i] *= 2 * c_at(i) * dt / dd
+        epos, row, sign = _as_points(pos, Nx, Ny, ws) if nm == "ux" else _as_points_u(pos, Nx, Ny)
+        out[ax] = (epos, sign[:, None] * sig[rows[row]], mode)
+    return out
+
+
 def _as_pressure_source(src, med, kgrid, scale, ws=False):
     """source.p of an axisymmetric simulation on the expanded grid: one
     series per expanded point, signed as the p-like layout, scaled first on
@@ -1351,11 +1435,11 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
     added by the library (kw_problem.axisymmetric 1 or 3).
     Supported here: source.p0 or a pressure
     source (every p_mode, laid out on the expanded grid with the signs of
-    the p-like layout), lossless or Stokes media (with or without B/A),
+    the p-like layout), velocity sources source.ux (axial) and source.uy
+    (radial; not with 'WSWS'), lossless or Stokes media (with or without B/A),
     binary sensor masks and time reversal; 'SaveToDisk' writes the radial
     grid's input file (axisymmetric_flag 1, 'WSWA'), which kw_run_file and
-    kspaceFirstOrder-HIP run; velocity sources and 'PMLInside' false are
-    refused."""
+    kspaceFirstOrder-HIP run; 'PMLInside' false is refused."""
     opts = _parse_options(args, kwargs)
     sym = opts.pop("RadialSymmetry", "WSWA-FFT")
     if sym not in ("WSWA", "WSWS", "WSWA-FFT", "WSWS-FFT"):
@@ -1391,9 +1475,10 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
                   "user value ignored.")
         med["alpha_mode"] = "stokes"
         med["alpha_power"] = 2.0
-    if set(src) - {"p0", "p", "p_mask", "p_mode", "p_frequency_ref"}:
-        raise NotImplementedError("axisymmetric simulations in kwhip take source.p0 or a "
-                                  "pressure source (source.p, p_mask, p_mode, p_frequency_ref)")
+    for k in src:
+        if k not in ("p0", "p", "p_mask", "p_mode", "p_frequency_ref", "u_mask", "u_mode",
+                     "u_frequency_ref", "ux", "uy", "uz"):
+            raise ValueError(f"'{k}' is not a valid field for source")
     time_rev = sen is not None and "time_reversal_boundary_data" in sen
     if sen is not None:
         for k in sen:
@@ -1421,6 +1506,9 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
         esrc["p0"] = _as_mirror(p0, "p", True, ws)
     if "p" in src:
         esrc.update(_as_pressure_source(src, med, kgrid, opts.get("ScaleSourceTerms", True), ws))
+    usrc = {}
+    if any(k in src for k in ("ux", "uy", "uz")):
+        usrc = _as_velocity_sources(src, med, kgrid, opts.get("ScaleSourceTerms", True), ws)
     if np.size(np.atleast_1d(sm)) > 1 and any(np.atleast_1d(sm)[1:]):
         raise NotImplementedError("smoothing c0 or rho0 is not supported by kwhip for "
                                   "axisymmetric simulations")
@@ -1478,6 +1566,15 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
     opts["ScaleSourceTerms"] = False
     pp = prepare(kg, emed, esrc, esen, 2, opts)
     pp.prob.axisymmetric = 3 if ws else 1
+    for ax, (epos, sig, mode) in usrc.items():
+        ts = pp.prob.u_source[ax]
+        ts.flag = sig.shape[1]
+        ts.mode = _abi.SRC_MODE[mode]
+        idx = pp.hold(np.ascontiguousarray(epos.astype(np.uint64) + 1))
+        ts.num_points = len(idx)
+        ts.index = _abi.uptr(idx)
+        ts.num_series = sig.shape[0]
+        ts.input = _abi.dptr(pp.hold(np.ascontiguousarray(sig.ravel(order="F"))))
     rho = np.asarray(med["density"], dtype=np.float64)
     if rho.size > 1:
         # kspaceFirstOrderAS.m:440-456 on the radial grid, then laid out

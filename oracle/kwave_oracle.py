@@ -1446,7 +1446,8 @@ def kspace_first_order_as(kgrid: KGrid, medium, source, sensor, *, data_cast="of
     """kspaceFirstOrderAS.m:355-1435 with 'RadialSymmetry' 'WSWA-FFT' (or
     'WSWS-FFT', the expanded grid of 2 Ny - 2 points) for the
     cases the library runs: an initial pressure or a pressure source (every
-    p_mode, :1055-1128, and p_frequency_ref), lossless or Stokes media
+    p_mode, :1055-1128, and p_frequency_ref), velocity sources ux and uy
+    (every u_mode, :788-936), lossless or Stokes media
     (linear or with B/A; the axisymmetric code takes medium.alpha_coeff as
     Stokes absorption with alpha_power 2, kspaceFirstOrder_inputChecking.m:
     173-190, kspaceFirstOrderAS.m:1134-1163), a binary sensor mask recording
@@ -1511,6 +1512,32 @@ def kspace_first_order_as(kgrid: KGrid, medium, source, sensor, *, data_cast="of
             ci = cflat[pos[i]]
             sig[i] = sig[i] / (2 * ci ** 2) if mode == "dirichlet" else sig[i] * (2 * dt / (2 * ci * dx))
         psrc = (pos, sig, sig_idx, mode)
+    # velocity sources (kspaceFirstOrderAS.m:788-936): one mask and mode for
+    # ux (axial) and uy (radial), scaled as scaleSourceTerms.m:286-350 does
+    # (2 c0 dt / dx and 2 c0 dt / dy, row i with c0 at source point i)
+    usrc = {}
+    if "u_mask" in source:
+        umask = np.asarray(source["u_mask"])
+        upos = np.flatnonzero(umask.reshape(-1, order="F"))
+        umode = source.get("u_mode", "additive")
+        uwt = 1.0
+        if "u_frequency_ref" in source:
+            uwt = np.cos(2 * np.pi * float(source["u_frequency_ref"]) * dt / 2)
+            umode = "additive-no-correction"
+        cflat = np.broadcast_to(c0, (Nx, Ny)).reshape(-1, order="F")
+        for nm, dd in (("ux", dx), ("uy", dy)):
+            if nm not in source:
+                continue
+            sig = uwt * np.atleast_2d(np.asarray(source[nm], dtype=np.float64))
+            if sig.shape[0] == 1:
+                sidx = np.zeros(upos.size, dtype=int)
+            elif umask.max() > 1:
+                sidx = umask.reshape(-1, order="F")[upos].astype(int) - 1
+            else:
+                sidx = np.arange(upos.size)
+            if umode != "dirichlet":
+                sig = np.stack([sig[i] * (2 * cflat[upos[i]] * dt / dd) for i in range(sig.shape[0])])
+            usrc[nm] = (sig, sidx)
     tr = sensor.get("time_reversal_boundary_data") if isinstance(sensor, dict) else None
     # operators (kspaceFirstOrderAS.m:463-565)
     pml_x = get_pml(Nx, dx, dt, c_ref, px, ax, False)
@@ -1606,6 +1633,24 @@ def kspace_first_order_as(kgrid: KGrid, medium, source, sensor, *, data_cast="of
         dpdx, dpdy = grad(p)
         ux = pml_x_sg * (pml_x_sg * ux - dtc * rsx_inv * dpdx)
         uy = pml_y_sg * (pml_y_sg * uy - dtc * rsy_inv * dpdy)
+        for nm, layout in (("ux", mirror_p), ("uy", mirror_u)):
+            if nm not in usrc or t > usrc[nm][0].shape[1]:
+                continue
+            v = usrc[nm][0][usrc[nm][1], t - 1]
+            f = ux if nm == "ux" else uy
+            if umode == "dirichlet":
+                f = flat_set(f, upos, v)
+            elif umode == "additive":
+                sm_ = np.zeros(Nx * Ny)
+                sm_[upos] = v
+                sm_ = sm_.reshape((Nx, Ny), order="F")
+                f = (f + ifft2(source_kappa * fft2(layout(sm_)))[:, :Ny]).astype(rdt)
+            else:
+                f = flat_add(f, upos, v)
+            if nm == "ux":
+                ux = f
+            else:
+                uy = f
         duxdx = ifft2(ddx_neg * kappa * fft2(mirror_p(ux)))[:, :Ny]
         u_on_y = mirror_u_on_y(inv_y * uy)
         duydy = ifft2(kappa * y_neg * (ddy_k * fft2(mirror_u(uy)) + fft2(u_on_y)))[:, :Ny]

@@ -58,6 +58,22 @@ def case(name, Nx=32, Ny=16, seed=2, Nt=24, sym="WSWA-FFT"):
         cs["medium"]["alpha_coeff"] = 0.5 + rng.random((Nx, Ny))
         cs["medium"]["BonA"] = 6.0
         cs["source"] = {"p_mask": src_mask, "p": 1e6 * burst[None]}
+    elif name == "velocity_additive":
+        # axial and radial velocity sources, one series per point
+        cs["source"] = {"u_mask": src_mask, "u_mode": "additive",
+                        "ux": rng.standard_normal((n_src, 1)) * burst,
+                        "uy": rng.standard_normal((n_src, 1)) * burst}
+    elif name == "velocity_dirichlet":
+        cs["source"] = {"u_mask": src_mask, "u_mode": "dirichlet", "ux": 1e-4 * burst[None, :12],
+                        "uy": -1e-4 * burst[None, :12]}
+    elif name == "velocity_labelled":
+        lab = np.zeros((Nx, Ny))
+        lab[src_mask] = 1 + (rng.random(n_src) < 0.5)
+        cs["source"] = {"u_mask": lab, "u_mode": "additive-no-correction",
+                        "uy": np.stack([burst, -0.5 * burst])}
+    elif name == "velocity_axial_only":
+        cs["medium"]["alpha_coeff"] = 0.5 + rng.random((Nx, Ny))
+        cs["source"] = {"u_mask": src_mask, "ux": burst[None], "u_frequency_ref": 1e6}
     elif name == "time_reversal":
         sm = cs["sensor"]["mask"]
         cs["source"] = {}
@@ -70,10 +86,12 @@ def case(name, Nx=32, Ny=16, seed=2, Nt=24, sym="WSWA-FFT"):
 
 NAMES = ["homogeneous", "hetero", "nonlinear", "stokes", "stokes_nonlinear", "smoothed",
          "source_additive", "source_dirichlet", "source_labelled", "source_frequency_ref",
-         "source_stokes_nonlinear", "time_reversal"]
-# the cases run with 'WSWS-FFT' as well
+         "source_stokes_nonlinear", "time_reversal", "velocity_additive", "velocity_dirichlet",
+         "velocity_labelled", "velocity_axial_only"]
+# the cases run with 'WSWS-FFT' as well (radial velocity sources are not
+# taken with it)
 WS_NAMES = ["hetero", "stokes_nonlinear", "smoothed", "source_additive", "source_dirichlet",
-            "time_reversal"]
+            "time_reversal", "velocity_axial_only"]
 
 
 def run_kwhip(cs):

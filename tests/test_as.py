@@ -68,8 +68,11 @@ def test_as_option_checks():
     with pytest.raises(ValueError, match="source.p0 and source.p can't be defined"):
         kwhip.kspaceFirstOrderAS(kg, cs["medium"], {"p0": p0, "p_mask": p0 > 0.5,
                                                     "p": np.ones((1, 4))}, cs["sensor"])
-    with pytest.raises(NotImplementedError):
-        kwhip.kspaceFirstOrderAS(kg, cs["medium"], {"u_mask": p0 > 0.5, "ux": np.ones((1, 4))},
+    with pytest.raises(NotImplementedError, match="radial velocity sources"):
+        kwhip.kspaceFirstOrderAS(kg, cs["medium"], {"u_mask": p0 > 0.5, "uy": np.ones((1, 4))},
+                                 cs["sensor"], RadialSymmetry="WSWS")
+    with pytest.raises(ValueError, match="source.uz is not defined"):
+        kwhip.kspaceFirstOrderAS(kg, cs["medium"], {"u_mask": p0 > 0.5, "uz": np.ones((1, 4))},
                                  cs["sensor"])
     with pytest.raises(ValueError, match="number of time series must match"):
         kwhip.kspaceFirstOrderAS(kg, cs["medium"], {"p_mask": p0 > 0.5, "p": np.ones((2, 4))},
@@ -84,7 +87,7 @@ def test_as_option_checks():
 
 
 @pytest.mark.parametrize("name", ["hetero", "stokes_nonlinear", "source_additive",
-                                  "source_dirichlet"])
+                                  "source_dirichlet", "velocity_additive"])
 def test_as_file_contract(name, tmp_path):
     """'SaveToDisk' writes the radial grid (axisymmetric_flag 1, the
     variable set of a 2-D file), and kw_run_file runs it on the expanded

@@ -1,6 +1,6 @@
 """kspaceFirstOrderAS on the GPU: every case of tests/as_cases.py (initial
-pressure, pressure sources in every mode, time reversal; lossless, Stokes,
-nonlinear; 'WSWA-FFT', and 'WSWS-FFT' for six of them) on a 128 x 64 grid against the oracle (relative L-inf <= 1e-5
+pressure, pressure and velocity sources in every mode, time reversal;
+lossless, Stokes, nonlinear; 'WSWA-FFT', and 'WSWS-FFT' for seven of them) on a 128 x 64 grid against the oracle (relative L-inf <= 1e-5
 single, <= 1e-4 double), the spherical wave of a pulse centred on the axis against its
 closed form (the scheme's discretisation error, as the oracle measures it),
 the engine's refusal of axisymmetric problems, and the file contract:
@@ -80,7 +80,8 @@ def test_gpu_engine_refuses_axisymmetric(hip_lib):
             _abi.call(eng.lib.kw_submit, eng.h, C.byref(pp.prob), C.byref(tid))
 
 
-@pytest.mark.parametrize("name", ["hetero", "source_additive", "stokes_nonlinear"])
+@pytest.mark.parametrize("name", ["hetero", "source_additive", "stokes_nonlinear",
+                                  "velocity_additive"])
 def test_gpu_as_command_line(hip_lib, tmp_path, name):
     """kspaceFirstOrderAS(..., 'SaveToDisk') writes the radial grid's input
     file (for one case re-written the way MATLAB writes it, chunked and
