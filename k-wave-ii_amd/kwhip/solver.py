@@ -1424,6 +1424,62 @@ def _as_pressure_source(src, med, kgrid, scale, ws=False):
     return {"p_mask": emask, "p": sign[:, None] * sig[sig_idx[row]], "p_mode": mode}
 
 
+def _as_pml_outside(kgrid, medium, source, sensor, sym, opts):
+    """'PMLInside' false for the axisymmetric code: p0 smoothed on the
+    user's grid first (kspaceFirstOrder_inputChecking.m:1501-1535), then
+    the grid grows by the PML on both axial sides and on the outer radial
+    side only (kspaceFirstOrder_expandGridMatrices.m:38-48,62-70): medium
+    maps by their edge values, p0 and masks by zeros; the _final and _all
+    outputs are cut back to the user's grid (y1_inside = 1,
+    kspaceFirstOrder_inputChecking.m:1737-1749)."""
+    pml = opts.get("PMLSize", 20)
+    if isinstance(pml, str):
+        raise NotImplementedError("'PMLSize' 'auto' is not supported by kwhip for axisymmetric "
+                                  "simulations")
+    px, py = [int(v) for v in np.broadcast_to(np.atleast_1d(pml), (2,))]
+    Nx, Ny = kgrid.Nx, kgrid.Ny
+    pads = [(px, px), (0, py)]
+    med = dict(medium) if isinstance(medium, dict) else dict(vars(medium))
+    for nm in ("sound_speed", "density", "alpha_coeff", "BonA"):
+        if nm in med and np.size(med[nm]) > 1:
+            med[nm] = np.pad(np.asarray(med[nm], dtype=np.float64), pads, mode="edge")
+    src = dict(source) if isinstance(source, dict) else dict(vars(source))
+    sm = np.atleast_1d(opts.get("Smooth", True))
+    if sm.size > 1 and any(sm[1:]):
+        raise NotImplementedError("smoothing c0 or rho0 is not supported by kwhip for "
+                                  "axisymmetric simulations")
+    if "p0" in src:
+        p0 = np.asarray(src["p0"], dtype=np.float64)
+        if p0.shape != (Nx, Ny):
+            raise ValueError("source.p0 must be the same size as the computational grid.")
+        if bool(sm[0]):
+            p0 = smooth(_as_mirror(p0, "p", True, sym.startswith("WSWS")), True)[:, :Ny]
+        src["p0"] = np.pad(p0, pads)
+    for nm in ("p_mask", "u_mask"):
+        if nm in src:
+            src[nm] = np.pad(np.asarray(src[nm]), pads)
+    sen = None
+    if sensor is not None:
+        sen = dict(sensor) if isinstance(sensor, dict) else dict(vars(sensor))
+        sen["mask"] = np.pad(np.asarray(sen["mask"]), pads)
+    kg = kWaveGrid(Nx + 2 * px, kgrid.dx, Ny + py, kgrid.dy)
+    kg.setTime(kgrid.Nt, kgrid.dt)
+    o = dict(opts, PMLInside=True, Smooth=False, PMLSize=[px, py], RadialSymmetry=sym)
+    out = kspaceFirstOrderAS(kg, med, src, sen, **o)
+
+    def cut(a):
+        a = np.asarray(a)
+        if a.shape != (Nx + 2 * px, Ny + py):
+            a = a.reshape((Nx + 2 * px, Ny + py), order="F")
+        return a[px:px + Nx, :Ny]
+
+    if isinstance(out, dict):
+        return {k: (cut(v) if k.endswith(("_final", "_all")) else v) for k, v in out.items()}
+    if isinstance(out, list):
+        return out   # 'SaveToDisk' wrote the expanded grid and exited
+    return cut(out)   # time reversal: p_final
+
+
 def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa: N802
     """Axisymmetric time-domain simulation (reference
     +kwave/+legacy/kspaceFirstOrderAS.m): a 2-D grid with x the axial and y
@@ -1439,7 +1495,8 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
     (radial; not with 'WSWS'), lossless or Stokes media (with or without B/A),
     binary sensor masks and time reversal; 'SaveToDisk' writes the radial
     grid's input file (axisymmetric_flag 1, 'WSWA'), which kw_run_file and
-    kspaceFirstOrder-HIP run; 'PMLInside' false is refused."""
+    kspaceFirstOrder-HIP run; 'PMLInside' false grows the grid by the PML
+    outside the axial ends and the outer radius."""
     opts = _parse_options(args, kwargs)
     sym = opts.pop("RadialSymmetry", "WSWA-FFT")
     if sym not in ("WSWA", "WSWS", "WSWA-FFT", "WSWS-FFT"):
@@ -1458,8 +1515,7 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
     if not opts.get("UseSG", True):
         raise ValueError("Optional input 'UseSG' is not supported for axisymmetric simulations.")
     if not opts.get("PMLInside", True):
-        raise NotImplementedError("'PMLInside' false is not supported by kwhip for axisymmetric "
-                                  "simulations")
+        return _as_pml_outside(kgrid, medium, source, sensor, sym, opts)
     med = dict(medium) if isinstance(medium, dict) else dict(vars(medium))
     src = dict(source) if isinstance(source, dict) else dict(vars(source))
     sen = None if sensor is None else (dict(sensor) if isinstance(sensor, dict) else dict(vars(sensor)))

@@ -73,8 +73,11 @@ How each part is pinned (tests/test_oracle_*.py)
   the u_r / r term is caught; an on-axis Gaussian pressure source ('additive'
   and 'additive-no-correction') gives the field of the 3-D solver on a
   Cartesian grid, and dropping the k-space source correction is caught;
-  'dirichlet' holds p at its points to the series at every step; time
-  reversal equals a 'dirichlet' run of the flipped data one step shifted.
+  axial and radial velocity sources likewise; 'dirichlet' holds p at its
+  points to the series at every step; time reversal equals a 'dirichlet'
+  run of the flipped data one step shifted; 'WSWS-FFT' follows the same
+  spherical wave and catches planted layout parities; with the PML outside
+  the spherical wave holds on the whole user grid.
 """
 
 from __future__ import annotations
@@ -1441,8 +1444,44 @@ def as_mirror_u_ws(f, sign=-1):
     return np.concatenate([f[:, :-1], sign * f[:, -2::-1]], axis=1)
 
 
+def _as_pml_outside(kgrid, medium, source, sensor, *, pml_size, smooth, radial_symmetry, **kw):
+    """'PMLInside' false for the axisymmetric code: p0 smoothed on the
+    user's grid (kspaceFirstOrder_inputChecking.m:1501-1535), then the grid
+    grows by the PML on both axial sides and on the outer radial side only
+    (kspaceFirstOrder_expandGridMatrices.m:38-48,62-70), medium maps by
+    their edge values, p0 and masks by zeros; the _final outputs are cut
+    back to the user's grid (x1_inside.., y1_inside = 1,
+    kspaceFirstOrder_inputChecking.m:1737-1749)."""
+    px, py = (pml_size, pml_size) if np.ndim(pml_size) == 0 else pml_size
+    Nx, Ny = kgrid.Nx, kgrid.Ny
+    sizes = [(px, px), (0, py)]
+    grow = lambda a, edge: np.pad(np.asarray(a), sizes, mode="edge" if edge else "constant")  # noqa: E731
+    med = dict(medium)
+    for nm in ("sound_speed", "density", "alpha_coeff", "BonA"):
+        if nm in med and np.size(med[nm]) > 1:
+            med[nm] = grow(np.asarray(med[nm], dtype=np.float64), True)
+    src = dict(source)
+    if "p0" in src:
+        p0 = np.asarray(src["p0"], dtype=np.float64)
+        if smooth:
+            mirror = as_mirror_p_ws if radial_symmetry == "WSWS-FFT" else as_mirror_p
+            p0 = smooth_mat(mirror(p0), restore_max=True)[:, :Ny]
+        src["p0"] = grow(p0, False)
+    for nm in ("p_mask", "u_mask"):
+        if nm in src:
+            src[nm] = grow(src[nm], False)
+    sen = dict(sensor)
+    sen["mask"] = grow(np.asarray(sensor["mask"]), False)
+    kg = KGrid(Nx + 2 * px, kgrid.dx, Ny + py, kgrid.dy)
+    kg.set_time(kgrid.Nt, kgrid.dt)
+    out = kspace_first_order_as(kg, med, src, sen, pml_size=(px, py), smooth=False,
+                                radial_symmetry=radial_symmetry, **kw)
+    return {k: (v[px:px + Nx, :Ny] if k.endswith(("_final", "_all")) else v) for k, v in out.items()}
+
+
 def kspace_first_order_as(kgrid: KGrid, medium, source, sensor, *, data_cast="off",
-                          pml_size=20, pml_alpha=2.0, smooth=True, radial_symmetry="WSWA-FFT"):
+                          pml_size=20, pml_alpha=2.0, smooth=True, radial_symmetry="WSWA-FFT",
+                          pml_inside=True):
     """kspaceFirstOrderAS.m:355-1435 with 'RadialSymmetry' 'WSWA-FFT' (or
     'WSWS-FFT', the expanded grid of 2 Ny - 2 points) for the
     cases the library runs: an initial pressure or a pressure source (every
@@ -1456,6 +1495,10 @@ def kspace_first_order_as(kgrid: KGrid, medium, source, sensor, *, data_cast="of
     axis), or time reversal of sensor.time_reversal_boundary_data (:640-650,
     690-702, 1343-1345).  Returns the sensor data dict (p as [points x Nt]);
     {'p_final': p} for time reversal."""
+    if not pml_inside:
+        return _as_pml_outside(kgrid, medium, source, sensor, data_cast=data_cast,
+                               pml_size=pml_size, pml_alpha=pml_alpha, smooth=smooth,
+                               radial_symmetry=radial_symmetry)
     rdt = np.float32 if data_cast == "single" else np.float64
     cdt = np.complex64 if data_cast == "single" else np.complex128
     Nx, Ny = kgrid.Nx, kgrid.Ny

@@ -74,6 +74,15 @@ def case(name, Nx=32, Ny=16, seed=2, Nt=24, sym="WSWA-FFT"):
     elif name == "velocity_axial_only":
         cs["medium"]["alpha_coeff"] = 0.5 + rng.random((Nx, Ny))
         cs["source"] = {"u_mask": src_mask, "ux": burst[None], "u_frequency_ref": 1e6}
+    elif name == "pml_outside":
+        # the PML outside the grid, smoothed p0, Stokes absorption
+        cs["pml_inside"] = False
+        cs["smooth"] = True
+        cs["medium"]["alpha_coeff"] = 0.5 + rng.random((Nx, Ny))
+        cs["sensor"]["record"] = cs["sensor"]["record"] + ["p_max_all"]
+    elif name == "pml_outside_source":
+        cs["pml_inside"] = False
+        cs["source"] = {"p_mask": src_mask, "p": burst[None]}
     elif name == "time_reversal":
         sm = cs["sensor"]["mask"]
         cs["source"] = {}
@@ -87,11 +96,11 @@ def case(name, Nx=32, Ny=16, seed=2, Nt=24, sym="WSWA-FFT"):
 NAMES = ["homogeneous", "hetero", "nonlinear", "stokes", "stokes_nonlinear", "smoothed",
          "source_additive", "source_dirichlet", "source_labelled", "source_frequency_ref",
          "source_stokes_nonlinear", "time_reversal", "velocity_additive", "velocity_dirichlet",
-         "velocity_labelled", "velocity_axial_only"]
+         "velocity_labelled", "velocity_axial_only", "pml_outside", "pml_outside_source"]
 # the cases run with 'WSWS-FFT' as well (radial velocity sources are not
 # taken with it)
 WS_NAMES = ["hetero", "stokes_nonlinear", "smoothed", "source_additive", "source_dirichlet",
-            "time_reversal", "velocity_axial_only"]
+            "time_reversal", "velocity_axial_only", "pml_outside"]
 
 
 def run_kwhip(cs):
@@ -100,7 +109,8 @@ def run_kwhip(cs):
     kg.setTime(cs["Nt"], cs["dt"])
     out = kwhip.kspaceFirstOrderAS(kg, dict(cs["medium"]), dict(cs["source"]), dict(cs["sensor"]),
                                    PMLSize=cs["pml"], Smooth=cs["smooth"],
-                                   RadialSymmetry=cs.get("sym", "WSWA-FFT"))
+                                   RadialSymmetry=cs.get("sym", "WSWA-FFT"),
+                                   PMLInside=cs.get("pml_inside", True))
     return {"p_final": out} if "time_reversal_boundary_data" in cs["sensor"] else out
 
 
@@ -111,7 +121,7 @@ def run_oracle(cs, data_cast):
     sym = "WSWS-FFT" if cs.get("sym", "WSWA-FFT").startswith("WSWS") else "WSWA-FFT"
     return ko.kspace_first_order_as(kg, cs["medium"], cs["source"], cs["sensor"],
                                     data_cast=data_cast, pml_size=cs["pml"], smooth=cs["smooth"],
-                                    radial_symmetry=sym)
+                                    radial_symmetry=sym, pml_inside=cs.get("pml_inside", True))
 
 
 def compare(got, ref):
