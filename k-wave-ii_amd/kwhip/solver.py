@@ -219,10 +219,12 @@ def cart2grid(kgrid: kWaveGrid, cart_data):
     return (grid != 0), order_index, reorder_index
 
 
-def optimal_pml_size(grid_size, pml_range=(10, 40)):
+def optimal_pml_size(grid_size, pml_range=(10, 40), axisymmetric=None):
     """getOptimalPMLSize.m: for each axis the PML size in the inclusive range
     whose grown length N + 2 pml has the smallest largest prime factor (the
-    smallest such size)."""
+    smallest such size).  axisymmetric 'WSWA' / 'WSWS' (:124-133): the
+    radial axis grows by one PML and is scored by its expanded length,
+    4 (N + pml) or 2 (N + pml) - 2."""
     # MATLAB round: halves away from zero (10.5 -> 11)
     lo, hi = (int(math.copysign(math.floor(abs(float(v)) + 0.5), float(v))) for v in pml_range)
     if lo < 0 or hi < 0:
@@ -240,9 +242,14 @@ def optimal_pml_size(grid_size, pml_range=(10, 40)):
                 f += 1
         return max(last, n)
 
+    def grown(ax, N, p):
+        if axisymmetric and ax == 1:
+            return 4 * (N + p) if axisymmetric == "WSWA" else 2 * (N + p) - 2
+        return N + 2 * p
+
     sizes = []
-    for N in grid_size:
-        best = min(range(lo, hi + 1), key=lambda p: (largest_prime_factor(N + 2 * p), p))
+    for ax, N in enumerate(grid_size):
+        best = min(range(lo, hi + 1), key=lambda p: (largest_prime_factor(grown(ax, N, p)), p))
         sizes.append(best)
     return sizes
 
@@ -1434,8 +1441,11 @@ def _as_pml_outside(kgrid, medium, source, sensor, sym, opts):
     kspaceFirstOrder_inputChecking.m:1737-1749)."""
     pml = opts.get("PMLSize", 20)
     if isinstance(pml, str):
-        raise NotImplementedError("'PMLSize' 'auto' is not supported by kwhip for axisymmetric "
-                                  "simulations")
+        if pml != "auto":
+            raise ValueError("Optional input 'PMLSize' must be a 1 or 2 element numerical array, "
+                             "or set to 'auto'.")
+        # kspaceFirstOrder_checkOptionalInputs.m:484-512 with the symmetry
+        pml = optimal_pml_size((kgrid.Nx, kgrid.Ny), opts.get("PMLRange", [10, 40]), sym[:4])
     px, py = [int(v) for v in np.broadcast_to(np.atleast_1d(pml), (2,))]
     Nx, Ny = kgrid.Nx, kgrid.Ny
     pads = [(px, px), (0, py)]
@@ -1516,6 +1526,8 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
         raise ValueError("Optional input 'UseSG' is not supported for axisymmetric simulations.")
     if not opts.get("PMLInside", True):
         return _as_pml_outside(kgrid, medium, source, sensor, sym, opts)
+    if isinstance(opts.get("PMLSize"), str):
+        raise ValueError("'PMLSize' set to 'auto' is only supported with 'PMLInside' set to false.")
     med = dict(medium) if isinstance(medium, dict) else dict(vars(medium))
     src = dict(source) if isinstance(source, dict) else dict(vars(source))
     sen = None if sensor is None else (dict(sensor) if isinstance(sensor, dict) else dict(vars(sensor)))

@@ -52,7 +52,7 @@ def smoothed_noise(shape, seed, sigma):
         sh = [1] * len(shape)
         sh[ax] = -1
         filt = filt * np.exp(-2.0 * (math.pi ** 2) * (sigma ** 2) * k.reshape(sh) ** 2)
-    g = np.fft.irfftn(G * filt, s=shape)
+    g = np.fft.irfftn(G * filt, s=shape, axes=tuple(range(len(shape))))
     return g / np.max(np.abs(g))
 
 

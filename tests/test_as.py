@@ -147,3 +147,24 @@ def test_as_file_options(tmp_path, capsys):
         from kwhip.solver import run_prepared
         out_arr = run_prepared(pp)
     assert out_arr == {}
+
+
+def test_as_pml_auto():
+    """'PMLSize' 'auto' with the PML outside (getOptimalPMLSize.m with the
+    radial symmetry): the radial PML is scored by the expanded length."""
+    import kwhip
+    from kwhip.solver import optimal_pml_size
+    # 4 (16 + p) is a power of two at p = 16; 2 (16 + p) - 2 at p = 17
+    assert optimal_pml_size((32, 16), (10, 40), "WSWA") == [16, 16]
+    assert optimal_pml_size((32, 16), (10, 40), "WSWS") == [16, 17]
+    cs = ac.case("hetero")
+    kg = kwhip.kWaveGrid(cs["Nx"], cs["dx"], cs["Ny"], cs["dx"])
+    kg.setTime(cs["Nt"], cs["dt"])
+    a = kwhip.kspaceFirstOrderAS(kg, cs["medium"], dict(cs["source"]), cs["sensor"],
+                                 PMLSize="auto", PMLInside=False, Smooth=False)
+    b = kwhip.kspaceFirstOrderAS(kg, cs["medium"], dict(cs["source"]), cs["sensor"],
+                                 PMLSize=[16, 16], PMLInside=False, Smooth=False)
+    assert np.array_equal(np.asarray(a["p"]), np.asarray(b["p"]))
+    with pytest.raises(ValueError, match="only supported with 'PMLInside' set to false"):
+        kwhip.kspaceFirstOrderAS(kg, cs["medium"], dict(cs["source"]), cs["sensor"],
+                                 PMLSize="auto")
