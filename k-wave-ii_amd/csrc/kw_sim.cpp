@@ -512,6 +512,7 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream, Arena* ar
       ir[j] = (float)((first ? 1.0 : -1.0) / (r * d[1] + d[1] / 2));
     }
     S->as_ir = S->upload(ir);
+    S->scratch_grid = S->alloc<float>((size_t)G.ncell);  // ir .* u_r of every step
     if (S->as_ws) {
       // kspaceFirstOrderAS.m:440-456 leaves rho0_sgy of the last radial
       // column at rho0 there (the interpolation past the grid is NaN); the
