@@ -43,8 +43,9 @@ _VISUAL_OPTIONS = {"PlotSim", "PlotLayout", "PlotPML", "PlotScale", "PlotFreq", 
 _OPTIONS = {"PMLSize", "PMLAlpha", "PMLInside", "PMLRange", "MultiAxialPMLRatio", "UseFD",
             "Smooth", "DataCast", "DataRecast",
             "CartInterp", "UsekSpace", "UseSG", "ScaleSourceTerms", "SaveToDisk",
-            "SaveToDiskExit", "HDFCompressionLevel", "RadialSymmetry"} | _VISUAL_OPTIONS
-_UNSUPPORTED = {"StreamToDisk"}
+            "SaveToDiskExit", "HDFCompressionLevel", "RadialSymmetry",
+            "StreamToDisk"} | _VISUAL_OPTIONS
+_UNSUPPORTED = set()
 # kspaceFirstOrder_setDefaults.m:196
 SAVE_TO_DISK_FILENAME_DEF = "kwave_input_data.h5"
 
@@ -1171,11 +1172,43 @@ def save_to_disk(pp: PreparedProblem, filename: str):
     return filename
 
 
+def _check_stream_to_disk(dim, sensor, opts):
+    """'StreamToDisk' (kspaceFirstOrder_checkOptionalInputs.m:353-374,
+    kspaceFirstOrder_inputChecking.m:1369-1374).  The reference buffers the
+    recorded pressure for that many steps in host memory, appends each full
+    buffer to a temporary file and reads the whole record back at the end
+    (kspaceFirstOrder3D.m:1050-1145,1235-1290), so its sensor_data.p is the
+    record of a run without it.  Here the record stays in device memory for
+    the whole run (one float per sensor point and step), and the option is
+    checked and accepted."""
+    if "StreamToDisk" not in opts:
+        return
+    v = opts["StreamToDisk"]
+    if dim != 3:
+        raise ValueError("Optional input 'StreamToDisk' is currently only compatible with 3D fluid "
+                         "simulations.")
+    if not (np.size(v) == 1 and (isinstance(v, (bool, np.bool_)) or np.isscalar(v)
+                                 and isinstance(v, (int, float, np.number)))):
+        raise ValueError("Optional input 'StreamToDisk' must be a single scalar or Boolean value.")
+    if v is False or (not isinstance(v, (bool, np.bool_)) and float(v) == 0):
+        return
+    sen = None if sensor is None else (sensor if isinstance(sensor, dict) else vars(sensor))
+    mask = None if sen is None else sen.get("mask")
+    if (sen is None or mask is None or not np.any(np.asarray(mask))
+            or "time_reversal_boundary_data" in sen):
+        raise ValueError("The optional input 'StreamToDisk' is currently only compatible with "
+                         "forward simulations using a non-zero sensor mask.")
+    if "record" in sen and any(r != "p" for r in sen["record"]):
+        raise ValueError("The optional input 'StreamToDisk' is currently only compatible with "
+                         "sensor.record = {'p'} (the default).")
+
+
 def _solve(dim, kgrid, medium, source, sensor, args, kwargs):
     opts = _parse_options(args, kwargs)
     if "RadialSymmetry" in opts:
         raise ValueError("Unknown optional input 'RadialSymmetry'.")
     _check_use_fd(dim, opts)
+    _check_stream_to_disk(dim, sensor, opts)
     save = _save_to_disk_options(dim, opts)
     pp = prepare(kgrid, medium, source, sensor, dim, opts)
     if save is not None:
@@ -1508,6 +1541,7 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
     kspaceFirstOrder-HIP run; 'PMLInside' false grows the grid by the PML
     outside the axial ends and the outer radius."""
     opts = _parse_options(args, kwargs)
+    _check_stream_to_disk(2, sensor, opts)
     sym = opts.pop("RadialSymmetry", "WSWA-FFT")
     if sym not in ("WSWA", "WSWS", "WSWA-FFT", "WSWS-FFT"):
         raise ValueError("Optional input 'RadialSymmetry' must be set to 'WSWA', 'WSWS', "

@@ -152,3 +152,32 @@ def test_alpha_sign_and_pml_range_rounding():
     c = kwhip.kspaceFirstOrder2D(kg, base, {"p0": p0}, sens, Smooth=False, PMLSize=4)
     assert np.array_equal(a, b)
     assert not np.array_equal(a, c)
+
+
+def test_stream_to_disk():
+    """'StreamToDisk' (kspaceFirstOrder_checkOptionalInputs.m:353-374,
+    kspaceFirstOrder_inputChecking.m:1369-1374): 3-D fluid forward
+    simulations recording p only; the record is the one of a run without
+    it (the reference reads its streamed file back whole)."""
+    import kwhip
+    cs = cases.case("p0_hetero_absorbing", 3)
+    cs["sensor"] = {"mask": cs["sensor"]["mask"]}   # sensor.record absent: p
+    args = []
+    for n in cs["shape"]:
+        args += [n, cs["dx"]]
+    kg = kwhip.kWaveGrid(*args)
+    kg.setTime(cs["Nt"], cs["dt"])
+    run = lambda sen, **o: kwhip.kspaceFirstOrder3D(kg, cs["medium"], cs["source"], sen,  # noqa: E731
+                                                    **dict(cs["opts"], **o))
+    plain = run(cs["sensor"])
+    for v in (True, 5, 200):
+        assert np.array_equal(np.asarray(run(cs["sensor"], StreamToDisk=v)), np.asarray(plain))
+    with pytest.raises(ValueError, match="single scalar or Boolean"):
+        run(cs["sensor"], StreamToDisk=[1, 2])
+    with pytest.raises(ValueError, match="sensor.record = {'p'}"):
+        run(dict(cs["sensor"], record=["p", "p_max"]), StreamToDisk=True)
+    with pytest.raises(ValueError, match="non-zero sensor mask"):
+        run({"mask": np.zeros(cs["shape"])}, StreamToDisk=True)
+    with pytest.raises(ValueError, match="only compatible with 3D fluid"):
+        kwhip.kspaceFirstOrder1D(_grid(), {"sound_speed": 1500.0}, {"p0": np.ones(24)},
+                                 {"mask": np.ones(24)}, StreamToDisk=True)
