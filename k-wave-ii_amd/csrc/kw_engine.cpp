@@ -262,6 +262,7 @@ struct Ticket;
 struct Group {
   int dim = 0;
   bool absorbing = false;
+  bool axisym = false;  // axisymmetric members: stepped one by one
   void* stream = nullptr;
   void* ev = nullptr;       // recorded after the last round issued
   bool in_flight = false;   // that round has not finished
@@ -294,6 +295,9 @@ struct Group {
   // host state that changes from step to step (the per-step source grids of
   // several-series additive sources are chosen on the host)
   bool graphable() const {
+    // an axisymmetric member's step decides on the host what it records and
+    // whether it stores p for p_final, which a replayed round would not redo
+    if (axisym) return false;
     for (Sim* s : sims)
       if (s->p_grid || s->u_grid[0] || s->u_grid[1] || s->u_grid[2]) return false;
     return true;
@@ -411,6 +415,16 @@ struct Group {
 // one time step (t >= 2) of every member: the sequence of step_once for a
 // single simulation, with each pass launched once for the group
 void group_step(Group& g) {
+  if (g.axisym) {
+    // the u_r / r term, the mirror kernels and the last radial column of
+    // 'WSWS-FFT' are single-simulation launches: members step one by one
+    // on the group's stream
+    for (Sim* s : g.sims) {
+      step_once(*s);
+      g.launches += 16;
+    }
+    return;
+  }
   for (Sim* s : g.sims) {
     const int t = (int)(s->steps_done + 1);
     if (s->p_grid) src_grid(*s, 0, t);
@@ -680,15 +694,18 @@ struct Engine {
   // variants exist for every axis of its grid, the same grid size (the group
   // then runs those variants); other sizes share one group per pass sequence
   // and run the runtime-length variants.
-  static std::array<int, 5> group_key(const Sim& s) {
+  static std::array<int, 6> group_key(const Sim& s) {
     const int dim = s.G.dim;
     const int absorbing = s.P.eos == EOS_ABSORBING ? 1 : 0;
     const int Nl = dim == 3 ? s.G.Nz : s.G.Ny;
     const bool fixed = !std::getenv("KW_NO_FIXED") && fixed_variant(s.G.Nx, PK_X_GRAD) != 0 &&
                        (dim == 2 || fixed_variant(s.G.Ny, PK_YFWD) != 0) &&
                        fixed_variant(Nl, PK_SPEC) != 0;
-    if (fixed) return {dim, absorbing, s.G.Nx, s.G.Ny, s.G.Nz};
-    return {dim, absorbing, 0, 0, 0};
+    // axisymmetric simulations (1 'WSWA-FFT', 3 'WSWS-FFT') group with
+    // their own kind and size only
+    const int as = s.axisym ? (s.as_ws ? 3 : 1) : 0;
+    if (fixed || as) return {dim, absorbing, s.G.Nx, s.G.Ny, s.G.Nz, as};
+    return {dim, absorbing, 0, 0, 0, 0};
   }
 
   // A group has at most one round (steps_per_round steps of all members) in
@@ -699,7 +716,7 @@ struct Engine {
   // so short simulations are not queued behind the rounds of long ones and
   // large groups keep the device busy while small ones wait on latency.
   void step_loop() {
-    std::map<std::array<int, 5>, std::unique_ptr<Group>> groups;
+    std::map<std::array<int, 6>, std::unique_ptr<Group>> groups;
     std::vector<char> stream_busy(streams.size(), 0);
     auto last = Clock::now();
     std::unique_lock<std::mutex> lk(mu);
@@ -726,6 +743,7 @@ struct Engine {
           g = std::make_unique<Group>();
           g->dim = key[0];
           g->absorbing = key[1] != 0;
+          g->axisym = key[5] != 0;
           g->ev = dev_event_create();
         }
         g->joining.push_back(t);
@@ -970,8 +988,9 @@ int kw_submit(kw_engine* engine, const kw_problem* prob, int64_t* ticket, char* 
   return kw_guarded(err, errlen, [&]() {
     if (!engine || !ticket) kw::fail(KW_EINVAL, "NULL argument");
     kw::check_shape(prob);
-    if (prob->axisymmetric)
-      kw::fail(KW_EUNSUP, "axisymmetric problems run through kw_run or kw_sim_*, not the engine");
+    if (prob->axisymmetric == 2)
+      kw::fail(KW_EINVAL, "axisymmetric = 2 (the radial grid of the file contract) is run by "
+                          "kw_run_file");
     auto t = std::make_unique<kw::Ticket>();
     t->prob = std::make_unique<kw::ProblemCopy>(*prob);
     t->est_bytes = kw::estimate_bytes(*prob);

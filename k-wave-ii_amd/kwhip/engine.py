@@ -19,7 +19,7 @@ import ctypes as C
 import time
 
 from . import _abi
-from .solver import _parse_options, alloc_outputs, prepare, shape_outputs
+from .solver import _parse_options, _prepare_as, alloc_outputs, prepare, shape_outputs
 
 
 class Ticket:
@@ -55,6 +55,20 @@ class Engine:
         tid = C.c_int64()
         _abi.call(self.lib.kw_submit, self.h, C.byref(pp.prob), C.byref(tid))
         pp.keep.clear()  # the engine holds its own copy of every array
+        pp.prob = None
+        return Ticket(self, tid.value, pp)
+
+    def submit_as(self, kgrid, medium, source, sensor, *args, **kwargs) -> Ticket:
+        """Validate and pre-process as kspaceFirstOrderAS (the axisymmetric
+        solver), copy the problem into the engine and return at once; its
+        steps run on the engine's streams beside the other simulations."""
+        opts = _parse_options(args, kwargs)
+        if opts.get("SaveToDisk"):
+            raise ValueError("'SaveToDisk' writes a file: call kspaceFirstOrderAS")
+        pp = _prepare_as(kgrid, medium, source, sensor, opts)
+        tid = C.c_int64()
+        _abi.call(self.lib.kw_submit, self.h, C.byref(pp.prob), C.byref(tid))
+        pp.keep.clear()
         pp.prob = None
         return Ticket(self, tid.value, pp)
 

@@ -1508,19 +1508,11 @@ def _as_pml_outside(kgrid, medium, source, sensor, sym, opts):
     kg = kWaveGrid(Nx + 2 * px, kgrid.dx, Ny + py, kgrid.dy)
     kg.setTime(kgrid.Nt, kgrid.dt)
     o = dict(opts, PMLInside=True, Smooth=False, PMLSize=[px, py], RadialSymmetry=sym)
-    out = kspaceFirstOrderAS(kg, med, src, sen, **o)
-
-    def cut(a):
-        a = np.asarray(a)
-        if a.shape != (Nx + 2 * px, Ny + py):
-            a = a.reshape((Nx + 2 * px, Ny + py), order="F")
-        return a[px:px + Nx, :Ny]
-
-    if isinstance(out, dict):
-        return {k: (cut(v) if k.endswith(("_final", "_all")) else v) for k, v in out.items()}
-    if isinstance(out, list):
-        return out   # 'SaveToDisk' wrote the expanded grid and exited
-    return cut(out)   # time reversal: p_final
+    pp = _prepare_as(kg, med, src, sen, o)
+    if pp is not None:
+        # grid outputs come back on the user's part of the radial grid
+        pp.info.inside = (slice(px, px + Nx), slice(0, Ny))
+    return pp
 
 
 def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa: N802
@@ -1541,6 +1533,17 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
     kspaceFirstOrder-HIP run; 'PMLInside' false grows the grid by the PML
     outside the axial ends and the outer radius."""
     opts = _parse_options(args, kwargs)
+    pp = _prepare_as(kgrid, medium, source, sensor, opts)
+    if pp is None:
+        return []   # 'SaveToDisk' wrote the input file and exited (kspaceFirstOrderAS.m:587-596)
+    return run_prepared(pp)
+
+
+def _prepare_as(kgrid, medium, source, sensor, opts):
+    """The axisymmetric problem prepared for kw_run (or the engine): the
+    PreparedProblem on the expanded grid, or None when 'SaveToDisk' wrote
+    the input file and exits."""
+    opts = dict(opts)
     _check_stream_to_disk(2, sensor, opts)
     sym = opts.pop("RadialSymmetry", "WSWA-FFT")
     if sym not in ("WSWA", "WSWS", "WSWA-FFT", "WSWS-FFT"):
@@ -1629,7 +1632,7 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
         rp.prob.axisymmetric = 2
         save_to_disk(rp, save)
         if opts.get("SaveToDiskExit", True):
-            return []
+            return None
     kg = kWaveGrid(Nx, kgrid.dx, M, kgrid.dy)
     kg.setTime(kgrid.Nt, kgrid.dt)
     emed = {}
@@ -1692,4 +1695,4 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
         pp.prob.rho0_sg[1] = _abi.dptr(ey)
     # grid outputs (_all, _final) come back on the radial grid
     pp.info.inside = (slice(None), slice(0, Ny))
-    return run_prepared(pp)
+    return pp

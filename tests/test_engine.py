@@ -207,3 +207,40 @@ def test_engine_isolates_diverging_request_emu(few_pencils):
 def test_engine_isolates_diverging_request_hip(hip_lib):
     print("[engine] one diverging request between two stable ones", flush=True)
     _isolation("hip")
+
+
+def _as_mix(backend, steps_per_round):
+    """Axisymmetric problems ('WSWA-FFT' and 'WSWS-FFT', with p0, a source
+    and time reversal) submitted beside a 2-D problem of the same expanded
+    size: each result equals kspaceFirstOrderAS run alone."""
+    import kwhip
+    from kwhip import _abi
+    import as_cases as ac
+    css = [ac.case("hetero"), ac.case("source_additive", sym="WSWS-FFT"),
+           ac.case("time_reversal"), ac.case("nonlinear", sym="WSWS-FFT"),
+           ac.case("stokes_nonlinear")]
+    plain = cases.case("p0_hetero_absorbing", 2, (32, 64))
+    with _abi._backend_override(backend), kwhip.Engine(steps_per_round=steps_per_round) as eng:
+        tickets = []
+        for cs in css:
+            kg = kwhip.kWaveGrid(cs["Nx"], cs["dx"], cs["Ny"], cs["dx"])
+            kg.setTime(cs["Nt"], cs["dt"])
+            tickets.append(eng.submit_as(kg, dict(cs["medium"]), dict(cs["source"]),
+                                         dict(cs["sensor"]), PMLSize=cs["pml"],
+                                         Smooth=cs["smooth"], RadialSymmetry=cs["sym"]))
+        tp = _submit(eng, plain)
+        got = [eng.wait(t) for t in tickets]
+        got_plain = eng.wait(tp)
+        st = eng.stats()
+    assert st["completed"] == len(css) + 1 and st["failed"] == 0
+    with _abi._backend_override(backend):
+        for cs, g in zip(css, got):
+            alone = ac.run_kwhip(cs)
+            if "time_reversal_boundary_data" in cs["sensor"]:
+                g = {"p_final": g}
+            _same(g, alone)
+    _same(got_plain, _alone(plain, backend))
+
+
+def test_engine_axisymmetric_emu(few_pencils):
+    _as_mix("emu", 5)

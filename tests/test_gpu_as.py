@@ -3,7 +3,8 @@ pressure, pressure and velocity sources in every mode, time reversal;
 lossless, Stokes, nonlinear; 'WSWA-FFT', and 'WSWS-FFT' for seven of them) on a 128 x 64 grid against the oracle (relative L-inf <= 1e-5
 single, <= 1e-4 double), the spherical wave of a pulse centred on the axis against its
 closed form (the scheme's discretisation error, as the oracle measures it),
-the engine's refusal of axisymmetric problems, and the file contract:
+the engine stepping axisymmetric problems beside others, and the file
+contract:
 'SaveToDisk' files run by kspaceFirstOrder-HIP against the oracle."""
 
 import numpy as np
@@ -63,21 +64,30 @@ def test_gpu_as_spherical_wave(hip_lib):
     assert err < 6e-4
 
 
-def test_gpu_engine_refuses_axisymmetric(hip_lib):
+def test_gpu_engine_refuses_the_file_layout(hip_lib):
+    # the radial grid of the file contract (axisymmetric = 2) is run by
+    # kw_run_file, not by the engine
     import kwhip
     from kwhip.solver import prepare, _parse_options
     cs = ac.case("homogeneous")
-    kg = kwhip.kWaveGrid(cs["Nx"], cs["dx"], 4 * cs["Ny"], cs["dx"])
+    kg = kwhip.kWaveGrid(cs["Nx"], cs["dx"], cs["Ny"], cs["dx"])
     kg.setTime(4, cs["dt"])
     with kwhip.Engine() as eng:
-        pp = prepare(kg, cs["medium"], {"p0": np.ones((cs["Nx"], 4 * cs["Ny"]))}, None, 2,
+        pp = prepare(kg, cs["medium"], {"p0": np.ones((cs["Nx"], cs["Ny"]))}, None, 2,
                      _parse_options((), {"Smooth": False, "PMLSize": 4}))
-        pp.prob.axisymmetric = 1
+        pp.prob.axisymmetric = 2
         import ctypes as C
         from kwhip import _abi
         tid = C.c_int64()
-        with pytest.raises(kwhip.KWError, match="axisymmetric"):
+        with pytest.raises(kwhip.KWError, match="kw_run_file"):
             _abi.call(eng.lib.kw_submit, eng.h, C.byref(pp.prob), C.byref(tid))
+
+
+def test_gpu_engine_axisymmetric(hip_lib):
+    # axisymmetric problems stepped by the engine beside a 2-D problem equal
+    # kspaceFirstOrderAS run alone (full rounds and the last partial one)
+    import test_engine
+    test_engine._as_mix("hip", 8)
 
 
 @pytest.mark.parametrize("name", ["hetero", "source_additive", "stokes_nonlinear",
