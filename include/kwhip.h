@@ -345,6 +345,8 @@ const char* kw_kind_name(int k);
  * grid size where fixed-length kernels exist for it), whatever their media,
  * sources, sensors and current step.  Groups run on a pool of streams and
  * each group's rounds of steps are captured once as a graph and replayed.
+ * Axisymmetric problems (kw_problem.axisymmetric 1 or 3) group with their
+ * own kind and size and step one by one on their group's stream.
  * Results are the same as kw_run on the same problem (the per-simulation
  * arithmetic does not change).  Every engine function is thread-safe.
  * ------------------------------------------------------------------------- */
