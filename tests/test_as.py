@@ -122,6 +122,7 @@ def test_as_file_options(tmp_path, capsys):
     out, v = ac.save_and_run_file(cs, tmp_path, _abi.load("emu"), record=("p",),
                                   RadialSymmetry="WSWS")
     assert "changed to 'WSWA'" in capsys.readouterr().out
+    assert int(v["axisymmetric_flag"].ravel()[0]) == 1
     ref = ac.run_oracle(dict(cs, sensor=dict(cs["sensor"], record=["p"])), "single")
     got = np.squeeze(out["p"]).astype(np.float64)
     assert np.max(np.abs(got - ref["p"])) <= TOL32 * np.max(np.abs(ref["p"]))
