@@ -155,6 +155,64 @@ def cfg1_leg():
             "note": "whole call (host set-up, transfers, 906 steps) on both sides"}
 
 
+def axisym_leg(Nx=512, Ny=256, Nt=2000, cpu_steps=20, check_steps=100):
+    """kspaceFirstOrderAS on an Nx x Ny radial grid (the GPU steps the
+    4 Ny expanded grid of 'WSWA-FFT'): heterogeneous c/rho, Stokes
+    absorption, B/A = 6, a focused p0 on the axis, a binary sensor line
+    recording p and p_max.  The whole call (host set-up, transfers, Nt
+    steps) on the GPU; the CPU restatement of kspaceFirstOrderAS.m in single
+    precision for cpu_steps steps; the relative L-inf difference of p_final
+    between the two after check_steps steps."""
+    import numpy as np
+
+    import kwhip
+    from oracle import kwave_oracle as ko
+
+    dx = 1e-4
+    rng = np.random.default_rng(11)
+    x = (np.arange(Nx) - Nx // 2) * dx
+    r = np.arange(Ny) * dx
+    X, R = np.meshgrid(x, r, indexing="ij")
+    c = 1500 + 60 * np.tanh((X + 8e-3) / 2e-3) + 5 * rng.standard_normal((Nx, Ny))
+    rho = 1000 + 40 * np.tanh((X + 8e-3) / 2e-3)
+    med = {"sound_speed": c, "density": rho, "alpha_coeff": 0.75, "BonA": 6.0}
+    p0 = 2e5 * np.exp(-((X + 1e-2) ** 2 + R ** 2) / (2e-3) ** 2)
+    mask = np.zeros((Nx, Ny), bool)
+    mask[Nx // 2 + 40, :Ny - 30] = True
+    dt = 0.2 * dx / float(c.max())
+
+    def run(nt, rec):
+        kg = kwhip.kWaveGrid(Nx, dx, Ny, dx)
+        kg.setTime(nt, dt)
+        return kwhip.kspaceFirstOrderAS(kg, med, {"p0": p0}, {"mask": mask, "record": rec},
+                                        PMLSize=[20, 20], Smooth=False)
+
+    run(10, ["p"])  # warm-up (host set-up paths)
+    t0 = time.perf_counter()
+    run(Nt, ["p", "p_max"])
+    t_gpu = time.perf_counter() - t0
+    got = run(check_steps, ["p_final"])["p_final"]
+    kgo = ko.KGrid(Nx, dx, Ny, dx)
+    kgo.set_time(check_steps, dt)
+    ref = ko.kspace_first_order_as(kgo, med, {"p0": p0}, {"mask": mask, "record": ["p_final"]},
+                                   data_cast="single", pml_size=[20, 20], smooth=False)["p_final"]
+    kgc = ko.KGrid(Nx, dx, Ny, dx)
+    kgc.set_time(cpu_steps, dt)
+    t0 = time.perf_counter()
+    ko.kspace_first_order_as(kgc, med, {"p0": p0}, {"mask": mask, "record": ["p"]},
+                             data_cast="single", pml_size=[20, 20], smooth=False)
+    t_cpu = (time.perf_counter() - t0) / cpu_steps
+    cells = Nx * Ny
+    return {"grid": [Nx, Ny], "expanded": [Nx, 4 * Ny], "steps": Nt,
+            "gpu_s": round(t_gpu, 3), "gpu_ms_per_step": round(1e3 * t_gpu / Nt, 4),
+            "gpu_Mcell_steps_s": round(cells * Nt / t_gpu / 1e6, 1),
+            "cpu_fp32_ms_per_step": round(1e3 * t_cpu, 2),
+            "cpu_fp32_Mcell_steps_s": round(cells / t_cpu / 1e6, 2),
+            f"rel_linf_p_final_{check_steps}_steps_vs_fp32": ko.rel_linf(got, ref),
+            "note": "radial cells; GPU whole call (host set-up, transfers, steps); CPU: "
+                    f"the oracle's kspaceFirstOrderAS restatement, {cpu_steps} steps"}
+
+
 def cfg3_leg(N=384, Nt=2000, steps=None):
     """The nonlinear precision-study workload (B/A = 6, power-law absorption,
     1 MPa 0.5 MHz bowl source, 64 x 64 sensor plane recording p, p_max and
@@ -637,6 +695,9 @@ def main():
     ap.add_argument("--cfg3", type=int, default=384,
                     help="grid size of the nonlinear precision-study leg (0: skip)")
     ap.add_argument("--cfg3-steps", type=int, default=2000)
+    ap.add_argument("--axisym", type=int, nargs=2, default=[0, 0], metavar=("NX", "NY"),
+                    help="also run kspaceFirstOrderAS on an NX x NY radial grid (0 0: skip)")
+    ap.add_argument("--axisym-steps", type=int, default=2000)
     ap.add_argument("--batched", type=int, default=96,
                     help="problems in the batched time-reversal leg (0: skip)")
     ap.add_argument("--served", type=int, default=24,
@@ -784,6 +845,13 @@ def main():
         c3 = cfg3_leg(args.cfg3, 2000, args.cfg3_steps)
         log(f"[bench] nonlinear bowl workload: {c3['value']} Mcell-steps/s")
 
+    axs = None
+    if args.axisym[0] > 0:
+        log(f"[bench] kspaceFirstOrderAS {args.axisym[0]} x {args.axisym[1]} x "
+            f"{args.axisym_steps} steps, GPU and CPU (single precision)")
+        axs = axisym_leg(args.axisym[0], args.axisym[1], args.axisym_steps)
+        log(f"[bench] kspaceFirstOrderAS: {axs['gpu_Mcell_steps_s']} Mcell-steps/s")
+
     batched = batched_bench(args.batched) if args.batched > 0 else None
     served = (served_bench(args.served, args.served_rate, *args.served_steps)
               if args.served > 0 else None)
@@ -803,6 +871,7 @@ def main():
         "rel_err_64cubed_300steps": acc,
         "cfg1_2d_128": c1,
         "cfg3_nonlinear": c3,
+        "axisymmetric": axs,
         "device_gb": round(dev_gb, 3),
         "kernels": kernels,
         "batched": batched,
