@@ -81,7 +81,7 @@ def cpu_phase(N, nsteps, cores, cast, source_on):
     from oracle import kwave_oracle as ko
 
     Nt = 2 + nsteps
-    kg, med, src, sens, opts = workloads.cfg2(N, Nt=Nt)
+    kg, med, src, sens, opts = workloads.disc_source_3d(N, Nt=Nt)
     if not source_on:
         src = dict(src, p=src["p"][:, :1])
     kgo = ko.KGrid(N, kg.dx, N, kg.dy, N, kg.dz)
@@ -109,13 +109,13 @@ def cpu_baseline(N, nsteps, cores, casts=("single",), n_on=534, n_total=1000):
 
 
 def accuracy_check(N=64, Nt=300):
-    """cfg2 recipe at N^3 for Nt steps: GPU vs the FP64 and FP32 oracle
+    """The disc-source recipe at N^3 for Nt steps: GPU vs the FP64 and FP32 oracle
     (relative L-inf of p_max and p_rms)."""
     import kwhip
     from kwhip import workloads
     from oracle import kwave_oracle as ko
 
-    kg, med, src, sens, opts = workloads.cfg2(N, Nt=Nt, pml=10, src_radius=max(4, N // 8))
+    kg, med, src, sens, opts = workloads.disc_source_3d(N, Nt=Nt, pml=10, src_radius=max(4, N // 8))
     got = kwhip.kspaceFirstOrder3D(kg, med, src, sens, **opts)
     kgo = ko.KGrid(N, kg.dx, N, kg.dy, N, kg.dz)
     kgo.set_time(Nt, kg.dt)
@@ -127,7 +127,7 @@ def accuracy_check(N=64, Nt=300):
     return res
 
 
-def cfg1_leg():
+def circle_2d_leg():
     """The 2-D 128^2 lossless workload (Cartesian circle of 50 points,
     906 steps) in full: the GPU against the CPU restatement of the reference
     algorithm in double precision (the reference's default), both timed, and
@@ -136,7 +136,7 @@ def cfg1_leg():
     from kwhip import workloads
     from oracle import kwave_oracle as ko
 
-    kg, med, src, sens, opts = workloads.cfg1()
+    kg, med, src, sens, opts = workloads.circle_2d()
     kwhip.kspaceFirstOrder2D(kg, med, src, sens, **opts)  # warm-up (host set-up paths)
     t0 = time.perf_counter()
     got = kwhip.kspaceFirstOrder2D(kg, med, src, sens, **opts)
@@ -213,12 +213,12 @@ def axisym_leg(Nx=512, Ny=256, Nt=2000, cpu_steps=20, check_steps=100):
                     f"the oracle's kspaceFirstOrderAS restatement, {cpu_steps} steps"}
 
 
-def cfg3_leg(N=384, Nt=2000, steps=None):
+def nonlinear_bowl_leg(N=384, Nt=2000, steps=None):
     """The nonlinear precision-study workload (B/A = 6, power-law absorption,
     1 MPa 0.5 MHz bowl source, 64 x 64 sensor plane recording p, p_max and
     p_rms): throughput of the whole run at N^3 on the GPU (inputs resident),
     and its error against the oracle from the committed 2000-step fixtures
-    of the same recipe at smaller sizes (tests/golden/cfg3_<n>.npz: single-
+    of the same recipe at smaller sizes (tests/golden/nonlinear_bowl_<n>.npz: single-
     and double-precision oracle runs, p on a subsampled set of plane points
     and steps, p_max and p_rms on the whole plane)."""
     import numpy as np
@@ -227,7 +227,7 @@ def cfg3_leg(N=384, Nt=2000, steps=None):
     from kwhip import workloads
 
     steps = Nt if steps is None else steps
-    kg, med, src, sens, opts, zf = workloads.cfg3(N, Nt=Nt)
+    kg, med, src, sens, opts, zf = workloads.nonlinear_bowl_3d(N, Nt=Nt)
     sim = kwhip.Simulation(kg, med, src, sens, 3, **opts)
     sim.sync()
     t0 = time.perf_counter()
@@ -246,13 +246,13 @@ def cfg3_leg(N=384, Nt=2000, steps=None):
     import make_longrun as ml
     errs = {}
     for n in (96, 192):
-        path = ROOT / "tests" / "golden" / f"cfg3_{n}.npz"
+        path = ROOT / "tests" / "golden" / f"nonlinear_bowl_{n}.npz"
         if not path.exists():
             continue
         fx = np.load(path)
-        kgs, meds, srcs, senss, optss = ml.cfg3_inputs(n)
+        kgs, meds, srcs, senss, optss = ml.bowl_inputs(n)
         got = kwhip.kspaceFirstOrder3D(kgs, meds, srcs, senss, **optss)
-        gp = ml.cfg3_p_sub(got["p"]).astype(np.float64)
+        gp = ml.bowl_p_sub(got["p"]).astype(np.float64)
         e = {}
         for tag in ("fp32", "fp64"):
             r = fx[f"{tag}_p"].astype(np.float64)
@@ -582,7 +582,7 @@ def _counter_means(run_dir, counter, name_part):
 def reduce_traffic(fetch_dir, write_dir, cal_fetch_dir, cal_write_dir, kernel, variant, N):
     """HBM bytes per launch of one kernel from FETCH_SIZE and WRITE_SIZE runs
     (KiB per dispatch), each scaled by factor = known bytes / counter bytes of
-    the calibration copy; written to profiles/traffic_<kernel>_cfg2_<N>.json,
+    the calibration copy; written to profiles/traffic_<kernel>_disc_source_<N>.json,
     which the bench reports as roofline.traffic."""
     # "(" ends the kernel name: the record variant (<variant>_rec) is not it
     f_kib, nf = _counter_means(fetch_dir, "FETCH_SIZE", variant + "(")
@@ -599,7 +599,7 @@ def reduce_traffic(fetch_dir, write_dir, cal_fetch_dir, cal_write_dir, kernel, v
            "fetch_kib_raw": f_kib, "write_kib_raw": w_kib, "dispatches": [nf, nw],
            "copy_fetch_kib": cf_kib, "copy_write_kib": cw_kib,
            "fetch_factor": f_scale, "write_factor": w_scale}
-    path = ROOT / "profiles" / f"traffic_{kernel}_cfg2_{N}.json"
+    path = ROOT / "profiles" / f"traffic_{kernel}_disc_source_{N}.json"
     path.write_text(json.dumps(out, indent=1) + "\n")
     print(json.dumps(out))
 
@@ -691,10 +691,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-accuracy", action="store_true")
     ap.add_argument("--no-profile", action="store_true")
-    ap.add_argument("--no-cfg1", action="store_true")
-    ap.add_argument("--cfg3", type=int, default=384,
+    ap.add_argument("--no-circle-2d", action="store_true")
+    ap.add_argument("--bowl", type=int, default=384,
                     help="grid size of the nonlinear precision-study leg (0: skip)")
-    ap.add_argument("--cfg3-steps", type=int, default=2000)
+    ap.add_argument("--bowl-steps", type=int, default=2000)
     ap.add_argument("--axisym", type=int, nargs=2, default=[0, 0], metavar=("NX", "NY"),
                     help="also run kspaceFirstOrderAS on an NX x NY radial grid (0 0: skip)")
     ap.add_argument("--axisym-steps", type=int, default=2000)
@@ -739,7 +739,7 @@ def main():
     N, K, W = args.N, args.steps, args.warmup
     Nt = max(1000, K)
     log(f"[bench] building the {N}^3 workload")
-    kg, med, src, sens, opts = workloads.cfg2(N, Nt=Nt)
+    kg, med, src, sens, opts = workloads.disc_source_3d(N, Nt=Nt)
     ncell = N ** 3
     n_src_on = src["p"].shape[1]
 
@@ -794,7 +794,7 @@ def main():
         avg_s = d["avg_us"] * 1e-6
         achieved = d["alg_bytes_per_launch"] / avg_s / 1e9
         traffic = None
-        tf = ROOT / "profiles" / f"traffic_{dom}_cfg2_{N}.json"
+        tf = ROOT / "profiles" / f"traffic_{dom}_disc_source_{N}.json"
         if tf.exists():
             traffic = json.loads(tf.read_text()).get("bytes_per_launch")
         # the per-axis-pass model counts 8 B per cell for every axis transform
@@ -836,13 +836,13 @@ def main():
         acc = accuracy_check()
 
     c1 = None
-    if not args.no_cfg1:
+    if not args.no_circle_2d:
         log("[bench] 2-D 128^2 lossless workload in full, GPU and CPU (double precision)")
-        c1 = cfg1_leg()
+        c1 = circle_2d_leg()
     c3 = None
-    if args.cfg3 > 0:
-        log(f"[bench] nonlinear bowl workload {args.cfg3}^3 x {args.cfg3_steps} steps")
-        c3 = cfg3_leg(args.cfg3, 2000, args.cfg3_steps)
+    if args.bowl > 0:
+        log(f"[bench] nonlinear bowl workload {args.bowl}^3 x {args.bowl_steps} steps")
+        c3 = nonlinear_bowl_leg(args.bowl, 2000, args.bowl_steps)
         log(f"[bench] nonlinear bowl workload: {c3['value']} Mcell-steps/s")
 
     axs = None
@@ -869,8 +869,8 @@ def main():
                           "model": "per-axis-pass algorithmic bytes / 8 TB/s"},
         "cpu_baseline": cpu,
         "rel_err_64cubed_300steps": acc,
-        "cfg1_2d_128": c1,
-        "cfg3_nonlinear": c3,
+        "circle_2d_128": c1,
+        "nonlinear_bowl_384": c3,
         "axisymmetric": axs,
         "device_gb": round(dev_gb, 3),
         "kernels": kernels,

@@ -1,17 +1,17 @@
 """Synthetic workloads of the benchmark configurations, as reference-style
 (kgrid, medium, source, sensor, options) inputs.
 
-* ``cfg1``: 2D 128x128, dx = dy = 0.1 mm, homogeneous lossless medium, PML 20
+* ``circle_2d``: 2D 128x128, dx = dy = 0.1 mm, homogeneous lossless medium, PML 20
   (alpha 2), smoothed disc p0 of radius 5 at the centre, CFL 0.3,
   t_end = 1.5 x diagonal / c0, 50 Cartesian sensor points on a circle of
   radius 40 grid points recording p.
-* ``cfg2``: 3D N^3 (N = 256 in the benchmark), dx = 0.1 mm, sound speed
+* ``disc_source_3d``: 3D N^3 (N = 256 in the benchmark), dx = 0.1 mm, sound speed
   1500 + 100 g clipped to [1400, 1600] m/s and density 1000 + 100 g clipped to
   [900, 1100] kg/m^3 with g Gaussian-smoothed (sigma 4 points) uniform noise
   (seeds 0 and 1), power-law absorption 0.75 dB/(MHz^1.5 cm), PML 10, a 1 MHz
   10-cycle Hann-windowed 1 Pa tone burst on a disc of radius 30 points in the
   plane z = PML + 2, CFL 0.3, 1000 steps, full-grid p_max and p_rms.
-* ``cfg3``: 3D N^3 (N = 384 in the precision study), the cfg2 medium recipe
+* ``nonlinear_bowl_3d``: 3D N^3 (N = 384 in the precision study), the disc_source_3d medium recipe
   with B/A = 6, a 0.5 MHz 5-cycle Hann-windowed 1 MPa tone burst on a
   spherical-cap bowl (radius of curvature 150 points, aperture 120 points
   at N = 384, scaled with N) whose rear sits on the axis at z = PML + 2,
@@ -70,7 +70,7 @@ def cart_circle(radius, num_points):
     return np.stack([radius * np.cos(a), -radius * np.sin(a)])
 
 
-def cfg1(N=128):
+def circle_2d(N=128):
     dx = 1e-4
     c0, rho0 = 1500.0, 1000.0
     kg = kWaveGrid(N, dx, N, dx)
@@ -83,7 +83,7 @@ def cfg1(N=128):
     return kg, medium, source, sensor, opts
 
 
-def cfg2(N=256, Nt=1000, pml=10, src_radius=30, seeds=(0, 1)):
+def disc_source_3d(N=256, Nt=1000, pml=10, src_radius=30, seeds=(0, 1)):
     dx = 1e-4
     shape = (N, N, N)
     c = np.clip(1500.0 + 100.0 * smoothed_noise(shape, seeds[0], 4.0), 1400.0, 1600.0)
@@ -131,7 +131,7 @@ def bowl_mask(shape, apex, radius, diameter, focus):
     return mask
 
 
-def cfg3(N=384, Nt=2000, pml=10, seeds=(0, 1), plane=64):
+def nonlinear_bowl_3d(N=384, Nt=2000, pml=10, seeds=(0, 1), plane=64):
     """Precision-study workload (nonlinear absorbing, bowl source); returns
     (kgrid, medium, source, sensor, opts) and the 0-based z index of the
     sensor plane."""

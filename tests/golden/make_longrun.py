@@ -2,11 +2,11 @@
 full length through the oracle, in single precision (the reference's
 'DataCast','single') and in double precision.
 
-* cfg2_long_<N>.npz: the 256^3 benchmark recipe (heterogeneous c and rho,
+* disc_source_long_<N>.npz: the 256^3 benchmark recipe (heterogeneous c and rho,
   power-law absorption y = 1.5, additive tone-burst disc source) at N^3 for
   1000 steps, the source on for the first 529-534 steps and off for the
   rest; p_max, p_rms and p_final on every `stride`-th cell along each axis.
-* cfg3_<N>.npz: the nonlinear precision-study recipe (B/A = 6, 1 MPa
+* nonlinear_bowl_<N>.npz: the nonlinear precision-study recipe (B/A = 6, 1 MPa
   0.5 MHz burst on a bowl) at N^3 for 2000 steps; the sensor is the 64 x 64
   plane through the focus: p_max and p_rms at every plane point and the p
   time series at every 4th point along x and y, every 4th step.
@@ -16,10 +16,10 @@ The oracle takes minutes to tens of minutes per fixture, so the GPU tests
 it.  Each fixture stores the sizes, the subsampling and a CRC of the inputs.
 
 Run from the repository root:
-    python tests/golden/make_longrun.py cfg2 64
-    python tests/golden/make_longrun.py cfg2 128
-    python tests/golden/make_longrun.py cfg3 96
-    python tests/golden/make_longrun.py cfg3 192   (about 2 h on 8 cores)
+    python tests/golden/make_longrun.py disc 64
+    python tests/golden/make_longrun.py disc 128
+    python tests/golden/make_longrun.py bowl 96
+    python tests/golden/make_longrun.py bowl 192   (about 2 h on 8 cores)
 """
 
 import sys
@@ -33,25 +33,25 @@ HERE = Path(__file__).resolve().parent
 ROOT = HERE.parent.parent
 sys.path[:0] = [str(ROOT), str(ROOT / "k-wave-ii_amd")]
 
-CFG2_STEPS = 1000
-CFG3_STEPS = 2000
-CFG3_SUB = 4      # plane points along x and y, and time steps, of the p series
+DISC_STEPS = 1000
+BOWL_STEPS = 2000
+BOWL_SUB = 4      # plane points along x and y, and time steps, of the p series
 
 
-def cfg2_inputs(N, Nt=CFG2_STEPS):
+def disc_inputs(N, Nt=DISC_STEPS):
     from kwhip import workloads
-    kg, med, src, sens, opts = workloads.cfg2(N, Nt=Nt, pml=10, src_radius=max(4, N // 8))
+    kg, med, src, sens, opts = workloads.disc_source_3d(N, Nt=Nt, pml=10, src_radius=max(4, N // 8))
     sens = {"mask": sens["mask"], "record": ["p_max", "p_rms", "p_final"]}
     return kg, med, src, sens, opts
 
 
-def cfg3_inputs(N, Nt=CFG3_STEPS):
+def bowl_inputs(N, Nt=BOWL_STEPS):
     from kwhip import workloads
-    kg, med, src, sens, opts, zf = workloads.cfg3(N, Nt=Nt)
+    kg, med, src, sens, opts, zf = workloads.nonlinear_bowl_3d(N, Nt=Nt)
     return kg, med, src, sens, opts
 
 
-def cfg2_stride(N):
+def disc_stride(N):
     return max(1, N // 32)
 
 
@@ -73,17 +73,17 @@ def input_crc(med, src):
     return crc
 
 
-def cfg3_p_sub(p):
+def bowl_p_sub(p):
     """The fixture's subset of the plane's p series [4096 x Nt] (plane points
-    in column-major order): every CFG3_SUB-th point along x and y, every
-    CFG3_SUB-th step."""
+    in column-major order): every BOWL_SUB-th point along x and y, every
+    BOWL_SUB-th step."""
     p = np.asarray(p).reshape(64, 64, -1, order="F")
-    return p[::CFG3_SUB, ::CFG3_SUB, ::CFG3_SUB]
+    return p[::BOWL_SUB, ::BOWL_SUB, ::BOWL_SUB]
 
 
 def run(kind, N):
     from oracle import kwave_oracle as ko
-    kg, med, src, sens, opts = (cfg2_inputs if kind == "cfg2" else cfg3_inputs)(N)
+    kg, med, src, sens, opts = (disc_inputs if kind == "disc" else bowl_inputs)(N)
     kgo = ko.KGrid(N, kg.dx, N, kg.dy, N, kg.dz)
     kgo.set_time(kg.Nt, kg.dt)
     arrays = {}
@@ -92,8 +92,8 @@ def run(kind, N):
         ref = ko.kspace_first_order(kgo, med, src, sens, data_cast=cast, pml_size=opts["PMLSize"],
                                     pml_alpha=opts["PMLAlpha"])
         print(f"{kind} {N}^3 {tag}: {time.time() - t0:.0f} s", flush=True)
-        if kind == "cfg2":
-            s = cfg2_stride(N)
+        if kind == "disc":
+            s = disc_stride(N)
             for k in ("p_max", "p_rms", "p_final"):
                 v = as_grid(ref[k], N)
                 arrays[f"{tag}_{k}"] = v[::s, ::s, ::s].astype(np.float32)
@@ -101,12 +101,12 @@ def run(kind, N):
         else:
             for k in ("p_max", "p_rms"):
                 arrays[f"{tag}_{k}"] = np.asarray(ref[k], dtype=np.float32).ravel()
-            arrays[f"{tag}_p"] = cfg3_p_sub(ref["p"]).astype(np.float32)
+            arrays[f"{tag}_p"] = bowl_p_sub(ref["p"]).astype(np.float32)
             arrays[f"{tag}_p_absmax"] = np.float64(np.max(np.abs(ref["p"])))
             arrays[f"{tag}_p_l2"] = np.float64(np.sqrt(np.sum(np.asarray(ref["p"], np.float64) ** 2)))
-    out = HERE / f"{kind}_{'long_' if kind == 'cfg2' else ''}{N}.npz"
+    out = HERE / (f"disc_source_long_{N}.npz" if kind == "disc" else f"nonlinear_bowl_{N}.npz")
     np.savez_compressed(out, N=N, Nt=kg.Nt, crc=np.uint32(input_crc(med, src)),
-                        stride=np.int64(cfg2_stride(N) if kind == "cfg2" else CFG3_SUB), **arrays)
+                        stride=np.int64(disc_stride(N) if kind == "disc" else BOWL_SUB), **arrays)
     print(out, flush=True)
 
 

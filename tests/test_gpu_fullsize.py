@@ -30,7 +30,7 @@ def test_fullsize_workload_first_steps(hip_lib):
     from oracle import kwave_oracle as ko
 
     Nt = 3
-    kg, med, src, sens, opts = workloads.cfg2(256, Nt=Nt)
+    kg, med, src, sens, opts = workloads.disc_source_3d(256, Nt=Nt)
     src["p"] = src["p"][:, 250:]
     sens = {"mask": sens["mask"], "record": ["p_max", "p_rms", "p_final"]}
     print("[fullsize] 256^3 workload, 3 steps", flush=True)
@@ -72,7 +72,7 @@ def test_fullsize_2d_initial_value_workload(hip_lib):
     from kwhip import workloads
     from oracle import kwave_oracle as ko
 
-    kg, med, src, sens, opts = workloads.cfg1(128)
+    kg, med, src, sens, opts = workloads.circle_2d(128)
     print(f"[fullsize] 128^2 initial value, {kg.Nt} steps", flush=True)
     got = kwhip.kspaceFirstOrder2D(kg, med, src, sens, **opts)
     kgo = ko.KGrid(128, kg.dx, 128, kg.dy)

@@ -46,15 +46,15 @@ def _fixture(name):
 
 
 @pytest.mark.parametrize("N", [64, 128])
-def test_cfg2_recipe_full_length(N, hip_lib):
+def test_disc_source_recipe_full_length(N, hip_lib):
     import kwhip
     import make_longrun as ml
 
-    fx = _fixture(f"cfg2_long_{N}.npz")
-    kg, med, src, sens, opts = ml.cfg2_inputs(N)
+    fx = _fixture(f"disc_source_long_{N}.npz")
+    kg, med, src, sens, opts = ml.disc_inputs(N)
     assert int(fx["crc"]) == ml.input_crc(med, src)
     n_on = src["p"].shape[1]
-    print(f"[longrun] cfg2 recipe {N}^3, {kg.Nt} steps, source on for {n_on}", flush=True)
+    print(f"[longrun] disc-source recipe {N}^3, {kg.Nt} steps, source on for {n_on}", flush=True)
     assert n_on < kg.Nt
     got = kwhip.kspaceFirstOrder3D(kg, med, src, sens, **opts)
     s = int(fx["stride"])
@@ -71,26 +71,26 @@ def test_cfg2_recipe_full_length(N, hip_lib):
 
 
 @pytest.mark.parametrize("N", [96, 192])
-def test_cfg3_recipe(N, hip_lib):
+def test_nonlinear_bowl_recipe(N, hip_lib):
     import kwhip
     import make_longrun as ml
 
-    fx = _fixture(f"cfg3_{N}.npz")
-    kg, med, src, sens, opts = ml.cfg3_inputs(N)
+    fx = _fixture(f"nonlinear_bowl_{N}.npz")
+    kg, med, src, sens, opts = ml.bowl_inputs(N)
     assert int(fx["crc"]) == ml.input_crc(med, src)
-    print(f"[longrun] cfg3 recipe {N}^3, {kg.Nt} steps, bowl of {int(src['p_mask'].sum())} points",
+    print(f"[longrun] nonlinear bowl recipe {N}^3, {kg.Nt} steps, bowl of {int(src['p_mask'].sum())} points",
           flush=True)
     got = kwhip.kspaceFirstOrder3D(kg, med, src, sens, **opts)
-    gp = ml.cfg3_p_sub(got["p"]).astype(np.float64)
+    gp = ml.bowl_p_sub(got["p"]).astype(np.float64)
     for tag, tol in (("fp32", TOL32), ("fp64", TOL64)):
         r = fx[f"{tag}_p"].astype(np.float64)
         linf = np.max(np.abs(gp - r)) / float(fx[f"{tag}_p_absmax"])
         l2 = np.sqrt(np.sum((gp - r) ** 2) / np.sum(r ** 2))
-        print(f"[longrun] cfg3 {N}^3 p vs {tag} oracle: rel L-inf {linf:.2e}, rel L2 {l2:.2e}", flush=True)
+        print(f"[longrun] nonlinear bowl {N}^3 p vs {tag} oracle: rel L-inf {linf:.2e}, rel L2 {l2:.2e}", flush=True)
         assert linf <= tol, (N, tag, "p", linf)
         for k in ("p_max", "p_rms"):
             g = np.asarray(got[k], dtype=np.float64).ravel()
             r = fx[f"{tag}_{k}"].astype(np.float64)
             err = np.max(np.abs(g - r)) / np.max(np.abs(r))
-            print(f"[longrun] cfg3 {N}^3 {k} vs {tag} oracle: rel L-inf {err:.2e}", flush=True)
+            print(f"[longrun] nonlinear bowl {N}^3 {k} vs {tag} oracle: rel L-inf {err:.2e}", flush=True)
             assert err <= tol, (N, tag, k, err)
