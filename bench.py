@@ -51,7 +51,7 @@ B_ALG_SRC_ON, B_ALG_SRC_OFF = 492.0, 488.0
 #   x_eos    : 2 inverse + 1 forward x passes; sum(rho), c0, tau,
 #              eta; p_max and p_rms r/w in FP64                   72
 # (452 B source off: the kinds omit the intermediate p write/read and a few
-# operands the survey's per-step total counts.)
+# operands that the 488 B per-step total counts.)
 KIND_B_STEP = {"y_fwd": 48.0, "y_inv": 64.0, "spectral": 112.0, "x_grad": 84.0,
                "x_div": 72.0, "x_eos": 72.0}
 KIND_B_SRC_ON = {"x_div": 4.0}
