@@ -190,7 +190,8 @@ typedef struct kw_problem {
    * (with or without B/A), p0, pressure and axial velocity sources and
    * time-reversal data given on the expanded grid with the signs of the
    * p-like layout, radial velocity sources with those of the y-staggered
-   * layout, and binary sensors.  2: the radial grid
+   * layout, and binary or interpolated (sensor_tri on the first copy)
+   * sensors.  2: the radial grid
    * itself (Ny the radial size, inputs as kspaceFirstOrderAS.m prepares
    * them), the layout of the file contract: kw_save_problem writes it with
    * axisymmetric_flag = 1 and kw_run_file lays such a file out on the

@@ -350,8 +350,10 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream, Arena* ar
     // column, which the reference updates outside its layout.
     if (S->as_ws && pr->u_source[1].flag > 0)
       fail(KW_EUNSUP, "radial velocity sources (source.uy) with 'WSWS-FFT' are not supported here");
-    if (pr->sensor_tri || pr->directivity_angle || (pr->record_flags & KW_REC_U_SPLIT_FIELD))
-      fail(KW_EUNSUP, "axisymmetric simulations here take binary sensor masks");
+    // sensor points are grid points or simplices of the first copy (the
+    // radial grid); directivity and the velocity split are 2-D options
+    if (pr->directivity_angle || (pr->record_flags & KW_REC_U_SPLIT_FIELD))
+      fail(KW_EUNSUP, "axisymmetric simulations take no sensor directivity or u_split_field");
   }
   if (P.eos < 0 || P.eos > 2) fail(KW_EINVAL, "absorbing_flag must be 0, 1 or 2");
   const double y = P.eos == EOS_STOKES ? 2.0 : pr->alpha_power;

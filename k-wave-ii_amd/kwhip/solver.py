@@ -197,10 +197,12 @@ def smooth(mat, restore_max=False):
 # Cartesian sensor masks (+kwave/+legacy/cart2grid.m, nearest)
 # ---------------------------------------------------------------------------
 
-def cart2grid(kgrid: kWaveGrid, cart_data):
+def cart2grid(kgrid: kWaveGrid, cart_data, radial_ny=None):
     """cart2grid.m:59-194: nearest grid point with MATLAB round (half away
     from zero), later points overwrite earlier ones.  Returns the binary mask,
-    order_index and reorder_index (1-based)."""
+    order_index and reorder_index (1-based).  radial_ny: the axisymmetric
+    form (:111-116), y the radius from the first grid row, which lies in
+    the first radial_ny rows."""
     cart = np.asarray(cart_data, dtype=np.float64)
     if cart.ndim != 2 or cart.shape[0] != kgrid.dim:
         raise ValueError(f"Cartesian sensor.mask for a {kgrid.dim}D simulation must be given as a {kgrid.dim} by N array.")
@@ -208,8 +210,9 @@ def cart2grid(kgrid: kWaveGrid, cart_data):
     for ax in range(kgrid.dim):
         v = cart[ax] / kgrid.spacings[ax]
         v = np.sign(v) * np.floor(np.abs(v) + 0.5)
-        idx.append(v.astype(np.int64) + kgrid.shape[ax] // 2 + 1)
-        if idx[ax].max() > kgrid.shape[ax] or idx[ax].min() < 1:
+        radial = radial_ny is not None and ax == 1
+        idx.append(v.astype(np.int64) + (1 if radial else kgrid.shape[ax] // 2 + 1))
+        if idx[ax].max() > (radial_ny if radial else kgrid.shape[ax]) or idx[ax].min() < 1:
             raise ValueError("Cartesian points must lie within the grid defined by kgrid.")
     grid = np.zeros(kgrid.shape, dtype=np.int64)
     for p in range(cart.shape[1]):
@@ -302,7 +305,7 @@ def _split_cuboids(sd: dict, corners, dim):
     return out
 
 
-def grid_data_linear(kgrid: kWaveGrid, cart_data):
+def grid_data_linear(kgrid: kWaveGrid, cart_data, radial_ny=None):
     """record.tri / record.bc for 'CartInterp' 'linear'
     (kspaceFirstOrder_createStorageVariables.m:376-398 with gridDataFast2D.m /
     gridDataFast3D.m): for each Cartesian point the simplex of a Delaunay
@@ -319,13 +322,19 @@ def grid_data_linear(kgrid: kWaveGrid, cart_data):
     diagonal from (i, j) to (i+1, j+1)).
 
     Returns tri ([points, dim+1], 1-based column-major grid indices) and bc
-    (same shape, float64)."""
+    (same shape, float64).  radial_ny: the axisymmetric grid, y the radius
+    from the first grid row (kspaceFirstOrder_createStorageVariables.m:
+    390-391 shifts y by its smallest value), inside the first radial_ny
+    rows."""
     cart = np.asarray(cart_data, dtype=np.float64)
     dim, shape = kgrid.dim, kgrid.shape
     npts = cart.shape[1]
     # grid point n (0-based) of axis a sits at (n - floor(N_a/2)) d_a
-    u = np.stack([cart[a] / kgrid.spacings[a] + shape[a] // 2 for a in range(dim)])
+    u = np.stack([cart[a] / kgrid.spacings[a] + (0 if radial_ny is not None and a == 1
+                                                   else shape[a] // 2) for a in range(dim)])
     top = np.array(shape, dtype=np.float64)[:, None] - 1
+    if radial_ny is not None:
+        top[1] = radial_ny - 1
     if np.any(u < -1e-9) or np.any(u > top + 1e-9):
         raise ValueError("Cartesian points must lie within the k-space grid defined by kgrid.")
     corner = np.clip(np.floor(u), 0, np.maximum(top - 1, 0)).astype(np.int64)
@@ -665,11 +674,12 @@ def prepare(kgrid: kWaveGrid, medium, source, sensor, dim: int, opts: dict) -> P
                                  "compatible with a binary sensor mask.")
             # the nearest-grid mask and its checks apply to 'linear' too
             # (kspaceFirstOrder_inputChecking.m:612-618)
-            gmask, order_index, reorder_index = cart2grid(kgrid, mask)
+            radial_ny = opts.get("_radial_ny")   # an axisymmetric problem's radial grid
+            gmask, order_index, reorder_index = cart2grid(kgrid, mask, radial_ny)
             if cart_interp == "linear" and not info.time_rev:
                 # one output row per Cartesian point, in the given order
                 # (kspaceFirstOrder_createStorageVariables.m:36-46,63-67)
-                info.interp = grid_data_linear(kgrid, mask)
+                info.interp = grid_data_linear(kgrid, mask, radial_ny)
                 n_sens = mask.shape[1]
             else:
                 sens_idx = _find(gmask)
@@ -1504,7 +1514,10 @@ def _as_pml_outside(kgrid, medium, source, sensor, sym, opts):
     sen = None
     if sensor is not None:
         sen = dict(sensor) if isinstance(sensor, dict) else dict(vars(sensor))
-        sen["mask"] = np.pad(np.asarray(sen["mask"]), pads)
+        if np.shape(sen["mask"]) == (Nx, Ny):
+            sen["mask"] = np.pad(np.asarray(sen["mask"]), pads)
+        # Cartesian points keep their coordinates: the grown grid keeps x
+        # centred and y the radius from the axis
     kg = kWaveGrid(Nx + 2 * px, kgrid.dx, Ny + py, kgrid.dy)
     kg.setTime(kgrid.Nt, kgrid.dt)
     o = dict(opts, PMLInside=True, Smooth=False, PMLSize=[px, py], RadialSymmetry=sym)
@@ -1528,7 +1541,8 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
     source (every p_mode, laid out on the expanded grid with the signs of
     the p-like layout), velocity sources source.ux (axial) and source.uy
     (radial; not with 'WSWS'), lossless or Stokes media (with or without B/A),
-    binary sensor masks and time reversal; 'SaveToDisk' writes the radial
+    binary sensor masks or Cartesian points (x, r), time reversal;
+    'SaveToDisk' writes the radial
     grid's input file (axisymmetric_flag 1, 'WSWA'), which kw_run_file and
     kspaceFirstOrder-HIP run; 'PMLInside' false grows the grid by the PML
     outside the axial ends and the outer radius."""
@@ -1592,7 +1606,21 @@ def _prepare_as(kgrid, medium, source, sensor, opts):
                 raise ValueError(f"'{k}' is not a valid field for sensor")
         m = np.asarray(sen.get("mask"))
         if m.shape != (kgrid.Nx, kgrid.Ny):
-            raise NotImplementedError("axisymmetric simulations in kwhip take a binary sensor mask")
+            if not (m.ndim == 2 and m.shape[0] == 2):
+                raise ValueError("sensor.mask must be a binary grid or a 2 x N Cartesian array")
+            # Cartesian points (x, r): the nearest radial grid points
+            # (cart2grid.m with axisymmetric true); time reversal takes them,
+            # its data sorted into the mask's order
+            # (kspaceFirstOrder_inputChecking.m:612-633), and records either
+            # those points or linear interpolation on the radial grid
+            rk = kWaveGrid(kgrid.Nx, kgrid.dx, kgrid.Ny, kgrid.dy)
+            gm, order_index, _ = cart2grid(rk, m, kgrid.Ny)
+            if time_rev:
+                d = np.asarray(sen["time_reversal_boundary_data"], dtype=np.float64)
+                sen = dict(sen, mask=gm,
+                           time_reversal_boundary_data=d[np.argsort(order_index, kind="stable")])
+            else:
+                opts["_radial_ny"] = kgrid.Ny
     Nx, Ny = kgrid.Nx, kgrid.Ny
     if ws and Ny < 3:
         raise ValueError("'WSWS' needs at least 3 radial grid points")
@@ -1658,7 +1686,9 @@ def _prepare_as(kgrid, medium, source, sensor, opts):
             mask[epos] = True
             esen["time_reversal_boundary_data"] = sign[:, None] * data[row]
         mask = mask.reshape((Nx, M), order="F")
-        if not time_rev:
+        if "_radial_ny" in opts:
+            mask = np.asarray(sen["mask"], dtype=np.float64)   # Cartesian (x, r)
+        elif not time_rev:
             mask[:, :Ny] = np.asarray(sen["mask"]).astype(bool)
         esen["mask"] = mask
     pml = opts.get("PMLSize", 20)

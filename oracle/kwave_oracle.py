@@ -284,9 +284,10 @@ def matlab_round(x):
     return np.sign(x) * np.floor(np.abs(x) + 0.5)
 
 
-def cart2grid(kgrid: KGrid, cart_data):
+def cart2grid(kgrid: KGrid, cart_data, axisymmetric=False):
     """L/cart2grid.m:59-194 (nearest-neighbour mapping; later points
-    overwrite earlier ones at the same grid point).
+    overwrite earlier ones at the same grid point; axisymmetric: y is the
+    radius from the first grid row, :111-116).
 
     Returns (binary grid mask, order_index, reorder_index) with 1-based
     order/reorder values as in MATLAB."""
@@ -296,7 +297,8 @@ def cart2grid(kgrid: KGrid, cart_data):
     for ax in range(dim):
         d = kgrid.spacings[ax]
         N = kgrid.shape[ax]
-        v = matlab_round(cart[ax] / d) + math.floor(N / 2) + 1
+        shift = 1 if (axisymmetric and ax == 1) else math.floor(N / 2) + 1
+        v = matlab_round(cart[ax] / d) + shift
         idx.append(v.astype(np.int64))
     for ax in range(dim):
         if idx[ax].max() > kgrid.shape[ax] or idx[ax].min() < 1:
@@ -311,7 +313,7 @@ def cart2grid(kgrid: KGrid, cart_data):
     return mask, order_index, reorder_index
 
 
-def grid_data_linear(kgrid: KGrid, cart_data):
+def grid_data_linear(kgrid: KGrid, cart_data, axisymmetric=False):
     """Linear interpolation of grid values at Cartesian points
     (LP/kspaceFirstOrder_createStorageVariables.m:376-398 with
     LP/gridDataFast2D.m / gridDataFast3D.m: the enclosing simplex of a
@@ -327,7 +329,9 @@ def grid_data_linear(kgrid: KGrid, cart_data):
     1 - f[s1], f[s1] - f[s2], ..., f[s_dim].  In 2-D this cuts every cell
     along its diagonal from (i, j) to (i+1, j+1).
 
-    Grid point n (0-based) of axis a lies at (n - floor(N_a/2)) d_a.
+    Grid point n (0-based) of axis a lies at (n - floor(N_a/2)) d_a (the
+    axisymmetric grid, LP/kspaceFirstOrder_createStorageVariables.m:390-391:
+    y at n d_y, the radius).
     Returns (tri, bc): [npts, dim+1] 0-based column-major linear indices and
     float64 weights."""
     cart = np.asarray(cart_data, dtype=np.float64)
@@ -337,7 +341,7 @@ def grid_data_linear(kgrid: KGrid, cart_data):
     frac = np.zeros((dim, npts))
     for a in range(dim):
         N = kgrid.shape[a]
-        u = cart[a] / kgrid.spacings[a] + math.floor(N / 2)
+        u = cart[a] / kgrid.spacings[a] + (0 if (axisymmetric and a == 1) else math.floor(N / 2))
         if np.any(u < -1e-9) or np.any(u > N - 1 + 1e-9):
             raise ValueError("Cartesian points must lie within the k-space grid defined by kgrid.")
         i0 = np.clip(np.floor(u), 0, max(N - 2, 0)).astype(np.int64)
@@ -1481,7 +1485,7 @@ def _as_pml_outside(kgrid, medium, source, sensor, *, pml_size, smooth, radial_s
 
 def kspace_first_order_as(kgrid: KGrid, medium, source, sensor, *, data_cast="off",
                           pml_size=20, pml_alpha=2.0, smooth=True, radial_symmetry="WSWA-FFT",
-                          pml_inside=True):
+                          pml_inside=True, cart_interp="linear"):
     """kspaceFirstOrderAS.m:355-1435 with 'RadialSymmetry' 'WSWA-FFT' (or
     'WSWS-FFT', the expanded grid of 2 Ny - 2 points) for the
     cases the library runs: an initial pressure or a pressure source (every
@@ -1620,8 +1624,26 @@ def kspace_first_order_as(kgrid: KGrid, medium, source, sensor, *, data_cast="of
     inv_y = (1.0 / y_sg).astype(rdt)
     dtc = rdt(dt)
 
-    mask = np.asarray(sensor["mask"], dtype=bool)
+    mask = np.asarray(sensor["mask"])
+    interp = reorder = None
+    if mask.shape != (Nx, Ny):
+        # Cartesian points (x, r) (kspaceFirstOrder_inputChecking.m:590-633,
+        # cart2grid.m with axisymmetric true); time reversal takes the
+        # nearest grid points with its data sorted into their order
+        gm, order_index, reorder_index = cart2grid(kgrid, mask, axisymmetric=True)
+        if tr is not None:
+            tr = np.asarray(tr, dtype=np.float64)[np.argsort(order_index, kind="stable")]
+        elif cart_interp == "linear":
+            interp = grid_data_linear(kgrid, mask, axisymmetric=True)
+        else:
+            reorder = reorder_index
+        mask = gm
+    mask = np.asarray(mask, dtype=bool)
     idx = np.flatnonzero(mask.reshape(-1, order="F"))
+    if interp is not None:
+        # one row per Cartesian point (extractSensorData.m: sum(f(tri) .* bc, 2))
+        interp = _Interp(interp[0], interp[1], rdt)
+        idx = np.arange(len(interp))
     record = list(sensor.get("record", ["p"]))
     start = int(sensor.get("record_start_index", 1))
     out = {}
@@ -1665,6 +1687,8 @@ def kspace_first_order_as(kgrid: KGrid, medium, source, sensor, *, data_cast="of
         return ifft2(ddx_pos * pk)[:, :Ny], ifft2(ddy_k * y_pos * pk)[:, :Ny]
 
     def pick(f):
+        if interp is not None:
+            return interp.get(f).astype(np.float64)
         return np.reshape(f, -1, order="F")[idx].astype(np.float64)
 
     for t in range(1, Nt_loop + 1):
@@ -1761,4 +1785,10 @@ def kspace_first_order_as(kgrid: KGrid, medium, source, sensor, *, data_cast="of
         res["ux"], res["uy"] = out["ux"], out["uy"]
     if "u_final" in record:
         res["ux_final"], res["uy_final"] = ux.astype(np.float64), uy.astype(np.float64)
+    if reorder is not None:
+        # LP/kspaceFirstOrder_reorderCartData.m: rows back in the order of
+        # the Cartesian points
+        for k in ("p", "p_max", "p_min", "p_rms", "ux", "uy"):
+            if k in res:
+                res[k] = reorder_binary_sensor_data(res[k], reorder)
     return res

@@ -83,6 +83,24 @@ def case(name, Nx=32, Ny=16, seed=2, Nt=24, sym="WSWA-FFT"):
     elif name == "pml_outside_source":
         cs["pml_inside"] = False
         cs["source"] = {"p_mask": src_mask, "p": burst[None]}
+    elif name in ("cartesian_linear", "cartesian_nearest", "cartesian_tr"):
+        # Cartesian points (x, r): distinct grid nodes plus offsets below half
+        # a cell, so the nearest nodes do not repeat
+        n_pts = 40
+        cells = rng.choice((Nx - 12) * (Ny - 6), n_pts, replace=False)
+        ix, iy = cells % (Nx - 12) + 6, cells // (Nx - 12)
+        xs = (ix - Nx // 2 + rng.uniform(-0.45, 0.45, n_pts)) * dx
+        rs = (iy + rng.uniform(0, 0.45, n_pts)) * dx
+        cart = np.stack([xs, rs])
+        cs["cart_interp"] = "nearest" if name == "cartesian_nearest" else "linear"
+        if name == "cartesian_tr":
+            cs["source"] = {}
+            cs["sensor"] = {"mask": cart, "time_reversal_boundary_data":
+                            rng.standard_normal((n_pts, 1)) * burst}
+        else:
+            # point records only: the final field is normalised by the largest
+            # recorded p_max, which a few points need not see
+            cs["sensor"] = {"mask": cart, "record": ["p", "p_max", "p_rms", "u"]}
     elif name == "time_reversal":
         sm = cs["sensor"]["mask"]
         cs["source"] = {}
@@ -96,11 +114,12 @@ def case(name, Nx=32, Ny=16, seed=2, Nt=24, sym="WSWA-FFT"):
 NAMES = ["homogeneous", "hetero", "nonlinear", "stokes", "stokes_nonlinear", "smoothed",
          "source_additive", "source_dirichlet", "source_labelled", "source_frequency_ref",
          "source_stokes_nonlinear", "time_reversal", "velocity_additive", "velocity_dirichlet",
-         "velocity_labelled", "velocity_axial_only", "pml_outside", "pml_outside_source"]
+         "velocity_labelled", "velocity_axial_only", "pml_outside", "pml_outside_source",
+         "cartesian_linear", "cartesian_nearest", "cartesian_tr"]
 # the cases run with 'WSWS-FFT' as well (radial velocity sources are not
 # taken with it)
 WS_NAMES = ["hetero", "stokes_nonlinear", "smoothed", "source_additive", "source_dirichlet",
-            "time_reversal", "velocity_axial_only", "pml_outside"]
+            "time_reversal", "velocity_axial_only", "pml_outside", "cartesian_linear"]
 
 
 def run_kwhip(cs):
@@ -110,7 +129,8 @@ def run_kwhip(cs):
     out = kwhip.kspaceFirstOrderAS(kg, dict(cs["medium"]), dict(cs["source"]), dict(cs["sensor"]),
                                    PMLSize=cs["pml"], Smooth=cs["smooth"],
                                    RadialSymmetry=cs.get("sym", "WSWA-FFT"),
-                                   PMLInside=cs.get("pml_inside", True))
+                                   PMLInside=cs.get("pml_inside", True),
+                                   CartInterp=cs.get("cart_interp", "linear"))
     return {"p_final": out} if "time_reversal_boundary_data" in cs["sensor"] else out
 
 
@@ -121,7 +141,8 @@ def run_oracle(cs, data_cast):
     sym = "WSWS-FFT" if cs.get("sym", "WSWA-FFT").startswith("WSWS") else "WSWA-FFT"
     return ko.kspace_first_order_as(kg, cs["medium"], cs["source"], cs["sensor"],
                                     data_cast=data_cast, pml_size=cs["pml"], smooth=cs["smooth"],
-                                    radial_symmetry=sym, pml_inside=cs.get("pml_inside", True))
+                                    radial_symmetry=sym, pml_inside=cs.get("pml_inside", True),
+                                    cart_interp=cs.get("cart_interp", "linear"))
 
 
 def compare(got, ref):

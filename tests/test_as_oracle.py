@@ -24,7 +24,10 @@
 * time reversal equals a 'dirichlet' run of the flipped data one column
   ahead; without the flip it is far off;
 * with 'PMLInside' false the spherical wave holds on the whole user grid
-  (5.1e-4), where the PML inside is off by 0.74.
+  (5.1e-4), where the PML inside is off by 0.74;
+* Cartesian sensor points (x, r) on grid nodes record, by linear
+  interpolation and by the nearest mapping, what the binary mask of those
+  nodes records, in the points' order.
 """
 
 import numpy as np
@@ -284,3 +287,35 @@ def test_pml_outside_follows_the_wave_over_the_whole_grid():
         assert pf.shape == (Nx, Ny)
         errs[inside] = float(np.max(np.abs(pf - exact)) / np.max(np.abs(exact)))
     assert errs[False] < 6e-4 and errs[True] > 0.3, errs
+
+
+def test_cartesian_sensor_points():
+    # Cartesian points (x, r) on grid nodes: the linear interpolation and
+    # the nearest mapping record what the binary mask of those nodes
+    # records, in the points' order (the radius counts from the first row;
+    # an offset of Ny/2 rows, the non-axisymmetric convention, fails this)
+    Nx, Ny, dx, Nt = 32, 16, 1e-4, 16
+    med = _hetero(Nx, Ny, 8)
+    dt = 0.25 * dx / med["sound_speed"].max()
+    x = (np.arange(Nx) - Nx // 2) * dx
+    X, R = np.meshgrid(x, np.arange(Ny) * dx, indexing="ij")
+    p0 = np.exp(-(X ** 2 + R ** 2) / (3 * dx) ** 2)
+    rng = np.random.default_rng(9)
+    cells = rng.choice(Nx * Ny, 12, replace=False)
+    ix, iy = cells % Nx, cells // Nx
+    cart = np.stack([(ix - Nx // 2) * dx, iy * dx])
+    binary = np.zeros((Nx, Ny), bool)
+    binary[ix, iy] = True
+    kg = ko.KGrid(Nx, dx, Ny, dx)
+    kg.set_time(Nt, dt)
+    run = lambda mask, ci: ko.kspace_first_order_as(  # noqa: E731
+        kg, med, {"p0": p0}, {"mask": mask, "record": ["p", "u"]}, pml_size=[6, 4], smooth=False,
+        cart_interp=ci)
+    ref = run(binary, "linear")
+    order = np.argsort(np.ravel_multi_index((ix, iy), (Nx, Ny), order="F"))
+    rows = np.empty_like(order)
+    rows[order] = np.arange(order.size)   # row of each point in the binary record
+    for ci in ("linear", "nearest"):
+        got = run(cart if ci == "linear" else cart + np.array([[0.3 * dx], [0.2 * dx]]), ci)
+        for k in ("p", "ux", "uy"):
+            assert np.allclose(got[k], ref[k][rows], rtol=0, atol=1e-12 * np.abs(ref[k]).max()), (ci, k)
