@@ -1604,6 +1604,13 @@ def _prepare_as(kgrid, medium, source, sensor, opts):
             if k not in ("mask", "frequency_response", "record_mode", "record",
                          "record_start_index", "time_reversal_boundary_data"):
                 raise ValueError(f"'{k}' is not a valid field for sensor")
+        nonstag = {"u_non_staggered", "I", "I_avg"} & set(sen.get("record", []))
+        if nonstag:
+            # kspaceFirstOrder_extractSensorData.m shifts u_r to the
+            # non-staggered grid with a periodic FFT over the radial grid
+            # alone, not over the symmetric extension these runs step
+            raise NotImplementedError("axisymmetric records of the velocity on the non-staggered grid "
+                                      f"are not supported by kwhip: {sorted(nonstag)}")
         m = np.asarray(sen.get("mask"))
         if m.shape != (kgrid.Nx, kgrid.Ny):
             if not (m.ndim == 2 and m.shape[0] == 2):

@@ -71,6 +71,9 @@ def test_as_option_checks():
     with pytest.raises(NotImplementedError, match="radial velocity sources"):
         kwhip.kspaceFirstOrderAS(kg, cs["medium"], {"u_mask": p0 > 0.5, "uy": np.ones((1, 4))},
                                  cs["sensor"], RadialSymmetry="WSWS")
+    with pytest.raises(NotImplementedError, match="non-staggered"):
+        kwhip.kspaceFirstOrderAS(kg, cs["medium"], dict(cs["source"]),
+                                 dict(cs["sensor"], record=["p", "I_avg"]))
     with pytest.raises(ValueError, match="source.uz is not defined"):
         kwhip.kspaceFirstOrderAS(kg, cs["medium"], {"u_mask": p0 > 0.5, "uz": np.ones((1, 4))},
                                  cs["sensor"])
