@@ -172,3 +172,26 @@ def test_as_pml_auto():
     with pytest.raises(ValueError, match="only supported with 'PMLInside' set to false"):
         kwhip.kspaceFirstOrderAS(kg, cs["medium"], dict(cs["source"]), cs["sensor"],
                                  PMLSize="auto")
+
+
+def test_as_record_start_and_frequency_response():
+    """sensor.record_start_index against the oracle, and
+    sensor.frequency_response (kspaceFirstOrderAS.m:1401-1403: the recorded
+    p filtered by gaussianFilter.m) against the oracle's record filtered by
+    the oracle's own gaussian_filter."""
+    from oracle import kwave_oracle as ko
+    cs = ac.case("hetero")
+    cs["sensor"] = dict(cs["sensor"], record=["p", "p_max", "u"], record_start_index=9)
+    got = ac.run_kwhip(cs)
+    ref = ac.run_oracle(cs, "single")
+    e = ac.compare(got, ref)
+    assert np.asarray(got["p"]).shape[1] == cs["Nt"] - 8
+    for k in e:
+        assert e[k] <= TOL32, (k, e[k])
+    fr = [2e6, 80.0]
+    cs2 = ac.case("hetero")
+    cs2["sensor"] = dict(cs2["sensor"], record=["p"], frequency_response=fr)
+    got2 = np.asarray(ac.run_kwhip(cs2)["p"], dtype=np.float64)
+    ref2 = ac.run_oracle(dict(cs2, sensor=dict(cs2["sensor"], record=["p"])), "single")["p"]
+    want = ko.gaussian_filter(ref2, 1.0 / cs2["dt"], fr[0], fr[1])
+    assert np.max(np.abs(got2 - want)) <= TOL32 * np.max(np.abs(want))
