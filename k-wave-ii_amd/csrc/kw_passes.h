@@ -488,14 +488,20 @@ KW_BODY void body_spec(Ctx& c, const Batch& B, const PassArgs& A) {
       LS::template fft_tw<1, true>(c, pl, tw, g, G, active, src, st, lw);
     }
   } else {
-    // SM_ABS: S0 <- nabla1 .* S0, S1 <- nabla2 .* S1;  SM_FILTER: one table
+    // SM_ABS: S0 <- nabla1 .* S0, S1 <- nabla2 .* S1;  SM_FILTER: one table,
+    // times the last axis's velocity shift ushift (kspaceFirstOrderAS.m:
+    // 954-969, the u_r / r term) when flags has SF_SHIFT
     const int nb = A.mode == SM_ABS ? 2 : 1;
+    const bool fshift = A.mode == SM_FILTER && (A.flags & SF_SHIFT);
     for (int j = 0; j < nb; ++j) {
       cf* S = P.S[A.mode == SM_ABS ? j : A.buf[0]];
       const float* tab = A.mode == SM_ABS ? (j == 0 ? P.nabla1 : P.nabla2) : A.table;
       auto ld = [&](int, int n) -> cf { return KW_LD32(S, addr(n)); };
       LS::template fft_tw<1, false>(c, pl, tw, g, G, active, ld, to_a, la);
-      auto src = [&](int, int n) -> cf { return cscale(la.get(0, n), KW_LD32(tab, taddr(n))); };
+      auto src = [&](int, int n) -> cf {
+        const cf v = cscale(la.get(0, n), KW_LD32(tab, taddr(n)));
+        return fshift ? cmul(KW_LD32(P.ushift[last], (uint32_t)n), v) : v;
+      };
       auto st = [&](int, int n, cf v, int) { KW_ST32(S, addr(n), v); };
       LS::template fft_tw<1, true>(c, pl, tw, g, G, active, src, st, lw);
     }
@@ -513,12 +519,17 @@ KW_BODY void body_x_r2c(Ctx& cb, const Batch& B, const PassArgs& A) {
   const ProbDev& P = batch_prob(cb, B, tile);
   const XTile T = x_tile<XS>(cb, P, tile);
   PencilCtx<Ctx> c{cb, T.wave};
+  // A.table: a factor per y row (the axisymmetric inverse radius), applied
+  // as the rows are read
+  const float* rs = A.table;
+  const float s0 = rs ? KW_LD(rs + (T.c0 / (uint32_t)P.Nx) % (uint32_t)P.Ny) : 1.f;
+  const float s1 = rs ? KW_LD(rs + (T.c1 / (uint32_t)P.Nx) % (uint32_t)P.Ny) : 1.f;
   for (int b = 0; b < A.nbuf; ++b) {
     const float* f = A.real[b] ? A.real[b] : P.p;
     auto src = [&](int, int n) -> cf {
       const float a = KW_LD32(f, T.c0 + n);
       const float bb = KW_LD32(f, T.c1 + n);
-      return cf{a, T.has1 ? bb : 0.f};
+      return cf{a * s0, T.has1 ? bb * s1 : 0.f};
     };
     auto dst = [&](int j, int n, cf v, int) { T.lds.set(j, n, v); };
     XS::template fft_tw<1, false>(c, P.fx, T.tw, T.g, T.G, T.active, src, dst, T.lds);

@@ -242,6 +242,7 @@ struct PassArgs {
 };
 
 enum : int32_t {
+  SF_SHIFT = 1,    // SM_FILTER: also multiply by ushift of the last axis
   XG_INIT = 1,     // p0 initial condition: u = dt/2 * rinv * grad p, no fft(u)
   XC_ACCUM = 1,    // c2r: add into the real grid instead of overwriting
   X_STORE_P = 2,   // producer of p must store p to HBM this step

@@ -492,10 +492,10 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream, Arena* ar
   });
   if (S->axisym) {
     // the radial divergence adds shift_y^- κ fft(u_y / r) (kspaceFirstOrderAS.m:
-    // 954-969): κ / Ny in a filter pass followed by the y shift (which
-    // carries the 1/(Nx Ny) of the round trip)
+    // 954-969): κ times the y shift (which carries the 1/(Nx Ny) of the
+    // round trip) in one last-axis pass
     S->as_kappa = S->spectral_table([&](int ix, int iy, int iz) {
-      return (use_k ? sinc(cref * kmag(ix, iy, iz) * dt / 2) : 1.0) / (double)G.Ny;
+      return use_k ? sinc(cref * kmag(ix, iy, iz) * dt / 2) : 1.0;
     });
     // ir: + on the copies that hold u_r, - on the mirrored ones, so that
     // ir .* u_r has the even layout of u_r / r (:966-977)
@@ -514,7 +514,6 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream, Arena* ar
       ir[j] = (float)((first ? 1.0 : -1.0) / (r * d[1] + d[1] / 2));
     }
     S->as_ir = S->upload(ir);
-    S->scratch_grid = S->alloc<float>((size_t)G.ncell);  // ir .* u_r of every step
     if (S->as_ws) {
       // kspaceFirstOrderAS.m:440-456 leaves rho0_sgy of the last radial
       // column at rho0 there (the interpolation past the grid is NaN); the
@@ -983,33 +982,24 @@ static void transition_gradient(Sim& S) {
 // 954-969 with 'WSWA-FFT'): the radial divergence is
 // shift_y^- κ (i ky fft2(u_y) + fft2(ir .* u_y)) on the expanded grid, ir the
 // signed inverse radius of each copy.  The second part goes through the
-// spare spectral buffer 2: ir .* u_y, its x transform, a filter pass with
-// κ / Ny and the y shift pass (which carries the 1/(Nx Ny) of the round
-// trip), which leaves it scaled as the divergence pass leaves its output;
-// it is added to the y component after that pass (divergence_radial), so
-// the density update, the nonlinear coefficient and the equation of state
-// see the whole divergence.
+// spare spectral buffer 2: the x transform of u_y with each row scaled by
+// ir as it is read, then one last-axis pass with κ and the y shift (whose
+// table carries the 1/(Nx Ny) of the round trip), which leaves it scaled as
+// the divergence pass leaves its output; it is added to the y component
+// after that pass (divergence_radial), so the density update, the
+// nonlinear coefficient and the equation of state see the whole
+// divergence.
 static void radial_term(Sim& S) {
   ProbDev& P = S.P;
-  if (!S.scratch_grid) S.scratch_grid = S.alloc<float>((size_t)S.G.ncell);
-  EwArgs m{};
-  m.kind = EW_AXIS_MUL;
-  m.which = 1;
-  m.p0 = P.u[1];
-  m.ptr = S.scratch_grid;
-  m.prof = S.as_ir;
-  S.ew(m, S.lcell.nb);
   PassArgs f = bufs(1, 2);
-  f.real[0] = S.scratch_grid;
+  f.real[0] = P.u[1];
+  f.table = S.as_ir;
   S.pass(PK_X_R2C, f);
-  PassArgs fl = f;
+  PassArgs fl = bufs(1, 2);
   fl.mode = SM_FILTER;
   fl.table = S.as_kappa;
+  fl.flags = SF_SHIFT;
   S.pass(PK_SPEC, fl);
-  PassArgs sh = f;
-  sh.mode = SM_SHIFT;
-  sh.flags = 1;
-  S.pass(PK_SPEC, sh);
 }
 
 // Nonlinear axisymmetric steps: the nonlinear mass-conservation coefficient
