@@ -321,16 +321,20 @@ KW_BODY void spec_aux(Ctx& c, const ProbDev& P, const PassArgs& A, const FftPlan
   if (A.mode == SM_SHIFT) {
     // S <- ushift_d .* S for the velocity on the non-staggered grid
     // (kspaceFirstOrder_extractSensorData.m:47-56): a per-pencil constant for
-    // x and y (3-D), varying along the pencil for the last axis
+    // x and y (3-D), varying along the pencil for the last axis; with a
+    // table, S <- table .* ushift_d .* S (the axisymmetric u_r / r term,
+    // kspaceFirstOrderAS.m:954-969: kappa and the radial shift in one pass)
     const int d = A.flags;
     cf* S = P.S[A.buf[0]];
     const cf* sh = P.ushift[d];
+    const float* tab = A.table;
     auto ld = [&](int, int n) -> cf { return KW_LD32(S, addr(n)); };
     LS::template fft_tw<1, false>(c, pl, tw, g, G, active, ld, to_a, la);
     const bool along = d == last;
     const cf sc = (along || !active) ? cf{0.f, 0.f} : KW_LD(sh + (d == 0 ? kx : ky));
     auto src = [&](int, int n) -> cf {
-      return cmul(along ? KW_LD32(sh, (uint32_t)n) : sc, la.get(0, n));
+      const cf v = cmul(along ? KW_LD32(sh, (uint32_t)n) : sc, la.get(0, n));
+      return tab ? cscale(v, KW_LD32(tab, taddr(n))) : v;
     };
     auto st = [&](int, int n, cf v, int) { KW_ST32(S, addr(n), v); };
     LS::template fft_tw<1, true>(c, pl, tw, g, G, active, src, st, lw);
@@ -488,20 +492,14 @@ KW_BODY void body_spec(Ctx& c, const Batch& B, const PassArgs& A) {
       LS::template fft_tw<1, true>(c, pl, tw, g, G, active, src, st, lw);
     }
   } else {
-    // SM_ABS: S0 <- nabla1 .* S0, S1 <- nabla2 .* S1;  SM_FILTER: one table,
-    // times the last axis's velocity shift ushift (kspaceFirstOrderAS.m:
-    // 954-969, the u_r / r term) when flags has SF_SHIFT
+    // SM_ABS: S0 <- nabla1 .* S0, S1 <- nabla2 .* S1;  SM_FILTER: one table
     const int nb = A.mode == SM_ABS ? 2 : 1;
-    const bool fshift = A.mode == SM_FILTER && (A.flags & SF_SHIFT);
     for (int j = 0; j < nb; ++j) {
       cf* S = P.S[A.mode == SM_ABS ? j : A.buf[0]];
       const float* tab = A.mode == SM_ABS ? (j == 0 ? P.nabla1 : P.nabla2) : A.table;
       auto ld = [&](int, int n) -> cf { return KW_LD32(S, addr(n)); };
       LS::template fft_tw<1, false>(c, pl, tw, g, G, active, ld, to_a, la);
-      auto src = [&](int, int n) -> cf {
-        const cf v = cscale(la.get(0, n), KW_LD32(tab, taddr(n)));
-        return fshift ? cmul(KW_LD32(P.ushift[last], (uint32_t)n), v) : v;
-      };
+      auto src = [&](int, int n) -> cf { return cscale(la.get(0, n), KW_LD32(tab, taddr(n))); };
       auto st = [&](int, int n, cf v, int) { KW_ST32(S, addr(n), v); };
       LS::template fft_tw<1, true>(c, pl, tw, g, G, active, src, st, lw);
     }

@@ -230,7 +230,8 @@ struct PassArgs {
   int32_t nbuf;     // number of spectral buffers touched (y passes, r2c/c2r)
   int32_t buf[3];   // spectral buffer indices
   float* real[3];   // real grids for r2c/c2r (per problem when nprob == 1)
-  const float* table;  // filter table for SM_FILTER (nprob == 1)
+  const float* table;  // filter table for SM_FILTER (and SM_SHIFT), a row
+                       // factor for x_r2c (nprob == 1)
   int32_t flags;
   // set on the first pass of every time step (the gradient's y pass in 3D,
   // its last-axis pass in 2D), which reads no step number: one thread per
@@ -242,7 +243,6 @@ struct PassArgs {
 };
 
 enum : int32_t {
-  SF_SHIFT = 1,    // SM_FILTER: also multiply by ushift of the last axis
   XG_INIT = 1,     // p0 initial condition: u = dt/2 * rinv * grad p, no fft(u)
   XC_ACCUM = 1,    // c2r: add into the real grid instead of overwriting
   X_STORE_P = 2,   // producer of p must store p to HBM this step

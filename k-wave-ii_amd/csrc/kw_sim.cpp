@@ -995,11 +995,11 @@ static void radial_term(Sim& S) {
   f.real[0] = P.u[1];
   f.table = S.as_ir;
   S.pass(PK_X_R2C, f);
-  PassArgs fl = bufs(1, 2);
-  fl.mode = SM_FILTER;
-  fl.table = S.as_kappa;
-  fl.flags = SF_SHIFT;
-  S.pass(PK_SPEC, fl);
+  PassArgs sh = bufs(1, 2);
+  sh.mode = SM_SHIFT;
+  sh.flags = 1;
+  sh.table = S.as_kappa;
+  S.pass(PK_SPEC, sh);
 }
 
 // Nonlinear axisymmetric steps: the nonlinear mass-conservation coefficient
