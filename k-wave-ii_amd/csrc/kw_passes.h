@@ -1063,19 +1063,24 @@ KW_BODY void body_ew(Ctx& c, const Batch& B, const EwArgs& A) {
       // field is [f, 0, -flip(f(2:end)), -f, 0, flip(f(2:end))]
       // (kspaceFirstOrderAS.m:703-707); nonlinear terms do not keep that
       // symmetry, so copies 2-4 are rewritten from copy 1 after each update
+      // (A.ptr, or with which = 3 rho_x, rho_y and p in one launch)
       if (i >= P.ncell) return;
       const int No = P.Ny / 4;
       const int64_t x = i % P.Nx, y = i / P.Nx;
       const int b = (int)(y / No), r = (int)(y % No);
       if (b == 0) return;
-      float v = 0.f;
-      if (b == 2) {
-        v = -KW_LD(A.ptr + x + (int64_t)P.Nx * r);
-      } else if (r != 0) {
-        const float f = KW_LD(A.ptr + x + (int64_t)P.Nx * (No - r));
-        v = b == 1 ? -f : f;
+      const int nf = A.which == 3 ? 3 : 1;
+      for (int q = 0; q < nf; ++q) {
+        float* f = A.which == 3 ? (q == 0 ? P.rho[0] : q == 1 ? P.rho[1] : P.p) : A.ptr;
+        float v = 0.f;
+        if (b == 2) {
+          v = -KW_LD(f + x + (int64_t)P.Nx * r);
+        } else if (r != 0) {
+          const float m = KW_LD(f + x + (int64_t)P.Nx * (No - r));
+          v = b == 1 ? -m : m;
+        }
+        KW_ST(f + i, v);
       }
-      KW_ST(A.ptr + i, v);
       return;
     }
     case EW_AS_WS_COL: {

@@ -1009,13 +1009,10 @@ static void radial_term(Sim& S) {
 // the spectrum of p that the next step's gradient starts from is formed
 // again.  The velocities stay symmetric: their update is linear in p.
 static void mirror_copies(Sim& S) {
-  float* f[3] = {S.P.rho[0], S.P.rho[1], S.P.p};
-  for (float* q : f) {
-    EwArgs e{};
-    e.kind = EW_AS_MIRROR;
-    e.ptr = q;
-    S.ew(e, S.lcell.nb);
-  }
+  EwArgs e{};
+  e.kind = EW_AS_MIRROR;
+  e.which = 3;  // rho_x, rho_y and p in one launch
+  S.ew(e, S.lcell.nb);
   r2c_p(S);
 }
 
