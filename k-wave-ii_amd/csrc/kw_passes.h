@@ -322,10 +322,12 @@ KW_BODY void spec_aux(Ctx& c, const ProbDev& P, const PassArgs& A, const FftPlan
     // S <- ushift_d .* S for the velocity on the non-staggered grid
     // (kspaceFirstOrder_extractSensorData.m:47-56): a per-pencil constant for
     // x and y (3-D), varying along the pencil for the last axis; with a
-    // table, S <- table .* ushift_d .* S (the axisymmetric u_r / r term,
-    // kspaceFirstOrderAS.m:954-969: kappa and the radial shift in one pass)
+    // table, table .* ushift_d .* S (the axisymmetric u_r / r term,
+    // kspaceFirstOrderAS.m:954-969: kappa and the radial shift in one pass),
+    // added to S[buf[1]] when nbuf is 2
     const int d = A.flags;
     cf* S = P.S[A.buf[0]];
+    cf* D = A.nbuf == 2 ? P.S[A.buf[1]] : S;
     const cf* sh = P.ushift[d];
     const float* tab = A.table;
     auto ld = [&](int, int n) -> cf { return KW_LD32(S, addr(n)); };
@@ -336,7 +338,9 @@ KW_BODY void spec_aux(Ctx& c, const ProbDev& P, const PassArgs& A, const FftPlan
       const cf v = cmul(along ? KW_LD32(sh, (uint32_t)n) : sc, la.get(0, n));
       return tab ? cscale(v, KW_LD32(tab, taddr(n))) : v;
     };
-    auto st = [&](int, int n, cf v, int) { KW_ST32(S, addr(n), v); };
+    auto st = [&](int, int n, cf v, int) {
+      KW_ST32(D, addr(n), D == S ? v : cadd(KW_LD32(D, addr(n)), v));
+    };
     LS::template fft_tw<1, true>(c, pl, tw, g, G, active, src, st, lw);
   } else if (A.mode == SM_SPLIT) {
     // compressional part of the velocity (kspaceFirstOrder_extractSensorData.m:

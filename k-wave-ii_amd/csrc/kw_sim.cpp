@@ -983,23 +983,17 @@ static void transition_gradient(Sim& S) {
 // shift_y^- κ (i ky fft2(u_y) + fft2(ir .* u_y)) on the expanded grid, ir the
 // signed inverse radius of each copy.  The second part goes through the
 // spare spectral buffer 2: the x transform of u_y with each row scaled by
-// ir as it is read, then one last-axis pass with κ and the y shift (whose
-// table carries the 1/(Nx Ny) of the round trip), which leaves it scaled as
-// the divergence pass leaves its output; it is added to the y component
-// after that pass (divergence_radial), so the density update, the
+// ir as it is read (radial_term), then, after the divergence pass, one
+// last-axis pass with κ and the y shift (whose table carries the
+// 1/(Nx Ny) of the round trip) that adds the result to the y component the
+// divergence pass left (divergence_radial), so the density update, the
 // nonlinear coefficient and the equation of state see the whole
 // divergence.
 static void radial_term(Sim& S) {
-  ProbDev& P = S.P;
   PassArgs f = bufs(1, 2);
-  f.real[0] = P.u[1];
+  f.real[0] = S.P.u[1];
   f.table = S.as_ir;
   S.pass(PK_X_R2C, f);
-  PassArgs sh = bufs(1, 2);
-  sh.mode = SM_SHIFT;
-  sh.flags = 1;
-  sh.table = S.as_kappa;
-  S.pass(PK_SPEC, sh);
 }
 
 // Nonlinear axisymmetric steps: the nonlinear mass-conservation coefficient
@@ -1028,12 +1022,11 @@ static void ws_column(Sim& S, int which) {
 }
 
 static void divergence_radial(Sim& S) {
-  EwArgs e{};
-  e.kind = EW_ADD;
-  e.n = 2 * S.G.nspec;
-  e.ptr = (float*)S.P.S[1];
-  e.p0 = (const float*)S.P.S[2];
-  S.ew(e, (int)((e.n + EW_T - 1) / EW_T));
+  PassArgs sh = bufs(2, 2, 1);
+  sh.mode = SM_SHIFT;
+  sh.flags = 1;
+  sh.table = S.as_kappa;
+  S.pass(PK_SPEC, sh);
 }
 
 static void record_step(Sim& S, int t, bool init) {
