@@ -1330,14 +1330,23 @@ def _as_points(pos, nx, ny, ws=False):
     return lin[order], row[order], sign[order]
 
 
-def _as_points_u(pos, nx, ny):
+def _as_points_u(pos, nx, ny, ws=False):
     """Points of the radial grid's y-staggered layout on the 'WSWA-FFT'
     expanded grid ([f, flip(f), -f, -flip(f)], kspaceFirstOrderAS.m:
     944-948): each point (i, j) at columns j (+), 2 Ny - 1 - j (+),
-    2 Ny + j (-) and 4 Ny - 1 - j (-).  Same returns as _as_points."""
+    2 Ny + j (-) and 4 Ny - 1 - j (-) ('WSWS-FFT', [f(1:end-1),
+    -flip(f(1:end-1))], :966-970: j (+) and 2 Ny - 3 - j (-) for j < Ny - 1).
+    Same returns as _as_points."""
     pos = np.asarray(pos, dtype=np.int64)
     ix, iy = pos % nx, pos // nx
     src = np.arange(pos.size)
+    if ws:
+        keep = iy < ny - 1
+        lin = np.concatenate([pos[keep], ix[keep] + nx * (2 * ny - 3 - iy[keep])])
+        row = np.concatenate([src[keep], src[keep]])
+        sign = np.concatenate([np.ones(keep.sum()), -np.ones(keep.sum())])
+        order = np.argsort(lin, kind="stable")
+        return lin[order], row[order], sign[order]
     cols = [iy, 2 * ny - 1 - iy, 2 * ny + iy, 4 * ny - 1 - iy]
     lin = np.concatenate([ix + nx * c for c in cols])
     row = np.tile(src, 4)
@@ -1401,15 +1410,22 @@ def _as_velocity_sources(src, med, kgrid, scale, ws=False):
     for ax, nm, dd in ((0, "ux", kgrid.dx), (1, "uy", kgrid.dy)):
         if nm not in src:
             continue
-        if nm == "uy" and ws:
-            raise NotImplementedError("radial velocity sources (source.uy) with 'WSWS' are not "
-                                      "supported by kwhip")
+        if nm == "uy" and ws and (mode == "additive" or np.any(pos // Nx >= Ny - 2)):
+            # the mirror point that stands in for the last radial column
+            # must not be driven: no k-space filtered source, no points in
+            # the last two radial columns
+            raise NotImplementedError("radial velocity sources (source.uy) with 'WSWS' are supported "
+                                      "by kwhip in the 'dirichlet' and 'additive-no-correction' modes "
+                                      "away from the last two radial columns")
         sig = w * np.atleast_2d(np.asarray(src[nm], dtype=np.float64))
         rows = _as_signal_rows(m, sig, pos)
         if scale and mode != "dirichlet":
             for i in range(sig.shape[0]):
                 sig[i] *= 2 * c_at(i) * dt / dd
-        epos, row, sign = _as_points(pos, Nx, Ny, ws) if nm == "ux" else _as_points_u(pos, Nx, Ny)
+        if nm == "ux":
+            epos, row, sign = _as_points(pos, Nx, Ny, ws)
+        else:
+            epos, row, sign = _as_points_u(pos, Nx, Ny, ws)
         out[ax] = (epos, sign[:, None] * sig[rows[row]], mode)
     return out
 
@@ -1540,7 +1556,9 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
     Supported here: source.p0 or a pressure
     source (every p_mode, laid out on the expanded grid with the signs of
     the p-like layout), velocity sources source.ux (axial) and source.uy
-    (radial; not with 'WSWS'), lossless or Stokes media (with or without B/A),
+    (radial; with 'WSWS' in the 'dirichlet' and 'additive-no-correction'
+    modes away from the last two radial columns), lossless or Stokes media
+    (with or without B/A),
     binary sensor masks or Cartesian points (x, r), time reversal;
     'SaveToDisk' writes the radial
     grid's input file (axisymmetric_flag 1, 'WSWA'), which kw_run_file and

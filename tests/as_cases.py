@@ -71,6 +71,18 @@ def case(name, Nx=32, Ny=16, seed=2, Nt=24, sym="WSWA-FFT"):
         lab[src_mask] = 1 + (rng.random(n_src) < 0.5)
         cs["source"] = {"u_mask": lab, "u_mode": "additive-no-correction",
                         "uy": np.stack([burst, -0.5 * burst])}
+    elif name in ("velocity_radial_dirichlet", "velocity_radial_nocorr"):
+        # radial (and axial) velocity sources away from the last two radial
+        # columns, which 'WSWS-FFT' takes in these modes
+        m = src_mask.copy()
+        m[:, Ny - 2:] = False
+        n_m = int(m.sum())
+        if name == "velocity_radial_dirichlet":
+            cs["source"] = {"u_mask": m, "u_mode": "dirichlet", "uy": 1e-4 * burst[None, :12],
+                            "ux": -5e-5 * burst[None, :12]}
+        else:
+            cs["source"] = {"u_mask": m, "u_mode": "additive-no-correction",
+                            "uy": rng.standard_normal((n_m, 1)) * burst}
     elif name == "velocity_axial_only":
         cs["medium"]["alpha_coeff"] = 0.5 + rng.random((Nx, Ny))
         cs["source"] = {"u_mask": src_mask, "ux": burst[None], "u_frequency_ref": 1e6}
@@ -115,11 +127,14 @@ NAMES = ["homogeneous", "hetero", "nonlinear", "stokes", "stokes_nonlinear", "sm
          "source_additive", "source_dirichlet", "source_labelled", "source_frequency_ref",
          "source_stokes_nonlinear", "time_reversal", "velocity_additive", "velocity_dirichlet",
          "velocity_labelled", "velocity_axial_only", "pml_outside", "pml_outside_source",
-         "cartesian_linear", "cartesian_nearest", "cartesian_tr"]
-# the cases run with 'WSWS-FFT' as well (radial velocity sources are not
-# taken with it)
+         "cartesian_linear", "cartesian_nearest", "cartesian_tr", "velocity_radial_dirichlet",
+         "velocity_radial_nocorr"]
+# the cases run with 'WSWS-FFT' as well (it takes radial velocity sources
+# in the 'dirichlet' and 'additive-no-correction' modes away from the last
+# two radial columns)
 WS_NAMES = ["hetero", "stokes_nonlinear", "smoothed", "source_additive", "source_dirichlet",
-            "time_reversal", "velocity_axial_only", "pml_outside", "cartesian_linear"]
+            "time_reversal", "velocity_axial_only", "pml_outside", "cartesian_linear",
+            "velocity_radial_dirichlet", "velocity_radial_nocorr"]
 
 
 def run_kwhip(cs):
