@@ -200,9 +200,9 @@ typedef struct kw_problem {
    * grids [f, flip(f(2:end-1))], y-staggered ones [f(1:end-1),
    * flip(f(1:end-1))], kspaceFirstOrderAS.m:720-722, 966-977; sources and
    * time-reversal points at columns j and 2Ny_r - 2 - j, both +; radial
-   * velocity sources at j and 2Ny_r - 3 - j, + and -, in the 'dirichlet'
-   * and 'additive-no-correction' modes with no point on row Ny/2 (others
-   * return KW_EUNSUP)).  0 otherwise. */
+   * velocity sources at j and 2Ny_r - 3 - j, + and -, 'dirichlet' and
+   * 'additive-no-correction' ones with no point on row Ny/2 (KW_EUNSUP
+   * otherwise)).  0 otherwise. */
   int32_t axisymmetric;
 } kw_problem;
 

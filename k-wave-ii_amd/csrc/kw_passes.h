@@ -1109,9 +1109,21 @@ KW_BODY void body_ew(Ctx& c, const Batch& B, const EwArgs& A) {
       }
       const float mn = KW_LD(row + i);
       const float k = KW_LD(A.prof + i);
+      // an additive radial velocity source adds the same filtered value to
+      // the mirror point and to the reference's last column
+      // (kspaceFirstOrderAS.m:879-905 trims the filtered expanded source
+      // to the radial grid, that column included)
+      float sv = 0.f;
+      const TimeSource& us = P.srcu[1];
+      if (A.which == 1 && us.mode == SRC_ADDITIVE && us.len >= KW_LD(P.t_index)) {
+        const int64_t c = (int64_t)(P.Ny / 2) * P.Nx + i;
+        sv = us.grid ? KW_LD(us.grid + c)
+                     : KW_LD(us.km + c) * us.value(0, KW_LD(P.t_index) - 1);
+      }
       const float un = A.which == 2 ? k * mn
                                     : A.alpha * (A.alpha * KW_LD(uc + i) -
-                                                 k * (A.beta * KW_LD(mc + i) - mn / A.beta));
+                                                 k * (A.beta * KW_LD(mc + i) - (mn - sv) / A.beta)) +
+                                          sv;
       KW_ST(mc + i, mn);
       KW_ST(uc + i, un);
       KW_ST(row + i, un);

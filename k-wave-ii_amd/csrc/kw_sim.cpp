@@ -348,17 +348,15 @@ std::unique_ptr<Sim> create_sim(const kw_problem* pr_in, void* stream, Arena* ar
     // applies them as given.  A radial velocity source of 'WSWS-FFT' would
     // also drive the mirror point that stands in for the last radial
     // column, which the reference updates outside its layout.
-    if (S->as_ws && pr->u_source[1].flag > 0) {
-      // only sources that leave that mirror point alone: not k-space
-      // filtered, no point on its row
+    if (S->as_ws && pr->u_source[1].flag > 0 && pr->u_source[1].mode != SRC_ADDITIVE) {
+      // a point source on that mirror point would set it apart from the
+      // update the last column's value is recovered from
       const kw_time_source& us = pr->u_source[1];
-      bool ok = us.mode != SRC_ADDITIVE;
       const int64_t row_last = G.Ny / 2;
-      for (int64_t k = 0; ok && k < us.num_points; ++k)
-        ok = (((int64_t)us.index[k] - 1) / G.Nx) % G.Ny != row_last;
-      if (!ok)
-        fail(KW_EUNSUP, "radial velocity sources (source.uy) with 'WSWS-FFT' are taken in the "
-                        "'dirichlet' and 'additive-no-correction' modes away from the last radial column");
+      for (int64_t k = 0; k < us.num_points; ++k)
+        if ((((int64_t)us.index[k] - 1) / G.Nx) % G.Ny == row_last)
+          fail(KW_EUNSUP, "radial velocity sources (source.uy) with 'WSWS-FFT' in the 'dirichlet' "
+                          "and 'additive-no-correction' modes are taken away from the last radial column");
     }
     // sensor points are grid points or simplices of the first copy (the
     // radial grid); directivity and the velocity split are 2-D options

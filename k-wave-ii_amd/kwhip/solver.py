@@ -1410,13 +1410,12 @@ def _as_velocity_sources(src, med, kgrid, scale, ws=False):
     for ax, nm, dd in ((0, "ux", kgrid.dx), (1, "uy", kgrid.dy)):
         if nm not in src:
             continue
-        if nm == "uy" and ws and (mode == "additive" or np.any(pos // Nx >= Ny - 2)):
-            # the mirror point that stands in for the last radial column
-            # must not be driven: no k-space filtered source, no points in
-            # the last two radial columns
-            raise NotImplementedError("radial velocity sources (source.uy) with 'WSWS' are supported "
-                                      "by kwhip in the 'dirichlet' and 'additive-no-correction' modes "
-                                      "away from the last two radial columns")
+        if nm == "uy" and ws and mode != "additive" and np.any(pos // Nx >= Ny - 2):
+            # a point source must not set or add to the mirror point that
+            # stands in for the last radial column
+            raise NotImplementedError("radial velocity sources (source.uy) with 'WSWS' in the "
+                                      "'dirichlet' and 'additive-no-correction' modes are supported "
+                                      "by kwhip away from the last two radial columns")
         sig = w * np.atleast_2d(np.asarray(src[nm], dtype=np.float64))
         rows = _as_signal_rows(m, sig, pos)
         if scale and mode != "dirichlet":
@@ -1556,7 +1555,7 @@ def kspaceFirstOrderAS(kgrid, medium, source, sensor, *args, **kwargs):  # noqa:
     Supported here: source.p0 or a pressure
     source (every p_mode, laid out on the expanded grid with the signs of
     the p-like layout), velocity sources source.ux (axial) and source.uy
-    (radial; with 'WSWS' in the 'dirichlet' and 'additive-no-correction'
+    (radial; with 'WSWS' its 'dirichlet' and 'additive-no-correction'
     modes away from the last two radial columns), lossless or Stokes media
     (with or without B/A),
     binary sensor masks or Cartesian points (x, r), time reversal;

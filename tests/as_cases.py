@@ -71,6 +71,13 @@ def case(name, Nx=32, Ny=16, seed=2, Nt=24, sym="WSWA-FFT"):
         lab[src_mask] = 1 + (rng.random(n_src) < 0.5)
         cs["source"] = {"u_mask": lab, "u_mode": "additive-no-correction",
                         "uy": np.stack([burst, -0.5 * burst])}
+    elif name == "velocity_radial_additive":
+        # the k-space filtered radial source, points up to the last radial
+        # column (whose 'WSWS' value takes the filtered source too)
+        m = src_mask.copy()
+        m[Nx // 2, Ny - 2:] = True
+        cs["source"] = {"u_mask": m, "u_mode": "additive",
+                        "uy": rng.standard_normal((int(m.sum()), 1)) * burst}
     elif name in ("velocity_radial_dirichlet", "velocity_radial_nocorr"):
         # radial (and axial) velocity sources away from the last two radial
         # columns, which 'WSWS-FFT' takes in these modes
@@ -128,13 +135,13 @@ NAMES = ["homogeneous", "hetero", "nonlinear", "stokes", "stokes_nonlinear", "sm
          "source_stokes_nonlinear", "time_reversal", "velocity_additive", "velocity_dirichlet",
          "velocity_labelled", "velocity_axial_only", "pml_outside", "pml_outside_source",
          "cartesian_linear", "cartesian_nearest", "cartesian_tr", "velocity_radial_dirichlet",
-         "velocity_radial_nocorr"]
-# the cases run with 'WSWS-FFT' as well (it takes radial velocity sources
-# in the 'dirichlet' and 'additive-no-correction' modes away from the last
-# two radial columns)
+         "velocity_radial_nocorr", "velocity_radial_additive"]
+# the cases run with 'WSWS-FFT' as well (it takes 'dirichlet' and
+# 'additive-no-correction' radial velocity sources away from the last two
+# radial columns)
 WS_NAMES = ["hetero", "stokes_nonlinear", "smoothed", "source_additive", "source_dirichlet",
             "time_reversal", "velocity_axial_only", "pml_outside", "cartesian_linear",
-            "velocity_radial_dirichlet", "velocity_radial_nocorr"]
+            "velocity_radial_dirichlet", "velocity_radial_nocorr", "velocity_radial_additive"]
 
 
 def run_kwhip(cs):

@@ -68,8 +68,11 @@ def test_as_option_checks():
     with pytest.raises(ValueError, match="source.p0 and source.p can't be defined"):
         kwhip.kspaceFirstOrderAS(kg, cs["medium"], {"p0": p0, "p_mask": p0 > 0.5,
                                                     "p": np.ones((1, 4))}, cs["sensor"])
+    edge = np.zeros_like(p0, dtype=bool)
+    edge[4, -2] = True
     with pytest.raises(NotImplementedError, match="radial velocity sources"):
-        kwhip.kspaceFirstOrderAS(kg, cs["medium"], {"u_mask": p0 > 0.5, "uy": np.ones((1, 4))},
+        kwhip.kspaceFirstOrderAS(kg, cs["medium"], {"u_mask": edge, "uy": np.ones((1, 4)),
+                                                    "u_mode": "dirichlet"},
                                  cs["sensor"], RadialSymmetry="WSWS")
     with pytest.raises(NotImplementedError, match="non-staggered"):
         kwhip.kspaceFirstOrderAS(kg, cs["medium"], dict(cs["source"]),
